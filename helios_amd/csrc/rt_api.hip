@@ -1,0 +1,382 @@
+// rt_api.hip -- extern "C" entry points declared in include/helios_rt.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "rt_solve.h"
+
+namespace rt {
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+}  // namespace rt
+
+using namespace rt;
+
+struct rt_ctx {
+    int device = 0;
+    std::string name;
+    SolveCtx sx;
+    std::mutex mu;
+};
+
+struct rt_scene {
+    Scene s;
+};
+
+#define CKR(x)                                                                    \
+    do {                                                                          \
+        hipError_t e_ = (x);                                                      \
+        if (e_ != hipSuccess) {                                                   \
+            set_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " #x); \
+            return -1;                                                            \
+        }                                                                         \
+    } while (0)
+
+#define NN(p, what)                                        \
+    do {                                                   \
+        if (!(p)) {                                        \
+            set_error(std::string(what) + " is NULL");     \
+            return -1;                                     \
+        }                                                  \
+    } while (0)
+
+extern "C" {
+
+const char *rt_last_error(void) { return g_err.c_str(); }
+
+int rt_create(int device, rt_ctx **out) {
+    NN(out, "out");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        set_error("no HIP device available (this backend has no CPU path)");
+        return -1;
+    }
+    if (device < 0 || device >= n) { set_error("device index out of range"); return -1; }
+    CKR(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    CKR(hipGetDeviceProperties(&prop, device));
+    std::string arch = prop.gcnArchName;
+    if (arch.rfind("gfx950", 0) != 0) {
+        set_error("device " + std::to_string(device) + " is " + arch + "; this library is built for gfx950 only");
+        return -1;
+    }
+    rt_ctx *c = new rt_ctx();
+    c->device = device;
+    c->name = std::string("hip-gfx950 (") + prop.name + ")";
+    c->sx.ncu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->sx.stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        set_error("hipStreamCreate failed");
+        return -1;
+    }
+    *out = c;
+    return 0;
+}
+
+int rt_destroy(rt_ctx *ctx) {
+    if (!ctx) return 0;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->sx.stream);
+    for (auto &e : ctx->sx.ev) { if (e.a) (void)hipEventDestroy(e.a); if (e.b) (void)hipEventDestroy(e.b); }
+    ctx->sx.ws.release();
+    (void)hipStreamDestroy(ctx->sx.stream);
+    delete ctx;
+    return 0;
+}
+
+const char *rt_backend_name(const rt_ctx *ctx) { return ctx ? ctx->name.c_str() : "none"; }
+
+int rt_query_memory(rt_ctx *ctx, size_t *free_bytes, size_t *total_bytes) {
+    NN(ctx, "ctx");
+    CKR(hipSetDevice(ctx->device));
+    size_t f = 0, t = 0;
+    CKR(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return 0;
+}
+
+int rt_scene_create(rt_ctx *ctx, uint32_t nprims, const float *transforms, const uint32_t *types,
+                    const uint32_t *uuids, const uint32_t *twosided, int periodic_x, int periodic_y, rt_scene **out) {
+    NN(ctx, "ctx");
+    NN(out, "out");
+    if (nprims) { NN(transforms, "transforms"); NN(types, "types"); NN(twosided, "twosided"); }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    CKR(hipSetDevice(ctx->device));
+    rt_scene *s = new rt_scene();
+    if (scene_build(s->s, nprims, transforms, types, uuids, twosided, periodic_x ? 1 : 0, periodic_y ? 1 : 0,
+                    ctx->sx.stream)) {
+        delete s;
+        return -1;
+    }
+    *out = s;
+    return 0;
+}
+
+int rt_scene_destroy(rt_ctx *ctx, rt_scene *scene) {
+    if (!scene) return 0;
+    if (ctx) {
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->sx.stream);
+    }
+    scene_free(scene->s);
+    delete scene;
+    return 0;
+}
+
+int rt_scene_get_info(const rt_scene *scene, rt_scene_info *info) {
+    NN(scene, "scene");
+    NN(info, "info");
+    info->nprims = scene->s.h.P;
+    info->nleaves = scene->s.h.nleaf;
+    info->depth = scene->s.h.depth;
+    info->nbbox = scene->s.h.nbbox;
+    for (int k = 0; k < 4; k++) info->domain[k] = scene->s.h.dom[k];
+    info->device_bytes = scene->s.bytes;
+    return 0;
+}
+
+int rt_cast_rays_soa_device(rt_ctx *ctx, const rt_scene *scene, const float *o, const float *d, size_t n,
+                            float max_dist, float *t, float *nrm, uint32_t *uuid) {
+    NN(ctx, "ctx");
+    NN(scene, "scene");
+    if (n == 0) return 0;
+    NN(o, "origins"); NN(d, "dirs"); NN(t, "out_t"); NN(nrm, "out_normal"); NN(uuid, "out_uuid");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    CKR(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->sx.stream;
+    if (ctx->sx.ws.reserve(64)) return -1;
+    int *err = (int *)ctx->sx.ws.base;
+    CKR(hipMemsetAsync(err, 0, sizeof(int), st));
+    hipEvent_t a = nullptr, b = nullptr;
+    if (ctx->sx.timing) { CKR(hipEventCreate(&a)); CKR(hipEventCreate(&b)); CKR(hipEventRecord(a, st)); }
+    if (launch_cast(scene->s.d, n, o, d, max_dist, t, nrm, uuid, err, st)) return -1;
+    if (b) CKR(hipEventRecord(b, st));
+    int herr = 0;
+    CKR(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, st));
+    CKR(hipStreamSynchronize(st));
+    if (a) {
+        float ms = 0.f;
+        CKR(hipEventElapsedTime(&ms, a, b));
+        ctx->sx.timing_info = rt_timing{};
+        ctx->sx.timing_info.direct_ms = ms;  // castRays time reported in the first slot
+        ctx->sx.timing_info.direct_rays = n;
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+    }
+    if (herr) { set_error("traversal error code " + std::to_string(herr) + " (1 = stack overflow, 2 = malformed tree)"); return -1; }
+    return 0;
+}
+
+int rt_cast_rays_soa(rt_ctx *ctx, const rt_scene *scene, const float *o, const float *d, size_t n, float max_dist,
+                     float *t, float *nrm, uint32_t *uuid) {
+    NN(ctx, "ctx");
+    NN(scene, "scene");
+    if (n == 0) return 0;
+    NN(o, "origins"); NN(d, "dirs"); NN(t, "out_t"); NN(nrm, "out_normal"); NN(uuid, "out_uuid");
+    CKR(hipSetDevice(ctx->device));
+    float *dbuf = nullptr;
+    size_t per = 3 + 3 + 1 + 3 + 1;  // o, d, t, n, uuid (4 bytes each)
+    CKR(hipMalloc(&dbuf, n * per * 4));
+    float *dO = dbuf, *dD = dbuf + n * 3, *dT = dbuf + n * 6, *dN = dbuf + n * 7;
+    uint32_t *dU = reinterpret_cast<uint32_t *>(dbuf + n * 10);
+    int rc = 0;
+    do {
+        if (hipMemcpy(dO, o, n * 12, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(dD, d, n * 12, hipMemcpyHostToDevice) != hipSuccess) {
+            set_error("castRays upload failed");
+            rc = -1;
+            break;
+        }
+        if (rt_cast_rays_soa_device(ctx, scene, dO, dD, n, max_dist, dT, dN, dU)) { rc = -1; break; }
+        // misses leave the caller's out_t / out_normal untouched (CD castRaysSoA contract)
+        std::vector<float> ht(n), hn(n * 3);
+        if (hipMemcpy(ht.data(), dT, n * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(hn.data(), dN, n * 12, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(uuid, dU, n * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+            set_error("castRays download failed");
+            rc = -1;
+            break;
+        }
+        for (size_t r = 0; r < n; r++) {
+            if (uuid[r] == RT_MISS_UUID) continue;
+            t[r] = ht[r];
+            nrm[r * 3] = hn[r * 3]; nrm[r * 3 + 1] = hn[r * 3 + 1]; nrm[r * 3 + 2] = hn[r * 3 + 2];
+        }
+    } while (0);
+    (void)hipFree(dbuf);
+    return rc;
+}
+
+static int check_request(const rt_scene *scene, const rt_band_request *q) {
+    NN(scene, "scene");
+    NN(q, "request");
+    if (q->nbands == 0) { set_error("request has no bands"); return -1; }
+    NN(q->band_depth, "band_depth");
+    NN(q->emission, "emission");
+    if (scene->s.h.P) { NN(q->rho, "rho"); NN(q->tau, "tau"); }
+    uint32_t maxd = 0;
+    for (uint32_t b = 0; b < q->nbands; b++) maxd = std::max(maxd, q->band_depth[b]);
+    NN(q->diffuse_seed, "diffuse_seed");
+    (void)maxd;
+    return 0;
+}
+
+int rt_run_band(rt_ctx *ctx, const rt_scene *scene, const rt_band_request *q, float *R_out) {
+    NN(ctx, "ctx");
+    if (check_request(scene, q)) return -1;
+    NN(R_out, "R_out");
+    if (q->nsources) { NN(q->source_dir, "source_dir"); NN(q->source_flux, "source_flux"); }
+    NN(q->diffuse_flux, "diffuse_flux");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    CKR(hipSetDevice(ctx->device));
+    static const float zero3[3] = {0.f, 0.f, 1.f};
+    SunSet s{1, q->nsources ? q->source_dir : zero3, q->source_flux, q->diffuse_flux, &q->direct_seed};
+    return run_solve(ctx->sx, scene->s, *q, s, R_out, 0);
+}
+
+int rt_run_band_batch(rt_ctx *ctx, const rt_scene *scene, const rt_band_request *base, const rt_sun_batch *suns,
+                      float *R_out, int out_on_device) {
+    NN(ctx, "ctx");
+    if (check_request(scene, base)) return -1;
+    NN(suns, "suns");
+    NN(R_out, "R_out");
+    if (suns->nsuns == 0) { set_error("batch has no suns"); return -1; }
+    if (base->nsources) { NN(suns->source_dir, "suns.source_dir"); NN(suns->source_flux, "suns.source_flux"); }
+    NN(suns->diffuse_flux, "suns.diffuse_flux");
+    NN(suns->direct_seed, "suns.direct_seed");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    CKR(hipSetDevice(ctx->device));
+    SunSet s{suns->nsuns, suns->source_dir, suns->source_flux, suns->diffuse_flux, suns->direct_seed};
+    return run_solve(ctx->sx, scene->s, *base, s, R_out, out_on_device);
+}
+
+int rt_energy_residual_fp64(rt_ctx *ctx, double *out, uint32_t nchannels) {
+    NN(ctx, "ctx");
+    NN(out, "out");
+    size_t C = ctx->sx.energy.size() / 3;
+    if (nchannels != C) { set_error("channel count does not match the last solve"); return -1; }
+    for (size_t k = 0; k < 3 * C; k++) out[k] = ctx->sx.energy[k];
+    return 0;
+}
+
+int rt_set_launch_cfg(rt_ctx *ctx, int kernel, const rt_launch_cfg *cfg) {
+    NN(ctx, "ctx");
+    NN(cfg, "cfg");
+    if (kernel < 0 || kernel > 2) { set_error("unknown kernel id"); return -1; }
+    if (!(cfg->waves_per_block == 1 || cfg->waves_per_block == 2 || cfg->waves_per_block == 4)) {
+        set_error("waves_per_block must be 1, 2 or 4");
+        return -1;
+    }
+    if (cfg->blocks_per_cu < 1 || cfg->blocks_per_cu > 64) { set_error("blocks_per_cu must be in [1, 64]"); return -1; }
+    ctx->sx.cfg[kernel].waves_per_block = cfg->waves_per_block;
+    ctx->sx.cfg[kernel].blocks_per_cu = cfg->blocks_per_cu;
+    return 0;
+}
+
+int rt_get_launch_cfg(rt_ctx *ctx, int kernel, rt_launch_cfg *cfg) {
+    NN(ctx, "ctx");
+    NN(cfg, "cfg");
+    if (kernel < 0 || kernel > 2) { set_error("unknown kernel id"); return -1; }
+    cfg->waves_per_block = ctx->sx.cfg[kernel].waves_per_block;
+    cfg->blocks_per_cu = ctx->sx.cfg[kernel].blocks_per_cu;
+    return 0;
+}
+
+int rt_enable_timing(rt_ctx *ctx, int on) {
+    NN(ctx, "ctx");
+    ctx->sx.timing = on != 0;
+    return 0;
+}
+
+int rt_get_timing(rt_ctx *ctx, rt_timing *out) {
+    NN(ctx, "ctx");
+    NN(out, "out");
+    *out = ctx->sx.timing_info;
+    return 0;
+}
+
+int rt_debug_diffuse_hits(rt_ctx *ctx, const rt_scene *scene, uint32_t n, uint32_t seed, uint32_t *out) {
+    NN(ctx, "ctx");
+    NN(scene, "scene");
+    NN(out, "out");
+    if (n == 0) { set_error("n must be positive"); return -1; }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    CKR(hipSetDevice(ctx->device));
+    return debug_diffuse_hits(ctx->sx, scene->s, n, seed, out);
+}
+
+int rt_debug_diffuse_rays(rt_ctx *ctx, const rt_scene *scene, uint32_t n, uint32_t seed, float *out) {
+    NN(ctx, "ctx");
+    NN(scene, "scene");
+    NN(out, "out");
+    if (n == 0) { set_error("n must be positive"); return -1; }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    CKR(hipSetDevice(ctx->device));
+    size_t bytes = (size_t)scene->s.h.P * 2 * n * n * 6 * 4;
+    float *d = nullptr;
+    CKR(hipMalloc(&d, bytes + 4));
+    int rc = launch_debug_rays(scene->s.d, scene->s.h.P, n, seed, d, ctx->sx.stream);
+    if (!rc && (hipMemcpyAsync(out, d, bytes, hipMemcpyDeviceToHost, ctx->sx.stream) != hipSuccess ||
+                hipStreamSynchronize(ctx->sx.stream) != hipSuccess)) {
+        set_error("debug ray download failed");
+        rc = -1;
+    }
+    (void)hipFree(d);
+    return rc;
+}
+
+int rt_debug_trace(rt_ctx *ctx, const rt_scene *scene, uint32_t n, const float *rays, const uint32_t *skip, float tmin,
+                   int brute, uint32_t *hit, float *t) {
+    NN(ctx, "ctx");
+    NN(scene, "scene");
+    if (n == 0) return 0;
+    NN(rays, "rays"); NN(skip, "skip"); NN(hit, "hit"); NN(t, "t");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    CKR(hipSetDevice(ctx->device));
+    char *d = nullptr;
+    size_t b_r = (size_t)n * 24, b_s = (size_t)n * 4;
+    CKR(hipMalloc(&d, b_r + 3 * b_s + 64));
+    float *dr = (float *)d;
+    uint32_t *ds = (uint32_t *)(d + b_r), *dh = (uint32_t *)(d + b_r + b_s);
+    float *dt = (float *)(d + b_r + 2 * b_s);
+    int *err = (int *)(d + b_r + 3 * b_s);
+    hipStream_t st = ctx->sx.stream;
+    int rc = 0, herr = 0;
+    if (hipMemcpyAsync(dr, rays, b_r, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(ds, skip, b_s, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemsetAsync(err, 0, 4, st) != hipSuccess) {
+        set_error("debug trace upload failed");
+        rc = -1;
+    }
+    if (!rc) rc = launch_debug_trace(scene->s.d, n, dr, ds, tmin, brute, dh, dt, err, st);
+    if (!rc && (hipMemcpyAsync(hit, dh, b_s, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipMemcpyAsync(t, dt, b_s, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess)) {
+        set_error("debug trace download failed");
+        rc = -1;
+    }
+    (void)hipFree(d);
+    if (!rc && herr) { set_error("traversal error code " + std::to_string(herr)); rc = -1; }
+    return rc;
+}
+
+int rt_debug_direct_sums(rt_ctx *ctx, const rt_scene *scene, uint32_t d, uint32_t seed, uint32_t nsrc,
+                         const float *src_dir, float *out) {
+    NN(ctx, "ctx");
+    NN(scene, "scene");
+    NN(out, "out");
+    NN(src_dir, "src_dir");
+    if (d == 0 || nsrc == 0) { set_error("d and nsrc must be positive"); return -1; }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    CKR(hipSetDevice(ctx->device));
+    return debug_direct_sums(ctx->sx, scene->s, d, seed, nsrc, src_dir, out);
+}
+
+}  // extern "C"

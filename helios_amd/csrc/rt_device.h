@@ -1,0 +1,206 @@
+// rt_device.h -- FP32 ray generation and intersection arithmetic shared by every gfx950 kernel.
+//
+// Determinism contract: the whole library is built with -ffp-contract=off and these helpers use
+// correctly rounded division and square root, so a ray's origin, direction and intersection
+// decisions are bit-identical to a plain IEEE-754 binary32 evaluation of the same expressions in
+// the same order.  That is what lets the hit tables be compared bit for bit with a CPU evaluation.
+//
+// Reference semantics (Helios tree, R = plugins/radiation, CD = plugins/collisiondetection):
+//   RNG            R/shaders/common/random.glsl:10-45
+//   sample points  R/shaders/direct_raygen.comp:186-224, R/shaders/diffuse_raygen.comp:197-234
+//   hemisphere     R/shaders/diffuse_raygen.comp:461-482, rotate_to_normal :303-327
+//   predicates     R/shaders/common/intersections.glsl:26-112, CD/src/CollisionDetection_RayTracing.cpp:1793-1884
+//   periodic wrap  R/shaders/common/periodic_boundary.glsl:28-112
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define RT_HD __host__ __device__ __forceinline__
+#define RT_MISS 0xFFFFFFFFu
+
+struct v3f { float x, y, z; };
+
+RT_HD v3f mk3(float x, float y, float z) { v3f r; r.x = x; r.y = y; r.z = z; return r; }
+RT_HD v3f sub3(v3f a, v3f b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
+RT_HD v3f add3(v3f a, v3f b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+RT_HD v3f mul3(v3f a, float s) { return mk3(a.x * s, a.y * s, a.z * s); }
+RT_HD float dot3(v3f a, v3f b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+RT_HD v3f cross3(v3f a, v3f b) { return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+
+// correctly rounded division / square root on both sides
+RT_HD float rdiv(float a, float b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __fdiv_rn(a, b);
+#else
+    return a / b;
+#endif
+}
+// correctly rounded square root.  On gfx950 __builtin_sqrtf lowers to v_sqrt_f32 followed by a
+// +-1 ulp residual correction (correct rounding); HIP's __fsqrt_rn lowers to the bare 1-ulp v_sqrt_f32.
+RT_HD float rsqrt_cr(float a) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_sqrtf(a);
+#else
+    return sqrtf(a);
+#endif
+}
+
+// normalize by multiplying with 1/sqrt(|v|^2) (core/include/helios_vector_types.h:891-900)
+RT_HD v3f nrm3(v3f a) {
+    float m2 = a.x * a.x + a.y * a.y + a.z * a.z;
+    float inv = rdiv(1.0f, rsqrt_cr(m2));
+    return mul3(a, inv);
+}
+
+// row-major 4x4 affine point transform, terms summed left to right
+RT_HD v3f xform3(const float *t, float px, float py, float pz) {
+    return mk3(t[0] * px + t[1] * py + t[2] * pz + t[3],
+               t[4] * px + t[5] * py + t[6] * pz + t[7],
+               t[8] * px + t[9] * py + t[10] * pz + t[11]);
+}
+
+// ---- RNG: 16-round TEA seeding + 24-bit LCG ----
+RT_HD uint32_t tea16(uint32_t v0, uint32_t v1) {
+    uint32_t s0 = 0;
+#pragma unroll
+    for (int n = 0; n < 16; n++) {
+        s0 += 0x9e3779b9u;
+        v0 += ((v1 << 4) + 0xa341316cu) ^ (v1 + s0) ^ ((v1 >> 5) + 0xc8013ea4u);
+        v1 += ((v0 << 4) + 0xad90777du) ^ (v0 + s0) ^ ((v0 >> 5) + 0x7e95761eu);
+    }
+    return v0;
+}
+
+RT_HD float rnd24(uint32_t &st) {
+    st = 1664525u * st + 1013904223u;
+    return (float)(st & 0x00FFFFFFu) * (1.0f / 16777216.0f);  // exact: 24-bit integer times 2^-24
+}
+
+// sin/cos of 2*pi*r: quadrant reduction (exact) then Taylor polynomials on [-pi/4, pi/4]
+RT_HD void sincos_2pi(float r, float &s, float &c) {
+    float u = r * 4.0f;
+    float q = floorf(u + 0.5f);
+    float x = u - q;
+    float a = x * 1.57079632679489662f;
+    float a2 = a * a;
+    float ps = a + a * a2 * (-0.166666666666666667f + a2 * (0.00833333333333333333f +
+                                a2 * (-0.000198412698412698413f + a2 * 2.75573192239858907e-6f)));
+    float pc = 1.0f + a2 * (-0.5f + a2 * (0.0416666666666666667f + a2 * (-0.00138888888888888889f +
+                                a2 * (2.48015873015873016e-5f + a2 * -2.75573192239858907e-7f))));
+    int qi = ((int)q) & 3;
+    if (qi == 0) { s = ps; c = pc; }
+    else if (qi == 1) { s = pc; c = -ps; }
+    else if (qi == 2) { s = -ps; c = -pc; }
+    else { s = -pc; c = ps; }
+}
+
+// ---- primitive record (64 B, leaf order) ----
+//   r0 = v0.xyz, bits(prim_index | PATCH_BIT)
+//   r1 = a|e1 .xyz, |a|^2
+//   r2 = b|e2 .xyz, |b|^2
+//   r3 = patch plane normal .xyz, 0
+#define RT_PATCH_BIT 0x80000000u
+
+// Radiation predicates (intersections.glsl) and CD predicates share the arithmetic; only the
+// acceptance thresholds differ.
+template <bool CD>
+RT_HD bool isect_rec(const float4 r0, const float4 r1, const float4 r2, const float4 r3, bool patch, v3f o, v3f d,
+                     float &t_out) {
+    v3f v0 = mk3(r0.x, r0.y, r0.z);
+    v3f e1 = mk3(r1.x, r1.y, r1.z);
+    v3f e2 = mk3(r2.x, r2.y, r2.z);
+    if (patch) {
+        v3f n = mk3(r3.x, r3.y, r3.z);
+        float denom = dot3(d, n);
+        if (CD) { if (!(fabsf(denom) > 1e-8f)) return false; }
+        else { if (fabsf(denom) < 1e-8f) return false; }
+        float t = rdiv(dot3(sub3(v0, o), n), denom);
+        if (CD) { if (!(t > 1e-8f && t < 1e8f)) return false; }
+        else { if (t < 1e-8f || t > 1e8f) return false; }
+        v3f p = add3(o, mul3(d, t));
+        v3f dd = sub3(p, v0);
+        float da = dot3(dd, e1), db = dot3(dd, e2);
+        bool in;
+        if (CD) in = (da >= -1e-8f && da <= r1.w + 1e-8f && db >= -1e-8f && db <= r2.w + 1e-8f);
+        else in = (da > 0.0f && da < r1.w && db > 0.0f && db < r2.w);
+        if (!in) return false;
+        t_out = t;
+        return true;
+    } else {
+        const float EPS = CD ? 1e-5f : 1e-8f;
+        v3f h = cross3(d, e2);
+        float a = dot3(e1, h);
+        if (CD) { if (a > -EPS && a < EPS) return false; }
+        else { if (fabsf(a) < EPS) return false; }
+        float f = rdiv(1.0f, a);
+        v3f s = sub3(o, v0);
+        float u = f * dot3(s, h);
+        if (CD) { if (u < -EPS || u > 1.0f + EPS) return false; }
+        else { if (u < 0.0f || u > 1.0f) return false; }
+        v3f q = cross3(s, e1);
+        float v = f * dot3(d, q);
+        if (CD) { if (v < -EPS || u + v > 1.0f + EPS) return false; }
+        else { if (v < 0.0f || u + v > 1.0f) return false; }
+        float t = f * dot3(e2, q);
+        if (CD) { if (!(t > EPS)) return false; }
+        else { if (!(t > 1e-8f && t < 1e8f)) return false; }
+        t_out = t;
+        return true;
+    }
+}
+
+// Direct-pass sample point in primitive-local coordinates (direct_raygen.comp:360-428)
+RT_HD void direct_sample(uint32_t prim, uint32_t lx, uint32_t ly, uint32_t d, uint32_t seed, bool patch, float &spx,
+                         float &spy) {
+    uint32_t Nrays = d * d;
+    uint32_t st = tea16(d * ly + lx + Nrays * prim, seed);
+    float Rx = rnd24(st), Ry = rnd24(st);
+    if (patch) {
+        float fd = (float)d;
+        spx = (-0.5f + rdiv((float)lx, fd)) + rdiv(Rx, fd);
+        spy = (-0.5f + rdiv((float)ly, fd)) + rdiv(Ry, fd);
+    } else {
+        if (Rx < Ry) { spx = Rx; spy = Ry; } else { spx = Ry; spy = Rx; }
+    }
+}
+
+// Diffuse-pass sample point and local hemisphere direction (diffuse_raygen.comp:375-482)
+RT_HD void diffuse_sample(uint32_t prim, uint32_t lx, uint32_t ly, uint32_t n, uint32_t seed, bool patch, float &spx,
+                          float &spy, v3f &dl) {
+    uint32_t st = tea16(n * ly + lx + n * n * prim, seed);
+    float Rx = rnd24(st), Ry = rnd24(st);
+    if (patch) { spx = -0.5f + Rx; spy = -0.5f + Ry; }
+    else { if (Rx < Ry) { spx = Rx; spy = Ry; } else { spx = Ry; spy = Rx; } }
+    float fn = (float)n;
+    float Rt = rdiv((float)lx + rnd24(st), fn);
+    float Rp = rdiv((float)ly + rnd24(st), fn);
+    float sth = rsqrt_cr(Rt);
+    float cth = rsqrt_cr(1.0f - Rt);
+    float sp, cp;
+    sincos_2pi(Rp, sp, cp);
+    dl = mk3(sth * cp, sth * sp, cth);
+}
+
+// rotate_to_normal with the per-primitive frame (ce, se, ca, sa) = cos/sin of elevation and azimuth
+RT_HD v3f frame_rotate(v3f dl, float ce, float se, float ca, float sa) {
+    v3f dw;
+    dw.x = dl.x * (ce * ca) - dl.y * sa + dl.z * (se * ca);
+    dw.y = dl.x * (ce * sa) + dl.y * ca + dl.z * (se * sa);
+    dw.z = -dl.x * se + dl.z * ce;
+    return nrm3(dw);
+}
+
+// Neumaier-compensated FP32 accumulator
+struct ksum {
+    float s, c;
+    RT_HD void init() { s = 0.f; c = 0.f; }
+    RT_HD void add(float x) {
+        float t = s + x;
+        if (fabsf(s) >= fabsf(x)) c += (s - t) + x;
+        else c += (x - t) + s;
+        s = t;
+    }
+    RT_HD void merge(float os, float oc) { add(os); c += oc; }
+    RT_HD float value() const { return s + c; }
+};

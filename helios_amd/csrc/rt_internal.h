@@ -1,0 +1,47 @@
+// rt_internal.h -- host-side runtime objects behind the C ABI (not part of the public interface).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "rt_types.h"
+
+namespace rt {
+
+void set_error(const std::string &msg);
+
+// A resident scene: device arrays plus the DevScene descriptor (host copy h, device copy d).
+struct Scene {
+    DevScene h{};
+    DevScene *d = nullptr;
+    uint32_t *twos = nullptr;
+    std::vector<void *> own;
+    size_t bytes = 0;
+};
+
+int scene_build(Scene &S, uint32_t P, const float *T, const uint32_t *type, const uint32_t *uuid,
+                const uint32_t *twos, int px, int py, hipStream_t st);
+void scene_free(Scene &S);
+
+// Launch configuration chosen per kernel (autotuned or default).
+struct LaunchCfg {
+    uint32_t waves_per_block = 4;  // 64-lane waves per workgroup
+    uint32_t blocks_per_cu = 4;    // persistent workgroups per CU
+};
+
+// kernel launchers (rt_trace.hip)
+int launch_direct(const DirectJob *jobs_dev, uint32_t njobs, uint32_t total_items, uint32_t *counter, int *err,
+                  const LaunchCfg &cfg, int ncu, hipStream_t st);
+int launch_diffuse(const DiffuseJob *jobs_dev, uint32_t njobs, uint32_t total_items, uint32_t *counter, int *err,
+                   const LaunchCfg &cfg, int ncu, hipStream_t st);
+int launch_gather(const GatherJob *jobs_dev, uint32_t njobs, uint32_t total_items, uint32_t cmax, uint32_t *counter,
+                  const LaunchCfg &cfg, int ncu, hipStream_t st);
+int launch_cast(const DevScene *sc, uint64_t n, const float *o, const float *d, float maxd, float *t, float *nrm,
+                uint32_t *uuid, int *err, hipStream_t st);
+int launch_debug_rays(const DevScene *sc, uint32_t P, uint32_t n, uint32_t seed, float *out, hipStream_t st);
+int launch_debug_trace(const DevScene *sc, uint32_t n, const float *rays, const uint32_t *skip, float tmin, int brute,
+                       uint32_t *hit, float *t, int *err, hipStream_t st);
+
+}  // namespace rt

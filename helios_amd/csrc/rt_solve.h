@@ -1,0 +1,53 @@
+// rt_solve.h -- solve orchestration shared by the C ABI, the batcher and the request server.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "../../include/helios_rt.h"
+#include "rt_internal.h"
+
+namespace rt {
+
+// grow-only device workspace (one per stream that runs solves)
+struct Workspace {
+    void *base = nullptr;
+    size_t cap = 0;
+    int reserve(size_t bytes);
+    void release();
+};
+
+struct EvRec {
+    int cls;
+    hipEvent_t a, b;
+};
+
+// Everything one stream needs to run solves.
+struct SolveCtx {
+    hipStream_t stream = nullptr;
+    int ncu = 256;
+    LaunchCfg cfg[3];
+    Workspace ws;
+    bool timing = false;
+    std::vector<EvRec> ev;
+    rt_timing timing_info{};
+    std::vector<double> energy;
+};
+
+// The per-sun part of a solve (K suns share everything else).
+struct SunSet {
+    uint32_t K;
+    const float *dir;     // [K][S][3]
+    const float *flux;    // [K][S][B]
+    const float *dflux;   // [K][B]
+    const uint32_t *seed; // [K]
+};
+
+int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunSet &suns, float *R_out,
+              int out_on_device);
+int debug_diffuse_hits(SolveCtx &sx, const Scene &S, uint32_t n, uint32_t seed, uint32_t *out);
+int debug_direct_sums(SolveCtx &sx, const Scene &S, uint32_t d, uint32_t seed, uint32_t nsrc, const float *src_dir,
+                      float *out);
+
+}  // namespace rt

@@ -1,0 +1,502 @@
+// rt_solve.hip -- device-resident runBand pass schedule (piece d) and its elementwise kernels.
+//
+// The schedule follows R/src/RadiationModel.cpp:3470-4672 step by step, but every buffer stays in
+// HBM: the scatter -> radiation_out hand-offs that the reference does on the host between passes
+// (VulkanComputeBackend.cpp:1765-1877, 2055-2160) are elementwise kernels here, and a whole solve is
+// one stream of launches with a single synchronisation at the end.
+//
+// Channels: a solve works on C = K*B channels (K suns sharing scene, bands, materials and diffuse
+// seeds; B bands).  Buffers are [P][C]; channel c belongs to sun c / B and band c % B.  Every
+// per-sun decision of the reference schedule (rundirect, diffuse enabled, copy at s = 0) is a
+// per-channel mask, so each sun's channels evolve exactly as in a solve of that sun alone.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rt_solve.h"
+
+namespace rt {
+
+__global__ void k_deposit_direct(uint32_t P, uint32_t C, uint32_t B, uint32_t S, const float *__restrict__ dsum,
+                                 const float *__restrict__ flux, const float *__restrict__ rho,
+                                 const float *__restrict__ tau, const uint8_t *__restrict__ mask, float *in, float *st,
+                                 float *sb, double *inc) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (size_t)P * C) return;
+    uint32_t i = (uint32_t)(q / C), c = (uint32_t)(q % C);
+    if (!mask[c]) return;
+    uint32_t k = c / B, b = c % B;
+    float r = rho[(size_t)i * B + b], t = tau[(size_t)i * B + b];
+    float a_in = 0.f, a_st = 0.f, a_sb = 0.f;
+    for (uint32_t s = 0; s < S; s++) {
+        const float *ds = dsum + (((size_t)k * S + s) * P + i) * 2;
+        float stp = ds[0], sbt = ds[1];
+        float F = flux[((size_t)k * S + s) * B + b];
+        a_in += F * (stp + sbt) * (1.0f - r - t);
+        a_st += F * (stp * r + sbt * t);
+        a_sb += F * (stp * t + sbt * r);
+    }
+    in[q] += a_in;
+    st[q] += a_st;
+    sb[q] += a_sb;
+}
+
+// scattered direct energy -> radiation_out, one-sided primitives combine both faces
+// (RadiationModel.cpp:3999-4036)
+__global__ void k_after_direct(uint32_t P, uint32_t C, const float4 *__restrict__ nrm, const uint8_t *__restrict__ mask,
+                               float *st, float *sb, float *ot, float *ob) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (size_t)P * C) return;
+    uint32_t i = (uint32_t)(q / C), c = (uint32_t)(q % C);
+    if (!mask[c]) return;
+    float t = st[q], b = sb[q];
+    if (__float_as_uint(nrm[i].w) == 0u) { float tot = t + b; t = tot; b = tot; }
+    ot[q] = t;
+    ob[q] = b;
+    st[q] = 0.f;
+    sb[q] = 0.f;
+}
+
+// emitted flux sigma*eps*T^4 added to radiation_out (RadiationModel.cpp:4041-4137)
+__global__ void k_emission(uint32_t P, uint32_t C, uint32_t B, const float4 *__restrict__ nrm,
+                           const float *__restrict__ eps, const float *__restrict__ temp,
+                           const uint32_t *__restrict__ depth, const uint8_t *__restrict__ mask, float *ot, float *ob) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (size_t)P * C) return;
+    uint32_t i = (uint32_t)(q / C), c = (uint32_t)(q % C);
+    if (!mask[c]) return;
+    uint32_t b = c % B;
+    float e = eps ? eps[(size_t)i * B + b] : 1.0f;
+    if (depth[b] == 0 && e != 1.f) e = 1.f;
+    float T = temp ? temp[i] : 300.f;
+    if (T < 0.f) T = 300.f;
+    float out = (float)(5.6703744E-8 * (double)e * ((double)T * (double)T * (double)T * (double)T));
+    ot[q] += out;
+    if (__float_as_uint(nrm[i].w) != 0u) ob[q] += out;
+}
+
+// per-iteration scatter hand-off: optional deposit of a retiring band's leftover, copy
+// scatter -> radiation_out, zero scatter (RadiationModel.cpp:4255-4289)
+__global__ void k_copy_zero(uint32_t P, uint32_t C, const uint8_t *__restrict__ retire, const uint8_t *__restrict__ copy,
+                            const uint8_t *__restrict__ zero, float *in, float *st, float *sb, float *ot, float *ob) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (size_t)P * C) return;
+    uint32_t c = (uint32_t)(q % C);
+    float t = st[q], b = sb[q];
+    if (retire[c]) { in[q] += t + b; t = 0.f; b = 0.f; st[q] = 0.f; sb[q] = 0.f; }
+    if (copy[c]) { ot[q] = t; ob[q] = b; }
+    if (zero[c]) { st[q] = 0.f; sb[q] = 0.f; }
+}
+
+// R = radiation_in + scatter_top + scatter_bottom (RadiationModel.cpp:4631-4654), written [K][P][B]
+__global__ void k_final(uint32_t P, uint32_t C, uint32_t B, const float *__restrict__ in, const float *__restrict__ st,
+                        const float *__restrict__ sb, float *__restrict__ R) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (size_t)P * C) return;
+    uint32_t i = (uint32_t)(q / C), c = (uint32_t)(q % C);
+    uint32_t k = c / B, b = c % B;
+    R[((size_t)k * P + i) * B + b] = in[q] + st[q] + sb[q];
+}
+
+// FP64 per-channel totals weighted by primitive area (absorbed, direct incident)
+__global__ void k_energy(uint32_t P, uint32_t C, uint32_t B, uint32_t S, const float4 *__restrict__ rec_unused,
+                         const double *__restrict__ area, const float *__restrict__ R, const float *__restrict__ dsum,
+                         const float *__restrict__ flux, double *out) {
+    uint32_t c = blockIdx.x;
+    uint32_t k = c / B, b = c % B;
+    double a0 = 0.0, a1 = 0.0;
+    for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
+        double A = area[i];
+        a0 += A * (double)R[((size_t)k * P + i) * B + b];
+        if (dsum) {
+            for (uint32_t s = 0; s < S; s++) {
+                const float *ds = dsum + (((size_t)k * S + s) * P + i) * 2;
+                a1 += A * (double)flux[((size_t)k * S + s) * B + b] * ((double)ds[0] + (double)ds[1]);
+            }
+        }
+    }
+    __shared__ double r0[256], r1[256];
+    r0[threadIdx.x] = a0;
+    r1[threadIdx.x] = a1;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) { r0[threadIdx.x] += r0[threadIdx.x + off]; r1[threadIdx.x] += r1[threadIdx.x + off]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { out[c] = r0[0]; out[C + c] = r1[0]; }
+}
+
+// primitive areas in FP64 from the transforms (|T e_x x T e_y| for patches, half that for triangles)
+__global__ void k_area(uint32_t P, const float *__restrict__ T, const uint32_t *__restrict__ type, double *area) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    const float *t = T + (size_t)i * 16;
+    uint32_t ty = type[i];
+    double ax, ay, az, bx, by, bz;
+    if (ty == 0 || ty == 3) { ax = t[0]; ay = t[4]; az = t[8]; bx = t[1]; by = t[5]; bz = t[9]; }
+    else { ax = t[1]; ay = t[5]; az = t[9]; bx = (double)t[0] + t[1]; by = (double)t[4] + t[5]; bz = (double)t[8] + t[9]; }
+    double cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
+    double a = sqrt(cx * cx + cy * cy + cz * cz);
+    area[i] = (ty == 0 || ty == 3) ? a : (ty == 1 ? 0.5 * a : 0.0);
+}
+
+#define CK(x)                                                                     \
+    do {                                                                          \
+        hipError_t e_ = (x);                                                      \
+        if (e_ != hipSuccess) {                                                   \
+            set_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " #x); \
+            return -1;                                                            \
+        }                                                                         \
+    } while (0)
+#define LK()                                                                      \
+    do {                                                                          \
+        hipError_t e_ = hipGetLastError();                                        \
+        if (e_ != hipSuccess) {                                                   \
+            set_error(std::string("kernel launch failed: ") + hipGetErrorString(e_)); \
+            return -1;                                                            \
+        }                                                                         \
+    } while (0)
+
+static inline uint32_t nb(size_t n) { return (uint32_t)((n + 255) / 256); }
+
+int Workspace::reserve(size_t bytes) {
+    if (bytes <= cap) return 0;
+    if (base) (void)hipFree(base);
+    base = nullptr;
+    cap = 0;
+    size_t want = bytes + bytes / 8 + (1u << 20);
+    CK(hipMalloc(&base, want));
+    cap = want;
+    return 0;
+}
+
+void Workspace::release() {
+    if (base) (void)hipFree(base);
+    base = nullptr;
+    cap = 0;
+}
+
+namespace {
+struct Carve {
+    char *p;
+    size_t off = 0;
+    template <typename T>
+    T *take(size_t n) {
+        off = (off + 255) & ~(size_t)255;
+        T *r = reinterpret_cast<T *>(p + off);
+        off += n * sizeof(T);
+        return r;
+    }
+};
+}  // namespace
+
+static void tick(SolveCtx &sx, int cls, hipEvent_t a, hipEvent_t b) { sx.ev.push_back({cls, a, b}); }
+
+static int ev_pair(SolveCtx &sx, hipEvent_t &a, hipEvent_t &b) {
+    if (!sx.timing) { a = b = nullptr; return 0; }
+    CK(hipEventCreate(&a));
+    CK(hipEventCreate(&b));
+    return 0;
+}
+
+int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunSet &suns, float *R_out,
+              int out_on_device) {
+    hipStream_t st = sx.stream;
+    const uint32_t P = S.h.P, B = q.nbands, K = suns.K, Sn = q.nsources, C = K * B;
+    if (B == 0 || K == 0) { set_error("run_band: no bands or no suns"); return -1; }
+    if (C > 256) { set_error("run_band: more than 256 channels (suns x bands) in one solve"); return -1; }
+    for (auto &e : sx.ev) { if (e.a) (void)hipEventDestroy(e.a); if (e.b) (void)hipEventDestroy(e.b); }
+    sx.ev.clear();
+    sx.timing_info = rt_timing{};
+    uint32_t maxdepth = 0;
+    int scattering = 0, emission = 0;
+    for (uint32_t b = 0; b < B; b++) {
+        maxdepth = std::max(maxdepth, q.band_depth[b]);
+        if (q.band_depth[b] > 0) scattering = 1;
+        if (q.emission[b]) emission = 1;
+    }
+    std::vector<uint8_t> rundirect(K, 0), diffuse(K, 0);
+    for (uint32_t k = 0; k < K; k++) {
+        for (uint32_t s = 0; s < Sn; s++)
+            for (uint32_t b = 0; b < B; b++)
+                if (suns.flux[((size_t)k * Sn + s) * B + b] > 0.f) rundirect[k] = 1;
+        for (uint32_t b = 0; b < B; b++)
+            if (suns.dflux[(size_t)k * B + b] > 0.f) diffuse[k] = 1;
+    }
+    const uint32_t d = (uint32_t)std::ceil(std::sqrt((double)q.direct_rays));
+    const uint32_t n = (uint32_t)std::ceil(std::sqrt((double)q.diffuse_rays));
+    const uint32_t nn = n * n;
+    bool any_direct = false, any_primary = false;
+    for (uint32_t k = 0; k < K; k++) {
+        if (rundirect[k] && Sn > 0 && d > 0) any_direct = true;
+        if (emission || diffuse[k]) any_primary = true;
+    }
+    bool need_hits = (any_primary || scattering) && n > 0;
+
+    // ---- masks [nm][C] ----
+    enum { M_RD, M_AFTER, M_PRIM, M_EMIT, M_POST, M_NONE, M_BASE };
+    const uint32_t nm = M_BASE + 4 * std::max(maxdepth, 1u);
+    std::vector<uint8_t> mk((size_t)nm * C, 0);
+    auto M = [&](uint32_t m, uint32_t c) -> uint8_t & { return mk[(size_t)m * C + c]; };
+    for (uint32_t c = 0; c < C; c++) {
+        uint32_t k = c / B, b = c % B;
+        bool rd = rundirect[k] && Sn > 0 && d > 0;
+        bool prim = emission || diffuse[k];
+        M(M_RD, c) = rd;
+        M(M_AFTER, c) = scattering && rundirect[k];
+        M(M_PRIM, c) = prim && n > 0;
+        M(M_EMIT, c) = prim && q.emission[b];
+        M(M_POST, c) = scattering && prim && !rundirect[k];
+        bool loop = scattering && (emission || diffuse[k] || rundirect[k]);
+        bool was_active = loop;
+        for (uint32_t s = 0; s < maxdepth; s++) {
+            bool active = loop && (s + 1 <= q.band_depth[b]);
+            bool retire = (q.schedule == 1) && was_active && !active;
+            bool copy = loop && ((s > 0) || (q.schedule == 1 ? prim : (emission && rundirect[k])));
+            M(M_BASE + 4 * s + 0, c) = retire;
+            M(M_BASE + 4 * s + 1, c) = copy;
+            M(M_BASE + 4 * s + 2, c) = loop;
+            M(M_BASE + 4 * s + 3, c) = active && n > 0;
+            was_active = active;
+        }
+    }
+    std::vector<float> dfl((size_t)C, 0.f);
+    for (uint32_t c = 0; c < C; c++) dfl[c] = suns.dflux[(size_t)(c / B) * B + c % B];
+
+    // ---- workspace ----
+    size_t PC = (size_t)P * C;
+    size_t bytes = 5 * PC * 4 + (need_hits ? (size_t)P * 2 * nn * 4 : 0) + (size_t)K * Sn * P * 2 * 4 +
+                   (size_t)P * B * 4 * 3 + (size_t)P * 4 + (size_t)K * Sn * B * 4 + (size_t)C * 4 * 2 + mk.size() +
+                   (size_t)B * 4 + (size_t)P * 8 + (size_t)C * 8 * 3 + (size_t)K * Sn * sizeof(DirectJob) +
+                   (sizeof(DiffuseJob) + sizeof(GatherJob)) * (1 + maxdepth) + 64 * 64;
+    if (sx.ws.reserve(bytes)) return -1;
+    Carve cv{(char *)sx.ws.base};
+    float *in = cv.take<float>(PC), *stb = cv.take<float>(PC), *sbb = cv.take<float>(PC);
+    float *ot = cv.take<float>(PC), *ob = cv.take<float>(PC);
+    uint32_t *hits = need_hits ? cv.take<uint32_t>((size_t)P * 2 * nn) : nullptr;
+    float *dsum = cv.take<float>((size_t)K * Sn * P * 2 + 1);
+    float *rho = cv.take<float>((size_t)P * B), *tau = cv.take<float>((size_t)P * B);
+    float *eps = q.eps ? cv.take<float>((size_t)P * B) : nullptr;
+    float *temp = q.temperature ? cv.take<float>(P) : nullptr;
+    float *flux = cv.take<float>((size_t)K * Sn * B + 1);
+    float *dflux = cv.take<float>(C), *zflux = cv.take<float>(C);
+    uint8_t *masks = cv.take<uint8_t>(mk.size());
+    uint32_t *depth = cv.take<uint32_t>(B);
+    double *area = cv.take<double>(P);
+    double *energy = cv.take<double>((size_t)C * 3);
+    DirectJob *djobs = cv.take<DirectJob>((size_t)K * Sn + 1);
+    DiffuseJob *fjobs = cv.take<DiffuseJob>(1 + maxdepth);
+    GatherJob *gjobs = cv.take<GatherJob>(1 + maxdepth);
+    uint32_t *counter = cv.take<uint32_t>(16);
+    int *err = reinterpret_cast<int *>(counter + 8);
+
+    // ---- uploads (pageable host -> device, stream ordered) ----
+    CK(hipMemsetAsync(in, 0, PC * 4, st));
+    CK(hipMemsetAsync(stb, 0, PC * 4, st));
+    CK(hipMemsetAsync(sbb, 0, PC * 4, st));
+    CK(hipMemsetAsync(ot, 0, PC * 4, st));
+    CK(hipMemsetAsync(ob, 0, PC * 4, st));
+    CK(hipMemsetAsync(err, 0, sizeof(int), st));
+    CK(hipMemcpyAsync(rho, q.rho, (size_t)P * B * 4, hipMemcpyHostToDevice, st));
+    CK(hipMemcpyAsync(tau, q.tau, (size_t)P * B * 4, hipMemcpyHostToDevice, st));
+    if (eps) CK(hipMemcpyAsync(eps, q.eps, (size_t)P * B * 4, hipMemcpyHostToDevice, st));
+    if (temp) CK(hipMemcpyAsync(temp, q.temperature, (size_t)P * 4, hipMemcpyHostToDevice, st));
+    if (Sn) CK(hipMemcpyAsync(flux, suns.flux, (size_t)K * Sn * B * 4, hipMemcpyHostToDevice, st));
+    CK(hipMemcpyAsync(dflux, dfl.data(), (size_t)C * 4, hipMemcpyHostToDevice, st));
+    CK(hipMemsetAsync(zflux, 0, (size_t)C * 4, st));
+    CK(hipMemcpyAsync(masks, mk.data(), mk.size(), hipMemcpyHostToDevice, st));
+    CK(hipMemcpyAsync(depth, q.band_depth, (size_t)B * 4, hipMemcpyHostToDevice, st));
+
+    std::vector<DirectJob> hd;
+    uint32_t items = 0;
+    if (any_direct) {
+        for (uint32_t k = 0; k < K; k++) {
+            if (!rundirect[k]) continue;
+            for (uint32_t s = 0; s < Sn; s++) {
+                DirectJob j{};
+                j.sc = S.d;
+                const float *sd = suns.dir + ((size_t)k * Sn + s) * 3;
+                j.sdir[0] = sd[0]; j.sdir[1] = sd[1]; j.sdir[2] = sd[2];
+                j.d = d;
+                j.seed = suns.seed[k];
+                j.out = dsum + ((size_t)k * Sn + s) * P * 2;
+                j.item_begin = items;
+                j.P = P;
+                items += P;
+                hd.push_back(j);
+            }
+        }
+        CK(hipMemcpyAsync(djobs, hd.data(), hd.size() * sizeof(DirectJob), hipMemcpyHostToDevice, st));
+    }
+    std::vector<DiffuseJob> hf(1 + maxdepth);
+    for (uint32_t g = 0; g < hf.size(); g++) {
+        DiffuseJob &fj = hf[g];
+        fj.sc = S.d; fj.n = n; fj.seed = q.diffuse_seed[g]; fj.hits = hits; fj.item_begin = 0; fj.P = P;
+    }
+    CK(hipMemcpyAsync(fjobs, hf.data(), hf.size() * sizeof(DiffuseJob), hipMemcpyHostToDevice, st));
+    std::vector<GatherJob> hg(1 + maxdepth);
+    for (uint32_t g = 0; g < hg.size(); g++) {
+        GatherJob &J = hg[g];
+        J.sc = S.d; J.hits = hits; J.n = n; J.C = C; J.B = B;
+        J.out_top = ot; J.out_bot = ob;
+        J.dflux = g == 0 ? dflux : zflux;
+        J.active = g == 0 ? masks + (size_t)M_PRIM * C : masks + (size_t)(M_BASE + 4 * (g - 1) + 3) * C;
+        J.rho = rho; J.tau = tau; J.in = in; J.st = stb; J.sb = sbb;
+        J.item_begin = 0; J.P = P;
+    }
+    CK(hipMemcpyAsync(gjobs, hg.data(), hg.size() * sizeof(GatherJob), hipMemcpyHostToDevice, st));
+    // the host vectors above must outlive the async copies
+    CK(hipStreamSynchronize(st));
+
+    k_area<<<nb(P), 256, 0, st>>>(P, S.h.T, S.h.type, area);
+    LK();
+
+    // ---- DIRECT (RadiationModel.cpp:3847-3982) ----
+    if (any_direct) {
+        hipEvent_t a, b;
+        if (ev_pair(sx, a, b)) return -1;
+        if (a) CK(hipEventRecord(a, st));
+        if (launch_direct(djobs, (uint32_t)hd.size(), items, counter, err, sx.cfg[0], sx.ncu, st)) return -1;
+        if (b) { CK(hipEventRecord(b, st)); tick(sx, 0, a, b); }
+        sx.timing_info.direct_rays += (uint64_t)items * d * d;
+        sx.timing_info.direct_launches++;
+        k_deposit_direct<<<nb(PC), 256, 0, st>>>(P, C, B, Sn, dsum, flux, rho, tau, masks + (size_t)M_RD * C, in,
+                                                   stb, sbb, nullptr);
+        LK();
+    }
+    if (scattering) {
+        k_after_direct<<<nb(PC), 256, 0, st>>>(P, C, S.h.nrm, masks + (size_t)M_AFTER * C, stb, sbb, ot, ob);
+        LK();
+    }
+    auto diffuse_pass = [&](uint32_t g) -> int {
+        hipEvent_t a, b;
+        if (ev_pair(sx, a, b)) return -1;
+        if (a) CK(hipEventRecord(a, st));
+        if (launch_diffuse(fjobs + g, 1, P, counter, err, sx.cfg[1], sx.ncu, st)) return -1;
+        if (b) { CK(hipEventRecord(b, st)); tick(sx, 1, a, b); }
+        sx.timing_info.diffuse_rays += (uint64_t)P * 2 * nn;
+        sx.timing_info.diffuse_launches++;
+        hipEvent_t c2, d2;
+        if (ev_pair(sx, c2, d2)) return -1;
+        if (c2) CK(hipEventRecord(c2, st));
+        if (launch_gather(gjobs + g, 1, P, C, counter, sx.cfg[2], sx.ncu, st)) return -1;
+        if (d2) { CK(hipEventRecord(d2, st)); tick(sx, 2, c2, d2); }
+        return 0;
+    };
+    // ---- EMISSION + PRIMARY DIFFUSE (RadiationModel.cpp:4040-4222) ----
+    if (any_primary && n > 0) {
+        if (emission) {
+            k_emission<<<nb(PC), 256, 0, st>>>(P, C, B, S.h.nrm, eps, temp, depth, masks + (size_t)M_EMIT * C, ot, ob);
+            LK();
+        }
+        if (diffuse_pass(0)) return -1;
+    }
+    if (scattering) {
+        // !rundirect: radiation_out := scatter after the primary pass (RadiationModel.cpp:4225-4234)
+        const uint8_t *none = masks + (size_t)M_NONE * C;  // all-zero row
+        k_copy_zero<<<nb(PC), 256, 0, st>>>(P, C, none, masks + (size_t)M_POST * C, masks + (size_t)M_POST * C, in,
+                                             stb, sbb, ot, ob);
+        LK();
+        // ---- SCATTERING ITERATIONS (RadiationModel.cpp:4236-4356) ----
+        for (uint32_t s = 0; s < maxdepth; s++) {
+            const uint8_t *row = masks + (size_t)(M_BASE + 4 * s) * C;
+            k_copy_zero<<<nb(PC), 256, 0, st>>>(P, C, row, row + C, row + 2 * C, in, stb, sbb, ot, ob);
+            LK();
+            bool any_active = false;
+            for (uint32_t c = 0; c < C; c++) any_active |= mk[(size_t)(M_BASE + 4 * s + 3) * C + c] != 0;
+            if (any_active && n > 0)
+                if (diffuse_pass(1 + s)) return -1;
+        }
+    }
+    // ---- final (RadiationModel.cpp:4631-4654) ----
+    float *R = R_out;
+    float *Rd = nullptr;
+    if (!out_on_device) {
+        Rd = ot;  // reuse a [P][C] buffer: radiation_out is dead here
+        R = Rd;
+    }
+    k_final<<<nb(PC), 256, 0, st>>>(P, C, B, in, stb, sbb, R);
+    LK();
+    k_energy<<<C, 256, 0, st>>>(P, C, B, Sn, nullptr, area, R, any_direct ? dsum : nullptr, flux, energy);
+    LK();
+    if (!out_on_device) CK(hipMemcpyAsync(R_out, Rd, PC * 4, hipMemcpyDeviceToHost, st));
+    sx.energy.assign((size_t)C * 3, 0.0);
+    CK(hipMemcpyAsync(sx.energy.data(), energy, (size_t)C * 2 * 8, hipMemcpyDeviceToHost, st));
+    CK(hipStreamSynchronize(st));
+    int herr = 0;
+    CK(hipMemcpy(&herr, err, sizeof(int), hipMemcpyDeviceToHost));
+    if (herr) { set_error("traversal error code " + std::to_string(herr) + " (1 = stack overflow, 2 = malformed tree)"); return -1; }
+    for (auto &e : sx.ev) {
+        float ms = 0.f;
+        CK(hipEventElapsedTime(&ms, e.a, e.b));
+        if (e.cls == 0) sx.timing_info.direct_ms += ms;
+        else if (e.cls == 1) sx.timing_info.diffuse_ms += ms;
+        else sx.timing_info.gather_ms += ms;
+    }
+    return 0;
+}
+
+int debug_diffuse_hits(SolveCtx &sx, const Scene &S, uint32_t n, uint32_t seed, uint32_t *out) {
+    hipStream_t st = sx.stream;
+    const uint32_t P = S.h.P, nn = n * n;
+    size_t bytes = (size_t)P * 2 * nn * 4 + sizeof(DiffuseJob) + 4096;
+    if (sx.ws.reserve(bytes)) return -1;
+    Carve cv{(char *)sx.ws.base};
+    uint32_t *hits = cv.take<uint32_t>((size_t)P * 2 * nn);
+    DiffuseJob *fjob = cv.take<DiffuseJob>(1);
+    uint32_t *counter = cv.take<uint32_t>(16);
+    int *err = reinterpret_cast<int *>(counter + 8);
+    DiffuseJob fj{};
+    fj.sc = S.d; fj.n = n; fj.seed = seed; fj.hits = hits; fj.item_begin = 0; fj.P = P;
+    CK(hipMemsetAsync(err, 0, 4, st));
+    CK(hipMemcpyAsync(fjob, &fj, sizeof(fj), hipMemcpyHostToDevice, st));
+    if (launch_diffuse(fjob, 1, P, counter, err, sx.cfg[1], sx.ncu, st)) return -1;
+    CK(hipMemcpyAsync(out, hits, (size_t)P * 2 * nn * 4, hipMemcpyDeviceToHost, st));
+    CK(hipStreamSynchronize(st));
+    int herr = 0;
+    CK(hipMemcpy(&herr, err, sizeof(int), hipMemcpyDeviceToHost));
+    if (herr) { set_error("traversal error code " + std::to_string(herr) + " (1 = stack overflow, 2 = malformed tree)"); return -1; }
+    return 0;
+}
+
+int debug_direct_sums(SolveCtx &sx, const Scene &S, uint32_t d, uint32_t seed, uint32_t nsrc, const float *src_dir,
+                      float *out) {
+    hipStream_t st = sx.stream;
+    const uint32_t P = S.h.P;
+    size_t bytes = (size_t)nsrc * P * 2 * 4 + nsrc * sizeof(DirectJob) + 4096;
+    if (sx.ws.reserve(bytes)) return -1;
+    Carve cv{(char *)sx.ws.base};
+    float *sums = cv.take<float>((size_t)nsrc * P * 2);
+    DirectJob *jobs = cv.take<DirectJob>(nsrc);
+    uint32_t *counter = cv.take<uint32_t>(16);
+    int *err = reinterpret_cast<int *>(counter + 8);
+    std::vector<DirectJob> hj(nsrc);
+    for (uint32_t s = 0; s < nsrc; s++) {
+        DirectJob &j = hj[s];
+        j.sc = S.d;
+        j.sdir[0] = src_dir[s * 3]; j.sdir[1] = src_dir[s * 3 + 1]; j.sdir[2] = src_dir[s * 3 + 2];
+        j.d = d; j.seed = seed; j.out = sums + (size_t)s * P * 2; j.item_begin = s * P; j.P = P;
+    }
+    CK(hipMemsetAsync(err, 0, 4, st));
+    CK(hipMemcpyAsync(jobs, hj.data(), nsrc * sizeof(DirectJob), hipMemcpyHostToDevice, st));
+    if (launch_direct(jobs, nsrc, nsrc * P, counter, err, sx.cfg[0], sx.ncu, st)) return -1;
+    std::vector<float> tmp((size_t)nsrc * P * 2);
+    CK(hipMemcpyAsync(tmp.data(), sums, tmp.size() * 4, hipMemcpyDeviceToHost, st));
+    CK(hipStreamSynchronize(st));
+    int herr = 0;
+    CK(hipMemcpy(&herr, err, sizeof(int), hipMemcpyDeviceToHost));
+    if (herr) { set_error("traversal error code " + std::to_string(herr) + " (1 = stack overflow, 2 = malformed tree)"); return -1; }
+    // [S][P][2] -> [P][S][2]
+    for (uint32_t s = 0; s < nsrc; s++)
+        for (uint32_t i = 0; i < P; i++) {
+            out[((size_t)i * nsrc + s) * 2] = tmp[((size_t)s * P + i) * 2];
+            out[((size_t)i * nsrc + s) * 2 + 1] = tmp[((size_t)s * P + i) * 2 + 1];
+        }
+    return 0;
+}
+
+}  // namespace rt

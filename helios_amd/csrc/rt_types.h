@@ -1,0 +1,80 @@
+// rt_types.h -- device-resident scene and job descriptors shared by the kernels and the host runtime.
+#pragma once
+
+#include <stdint.h>
+
+#include "rt_device.h"
+
+// Traversal stack: one LDS column per lane.  The LBVH build rejects trees deeper than this.
+#ifndef RT_STACK
+#define RT_STACK 64
+#endif
+
+#define RT_LEAF_BIT 0x80000000u
+#define RT_NOT_LAUNCHED 0xFFFFFFFEu
+#define RT_HIT_BOTTOM 0x80000000u
+
+// Periodic boundary face in predicate form (R/shaders/common/periodic_boundary.glsl:28-59):
+// anchor v0, edge a = v1-v0, edge b = v3-v0, plane normal n = normalize(cross(v1-v0, v2-v0)).
+struct BBoxFace {
+    float v0[3], a[3], b[3], n[3], aa, bb;
+};
+
+// Everything a kernel needs to trace one scene.  Lives in device memory; kernels read it through a
+// pointer held in the job descriptor.
+struct DevScene {
+    const float4 *rec;     // [P][4] primitive records in leaf (Morton) order
+    const float4 *node;    // [P-1][4] BVH2 nodes: both child boxes + child links
+    const float4 *nrm;     // [P] world normal .xyz, .w = bits(twosided flag)
+    const float4 *frame;   // [P] (cos elev, sin elev, cos az, sin az) of the normal
+    const float *T;        // [P][16] row-major transforms
+    const uint32_t *type;  // [P] 0 patch, 1 triangle, 3 tile, other = not traced
+    const uint32_t *uuid;  // [P]
+    uint32_t P;            // primitives (original order)
+    uint32_t nleaf;        // traced primitives (leaves)
+    uint32_t depth;        // max leaf depth
+    uint32_t nbbox;        // periodic faces (0, 2 or 4)
+    int px, py;
+    float dom[4];          // xmin, xmax, ymin, ymax of the periodic faces
+    BBoxFace bbox[4];
+};
+
+// one direct trace job: one scene, one collimated source
+struct DirectJob {
+    const DevScene *sc;
+    float sdir[3];         // raw direction towards the source (normalised in-kernel)
+    uint32_t d;            // rays per side (d*d rays per primitive)
+    uint32_t seed;
+    float *out;            // [P][2] strength sums lit-from-top / lit-from-bottom
+    uint32_t item_begin;   // first work item of this job
+    uint32_t P;
+};
+
+// one diffuse trace pass: writes hit words [P][2][n*n]
+struct DiffuseJob {
+    const DevScene *sc;
+    uint32_t n;
+    uint32_t seed;
+    uint32_t *hits;
+    uint32_t item_begin;
+    uint32_t P;
+};
+
+// one gather over a hit table: C channels, band of channel c is c % B
+struct GatherJob {
+    const DevScene *sc;
+    const uint32_t *hits;  // [P][2][n*n]
+    uint32_t n;
+    uint32_t C, B;
+    const float *out_top;  // [P][C]
+    const float *out_bot;  // [P][C]
+    const float *dflux;    // [C] sky flux (0 in scattering passes)
+    const uint8_t *active; // [C]
+    const float *rho;      // [P][B]
+    const float *tau;      // [P][B]
+    float *in;             // [P][C]
+    float *st;             // [P][C]
+    float *sb;             // [P][C]
+    uint32_t item_begin;
+    uint32_t P;
+};

@@ -1,0 +1,338 @@
+"""ctypes binding of ``libhelios_rt.so`` (the C ABI declared in ``include/helios_rt.h``).
+
+There is no fallback: if the library is missing or no gfx950 device is present, every entry point
+raises :class:`RTError` naming the reason.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhelios_rt.so")
+
+u32p = C.POINTER(C.c_uint32)
+f32p = C.POINTER(C.c_float)
+f64p = C.POINTER(C.c_double)
+
+
+class RTError(RuntimeError):
+    pass
+
+
+class rt_band_request(C.Structure):
+    _fields_ = [
+        ("nbands", C.c_uint32), ("nsources", C.c_uint32),
+        ("direct_rays", C.c_uint32), ("diffuse_rays", C.c_uint32),
+        ("band_depth", u32p), ("emission", u32p),
+        ("source_dir", f32p), ("source_flux", f32p), ("diffuse_flux", f32p),
+        ("rho", f32p), ("tau", f32p), ("eps", f32p), ("temperature", f32p),
+        ("direct_seed", C.c_uint32), ("diffuse_seed", u32p), ("schedule", C.c_uint32),
+    ]
+
+
+class rt_sun_batch(C.Structure):
+    _fields_ = [("nsuns", C.c_uint32), ("source_dir", f32p), ("source_flux", f32p),
+                ("diffuse_flux", f32p), ("direct_seed", u32p)]
+
+
+class rt_scene_info(C.Structure):
+    _fields_ = [("nprims", C.c_uint32), ("nleaves", C.c_uint32), ("depth", C.c_uint32), ("nbbox", C.c_uint32),
+                ("domain", C.c_float * 4), ("device_bytes", C.c_uint64)]
+
+
+class rt_launch_cfg(C.Structure):
+    _fields_ = [("waves_per_block", C.c_uint32), ("blocks_per_cu", C.c_uint32)]
+
+
+class rt_timing(C.Structure):
+    _fields_ = [("direct_ms", C.c_double), ("diffuse_ms", C.c_double), ("gather_ms", C.c_double),
+                ("other_ms", C.c_double), ("direct_rays", C.c_uint64), ("diffuse_rays", C.c_uint64),
+                ("direct_launches", C.c_uint32), ("diffuse_launches", C.c_uint32)]
+
+
+# name -> (restype, argtypes); the ABI test checks that every declared symbol is exported
+SIGNATURES = {
+    "rt_last_error": (C.c_char_p, []),
+    "rt_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "rt_destroy": (C.c_int, [C.c_void_p]),
+    "rt_backend_name": (C.c_char_p, [C.c_void_p]),
+    "rt_query_memory": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "rt_scene_create": (C.c_int, [C.c_void_p, C.c_uint32, f32p, u32p, u32p, u32p, C.c_int, C.c_int,
+                                  C.POINTER(C.c_void_p)]),
+    "rt_scene_destroy": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "rt_scene_get_info": (C.c_int, [C.c_void_p, C.POINTER(rt_scene_info)]),
+    "rt_cast_rays_soa": (C.c_int, [C.c_void_p, C.c_void_p, f32p, f32p, C.c_size_t, C.c_float, f32p, f32p, u32p]),
+    "rt_cast_rays_soa_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_float,
+                                          C.c_void_p, C.c_void_p, C.c_void_p]),
+    "rt_run_band": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(rt_band_request), f32p]),
+    "rt_run_band_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(rt_band_request), C.POINTER(rt_sun_batch),
+                                    C.c_void_p, C.c_int]),
+    "rt_energy_residual_fp64": (C.c_int, [C.c_void_p, f64p, C.c_uint32]),
+    "rt_set_launch_cfg": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(rt_launch_cfg)]),
+    "rt_get_launch_cfg": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(rt_launch_cfg)]),
+    "rt_enable_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "rt_get_timing": (C.c_int, [C.c_void_p, C.POINTER(rt_timing)]),
+    "rt_debug_diffuse_hits": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, u32p]),
+    "rt_debug_direct_sums": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, f32p, f32p]),
+    "rt_debug_diffuse_rays": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, f32p]),
+    "rt_debug_trace": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, f32p, u32p, C.c_float, C.c_int, u32p, f32p]),
+}
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libhelios_rt.so and bind every declared signature (raises RTError if absent)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RTError(f"{path} is not built: run __graft_entry__.build() (no CPU fallback exists)")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise RTError(_lib.rt_last_error().decode())
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _u32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def _pf(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(f32p)
+
+
+def _pu(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(u32p)
+
+
+@dataclass
+class BandSpec:
+    """Per-band settings of one runBand call (names follow RadiationModel's setters)."""
+    label: str
+    scattering_depth: int = 0
+    emission: bool = False
+    diffuse_flux: float = 0.0
+
+
+class Context:
+    """One gfx950 device context (RayTracingBackend equivalent)."""
+
+    def __init__(self, device: int = 0):
+        lib = load_library()
+        h = C.c_void_p()
+        _check(lib.rt_create(device, C.byref(h)))
+        self._h = h
+        self.lib = lib
+
+    @property
+    def name(self) -> str:
+        return self.lib.rt_backend_name(self._h).decode()
+
+    def close(self):
+        if self._h:
+            self.lib.rt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def memory(self):
+        f, t = C.c_size_t(), C.c_size_t()
+        _check(self.lib.rt_query_memory(self._h, C.byref(f), C.byref(t)))
+        return f.value, t.value
+
+    def scene(self, transforms, types, twosided, uuids=None, periodic=(False, False)) -> "Scene":
+        return Scene(self, transforms, types, twosided, uuids, periodic)
+
+    def set_launch_cfg(self, kernel: int, waves_per_block: int, blocks_per_cu: int):
+        cfg = rt_launch_cfg(waves_per_block, blocks_per_cu)
+        _check(self.lib.rt_set_launch_cfg(self._h, kernel, C.byref(cfg)))
+
+    def get_launch_cfg(self, kernel: int):
+        cfg = rt_launch_cfg()
+        _check(self.lib.rt_get_launch_cfg(self._h, kernel, C.byref(cfg)))
+        return cfg.waves_per_block, cfg.blocks_per_cu
+
+    def enable_timing(self, on: bool = True):
+        _check(self.lib.rt_enable_timing(self._h, 1 if on else 0))
+
+    def timing(self) -> dict:
+        t = rt_timing()
+        _check(self.lib.rt_get_timing(self._h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in rt_timing._fields_}
+
+
+class Scene:
+    """A resident scene (geometry + LBVH in HBM)."""
+
+    def __init__(self, ctx: Context, transforms, types, twosided, uuids=None, periodic=(False, False)):
+        self.ctx = ctx
+        T = _f32(transforms).reshape(-1, 16)
+        self.P = T.shape[0]
+        ty = _u32(types)
+        tw = _u32(twosided)
+        uu = None if uuids is None else _u32(uuids)
+        h = C.c_void_p()
+        _check(ctx.lib.rt_scene_create(ctx._h, self.P, _pf(T), _pu(ty), _pu(uu), _pu(tw), int(periodic[0]),
+                                       int(periodic[1]), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            self.ctx.lib.rt_scene_destroy(self.ctx._h, self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> dict:
+        i = rt_scene_info()
+        _check(self.ctx.lib.rt_scene_get_info(self._h, C.byref(i)))
+        return {"nprims": i.nprims, "nleaves": i.nleaves, "depth": i.depth, "nbbox": i.nbbox,
+                "domain": list(i.domain), "device_bytes": i.device_bytes}
+
+    # ---- CollisionDetection::castRaysSoA ----
+    def cast_rays(self, origins, dirs, max_dist: float = -1.0):
+        O = _f32(origins).reshape(-1, 3)
+        D = _f32(dirs).reshape(-1, 3)
+        n = O.shape[0]
+        t = np.full(n, np.nan, np.float32)
+        nrm = np.full((n, 3), np.nan, np.float32)
+        uu = np.zeros(n, np.uint32)
+        _check(self.ctx.lib.rt_cast_rays_soa(self.ctx._h, self._h, _pf(O), _pf(D), n, max_dist, _pf(t), _pf(nrm),
+                                             _pu(uu)))
+        return t, nrm, uu
+
+    def cast_rays_device(self, d_origins: int, d_dirs: int, n: int, max_dist: float, d_t: int, d_n: int, d_u: int):
+        _check(self.ctx.lib.rt_cast_rays_soa_device(self.ctx._h, self._h, C.c_void_p(d_origins), C.c_void_p(d_dirs),
+                                                    n, max_dist, C.c_void_p(d_t), C.c_void_p(d_n), C.c_void_p(d_u)))
+
+    # ---- parity hooks ----
+    def diffuse_hits(self, n: int, seed: int) -> np.ndarray:
+        out = np.zeros((self.P, 2, n * n), np.uint32)
+        _check(self.ctx.lib.rt_debug_diffuse_hits(self.ctx._h, self._h, n, seed, _pu(out)))
+        return out
+
+    def diffuse_rays(self, n: int, seed: int) -> np.ndarray:
+        out = np.zeros((self.P, 2, n * n, 6), np.float32)
+        _check(self.ctx.lib.rt_debug_diffuse_rays(self.ctx._h, self._h, n, seed, _pf(out)))
+        return out
+
+    def trace(self, rays, skip, tmin: float, brute: bool = False):
+        R = _f32(rays).reshape(-1, 6)
+        S = _u32(skip).reshape(-1)
+        n = R.shape[0]
+        h = np.zeros(n, np.uint32)
+        t = np.zeros(n, np.float32)
+        _check(self.ctx.lib.rt_debug_trace(self.ctx._h, self._h, n, _pf(R), _pu(S), tmin, int(brute), _pu(h), _pf(t)))
+        return h, t
+
+    def direct_sums(self, d: int, seed: int, src_dirs) -> np.ndarray:
+        S = _f32(src_dirs).reshape(-1, 3)
+        out = np.zeros((self.P, S.shape[0], 2), np.float32)
+        _check(self.ctx.lib.rt_debug_direct_sums(self.ctx._h, self._h, d, seed, S.shape[0], _pf(S), _pf(out)))
+        return out
+
+
+class BandRequest:
+    """All inputs of one runBand call except the per-sun part (owns its numpy buffers)."""
+
+    def __init__(self, P: int, bands: Sequence[BandSpec], rho, tau, direct_rays: int, diffuse_rays: int,
+                 source_dirs=None, source_flux=None, direct_seed: int = 1, diffuse_seeds=None, eps=None,
+                 temperature=None, schedule: int = 0):
+        B = len(bands)
+        self.P, self.B = P, B
+        self.bands = list(bands)
+        self.depth = _u32([b.scattering_depth for b in bands])
+        self.emission = _u32([1 if b.emission else 0 for b in bands])
+        self.dflux = _f32([b.diffuse_flux for b in bands])
+        maxd = int(self.depth.max()) if B else 0
+        if diffuse_seeds is None:
+            diffuse_seeds = [1000 + k for k in range(maxd + 1)]
+        self.dseeds = _u32(list(diffuse_seeds) + [0] * max(0, maxd + 1 - len(diffuse_seeds)))
+        self.rho = _f32(rho).reshape(P, B)
+        self.tau = _f32(tau).reshape(P, B)
+        self.eps = None if eps is None else _f32(eps).reshape(P, B)
+        self.temp = None if temperature is None else _f32(temperature).reshape(P)
+        self.sdir = _f32(source_dirs if source_dirs is not None else np.zeros((0, 3))).reshape(-1, 3)
+        S = self.sdir.shape[0]
+        self.sflux = _f32(source_flux if source_flux is not None else np.zeros((S, B))).reshape(S, B)
+        self.direct_seed = int(direct_seed)
+        self.direct_rays, self.diffuse_rays = int(direct_rays), int(diffuse_rays)
+        self.schedule = int(schedule)
+
+    def c_struct(self) -> rt_band_request:
+        return rt_band_request(
+            self.B, self.sdir.shape[0], self.direct_rays, self.diffuse_rays, _pu(self.depth), _pu(self.emission),
+            _pf(self.sdir), _pf(self.sflux), _pf(self.dflux), _pf(self.rho), _pf(self.tau), _pf(self.eps),
+            _pf(self.temp), self.direct_seed, _pu(self.dseeds), self.schedule)
+
+
+def run_band(scene: Scene, req: BandRequest) -> np.ndarray:
+    """Whole runBand on the device -> radiation_flux [P][B]."""
+    R = np.zeros((scene.P, req.B), np.float32)
+    cs = req.c_struct()
+    _check(scene.ctx.lib.rt_run_band(scene.ctx._h, scene._h, C.byref(cs), _pf(R)))
+    return R
+
+
+def run_band_batch(scene: Scene, req: BandRequest, sun_dirs, sun_flux, sky_flux, direct_seeds,
+                   out_device_ptr: Optional[int] = None) -> Optional[np.ndarray]:
+    """K suns sharing bands/materials/diffuse seeds -> radiation_flux [K][P][B]."""
+    D = _f32(sun_dirs)
+    K = D.shape[0]
+    D = D.reshape(K, -1, 3)
+    F = _f32(sun_flux).reshape(K, D.shape[1], req.B)
+    Q = _f32(sky_flux).reshape(K, req.B)
+    seeds = _u32(direct_seeds).reshape(K)
+    sb = rt_sun_batch(K, _pf(D), _pf(F), _pf(Q), _pu(seeds))
+    cs = req.c_struct()
+    if out_device_ptr is not None:
+        _check(scene.ctx.lib.rt_run_band_batch(scene.ctx._h, scene._h, C.byref(cs), C.byref(sb),
+                                               C.c_void_p(out_device_ptr), 1))
+        return None
+    R = np.zeros((K, scene.P, req.B), np.float32)
+    _check(scene.ctx.lib.rt_run_band_batch(scene.ctx._h, scene._h, C.byref(cs), C.byref(sb),
+                                           R.ctypes.data_as(C.c_void_p), 0))
+    return R
+
+
+def energy_fp64(ctx: Context, nchannels: int) -> np.ndarray:
+    out = np.zeros((3, nchannels), np.float64)
+    _check(ctx.lib.rt_energy_residual_fp64(ctx._h, out.ctypes.data_as(f64p), nchannels))
+    return out
+
+
+def sun_direction(zenith_deg: float, azimuth_deg: float) -> np.ndarray:
+    """Direction towards the sun (Helios sphere2cart: azimuth from +y towards +x,
+    core/src/global.cpp:833-835)."""
+    z = math.radians(zenith_deg)
+    a = math.radians(azimuth_deg)
+    return np.array([math.sin(z) * math.sin(a), math.sin(z) * math.cos(a), math.cos(z)], np.float32)

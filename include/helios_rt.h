@@ -1,0 +1,178 @@
+/*
+ * helios_rt.h -- C ABI of the MI355X (gfx950) radiation-transfer backend.
+ *
+ * Plain C: opaque handles, POD arrays, sizes, int status (0 = success, -1 = failure with the
+ * message in rt_last_error()).  No C++ or torch types cross this boundary.  Each entry point names
+ * the reference interface it replaces (Helios tree: R = plugins/radiation, CD = plugins/collisiondetection).
+ *
+ * Conventions shared with the reference:
+ *   - primitive arrays are in the order the caller passes them; that position is the "primitive
+ *     index" that seeds the per-ray RNG (R/shaders/direct_raygen.comp:360-362);
+ *   - transforms are 4x4 row-major; canonical patch (-0.5..0.5)^2, canonical triangle
+ *     (0,0,0),(0,1,0),(1,1,0) (R/src/backends/VulkanComputeBackend.cpp:328-329);
+ *   - types: 0 patch, 1 triangle, 3 tile (traced as patch); others are carried but not traced
+ *     (the Vulkan backend traces patch/tile/triangle only, R/shaders/common/bvh_traversal.glsl:311-325);
+ *   - twosided: raw Helios flag 0..3 (R/src/RadiationModel.cpp:5941);
+ *   - radiation buffers are [primitive][band] (R/include/BufferIndexing.h:65-262).
+ */
+#ifndef HELIOS_RT_H
+#define HELIOS_RT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rt_ctx rt_ctx;
+typedef struct rt_scene rt_scene;
+
+#define RT_MISS_UUID 0xFFFFFFFFu
+
+/* Message of the last failed call on this thread. */
+const char *rt_last_error(void);
+
+/* ---------------------------------------------------------------------------------------------
+ * Context.  Replaces RayTracingBackend::create("auto") + initialize()
+ * (R/src/backends/BackendFactory.cpp:46-105, R/include/RayTracingBackend.h:54) and shutdown()
+ * (RayTracingBackend.h:61).  Fails (returns -1) when no gfx950 device is usable: there is no CPU
+ * fallback.
+ * ------------------------------------------------------------------------------------------- */
+int rt_create(int device, rt_ctx **out);
+int rt_destroy(rt_ctx *ctx);
+/* RayTracingBackend::getBackendName (RayTracingBackend.h:272) */
+const char *rt_backend_name(const rt_ctx *ctx);
+/* RayTracingBackend::queryGPUMemory (RayTracingBackend.h:265) */
+int rt_query_memory(rt_ctx *ctx, size_t *free_bytes, size_t *total_bytes);
+
+/* ---------------------------------------------------------------------------------------------
+ * Resident scenes.  Replaces RayTracingBackend::updateGeometry + buildAccelerationStructure
+ * (RayTracingBackend.h:73, :81; VulkanComputeBackend.cpp:213-590) and CollisionDetection::buildBVH
+ * (CD/src/CollisionDetection.cpp:341-513): world vertices, normals and a Morton LBVH are built on
+ * the GPU and stay in HBM until rt_scene_destroy.
+ * ------------------------------------------------------------------------------------------- */
+int rt_scene_create(rt_ctx *ctx, uint32_t nprims, const float *transforms, const uint32_t *types,
+                    const uint32_t *uuids, const uint32_t *twosided, int periodic_x, int periodic_y,
+                    rt_scene **out);
+int rt_scene_destroy(rt_ctx *ctx, rt_scene *scene);
+
+typedef struct {
+    uint32_t nprims;       /* primitives passed in                        */
+    uint32_t nleaves;      /* traced primitives                           */
+    uint32_t depth;        /* deepest leaf of the BVH                     */
+    uint32_t nbbox;        /* periodic boundary faces                     */
+    float domain[4];       /* xmin, xmax, ymin, ymax of the periodic faces */
+    uint64_t device_bytes; /* resident bytes                              */
+} rt_scene_info;
+int rt_scene_get_info(const rt_scene *scene, rt_scene_info *info);
+
+/* ---------------------------------------------------------------------------------------------
+ * CollisionDetection::castRaysSoA (CD/include/CollisionDetection.h:395,
+ * CD/src/CollisionDetection_RayTracing.cpp:551-680): closest hit per ray with the CD predicates.
+ * origins/dirs [n][3] (directions are not normalised, as on the reference CPU path); max_dist <= 0
+ * means unbounded; a miss writes RT_MISS_UUID and leaves out_t / out_normal untouched.
+ * ------------------------------------------------------------------------------------------- */
+int rt_cast_rays_soa(rt_ctx *ctx, const rt_scene *scene, const float *origins, const float *dirs, size_t n,
+                     float max_dist, float *out_t, float *out_normal, uint32_t *out_uuid);
+/* Same, with every array already in device memory; runs on the context stream and waits for it. */
+int rt_cast_rays_soa_device(rt_ctx *ctx, const rt_scene *scene, const float *d_origins, const float *d_dirs,
+                            size_t n, float max_dist, float *d_out_t, float *d_out_normal, uint32_t *d_out_uuid);
+
+/* ---------------------------------------------------------------------------------------------
+ * Whole runBand on the device.  Replaces the pass schedule of RadiationModel::runBand
+ * (R/src/RadiationModel.cpp:3470-4672: direct pass, primary diffuse pass on both faces, scattering
+ * iterations, final radiation_flux = radiation_in + scatter_top + scatter_bottom) with no host
+ * round trips between passes.  Seeds are explicit (the reference draws them from the clock).
+ * ------------------------------------------------------------------------------------------- */
+#define RT_SCHEDULE_REFERENCE 0u  /* pass schedule exactly as RadiationModel.cpp:4236-4356          */
+#define RT_SCHEDULE_CONSERVING 1u /* also propagates the primary pass's scatter when emission is off */
+
+typedef struct {
+    uint32_t nbands;             /* B, launched bands (alphabetical order)                         */
+    uint32_t nsources;           /* S, collimated sources                                          */
+    uint32_t direct_rays;        /* max directRayCount over the bands (d = ceil(sqrt))             */
+    uint32_t diffuse_rays;       /* max diffuseRayCount over the bands (n = ceil(sqrt))            */
+    const uint32_t *band_depth;  /* [B] scatteringDepth                                            */
+    const uint32_t *emission;    /* [B] emission flag                                              */
+    const float *source_dir;     /* [S][3] direction towards each collimated source                */
+    const float *source_flux;    /* [S][B]                                                         */
+    const float *diffuse_flux;   /* [B]                                                            */
+    const float *rho;            /* [P][B] reflectivity (validated, source-0 slice)                */
+    const float *tau;            /* [P][B] transmissivity                                          */
+    const float *eps;            /* [P][B] emissivity, NULL = 1                                    */
+    const float *temperature;    /* [P] kelvin, NULL = 300                                         */
+    uint32_t direct_seed;
+    const uint32_t *diffuse_seed;/* [1 + max depth]: primary diffuse pass, then each scattering pass */
+    uint32_t schedule;           /* RT_SCHEDULE_*                                                   */
+} rt_band_request;
+
+/* R_out: [P][B] radiation_flux_<band> (host memory). */
+int rt_run_band(rt_ctx *ctx, const rt_scene *scene, const rt_band_request *req, float *R_out);
+
+/* Batched solves of one scene: K suns (own directions, fluxes, sky flux and direct seed) share
+ * the bands, materials and diffuse seeds of `base`, so every diffuse/scattering pass is traced once
+ * and gathered for all K*B channels.  Each sun's result equals rt_run_band with that sun alone.
+ * base->source_dir / source_flux / diffuse_flux / direct_seed are ignored. */
+typedef struct {
+    uint32_t nsuns;              /* K                                      */
+    const float *source_dir;     /* [K][S][3]                              */
+    const float *source_flux;    /* [K][S][B]                              */
+    const float *diffuse_flux;   /* [K][B]                                 */
+    const uint32_t *direct_seed; /* [K]                                    */
+} rt_sun_batch;
+/* R_out: [K][P][B]; if out_on_device != 0, R_out is a device pointer. */
+int rt_run_band_batch(rt_ctx *ctx, const rt_scene *scene, const rt_band_request *base, const rt_sun_batch *suns,
+                      float *R_out, int out_on_device);
+
+/* FP64 energy bookkeeping of the last rt_run_band / rt_run_band_batch on this context, per channel
+ * (channel = sun*B + band): [0][C] sum(area*R) absorbed, [1][C] sum(area*direct incident),
+ * [2][C] sum(area*sky flux gathered by the primary pass). */
+int rt_energy_residual_fp64(rt_ctx *ctx, double *out, uint32_t nchannels);
+
+/* ---------------------------------------------------------------------------------------------
+ * Launch tuning: rays-per-wave grouping, persistent workgroups per CU and waves per workgroup are
+ * picked by rt_autotune (timing candidate launch shapes on the given scene) or set by hand.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct {
+    uint32_t waves_per_block;   /* 1, 2 or 4 */
+    uint32_t blocks_per_cu;     /* persistent workgroups per CU */
+} rt_launch_cfg;
+#define RT_KERNEL_DIRECT 0
+#define RT_KERNEL_DIFFUSE 1
+#define RT_KERNEL_GATHER 2
+int rt_set_launch_cfg(rt_ctx *ctx, int kernel, const rt_launch_cfg *cfg);
+int rt_get_launch_cfg(rt_ctx *ctx, int kernel, rt_launch_cfg *cfg);
+
+/* ---------------------------------------------------------------------------------------------
+ * Timing of the last solve's kernels (HIP events on the context stream), milliseconds.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct {
+    double direct_ms;   /* direct trace kernels            */
+    double diffuse_ms;  /* diffuse/scatter trace kernels   */
+    double gather_ms;   /* gather kernels                  */
+    double other_ms;    /* deposit/copy/emission/final     */
+    uint64_t direct_rays;
+    uint64_t diffuse_rays;
+    uint32_t direct_launches;
+    uint32_t diffuse_launches;
+} rt_timing;
+int rt_enable_timing(rt_ctx *ctx, int on);
+int rt_get_timing(rt_ctx *ctx, rt_timing *out);
+
+/* Debug/parity access: one diffuse pass's hit words [P][2][n*n] (word = prim | bottom<<31,
+ * 0xFFFFFFFF miss, 0xFFFFFFFE face not launched) and one direct pass's strength sums [P][S][2]. */
+int rt_debug_diffuse_hits(rt_ctx *ctx, const rt_scene *scene, uint32_t n, uint32_t seed, uint32_t *out);
+int rt_debug_direct_sums(rt_ctx *ctx, const rt_scene *scene, uint32_t d, uint32_t seed, uint32_t nsrc,
+                         const float *src_dir, float *out);
+/* Generated diffuse rays of one pass, [P][2][n*n][6] = origin.xyz, direction.xyz (no tracing). */
+int rt_debug_diffuse_rays(rt_ctx *ctx, const rt_scene *scene, uint32_t n, uint32_t seed, float *out);
+/* Closest hit of arbitrary rays [n][6] with the radiation predicates, skipping skip[r]; brute != 0
+ * tests every primitive instead of traversing the BVH.  hit = primitive index or 0xFFFFFFFF. */
+int rt_debug_trace(rt_ctx *ctx, const rt_scene *scene, uint32_t n, const float *rays, const uint32_t *skip, float tmin,
+                   int brute, uint32_t *hit, float *t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,953 @@
+/*
+ * rt_oracle.c -- CPU restatement of the Helios radiation-transfer hot path (test infrastructure only;
+ * the product never links it).  See rt_oracle.h for what pins each part.
+ *
+ * Arithmetic contract: every per-ray quantity (sample point, direction, intersection predicate) is FP32
+ * with the operation order written out below and no contraction (built with -ffp-contract=off), so it
+ * reproduces the HIP kernels' rays bit for bit.  Energy accumulation is FP64 ("FP64 host oracle").
+ */
+#include "rt_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define MISS 0xFFFFFFFFu
+
+/* ------------------------------------------------------------------------------------------------ */
+/* RNG: R/shaders/common/random.glsl:10-45 (== R/include/RayTracing.cuh:237-262)                     */
+/* ------------------------------------------------------------------------------------------------ */
+
+/* Generic TEA encryption (Wheeler & Needham 1994); the Helios variant is 16 rounds with the key
+ * below (random.glsl:10-23).  The general form exists so the published 32-round vectors pin it. */
+uint32_t orc_tea(uint32_t v0, uint32_t v1, uint32_t rounds, const uint32_t key[4], uint32_t *v1_out) {
+    uint32_t s0 = 0;
+    for (uint32_t n = 0; n < rounds; n++) {
+        s0 += 0x9e3779b9u;
+        v0 += ((v1 << 4) + key[0]) ^ (v1 + s0) ^ ((v1 >> 5) + key[1]);
+        v1 += ((v0 << 4) + key[2]) ^ (v0 + s0) ^ ((v0 >> 5) + key[3]);
+    }
+    if (v1_out) *v1_out = v1;
+    return v0;
+}
+
+static const uint32_t HELIOS_TEA_KEY[4] = {0xa341316cu, 0xc8013ea4u, 0xad90777du, 0x7e95761eu};
+
+uint32_t orc_tea16(uint32_t val0, uint32_t val1) { return orc_tea(val0, val1, 16, HELIOS_TEA_KEY, NULL); }
+
+/* random.glsl:36-39: Numerical Recipes quick LCG */
+uint32_t orc_lcg_next(uint32_t state) { return 1664525u * state + 1013904223u; }
+
+/* random.glsl:42-44: 24-bit mantissa uniform in [0,1) (exact) */
+float orc_rnd(uint32_t *state) {
+    *state = orc_lcg_next(*state);
+    return (float)(*state & 0x00FFFFFFu) / 16777216.0f;
+}
+
+/* sin and cos of 2*pi*r.  Quadrant reduction u = 4r (exact), q = nearest integer, x = u - q (exact),
+ * a = x*pi/2 in [-pi/4, pi/4]; Taylor series to a^9 (sin) and a^10 (cos), whose truncation error at
+ * pi/4 is below 2e-9.  The reference evaluates sin(2*pi*Rp)/cos(...) with the device libm; this
+ * restatement replaces that with a function that is identical on every platform. */
+void orc_sincos_2pi(float r, float *s, float *c) {
+    float u = r * 4.0f;
+    float q = floorf(u + 0.5f);
+    float x = u - q;
+    float a = x * 1.57079632679489662f;
+    float a2 = a * a;
+    float ps = a + a * a2 * (-0.166666666666666667f + a2 * (0.00833333333333333333f +
+                                a2 * (-0.000198412698412698413f + a2 * 2.75573192239858907e-6f)));
+    float pc = 1.0f + a2 * (-0.5f + a2 * (0.0416666666666666667f + a2 * (-0.00138888888888888889f +
+                                a2 * (2.48015873015873016e-5f + a2 * -2.75573192239858907e-7f))));
+    int qi = ((int)q) & 3;
+    switch (qi) {
+        case 0: *s = ps; *c = pc; break;
+        case 1: *s = pc; *c = -ps; break;
+        case 2: *s = -ps; *c = -pc; break;
+        default: *s = -pc; *c = ps; break;
+    }
+}
+
+/* ------------------------------------------------------------------------------------------------ */
+/* small FP32 vector helpers (left-to-right evaluation, no contraction)                              */
+/* ------------------------------------------------------------------------------------------------ */
+typedef struct { float x, y, z; } v3;
+
+static inline v3 mk(float x, float y, float z) { v3 r = {x, y, z}; return r; }
+static inline v3 sub(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 add(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 scl(v3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+static inline float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline v3 cross(v3 a, v3 b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+/* core/include/helios_vector_types.h:891-900 (normalize multiplies by 1/sqrt(m2)) */
+static inline v3 nrm(v3 a) {
+    float m2 = a.x * a.x + a.y * a.y + a.z * a.z;
+    float inv = 1.0f / sqrtf(m2);
+    return scl(a, inv);
+}
+static inline v3 ld3(const float *p) { return mk(p[0], p[1], p[2]); }
+
+/* Row-major affine transform of a point (R/src/backends/VulkanComputeBackend.cpp:341-352) */
+static inline v3 xform(const float *t, float px, float py, float pz) {
+    return mk(t[0] * px + t[1] * py + t[2] * pz + t[3],
+              t[4] * px + t[5] * py + t[6] * pz + t[7],
+              t[8] * px + t[9] * py + t[10] * pz + t[11]);
+}
+
+/* ------------------------------------------------------------------------------------------------ */
+/* scene                                                                                             */
+/* ------------------------------------------------------------------------------------------------ */
+typedef struct {
+    float lo[3], hi[3];
+    uint32_t left, right; /* right == MISS => leaf */
+    uint32_t start, count;
+} onode;
+
+struct orc_scene {
+    uint32_t P;
+    float *T;          /* [P][16] */
+    uint32_t *type;    /* [P] */
+    uint32_t *twos;    /* [P] */
+    uint32_t *uuid;    /* [P] */
+    v3 *verts;         /* [P][4] world vertices */
+    v3 *normal;        /* [P] precomputed world normal */
+    float *frame;      /* [P][4] ce, se, ca, sa */
+    v3 *pn;            /* [P] patch plane normal (intersections.glsl:33) */
+    float *aa, *bb;    /* [P] |a|^2, |b|^2 */
+    float *blo, *bhi;  /* [P][3] inflated AABBs */
+    /* periodic */
+    uint32_t nbbox;
+    v3 bbox[4][4];
+    int px, py;
+    float dom[4];
+    /* SAH BVH */
+    onode *nodes;
+    uint32_t nnodes, cap;
+    uint32_t *pidx;
+};
+
+static int is_patch(uint32_t t) { return t == 0 || t == 3; }
+
+/* World vertices: canonical quad (VulkanComputeBackend.cpp:328) / triangle (:329) */
+static void prep_prim(orc_scene *s, uint32_t i) {
+    const float *t = s->T + (size_t)i * 16;
+    v3 *v = s->verts + (size_t)i * 4;
+    uint32_t ty = s->type[i];
+    if (is_patch(ty)) {
+        v[0] = xform(t, -0.5f, -0.5f, 0.f);
+        v[1] = xform(t, 0.5f, -0.5f, 0.f);
+        v[2] = xform(t, 0.5f, 0.5f, 0.f);
+        v[3] = xform(t, -0.5f, 0.5f, 0.f);
+    } else if (ty == 1) {
+        v[0] = xform(t, 0.f, 0.f, 0.f);
+        v[1] = xform(t, 0.f, 1.f, 0.f);
+        v[2] = xform(t, 1.f, 1.f, 0.f);
+        v[3] = v[2];
+    } else {
+        v[0] = v[1] = v[2] = v[3] = xform(t, 0.f, 0.f, 0.f);
+    }
+    /* Precomputed world normal (VulkanComputeBackend.cpp:377-435) */
+    v3 w0, w1, w2, n;
+    if (is_patch(ty) || ty == 2) {
+        w0 = xform(t, 0.f, 0.f, 0.f); w1 = xform(t, 1.f, 0.f, 0.f); w2 = xform(t, 0.f, 1.f, 0.f);
+    } else {
+        w0 = xform(t, 0.f, 0.f, 0.f); w1 = xform(t, 0.f, 1.f, 0.f); w2 = xform(t, 1.f, 1.f, 0.f);
+    }
+    if (ty == 0 || ty == 1 || ty == 2 || ty == 3) {
+        n = cross(sub(w1, w0), sub(w2, w0));
+        float len = sqrtf(n.x * n.x + n.y * n.y + n.z * n.z);
+        if (len > 1e-8f) { float inv = 1.0f / len; n = scl(n, inv); }
+        else n = mk(0.f, 0.f, 1.f);
+    } else {
+        n = mk(0.f, 0.f, 1.f);
+    }
+    s->normal[i] = n;
+    /* diffuse frame: cos/sin of elevation acos(n.z) and azimuth atan2(n.y, n.x)
+     * (diffuse_raygen.comp:303-327), evaluated with exact identities. */
+    float ce = n.z < -1.f ? -1.f : (n.z > 1.f ? 1.f : n.z);
+    float se = sqrtf((1.0f - ce) * (1.0f + ce));
+    float r = sqrtf(n.x * n.x + n.y * n.y);
+    float ca, sa;
+    if (r > 0.f) { ca = n.x / r; sa = n.y / r; }
+    else { ca = signbit(n.x) ? -1.f : 1.f; sa = 0.f; }
+    s->frame[i * 4 + 0] = ce; s->frame[i * 4 + 1] = se; s->frame[i * 4 + 2] = ca; s->frame[i * 4 + 3] = sa;
+    /* patch predicate data (intersections.glsl:31-53) */
+    if (is_patch(ty)) {
+        s->pn[i] = nrm(cross(sub(v[1], v[0]), sub(v[2], v[0])));
+        v3 a = sub(v[1], v[0]), b = sub(v[3], v[0]);
+        s->aa[i] = dot(a, a);
+        s->bb[i] = dot(b, b);
+    } else {
+        s->pn[i] = mk(0, 0, 1); s->aa[i] = 0; s->bb[i] = 0;
+    }
+    /* conservative AABB: exact vertex bounds widened by 2^-18 of the extent plus 2^-20 of the
+     * magnitude, so a slab test can never reject a ray that a predicate (incl. the 1e-5 barycentric
+     * slack of the CD triangle test) would accept. */
+    int nv = is_patch(ty) ? 4 : 3;
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (int k = 0; k < nv; k++) {
+        float c[3] = {v[k].x, v[k].y, v[k].z};
+        for (int a = 0; a < 3; a++) { if (c[a] < lo[a]) lo[a] = c[a]; if (c[a] > hi[a]) hi[a] = c[a]; }
+    }
+    float ext = 0.f;
+    for (int a = 0; a < 3; a++) if (hi[a] - lo[a] > ext) ext = hi[a] - lo[a];
+    for (int a = 0; a < 3; a++) {
+        float m = fmaxf(fabsf(lo[a]), fabsf(hi[a]));
+        float pad = ext * 3.0e-5f + m * 1.0e-6f + 1e-30f;
+        s->blo[i * 3 + a] = lo[a] - pad;
+        s->bhi[i * 3 + a] = hi[a] + pad;
+    }
+}
+
+/* ---- binned SAH BVH2 (CD/src/CollisionDetection.cpp:940-1216: 16 bins, <=8 prims/leaf, depth<=64,
+ *      median fallback).  The tree shape is not a parity target; it only has to be exact. ---- */
+static uint32_t new_node(orc_scene *s) {
+    if (s->nnodes == s->cap) {
+        s->cap = s->cap ? s->cap * 2 : 1024;
+        s->nodes = (onode *)realloc(s->nodes, sizeof(onode) * s->cap);
+    }
+    return s->nnodes++;
+}
+
+static void centroid(const orc_scene *s, uint32_t p, float c[3]) {
+    for (int a = 0; a < 3; a++) c[a] = (s->blo[p * 3 + a] + s->bhi[p * 3 + a]) * 0.5f;
+}
+
+static float sarea(const float lo[3], const float hi[3]) {
+    float d0 = hi[0] - lo[0], d1 = hi[1] - lo[1], d2 = hi[2] - lo[2];
+    if (d0 < 0 || d1 < 0 || d2 < 0) return 0.f;
+    return 2.0f * (d0 * d1 + d1 * d2 + d2 * d0);
+}
+
+static int cmp_axis;
+static const orc_scene *cmp_scene;
+static int cmp_centroid(const void *A, const void *B) {
+    uint32_t a = *(const uint32_t *)A, b = *(const uint32_t *)B;
+    float ca[3], cb[3];
+    centroid(cmp_scene, a, ca); centroid(cmp_scene, b, cb);
+    if (ca[cmp_axis] < cb[cmp_axis]) return -1;
+    if (ca[cmp_axis] > cb[cmp_axis]) return 1;
+    return (a < b) ? -1 : (a > b);
+}
+
+static void build_rec(orc_scene *s, uint32_t ni, uint32_t start, uint32_t count, int depth) {
+    onode nd;
+    for (int a = 0; a < 3; a++) { nd.lo[a] = FLT_MAX; nd.hi[a] = -FLT_MAX; }
+    for (uint32_t k = 0; k < count; k++) {
+        uint32_t p = s->pidx[start + k];
+        for (int a = 0; a < 3; a++) {
+            if (s->blo[p * 3 + a] < nd.lo[a]) nd.lo[a] = s->blo[p * 3 + a];
+            if (s->bhi[p * 3 + a] > nd.hi[a]) nd.hi[a] = s->bhi[p * 3 + a];
+        }
+    }
+    nd.start = start; nd.count = count; nd.left = MISS; nd.right = MISS;
+    if (count <= 8 || depth >= 64) { s->nodes[ni] = nd; return; }
+
+    float cmin[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, cmax[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (uint32_t k = 0; k < count; k++) {
+        float c[3]; centroid(s, s->pidx[start + k], c);
+        for (int a = 0; a < 3; a++) { if (c[a] < cmin[a]) cmin[a] = c[a]; if (c[a] > cmax[a]) cmax[a] = c[a]; }
+    }
+    enum { NB = 16 };
+    int best_axis = -1, best_b = -1;
+    float best_cost = FLT_MAX;
+    for (int a = 0; a < 3; a++) {
+        float ext = cmax[a] - cmin[a];
+        if (ext <= 0.f) continue;
+        float scale = (float)NB / ext;
+        uint32_t cnt[NB] = {0};
+        float blo[NB][3], bhi[NB][3];
+        for (int b = 0; b < NB; b++) for (int q = 0; q < 3; q++) { blo[b][q] = FLT_MAX; bhi[b][q] = -FLT_MAX; }
+        for (uint32_t k = 0; k < count; k++) {
+            uint32_t p = s->pidx[start + k];
+            float c[3]; centroid(s, p, c);
+            int b = (int)((c[a] - cmin[a]) * scale);
+            if (b >= NB) b = NB - 1;
+            if (b < 0) b = 0;
+            cnt[b]++;
+            for (int q = 0; q < 3; q++) {
+                if (s->blo[p * 3 + q] < blo[b][q]) blo[b][q] = s->blo[p * 3 + q];
+                if (s->bhi[p * 3 + q] > bhi[b][q]) bhi[b][q] = s->bhi[p * 3 + q];
+            }
+        }
+        for (int split = 1; split < NB; split++) {
+            float llo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, lhi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+            float rlo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, rhi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+            uint32_t nl = 0, nr = 0;
+            for (int b = 0; b < NB; b++) {
+                if (!cnt[b]) continue;
+                float *lo = b < split ? llo : rlo, *hi = b < split ? lhi : rhi;
+                for (int q = 0; q < 3; q++) { if (blo[b][q] < lo[q]) lo[q] = blo[b][q]; if (bhi[b][q] > hi[q]) hi[q] = bhi[b][q]; }
+                if (b < split) nl += cnt[b]; else nr += cnt[b];
+            }
+            if (!nl || !nr) continue;
+            float cost = sarea(llo, lhi) * (float)nl + sarea(rlo, rhi) * (float)nr;
+            if (cost < best_cost) { best_cost = cost; best_axis = a; best_b = split; }
+        }
+    }
+    uint32_t mid;
+    if (best_axis < 0) {
+        /* median fallback on the longest node axis */
+        int ax = 0;
+        float e0 = nd.hi[0] - nd.lo[0], e1 = nd.hi[1] - nd.lo[1], e2 = nd.hi[2] - nd.lo[2];
+        if (e1 > e0 && e1 >= e2) ax = 1; else if (e2 > e0 && e2 > e1) ax = 2;
+        cmp_axis = ax; cmp_scene = s;
+        qsort(s->pidx + start, count, sizeof(uint32_t), cmp_centroid);
+        mid = count / 2;
+    } else {
+        float scale = (float)NB / (cmax[best_axis] - cmin[best_axis]);
+        uint32_t i = start, j = start + count;
+        while (i < j) {
+            float c[3]; centroid(s, s->pidx[i], c);
+            int b = (int)((c[best_axis] - cmin[best_axis]) * scale);
+            if (b >= NB) b = NB - 1;
+            if (b < 0) b = 0;
+            if (b < best_b) i++;
+            else { uint32_t t = s->pidx[i]; s->pidx[i] = s->pidx[--j]; s->pidx[j] = t; }
+        }
+        mid = i - start;
+        if (mid == 0 || mid == count) mid = count / 2;
+    }
+    uint32_t l = new_node(s), r = new_node(s);
+    nd.left = l; nd.right = r;
+    s->nodes[ni] = nd;
+    build_rec(s, l, start, mid, depth + 1);
+    build_rec(s, r, start + mid, count - mid, depth + 1);
+}
+
+/* Periodic boundary faces (R/src/RadiationModel.cpp:6023-6105) and domain bounds
+ * (R/src/backends/VulkanComputeBackend.cpp:544-582). */
+static void build_periodic(orc_scene *s) {
+    s->nbbox = 0;
+    s->dom[0] = s->dom[1] = s->dom[2] = s->dom[3] = 0.f;
+    if (!s->px && !s->py) return;
+    float xb[2] = {FLT_MAX, -FLT_MAX}, yb[2] = {FLT_MAX, -FLT_MAX}, zb[2] = {FLT_MAX, -FLT_MAX};
+    for (uint32_t i = 0; i < s->P; i++) {
+        int nv = is_patch(s->type[i]) ? 4 : 3;
+        for (int k = 0; k < nv; k++) {
+            v3 v = s->verts[(size_t)i * 4 + k];
+            if (v.x < xb[0]) xb[0] = v.x; if (v.x > xb[1]) xb[1] = v.x;
+            if (v.y < yb[0]) yb[0] = v.y; if (v.y > yb[1]) yb[1] = v.y;
+            if (v.z < zb[0]) zb[0] = v.z; if (v.z > zb[1]) zb[1] = v.z;
+        }
+    }
+    xb[0] = (float)((double)xb[0] - 1e-5); xb[1] = (float)((double)xb[1] + 1e-5);
+    yb[0] = (float)((double)yb[0] - 1e-5); yb[1] = (float)((double)yb[1] + 1e-5);
+    zb[0] = (float)((double)zb[0] - 1e-5); zb[1] = (float)((double)zb[1] + 1e-5);
+    uint32_t k = 0;
+    if (s->px) {
+        s->bbox[k][0] = mk(xb[0], yb[0], zb[0]); s->bbox[k][1] = mk(xb[0], yb[1], zb[0]);
+        s->bbox[k][2] = mk(xb[0], yb[1], zb[1]); s->bbox[k][3] = mk(xb[0], yb[0], zb[1]); k++;
+        s->bbox[k][0] = mk(xb[1], yb[0], zb[0]); s->bbox[k][1] = mk(xb[1], yb[1], zb[0]);
+        s->bbox[k][2] = mk(xb[1], yb[1], zb[1]); s->bbox[k][3] = mk(xb[1], yb[0], zb[1]); k++;
+    }
+    if (s->py) {
+        s->bbox[k][0] = mk(xb[0], yb[0], zb[0]); s->bbox[k][1] = mk(xb[1], yb[0], zb[0]);
+        s->bbox[k][2] = mk(xb[1], yb[0], zb[1]); s->bbox[k][3] = mk(xb[0], yb[0], zb[1]); k++;
+        s->bbox[k][0] = mk(xb[0], yb[1], zb[0]); s->bbox[k][1] = mk(xb[1], yb[1], zb[0]);
+        s->bbox[k][2] = mk(xb[1], yb[1], zb[1]); s->bbox[k][3] = mk(xb[0], yb[1], zb[1]); k++;
+    }
+    s->nbbox = k;
+    float x0 = 1e30f, x1 = -1e30f, y0 = 1e30f, y1 = -1e30f;
+    for (uint32_t f = 0; f < k; f++)
+        for (int q = 0; q < 4; q++) {
+            v3 v = s->bbox[f][q];
+            if (v.x < x0) x0 = v.x; if (v.x > x1) x1 = v.x;
+            if (v.y < y0) y0 = v.y; if (v.y > y1) y1 = v.y;
+        }
+    s->dom[0] = x0; s->dom[1] = x1; s->dom[2] = y0; s->dom[3] = y1;
+}
+
+orc_scene *orc_scene_create(uint32_t P, const float *transforms, const uint32_t *types, const uint32_t *twosided,
+                            const uint32_t *uuids, int periodic_x, int periodic_y) {
+    orc_scene *s = (orc_scene *)calloc(1, sizeof(orc_scene));
+    s->P = P;
+    s->T = (float *)malloc(sizeof(float) * 16 * (P ? P : 1));
+    memcpy(s->T, transforms, sizeof(float) * 16 * P);
+    s->type = (uint32_t *)malloc(sizeof(uint32_t) * (P ? P : 1));
+    memcpy(s->type, types, sizeof(uint32_t) * P);
+    s->twos = (uint32_t *)malloc(sizeof(uint32_t) * (P ? P : 1));
+    memcpy(s->twos, twosided, sizeof(uint32_t) * P);
+    s->uuid = (uint32_t *)malloc(sizeof(uint32_t) * (P ? P : 1));
+    for (uint32_t i = 0; i < P; i++) s->uuid[i] = uuids ? uuids[i] : i;
+    s->verts = (v3 *)malloc(sizeof(v3) * 4 * (P ? P : 1));
+    s->normal = (v3 *)malloc(sizeof(v3) * (P ? P : 1));
+    s->frame = (float *)malloc(sizeof(float) * 4 * (P ? P : 1));
+    s->pn = (v3 *)malloc(sizeof(v3) * (P ? P : 1));
+    s->aa = (float *)malloc(sizeof(float) * (P ? P : 1));
+    s->bb = (float *)malloc(sizeof(float) * (P ? P : 1));
+    s->blo = (float *)malloc(sizeof(float) * 3 * (P ? P : 1));
+    s->bhi = (float *)malloc(sizeof(float) * 3 * (P ? P : 1));
+    s->px = periodic_x; s->py = periodic_y;
+    for (uint32_t i = 0; i < P; i++) prep_prim(s, i);
+    build_periodic(s);
+    s->pidx = (uint32_t *)malloc(sizeof(uint32_t) * (P ? P : 1));
+    uint32_t np = 0;
+    for (uint32_t i = 0; i < P; i++)
+        if (s->type[i] <= 1 || s->type[i] == 3) s->pidx[np++] = i;
+    if (np) {
+        uint32_t root = new_node(s);
+        build_rec(s, root, 0, np, 0);
+    }
+    return s;
+}
+
+void orc_scene_destroy(orc_scene *s) {
+    if (!s) return;
+    free(s->T); free(s->type); free(s->twos); free(s->uuid); free(s->verts); free(s->normal); free(s->frame);
+    free(s->pn); free(s->aa); free(s->bb); free(s->blo); free(s->bhi); free(s->nodes); free(s->pidx);
+    free(s);
+}
+
+void orc_scene_geometry(const orc_scene *s, float *verts, float *normals, float *frame, float *domain, uint32_t *nbbox) {
+    for (uint32_t i = 0; i < s->P; i++) {
+        if (verts) for (int k = 0; k < 4; k++) {
+            verts[i * 12 + k * 3 + 0] = s->verts[i * 4 + k].x;
+            verts[i * 12 + k * 3 + 1] = s->verts[i * 4 + k].y;
+            verts[i * 12 + k * 3 + 2] = s->verts[i * 4 + k].z;
+        }
+        if (normals) { normals[i * 3] = s->normal[i].x; normals[i * 3 + 1] = s->normal[i].y; normals[i * 3 + 2] = s->normal[i].z; }
+        if (frame) for (int k = 0; k < 4; k++) frame[i * 4 + k] = s->frame[i * 4 + k];
+    }
+    if (domain) for (int k = 0; k < 4; k++) domain[k] = s->dom[k];
+    if (nbbox) *nbbox = s->nbbox;
+}
+
+void orc_areas(const orc_scene *s, double *out) {
+    for (uint32_t i = 0; i < s->P; i++) {
+        const v3 *v = s->verts + (size_t)i * 4;
+        double e1[3] = {(double)v[1].x - v[0].x, (double)v[1].y - v[0].y, (double)v[1].z - v[0].z};
+        double e2[3], c[3];
+        if (is_patch(s->type[i])) { e2[0] = (double)v[3].x - v[0].x; e2[1] = (double)v[3].y - v[0].y; e2[2] = (double)v[3].z - v[0].z; }
+        else { e2[0] = (double)v[2].x - v[0].x; e2[1] = (double)v[2].y - v[0].y; e2[2] = (double)v[2].z - v[0].z; }
+        c[0] = e1[1] * e2[2] - e1[2] * e2[1]; c[1] = e1[2] * e2[0] - e1[0] * e2[2]; c[2] = e1[0] * e2[1] - e1[1] * e2[0];
+        double a = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+        out[i] = is_patch(s->type[i]) ? a : (s->type[i] == 1 ? 0.5 * a : 0.0);
+    }
+}
+
+/* ------------------------------------------------------------------------------------------------ */
+/* intersection predicates                                                                           */
+/* ------------------------------------------------------------------------------------------------ */
+
+/* R/shaders/common/intersections.glsl:26-70 */
+static int isect_patch_rad(const orc_scene *s, uint32_t i, v3 o, v3 d, float *t_out) {
+    v3 n = s->pn[i];
+    float denom = dot(d, n);
+    if (fabsf(denom) < 1e-8f) return 0;
+    v3 anchor = s->verts[(size_t)i * 4];
+    float t = dot(sub(anchor, o), n) / denom;
+    if (t < 1e-8f || t > 1e8f) return 0;
+    v3 p = add(o, scl(d, t));
+    v3 dd = sub(p, anchor);
+    v3 a = sub(s->verts[i * 4 + 1], anchor), b = sub(s->verts[i * 4 + 3], anchor);
+    float da = dot(dd, a), db = dot(dd, b);
+    if (da > 0.0f && da < s->aa[i] && db > 0.0f && db < s->bb[i]) { *t_out = t; return 1; }
+    return 0;
+}
+
+/* R/shaders/common/intersections.glsl:73-112 (Moller-Trumbore) */
+static int isect_tri_rad(const orc_scene *s, uint32_t i, v3 o, v3 d, float *t_out) {
+    v3 v0 = s->verts[(size_t)i * 4];
+    v3 e1 = sub(s->verts[i * 4 + 1], v0), e2 = sub(s->verts[i * 4 + 2], v0);
+    v3 h = cross(d, e2);
+    float a = dot(e1, h);
+    if (fabsf(a) < 1e-8f) return 0;
+    float f = 1.0f / a;
+    v3 sv = sub(o, v0);
+    float u = f * dot(sv, h);
+    if (u < 0.0f || u > 1.0f) return 0;
+    v3 q = cross(sv, e1);
+    float v = f * dot(d, q);
+    if (v < 0.0f || u + v > 1.0f) return 0;
+    float t = f * dot(e2, q);
+    if (t > 1e-8f && t < 1e8f) { *t_out = t; return 1; }
+    return 0;
+}
+
+/* CD/src/CollisionDetection_RayTracing.cpp:1793-1847 */
+static int isect_tri_cd(const orc_scene *s, uint32_t i, v3 o, v3 d, float *t_out) {
+    const float EPS = 1e-5f;
+    v3 v0 = s->verts[(size_t)i * 4];
+    v3 e1 = sub(s->verts[i * 4 + 1], v0), e2 = sub(s->verts[i * 4 + 2], v0);
+    v3 h = cross(d, e2);
+    float a = e1.x * h.x + e1.y * h.y + e1.z * h.z;
+    if (a > -EPS && a < EPS) return 0;
+    float f = 1.0f / a;
+    v3 sv = sub(o, v0);
+    float u = f * (sv.x * h.x + sv.y * h.y + sv.z * h.z);
+    if (u < -EPS || u > 1.0f + EPS) return 0;
+    v3 q = cross(sv, e1);
+    float v = f * (d.x * q.x + d.y * q.y + d.z * q.z);
+    if (v < -EPS || u + v > 1.0f + EPS) return 0;
+    float t = f * (e2.x * q.x + e2.y * q.y + e2.z * q.z);
+    if (t > EPS) { *t_out = t; return 1; }
+    return 0;
+}
+
+/* CD/src/CollisionDetection_RayTracing.cpp:1849-1884 */
+static int isect_patch_cd(const orc_scene *s, uint32_t i, v3 o, v3 d, float *t_out) {
+    const float EPS = 1e-8f;
+    v3 v0 = s->verts[(size_t)i * 4];
+    v3 n = nrm(cross(sub(s->verts[i * 4 + 1], v0), sub(s->verts[i * 4 + 2], v0)));
+    v3 a = sub(s->verts[i * 4 + 1], v0), b = sub(s->verts[i * 4 + 3], v0);
+    float denom = n.x * d.x + n.y * d.y + n.z * d.z;
+    if (fabsf(denom) > EPS) {
+        v3 ao = sub(v0, o);
+        float t = (n.x * ao.x + n.y * ao.y + n.z * ao.z) / denom;
+        if (t > EPS && t < 1e8f) {
+            v3 p = add(o, scl(d, t));
+            v3 dd = sub(p, v0);
+            float da = a.x * dd.x + a.y * dd.y + a.z * dd.z;
+            float db = b.x * dd.x + b.y * dd.y + b.z * dd.z;
+            float a2 = a.x * a.x + a.y * a.y + a.z * a.z;
+            float b2 = b.x * b.x + b.y * b.y + b.z * b.z;
+            if (da >= -EPS && da <= a2 + EPS && db >= -EPS && db <= b2 + EPS) { *t_out = t; return 1; }
+        }
+    }
+    return 0;
+}
+
+/* conservative slab test in FP64 (boxes are inflated; culling decisions never change results) */
+static int slab(const float *lo, const float *hi, const double o[3], const double inv[3], double tmax) {
+    double tn = -1e300, tf = 1e300;
+    for (int a = 0; a < 3; a++) {
+        double t1 = (lo[a] - o[a]) * inv[a], t2 = (hi[a] - o[a]) * inv[a];
+        if (isnan(t1) || isnan(t2)) { if (o[a] < lo[a] || o[a] > hi[a]) return 0; continue; }
+        if (t1 > t2) { double t = t1; t1 = t2; t2 = t; }
+        if (t1 > tn) tn = t1;
+        if (t2 < tf) tf = t2;
+    }
+    return tn <= tf && tf >= 0.0 && tn <= tmax;
+}
+
+typedef int (*isect_fn)(const orc_scene *, uint32_t, v3, v3, float *);
+
+/* Closest accepted hit; ties in t resolve to the lower primitive index (deterministic canonical rule
+ * replacing the reference's traversal-order rule, R/shaders/common/bvh_traversal.glsl:328-334). */
+static uint32_t closest(const orc_scene *s, v3 o, v3 d, float tmin, uint32_t skip, int brute, int cd_mode,
+                        float tstart, float *t_out) {
+    uint32_t best = MISS;
+    float bt = tstart;
+    double od[3] = {o.x, o.y, o.z}, inv[3] = {1.0 / (double)d.x, 1.0 / (double)d.y, 1.0 / (double)d.z};
+    uint32_t stack[256];
+    int sp = 0;
+    uint32_t np = s->nnodes ? s->nodes[0].count : 0;
+    (void)np;
+    if (!s->nnodes) { *t_out = bt; return MISS; }
+    if (brute) {
+        for (uint32_t i = 0; i < s->P; i++) {
+            uint32_t ty = s->type[i];
+            if (i == skip || !(ty <= 1 || ty == 3)) continue;
+            float t;
+            int h;
+            if (cd_mode) h = is_patch(ty) ? isect_patch_cd(s, i, o, d, &t) : isect_tri_cd(s, i, o, d, &t);
+            else h = is_patch(ty) ? isect_patch_rad(s, i, o, d, &t) : isect_tri_rad(s, i, o, d, &t);
+            if (!h || !(t > tmin)) continue;
+            if (t < bt || (t == bt && best != MISS && i < best)) { bt = t; best = i; }
+        }
+        *t_out = bt;
+        return best;
+    }
+    stack[sp++] = 0;
+    while (sp) {
+        const onode *nd = &s->nodes[stack[--sp]];
+        if (!slab(nd->lo, nd->hi, od, inv, (double)bt)) continue;
+        if (nd->right == MISS) {
+            for (uint32_t k = 0; k < nd->count; k++) {
+                uint32_t i = s->pidx[nd->start + k];
+                if (i == skip) continue;
+                uint32_t ty = s->type[i];
+                float t;
+                int h;
+                if (cd_mode) h = is_patch(ty) ? isect_patch_cd(s, i, o, d, &t) : isect_tri_cd(s, i, o, d, &t);
+                else h = is_patch(ty) ? isect_patch_rad(s, i, o, d, &t) : isect_tri_rad(s, i, o, d, &t);
+                if (!h || !(t > tmin)) continue;
+                if (t < bt || (t == bt && best != MISS && i < best)) { bt = t; best = i; }
+            }
+        } else {
+            if (sp + 2 > 256) abort();
+            stack[sp++] = nd->right;
+            stack[sp++] = nd->left;
+        }
+    }
+    *t_out = bt;
+    return best;
+}
+
+uint32_t orc_trace(const orc_scene *s, const float o[3], const float d[3], float tmin, uint32_t skip, int brute, float *t_out) {
+    float t;
+    uint32_t h = closest(s, ld3(o), ld3(d), tmin, skip, brute, 0, 1e30f, &t);
+    if (t_out) *t_out = t;
+    return h;
+}
+
+/* R/shaders/common/periodic_boundary.glsl:28-59 */
+static float isect_bbox_face(const orc_scene *s, uint32_t f, v3 o, v3 d) {
+    v3 v0 = s->bbox[f][0], v1 = s->bbox[f][1], v2 = s->bbox[f][2], v3_ = s->bbox[f][3];
+    v3 n = nrm(cross(sub(v1, v0), sub(v2, v0)));
+    float denom = dot(d, n);
+    if (fabsf(denom) < 1e-8f) return -1.0f;
+    float t = dot(sub(v0, o), n) / denom;
+    if (t <= 1e-5f || t > 1e8f) return -1.0f;
+    v3 p = add(o, scl(d, t));
+    v3 dd = sub(p, v0);
+    v3 a = sub(v1, v0), b = sub(v3_, v0);
+    float da = dot(dd, a), db = dot(dd, b);
+    if (da > 0.0f && da < dot(a, a) && db > 0.0f && db < dot(b, b)) return t;
+    return -1.0f;
+}
+
+/* R/shaders/common/periodic_boundary.glsl:73-112; returns 1 and the wrapped origin if a face is hit */
+static int periodic_wrap(const orc_scene *s, v3 o, v3 d, float bvh_t, v3 *wrapped) {
+    float bt = bvh_t;
+    int cf = -1;
+    for (uint32_t f = 0; f < s->nbbox; f++) {
+        float t = isect_bbox_face(s, f, o, d);
+        if (t > 0.0f && t < bt) { bt = t; cf = (int)f; }
+    }
+    if (cf < 0) return 0;
+    v3 hp = add(o, scl(d, bt));
+    const float eps = 1e-5f;
+    float wx = s->dom[1] - s->dom[0], wy = s->dom[3] - s->dom[2];
+    *wrapped = hp;
+    if (s->px && fabsf(hp.x - s->dom[0]) <= eps) wrapped->x += wx - eps;
+    else if (s->px && fabsf(hp.x - s->dom[1]) <= eps) wrapped->x -= wx - eps;
+    else if (s->py && fabsf(hp.y - s->dom[2]) <= eps) wrapped->y += wy - eps;
+    else if (s->py && fabsf(hp.y - s->dom[3]) <= eps) wrapped->y -= wy - eps;
+    return 1;
+}
+
+/* direct_raygen.comp:603-668 / diffuse_raygen.comp:498-560 without translucent covers: up to 32 traces,
+ * each followed by a periodic-face check; the last traversal result is returned. */
+static uint32_t trace_loop(const orc_scene *s, v3 o, v3 d, float tmin, uint32_t skip, int *exhausted) {
+    uint32_t hit = MISS;
+    v3 cur = o;
+    *exhausted = 1;
+    for (int step = 0; step < 32; ++step) {
+        float t;
+        hit = closest(s, cur, d, tmin, skip, 0, 0, 1e30f, &t);
+        if (s->nbbox) {
+            v3 w;
+            if (periodic_wrap(s, cur, d, t, &w)) { cur = w; continue; }
+        }
+        *exhausted = 0;
+        break;
+    }
+    return hit;
+}
+
+uint32_t orc_trace_periodic(const orc_scene *s, const float o[3], const float d[3], float tmin, uint32_t skip) {
+    int ex;
+    return trace_loop(s, ld3(o), ld3(d), tmin, skip, &ex);
+}
+
+/* ------------------------------------------------------------------------------------------------ */
+/* ray generation                                                                                    */
+/* ------------------------------------------------------------------------------------------------ */
+
+/* direct_raygen.comp:360-480 for patch/tile (stratified, subdivisions (1,1)) and triangle samples */
+static v3 direct_origin(const orc_scene *s, uint32_t i, uint32_t lx, uint32_t ly, uint32_t d, uint32_t seed) {
+    uint32_t Nrays = d * d;
+    uint32_t ray_index = d * ly + lx;
+    uint32_t st = orc_tea16(ray_index + Nrays * i, seed);
+    float spx, spy;
+    uint32_t ty = s->type[i];
+    float Rx = orc_rnd(&st), Ry = orc_rnd(&st);
+    if (is_patch(ty)) {
+        spx = (-0.5f + (float)lx / (float)d) + Rx / (float)d;
+        spy = (-0.5f + (float)ly / (float)d) + Ry / (float)d;
+    } else {
+        if (Rx < Ry) { spx = Rx; spy = Ry; } else { spx = Ry; spy = Rx; }
+    }
+    return xform(s->T + (size_t)i * 16, spx, spy, 0.0f);
+}
+
+void orc_direct_ray(const orc_scene *s, uint32_t prim, uint32_t lx, uint32_t ly, uint32_t d, uint32_t seed, float origin[3]) {
+    v3 o = direct_origin(s, prim, lx, ly, d, seed);
+    origin[0] = o.x; origin[1] = o.y; origin[2] = o.z;
+}
+
+/* diffuse_raygen.comp:375-482 */
+static void diffuse_ray(const orc_scene *s, uint32_t i, uint32_t lx, uint32_t ly, uint32_t n, uint32_t face,
+                        uint32_t seed, v3 *o, v3 *dir) {
+    uint32_t dimx = n * n;
+    uint32_t ray_index = n * ly + lx;
+    uint32_t st = orc_tea16(ray_index + dimx * i, seed);
+    uint32_t ty = s->type[i];
+    float spx, spy;
+    float Rx = orc_rnd(&st), Ry = orc_rnd(&st);
+    if (is_patch(ty)) { spx = -0.5f + Rx; spy = -0.5f + Ry; }
+    else { if (Rx < Ry) { spx = Rx; spy = Ry; } else { spx = Ry; spy = Rx; } }
+    *o = xform(s->T + (size_t)i * 16, spx, spy, 0.0f);
+    float Rt = ((float)lx + orc_rnd(&st)) / (float)n;
+    float Rp = ((float)ly + orc_rnd(&st)) / (float)n;
+    float st_ = sqrtf(Rt);           /* sin(asin(sqrt(Rt))) */
+    float ct = sqrtf(1.0f - Rt);     /* cos(asin(sqrt(Rt))) */
+    float sp, cp;
+    orc_sincos_2pi(Rp, &sp, &cp);
+    v3 dl = mk(st_ * cp, st_ * sp, ct);
+    const float *fr = s->frame + (size_t)i * 4;
+    float ce = fr[0], se = fr[1], ca = fr[2], sa = fr[3];
+    v3 dw;
+    dw.x = dl.x * (ce * ca) - dl.y * sa + dl.z * (se * ca);
+    dw.y = dl.x * (ce * sa) + dl.y * ca + dl.z * (se * sa);
+    dw.z = -dl.x * se + dl.z * ce;
+    v3 r = nrm(dw);
+    if (face == 0 && s->twos[i] == 1) r = mk(-r.x, -r.y, -r.z);
+    *dir = r;
+}
+
+void orc_diffuse_ray(const orc_scene *s, uint32_t prim, uint32_t lx, uint32_t ly, uint32_t n, uint32_t face,
+                     uint32_t seed, float origin[3], float dir[3]) {
+    v3 o, d;
+    diffuse_ray(s, prim, lx, ly, n, face, seed, &o, &d);
+    origin[0] = o.x; origin[1] = o.y; origin[2] = o.z;
+    dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
+}
+
+static int traced_type(uint32_t ty) { return ty == 0 || ty == 1 || ty == 3; }
+
+void orc_diffuse_hits(const orc_scene *s, uint32_t n, uint32_t seed, uint32_t *out) {
+    uint32_t nn = n * n;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long long ii = 0; ii < (long long)s->P; ii++) {
+        uint32_t i = (uint32_t)ii;
+        for (uint32_t face = 0; face < 2; face++) {
+            uint32_t *w = out + ((size_t)i * 2 + (1 - face)) * nn; /* slot 0 = top (face 1), slot 1 = bottom */
+            int launched = traced_type(s->type[i]) && !(face == 0 && s->twos[i] == 0);
+            for (uint32_t r = 0; r < nn; r++) {
+                if (!launched) { w[r] = 0xFFFFFFFEu; continue; }
+                uint32_t lx = r % n, ly = r / n;
+                v3 o, d;
+                diffuse_ray(s, i, lx, ly, n, face, seed, &o, &d);
+                int ex;
+                uint32_t h = trace_loop(s, o, d, 1e-5f, i, &ex);
+                if (h == MISS) { w[r] = MISS; continue; }
+                int top = dot(s->normal[h], d) < 0.0f;
+                w[r] = h | (top ? 0u : 0x80000000u);
+            }
+        }
+    }
+}
+
+/* direct_raygen.comp:413-670 for collimated sources: per (prim, source) sums of ray strength over
+ * unoccluded rays, split by the lit face. */
+void orc_direct_sums(const orc_scene *s, uint32_t d, uint32_t seed, uint32_t nsrc, const float *src_dir, double *out) {
+    uint32_t Nrays = d * d;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long long ii = 0; ii < (long long)s->P; ii++) {
+        uint32_t i = (uint32_t)ii;
+        for (uint32_t k = 0; k < nsrc * 2; k++) out[(size_t)i * nsrc * 2 + k] = 0.0;
+        if (!traced_type(s->type[i])) continue;
+        v3 n = s->normal[i];
+        for (uint32_t r = 0; r < Nrays; r++) {
+            uint32_t lx = r % d, ly = r / d;
+            v3 o = direct_origin(s, i, lx, ly, d, seed);
+            for (uint32_t src = 0; src < nsrc; src++) {
+                v3 dir = nrm(ld3(src_dir + src * 3));
+                float strength = fabsf(dot(n, dir)) / (float)Nrays;
+                int top = dot(dir, n) > 0.0f;
+                uint32_t tw = s->twos[i];
+                if (tw == 3 || (!top && tw == 0)) continue;
+                int ex;
+                uint32_t h = trace_loop(s, o, dir, 1e-4f, i, &ex);
+                if (h != MISS && !ex) continue; /* opaque occluder */
+                out[((size_t)i * nsrc + src) * 2 + (top ? 0 : 1)] += (double)strength;
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------------------------------ */
+/* castRaysSoA (CD/src/CollisionDetection_RayTracing.cpp:551-680, 980-1080, 1555-1620)               */
+/* ------------------------------------------------------------------------------------------------ */
+void orc_cast_rays(const orc_scene *s, uint64_t n, const float *origins, const float *dirs, float max_dist,
+                   float *out_t, float *out_normal, uint32_t *out_uuid, int brute) {
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long long r = 0; r < (long long)n; r++) {
+        v3 o = ld3(origins + r * 3), d = ld3(dirs + r * 3);
+        float start = max_dist > 0 ? max_dist : FLT_MAX;
+        float t;
+        /* the CD acceptance is distance > 1e-6 and distance < closest */
+        uint32_t h = closest(s, o, d, 1e-6f, MISS, brute, 1, start, &t);
+        if (h == MISS) { out_uuid[r] = MISS; continue; }
+        out_uuid[r] = s->uuid[h];
+        out_t[r] = t;
+        const v3 *v = s->verts + (size_t)h * 4;
+        v3 nn = cross(sub(v[1], v[0]), sub(v[2], v[0]));
+        float mag = sqrtf(nn.x * nn.x + nn.y * nn.y + nn.z * nn.z);
+        if (mag > 1e-8f) {
+            float inv = 1.0f / mag;
+            nn = scl(nn, inv);
+            v3 ip = add(o, scl(d, t));
+            v3 to = sub(o, ip);
+            if (nn.x * to.x + nn.y * to.y + nn.z * to.z < 0) nn = scl(nn, -1.0f);
+        } else {
+            nn = mk(-d.x, -d.y, -d.z);
+            float m2 = sqrtf(nn.x * nn.x + nn.y * nn.y + nn.z * nn.z);
+            float inv = 1.0f / m2;
+            nn = scl(nn, inv);
+        }
+        out_normal[r * 3] = nn.x; out_normal[r * 3 + 1] = nn.y; out_normal[r * 3 + 2] = nn.z;
+    }
+}
+
+/* ------------------------------------------------------------------------------------------------ */
+/* runBand pass schedule (R/src/RadiationModel.cpp:3470-4672)                                        */
+/* ------------------------------------------------------------------------------------------------ */
+
+/* One diffuse pass, faces 1 then 0 (RadiationModel.cpp:4204-4209, diffuse_raygen.comp:330-656). */
+static void diffuse_pass(const orc_scene *s, const orc_band_params *p, uint32_t n, uint32_t seed,
+                         const double *out_top, const double *out_bot, const float *dflux, const uint8_t *active,
+                         double *in, double *st, double *sb) {
+    uint32_t B = p->nbands, nn = n * n;
+    float strength = 1.0f / (float)nn;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long long ii = 0; ii < (long long)s->P; ii++) {
+        uint32_t i = (uint32_t)ii;
+        if (!traced_type(s->type[i])) continue;
+        for (int fi = 0; fi < 2; fi++) {
+            uint32_t face = fi == 0 ? 1u : 0u;
+            if (face == 0 && s->twos[i] == 0) continue;
+            for (uint32_t r = 0; r < nn; r++) {
+                v3 o, d;
+                diffuse_ray(s, i, r % n, r / n, n, face, seed, &o, &d);
+                int ex;
+                uint32_t h = trace_loop(s, o, d, 1e-5f, i, &ex);
+                int top = 0;
+                if (h != MISS) top = dot(s->normal[h], d) < 0.0f;
+                for (uint32_t b = 0; b < B; b++) {
+                    if (!active[b]) continue;
+                    double E = 0.0;
+                    if (h != MISS) {
+                        double inc = top ? out_top[(size_t)h * B + b] : out_bot[(size_t)h * B + b];
+                        E = inc * (double)strength;
+                    } else if (dflux[b] > 0.f) {
+                        E = 1.0 * (double)dflux[b] * (double)strength; /* isotropic sky: fd = 1 */
+                    }
+                    if (E == 0.0) continue;
+                    double rho = p->rho[(size_t)i * B + b], tau = p->tau[(size_t)i * B + b];
+                    size_t k = (size_t)i * B + b;
+                    in[k] += E * (1.0 - rho - tau);
+                    if (face == 1) { st[k] += E * rho; sb[k] += E * tau; }
+                    else { sb[k] += E * rho; st[k] += E * tau; }
+                }
+            }
+        }
+    }
+}
+
+int orc_run_band(const orc_scene *s, const orc_band_params *p, double *R, double *diag) {
+    const uint32_t P = s->P, B = p->nbands, S = p->nsources;
+    const double sigma = 5.6703744E-8; /* R/include/RadiationModel.h:2210 */
+    size_t PB = (size_t)P * B;
+    double *in = (double *)calloc(PB, sizeof(double));
+    double *st = (double *)calloc(PB, sizeof(double));
+    double *sb = (double *)calloc(PB, sizeof(double));
+    double *ot = (double *)calloc(PB, sizeof(double));
+    double *ob = (double *)calloc(PB, sizeof(double));
+    double *ft = (double *)calloc(PB, sizeof(double));
+    double *fb = (double *)calloc(PB, sizeof(double));
+    float *dflux = (float *)calloc(B ? B : 1, sizeof(float));
+    uint8_t *active = (uint8_t *)calloc(B ? B : 1, 1);
+    double *areas = (double *)calloc(P ? P : 1, sizeof(double));
+    orc_areas(s, areas);
+    if (diag) for (uint32_t k = 0; k < 3 * B; k++) diag[k] = 0.0;
+
+    uint32_t maxdepth = 0;
+    int scattering = 0, emission = 0, diffuse = 0, rundirect = 0;
+    for (uint32_t b = 0; b < B; b++) {
+        if (p->band_depth[b] > maxdepth) maxdepth = p->band_depth[b];
+        if (p->band_depth[b] > 0) scattering = 1;                      /* :3686-3692 */
+        if (p->emission[b]) emission = 1;                              /* :3772-3777 */
+        if (p->diffuse_flux[b] > 0.f) diffuse = 1;                     /* :3714-3720 */
+        for (uint32_t k = 0; k < S; k++) if (p->source_flux[k * B + b] > 0.f) rundirect = 1; /* :3818-3826 */
+    }
+    uint32_t d = (uint32_t)ceil(sqrt((double)p->direct_rays));
+    uint32_t n = (uint32_t)ceil(sqrt((double)p->diffuse_rays));
+
+    /* DIRECT (:3847-3982) */
+    if (S > 0 && rundirect && d > 0) {
+        double *sums = (double *)calloc((size_t)P * S * 2, sizeof(double));
+        orc_direct_sums(s, d, p->direct_seed, S, p->source_dir, sums);
+        for (uint32_t i = 0; i < P; i++)
+            for (uint32_t k = 0; k < S; k++) {
+                double stp = sums[((size_t)i * S + k) * 2], sbt = sums[((size_t)i * S + k) * 2 + 1];
+                for (uint32_t b = 0; b < B; b++) {
+                    double F = p->source_flux[k * B + b];
+                    double rho = p->rho[(size_t)i * B + b], tau = p->tau[(size_t)i * B + b];
+                    size_t q = (size_t)i * B + b;
+                    in[q] += F * (stp + sbt) * (1.0 - rho - tau);
+                    st[q] += F * (stp * rho + sbt * tau);
+                    sb[q] += F * (stp * tau + sbt * rho);
+                    if (diag) diag[2 * B + b] += F * (stp + sbt) * areas[i];
+                }
+            }
+        free(sums);
+    }
+    /* scattered direct energy -> radiation_out; one-sided primitives combine faces (:3999-4036) */
+    if (scattering && rundirect) {
+        for (uint32_t i = 0; i < P; i++)
+            for (uint32_t b = 0; b < B; b++) {
+                size_t q = (size_t)i * B + b;
+                ft[q] = st[q]; fb[q] = sb[q];
+                if (s->twos[i] == 0) { double tot = ft[q] + fb[q]; ft[q] = tot; fb[q] = tot; }
+            }
+        memcpy(ot, ft, PB * sizeof(double)); memcpy(ob, fb, PB * sizeof(double));
+        memset(st, 0, PB * sizeof(double)); memset(sb, 0, PB * sizeof(double));
+    }
+    int primary = 0;
+    if (emission || diffuse) {
+        if (emission) { /* :4041-4137 */
+            for (uint32_t b = 0; b < B; b++) {
+                if (!p->emission[b]) continue;
+                for (uint32_t i = 0; i < P; i++) {
+                    float eps = p->eps ? p->eps[(size_t)i * B + b] : 1.0f;
+                    if (p->band_depth[b] == 0 && eps != 1.f) eps = 1.f;
+                    float T = p->temperature ? p->temperature[i] : 300.f;
+                    if (T < 0) T = 300.f;
+                    double out = sigma * (double)eps * pow((double)T, 4.0);
+                    size_t q = (size_t)i * B + b;
+                    ft[q] += out;
+                    if (s->twos[i] != 0) fb[q] += out;
+                }
+            }
+        }
+        memcpy(ot, ft, PB * sizeof(double)); memcpy(ob, fb, PB * sizeof(double));
+        for (uint32_t b = 0; b < B; b++) { dflux[b] = p->diffuse_flux[b]; active[b] = 1; }
+        diffuse_pass(s, p, n, p->diffuse_seed[0], ot, ob, dflux, active, in, st, sb);
+        primary = 1;
+        if (diag) for (uint32_t b = 0; b < B; b++) diag[1 * B + b] = 0.0;
+    }
+    if (scattering && (emission || diffuse) && !rundirect) { /* :4225-4234 */
+        memcpy(ot, st, PB * sizeof(double)); memcpy(ob, sb, PB * sizeof(double));
+        memset(st, 0, PB * sizeof(double)); memset(sb, 0, PB * sizeof(double));
+    }
+    if (scattering && (emission || diffuse || rundirect)) { /* :4236-4356 */
+        for (uint32_t b = 0; b < B; b++) { dflux[b] = 0.f; active[b] = 1; }
+        for (uint32_t sidx = 0; sidx < maxdepth; sidx++) {
+            for (uint32_t b = 0; b < B; b++) {
+                if (active[b] && sidx + 1 > p->band_depth[b]) {
+                    active[b] = 0;
+                    if (p->schedule == 1) { /* conserving: a retiring band deposits its leftover scatter */
+                        for (uint32_t i = 0; i < P; i++) {
+                            size_t q = (size_t)i * B + b;
+                            in[q] += st[q] + sb[q]; st[q] = 0; sb[q] = 0;
+                        }
+                    }
+                }
+            }
+            int copy = (sidx > 0) || (emission && rundirect);
+            if (p->schedule == 1) copy = (sidx > 0) || primary;
+            if (copy) { memcpy(ot, st, PB * sizeof(double)); memcpy(ob, sb, PB * sizeof(double)); }
+            memset(st, 0, PB * sizeof(double)); memset(sb, 0, PB * sizeof(double));
+            diffuse_pass(s, p, n, p->diffuse_seed[1 + sidx], ot, ob, dflux, active, in, st, sb);
+        }
+    }
+    /* leftover scatter deposited as absorbed (:4631-4654) */
+    for (size_t q = 0; q < PB; q++) R[q] = in[q] + st[q] + sb[q];
+    if (diag) for (uint32_t i = 0; i < P; i++) for (uint32_t b = 0; b < B; b++) diag[b] += R[(size_t)i * B + b] * areas[i];
+    free(in); free(st); free(sb); free(ot); free(ob); free(ft); free(fb); free(dflux); free(active); free(areas);
+    return 0;
+}

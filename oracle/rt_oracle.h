@@ -1,0 +1,108 @@
+/*
+ * rt_oracle.h -- CPU restatement of the Helios radiation-transfer hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (helios_amd/, include/) links, loads or calls
+ * this library; only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it, as the
+ * checker.  It restates, in plain C, the per-ray FP32 arithmetic of the reference shaders and the
+ * pass schedule of RadiationModel::runBand, with FP64 per-primitive accumulation.
+ *
+ * Every function cites the reference file:line it follows (paths relative to the Helios tree;
+ * R = plugins/radiation, CD = plugins/collisiondetection).
+ *
+ * Pinning (how each part is checked against something other than itself; see tests/):
+ *   orc_tea            published TEA test vectors (32 rounds, key 0) -- tests/test_oracle_rng.py
+ *   orc_lcg_next       Numerical Recipes ranqd1 published sequence from seed 0
+ *   orc_sincos_2pi     libm sin/cos in double, max error <= 2 ulp over a dense sweep
+ *   predicates         CD selfTest golden cases (CD/tests/selfTest.cpp:3253-3297, 4522-4630,
+ *                      5111-5237), closed-form hit distances
+ *   orc_trace / castRays  SAH-BVH traversal == brute force on small random scenes
+ *   orc_run_band       Simple Direct (R/tests/selfTest.cpp:518-545), black parallel rectangles
+ *                      view factor (:670-729), Beer's-law canopy (:1372-1468), furnace-style
+ *                      second-law enclosure (:1061-1132), closed-form pass-schedule cases
+ */
+#ifndef HELIOS_RT_ORACLE_H
+#define HELIOS_RT_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- RNG (R/shaders/common/random.glsl:10-45) ---- */
+uint32_t orc_tea(uint32_t v0, uint32_t v1, uint32_t rounds, const uint32_t key[4], uint32_t *v1_out);
+uint32_t orc_tea16(uint32_t val0, uint32_t val1);
+uint32_t orc_lcg_next(uint32_t state);
+float orc_rnd(uint32_t *state);
+void orc_sincos_2pi(float r, float *s, float *c);
+
+/* ---- scene ---- */
+typedef struct orc_scene orc_scene;
+
+/* transforms: [P][16] row-major; types: 0 patch, 1 triangle, 3 tile (treated as patch);
+ * twosided: raw Helios flag 0..3; uuids: [P] (returned by castRays). */
+orc_scene *orc_scene_create(uint32_t P, const float *transforms, const uint32_t *types,
+                            const uint32_t *twosided, const uint32_t *uuids, int periodic_x, int periodic_y);
+void orc_scene_destroy(orc_scene *s);
+
+/* world vertices [P][4][3] (triangles use 3), normals [P][3], diffuse frame [P][4] (ce,se,ca,sa),
+ * domain bounds [4] (xmin,xmax,ymin,ymax of the periodic faces), bbox count */
+void orc_scene_geometry(const orc_scene *s, float *verts, float *normals, float *frame, float *domain, uint32_t *nbbox);
+
+/* Closest hit with the radiation predicates (intersections.glsl).  Returns prim index or 0xFFFFFFFF.
+ * brute != 0 tests every primitive (pin for the BVH path). */
+uint32_t orc_trace(const orc_scene *s, const float o[3], const float d[3], float tmin, uint32_t skip,
+                   int brute, float *t_out);
+
+/* One full radiation trace step with the periodic wrap loop (direct_raygen.comp:603-668 without glass);
+ * returns final hit prim or 0xFFFFFFFF. */
+uint32_t orc_trace_periodic(const orc_scene *s, const float o[3], const float d[3], float tmin, uint32_t skip);
+
+/* Ray generation (bit-exact FP32 restatement). */
+void orc_direct_ray(const orc_scene *s, uint32_t prim, uint32_t lx, uint32_t ly, uint32_t d, uint32_t seed,
+                    float origin[3]);
+void orc_diffuse_ray(const orc_scene *s, uint32_t prim, uint32_t lx, uint32_t ly, uint32_t n, uint32_t face,
+                     uint32_t seed, float origin[3], float dir[3]);
+
+/* Diffuse hit words for one pass: out[P][2][n*n], word = prim | (bottom_face_hit << 31), miss 0xFFFFFFFF,
+ * face slot not launched = 0xFFFFFFFE. */
+void orc_diffuse_hits(const orc_scene *s, uint32_t n, uint32_t seed, uint32_t *out);
+
+/* Direct visibility sums: out[P][S][2] = (sum strength lit-from-top, sum strength lit-from-bottom) */
+void orc_direct_sums(const orc_scene *s, uint32_t d, uint32_t seed, uint32_t nsrc, const float *src_dir, double *out);
+
+/* CollisionDetection::castRaysSoA semantics. */
+void orc_cast_rays(const orc_scene *s, uint64_t n, const float *origins, const float *dirs, float max_dist,
+                   float *out_t, float *out_normal, uint32_t *out_uuid, int brute);
+
+/* ---- runBand ---- */
+typedef struct {
+    uint32_t nbands;           /* B (launched bands, alphabetical order)               */
+    uint32_t nsources;         /* S (collimated)                                        */
+    uint32_t direct_rays;      /* max directRayCount over bands                         */
+    uint32_t diffuse_rays;     /* max diffuseRayCount over bands                        */
+    const uint32_t *band_depth;/* [B] scatteringDepth                                   */
+    const uint32_t *emission;  /* [B] emission flag                                     */
+    const float *source_dir;   /* [S][3] direction towards each collimated source       */
+    const float *source_flux;  /* [S][B]                                                */
+    const float *diffuse_flux; /* [B]                                                   */
+    const float *rho;          /* [P][B]                                                */
+    const float *tau;          /* [P][B]                                                */
+    const float *eps;          /* [P][B] (used for emission bands; NULL = 1)            */
+    const float *temperature;  /* [P] (NULL = 300 K)                                    */
+    uint32_t direct_seed;
+    const uint32_t *diffuse_seed; /* [1 + max depth]: primary diffuse pass, then each scattering pass */
+    uint32_t schedule;         /* 0 = reference pass schedule, 1 = energy-conserving    */
+} orc_band_params;
+
+/* R[P][B] = radiation_flux_<band>; diag (may be NULL) receives [B] FP64 energy totals:
+ * diag[0*B+b] absorbed sum(A*R), diag[1*B+b] primary diffuse incident, diag[2*B+b] direct incident */
+int orc_run_band(const orc_scene *s, const orc_band_params *p, double *R, double *diag);
+
+/* primitive areas [P] (patch |a x b|, triangle |e1 x e2|/2) in FP64 */
+void orc_areas(const orc_scene *s, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
