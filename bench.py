@@ -1,0 +1,332 @@
+#!/usr/bin/env python3
+"""One-GPU benchmark of the radiation-transfer hot path; prints ONE JSON line.
+
+Default workload (config 3, the diurnal series): an orchard of 25 trees (1,000,000 triangles + 1,600
+ground patches), 64 sun positions x 3 bands (LW with emission, NIR, PAR), 200 direct + 200 diffuse rays
+per primitive (d = n = 15), scattering depth 4.  One step = one diurnal series solved by the batched
+backend (all 64 suns in one set of persistent launches).  `value` = solves/s of the batched backend
+(64 solves per step / step time).  The plain one-solve-per-launch backend is timed on a few suns of the
+same series and reported beside it, with the speedup.
+
+Other workloads: --workload config2 (200k-triangle periodic canopy, one 2-band solve per step) and
+--workload castrays (CollisionDetection::castRaysSoA of 1e8 rays over the 5x tiled orchard).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from helios_amd import native, scenes  # noqa: E402
+from helios_amd import workloads as W  # noqa: E402
+
+METRIC = "Mrays/s & solves/s batched vs plain at <=0.5% flux err; p50/p99 solve latency"
+HBM_PEAK_GBS = 8000.0
+# canonical algorithmic bytes per traced ray: 64 B per BVH node visit, 48 B per primitive test,
+# 32 B of ray/result I/O (node/prim counts are measured by the kernels' own counters)
+B_NODE, B_PRIM, B_IO = 64, 48, 32
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def device_sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def pct(a, q):
+    return float(np.percentile(np.asarray(a, np.float64), q)) if len(a) else None
+
+
+def bytes_per_ray(c):
+    if not c["rays"]:
+        return None
+    return B_NODE * c["nodes"] / c["rays"] + B_PRIM * c["prims"] / c["rays"] + B_IO
+
+
+def cpu_baseline_diurnal(s, base, dirs, seconds):
+    """Time the CPU oracle (the restatement of the reference CPU path) on bounded samples of
+    primitives for one direct and one diffuse pass; extrapolate to plain solves/s by ray count."""
+    import ctypes as C
+
+    from oracle import oracle as O
+    o = O.OracleScene.from_arrays(s)
+    L = O.lib()
+    u32p, f64p, f32p = C.POINTER(C.c_uint32), C.POINTER(C.c_double), C.POINTER(C.c_float)
+    L.orc_diffuse_hits_range.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, u32p]
+    L.orc_direct_sums_range.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, f32p, C.c_uint32, C.c_uint32,
+                                        f64p]
+    rp = W.rays_per_solve(s, base)
+    d, n = rp["d"], rp["n"]
+    sd = np.ascontiguousarray(dirs[0], np.float32)
+    P = s.P
+    fracs = (0.1, 0.3, 0.5, 0.7, 0.9)
+
+    def run(width):
+        tdif = tdir = 0.0
+        rdif = rdir = 0
+        for f in fracs:
+            p0 = int(f * P)
+            p1 = min(P, p0 + width)
+            out = np.zeros((p1 - p0, 2, n * n), np.uint32)
+            t = time.perf_counter()
+            L.orc_diffuse_hits_range(o.h, n, 77, p0, p1, out.ctypes.data_as(u32p))
+            tdif += time.perf_counter() - t
+            rdif += int(np.sum(out != 0xFFFFFFFE))
+            outd = np.zeros((p1 - p0, 1, 2))
+            t = time.perf_counter()
+            L.orc_direct_sums_range(o.h, d, 5, 1, sd.ctypes.data_as(f32p), p0, p1, outd.ctypes.data_as(f64p))
+            tdir += time.perf_counter() - t
+            rdir += (p1 - p0) * d * d
+        return tdif, rdif, tdir, rdir
+
+    tdif, rdif, tdir, rdir = run(200)
+    per = (tdif + tdir) / 200.0
+    width = max(200, int(seconds / max(per, 1e-9)))
+    tdif, rdif, tdir, rdir = run(width)
+    dif_rate, dir_rate = rdif / tdif, rdir / tdir
+    t_solve = rp["direct"] / dir_rate + rp["diffuse"] / dif_rate
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": cores, "kind": "port",
+            "sample": (f"CPU oracle (FP32 rays, FP64 sums, OpenMP) on {5 * width} primitives x (one direct pass d={d} + "
+                       f"one diffuse pass n={n}, both faces) of the config-3 orchard: {rdir + rdif} rays in "
+                       f"{tdif + tdir:.1f} s ({dir_rate / 1e6:.2f} Mrays/s direct, {dif_rate / 1e6:.2f} Mrays/s diffuse); "
+                       f"plain solve = {rp['total']} rays -> {t_solve:.0f} s"),
+            "mrays_per_s": (rdir + rdif) / (tdif + tdir) / 1e6}
+
+
+def bench_diurnal(args):
+    K = args.suns
+    log(f"[bench] generating config-3 orchard ...")
+    s = scenes.orchard(3)
+    ctx = native.Context(0)
+    t0 = time.perf_counter()
+    g = ctx.scene(s.transforms, s.types, s.twosided)
+    build_s = time.perf_counter() - t0
+    info = g.info()
+    log(f"[bench] scene {info}; build {build_s * 1e3:.1f} ms")
+    base = W.diurnal_request(s)
+    dirs, flux, sky, seeds = W.diurnal_suns(K)
+    rp = W.rays_per_solve(s, base)
+    P, B = s.P, base.B
+    import ctypes as C
+    out_dev = C.c_void_p()
+    lib = ctx.lib
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipFree.argtypes = [C.c_void_p]
+    if hip.hipMalloc(C.byref(out_dev), K * P * B * 4) != 0:
+        raise RuntimeError("hipMalloc failed")
+    ctx.enable_timing(True)
+
+    def step():
+        native.run_band_batch(g, base, dirs, flux, sky, seeds, out_device_ptr=out_dev.value)
+
+    for w in range(args.warmup):
+        log(f"[bench] warmup {w}")
+        step()
+    # counted traversal work (one untimed series)
+    ctx.set_counting(True)
+    step()
+    counts = ctx.counters()
+    ctx.set_counting(False)
+    log(f"[bench] counters {counts}")
+    device_sync()
+    lat = []
+    kt = {"direct_ms": 0.0, "diffuse_ms": 0.0, "gather_ms": 0.0, "direct_launches": 0, "diffuse_launches": 0}
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        t = time.perf_counter()
+        step()
+        lat.append((time.perf_counter() - t) * 1e3)
+        tm = ctx.timing()
+        for key in kt:
+            kt[key] += tm[key]
+        log(f"[bench] step {k}: {lat[-1]:.1f} ms  {tm}")
+    device_sync()
+    wall = time.perf_counter() - t_start
+    ms_per_step = wall * 1e3 / args.steps
+    solves_s = K / (ms_per_step / 1e3)
+
+    # plain backend: one solve per launch sequence, a few suns of the same series
+    plain_lat = []
+    idx = np.linspace(0, K - 1, args.plain_suns).round().astype(int)
+    native.run_band(g, W.single_sun_request(base, dirs[idx[0]], flux[idx[0]], sky[idx[0]], seeds[idx[0]]))
+    for k in idx:
+        rq = W.single_sun_request(base, dirs[k], flux[k], sky[k], seeds[k])
+        t = time.perf_counter()
+        native.run_band(g, rq)
+        plain_lat.append((time.perf_counter() - t) * 1e3)
+        log(f"[bench] plain sun {k}: {plain_lat[-1]:.1f} ms")
+    plain_solves_s = 1e3 / float(np.mean(plain_lat))
+
+    # roofline of the dominant kernel class (HIP events on the solve stream)
+    per_step = {c: kt[f"{c}_ms"] / args.steps for c in ("direct", "diffuse", "gather")}
+    dom = max(("direct", "diffuse"), key=lambda c: per_step[c])
+    launches = kt[f"{dom}_launches"] / args.steps
+    avg_launch_ms = per_step[dom] / max(launches, 1)
+    brays = bytes_per_ray(counts[dom])
+    rays_per_launch = counts[dom]["rays"] / max(launches, 1)
+    achieved = brays * rays_per_launch / (avg_launch_ms / 1e3) / 1e9
+    traced_rays = counts["direct"]["rays"] + counts["diffuse"]["rays"]
+    res = {
+        "metric": METRIC, "value": round(solves_s, 4), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 2), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": (f"config3 diurnal series: orchard 25 trees ({P} prims: 1,000,000 triangles + 1,600 "
+                                f"ground patches), {K} suns x {B} bands (LW emission, NIR, PAR), d=n=15, depth 4, "
+                                f"batched (one series per step)"),
+                   "solves_per_step": K, "prims": P, "bands": B, "suns": K},
+        "mrays_per_s": round(rp["total"] * solves_s / 1e6, 1),
+        "mrays_per_s_traced": round(traced_rays / (ms_per_step / 1e3) / 1e6, 1),
+        "plain_solves_per_s": round(plain_solves_s, 4),
+        "plain_mrays_per_s": round(rp["total"] * plain_solves_s / 1e6, 1),
+        "speedup_batched_vs_plain": round(solves_s / plain_solves_s, 2),
+        "latency_ms": {"series_p50": pct(lat, 50), "series_p99": pct(lat, 99),
+                       "plain_solve_p50": pct(plain_lat, 50), "plain_solve_p99": pct(plain_lat, 99)},
+        "kernel_ms_per_step": {k: round(v, 2) for k, v in per_step.items()},
+        "rays_per_solve": rp["total"],
+        "scene_build_ms": round(build_s * 1e3, 1),
+        "roofline": {"bound": "hbm", "kernel": f"k_{dom}", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_ray": round(brays, 1),
+                     "nodes_per_ray": round(counts[dom]["nodes"] / counts[dom]["rays"], 2),
+                     "prims_per_ray": round(counts[dom]["prims"] / counts[dom]["rays"], 2),
+                     "avg_launch_ms": round(avg_launch_ms, 3), "rays_per_launch": int(rays_per_launch)},
+    }
+    hip.hipFree(out_dev)
+    if not args.no_cpu_baseline:
+        log("[bench] cpu baseline (oracle) ...")
+        res["cpu_baseline"] = cpu_baseline_diurnal(s, base, dirs, args.cpu_seconds)
+    return res
+
+
+def bench_config2(args):
+    s = scenes.config2(2)
+    ctx = native.Context(0)
+    g = ctx.scene(s.transforms, s.types, s.twosided, periodic=s.periodic)
+    req = W.config2_request(s)
+    rp = W.rays_per_solve(s, req)
+    ctx.enable_timing(True)
+    for _ in range(args.warmup):
+        native.run_band(g, req)
+    ctx.set_counting(True)
+    native.run_band(g, req)
+    counts = ctx.counters()
+    ctx.set_counting(False)
+    lat = []
+    kt = {"direct_ms": 0.0, "diffuse_ms": 0.0, "gather_ms": 0.0, "diffuse_launches": 0, "direct_launches": 0}
+    device_sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        native.run_band(g, req)
+        lat.append((time.perf_counter() - t) * 1e3)
+        tm = ctx.timing()
+        for key in kt:
+            kt[key] += tm[key]
+    device_sync()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    dom = "diffuse"
+    per = kt["diffuse_ms"] / args.steps / max(kt["diffuse_launches"] / args.steps, 1)
+    brays = bytes_per_ray(counts[dom])
+    rpl = counts[dom]["rays"] / max(kt["diffuse_launches"] / args.steps, 1)
+    ach = brays * rpl / (per / 1e3) / 1e9
+    return {"metric": METRIC, "value": round(1e3 / ms, 4), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(ms, 2), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "config2: 200k leaf triangles + 400 ground patches, periodic, NIR+PAR, d=n=32, depth 5"},
+            "mrays_per_s": round(rp["total"] / (ms / 1e3) / 1e6, 1),
+            "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99)},
+            "roofline": {"bound": "hbm", "kernel": "k_diffuse", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                         "bytes_per_ray": round(brays, 1)}}
+
+
+def bench_castrays(args):
+    import ctypes as C
+    s = scenes.orchard(3, tiles=5, jitter=0.05)
+    ctx = native.Context(0)
+    g = ctx.scene(s.transforms, s.types, s.twosided)
+    n = args.rays
+    T = s.transforms.reshape(-1, 16)
+    lo = [float(T[:, 3].min()), float(T[:, 7].min()), 0.0]
+    hi = [float(T[:, 3].max()), float(T[:, 7].max()), 3.6]
+    O, D = scenes.rays(11, n, lo, hi)
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    bufs = [C.c_void_p() for _ in range(5)]
+    sizes = [n * 12, n * 12, n * 4, n * 12, n * 4]
+    for b, sz in zip(bufs, sizes):
+        assert hip.hipMalloc(C.byref(b), sz) == 0
+    hip.hipMemcpy(bufs[0], O.ctypes.data, n * 12, 1)
+    hip.hipMemcpy(bufs[1], D.ctypes.data, n * 12, 1)
+    ctx.enable_timing(True)
+    for _ in range(args.warmup):
+        g.cast_rays_device(bufs[0].value, bufs[1].value, n, -1.0, bufs[2].value, bufs[3].value, bufs[4].value)
+    ctx.set_counting(True)
+    g.cast_rays_device(bufs[0].value, bufs[1].value, n, -1.0, bufs[2].value, bufs[3].value, bufs[4].value)
+    counts = ctx.counters()["cast"]
+    ctx.set_counting(False)
+    device_sync()
+    t0 = time.perf_counter()
+    kms = 0.0
+    for _ in range(args.steps):
+        g.cast_rays_device(bufs[0].value, bufs[1].value, n, -1.0, bufs[2].value, bufs[3].value, bufs[4].value)
+        kms += ctx.timing()["direct_ms"]
+    device_sync()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    brays = bytes_per_ray(counts)
+    ach = brays * n / (kms / args.steps / 1e3) / 1e9
+    return {"metric": "castRays Mrays/s", "value": round(n / (ms / 1e3) / 1e6, 1), "unit": "Mrays/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 2), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"config5 castRaysSoA: {n} rays (uniform in scene AABB, uniform directions) over "
+                                   f"the 5x tiled orchard ({s.P} prims)"},
+            "roofline": {"bound": "hbm", "kernel": "k_cast", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                         "bytes_per_ray": round(brays, 1)}}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="diurnal", choices=["diurnal", "config2", "castrays"])
+    ap.add_argument("--suns", type=int, default=64)
+    ap.add_argument("--plain-suns", type=int, default=4)
+    ap.add_argument("--rays", type=int, default=100_000_000)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is described in "
+              f"DESIGN.md and is not part of this build", file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "diurnal":
+        res = bench_diurnal(args)
+    elif args.workload == "config2":
+        res = bench_config2(args)
+    else:
+        res = bench_castrays(args)
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -84,6 +84,7 @@ int rt_destroy(rt_ctx *ctx) {
     (void)hipStreamSynchronize(ctx->sx.stream);
     for (auto &e : ctx->sx.ev) { if (e.a) (void)hipEventDestroy(e.a); if (e.b) (void)hipEventDestroy(e.b); }
     ctx->sx.ws.release();
+    if (ctx->sx.stats) (void)hipFree(ctx->sx.stats);
     (void)hipStreamDestroy(ctx->sx.stream);
     delete ctx;
     return 0;
@@ -155,7 +156,7 @@ int rt_cast_rays_soa_device(rt_ctx *ctx, const rt_scene *scene, const float *o, 
     CKR(hipMemsetAsync(err, 0, sizeof(int), st));
     hipEvent_t a = nullptr, b = nullptr;
     if (ctx->sx.timing) { CKR(hipEventCreate(&a)); CKR(hipEventCreate(&b)); CKR(hipEventRecord(a, st)); }
-    if (launch_cast(scene->s.d, n, o, d, max_dist, t, nrm, uuid, err, st)) return -1;
+    if (launch_cast(scene->s.d, scene->s.h.depth, n, o, d, max_dist, t, nrm, uuid, err, ctx->sx.stat_ptr(2), st)) return -1;
     if (b) CKR(hipEventRecord(b, st));
     int herr = 0;
     CKR(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -285,6 +286,27 @@ int rt_get_launch_cfg(rt_ctx *ctx, int kernel, rt_launch_cfg *cfg) {
     if (kernel < 0 || kernel > 2) { set_error("unknown kernel id"); return -1; }
     cfg->waves_per_block = ctx->sx.cfg[kernel].waves_per_block;
     cfg->blocks_per_cu = ctx->sx.cfg[kernel].blocks_per_cu;
+    return 0;
+}
+
+int rt_set_counting(rt_ctx *ctx, int on) {
+    NN(ctx, "ctx");
+    CKR(hipSetDevice(ctx->device));
+    if (!ctx->sx.stats) CKR(hipMalloc(&ctx->sx.stats, 9 * sizeof(unsigned long long)));
+    CKR(hipMemset(ctx->sx.stats, 0, 9 * sizeof(unsigned long long)));
+    ctx->sx.counting = on != 0;
+    return 0;
+}
+
+int rt_get_counters(rt_ctx *ctx, uint64_t *out) {
+    NN(ctx, "ctx");
+    NN(out, "out");
+    CKR(hipSetDevice(ctx->device));
+    if (!ctx->sx.stats) { for (int k = 0; k < 9; k++) out[k] = 0; return 0; }
+    CKR(hipStreamSynchronize(ctx->sx.stream));
+    unsigned long long h[9];
+    CKR(hipMemcpy(h, ctx->sx.stats, sizeof(h), hipMemcpyDeviceToHost));
+    for (int k = 0; k < 9; k++) out[k] = h[k];
     return 0;
 }
 

@@ -32,14 +32,15 @@ struct LaunchCfg {
 };
 
 // kernel launchers (rt_trace.hip)
-int launch_direct(const DirectJob *jobs_dev, uint32_t njobs, uint32_t total_items, uint32_t *counter, int *err,
-                  const LaunchCfg &cfg, int ncu, hipStream_t st);
-int launch_diffuse(const DiffuseJob *jobs_dev, uint32_t njobs, uint32_t total_items, uint32_t *counter, int *err,
-                   const LaunchCfg &cfg, int ncu, hipStream_t st);
+// stats (may be null): [0] BVH node fetches, [1] primitive tests, [2] rays traced
+int launch_direct(const DirectJob *jobs_dev, uint32_t njobs, uint32_t total_items, uint32_t depth, uint32_t *counter,
+                  int *err, unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st);
+int launch_diffuse(const DiffuseJob *jobs_dev, uint32_t njobs, uint32_t total_items, uint32_t depth, uint32_t *counter,
+                   int *err, unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st);
 int launch_gather(const GatherJob *jobs_dev, uint32_t njobs, uint32_t total_items, uint32_t cmax, uint32_t *counter,
                   const LaunchCfg &cfg, int ncu, hipStream_t st);
-int launch_cast(const DevScene *sc, uint64_t n, const float *o, const float *d, float maxd, float *t, float *nrm,
-                uint32_t *uuid, int *err, hipStream_t st);
+int launch_cast(const DevScene *sc, uint32_t depth, uint64_t n, const float *o, const float *d, float maxd, float *t,
+                float *nrm, uint32_t *uuid, int *err, unsigned long long *stats, hipStream_t st);
 int launch_debug_rays(const DevScene *sc, uint32_t P, uint32_t n, uint32_t seed, float *out, hipStream_t st);
 int launch_debug_trace(const DevScene *sc, uint32_t n, const float *rays, const uint32_t *skip, float tmin, int brute,
                        uint32_t *hit, float *t, int *err, hipStream_t st);

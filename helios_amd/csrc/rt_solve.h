@@ -33,6 +33,10 @@ struct SolveCtx {
     std::vector<EvRec> ev;
     rt_timing timing_info{};
     std::vector<double> energy;
+    // traversal counters: [0..2] direct, [3..5] diffuse, [6..8] castRays (node fetches, prim tests, rays)
+    bool counting = false;
+    unsigned long long *stats = nullptr;  // device, 9 counters
+    unsigned long long *stat_ptr(int k) { return counting ? stats + 3 * k : nullptr; }
 };
 
 // The per-sun part of a solve (K suns share everything else).

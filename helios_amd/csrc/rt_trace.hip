@@ -4,6 +4,9 @@
 //                   closest hit with the reference acceptance rule (t > tmin, skip the origin
 //                   primitive; R/shaders/common/bvh_traversal.glsl:68-170) and a canonical tie rule
 //                   (equal t -> lower primitive index) instead of traversal order.
+//   trace_any       occlusion query: any accepted hit with t <= tmax ends the traversal.  The direct
+//                   pass only needs "occluded or not"; with the nearest periodic face as tmax this
+//                   reproduces the reference decision (closest hit vs periodic face) exactly.
 //   k_direct        direct rays (R/shaders/direct_raygen.comp:323-731): one wave per (job, primitive),
 //                   d*d stratified samples, occlusion trace with periodic wrap, compensated sums of
 //                   ray strength split by lit face.
@@ -25,6 +28,11 @@ namespace rt {
 __device__ __forceinline__ float min3f(float a, float b, float c) { return fminf(fminf(a, b), c); }
 __device__ __forceinline__ float max3f(float a, float b, float c) { return fmaxf(fmaxf(a, b), c); }
 
+// traversal work counters (node fetches, primitive tests) of one lane
+struct TCnt {
+    uint32_t nodes, prims;
+};
+
 struct RayPre {
     float ix, iy, iz, ox, oy, oz;
 };
@@ -40,19 +48,64 @@ __device__ __forceinline__ RayPre ray_pre(v3f o, v3f d) {
     return r;
 }
 
+// BVH nodes and primitive records do not change while a trace kernel runs: loading them through the
+// constant address space lets the compiler use scalar loads when the index is wave-uniform (the
+// packet traversal) and ordinary vector loads otherwise.
+__device__ __forceinline__ float4 ldc(const float4 *p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    typedef __attribute__((address_space(4))) const float cfloat_t;
+    const cfloat_t *q = (const cfloat_t *)(const float *)p;
+    return make_float4(q[0], q[1], q[2], q[3]);
+#else
+    return *p;
+#endif
+}
+
+// wave-uniform copy of a pointer (lets the compiler keep it, and addresses built from uniform
+// indices, in scalar registers)
+template <typename T>
+__device__ __forceinline__ T *uptr(T *p) {
+    uint64_t v = (uint64_t)p;
+    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (T *)(((uint64_t)hi << 32) | lo);
+}
+
+// Intersect the primitive in leaf slot k; returns its index (RT_MISS if skipped) and the hit.
+template <bool CD>
+__device__ __forceinline__ bool leaf_hit_p(const float4 *recs, uint32_t k, v3f o, v3f d, uint32_t skip, uint32_t &id,
+                                           float &t) {
+    const float4 *r = recs + (size_t)k * 4;
+    float4 r0 = ldc(r);
+    uint32_t tag = __float_as_uint(r0.w);
+    id = tag & ~RT_PATCH_BIT;
+    if (id == skip) return false;
+    bool patch = (tag & RT_PATCH_BIT) != 0;
+    float4 r1 = ldc(r + 1), r2 = ldc(r + 2);
+    float4 r3 = patch ? ldc(r + 3) : make_float4(0.f, 0.f, 0.f, 0.f);
+    return isect_rec<CD>(r0, r1, r2, r3, patch, o, d, t);
+}
+
+template <bool CD>
+__device__ __forceinline__ bool leaf_hit(const DevScene &sc, uint32_t k, v3f o, v3f d, uint32_t skip, uint32_t &id,
+                                         float &t) {
+    const float4 *r = sc.rec + (size_t)k * 4;
+    float4 r0 = ldc(r);
+    uint32_t tag = __float_as_uint(r0.w);
+    id = tag & ~RT_PATCH_BIT;
+    if (id == skip) return false;
+    bool patch = (tag & RT_PATCH_BIT) != 0;
+    float4 r1 = ldc(r + 1), r2 = ldc(r + 2);
+    float4 r3 = patch ? ldc(r + 3) : make_float4(0.f, 0.f, 0.f, 0.f);
+    return isect_rec<CD>(r0, r1, r2, r3, patch, o, d, t);
+}
+
 template <bool CD>
 __device__ __forceinline__ void test_leaf(const DevScene &sc, uint32_t k, v3f o, v3f d, float tmin, uint32_t skip,
                                           float &best_t, uint32_t &best, uint32_t &best_slot) {
-    const float4 *r = sc.rec + (size_t)k * 4;
-    float4 r0 = r[0];
-    uint32_t tag = __float_as_uint(r0.w);
-    uint32_t id = tag & ~RT_PATCH_BIT;
-    if (id == skip) return;
-    bool patch = (tag & RT_PATCH_BIT) != 0;
-    float4 r1 = r[1], r2 = r[2];
-    float4 r3 = patch ? r[3] : make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t id;
     float t;
-    if (isect_rec<CD>(r0, r1, r2, r3, patch, o, d, t) && t > tmin) {
+    if (leaf_hit<CD>(sc, k, o, d, skip, id, t) && t > tmin) {
         if (t < best_t || (t == best_t && best != RT_MISS && id < best)) {
             best_t = t;
             best = id;
@@ -61,13 +114,51 @@ __device__ __forceinline__ void test_leaf(const DevScene &sc, uint32_t k, v3f o,
     }
 }
 
-template <bool CD>
+// slab tests of both children of node ni against [.., tlim]
+struct NodeTest {
+    float tn0, tn1;
+    uint32_t c0, c1;
+    bool h0, h1;
+};
+
+__device__ __forceinline__ NodeTest node_test_p(const float4 *nodes, uint32_t ni, const RayPre &rp, float tlim);
+
+__device__ __forceinline__ NodeTest node_test(const DevScene &sc, uint32_t ni, const RayPre &rp, float tlim) {
+    return node_test_p(sc.node, ni, rp, tlim);
+}
+
+__device__ __forceinline__ NodeTest node_test_p(const float4 *nodes, uint32_t ni, const RayPre &rp, float tlim) {
+    const float4 *nd = nodes + (size_t)ni * 4;
+    float4 n0 = ldc(nd), n1 = ldc(nd + 1), n2 = ldc(nd + 2), n3 = ldc(nd + 3);
+    float ax = fmaf(n0.x, rp.ix, -rp.ox), bx = fmaf(n0.y, rp.ix, -rp.ox);
+    float ay = fmaf(n0.z, rp.iy, -rp.oy), by = fmaf(n0.w, rp.iy, -rp.oy);
+    float az = fmaf(n1.x, rp.iz, -rp.oz), bz = fmaf(n1.y, rp.iz, -rp.oz);
+    float tn0 = max3f(fminf(ax, bx), fminf(ay, by), fminf(az, bz));
+    float tf0 = min3f(fmaxf(ax, bx), fmaxf(ay, by), fmaxf(az, bz));
+    float cx = fmaf(n1.z, rp.ix, -rp.ox), dx = fmaf(n1.w, rp.ix, -rp.ox);
+    float cy = fmaf(n2.x, rp.iy, -rp.oy), dy = fmaf(n2.y, rp.iy, -rp.oy);
+    float cz = fmaf(n2.z, rp.iz, -rp.oz), dz = fmaf(n2.w, rp.iz, -rp.oz);
+    float tn1 = max3f(fminf(cx, dx), fminf(cy, dy), fminf(cz, dz));
+    float tf1 = min3f(fmaxf(cx, dx), fmaxf(cy, dy), fmaxf(cz, dz));
+    NodeTest r;
+    r.tn0 = tn0;
+    r.tn1 = tn1;
+    r.h0 = tn0 <= tf0 && tf0 >= 0.f && tn0 <= tlim;
+    r.h1 = tn1 <= tf1 && tf1 >= 0.f && tn1 <= tlim;
+    r.c0 = __float_as_uint(n3.x);
+    r.c1 = __float_as_uint(n3.y);
+    return r;
+}
+
+// Closest accepted hit (t > tmin, t < tstart, ties -> lower primitive index).
+template <bool CD, int STK>
 __device__ uint32_t trace_closest(const DevScene &sc, v3f o, v3f d, float tmin, uint32_t skip, float tstart,
-                                  float &t_out, uint32_t &slot_out, uint32_t *stk, uint32_t stride, int *err) {
+                                  float &t_out, uint32_t &slot_out, uint32_t *stk, uint32_t stride, int *err,
+                                  TCnt &cnt) {
     float best_t = tstart;
     uint32_t best = RT_MISS, best_slot = RT_MISS;
     if (sc.nleaf <= 1) {
-        if (sc.nleaf == 1) test_leaf<CD>(sc, 0, o, d, tmin, skip, best_t, best, best_slot);
+        if (sc.nleaf == 1) { cnt.prims++; test_leaf<CD>(sc, 0, o, d, tmin, skip, best_t, best, best_slot); }
         t_out = best_t;
         slot_out = best_slot;
         return best;
@@ -82,38 +173,25 @@ __device__ uint32_t trace_closest(const DevScene &sc, v3f o, v3f d, float tmin, 
             atomicOr(err, 2);
             break;
         }
-        const float4 *nd = sc.node + (size_t)ni * 4;
-        float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
-        float ax = fmaf(n0.x, rp.ix, -rp.ox), bx = fmaf(n0.y, rp.ix, -rp.ox);
-        float ay = fmaf(n0.z, rp.iy, -rp.oy), by = fmaf(n0.w, rp.iy, -rp.oy);
-        float az = fmaf(n1.x, rp.iz, -rp.oz), bz = fmaf(n1.y, rp.iz, -rp.oz);
-        float tn0 = max3f(fminf(ax, bx), fminf(ay, by), fminf(az, bz));
-        float tf0 = min3f(fmaxf(ax, bx), fmaxf(ay, by), fmaxf(az, bz));
-        float cx = fmaf(n1.z, rp.ix, -rp.ox), dx = fmaf(n1.w, rp.ix, -rp.ox);
-        float cy = fmaf(n2.x, rp.iy, -rp.oy), dy = fmaf(n2.y, rp.iy, -rp.oy);
-        float cz = fmaf(n2.z, rp.iz, -rp.oz), dz = fmaf(n2.w, rp.iz, -rp.oz);
-        float tn1 = max3f(fminf(cx, dx), fminf(cy, dy), fminf(cz, dz));
-        float tf1 = min3f(fmaxf(cx, dx), fmaxf(cy, dy), fmaxf(cz, dz));
-        bool h0 = tn0 <= tf0 && tf0 >= 0.f && tn0 <= best_t;
-        bool h1 = tn1 <= tf1 && tf1 >= 0.f && tn1 <= best_t;
-        uint32_t c0 = __float_as_uint(n3.x), c1 = __float_as_uint(n3.y);
-        bool l0 = (c0 & RT_LEAF_BIT) != 0, l1 = (c1 & RT_LEAF_BIT) != 0;
-        if (h0 && l0) test_leaf<CD>(sc, c0 & ~RT_LEAF_BIT, o, d, tmin, skip, best_t, best, best_slot);
-        if (h1 && l1) test_leaf<CD>(sc, c1 & ~RT_LEAF_BIT, o, d, tmin, skip, best_t, best, best_slot);
-        bool g0 = h0 && !l0, g1 = h1 && !l1;
+        cnt.nodes++;
+        NodeTest nt = node_test(sc, ni, rp, best_t);
+        bool l0 = (nt.c0 & RT_LEAF_BIT) != 0, l1 = (nt.c1 & RT_LEAF_BIT) != 0;
+        if (nt.h0 && l0) { cnt.prims++; test_leaf<CD>(sc, nt.c0 & ~RT_LEAF_BIT, o, d, tmin, skip, best_t, best, best_slot); }
+        if (nt.h1 && l1) { cnt.prims++; test_leaf<CD>(sc, nt.c1 & ~RT_LEAF_BIT, o, d, tmin, skip, best_t, best, best_slot); }
+        bool g0 = nt.h0 && !l0, g1 = nt.h1 && !l1;
         if (g0 && g1) {
-            bool first0 = tn0 <= tn1;
-            if (sp >= RT_STACK) {
+            bool first0 = nt.tn0 <= nt.tn1;
+            if (sp >= STK) {
                 atomicOr(err, 1);
                 break;
             }
-            stk[(uint32_t)sp * stride] = first0 ? c1 : c0;
+            stk[(uint32_t)sp * stride] = first0 ? nt.c1 : nt.c0;
             sp++;
-            ni = first0 ? c0 : c1;
+            ni = first0 ? nt.c0 : nt.c1;
         } else if (g0) {
-            ni = c0;
+            ni = nt.c0;
         } else if (g1) {
-            ni = c1;
+            ni = nt.c1;
         } else {
             if (sp == 0) break;
             sp--;
@@ -123,6 +201,139 @@ __device__ uint32_t trace_closest(const DevScene &sc, v3f o, v3f d, float tmin, 
     t_out = best_t;
     slot_out = best_slot;
     return best;
+}
+
+// Any accepted radiation-predicate hit with tmin < t <= tmax?
+template <int STK>
+__device__ bool trace_any(const DevScene &sc, v3f o, v3f d, float tmin, uint32_t skip, float tmax, uint32_t *stk,
+                          uint32_t stride, int *err, TCnt &cnt) {
+    uint32_t id;
+    float t;
+    if (sc.nleaf <= 1) {
+        if (sc.nleaf == 1) {
+            cnt.prims++;
+            return leaf_hit<false>(sc, 0, o, d, skip, id, t) && t > tmin && t <= tmax;
+        }
+        return false;
+    }
+    RayPre rp = ray_pre(o, d);
+    uint32_t ni = 0;
+    int sp = 0;
+    uint32_t budget = sc.nleaf + 8;
+    while (true) {
+        if (budget-- == 0 || ni >= sc.nleaf - 1) {
+            atomicOr(err, 2);
+            return false;
+        }
+        cnt.nodes++;
+        NodeTest nt = node_test(sc, ni, rp, tmax);
+        bool l0 = (nt.c0 & RT_LEAF_BIT) != 0, l1 = (nt.c1 & RT_LEAF_BIT) != 0;
+        if (nt.h0 && l0) {
+            cnt.prims++;
+            if (leaf_hit<false>(sc, nt.c0 & ~RT_LEAF_BIT, o, d, skip, id, t) && t > tmin && t <= tmax) return true;
+        }
+        if (nt.h1 && l1) {
+            cnt.prims++;
+            if (leaf_hit<false>(sc, nt.c1 & ~RT_LEAF_BIT, o, d, skip, id, t) && t > tmin && t <= tmax) return true;
+        }
+        bool g0 = nt.h0 && !l0, g1 = nt.h1 && !l1;
+        if (g0 && g1) {
+            bool first0 = nt.tn0 <= nt.tn1;
+            if (sp >= STK) {
+                atomicOr(err, 1);
+                return false;
+            }
+            stk[(uint32_t)sp * stride] = first0 ? nt.c1 : nt.c0;
+            sp++;
+            ni = first0 ? nt.c0 : nt.c1;
+        } else if (g0) {
+            ni = nt.c0;
+        } else if (g1) {
+            ni = nt.c1;
+        } else {
+            if (sp == 0) return false;
+            sp--;
+            ni = stk[(uint32_t)sp * stride];
+        }
+    }
+}
+
+// Wave-coherent any-hit query: the 64 lanes (rays of one primitive towards one source, hence nearly
+// parallel and close together) walk the BVH together.  The node index and the stack are wave-uniform
+// (node fetched once per wave through scalar loads, stack in a per-wave LDS column); every lane tests
+// the child boxes and primitives itself and stops searching at its first accepted hit.  A lane reaches
+// every node whose box chain it intersects, so its answer equals the per-lane traversal's.
+template <int STK>
+__device__ bool packet_any(const DevScene &sc, v3f o, v3f d, float tmin, uint32_t skip, float tmax, bool active,
+                           uint32_t *wstk, int *err, TCnt &cnt) {
+    bool act = active, found = false;
+    uint32_t id;
+    float t;
+    if (sc.nleaf <= 1) {
+        if (sc.nleaf == 1 && act) {
+            cnt.prims++;
+            found = leaf_hit<false>(sc, 0, o, d, skip, id, t) && t > tmin && t <= tmax;
+        }
+        return found;
+    }
+    RayPre rp = ray_pre(o, d);
+    const float4 *nodes = uptr(sc.node), *recs = uptr(sc.rec);
+    const uint32_t nleaf = __builtin_amdgcn_readfirstlane(sc.nleaf);
+    uint32_t ni = 0;
+    uint32_t sp = 0;
+    uint32_t budget = nleaf + 8;
+    while (__ballot(act) != 0) {
+        ni = __builtin_amdgcn_readfirstlane(ni);
+        if (budget-- == 0 || ni >= nleaf - 1) {
+            atomicOr(err, 2);
+            break;
+        }
+        if (act) cnt.nodes++;  // per-ray visits (one fetch serves the whole wave)
+        NodeTest nt = node_test_p(nodes, ni, rp, tmax);
+        uint32_t c0 = __builtin_amdgcn_readfirstlane(nt.c0), c1 = __builtin_amdgcn_readfirstlane(nt.c1);
+        nt.c0 = c0;
+        nt.c1 = c1;
+        bool l0 = (nt.c0 & RT_LEAF_BIT) != 0, l1 = (nt.c1 & RT_LEAF_BIT) != 0;
+        bool h0 = act && nt.h0, h1 = act && nt.h1;
+        if (l0 && h0) {
+            cnt.prims++;
+            if (leaf_hit_p<false>(recs, nt.c0 & ~RT_LEAF_BIT, o, d, skip, id, t) && t > tmin && t <= tmax) {
+                found = true;
+                act = false;
+            }
+        }
+        if (l1 && h1 && act) {
+            cnt.prims++;
+            if (leaf_hit_p<false>(recs, nt.c1 & ~RT_LEAF_BIT, o, d, skip, id, t) && t > tmin && t <= tmax) {
+                found = true;
+                act = false;
+            }
+        }
+        bool g0 = !l0 && __ballot(h0 && act) != 0;
+        bool g1 = !l1 && __ballot(h1 && act) != 0;
+        if (g0 && g1) {
+            // nearer child first, as seen by the wave's first still-searching lane
+            uint64_t m = __ballot(act && nt.tn0 <= nt.tn1);
+            uint64_t a = __ballot(act);
+            bool first0 = (m & (a & (~a + 1))) != 0;
+            if (sp >= STK) {
+                atomicOr(err, 1);
+                break;
+            }
+            wstk[sp] = first0 ? nt.c1 : nt.c0;  // same value from every lane
+            sp++;
+            ni = first0 ? nt.c0 : nt.c1;
+        } else if (g0) {
+            ni = nt.c0;
+        } else if (g1) {
+            ni = nt.c1;
+        } else {
+            if (sp == 0) break;
+            sp--;
+            ni = wstk[sp];
+        }
+    }
+    return found;
 }
 
 // R/shaders/common/periodic_boundary.glsl:28-59
@@ -141,45 +352,88 @@ __device__ __forceinline__ float isect_bbox_face(const BBoxFace &f, v3f o, v3f d
     return -1.0f;
 }
 
-// R/shaders/common/periodic_boundary.glsl:73-112
-__device__ __forceinline__ bool periodic_wrap(const DevScene &sc, v3f o, v3f d, float bvh_t, v3f &w) {
-    float bt = bvh_t;
+// nearest periodic face strictly below tlim (periodic_boundary.glsl:84-92); returns its t or -1
+__device__ __forceinline__ float nearest_face(const DevScene &sc, v3f o, v3f d, float tlim) {
+    float bt = tlim;
     int cf = -1;
     for (uint32_t f = 0; f < sc.nbbox; f++) {
         float t = isect_bbox_face(sc.bbox[f], o, d);
         if (t > 0.0f && t < bt) { bt = t; cf = (int)f; }
     }
-    if (cf < 0) return false;
+    return cf < 0 ? -1.0f : bt;
+}
+
+// wrapped origin for a face hit at distance bt (periodic_boundary.glsl:94-111)
+__device__ __forceinline__ v3f wrap_at(const DevScene &sc, v3f o, v3f d, float bt) {
     v3f hp = add3(o, mul3(d, bt));
     const float eps = 1e-5f;
     float wx = sc.dom[1] - sc.dom[0], wy = sc.dom[3] - sc.dom[2];
-    w = hp;
+    v3f w = hp;
     if (sc.px && fabsf(hp.x - sc.dom[0]) <= eps) w.x += wx - eps;
     else if (sc.px && fabsf(hp.x - sc.dom[1]) <= eps) w.x -= wx - eps;
     else if (sc.py && fabsf(hp.y - sc.dom[2]) <= eps) w.y += wy - eps;
     else if (sc.py && fabsf(hp.y - sc.dom[3]) <= eps) w.y -= wy - eps;
-    return true;
+    return w;
 }
 
-// Up to 32 traces with a periodic-face check after each (direct_raygen.comp:603-668,
-// diffuse_raygen.comp:498-560, without translucent covers).  Returns the last traversal result.
+// Closest-hit trace with up to 32 periodic wraps (diffuse_raygen.comp:498-560 without translucent
+// covers).  Returns the last traversal result.
+template <int STK>
 __device__ __forceinline__ uint32_t trace_loop(const DevScene &sc, v3f o, v3f d, float tmin, uint32_t skip,
-                                               bool &exhausted, uint32_t *stk, uint32_t stride, int *err) {
+                                               bool &exhausted, uint32_t *stk, uint32_t stride, int *err, TCnt &cnt) {
     uint32_t hit = RT_MISS;
     v3f cur = o;
     exhausted = true;
     for (int step = 0; step < 32; ++step) {
         float t;
         uint32_t slot;
-        hit = trace_closest<false>(sc, cur, d, tmin, skip, 1e30f, t, slot, stk, stride, err);
+        hit = trace_closest<false, STK>(sc, cur, d, tmin, skip, 1e30f, t, slot, stk, stride, err, cnt);
         if (sc.nbbox) {
-            v3f w;
-            if (periodic_wrap(sc, cur, d, t, w)) { cur = w; continue; }
+            float bt = nearest_face(sc, cur, d, t);
+            if (bt > 0.f) { cur = wrap_at(sc, cur, d, bt); continue; }
         }
         exhausted = false;
         break;
     }
     return hit;
+}
+
+// Occlusion towards a collimated source with periodic wraps (direct_raygen.comp:603-672 without
+// translucent covers).  The reference wraps when the nearest periodic face is strictly closer than
+// the closest hit, and is occluded otherwise if anything was hit; "some hit with t <= t_face" is the
+// same decision, so an any-hit query suffices.  32 wraps without resolution count as unoccluded.
+template <int STK>
+__device__ __forceinline__ bool occluded(const DevScene &sc, v3f o, v3f d, float tmin, uint32_t skip, uint32_t *stk,
+                                         uint32_t stride, int *err, TCnt &cnt) {
+    v3f cur = o;
+    for (int step = 0; step < 32; ++step) {
+        float tf = sc.nbbox ? nearest_face(sc, cur, d, 1e30f) : -1.0f;
+        float tmax = tf > 0.f ? tf : 1e30f;
+        if (trace_any<STK>(sc, cur, d, tmin, skip, tmax, stk, stride, err, cnt)) return true;
+        if (tf <= 0.f) return false;
+        cur = wrap_at(sc, cur, d, tf);
+    }
+    return false;
+}
+
+// occluded() for a whole wave with packet_any(); each lane keeps its own wrap state.
+template <int STK>
+__device__ __forceinline__ bool packet_occluded(const DevScene &sc, v3f o, v3f d, float tmin, uint32_t skip,
+                                                bool active, uint32_t *wstk, int *err, TCnt &cnt) {
+    v3f cur = o;
+    bool need = active, occl = false;
+    for (int step = 0; step < 32; ++step) {
+        if (__ballot(need) == 0) break;
+        float tf = (need && sc.nbbox) ? nearest_face(sc, cur, d, 1e30f) : -1.0f;
+        float tmax = tf > 0.f ? tf : 1e30f;
+        bool f = packet_any<STK>(sc, cur, d, tmin, skip, tmax, need, wstk, err, cnt);
+        if (need) {
+            if (f) { occl = true; need = false; }
+            else if (tf <= 0.f) need = false;
+            else cur = wrap_at(sc, cur, d, tf);
+        }
+    }
+    return occl;
 }
 
 // Next work item of this wave.  No lane-divergent branch around the fetch: every lane issues the
@@ -212,16 +466,25 @@ __device__ __forceinline__ void wave_reduce(ksum &k) {
 
 __device__ __forceinline__ bool traced_type(uint32_t t) { return t == 0 || t == 1 || t == 3; }
 
+__device__ __forceinline__ void add_stats(unsigned long long *stats, const TCnt &c, uint32_t rays) {
+    if (stats) {
+        atomicAdd(&stats[0], (unsigned long long)c.nodes);
+        atomicAdd(&stats[1], (unsigned long long)c.prims);
+        atomicAdd(&stats[2], (unsigned long long)rays);
+    }
+}
+
 // ------------------------------------------------------------------------------------------------
 // direct rays
 // ------------------------------------------------------------------------------------------------
-template <int WPB>
+template <int WPB, int STK>
 __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict__ jobs, uint32_t njobs, uint32_t total,
-                                                     uint32_t *counter, int *err) {
-    __shared__ uint32_t stk_all[RT_STACK * WPB * 64];
-    uint32_t *stk = stk_all + threadIdx.x;
-    const uint32_t stride = WPB * 64;
+                                                     uint32_t *counter, int *err, unsigned long long *stats) {
+    __shared__ uint32_t wstk_all[STK * WPB];
+    uint32_t *wstk = wstk_all + (threadIdx.x >> 6) * STK;
     const uint32_t lane = threadIdx.x & 63;
+    TCnt cnt{0, 0};
+    uint32_t nray = 0;
     while (true) {
         uint32_t item = wave_item(counter);
         if (item >= total) break;
@@ -248,14 +511,18 @@ __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict
                 for (int q = 0; q < 12; q++) tm[q] = Tp[q];
                 ksum acc;
                 acc.init();
-                for (uint32_t r = lane; r < Nrays; r += 64) {
-                    uint32_t lx = r % d, ly = r / d;
-                    float spx, spy;
-                    direct_sample(i, lx, ly, d, J.seed, patch, spx, spy);
-                    v3f o = xform3(tm, spx, spy, 0.0f);
-                    bool ex;
-                    uint32_t h = trace_loop(sc, o, dir, 1e-4f, i, ex, stk, stride, err);
-                    if (h == RT_MISS || ex) acc.add(strength);
+                for (uint32_t base = 0; base < Nrays; base += 64) {
+                    uint32_t r = base + lane;
+                    bool need = r < Nrays;
+                    v3f o = mk3(0.f, 0.f, 0.f);
+                    if (need) {
+                        float spx, spy;
+                        direct_sample(i, r % d, r / d, d, J.seed, patch, spx, spy);
+                        o = xform3(tm, spx, spy, 0.0f);
+                        nray++;
+                    }
+                    bool occl = packet_occluded<STK>(sc, o, dir, 1e-4f, i, need, wstk, err, cnt);
+                    if (need && !occl) acc.add(strength);
                 }
                 wave_reduce(acc);
                 sum_out = acc.value();
@@ -266,18 +533,22 @@ __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict
             J.out[(size_t)i * 2 + 1] = top ? 0.f : sum_out;
         }
     }
+    add_stats(stats, cnt, nray);
 }
 
 // ------------------------------------------------------------------------------------------------
 // diffuse / scattering rays -> hit words
 // ------------------------------------------------------------------------------------------------
-template <int WPB>
+template <int WPB, int STK>
 __global__ __launch_bounds__(WPB * 64) void k_diffuse(const DiffuseJob *__restrict__ jobs, uint32_t njobs,
-                                                      uint32_t total, uint32_t *counter, int *err) {
-    __shared__ uint32_t stk_all[RT_STACK * WPB * 64];
+                                                      uint32_t total, uint32_t *counter, int *err,
+                                                      unsigned long long *stats) {
+    __shared__ uint32_t stk_all[STK * WPB * 64];
     uint32_t *stk = stk_all + threadIdx.x;
     const uint32_t stride = WPB * 64;
     const uint32_t lane = threadIdx.x & 63;
+    TCnt cnt{0, 0};
+    uint32_t nray = 0;
     while (true) {
         uint32_t item = wave_item(counter);
         if (item >= total) break;
@@ -310,7 +581,8 @@ __global__ __launch_bounds__(WPB * 64) void k_diffuse(const DiffuseJob *__restri
             v3f dir = frame_rotate(dl, fr.x, fr.y, fr.z, fr.w);
             if (slot == 1 && tw == 1) dir = mk3(-dir.x, -dir.y, -dir.z);
             bool ex;
-            uint32_t h = trace_loop(sc, o, dir, 1e-5f, i, ex, stk, stride, err);
+            nray++;
+            uint32_t h = trace_loop<STK>(sc, o, dir, 1e-5f, i, ex, stk, stride, err, cnt);
             uint32_t word = RT_MISS;
             if (h != RT_MISS) {
                 float4 hn = sc.nrm[h];
@@ -320,6 +592,7 @@ __global__ __launch_bounds__(WPB * 64) void k_diffuse(const DiffuseJob *__restri
             w[r] = word;
         }
     }
+    add_stats(stats, cnt, nray);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -461,19 +734,24 @@ __global__ __launch_bounds__(256) void k_gather_chan(const GatherJob *__restrict
 // ------------------------------------------------------------------------------------------------
 // castRaysSoA
 // ------------------------------------------------------------------------------------------------
+template <int STK>
 __global__ __launch_bounds__(256) void k_cast(const DevScene *__restrict__ scp, uint64_t n, const float *__restrict__ O,
                                               const float *__restrict__ D, float maxd, float *__restrict__ T,
-                                              float *__restrict__ N, uint32_t *__restrict__ U, int *err) {
-    __shared__ uint32_t stk_all[RT_STACK * 256];
+                                              float *__restrict__ N, uint32_t *__restrict__ U, int *err,
+                                              unsigned long long *stats) {
+    __shared__ uint32_t stk_all[STK * 256];
     uint32_t *stk = stk_all + threadIdx.x;
     const DevScene &sc = *scp;
+    TCnt cnt{0, 0};
+    uint32_t nray = 0;
     for (uint64_t r = (uint64_t)blockIdx.x * 256 + threadIdx.x; r < n; r += (uint64_t)gridDim.x * 256) {
         v3f o = mk3(O[r * 3], O[r * 3 + 1], O[r * 3 + 2]);
         v3f d = mk3(D[r * 3], D[r * 3 + 1], D[r * 3 + 2]);
         float tstart = maxd > 0.f ? maxd : FLT_MAX;
         float t;
         uint32_t slot;
-        uint32_t h = trace_closest<true>(sc, o, d, 1e-6f, RT_MISS, tstart, t, slot, stk, 256, err);
+        nray++;
+        uint32_t h = trace_closest<true, STK>(sc, o, d, 1e-6f, RT_MISS, tstart, t, slot, stk, 256, err, cnt);
         if (h == RT_MISS) { U[r] = RT_MISS; continue; }
         U[r] = sc.uuid[h];
         T[r] = t;
@@ -504,6 +782,7 @@ __global__ __launch_bounds__(256) void k_cast(const DevScene *__restrict__ scp, 
         N[r * 3 + 1] = nn.y;
         N[r * 3 + 2] = nn.z;
     }
+    add_stats(stats, cnt, nray);
 }
 
 __global__ void k_zero_u32(uint32_t *p) { *p = 0u; }
@@ -537,6 +816,7 @@ __global__ __launch_bounds__(256) void k_debug_trace(const DevScene *__restrict_
                                                      float tmin, int brute, uint32_t *__restrict__ hit,
                                                      float *__restrict__ tout, int *err) {
     __shared__ uint32_t stk_all[RT_STACK * 256];
+    TCnt cnt{0, 0};
     uint32_t *stk = stk_all + threadIdx.x;
     const DevScene &sc = *scp;
     uint32_t r = blockIdx.x * 256 + threadIdx.x;
@@ -548,7 +828,7 @@ __global__ __launch_bounds__(256) void k_debug_trace(const DevScene *__restrict_
     if (brute) {
         for (uint32_t k = 0; k < sc.nleaf; k++) test_leaf<false>(sc, k, o, d, tmin, skip[r], t, h, slot);
     } else {
-        h = trace_closest<false>(sc, o, d, tmin, skip[r], 1e30f, t, slot, stk, 256, err);
+        h = trace_closest<false, RT_STACK>(sc, o, d, tmin, skip[r], 1e30f, t, slot, stk, 256, err, cnt);
     }
     hit[r] = h;
     tout[r] = t;
@@ -592,31 +872,44 @@ static inline uint32_t grid_for(const LaunchCfg &cfg, int ncu, uint32_t total, u
         }                                                                               \
     } while (0)
 
-int launch_direct(const DirectJob *jobs, uint32_t njobs, uint32_t total, uint32_t *counter, int *err,
-                  const LaunchCfg &cfg, int ncu, hipStream_t st) {
+// LDS stack depth chosen from the BVH depth: 32 entries (32 KB per 256-lane block, 5 blocks per CU)
+// when the tree allows it, else 64.
+#define PICK_STK(depth, CALL32, CALL64) \
+    do {                                 \
+        if ((depth) + 1 <= 32) { CALL32; } else { CALL64; } \
+    } while (0)
+
+int launch_direct(const DirectJob *jobs, uint32_t njobs, uint32_t total, uint32_t depth, uint32_t *counter, int *err,
+                  unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st) {
     if (total == 0) return 0;
     k_zero_u32<<<1, 1, 0, st>>>(counter);
     uint32_t wpb = cfg.waves_per_block;
     uint32_t g = grid_for(cfg, ncu, total, wpb);
     switch (wpb) {
-        case 1: k_direct<1><<<g, 64, 0, st>>>(jobs, njobs, total, counter, err); break;
-        case 2: k_direct<2><<<g, 128, 0, st>>>(jobs, njobs, total, counter, err); break;
-        default: k_direct<4><<<g, 256, 0, st>>>(jobs, njobs, total, counter, err); break;
+        case 1: PICK_STK(depth, (k_direct<1, 32><<<g, 64, 0, st>>>(jobs, njobs, total, counter, err, stats)),
+                         (k_direct<1, 64><<<g, 64, 0, st>>>(jobs, njobs, total, counter, err, stats))); break;
+        case 2: PICK_STK(depth, (k_direct<2, 32><<<g, 128, 0, st>>>(jobs, njobs, total, counter, err, stats)),
+                         (k_direct<2, 64><<<g, 128, 0, st>>>(jobs, njobs, total, counter, err, stats))); break;
+        default: PICK_STK(depth, (k_direct<4, 32><<<g, 256, 0, st>>>(jobs, njobs, total, counter, err, stats)),
+                          (k_direct<4, 64><<<g, 256, 0, st>>>(jobs, njobs, total, counter, err, stats))); break;
     }
     LCHECK();
     return 0;
 }
 
-int launch_diffuse(const DiffuseJob *jobs, uint32_t njobs, uint32_t total, uint32_t *counter, int *err,
-                   const LaunchCfg &cfg, int ncu, hipStream_t st) {
+int launch_diffuse(const DiffuseJob *jobs, uint32_t njobs, uint32_t total, uint32_t depth, uint32_t *counter, int *err,
+                   unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st) {
     if (total == 0) return 0;
     k_zero_u32<<<1, 1, 0, st>>>(counter);
     uint32_t wpb = cfg.waves_per_block;
     uint32_t g = grid_for(cfg, ncu, total, wpb);
     switch (wpb) {
-        case 1: k_diffuse<1><<<g, 64, 0, st>>>(jobs, njobs, total, counter, err); break;
-        case 2: k_diffuse<2><<<g, 128, 0, st>>>(jobs, njobs, total, counter, err); break;
-        default: k_diffuse<4><<<g, 256, 0, st>>>(jobs, njobs, total, counter, err); break;
+        case 1: PICK_STK(depth, (k_diffuse<1, 32><<<g, 64, 0, st>>>(jobs, njobs, total, counter, err, stats)),
+                         (k_diffuse<1, 64><<<g, 64, 0, st>>>(jobs, njobs, total, counter, err, stats))); break;
+        case 2: PICK_STK(depth, (k_diffuse<2, 32><<<g, 128, 0, st>>>(jobs, njobs, total, counter, err, stats)),
+                         (k_diffuse<2, 64><<<g, 128, 0, st>>>(jobs, njobs, total, counter, err, stats))); break;
+        default: PICK_STK(depth, (k_diffuse<4, 32><<<g, 256, 0, st>>>(jobs, njobs, total, counter, err, stats)),
+                          (k_diffuse<4, 64><<<g, 256, 0, st>>>(jobs, njobs, total, counter, err, stats))); break;
     }
     LCHECK();
     return 0;
@@ -643,12 +936,13 @@ int launch_gather(const GatherJob *jobs, uint32_t njobs, uint32_t total, uint32_
     return 0;
 }
 
-int launch_cast(const DevScene *sc, uint64_t n, const float *o, const float *d, float maxd, float *t, float *nrm,
-                uint32_t *uuid, int *err, hipStream_t st) {
+int launch_cast(const DevScene *sc, uint32_t depth, uint64_t n, const float *o, const float *d, float maxd, float *t,
+                float *nrm, uint32_t *uuid, int *err, unsigned long long *stats, hipStream_t st) {
     if (n == 0) return 0;
     uint64_t need = (n + 255) / 256;
     uint32_t g = (uint32_t)(need < 65536ull * 16 ? need : 65536ull * 16);
-    k_cast<<<g, 256, 0, st>>>(sc, n, o, d, maxd, t, nrm, uuid, err);
+    PICK_STK(depth, (k_cast<32><<<g, 256, 0, st>>>(sc, n, o, d, maxd, t, nrm, uuid, err, stats)),
+             (k_cast<64><<<g, 256, 0, st>>>(sc, n, o, d, maxd, t, nrm, uuid, err, stats)));
     LCHECK();
     return 0;
 }

@@ -77,6 +77,8 @@ SIGNATURES = {
     "rt_set_launch_cfg": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(rt_launch_cfg)]),
     "rt_get_launch_cfg": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(rt_launch_cfg)]),
     "rt_enable_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "rt_set_counting": (C.c_int, [C.c_void_p, C.c_int]),
+    "rt_get_counters": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "rt_get_timing": (C.c_int, [C.c_void_p, C.POINTER(rt_timing)]),
     "rt_debug_diffuse_hits": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, u32p]),
     "rt_debug_direct_sums": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, f32p, f32p]),
@@ -184,6 +186,18 @@ class Context:
         t = rt_timing()
         _check(self.lib.rt_get_timing(self._h, C.byref(t)))
         return {k: getattr(t, k) for k, _ in rt_timing._fields_}
+
+    def set_counting(self, on: bool = True):
+        """Zero and enable/disable the traversal work counters."""
+        _check(self.lib.rt_set_counting(self._h, 1 if on else 0))
+
+    def counters(self) -> dict:
+        """BVH node fetches, primitive tests and rays of the direct, diffuse and castRays kernels."""
+        out = (C.c_uint64 * 9)()
+        _check(self.lib.rt_get_counters(self._h, out))
+        v = list(out)
+        return {k: {"nodes": v[3 * i], "prims": v[3 * i + 1], "rays": v[3 * i + 2]}
+                for i, k in enumerate(("direct", "diffuse", "cast"))}
 
 
 class Scene:

@@ -158,6 +158,11 @@ typedef struct {
     uint32_t diffuse_launches;
 } rt_timing;
 int rt_enable_timing(rt_ctx *ctx, int on);
+/* Traversal work counters (zeroed by rt_set_counting): out[9] = direct (BVH node fetches, primitive
+ * tests, rays), diffuse/scatter (same three), castRays (same three).  They define the counted
+ * algorithmic bytes per ray used for the roofline. */
+int rt_set_counting(rt_ctx *ctx, int on);
+int rt_get_counters(rt_ctx *ctx, uint64_t *out);
 int rt_get_timing(rt_ctx *ctx, rt_timing *out);
 
 /* Debug/parity access: one diffuse pass's hit words [P][2][n*n] (word = prim | bottom<<31,

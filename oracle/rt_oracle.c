@@ -708,13 +708,13 @@ void orc_diffuse_ray(const orc_scene *s, uint32_t prim, uint32_t lx, uint32_t ly
 
 static int traced_type(uint32_t ty) { return ty == 0 || ty == 1 || ty == 3; }
 
-void orc_diffuse_hits(const orc_scene *s, uint32_t n, uint32_t seed, uint32_t *out) {
+void orc_diffuse_hits_range(const orc_scene *s, uint32_t n, uint32_t seed, uint32_t p0, uint32_t p1, uint32_t *out) {
     uint32_t nn = n * n;
 #pragma omp parallel for schedule(dynamic, 16)
-    for (long long ii = 0; ii < (long long)s->P; ii++) {
+    for (long long ii = (long long)p0; ii < (long long)p1; ii++) {
         uint32_t i = (uint32_t)ii;
         for (uint32_t face = 0; face < 2; face++) {
-            uint32_t *w = out + ((size_t)i * 2 + (1 - face)) * nn; /* slot 0 = top (face 1), slot 1 = bottom */
+            uint32_t *w = out + ((size_t)(i - p0) * 2 + (1 - face)) * nn; /* slot 0 = top (face 1), slot 1 = bottom */
             int launched = traced_type(s->type[i]) && !(face == 0 && s->twos[i] == 0);
             for (uint32_t r = 0; r < nn; r++) {
                 if (!launched) { w[r] = 0xFFFFFFFEu; continue; }
@@ -731,14 +731,20 @@ void orc_diffuse_hits(const orc_scene *s, uint32_t n, uint32_t seed, uint32_t *o
     }
 }
 
+void orc_diffuse_hits(const orc_scene *s, uint32_t n, uint32_t seed, uint32_t *out) {
+    orc_diffuse_hits_range(s, n, seed, 0, s->P, out);
+}
+
 /* direct_raygen.comp:413-670 for collimated sources: per (prim, source) sums of ray strength over
  * unoccluded rays, split by the lit face. */
-void orc_direct_sums(const orc_scene *s, uint32_t d, uint32_t seed, uint32_t nsrc, const float *src_dir, double *out) {
+void orc_direct_sums_range(const orc_scene *s, uint32_t d, uint32_t seed, uint32_t nsrc, const float *src_dir,
+                           uint32_t p0, uint32_t p1, double *out) {
     uint32_t Nrays = d * d;
 #pragma omp parallel for schedule(dynamic, 16)
-    for (long long ii = 0; ii < (long long)s->P; ii++) {
+    for (long long ii = (long long)p0; ii < (long long)p1; ii++) {
         uint32_t i = (uint32_t)ii;
-        for (uint32_t k = 0; k < nsrc * 2; k++) out[(size_t)i * nsrc * 2 + k] = 0.0;
+        double *o_i = out + (size_t)(i - p0) * nsrc * 2;
+        for (uint32_t k = 0; k < nsrc * 2; k++) o_i[k] = 0.0;
         if (!traced_type(s->type[i])) continue;
         v3 n = s->normal[i];
         for (uint32_t r = 0; r < Nrays; r++) {
@@ -753,10 +759,14 @@ void orc_direct_sums(const orc_scene *s, uint32_t d, uint32_t seed, uint32_t nsr
                 int ex;
                 uint32_t h = trace_loop(s, o, dir, 1e-4f, i, &ex);
                 if (h != MISS && !ex) continue; /* opaque occluder */
-                out[((size_t)i * nsrc + src) * 2 + (top ? 0 : 1)] += (double)strength;
+                o_i[src * 2 + (top ? 0 : 1)] += (double)strength;
             }
         }
     }
+}
+
+void orc_direct_sums(const orc_scene *s, uint32_t d, uint32_t seed, uint32_t nsrc, const float *src_dir, double *out) {
+    orc_direct_sums_range(s, d, seed, nsrc, src_dir, 0, s->P, out);
 }
 
 /* ------------------------------------------------------------------------------------------------ */

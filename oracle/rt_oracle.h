@@ -71,6 +71,11 @@ void orc_diffuse_hits(const orc_scene *s, uint32_t n, uint32_t seed, uint32_t *o
 /* Direct visibility sums: out[P][S][2] = (sum strength lit-from-top, sum strength lit-from-bottom) */
 void orc_direct_sums(const orc_scene *s, uint32_t d, uint32_t seed, uint32_t nsrc, const float *src_dir, double *out);
 
+/* Range versions (primitives [p0, p1) only; output indexed from p0) -- used to time bounded samples. */
+void orc_diffuse_hits_range(const orc_scene *s, uint32_t n, uint32_t seed, uint32_t p0, uint32_t p1, uint32_t *out);
+void orc_direct_sums_range(const orc_scene *s, uint32_t d, uint32_t seed, uint32_t nsrc, const float *src_dir,
+                           uint32_t p0, uint32_t p1, double *out);
+
 /* CollisionDetection::castRaysSoA semantics. */
 void orc_cast_rays(const orc_scene *s, uint64_t n, const float *origins, const float *dirs, float max_dist,
                    float *out_t, float *out_normal, uint32_t *out_uuid, int brute);
