@@ -256,7 +256,7 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
         for (uint32_t s = 0; s < maxdepth; s++) {
             bool active = loop && (s + 1 <= q.band_depth[b]);
             bool retire = (q.schedule == 1) && was_active && !active;
-            bool copy = loop && ((s > 0) || (q.schedule == 1 ? prim : (emission && rundirect[k])));
+            bool copy = loop && ((s > 0) || (q.schedule == 1 ? (prim && rundirect[k]) : (emission && rundirect[k])));
             M(M_BASE + 4 * s + 0, c) = retire;
             M(M_BASE + 4 * s + 1, c) = copy;
             M(M_BASE + 4 * s + 2, c) = loop;

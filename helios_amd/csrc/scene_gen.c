@@ -138,15 +138,23 @@ uint32_t sg_config1(uint32_t seed, float *T, uint32_t *type, uint32_t *twos) {
 }
 
 /* Generic homogeneous canopy of square leaves (used by the Beer's-law checks):
- * nleaves leaves of side w in [-D/2,D/2]^2 x [w/2, w/2 + h], ground D x D split ng x ng. */
-uint32_t sg_patch_canopy(uint32_t seed, uint32_t nleaves, float w, float D, float h, uint32_t ng, float *T,
-                         uint32_t *type, uint32_t *twos) {
+ * nleaves leaves of side w in [-D/2,D/2]^2 x [w/2, w/2 + h], ground D x D split ng x ng.
+ * inset != 0 keeps every leaf's horizontal extent inside [-D/2, D/2]^2, so that with periodic
+ * boundaries the ground alone defines the periodic cell (no uncovered seam strip). */
+uint32_t sg_patch_canopy(uint32_t seed, uint32_t nleaves, float w, float D, float h, uint32_t ng, uint32_t inset,
+                         float *T, uint32_t *type, uint32_t *twos) {
     sg_out o = {T, type, twos, 0};
     sg_rng r = {0xBEEFull ^ ((uint64_t)seed << 24)};
     for (uint32_t i = 0; i < nleaves; i++) {
-        float cx = (-0.5f + uni(&r)) * D, cy = (-0.5f + uni(&r)) * D, cz = 0.5f * w + uni(&r) * h;
+        float ux = uni(&r), uy = uni(&r), cz = 0.5f * w + uni(&r) * h;
         float R[9];
         spherical_leaf(&r, R);
+        float ex = 0.f, ey = 0.f;
+        if (inset) { /* half extents of the rotated square in x and y */
+            ex = 0.5f * w * (fabsf(R[0]) + fabsf(R[1]));
+            ey = 0.5f * w * (fabsf(R[3]) + fabsf(R[4]));
+        }
+        float cx = -0.5f * D + ex + ux * (D - 2.0f * ex), cy = -0.5f * D + ey + uy * (D - 2.0f * ey);
         put_patch(&o, cx, cy, cz, w, w, R, 1);
     }
     if (ng) ground(&o, -0.5f * D, 0.5f * D, -0.5f * D, 0.5f * D, ng, ng);
@@ -187,7 +195,12 @@ static const float RING_S[16] = {0.0f, 0.382683432365f, 0.707106781187f, 0.92387
                                  0.707106781187f, 0.382683432365f, 0.0f, -0.382683432365f, -0.707106781187f,
                                  -0.923879532511f, -1.0f, -0.923879532511f, -0.707106781187f, -0.382683432365f};
 
-static void tube(sg_out *o, const float c0[3], const float c1[3], float r0, float r1) {
+static void tube_n(sg_out *o, const float c0[3], const float c1[3], float r0, float r1, int ntri);
+
+static void tube(sg_out *o, const float c0[3], const float c1[3], float r0, float r1) { tube_n(o, c0, c1, r0, r1, 32); }
+
+/* first ntri (<= 32) triangles of a 16-gon tube segment */
+static void tube_n(sg_out *o, const float c0[3], const float c1[3], float r0, float r1, int ntri) {
     float ax[3] = {c1[0] - c0[0], c1[1] - c0[1], c1[2] - c0[2]};
     float L = sqrtf(ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2]);
     for (int a = 0; a < 3; a++) ax[a] /= L;
@@ -207,16 +220,17 @@ static void tube(sg_out *o, const float c0[3], const float c1[3], float r0, floa
             p11[a] = c1[a] + r1 * (RING_C[k2] * u[a] + RING_S[k2] * v[a]);
         }
         /* outward normal = cross(v1 - v0, v2 - v0) */
-        put_tri(o, p00, p01, p11, 0);
-        put_tri(o, p00, p11, p10, 0);
+        if (2 * k < ntri) put_tri(o, p00, p01, p11, 0);
+        if (2 * k + 1 < ntri) put_tri(o, p00, p11, p10, 0);
     }
 }
 
-/* One tree at (tx, ty): trunk + branches as 16-gon tubes (radii 0.15 -> 0.02 m, ntube_tris tube
- * triangles, a multiple of 32) and nleaf leaf triangles (area log-uniform 6-14 cm^2) in an
- * ellipsoidal crown. */
+/* One tree at (tx, ty): trunk + branches as 16-gon tubes (radii 0.15 -> 0.02 m, exactly ntube_tris
+ * tube triangles: whole 32-triangle ring segments plus one partial segment) and nleaf leaf
+ * triangles (area uniform 6-14 cm^2) in an ellipsoidal crown. */
 static void tree(sg_out *o, sg_rng *r, float tx, float ty, uint32_t nleaf, uint32_t ntube_tris, float jitter) {
     uint32_t segs = ntube_tris / 32;
+    uint32_t rest = ntube_tris % 32;
     uint32_t trunk = segs < 12 ? segs : 12;
     float base[3] = {tx, ty, 0.f};
     float top = 1.6f;
@@ -247,6 +261,10 @@ static void tree(sg_out *o, sg_rng *r, float tx, float ty, uint32_t nleaf, uint3
         }
         left -= ns;
         b++;
+    }
+    if (rest) { /* partial collar segment at the trunk top */
+        float c0[3] = {base[0], base[1], top}, c1[3] = {base[0], base[1], top + 0.05f};
+        tube_n(o, c0, c1, 0.08f, 0.07f, (int)rest);
     }
     /* crown: ellipsoid centred at (tx, ty, 2.4), radii (1.5, 1.5, 1.1) */
     for (uint32_t i = 0; i < nleaf; i++) {

@@ -21,7 +21,7 @@ def _load():
         f32p, u32p = C.POINTER(C.c_float), C.POINTER(C.c_uint32)
         u32 = C.c_uint32
         _lib.sg_config1.argtypes = [u32, f32p, u32p, u32p]
-        _lib.sg_patch_canopy.argtypes = [u32, u32, C.c_float, C.c_float, C.c_float, u32, f32p, u32p, u32p]
+        _lib.sg_patch_canopy.argtypes = [u32, u32, C.c_float, C.c_float, C.c_float, u32, u32, f32p, u32p, u32p]
         _lib.sg_config2.argtypes = [u32, u32, f32p, u32p, u32p]
         _lib.sg_orchard.argtypes = [u32, u32, u32, u32, u32, u32, C.c_float, f32p, u32p, u32p]
         _lib.sg_random_canopy.argtypes = [u32, u32, f32p, u32p, u32p]
@@ -70,8 +70,11 @@ def config1(seed: int = 1) -> SceneArrays:
     return _gen(_load().sg_config1, seed)
 
 
-def patch_canopy(seed: int, nleaves: int, w: float, D: float, h: float, ng: int, periodic=False) -> SceneArrays:
-    s = _gen(_load().sg_patch_canopy, seed, nleaves, w, D, h, ng)
+def patch_canopy(seed: int, nleaves: int, w: float, D: float, h: float, ng: int, periodic=False,
+                 inset=False) -> SceneArrays:
+    """Square leaves of side w, centres uniform in a D x D x h volume, over a D x D ground of ng x ng
+    patches; inset keeps each leaf inside the ground's footprint (exact periodic cell)."""
+    s = _gen(_load().sg_patch_canopy, seed, nleaves, w, D, h, ng, 1 if inset else 0)
     s.periodic = (periodic, periodic)
     return s
 

@@ -949,7 +949,9 @@ int orc_run_band(const orc_scene *s, const orc_band_params *p, double *R, double
                 }
             }
             int copy = (sidx > 0) || (emission && rundirect);
-            if (p->schedule == 1) copy = (sidx > 0) || primary;
+            /* conserving: also hand the primary pass's scatter on when direct ran and emission is off
+             * (without direct it was already moved to radiation_out after the primary pass) */
+            if (p->schedule == 1) copy = (sidx > 0) || (primary && rundirect);
             if (copy) { memcpy(ot, st, PB * sizeof(double)); memcpy(ob, sb, PB * sizeof(double)); }
             memset(st, 0, PB * sizeof(double)); memset(sb, 0, PB * sizeof(double));
             diffuse_pass(s, p, n, p->diffuse_seed[1 + sidx], ot, ob, dflux, active, in, st, sb);

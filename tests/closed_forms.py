@@ -1,0 +1,84 @@
+"""Scenes and analytic answers of the reference's closed-form radiation tests
+(plugins/radiation/tests/selfTest.cpp), restated with explicit transforms."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+SIGMA = 5.6703744e-8
+
+
+def patch_T(center, ex, ey):
+    """Row-major transform mapping the canonical patch (-0.5..0.5)^2 onto a patch spanned by ex, ey."""
+    T = np.zeros(16, np.float32)
+    T[0], T[4], T[8] = ex
+    T[1], T[5], T[9] = ey
+    T[2], T[6], T[10] = np.cross(ex, ey) / max(np.linalg.norm(np.cross(ex, ey)), 1e-30)
+    T[3], T[7], T[11] = center
+    T[15] = 1.0
+    return T
+
+
+def tri_T(v0, v1, v2):
+    v0, v1, v2 = (np.asarray(v, np.float32) for v in (v0, v1, v2))
+    T = np.zeros(16, np.float32)
+    c0, c1 = v2 - v1, v1 - v0
+    n = np.cross(c0, c1)
+    n = n / max(np.linalg.norm(n), 1e-30)
+    T[0], T[4], T[8] = c0
+    T[1], T[5], T[9] = c1
+    T[2], T[6], T[10] = n
+    T[3], T[7], T[11] = v0
+    T[15] = 1.0
+    return T
+
+
+def parallel_rectangles(a=1.0, b=2.0, c=0.5):
+    """Two one-sided a x b rectangles facing each other at distance c (selfTest.cpp:670-729)."""
+    T = np.stack([patch_T((0, 0, 0), (a, 0, 0), (0, b, 0)),           # faces +z
+                  patch_T((0, 0, c), (0, b, 0), (a, 0, 0))])           # faces -z
+    X, Y = a / c, b / c
+    X2, Y2 = X * X, Y * Y
+    F12 = 2.0 / (math.pi * X * Y) * (math.log(math.sqrt((1 + X2) * (1 + Y2) / (1 + X2 + Y2)))
+                                     + X * math.sqrt(1 + Y2) * math.atan(X / math.sqrt(1 + Y2))
+                                     + Y * math.sqrt(1 + X2) * math.atan(Y / math.sqrt(1 + X2))
+                                     - X * math.atan(X) - Y * math.atan(Y))
+    return T, np.zeros(2, np.uint32), np.zeros(2, np.uint32), F12
+
+
+def closed_box(L=10.0, m=5):
+    """Closed cube of side L, each face split m x m, patches facing inwards (selfTest.cpp:1061-1132)."""
+    h, s = L / 2, L / m
+    Ts = []
+    e = np.eye(3, dtype=np.float32) * s
+    faces = [  # (axis, sign of position, ex, ey) with cross(ex, ey) pointing inwards
+        (2, -1, e[0], e[1]), (2, +1, e[1], e[0]), (0, -1, e[1], e[2]), (0, +1, e[2], e[1]),
+        (1, -1, e[2], e[0]), (1, +1, e[0], e[2])]
+    for axis, sgn, ex, ey in faces:
+        for j in range(m):
+            for i in range(m):
+                u = -h + (i + 0.5) * s
+                v = -h + (j + 0.5) * s
+                c = np.zeros(3, np.float32)
+                c[axis] = sgn * h
+                # place (u, v) along the directions of ex, ey
+                c = c + (ex / s) * u + (ey / s) * v
+                Ts.append(patch_T(c, ex, ey))
+    T = np.stack(Ts)
+    return T, np.zeros(len(Ts), np.uint32), np.zeros(len(Ts), np.uint32)
+
+
+def beer_direct(lai, theta_s):
+    """Fraction of collimated flux intercepted by a spherical-LAD canopy (selfTest.cpp:1462)."""
+    return 1.0 - math.exp(-0.5 * lai / math.cos(theta_s))
+
+
+def beer_diffuse(lai, N=50):
+    """Fraction of isotropic diffuse flux intercepted (midpoint rule, selfTest.cpp:1453-1460)."""
+    dth = 0.5 * math.pi / N
+    tot = 0.0
+    for i in range(N):
+        th = (i + 0.5) * dth
+        tot += 2.0 * (1.0 - math.exp(-0.5 * lai / math.cos(th))) * math.cos(th) * math.sin(th) * dth
+    return tot

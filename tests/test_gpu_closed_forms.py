@@ -1,0 +1,121 @@
+"""The HIP path (through the C ABI) against the reference's closed forms and golden ray-casting cases,
+and batched-vs-plain identity on the many-channel gather path."""
+import math
+
+import numpy as np
+import pytest
+
+from helios_amd import native, scenes
+from helios_amd import workloads as W
+from oracle.oracle import OracleScene
+from tests import cd_cases
+from tests import closed_forms as cf
+from tests.helpers import rel_err
+from tests.test_oracle_physics import _beer_canopy, req1
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("case", cd_cases.load()["cases"], ids=lambda c: c["name"])
+def test_cd_golden_case_gpu(ctx, case):
+    T, ty, tw = cd_cases.case_scene(case)
+    g = ctx.scene(T, ty, tw)
+    t, n, u = g.cast_rays(np.array([case["origin"]], np.float32), np.array([case["dir"]], np.float32),
+                          case["max_dist"])
+    cd_cases.check_case(case, float(t[0]), n[0].astype(np.float64), int(u[0]))
+
+
+def test_cd_golden_grid_gpu(ctx):
+    T, ty, tw, O, D, expect = cd_cases.grid_scene()
+    t, n, u = ctx.scene(T, ty, tw).cast_rays(O, D)
+    assert np.array_equal(u != 0xFFFFFFFF, expect)
+
+
+def test_simple_direct_gpu(ctx):
+    T = cf.patch_T((0, 0, 0), (1, 0, 0), (0, 1, 0))[None]
+    g = ctx.scene(T, [0], [0])
+    R = native.run_band(g, req1(1, "SW", [[0.0]], [[0.0]], d=10000, sun=[0, 0, 1], F=1000.0))
+    assert abs(R[0, 0] - 1000.0) < 1e-2
+
+
+def test_black_parallel_rectangles_gpu(ctx):
+    T, ty, tw, F12 = cf.parallel_rectangles()
+    g = ctx.scene(T, ty, tw)
+    R = native.run_band(g, req1(2, "SW", np.zeros((2, 1)), np.zeros((2, 1)), n=40000, dflux=1.0))
+    assert np.max(np.abs(R[:, 0] - (1.0 - F12))) / (1.0 - F12) <= 0.005
+
+
+def test_second_law_enclosure_gpu(ctx):
+    T, ty, tw = cf.closed_box()
+    P = T.shape[0]
+    g = ctx.scene(T, ty, tw)
+    R = native.run_band(g, req1(P, "LW", np.full((P, 1), 0.2), np.zeros((P, 1)), n=10000, depth=5, emission=True,
+                                eps=np.full((P, 1), 0.8), temp=np.full(P, 300.0)))
+    ref = 0.8 * cf.SIGMA * 300.0 ** 4
+    assert np.mean(np.abs(R[:, 0] - ref) / ref) <= 0.005
+
+
+def test_beer_law_periodic_canopy_gpu(ctx):
+    s, D, _ = _beer_canopy()
+    g = ctx.scene(s.transforms, s.types, s.twosided, periodic=s.periodic)
+    o = OracleScene.from_arrays(s)
+    A = o.areas()
+    leaf = s.twosided == 1
+    dom = g.info()["domain"]
+    cell = (dom[1] - dom[0]) * (dom[3] - dom[2])
+    lai = A[leaf].sum() / cell
+    th = 0.2 * math.pi
+    P = s.P
+    Rd = native.run_band(g, req1(P, "direct", np.zeros((P, 1)), np.zeros((P, 1)), d=100,
+                                 sun=native.sun_direction(math.degrees(th), 0.0), F=1.0 / math.cos(th)))
+    Rf = native.run_band(g, req1(P, "diffuse", np.zeros((P, 1)), np.zeros((P, 1)), n=400, dflux=1.0))
+    for R, exact in ((Rd, cf.beer_direct(lai, th)), (Rf, cf.beer_diffuse(lai))):
+        leaf_int = float(np.sum(R[leaf, 0] * A[leaf])) / cell
+        ground_int = 1.0 - float(np.sum(R[~leaf, 0] * A[~leaf])) / float(np.sum(A[~leaf]))
+        assert abs(leaf_int - exact) <= 0.01, (leaf_int, exact)
+        assert abs(ground_int - exact) <= 0.01, (ground_int, exact)
+
+
+@pytest.mark.parametrize("schedule", [0, 1])
+def test_schedule_parity_config2_small(ctx, schedule):
+    s = scenes.config2(5, 20000)
+    g = ctx.scene(s.transforms, s.types, s.twosided, periodic=s.periodic)
+    req = W.config2_request(s, rays=64, depth=3)
+    req.schedule = schedule
+    Rg = native.run_band(g, req)
+    Ro = OracleScene.from_arrays(s).run_band(req)
+    assert rel_err(Rg, Ro, floor=1.0).max() <= 1e-4
+
+
+def test_batch_equals_plain_many_channels(ctx):
+    # 20 suns x 3 bands = 60 channels -> the lanes-over-channels gather (the diurnal bench path)
+    s = scenes.orchard(7, nx=2, ny=2, leaves_per_tree=3000, tube_tris=320)
+    g = ctx.scene(s.transforms, s.types, s.twosided)
+    base = W.diurnal_request(s, rays=36, depth=2)
+    dirs, flux, sky, seeds = W.diurnal_suns(20, seed=9)
+    Rb = native.run_band_batch(g, base, dirs, flux, sky, seeds)
+    worst = 0.0
+    for k in (0, 7, 19):
+        Rp = native.run_band(g, W.single_sun_request(base, dirs[k], flux[k], sky[k], seeds[k]))
+        worst = max(worst, float(rel_err(Rb[k], Rp, floor=1.0).max()))
+    assert worst <= 1e-5, worst
+
+
+def test_diurnal_sun_against_oracle(ctx):
+    # one sun of the batched diurnal request (emission band, conserving path) vs the FP64 oracle
+    s = scenes.orchard(8, nx=1, ny=2, leaves_per_tree=2500, tube_tris=200)
+    g = ctx.scene(s.transforms, s.types, s.twosided)
+    base = W.diurnal_request(s, rays=25, depth=2)
+    dirs, flux, sky, seeds = W.diurnal_suns(6, seed=4)
+    Rb = native.run_band_batch(g, base, dirs, flux, sky, seeds)
+    Ro = OracleScene.from_arrays(s).run_band(W.single_sun_request(base, dirs[3], flux[3], sky[3], seeds[3]))
+    assert rel_err(Rb[3], Ro, floor=1.0).max() <= 1e-4
+
+
+def test_repeatable(ctx):
+    s = scenes.config1(4)
+    g = ctx.scene(s.transforms, s.types, s.twosided)
+    req = W.config1_request(s)
+    a = native.run_band(g, req)
+    b = native.run_band(g, req)
+    assert np.array_equal(a, b)
