@@ -52,6 +52,20 @@ def pct(a, q):
     return float(np.percentile(np.asarray(a, np.float64), q)) if len(a) else None
 
 
+def pmc_traffic(kernel, profile="pmc_traffic_config3_diurnal.json"):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 PMC summary (FETCH_SIZE +
+    WRITE_SIZE of the batched dispatch, calibrated against a copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", profile)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    for name, cs in d.get("kernels", {}).items():
+        if kernel in name and "FETCH_SIZE" in cs and "WRITE_SIZE" in cs:
+            return cs["FETCH_SIZE"]["max_bytes"] + cs["WRITE_SIZE"]["max_bytes"]
+    return None
+
+
 def bytes_per_ray(c):
     if not c["rays"]:
         return None
@@ -109,8 +123,23 @@ def cpu_baseline_diurnal(s, base, dirs, seconds):
             "mrays_per_s": (rdir + rdif) / (tdif + tdir) / 1e6}
 
 
+def calib_copy(mib):
+    """A plain device-to-device copy of known size, so a PMC run can scale FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"[bench] calibration copy: {n * 4} bytes read, {n * 4} bytes written")
+    del src, dst
+
+
 def bench_diurnal(args):
     K = args.suns
+    if args.calib_copy_mib:
+        calib_copy(args.calib_copy_mib)
     log(f"[bench] generating config-3 orchard ...")
     s = scenes.orchard(3)
     ctx = native.Context(0)
@@ -202,7 +231,8 @@ def bench_diurnal(args):
         "rays_per_solve": rp["total"],
         "scene_build_ms": round(build_s * 1e3, 1),
         "roofline": {"bound": "hbm", "kernel": f"k_{dom}", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(f"k_{dom}"),
+                     "algorithmic_bytes_per_launch": round(brays * rays_per_launch),
                      "bytes_per_ray": round(brays, 1),
                      "nodes_per_ray": round(counts[dom]["nodes"] / counts[dom]["rays"], 2),
                      "prims_per_ray": round(counts[dom]["prims"] / counts[dom]["rays"], 2),
@@ -314,6 +344,8 @@ def main():
     ap.add_argument("--rays", type=int, default=100_000_000)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="run a plain copy of this many MiB first (PMC byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is described in "

@@ -1,0 +1,97 @@
+"""Summarise rocprofv3 SQLite results (``*_results.db``) into short text: per-kernel timing and,
+for PMC runs, per-dispatch counter values of the ray kernels and of the byte-counter calibration copy.
+
+usage: python -m helios_amd.rocprof_summary KERNEL_TRACE_DB [PMC_DB ...] [--json OUT.json]
+
+With --json, also writes HBM bytes per dispatch of each ray kernel: FETCH_SIZE and WRITE_SIZE
+(KiB) scaled by what the same counters read for the calibration copy of known size.
+"""
+from __future__ import annotations
+
+import collections
+import json
+import re
+import sqlite3
+import sys
+
+RAY_KERNELS = ("k_direct", "k_diffuse", "k_gather", "k_cast")
+
+
+def short(name: str) -> str:
+    return re.sub(r"\(.*", "", name)[:90]
+
+
+def kernel_rows(db):
+    cur = sqlite3.connect(db).cursor()
+    return list(cur.execute("select name, start, end, grid_x, workgroup_x, vgpr_count, sgpr_count, lds_size "
+                            "from kernels order by start"))
+
+
+def counter_rows(db):
+    cur = sqlite3.connect(db).cursor()
+    return list(cur.execute("select dispatch_id, kernel_name, counter_name, value from counters_collection "
+                            "order by dispatch_id"))
+
+
+def summarize(kt_db, pmc_dbs, calib_bytes=1 << 30):
+    out = []
+    rows = kernel_rows(kt_db)
+    agg = collections.OrderedDict()
+    res = {}
+    for name, s, e, g, w, v, sg, lds in rows:
+        agg.setdefault(short(name), []).append((e - s) / 1e6)
+        if any(t in name for t in RAY_KERNELS):
+            res[short(name)] = (g, w, v, sg, lds)
+    out.append("== kernel trace (durations in ms)")
+    out.append(f"{'kernel':90s} {'calls':>6s} {'total_ms':>12s} {'avg_ms':>12s} {'min_ms':>10s} {'max_ms':>10s}")
+    for k, v in sorted(agg.items(), key=lambda kv: -sum(kv[1])):
+        out.append(f"{k:90s} {len(v):6d} {sum(v):12.3f} {sum(v) / len(v):12.3f} {min(v):10.3f} {max(v):10.3f}")
+    out.append("\n== per-dispatch durations of the ray kernels (ms, launch order)")
+    for k, v in agg.items():
+        if any(t in k for t in RAY_KERNELS):
+            out.append(f"{k}: " + ", ".join(f"{x:.3f}" for x in v))
+    for k, (g, w, v, sg, lds) in res.items():
+        out.append(f"resources {k}: grid {g} workgroup {w} vgpr {v} sgpr {sg} lds {lds}")
+    traffic = collections.defaultdict(lambda: collections.defaultdict(list))
+    scale = {}
+    for db in pmc_dbs:
+        rows = counter_rows(db)
+        out.append(f"\n== {db.split('/')[-1]}: counter value per dispatch (KiB)")
+        for d, n, c, val in rows:
+            sn = short(n)
+            if "copyBuffer" in sn and val >= calib_bytes / 1024 / 4:
+                scale[c] = calib_bytes / 1024.0 / val
+                out.append(f"dispatch {d:6d} {sn:60s} {c:12s} {val:16.1f}  <- calibration copy of {calib_bytes} B")
+            elif any(t in sn for t in RAY_KERNELS):
+                traffic[sn][c].append(val)
+                out.append(f"dispatch {d:6d} {sn:60s} {c:12s} {val:16.1f}")
+    summary = {}
+    if scale:
+        out.append("\n== HBM bytes per dispatch (counter KiB x 1024 x calibration factor)")
+        out.append("calibration factors: " + ", ".join(f"{c} x{f:.4f}" for c, f in scale.items()))
+        for k, cs in traffic.items():
+            tot = {}
+            for c, vals in cs.items():
+                f = scale.get(c, 1.0)
+                tot[c] = [v * 1024.0 * f for v in vals]
+            out.append(f"{k}: " + "; ".join(f"{c} max {max(v) / 1e9:.2f} GB min {min(v) / 1e9:.2f} GB"
+                                            for c, v in tot.items()))
+            summary[k] = {c: {"max_bytes": max(v), "min_bytes": min(v)} for c, v in tot.items()}
+    return "\n".join(out), summary, scale
+
+
+def main(argv):
+    js = None
+    if "--json" in argv:
+        i = argv.index("--json")
+        js = argv[i + 1]
+        argv = argv[:i] + argv[i + 2:]
+    text, summary, scale = summarize(argv[0], argv[1:])
+    print(text)
+    if js:
+        with open(js, "w") as f:
+            json.dump({"calibration_factors": scale, "kernels": summary}, f, indent=1)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])
