@@ -7,7 +7,7 @@
 #include <string>
 #include <vector>
 
-#include "rt_solve.h"
+#include "rt_ctx.h"
 
 namespace rt {
 static thread_local std::string g_err;
@@ -16,33 +16,7 @@ void set_error(const std::string &msg) { g_err = msg; }
 
 using namespace rt;
 
-struct rt_ctx {
-    int device = 0;
-    std::string name;
-    SolveCtx sx;
-    std::mutex mu;
-};
-
-struct rt_scene {
-    Scene s;
-};
-
-#define CKR(x)                                                                    \
-    do {                                                                          \
-        hipError_t e_ = (x);                                                      \
-        if (e_ != hipSuccess) {                                                   \
-            set_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " #x); \
-            return -1;                                                            \
-        }                                                                         \
-    } while (0)
-
-#define NN(p, what)                                        \
-    do {                                                   \
-        if (!(p)) {                                        \
-            set_error(std::string(what) + " is NULL");     \
-            return -1;                                     \
-        }                                                  \
-    } while (0)
+void backend_release(BackendState &be);
 
 extern "C" {
 
@@ -84,6 +58,7 @@ int rt_destroy(rt_ctx *ctx) {
     (void)hipStreamSynchronize(ctx->sx.stream);
     for (auto &e : ctx->sx.ev) { if (e.a) (void)hipEventDestroy(e.a); if (e.b) (void)hipEventDestroy(e.b); }
     ctx->sx.ws.release();
+    backend_release(ctx->be);
     if (ctx->sx.stats) (void)hipFree(ctx->sx.stats);
     (void)hipStreamDestroy(ctx->sx.stream);
     delete ctx;

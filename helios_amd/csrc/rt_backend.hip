@@ -1,0 +1,490 @@
+// rt_backend.hip -- call-by-call backend entry points: the helios::RayTracingBackend interface
+// (R/include/RayTracingBackend.h:43-312) as C functions over one context-owned resident scene.
+//
+// A RadiationModel that keeps its own pass loop (R/src/RadiationModel.cpp:3804-4356) drives these
+// in the order it drives VulkanComputeBackend.  Every buffer stays in HBM between calls; only the
+// explicit upload/download calls move data over PCIe.  Launch calls run the same persistent
+// traversal kernels as rt_run_band: a direct launch is one direct trace job per source followed by
+// the deposit kernel, a diffuse launch is one trace pass of the requested face followed by the
+// gather kernel.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rt_ctx.h"
+
+using namespace rt;
+
+namespace {
+
+__global__ void k_select_bands(uint32_t P, uint32_t Bg, uint32_t Bl, const uint32_t *__restrict__ map,
+                               const float *__restrict__ rho, const float *__restrict__ tau, float *__restrict__ rl,
+                               float *__restrict__ tl) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (size_t)P * Bl) return;
+    uint32_t i = (uint32_t)(q / Bl), b = (uint32_t)(q % Bl);
+    size_t g = (size_t)i * Bg + map[b];
+    rl[q] = rho[g];
+    tl[q] = tau[g];
+}
+
+inline uint32_t nblk(size_t n) { return (uint32_t)((n + 255) / 256); }
+
+struct Carve {
+    char *p;
+    size_t off = 0;
+    template <typename T>
+    T *take(size_t n) {
+        off = (off + 255) & ~(size_t)255;
+        T *r = reinterpret_cast<T *>(p + off);
+        off += n * sizeof(T);
+        return r;
+    }
+};
+
+int grow(void *&p, size_t &cap, size_t bytes) {
+    if (bytes <= cap) return 0;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    CKR(hipMalloc(&p, bytes));
+    cap = bytes;
+    return 0;
+}
+
+int check_err(rt_ctx *ctx, int *err) {
+    int herr = 0;
+    CKR(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, ctx->sx.stream));
+    CKR(hipStreamSynchronize(ctx->sx.stream));
+    if (herr) {
+        set_error("traversal error code " + std::to_string(herr) + " (1 = stack overflow, 2 = malformed tree)");
+        return -1;
+    }
+    return 0;
+}
+
+int need_scene(const BackendState &be, const char *who) {
+    if (!be.has_scene) {
+        set_error(std::string(who) + ": no acceleration structure (call rt_update_geometry and "
+                                     "rt_build_acceleration_structure first)");
+        return -1;
+    }
+    return 0;
+}
+
+int need_launch_buffers(const BackendState &be, const char *who) {
+    if (need_scene(be, who)) return -1;
+    if (be.Bl == 0 || !be.rad) {
+        set_error(std::string(who) + ": radiation buffers not sized (call rt_zero_radiation_buffers first)");
+        return -1;
+    }
+    return 0;
+}
+
+// Validates a launch's band table and writes the launch -> global band map and the active flags.
+int launch_bands(const BackendState &be, const rt_launch_params *p, std::vector<uint32_t> &map,
+                 std::vector<uint8_t> &active, const char *who) {
+    if (p->num_bands_launch != be.Bl) {
+        set_error(std::string(who) + ": num_bands_launch " + std::to_string(p->num_bands_launch) +
+                  " differs from the launch band count " + std::to_string(be.Bl) + " set by rt_zero_radiation_buffers");
+        return -1;
+    }
+    if (!be.has_materials) { set_error(std::string(who) + ": no materials (call rt_update_materials)"); return -1; }
+    if (p->num_bands_global != be.Bg) {
+        set_error(std::string(who) + ": num_bands_global " + std::to_string(p->num_bands_global) +
+                  " differs from the materials' band count " + std::to_string(be.Bg));
+        return -1;
+    }
+    map.assign(be.Bl, 0);
+    active.assign(be.Bl, 1);
+    for (uint32_t b = 0; b < be.Bl; b++) {
+        map[b] = p->band_map ? p->band_map[b] : b;
+        if (map[b] >= be.Bg) {
+            set_error(std::string(who) + ": band_map[" + std::to_string(b) + "] = " + std::to_string(map[b]) +
+                      " is not a global band");
+            return -1;
+        }
+        if (p->band_active) active[b] = p->band_active[b] ? 1 : 0;
+    }
+    if (p->launch_offset > be.P || p->launch_count > be.P - p->launch_offset) {
+        set_error(std::string(who) + ": primitive range [" + std::to_string(p->launch_offset) + ", +" +
+                  std::to_string(p->launch_count) + ") exceeds the " + std::to_string(be.P) + " primitives");
+        return -1;
+    }
+    return 0;
+}
+
+// Work items of a launch are 0..launch_count-1 and a kernel's primitive is item - item_begin
+// (uint32 arithmetic).  With one job, item_begin = -launch_offset maps item k to primitive
+// launch_offset + k, so a launch of a primitive sub-range seeds, traces and writes exactly the
+// primitives (and the per-primitive rows of the output buffers) that a full launch would.
+inline uint32_t range_begin(uint32_t offset) { return 0u - offset; }
+
+}  // namespace
+
+void backend_release(BackendState &be) {
+    if (be.has_scene) scene_free(be.scene);
+    be.has_scene = false;
+    if (be.mat) (void)hipFree(be.mat);
+    if (be.flux) (void)hipFree(be.flux);
+    if (be.rad) (void)hipFree(be.rad);
+    be.mat = be.flux = be.rad = nullptr;
+    be.mat_cap = be.flux_cap = be.rad_cap = 0;
+    be.ws.release();
+}
+
+extern "C" {
+
+int rt_update_geometry(rt_ctx *ctx, uint32_t nprims, const float *transforms, const uint32_t *types,
+                       const uint32_t *uuids, const uint32_t *twosided, int periodic_x, int periodic_y) {
+    NN(ctx, "ctx");
+    if (nprims) { NN(transforms, "transforms"); NN(types, "types"); NN(twosided, "twosided"); }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    BackendState &be = ctx->be;
+    be.gP = nprims;
+    be.gT.assign(transforms, transforms + (size_t)nprims * 16);
+    be.gtype.assign(types, types + nprims);
+    be.gtwos.assign(twosided, twosided + nprims);
+    if (uuids) be.guuid.assign(uuids, uuids + nprims);
+    else {
+        be.guuid.resize(nprims);
+        for (uint32_t i = 0; i < nprims; i++) be.guuid[i] = i;
+    }
+    be.gpx = periodic_x ? 1 : 0;
+    be.gpy = periodic_y ? 1 : 0;
+    be.geometry_staged = true;
+    return 0;
+}
+
+int rt_build_acceleration_structure(rt_ctx *ctx) {
+    NN(ctx, "ctx");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    BackendState &be = ctx->be;
+    if (!be.geometry_staged) { set_error("rt_build_acceleration_structure: no geometry (call rt_update_geometry)"); return -1; }
+    CKR(hipSetDevice(ctx->device));
+    CKR(hipStreamSynchronize(ctx->sx.stream));
+    if (be.has_scene) { scene_free(be.scene); be.has_scene = false; }
+    be.scene = Scene{};
+    if (scene_build(be.scene, be.gP, be.gT.data(), be.gtype.data(), be.guuid.data(), be.gtwos.data(), be.gpx, be.gpy,
+                    ctx->sx.stream))
+        return -1;
+    be.has_scene = true;
+    if (be.P != be.gP) {
+        // per-primitive state of the previous geometry no longer applies
+        be.has_materials = false;
+        be.has_fluxes = false;
+        be.Bl = 0;
+    }
+    be.P = be.gP;
+    return 0;
+}
+
+int rt_update_materials(rt_ctx *ctx, uint32_t nsources, uint32_t nprims, uint32_t nbands_global, const float *rho,
+                        const float *tau) {
+    NN(ctx, "ctx");
+    NN(rho, "rho");
+    NN(tau, "tau");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    BackendState &be = ctx->be;
+    if (need_scene(be, "rt_update_materials")) return -1;
+    if (nprims != be.P) {
+        set_error("rt_update_materials: " + std::to_string(nprims) + " primitives, the scene has " + std::to_string(be.P));
+        return -1;
+    }
+    if (nsources == 0 || nbands_global == 0) { set_error("rt_update_materials: no sources or no bands"); return -1; }
+    CKR(hipSetDevice(ctx->device));
+    CKR(hipStreamSynchronize(ctx->sx.stream));
+    size_t PB = (size_t)be.P * nbands_global;
+    if (grow(be.mat, be.mat_cap, 2 * PB * sizeof(float) + 256)) return -1;
+    be.Bg = nbands_global;
+    // source-0 slice, as the Vulkan backend's shaders index rho/tau (R/shaders/direct_raygen.comp:684-687)
+    CKR(hipMemcpy(be.rho(), rho, PB * sizeof(float), hipMemcpyHostToDevice));
+    CKR(hipMemcpy(be.tau(), tau, PB * sizeof(float), hipMemcpyHostToDevice));
+    be.has_materials = true;
+    return 0;
+}
+
+int rt_update_sources(rt_ctx *ctx, uint32_t nsources, const uint32_t *types, const float *positions) {
+    NN(ctx, "ctx");
+    if (nsources) { NN(types, "types"); NN(positions, "positions"); }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    BackendState &be = ctx->be;
+    for (uint32_t s = 0; s < nsources; s++)
+        if (types[s] != 0) {
+            set_error("rt_update_sources: source " + std::to_string(s) + " has type " + std::to_string(types[s]) +
+                      "; this backend traces collimated sources (type 0) only");
+            return -1;
+        }
+    if (nsources != be.S) be.has_fluxes = false;
+    be.S = nsources;
+    be.src_dir.assign(positions, positions + (size_t)nsources * 3);
+    return 0;
+}
+
+int rt_zero_radiation_buffers(rt_ctx *ctx, uint32_t launch_band_count) {
+    NN(ctx, "ctx");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    BackendState &be = ctx->be;
+    if (need_scene(be, "rt_zero_radiation_buffers")) return -1;
+    if (launch_band_count == 0 || launch_band_count > 256) {
+        set_error("rt_zero_radiation_buffers: launch_band_count must be 1..256");
+        return -1;
+    }
+    CKR(hipSetDevice(ctx->device));
+    CKR(hipStreamSynchronize(ctx->sx.stream));
+    if (launch_band_count != be.Bl) be.has_fluxes = false;
+    be.Bl = launch_band_count;
+    size_t PB = (size_t)be.P * be.Bl;
+    if (grow(be.rad, be.rad_cap, 5 * PB * sizeof(float) + 256)) return -1;
+    // radiation_in and radiation_out; the scatter buffers are zeroed as well so that a new runBand
+    // starts from a clean state (VulkanComputeBackend.cpp:1935-1988 re-creates them when resized)
+    CKR(hipMemsetAsync(be.rad, 0, 5 * PB * sizeof(float), ctx->sx.stream));
+    CKR(hipStreamSynchronize(ctx->sx.stream));
+    return 0;
+}
+
+int rt_zero_scatter_buffers(rt_ctx *ctx) {
+    NN(ctx, "ctx");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    BackendState &be = ctx->be;
+    if (need_launch_buffers(be, "rt_zero_scatter_buffers")) return -1;
+    CKR(hipSetDevice(ctx->device));
+    size_t PB = (size_t)be.P * be.Bl;
+    CKR(hipMemsetAsync(be.rbuf(1), 0, 2 * PB * sizeof(float), ctx->sx.stream));
+    CKR(hipStreamSynchronize(ctx->sx.stream));
+    return 0;
+}
+
+int rt_copy_scatter_to_radiation(rt_ctx *ctx) {
+    NN(ctx, "ctx");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    BackendState &be = ctx->be;
+    if (need_launch_buffers(be, "rt_copy_scatter_to_radiation")) return -1;
+    CKR(hipSetDevice(ctx->device));
+    size_t PB = (size_t)be.P * be.Bl;
+    CKR(hipMemcpyAsync(be.rbuf(3), be.rbuf(1), PB * sizeof(float), hipMemcpyDeviceToDevice, ctx->sx.stream));
+    CKR(hipMemcpyAsync(be.rbuf(4), be.rbuf(2), PB * sizeof(float), hipMemcpyDeviceToDevice, ctx->sx.stream));
+    CKR(hipStreamSynchronize(ctx->sx.stream));
+    return 0;
+}
+
+int rt_upload_radiation_out(rt_ctx *ctx, const float *top, const float *bottom, size_t count) {
+    NN(ctx, "ctx");
+    NN(top, "top");
+    NN(bottom, "bottom");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    BackendState &be = ctx->be;
+    if (need_launch_buffers(be, "rt_upload_radiation_out")) return -1;
+    size_t PB = (size_t)be.P * be.Bl;
+    if (count != PB) {
+        set_error("rt_upload_radiation_out: " + std::to_string(count) + " values, expected P*B_launch = " + std::to_string(PB));
+        return -1;
+    }
+    CKR(hipSetDevice(ctx->device));
+    CKR(hipStreamSynchronize(ctx->sx.stream));
+    CKR(hipMemcpy(be.rbuf(3), top, PB * sizeof(float), hipMemcpyHostToDevice));
+    CKR(hipMemcpy(be.rbuf(4), bottom, PB * sizeof(float), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int rt_upload_source_fluxes(rt_ctx *ctx, const float *fluxes, size_t count) {
+    NN(ctx, "ctx");
+    NN(fluxes, "fluxes");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    BackendState &be = ctx->be;
+    if (need_launch_buffers(be, "rt_upload_source_fluxes")) return -1;
+    if (count != (size_t)be.S * be.Bl) {
+        set_error("rt_upload_source_fluxes: " + std::to_string(count) + " values, expected S*B_launch = " +
+                  std::to_string((size_t)be.S * be.Bl));
+        return -1;
+    }
+    CKR(hipSetDevice(ctx->device));
+    CKR(hipStreamSynchronize(ctx->sx.stream));
+    if (grow(be.flux, be.flux_cap, count * sizeof(float) + 256)) return -1;
+    if (count) CKR(hipMemcpy(be.flux, fluxes, count * sizeof(float), hipMemcpyHostToDevice));
+    be.has_fluxes = true;
+    return 0;
+}
+
+int rt_launch_direct_rays(rt_ctx *ctx, const rt_launch_params *p) {
+    NN(ctx, "ctx");
+    NN(p, "params");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    BackendState &be = ctx->be;
+    if (need_launch_buffers(be, "rt_launch_direct_rays")) return -1;
+    std::vector<uint32_t> map;
+    std::vector<uint8_t> active;
+    if (launch_bands(be, p, map, active, "rt_launch_direct_rays")) return -1;
+    if (be.S == 0 || p->launch_count == 0 || p->rays_per_primitive == 0) return 0;
+    if (!be.has_fluxes) { set_error("rt_launch_direct_rays: no source fluxes (call rt_upload_source_fluxes)"); return -1; }
+    CKR(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->sx.stream;
+    const uint32_t P = be.P, Bl = be.Bl, S = be.S;
+    const uint32_t d = (uint32_t)std::ceil(std::sqrt((double)p->rays_per_primitive));
+    size_t bytes = (size_t)S * P * 2 * 4 + 2 * (size_t)P * Bl * 4 + Bl * 8 + S * sizeof(DirectJob) + 16 * 256;
+    if (be.ws.reserve(bytes)) return -1;
+    Carve cv{(char *)be.ws.base};
+    float *dsum = cv.take<float>((size_t)S * P * 2);
+    float *rl = cv.take<float>((size_t)P * Bl), *tl = cv.take<float>((size_t)P * Bl);
+    uint32_t *dmap = cv.take<uint32_t>(Bl);
+    uint8_t *dact = cv.take<uint8_t>(Bl);
+    DirectJob *jobs = cv.take<DirectJob>(S);
+    uint32_t *counter = cv.take<uint32_t>(16);
+    int *err = reinterpret_cast<int *>(counter + 8);
+    std::vector<DirectJob> hj(S);
+    for (uint32_t s = 0; s < S; s++) {
+        DirectJob &j = hj[s];
+        j.sc = be.scene.d;
+        j.sdir[0] = be.src_dir[s * 3]; j.sdir[1] = be.src_dir[s * 3 + 1]; j.sdir[2] = be.src_dir[s * 3 + 2];
+        j.d = d;
+        j.seed = p->random_seed;
+        j.out = dsum + (size_t)s * P * 2;
+        j.item_begin = s * p->launch_count + range_begin(p->launch_offset);
+        j.P = P;
+    }
+    CKR(hipMemsetAsync(dsum, 0, (size_t)S * P * 2 * 4, st));
+    CKR(hipMemsetAsync(err, 0, sizeof(int), st));
+    CKR(hipMemcpyAsync(dmap, map.data(), Bl * 4, hipMemcpyHostToDevice, st));
+    CKR(hipMemcpyAsync(dact, active.data(), Bl, hipMemcpyHostToDevice, st));
+    CKR(hipMemcpyAsync(jobs, hj.data(), S * sizeof(DirectJob), hipMemcpyHostToDevice, st));
+    CKR(hipStreamSynchronize(st));
+    k_select_bands<<<nblk((size_t)P * Bl), 256, 0, st>>>(P, be.Bg, Bl, dmap, be.rho(), be.tau(), rl, tl);
+    CKR(hipGetLastError());
+    // Sources are jobs over items [s*count, (s+1)*count).  find_job needs ascending item_begin, which
+    // the modular offset of a sub-range breaks for S > 1: such launches go one source at a time.
+    if (p->launch_offset == 0) {
+        if (launch_direct(jobs, S, S * p->launch_count, be.scene.h.depth, counter, err, ctx->sx.stat_ptr(0),
+                          ctx->sx.cfg[0], ctx->sx.ncu, st))
+            return -1;
+    } else {
+        for (uint32_t s = 0; s < S; s++) {
+            DirectJob j = hj[s];
+            j.item_begin = range_begin(p->launch_offset);
+            CKR(hipMemcpyAsync(jobs + s, &j, sizeof(DirectJob), hipMemcpyHostToDevice, st));
+            CKR(hipStreamSynchronize(st));
+            if (launch_direct(jobs + s, 1, p->launch_count, be.scene.h.depth, counter, err, ctx->sx.stat_ptr(0),
+                              ctx->sx.cfg[0], ctx->sx.ncu, st))
+                return -1;
+        }
+    }
+    if (launch_deposit_direct(P, Bl, Bl, S, dsum, (const float *)be.flux, rl, tl, dact, be.rbuf(0), be.rbuf(1),
+                              be.rbuf(2), st))
+        return -1;
+    return check_err(ctx, err);
+}
+
+int rt_launch_diffuse_rays(rt_ctx *ctx, const rt_launch_params *p) {
+    NN(ctx, "ctx");
+    NN(p, "params");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    BackendState &be = ctx->be;
+    if (need_launch_buffers(be, "rt_launch_diffuse_rays")) return -1;
+    std::vector<uint32_t> map;
+    std::vector<uint8_t> active;
+    if (launch_bands(be, p, map, active, "rt_launch_diffuse_rays")) return -1;
+    if (p->launch_face > 1) { set_error("rt_launch_diffuse_rays: launch_face must be 1 (top) or 0 (bottom)"); return -1; }
+    if ((p->radiation_out_top == nullptr) != (p->radiation_out_bottom == nullptr)) {
+        set_error("rt_launch_diffuse_rays: radiation_out_top and radiation_out_bottom must both be given or both NULL");
+        return -1;
+    }
+    CKR(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->sx.stream;
+    const uint32_t P = be.P, Bl = be.Bl;
+    size_t PB = (size_t)P * Bl;
+    if (p->radiation_out_top) {
+        // launch parameters carry radiation_out (VulkanComputeBackend.cpp:1131-1134)
+        CKR(hipStreamSynchronize(st));
+        CKR(hipMemcpy(be.rbuf(3), p->radiation_out_top, PB * sizeof(float), hipMemcpyHostToDevice));
+        CKR(hipMemcpy(be.rbuf(4), p->radiation_out_bottom, PB * sizeof(float), hipMemcpyHostToDevice));
+    }
+    if (p->launch_count == 0 || p->rays_per_primitive == 0) return 0;
+    const uint32_t n = (uint32_t)std::ceil(std::sqrt((double)p->rays_per_primitive));
+    const size_t nn = (size_t)n * n;
+    size_t bytes = (size_t)P * 2 * nn * 4 + 2 * PB * 4 + Bl * 9 + sizeof(DiffuseJob) + sizeof(GatherJob) + 16 * 256;
+    if (be.ws.reserve(bytes)) return -1;
+    Carve cv{(char *)be.ws.base};
+    uint32_t *hits = cv.take<uint32_t>((size_t)P * 2 * nn);
+    float *rl = cv.take<float>(PB), *tl = cv.take<float>(PB);
+    uint32_t *dmap = cv.take<uint32_t>(Bl);
+    float *dfl = cv.take<float>(Bl);
+    uint8_t *dact = cv.take<uint8_t>(Bl);
+    DiffuseJob *fjob = cv.take<DiffuseJob>(1);
+    GatherJob *gjob = cv.take<GatherJob>(1);
+    uint32_t *counter = cv.take<uint32_t>(16);
+    int *err = reinterpret_cast<int *>(counter + 8);
+    std::vector<float> hf(Bl, 0.f);
+    if (p->diffuse_flux)
+        for (uint32_t b = 0; b < Bl; b++) hf[b] = p->diffuse_flux[b];
+    DiffuseJob fj{};
+    fj.sc = be.scene.d;
+    fj.n = n;
+    fj.seed = p->random_seed;
+    fj.hits = hits;
+    fj.item_begin = range_begin(p->launch_offset);
+    fj.P = P;
+    fj.faces = p->launch_face == 1 ? 1u : 2u;  // hit-table slot 0 = top launch, slot 1 = bottom launch
+    GatherJob gj{};
+    gj.sc = be.scene.d;
+    gj.hits = hits;
+    gj.n = n;
+    gj.C = Bl;
+    gj.B = Bl;
+    gj.out_top = be.rbuf(3);
+    gj.out_bot = be.rbuf(4);
+    gj.dflux = dfl;
+    gj.active = dact;
+    gj.rho = rl;
+    gj.tau = tl;
+    gj.in = be.rbuf(0);
+    gj.st = be.rbuf(1);
+    gj.sb = be.rbuf(2);
+    gj.item_begin = range_begin(p->launch_offset);
+    gj.P = P;
+    CKR(hipMemsetAsync(err, 0, sizeof(int), st));
+    CKR(hipMemcpyAsync(dmap, map.data(), Bl * 4, hipMemcpyHostToDevice, st));
+    CKR(hipMemcpyAsync(dfl, hf.data(), Bl * 4, hipMemcpyHostToDevice, st));
+    CKR(hipMemcpyAsync(dact, active.data(), Bl, hipMemcpyHostToDevice, st));
+    CKR(hipMemcpyAsync(fjob, &fj, sizeof(fj), hipMemcpyHostToDevice, st));
+    CKR(hipMemcpyAsync(gjob, &gj, sizeof(gj), hipMemcpyHostToDevice, st));
+    CKR(hipStreamSynchronize(st));
+    k_select_bands<<<nblk(PB), 256, 0, st>>>(P, be.Bg, Bl, dmap, be.rho(), be.tau(), rl, tl);
+    CKR(hipGetLastError());
+    if (launch_diffuse(fjob, 1, p->launch_count, be.scene.h.depth, counter, err, ctx->sx.stat_ptr(1), ctx->sx.cfg[1],
+                       ctx->sx.ncu, st))
+        return -1;
+    if (launch_gather(gjob, 1, p->launch_count, Bl, counter, ctx->sx.cfg[2], ctx->sx.ncu, st)) return -1;
+    return check_err(ctx, err);
+}
+
+int rt_get_radiation_results(rt_ctx *ctx, float *radiation_in, float *radiation_out_top, float *radiation_out_bottom,
+                             float *scatter_top, float *scatter_bottom) {
+    NN(ctx, "ctx");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    BackendState &be = ctx->be;
+    if (need_launch_buffers(be, "rt_get_radiation_results")) return -1;
+    CKR(hipSetDevice(ctx->device));
+    CKR(hipStreamSynchronize(ctx->sx.stream));
+    size_t bytes = (size_t)be.P * be.Bl * sizeof(float);
+    float *dst[5] = {radiation_in, scatter_top, scatter_bottom, radiation_out_top, radiation_out_bottom};
+    for (int k = 0; k < 5; k++)
+        if (dst[k]) CKR(hipMemcpy(dst[k], be.rbuf(k), bytes, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int rt_get_backend_info(rt_ctx *ctx, uint32_t *out) {
+    NN(ctx, "ctx");
+    NN(out, "out");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const BackendState &be = ctx->be;
+    out[0] = be.has_scene ? be.P : 0;
+    out[1] = be.Bg;
+    out[2] = be.Bl;
+    out[3] = be.S;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,69 @@
+// rt_ctx.h -- definitions of the opaque C-ABI handles (internal).
+#pragma once
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "rt_solve.h"
+
+// State of the call-by-call backend interface (the helios::RayTracingBackend counterpart,
+// R/include/RayTracingBackend.h:43-312): one resident scene, materials, sources and the five
+// [P][B_launch] radiation buffers that VulkanComputeBackend keeps (radiation_in, scatter_top,
+// scatter_bottom, radiation_out_top, radiation_out_bottom).
+struct BackendState {
+    // geometry staged by updateGeometry until buildAccelerationStructure
+    uint32_t gP = 0;
+    std::vector<float> gT;
+    std::vector<uint32_t> gtype, guuid, gtwos;
+    int gpx = 0, gpy = 0;
+    bool geometry_staged = false;
+
+    rt::Scene scene;
+    bool has_scene = false;
+    uint32_t P = 0;   // primitives of the built scene
+    uint32_t Bg = 0;  // global bands of the materials
+    uint32_t Bl = 0;  // launch bands (set by zeroRadiationBuffers)
+    uint32_t S = 0;   // sources
+    bool has_materials = false, has_fluxes = false;
+    std::vector<float> src_dir;  // [S][3]
+    void *mat = nullptr;         // rho [P][Bg] then tau [P][Bg] (source-0 slice)
+    size_t mat_cap = 0;
+    void *flux = nullptr;        // [S][Bl]
+    size_t flux_cap = 0;
+    void *rad = nullptr;         // in, st, sb, ot, ob, each [P][Bl]
+    size_t rad_cap = 0;
+    rt::Workspace ws;            // per-launch scratch (strength sums, hit words, band tables, jobs)
+    float *rho() const { return (float *)mat; }
+    float *tau() const { return (float *)mat + (size_t)P * Bg; }
+    float *rbuf(int k) const { return (float *)rad + (size_t)k * P * Bl; }  // 0 in 1 st 2 sb 3 ot 4 ob
+};
+
+struct rt_ctx {
+    int device = 0;
+    std::string name;
+    rt::SolveCtx sx;
+    std::mutex mu;
+    BackendState be;
+};
+
+struct rt_scene {
+    rt::Scene s;
+};
+
+#define CKR(x)                                                                         \
+    do {                                                                               \
+        hipError_t e_ = (x);                                                           \
+        if (e_ != hipSuccess) {                                                        \
+            rt::set_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " #x); \
+            return -1;                                                                 \
+        }                                                                              \
+    } while (0)
+
+#define NN(p, what)                                            \
+    do {                                                       \
+        if (!(p)) {                                            \
+            rt::set_error(std::string(what) + " is NULL");     \
+            return -1;                                         \
+        }                                                      \
+    } while (0)

@@ -50,6 +50,10 @@ struct SunSet {
 
 int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunSet &suns, float *R_out,
               int out_on_device);
+// radiation_in / scatter_top / scatter_bottom += direct strength sums x source flux x optical properties
+int launch_deposit_direct(uint32_t P, uint32_t C, uint32_t B, uint32_t S, const float *dsum, const float *flux,
+                          const float *rho, const float *tau, const uint8_t *mask, float *in, float *st, float *sb,
+                          hipStream_t s);
 int debug_diffuse_hits(SolveCtx &sx, const Scene &S, uint32_t n, uint32_t seed, uint32_t *out);
 int debug_direct_sums(SolveCtx &sx, const Scene &S, uint32_t d, uint32_t seed, uint32_t nsrc, const float *src_dir,
                       float *out);

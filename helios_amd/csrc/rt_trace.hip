@@ -566,6 +566,7 @@ __global__ __launch_bounds__(WPB * 64) void k_diffuse(const DiffuseJob *__restri
         uint32_t tw = __float_as_uint(sc.nrm[i].w);
         float4 fr = sc.frame[i];
         bool patch = ty == 0 || ty == 3;
+        uint32_t faces = J.faces ? J.faces : 3u;
         float tm[12];
         const float *Tp = sc.T + (size_t)i * 16;
 #pragma unroll
@@ -573,7 +574,7 @@ __global__ __launch_bounds__(WPB * 64) void k_diffuse(const DiffuseJob *__restri
         for (uint32_t r = lane; r < 2 * nn; r += 64) {
             uint32_t slot = r >= nn ? 1u : 0u;  // slot 0 = top launch (face 1), slot 1 = bottom (face 0)
             uint32_t rr = r - slot * nn;
-            if (slot == 1 && tw == 0) { w[r] = RT_NOT_LAUNCHED; continue; }
+            if ((slot == 1 && tw == 0) || !((faces >> slot) & 1u)) { w[r] = RT_NOT_LAUNCHED; continue; }
             float spx, spy;
             v3f dl;
             diffuse_sample(i, rr % n, rr / n, n, J.seed, patch, spx, spy, dl);

@@ -58,6 +58,8 @@ struct DiffuseJob {
     uint32_t *hits;
     uint32_t item_begin;
     uint32_t P;
+    uint32_t faces;        // bit 0: top launch (face 1), bit 1: bottom launch (face 0); 0 means both
+    uint32_t pad;
 };
 
 // one gather over a hit table: C channels, band of channel c is c % B

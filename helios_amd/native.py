@@ -56,6 +56,13 @@ class rt_timing(C.Structure):
                 ("direct_launches", C.c_uint32), ("diffuse_launches", C.c_uint32)]
 
 
+class rt_launch_params(C.Structure):
+    _fields_ = [("launch_offset", C.c_uint32), ("launch_count", C.c_uint32), ("rays_per_primitive", C.c_uint32),
+                ("random_seed", C.c_uint32), ("num_bands_global", C.c_uint32), ("num_bands_launch", C.c_uint32),
+                ("band_map", u32p), ("band_active", C.POINTER(C.c_uint8)), ("launch_face", C.c_uint32),
+                ("diffuse_flux", f32p), ("radiation_out_top", f32p), ("radiation_out_bottom", f32p)]
+
+
 # name -> (restype, argtypes); the ABI test checks that every declared symbol is exported
 SIGNATURES = {
     "rt_last_error": (C.c_char_p, []),
@@ -84,6 +91,19 @@ SIGNATURES = {
     "rt_debug_direct_sums": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, f32p, f32p]),
     "rt_debug_diffuse_rays": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, f32p]),
     "rt_debug_trace": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, f32p, u32p, C.c_float, C.c_int, u32p, f32p]),
+    "rt_update_geometry": (C.c_int, [C.c_void_p, C.c_uint32, f32p, u32p, u32p, u32p, C.c_int, C.c_int]),
+    "rt_build_acceleration_structure": (C.c_int, [C.c_void_p]),
+    "rt_update_materials": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, f32p, f32p]),
+    "rt_update_sources": (C.c_int, [C.c_void_p, C.c_uint32, u32p, f32p]),
+    "rt_zero_radiation_buffers": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "rt_zero_scatter_buffers": (C.c_int, [C.c_void_p]),
+    "rt_copy_scatter_to_radiation": (C.c_int, [C.c_void_p]),
+    "rt_upload_radiation_out": (C.c_int, [C.c_void_p, f32p, f32p, C.c_size_t]),
+    "rt_upload_source_fluxes": (C.c_int, [C.c_void_p, f32p, C.c_size_t]),
+    "rt_launch_direct_rays": (C.c_int, [C.c_void_p, C.POINTER(rt_launch_params)]),
+    "rt_launch_diffuse_rays": (C.c_int, [C.c_void_p, C.POINTER(rt_launch_params)]),
+    "rt_get_radiation_results": (C.c_int, [C.c_void_p, f32p, f32p, f32p, f32p, f32p]),
+    "rt_get_backend_info": (C.c_int, [C.c_void_p, u32p]),
 }
 
 _lib = None
@@ -273,6 +293,104 @@ class Scene:
         out = np.zeros((self.P, S.shape[0], 2), np.float32)
         _check(self.ctx.lib.rt_debug_direct_sums(self.ctx._h, self._h, d, seed, S.shape[0], _pf(S), _pf(out)))
         return out
+
+
+class Backend:
+    """The call-by-call backend interface of one context (helios::RayTracingBackend,
+    plugins/radiation/include/RayTracingBackend.h:43-312); method names follow the reference's."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.P = 0
+        self.B_launch = 0
+
+    def update_geometry(self, transforms, types, twosided, uuids=None, periodic=(False, False)):
+        T = _f32(transforms).reshape(-1, 16)
+        self._geom_P = T.shape[0]
+        _check(self.lib.rt_update_geometry(self.ctx._h, T.shape[0], _pf(T), _pu(_u32(types)),
+                                           _pu(None if uuids is None else _u32(uuids)), _pu(_u32(twosided)),
+                                           int(periodic[0]), int(periodic[1])))
+
+    def build_acceleration_structure(self):
+        _check(self.lib.rt_build_acceleration_structure(self.ctx._h))
+        self.P = self._geom_P
+
+    def update_materials(self, rho, tau):
+        """rho, tau: [S][P][B_global] (or [P][B_global] for one source)."""
+        R = _f32(rho)
+        Tt = _f32(tau)
+        if R.ndim == 2:
+            R, Tt = R[None], Tt[None]
+        S, P, B = R.shape
+        _check(self.lib.rt_update_materials(self.ctx._h, S, P, B, _pf(R), _pf(Tt)))
+
+    def update_sources(self, directions, types=None):
+        D = _f32(directions).reshape(-1, 3)
+        ty = _u32(np.zeros(D.shape[0]) if types is None else types)
+        _check(self.lib.rt_update_sources(self.ctx._h, D.shape[0], _pu(ty), _pf(D)))
+
+    def zero_radiation_buffers(self, launch_band_count: int):
+        _check(self.lib.rt_zero_radiation_buffers(self.ctx._h, launch_band_count))
+        self.B_launch = launch_band_count
+
+    def zero_scatter_buffers(self):
+        _check(self.lib.rt_zero_scatter_buffers(self.ctx._h))
+
+    def copy_scatter_to_radiation(self):
+        _check(self.lib.rt_copy_scatter_to_radiation(self.ctx._h))
+
+    def upload_radiation_out(self, top, bottom):
+        t, b = _f32(top).reshape(-1), _f32(bottom).reshape(-1)
+        if t.size != b.size:
+            raise RTError("upload_radiation_out: top and bottom differ in size")
+        _check(self.lib.rt_upload_radiation_out(self.ctx._h, _pf(t), _pf(b), t.size))
+
+    def upload_source_fluxes(self, fluxes):
+        f = _f32(fluxes).reshape(-1)
+        _check(self.lib.rt_upload_source_fluxes(self.ctx._h, _pf(f), f.size))
+
+    def _params(self, rays, seed, nbands_global, band_map=None, band_active=None, launch_face=1,
+                diffuse_flux=None, radiation_out=None, offset=0, count=None):
+        keep = []
+        bm = None if band_map is None else _u32(band_map)
+        ba = None if band_active is None else np.ascontiguousarray(band_active, dtype=np.uint8)
+        df = None if diffuse_flux is None else _f32(diffuse_flux)
+        ot = ob = None
+        if radiation_out is not None:
+            ot, ob = _f32(radiation_out[0]).reshape(-1), _f32(radiation_out[1]).reshape(-1)
+        keep += [bm, ba, df, ot, ob]
+        p = rt_launch_params(int(offset), int(self.P - offset if count is None else count), int(rays), int(seed),
+                             int(nbands_global), int(self.B_launch), _pu(bm),
+                             None if ba is None else ba.ctypes.data_as(C.POINTER(C.c_uint8)), int(launch_face),
+                             _pf(df), _pf(ot), _pf(ob))
+        return p, keep
+
+    def launch_direct_rays(self, rays_per_primitive: int, seed: int, nbands_global: int, band_map=None,
+                           offset: int = 0, count: Optional[int] = None):
+        p, keep = self._params(rays_per_primitive, seed, nbands_global, band_map, offset=offset, count=count)
+        _check(self.lib.rt_launch_direct_rays(self.ctx._h, C.byref(p)))
+
+    def launch_diffuse_rays(self, rays_per_primitive: int, seed: int, nbands_global: int, launch_face: int,
+                            diffuse_flux=None, band_map=None, band_active=None, radiation_out=None,
+                            offset: int = 0, count: Optional[int] = None):
+        p, keep = self._params(rays_per_primitive, seed, nbands_global, band_map, band_active, launch_face,
+                               diffuse_flux, radiation_out, offset, count)
+        _check(self.lib.rt_launch_diffuse_rays(self.ctx._h, C.byref(p)))
+
+    def get_radiation_results(self) -> dict:
+        shape = (self.P, self.B_launch)
+        out = {k: np.zeros(shape, np.float32) for k in ("radiation_in", "radiation_out_top", "radiation_out_bottom",
+                                                         "scatter_buff_top", "scatter_buff_bottom")}
+        _check(self.lib.rt_get_radiation_results(self.ctx._h, _pf(out["radiation_in"]), _pf(out["radiation_out_top"]),
+                                                 _pf(out["radiation_out_bottom"]), _pf(out["scatter_buff_top"]),
+                                                 _pf(out["scatter_buff_bottom"])))
+        return out
+
+    def info(self) -> dict:
+        o = np.zeros(4, np.uint32)
+        _check(self.lib.rt_get_backend_info(self.ctx._h, _pu(o)))
+        return {"nprims": int(o[0]), "nbands_global": int(o[1]), "nbands_launch": int(o[2]), "nsources": int(o[3])}
 
 
 class BandRequest:

@@ -127,8 +127,71 @@ int rt_run_band_batch(rt_ctx *ctx, const rt_scene *scene, const rt_band_request 
 
 /* FP64 energy bookkeeping of the last rt_run_band / rt_run_band_batch on this context, per channel
  * (channel = sun*B + band): [0][C] sum(area*R) absorbed, [1][C] sum(area*direct incident),
- * [2][C] sum(area*sky flux gathered by the primary pass). */
+ * [2][C] reserved, written as 0. */
 int rt_energy_residual_fp64(rt_ctx *ctx, double *out, uint32_t nchannels);
+
+/* ---------------------------------------------------------------------------------------------
+ * Call-by-call backend: helios::RayTracingBackend (R/include/RayTracingBackend.h:43-312) as C
+ * functions over one resident scene owned by the context, with the reference backend's five
+ * [P][B_launch] buffers (radiation_in, scatter_top, scatter_bottom, radiation_out_top,
+ * radiation_out_bottom) kept in HBM between calls.  A RadiationModel that keeps its own pass loop
+ * (R/src/RadiationModel.cpp:3804-4356) calls these where it calls VulkanComputeBackend.  Launches
+ * return after their kernels finish (traversal errors are reported by the launch call).
+ * ------------------------------------------------------------------------------------------- */
+/* updateGeometry (RayTracingBackend.h:73; VulkanComputeBackend.cpp:213): stages the primitive
+ * arrays, same meaning as rt_scene_create; uuids NULL = 0..P-1. */
+int rt_update_geometry(rt_ctx *ctx, uint32_t nprims, const float *transforms, const uint32_t *types,
+                       const uint32_t *uuids, const uint32_t *twosided, int periodic_x, int periodic_y);
+/* buildAccelerationStructure (RayTracingBackend.h:81): GPU LBVH of the staged geometry. */
+int rt_build_acceleration_structure(rt_ctx *ctx);
+/* updateMaterials (RayTracingBackend.h:90; R/include/RayTracingTypes.h:157-184): rho, tau are
+ * [S][P][B_global]; the source-0 slice serves every source, as in the Vulkan backend's shaders
+ * (R/shaders/direct_raygen.comp:684-687, diffuse_raygen.comp:575-579). */
+int rt_update_materials(rt_ctx *ctx, uint32_t nsources, uint32_t nprims, uint32_t nbands_global, const float *rho,
+                        const float *tau);
+/* updateSources (RayTracingBackend.h:99; RayTracingTypes.h:189-196): types[S] (0 = collimated;
+ * other types are rejected), positions[S][3] = direction towards each collimated source. */
+int rt_update_sources(rt_ctx *ctx, uint32_t nsources, const uint32_t *types, const float *positions);
+/* zeroRadiationBuffers (RayTracingBackend.h:194; VulkanComputeBackend.cpp:1935): sets B_launch
+ * (1..256) and zeroes all five radiation buffers. */
+int rt_zero_radiation_buffers(rt_ctx *ctx, uint32_t launch_band_count);
+/* zeroScatterBuffers (RayTracingBackend.h:199) */
+int rt_zero_scatter_buffers(rt_ctx *ctx);
+/* copyScatterToRadiation (RayTracingBackend.h:218): radiation_out_top/bottom := scatter_top/bottom */
+int rt_copy_scatter_to_radiation(rt_ctx *ctx);
+/* uploadRadiationOut (RayTracingBackend.h:228): count = P*B_launch values per face */
+int rt_upload_radiation_out(rt_ctx *ctx, const float *top, const float *bottom, size_t count);
+/* uploadSourceFluxes (RayTracingBackend.h:247): [S][B_launch] */
+int rt_upload_source_fluxes(rt_ctx *ctx, const float *fluxes, size_t count);
+
+/* RayTracingLaunchParams (RayTracingTypes.h:201-249), radiation fields.  The reference passes a
+ * [B_global] band_launch_flag and derives the launch -> global band map from it; here the map is
+ * explicit and a launch band that stops scattering keeps its slot with band_active = 0 (the
+ * caller-side adapter builds both from band_launch_flag). */
+typedef struct {
+    uint32_t launch_offset;        /* first primitive launched                                */
+    uint32_t launch_count;         /* primitives launched                                     */
+    uint32_t rays_per_primitive;   /* rays per primitive face (sqrt rounded up per side)      */
+    uint32_t random_seed;
+    uint32_t num_bands_global;
+    uint32_t num_bands_launch;
+    const uint32_t *band_map;      /* [B_launch] global band of each launch band, NULL = identity */
+    const uint8_t *band_active;    /* [B_launch] 0 = band takes no energy in this launch, NULL = all */
+    uint32_t launch_face;          /* diffuse launches: 1 = top face, 0 = bottom face         */
+    const float *diffuse_flux;     /* [B_launch] sky flux of a diffuse launch, NULL = 0       */
+    const float *radiation_out_top;    /* [P][B_launch] uploaded before a diffuse launch, or NULL */
+    const float *radiation_out_bottom; /* [P][B_launch], NULL iff radiation_out_top is NULL      */
+} rt_launch_params;
+/* launchDirectRays (RayTracingBackend.h:139; R/shaders/direct_raygen.comp): every source, then
+ * radiation_in / scatter_top / scatter_bottom += the deposits. */
+int rt_launch_direct_rays(rt_ctx *ctx, const rt_launch_params *params);
+/* launchDiffuseRays (RayTracingBackend.h:146; R/shaders/diffuse_raygen.comp): one face. */
+int rt_launch_diffuse_rays(rt_ctx *ctx, const rt_launch_params *params);
+/* getRadiationResults (RayTracingBackend.h:171): each output [P][B_launch] host memory, NULL skips. */
+int rt_get_radiation_results(rt_ctx *ctx, float *radiation_in, float *radiation_out_top, float *radiation_out_bottom,
+                             float *scatter_top, float *scatter_bottom);
+/* Backend state: out[4] = primitives of the built scene, B_global, B_launch, sources. */
+int rt_get_backend_info(rt_ctx *ctx, uint32_t *out);
 
 /* ---------------------------------------------------------------------------------------------
  * Launch tuning: rays-per-wave grouping, persistent workgroups per CU and waves per workgroup are

@@ -1,0 +1,144 @@
+"""Call-by-call backend entry points (the RayTracingBackend counterpart) through the C ABI.
+
+The reference's runBand pass loop, driven call by call (tests/backend_schedule.py), must reproduce
+the device-resident rt_run_band on the same seeds (the only differences are float summation order
+of the per-face deposits: bound 1e-5 relative) and the FP64 oracle within the 1e-4 parity bound.
+Sub-range launches must reproduce a full launch bit for bit."""
+import numpy as np
+import pytest
+
+from helios_amd import native, scenes
+from oracle.oracle import OracleScene
+from tests.backend_schedule import run_band_via_backend
+from tests.helpers import config1_request, rel_err, small_canopy, two_band_request
+
+pytestmark = pytest.mark.gpu
+
+
+def _backend(ctx, s, rho, tau, sdirs):
+    be = native.Backend(ctx)
+    be.update_geometry(s.transforms, s.types, s.twosided, periodic=s.periodic)
+    be.build_acceleration_structure()
+    be.update_materials(rho, tau)
+    be.update_sources(sdirs)
+    return be
+
+
+def _emission_request(s, depth=1):
+    P = s.P
+    leaf = s.twosided == 1
+    rng = np.random.default_rng(4)
+    rho = np.stack([np.where(leaf, 0.02, 0.05), np.where(leaf, 0.10, 0.15)], 1)
+    tau = np.stack([np.where(leaf, 0.01, 0.0), np.where(leaf, 0.05, 0.0)], 1)
+    eps = np.stack([np.where(leaf, 0.95, 0.9), np.ones(P)], 1)
+    temp = np.where(leaf, 298.0, 293.0) + rng.uniform(-2, 2, P)
+    bands = [native.BandSpec("LW", depth, True, 380.0), native.BandSpec("PAR", depth + 1, False, 120.0)]
+    return native.BandRequest(P, bands, rho=rho, tau=tau, direct_rays=49, diffuse_rays=49,
+                              source_dirs=[native.sun_direction(35.0, 200.0)], source_flux=[[0.0, 480.0]],
+                              direct_seed=5, diffuse_seeds=[61, 62, 63], eps=eps, temperature=temp)
+
+
+CASES = {
+    "config1_par_depth3": lambda: (scenes.config1(1), lambda s: config1_request(s.P, depth=3, nrays=64)),
+    "canopy_two_band": lambda: (small_canopy(9, 2000), lambda s: two_band_request(s, depth=2, nrays=49)),
+    "canopy_periodic": lambda: (small_canopy(3, 1500, periodic=True), lambda s: two_band_request(s, depth=1, nrays=36)),
+    "emission_mixed_depth": lambda: (small_canopy(4, 1500), _emission_request),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_backend_schedule_matches_run_band(ctx, case):
+    s, mk = CASES[case]()
+    req = mk(s)
+    g = ctx.scene(s.transforms, s.types, s.twosided, periodic=s.periodic)
+    R_dev = native.run_band(g, req)
+    be = _backend(ctx, s, req.rho, req.tau, req.sdir)
+    R_be = run_band_via_backend(be, req, s.twosided)
+    assert R_be.shape == R_dev.shape
+    e = rel_err(R_be, R_dev, floor=1e-2)
+    assert e.max() <= 1e-5, f"{case}: max rel err {e.max():.3e} at {np.unravel_index(e.argmax(), e.shape)}"
+
+
+def test_backend_schedule_matches_oracle(ctx):
+    s = small_canopy(9, 1200)
+    req = two_band_request(s, depth=2, nrays=36)
+    be = _backend(ctx, s, req.rho, req.tau, req.sdir)
+    R_be = run_band_via_backend(be, req, s.twosided)
+    o = OracleScene.from_arrays(s)
+    R_o = o.run_band(req)
+    e = rel_err(R_be, R_o, floor=1e-2)
+    assert e.max() <= 1e-4, f"max rel err {e.max():.3e}"
+
+
+def test_band_map_selects_global_bands(ctx):
+    """Materials for 3 global bands, launch bands = global 2 and 0 (in that order)."""
+    s = small_canopy(6, 1200)
+    base = two_band_request(s, depth=2, nrays=36)
+    P = s.P
+    rng = np.random.default_rng(1)
+    rho_g = np.stack([base.rho[:, 1], rng.uniform(0.0, 0.3, P), base.rho[:, 0]], 1).astype(np.float32)
+    tau_g = np.stack([base.tau[:, 1], rng.uniform(0.0, 0.3, P), base.tau[:, 0]], 1).astype(np.float32)
+    g = ctx.scene(s.transforms, s.types, s.twosided)
+    R_dev = native.run_band(g, base)
+    be = _backend(ctx, s, rho_g[None].repeat(2, 0), tau_g[None].repeat(2, 0), base.sdir)
+    R_be = run_band_via_backend(be, base, s.twosided, band_map=[2, 0], nbands_global=3)
+    e = rel_err(R_be, R_dev, floor=1e-2)
+    assert e.max() <= 1e-5, f"max rel err {e.max():.3e}"
+    assert be.info() == {"nprims": P, "nbands_global": 3, "nbands_launch": 2, "nsources": 1}
+
+
+def test_subrange_launches_equal_full_launch(ctx):
+    s = small_canopy(8, 1500)
+    req = two_band_request(s, depth=1, nrays=25)
+    P = s.P
+    results = []
+    for split in (None, [0, 700, 701, P]):
+        be = _backend(ctx, s, req.rho, req.tau, np.r_[req.sdir, [[0.3, -0.2, 0.9]]])
+        be.zero_radiation_buffers(req.B)
+        be.upload_source_fluxes(np.r_[req.sflux, [[120.0, 80.0]]])
+        ranges = [(0, P)] if split is None else list(zip(split[:-1], split[1:]))
+        for a, b in ranges:
+            be.launch_direct_rays(req.direct_rays, 17, req.B, offset=a, count=b - a)
+        out = (np.full((P, req.B), 3.0, np.float32), np.full((P, req.B), 1.5, np.float32))
+        for face in (1, 0):
+            for a, b in ranges:
+                be.launch_diffuse_rays(req.diffuse_rays, 23, req.B, face, diffuse_flux=req.dflux, radiation_out=out,
+                                       offset=a, count=b - a)
+        results.append(be.get_radiation_results())
+    for k in results[0]:
+        assert np.array_equal(results[0][k], results[1][k]), k
+
+
+def test_backend_errors():
+    s = scenes.config1(1)
+    fresh = native.Context(0)
+    try:
+        _check_errors(native.Backend(fresh), s)
+    finally:
+        fresh.close()
+
+
+def _check_errors(be, s):
+    with pytest.raises(native.RTError, match="no geometry"):
+        be.build_acceleration_structure()
+    be.update_geometry(s.transforms, s.types, s.twosided)
+    with pytest.raises(native.RTError, match="no acceleration structure"):
+        be.zero_radiation_buffers(1)
+    be.build_acceleration_structure()
+    with pytest.raises(native.RTError, match="collimated"):
+        be.update_sources([[0, 0, 1]], types=[1])
+    be.update_sources([[0, 0, 1]])
+    with pytest.raises(native.RTError, match="not sized"):
+        be.launch_direct_rays(16, 1, 1)
+    be.update_materials(np.full((s.P, 1), 0.1), np.full((s.P, 1), 0.05))
+    be.zero_radiation_buffers(1)
+    with pytest.raises(native.RTError, match="expected S\\*B_launch"):
+        be.upload_source_fluxes([1.0, 2.0])
+    with pytest.raises(native.RTError, match="no source fluxes"):
+        be.launch_direct_rays(16, 1, 1)
+    with pytest.raises(native.RTError, match="num_bands_global"):
+        be.launch_diffuse_rays(16, 1, 2, 1)
+    with pytest.raises(native.RTError, match="exceeds"):
+        be.launch_diffuse_rays(16, 1, 1, 1, offset=10, count=s.P)
+    with pytest.raises(native.RTError, match="launch_face"):
+        be.launch_diffuse_rays(16, 1, 1, 2)
