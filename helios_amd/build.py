@@ -2,6 +2,7 @@
 
 * ``helios_amd/libhelios_rt.so``   -- gfx950 HIP kernels + the C ABI of include/helios_rt.h
 * ``helios_amd/libhelios_scenes.so`` -- seeded synthetic scene generators (plain C, host only)
+* ``helios_amd/libhelios_host.so`` -- C++ RadiationModel-style interface over the C ABI (host only)
 
 Every HIP translation unit is built with ``-ffp-contract=off`` (the FP32 ray arithmetic must not be
 contracted into FMAs, so rays are reproducible bit for bit) and with packed-FP32 instruction
@@ -32,6 +33,7 @@ HIP_FLAGS = [
 
 RT_LIB = os.path.join(HERE, "libhelios_rt.so")
 SCENE_LIB = os.path.join(HERE, "libhelios_scenes.so")
+HOST_LIB = os.path.join(HERE, "libhelios_host.so")
 
 
 def _run(cmd):
@@ -73,9 +75,20 @@ def build_scenes(verbose: bool = False) -> str:
     return SCENE_LIB
 
 
+def build_host(verbose: bool = False) -> str:
+    tmp = HOST_LIB + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", os.path.join(HERE, "host", "radiation_model.cpp"),
+          "-o", tmp, "-L" + HERE, "-lhelios_rt", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, HOST_LIB)
+    if verbose:
+        print("built", HOST_LIB)
+    return HOST_LIB
+
+
 def build_all(verbose: bool = False) -> None:
     build_scenes(verbose)
     build_rt(verbose)
+    build_host(verbose)
 
 
 if __name__ == "__main__":

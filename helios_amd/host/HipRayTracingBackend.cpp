@@ -1,0 +1,279 @@
+// HipRayTracingBackend.cpp -- helios::RayTracingBackend on the gfx950 C ABI.  Builds inside a Helios tree
+// (plugins/radiation/src/backends/), linked with libhelios_rt.so.
+#include "HipRayTracingBackend.h"
+
+#include <iostream>
+#include <string>
+
+namespace helios {
+
+    namespace {
+        [[noreturn]] void unsupported(const char *who, const std::string &what) {
+            helios_runtime_error(std::string("ERROR (HipRayTracingBackend::") + who + "): " + what + " is not supported by the HIP backend.");
+            throw std::runtime_error(what);  // helios_runtime_error throws; this keeps [[noreturn]] honest
+        }
+    } // namespace
+
+    HipRayTracingBackend::HipRayTracingBackend(int device) : device_(device) {
+    }
+
+    HipRayTracingBackend::~HipRayTracingBackend() {
+        if (ctx_) {
+            rt_destroy(ctx_);
+            ctx_ = nullptr;
+        }
+    }
+
+    void HipRayTracingBackend::check(int rc, const char *who) const {
+        if (rc != 0) {
+            helios_runtime_error(std::string("ERROR (HipRayTracingBackend::") + who + "): " + rt_last_error());
+        }
+    }
+
+    void HipRayTracingBackend::initialize() {
+        if (!ctx_) {
+            check(rt_create(device_, &ctx_), "initialize");
+        }
+    }
+
+    void HipRayTracingBackend::shutdown() {
+        if (ctx_) {
+            check(rt_destroy(ctx_), "shutdown");
+            ctx_ = nullptr;
+        }
+    }
+
+    void HipRayTracingBackend::updateGeometry(const RayTracingGeometry &geometry) {
+        validateGeometryBeforeUpload(geometry);
+        const size_t P = geometry.primitive_count;
+        std::vector<uint32_t> types(geometry.primitive_types.begin(), geometry.primitive_types.end());
+        std::vector<uint32_t> uuids(geometry.primitive_UUIDs.begin(), geometry.primitive_UUIDs.end());
+        std::vector<uint32_t> twosided(P);
+        for (size_t i = 0; i < P; i++) {
+            twosided[i] = static_cast<uint32_t>(static_cast<unsigned char>(geometry.twosided_flags.at(i)));
+        }
+        for (size_t i = 0; i < geometry.mask_IDs.size(); i++) {
+            if (geometry.mask_IDs[i] >= 0) {
+                unsupported("updateGeometry", "texture-masked (alpha) primitives");
+            }
+        }
+        check(rt_update_geometry(ctx_, static_cast<uint32_t>(P), geometry.transform_matrices.data(), types.data(), uuids.empty() ? nullptr : uuids.data(), twosided.data(),
+                                 geometry.periodic_flag.x != 0.f ? 1 : 0, geometry.periodic_flag.y != 0.f ? 1 : 0),
+              "updateGeometry");
+        primitive_count_ = P;
+    }
+
+    void HipRayTracingBackend::buildAccelerationStructure() {
+        check(rt_build_acceleration_structure(ctx_), "buildAccelerationStructure");
+    }
+
+    void HipRayTracingBackend::updateMaterials(const RayTracingMaterial &materials) {
+        if (primitive_count_ == 0) {
+            return; // no geometry yet (VulkanComputeBackend.cpp:592-598)
+        }
+        for (char g: materials.is_glass) {
+            if (g != 0) {
+                unsupported("updateMaterials", "translucent cover (glass) material");
+            }
+        }
+        check(rt_update_materials(ctx_, static_cast<uint32_t>(materials.num_sources), static_cast<uint32_t>(materials.num_primitives), static_cast<uint32_t>(materials.num_bands), materials.reflectivity.data(),
+                                  materials.transmissivity.data()),
+              "updateMaterials");
+    }
+
+    void HipRayTracingBackend::updateSources(const std::vector<RayTracingSource> &sources) {
+        std::vector<uint32_t> types(sources.size());
+        std::vector<float> positions(3 * sources.size());
+        for (size_t s = 0; s < sources.size(); s++) {
+            types[s] = sources[s].type;
+            positions[3 * s] = sources[s].position.x;
+            positions[3 * s + 1] = sources[s].position.y;
+            positions[3 * s + 2] = sources[s].position.z;
+        }
+        check(rt_update_sources(ctx_, static_cast<uint32_t>(sources.size()), types.data(), positions.data()), "updateSources");
+    }
+
+    void HipRayTracingBackend::updateDiffuseRadiation(const std::vector<float> &flux, const std::vector<float> &extinction, const std::vector<helios::vec3> &peak_dir, const std::vector<float> &dist_norm,
+                                                      const std::vector<float> &sky_energy) {
+        for (float K: extinction) {
+            if (K > 0.f) {
+                unsupported("updateDiffuseRadiation", "anisotropic (power-law) diffuse distribution");
+            }
+        }
+    }
+
+    void HipRayTracingBackend::updateSkyModel(const std::vector<helios::vec4> &, const std::vector<float> &, const helios::vec3 &, const std::vector<float> &, float, const std::vector<float> &, const std::vector<uint32_t> &) {
+        // sky radiance feeds camera rendering only, which this backend does not provide
+    }
+
+    void HipRayTracingBackend::bandTable(const RayTracingLaunchParams &params, const char *who) {
+        const size_t Bl = params.num_bands_launch;
+        std::vector<uint32_t> flagged;
+        for (size_t g = 0; g < params.band_launch_flag.size(); g++) {
+            if (params.band_launch_flag[g]) {
+                flagged.push_back(static_cast<uint32_t>(g));
+            }
+        }
+        if (params.band_launch_flag.empty()) {
+            // no flags: identity map, as VulkanComputeBackend.cpp:1265-1270
+            band_map_.resize(Bl);
+            for (size_t b = 0; b < Bl; b++) {
+                band_map_[b] = static_cast<uint32_t>(b);
+            }
+            band_active_.assign(Bl, 1);
+        } else if (flagged.size() == Bl) {
+            band_map_ = flagged;
+            band_active_.assign(Bl, 1);
+        } else if (flagged.size() < Bl && band_map_.size() == Bl) {
+            // a scattering iteration with retired bands (RadiationModel.cpp:4255-4277): each launch band
+            // keeps its slot and stops taking energy once its global flag is cleared
+            for (size_t b = 0; b < Bl; b++) {
+                band_active_[b] = params.band_launch_flag.at(band_map_[b]) ? 1 : 0;
+            }
+        } else {
+            helios_runtime_error(std::string("ERROR (HipRayTracingBackend::") + who + "): band_launch_flag selects " + std::to_string(flagged.size()) + " bands for a launch of " + std::to_string(Bl) + ".");
+        }
+    }
+
+    rt_launch_params HipRayTracingBackend::cParams(const RayTracingLaunchParams &params) const {
+        rt_launch_params p{};
+        p.launch_offset = params.launch_offset;
+        p.launch_count = params.launch_count;
+        p.rays_per_primitive = params.rays_per_primitive;
+        p.random_seed = params.random_seed;
+        p.num_bands_global = params.num_bands_global;
+        p.num_bands_launch = params.num_bands_launch;
+        p.band_map = band_map_.data();
+        p.band_active = band_active_.data();
+        p.launch_face = params.launch_face;
+        p.diffuse_flux = params.diffuse_flux.empty() ? nullptr : params.diffuse_flux.data();
+        if (!params.radiation_out_top.empty() && !params.radiation_out_bottom.empty()) {
+            p.radiation_out_top = params.radiation_out_top.data();
+            p.radiation_out_bottom = params.radiation_out_bottom.data();
+        }
+        return p;
+    }
+
+    void HipRayTracingBackend::launchDirectRays(const RayTracingLaunchParams &params) {
+        if (primitive_count_ == 0) {
+            return;
+        }
+        if (params.specular_reflection_enabled != 0) {
+            unsupported("launchDirectRays", "specular reflection");
+        }
+        bandTable(params, "launchDirectRays");
+        rt_launch_params p = cParams(params);
+        check(rt_launch_direct_rays(ctx_, &p), "launchDirectRays");
+    }
+
+    void HipRayTracingBackend::launchDiffuseRays(const RayTracingLaunchParams &params) {
+        if (primitive_count_ == 0) {
+            return;
+        }
+        for (float K: params.diffuse_extinction) {
+            if (K > 0.f) {
+                unsupported("launchDiffuseRays", "anisotropic (power-law) diffuse distribution");
+            }
+        }
+        for (const auto &q: params.sky_radiance_params) {
+            if (q.w > 0.f) {
+                unsupported("launchDiffuseRays", "Prague sky radiance distribution");
+            }
+        }
+        bandTable(params, "launchDiffuseRays");
+        rt_launch_params p = cParams(params);
+        check(rt_launch_diffuse_rays(ctx_, &p), "launchDiffuseRays");
+    }
+
+    void HipRayTracingBackend::launchCameraRays(const RayTracingLaunchParams &) {
+        unsupported("launchCameraRays", "camera rendering");
+    }
+
+    void HipRayTracingBackend::launchPixelLabelRays(const RayTracingLaunchParams &) {
+        unsupported("launchPixelLabelRays", "camera pixel labelling");
+    }
+
+    void HipRayTracingBackend::getRadiationResults(RayTracingResults &results) {
+        if (primitive_count_ == 0 || launch_band_count_ == 0) {
+            return; // nothing to download (VulkanComputeBackend.cpp:1766-1768)
+        }
+        const size_t n = primitive_count_ * launch_band_count_;
+        results.radiation_in.resize(n);
+        results.radiation_out_top.resize(n);
+        results.radiation_out_bottom.resize(n);
+        results.scatter_buff_top.resize(n);
+        results.scatter_buff_bottom.resize(n);
+        results.scatter_buff_top_cam.clear();
+        results.scatter_buff_bottom_cam.clear();
+        results.radiation_specular.clear();
+        check(rt_get_radiation_results(ctx_, results.radiation_in.data(), results.radiation_out_top.data(), results.radiation_out_bottom.data(), results.scatter_buff_top.data(), results.scatter_buff_bottom.data()),
+              "getRadiationResults");
+        results.num_primitives = primitive_count_;
+        results.num_bands = launch_band_count_;
+    }
+
+    void HipRayTracingBackend::getCameraResults(std::vector<float> &, std::vector<uint> &, std::vector<float> &, uint, const helios::int2 &) {
+        unsupported("getCameraResults", "camera rendering");
+    }
+
+    void HipRayTracingBackend::zeroRadiationBuffers(size_t launch_band_count) {
+        launch_band_count_ = launch_band_count;
+        if (primitive_count_ == 0 || launch_band_count == 0) {
+            return;
+        }
+        band_map_.clear();
+        band_active_.clear();
+        check(rt_zero_radiation_buffers(ctx_, static_cast<uint32_t>(launch_band_count)), "zeroRadiationBuffers");
+    }
+
+    void HipRayTracingBackend::zeroScatterBuffers() {
+        if (primitive_count_ == 0 || launch_band_count_ == 0) {
+            return;
+        }
+        check(rt_zero_scatter_buffers(ctx_), "zeroScatterBuffers");
+    }
+
+    void HipRayTracingBackend::zeroCameraPixelBuffers(const helios::int2 &) {
+        unsupported("zeroCameraPixelBuffers", "camera rendering");
+    }
+
+    void HipRayTracingBackend::copyScatterToRadiation() {
+        if (primitive_count_ == 0 || launch_band_count_ == 0) {
+            return;
+        }
+        check(rt_copy_scatter_to_radiation(ctx_), "copyScatterToRadiation");
+    }
+
+    void HipRayTracingBackend::uploadRadiationOut(const std::vector<float> &radiation_out_top, const std::vector<float> &radiation_out_bottom) {
+        if (radiation_out_top.size() != radiation_out_bottom.size()) {
+            helios_runtime_error("ERROR (HipRayTracingBackend::uploadRadiationOut): top and bottom buffers differ in size.");
+        }
+        check(rt_upload_radiation_out(ctx_, radiation_out_top.data(), radiation_out_bottom.data(), radiation_out_top.size()), "uploadRadiationOut");
+    }
+
+    void HipRayTracingBackend::zeroCameraScatterBuffers(size_t) {
+        // no camera buffers exist on this backend
+    }
+
+    void HipRayTracingBackend::uploadSourceFluxes(const std::vector<float> &fluxes) {
+        if (fluxes.empty()) {
+            return;
+        }
+        check(rt_upload_source_fluxes(ctx_, fluxes.data(), fluxes.size()), "uploadSourceFluxes");
+    }
+
+    void HipRayTracingBackend::uploadSourceFluxesCam(const std::vector<float> &) {
+        // camera-weighted fluxes feed camera rendering only
+    }
+
+    void HipRayTracingBackend::queryGPUMemory() const {
+        size_t free_b = 0, total_b = 0;
+        check(rt_query_memory(ctx_, &free_b, &total_b), "queryGPUMemory");
+        std::cout << "GPU memory (" << rt_backend_name(ctx_) << "): " << (total_b - free_b) / (1024 * 1024) << " MB used of " << total_b / (1024 * 1024) << " MB" << std::endl;
+    }
+
+    std::string HipRayTracingBackend::getBackendName() const {
+        return ctx_ ? std::string(rt_backend_name(ctx_)) : std::string("HIP gfx950 (not initialized)");
+    }
+
+} // namespace helios
