@@ -477,11 +477,14 @@ __device__ __forceinline__ void add_stats(unsigned long long *stats, const TCnt 
 // ------------------------------------------------------------------------------------------------
 // direct rays
 // ------------------------------------------------------------------------------------------------
-template <int WPB, int STK>
+// COUNT: the counting variant (stats != nullptr) traces every ray on its own (per-lane LDS stack) so the
+// counters hold single-ray node visits and primitive tests -- the algorithmic work per ray -- rather than
+// the packet's visits.  Both variants make the same occlusion decision for every ray.
+template <int WPB, int STK, bool COUNT>
 __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict__ jobs, uint32_t njobs, uint32_t total,
                                                      uint32_t *counter, int *err, unsigned long long *stats) {
-    __shared__ uint32_t wstk_all[STK * WPB];
-    uint32_t *wstk = wstk_all + (threadIdx.x >> 6) * STK;
+    __shared__ uint32_t wstk_all[COUNT ? STK * WPB * 64 : STK * WPB];
+    uint32_t *wstk = COUNT ? wstk_all + threadIdx.x : wstk_all + (threadIdx.x >> 6) * STK;
     const uint32_t lane = threadIdx.x & 63;
     TCnt cnt{0, 0};
     uint32_t nray = 0;
@@ -521,7 +524,9 @@ __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict
                         o = xform3(tm, spx, spy, 0.0f);
                         nray++;
                     }
-                    bool occl = packet_occluded<STK>(sc, o, dir, 1e-4f, i, need, wstk, err, cnt);
+                    bool occl;
+                    if (COUNT) occl = need && occluded<STK>(sc, o, dir, 1e-4f, i, wstk, WPB * 64, err, cnt);
+                    else occl = packet_occluded<STK>(sc, o, dir, 1e-4f, i, need, wstk, err, cnt);
                     if (need && !occl) acc.add(strength);
                 }
                 wave_reduce(acc);
@@ -886,13 +891,20 @@ int launch_direct(const DirectJob *jobs, uint32_t njobs, uint32_t total, uint32_
     k_zero_u32<<<1, 1, 0, st>>>(counter);
     uint32_t wpb = cfg.waves_per_block;
     uint32_t g = grid_for(cfg, ncu, total, wpb);
+    if (stats) {
+        // counting variant: per-lane stacks, 4 waves per block
+        PICK_STK(depth, (k_direct<4, 32, true><<<g, 256, 0, st>>>(jobs, njobs, total, counter, err, stats)),
+                 (k_direct<4, 64, true><<<g, 256, 0, st>>>(jobs, njobs, total, counter, err, stats)));
+        LCHECK();
+        return 0;
+    }
     switch (wpb) {
-        case 1: PICK_STK(depth, (k_direct<1, 32><<<g, 64, 0, st>>>(jobs, njobs, total, counter, err, stats)),
-                         (k_direct<1, 64><<<g, 64, 0, st>>>(jobs, njobs, total, counter, err, stats))); break;
-        case 2: PICK_STK(depth, (k_direct<2, 32><<<g, 128, 0, st>>>(jobs, njobs, total, counter, err, stats)),
-                         (k_direct<2, 64><<<g, 128, 0, st>>>(jobs, njobs, total, counter, err, stats))); break;
-        default: PICK_STK(depth, (k_direct<4, 32><<<g, 256, 0, st>>>(jobs, njobs, total, counter, err, stats)),
-                          (k_direct<4, 64><<<g, 256, 0, st>>>(jobs, njobs, total, counter, err, stats))); break;
+        case 1: PICK_STK(depth, (k_direct<1, 32, false><<<g, 64, 0, st>>>(jobs, njobs, total, counter, err, stats)),
+                         (k_direct<1, 64, false><<<g, 64, 0, st>>>(jobs, njobs, total, counter, err, stats))); break;
+        case 2: PICK_STK(depth, (k_direct<2, 32, false><<<g, 128, 0, st>>>(jobs, njobs, total, counter, err, stats)),
+                         (k_direct<2, 64, false><<<g, 128, 0, st>>>(jobs, njobs, total, counter, err, stats))); break;
+        default: PICK_STK(depth, (k_direct<4, 32, false><<<g, 256, 0, st>>>(jobs, njobs, total, counter, err, stats)),
+                          (k_direct<4, 64, false><<<g, 256, 0, st>>>(jobs, njobs, total, counter, err, stats))); break;
     }
     LCHECK();
     return 0;
