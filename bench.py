@@ -8,8 +8,13 @@ backend (all 64 suns in one set of persistent launches).  `value` = solves/s of 
 (64 solves per step / step time).  The plain one-solve-per-launch backend is timed on a few suns of the
 same series and reported beside it, with the speedup.
 
-Other workloads: --workload config2 (200k-triangle periodic canopy, one 2-band solve per step) and
---workload castrays (CollisionDetection::castRaysSoA of 1e8 rays over the 5x tiled orchard).
+The default run also serves a config-4 request stream (Poisson arrivals over random canopies of 10k-2M
+primitives) through the request server and through the plain loop and reports both under "serving"
+(--no-serve skips it).
+
+Other workloads: --workload config2 (200k-triangle periodic canopy, one 2-band solve per step),
+--workload castrays (CollisionDetection::castRaysSoA of 1e8 rays over the 5x tiled orchard) and
+--workload serve (the serving leg alone).
 """
 from __future__ import annotations
 
@@ -333,17 +338,101 @@ def bench_castrays(args):
                          "bytes_per_ray": round(brays, 1)}}
 
 
+def bench_serve(args, ctx=None):
+    """Config 4: a Poisson stream of single-sun requests over seeded canopies (P log-uniform 10k-2M,
+    1-4 bands), served by the request server (resident models, memory-footprint admission, continuous
+    batching over `slots` streams) and, on the same stream, by the plain loop (one rt_run_band per request
+    in arrival order).  The arrival rate is `serve_load` x the plain loop's capacity."""
+    import threading
+    ctx = ctx or native.Context(0)
+    models = W.serve_models(seed=4, nscenes=args.serve_scenes)
+    log(f"[serve] model sizes {[s.P for s, _ in models]}, bands {[r.B for _, r in models]}")
+    scs = [ctx.scene(s.transforms, s.types, s.twosided) for s, _ in models]
+    reqs = W.serve_requests(models, args.serve_requests, seed=5)
+    single = [W.single_sun_request(models[m][1], d, f, q, seed) for (m, d, f, q, seed) in reqs]
+    # plain service time per model (one warm solve each) -> arrival rate
+    t_model = []
+    for (s, req), g in zip(models, scs):
+        native.run_band(g, req)
+        t = time.perf_counter()
+        native.run_band(g, req)
+        t_model.append(time.perf_counter() - t)
+    mean_service = float(np.mean([t_model[m] for (m, *_r) in reqs]))
+    rate = args.serve_load / mean_service
+    rng = np.random.default_rng(6)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(reqs)))
+    arrivals -= arrivals[0]
+    log(f"[serve] {len(reqs)} requests, mean plain service {mean_service * 1e3:.1f} ms, rate {rate:.2f}/s")
+
+    # plain loop: one solve at a time, in arrival order
+    plain_out, plain_lat = [], []
+    t0 = time.perf_counter()
+    for k, (m, *_r) in enumerate(reqs):
+        now = time.perf_counter() - t0
+        if now < arrivals[k]:
+            time.sleep(arrivals[k] - now)
+        plain_out.append(native.run_band(scs[m], single[k]))
+        plain_lat.append((time.perf_counter() - t0 - arrivals[k]) * 1e3)
+    plain_span = time.perf_counter() - t0
+    log(f"[serve] plain loop {plain_span:.2f} s")
+
+    srv = native.Server(ctx, slots=args.serve_slots, max_batch_suns=32)
+    ids = [srv.add_model(g, req) for g, (_, req) in zip(scs, models)]
+    tickets = [None] * len(reqs)
+
+    def producer():
+        t1 = time.perf_counter()
+        for k, (m, d, f, q, seed) in enumerate(reqs):
+            now = time.perf_counter() - t1
+            if now < arrivals[k]:
+                time.sleep(arrivals[k] - now)
+            tickets[k] = srv.submit(ids[m], d, f, q, seed)
+
+    t0 = time.perf_counter()
+    th = threading.Thread(target=producer)
+    th.start()
+    th.join()
+    worst = 0.0
+    for k in range(len(reqs)):
+        R, _ = srv.wait(tickets[k])
+        a = plain_out[k].astype(np.float64)
+        worst = max(worst, float(np.max(np.abs(R - a) / np.maximum(np.abs(a), 1.0))))
+    served_span = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    log(f"[serve] server {served_span:.2f} s, stats {st}")
+    return {
+        "workload": (f"config4: {len(reqs)} Poisson requests at {rate:.2f}/s ({args.serve_load}x plain capacity) over "
+                     f"{len(models)} random canopies (P = {sorted(s.P for s, _ in models)}, 1-4 bands each), "
+                     f"d=n=10, depth 2, one sun per request"),
+        "solves_per_s": round(len(reqs) / served_span, 3),
+        "p50_ms": round(st["p50_ms"], 1), "p99_ms": round(st["p99_ms"], 1), "mean_ms": round(st["mean_ms"], 1),
+        "plain_solves_per_s": round(len(reqs) / plain_span, 3),
+        "plain_p50_ms": round(pct(plain_lat, 50), 1), "plain_p99_ms": round(pct(plain_lat, 99), 1),
+        "speedup_vs_plain": round(plain_span / served_span, 2),
+        "slots": args.serve_slots, "batches": st["batches"], "max_batch": st["max_batch"],
+        "memory_budget_bytes": st["memory_budget"], "peak_workspace_bytes": st["peak_workspace_bytes"],
+        "resident_model_bytes": st["resident_model_bytes"],
+        "max_rel_err_vs_plain": worst,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="diurnal", choices=["diurnal", "config2", "castrays"])
+    ap.add_argument("--workload", default="diurnal", choices=["diurnal", "config2", "castrays", "serve"])
     ap.add_argument("--suns", type=int, default=64)
     ap.add_argument("--plain-suns", type=int, default=4)
     ap.add_argument("--rays", type=int, default=100_000_000)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="skip the config-4 serving leg of the default run")
+    ap.add_argument("--serve-requests", type=int, default=64)
+    ap.add_argument("--serve-scenes", type=int, default=6)
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-load", type=float, default=2.0)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run a plain copy of this many MiB first (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -353,6 +442,15 @@ def main():
         sys.exit(2)
     if args.workload == "diurnal":
         res = bench_diurnal(args)
+        if not args.no_serve:
+            log("[bench] config-4 serving leg ...")
+            res["serving"] = bench_serve(args)
+    elif args.workload == "serve":
+        res = bench_serve(args)
+        res = {"metric": "served solves/s with p50/p99 latency (config 4)", "value": res["solves_per_s"],
+               "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 1, "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": {"workload": res["workload"]},
+               "serving": res}
     elif args.workload == "config2":
         res = bench_config2(args)
     else:

@@ -16,6 +16,7 @@ struct Workspace {
     size_t cap = 0;
     int reserve(size_t bytes);
     void release();
+    static size_t capacity_for(size_t bytes);  // what reserve(bytes) allocates when it has to grow
 };
 
 struct EvRec {
@@ -48,8 +49,16 @@ struct SunSet {
     const uint32_t *seed; // [K]
 };
 
+// Optical properties already resident in device memory ([P][B] rho, tau, eps or null, [P] temp or null).
+struct DevMaterials {
+    const float *rho, *tau, *eps, *temp;
+};
+
+// Device bytes a solve of K suns needs in its workspace.
+size_t solve_workspace_bytes(uint32_t P, const rt_band_request &q, uint32_t K, bool resident_materials);
+
 int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunSet &suns, float *R_out,
-              int out_on_device);
+              int out_on_device, const DevMaterials *dm = nullptr);
 // radiation_in / scatter_top / scatter_bottom += direct strength sums x source flux x optical properties
 int launch_deposit_direct(uint32_t P, uint32_t C, uint32_t B, uint32_t S, const float *dsum, const float *flux,
                           const float *rho, const float *tau, const uint8_t *mask, float *in, float *st, float *sb,

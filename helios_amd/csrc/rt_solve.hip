@@ -178,7 +178,7 @@ int Workspace::reserve(size_t bytes) {
     if (base) (void)hipFree(base);
     base = nullptr;
     cap = 0;
-    size_t want = bytes + bytes / 8 + (1u << 20);
+    size_t want = capacity_for(bytes);
     CK(hipMalloc(&base, want));
     cap = want;
     return 0;
@@ -213,8 +213,28 @@ static int ev_pair(SolveCtx &sx, hipEvent_t &a, hipEvent_t &b) {
     return 0;
 }
 
+size_t solve_workspace_bytes(uint32_t P, const rt_band_request &q, uint32_t K, bool resident_materials) {
+    const uint32_t B = q.nbands, Sn = q.nsources, C = K * B;
+    uint32_t maxdepth = 0;
+    for (uint32_t b = 0; b < B; b++) maxdepth = std::max(maxdepth, q.band_depth[b]);
+    const uint32_t n = (uint32_t)std::ceil(std::sqrt((double)q.diffuse_rays));
+    const size_t nn = (size_t)n * n, PC = (size_t)P * C;
+    // upper bound: the hit table is counted whenever diffuse rays are requested (whether a diffuse
+    // pass runs also depends on the per-sun sky fluxes)
+    bool need_hits = n > 0;
+    const uint32_t nm = 6 + 4 * std::max(maxdepth, 1u);
+    size_t bytes = 5 * PC * 4 + (need_hits ? (size_t)P * 2 * nn * 4 : 0) + (size_t)K * Sn * P * 2 * 4 + (size_t)P * 8 +
+                   (size_t)K * Sn * B * 4 + (size_t)C * 4 * 2 + (size_t)nm * C + (size_t)B * 4 + (size_t)C * 8 * 3 +
+                   (size_t)K * Sn * sizeof(DirectJob) + (sizeof(DiffuseJob) + sizeof(GatherJob)) * (1 + maxdepth) +
+                   64 * 64;
+    if (!resident_materials) bytes += (size_t)P * B * 4 * 3 + (size_t)P * 4;
+    return bytes + 32 * 256;  // carve alignment
+}
+
+size_t Workspace::capacity_for(size_t bytes) { return bytes + bytes / 8 + (1u << 20); }
+
 int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunSet &suns, float *R_out,
-              int out_on_device) {
+              int out_on_device, const DevMaterials *dm) {
     hipStream_t st = sx.stream;
     const uint32_t P = S.h.P, B = q.nbands, K = suns.K, Sn = q.nsources, C = K * B;
     if (B == 0 || K == 0) { set_error("run_band: no bands or no suns"); return -1; }
@@ -279,19 +299,18 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
 
     // ---- workspace ----
     size_t PC = (size_t)P * C;
-    size_t bytes = 5 * PC * 4 + (need_hits ? (size_t)P * 2 * nn * 4 : 0) + (size_t)K * Sn * P * 2 * 4 +
-                   (size_t)P * B * 4 * 3 + (size_t)P * 4 + (size_t)K * Sn * B * 4 + (size_t)C * 4 * 2 + mk.size() +
-                   (size_t)B * 4 + (size_t)P * 8 + (size_t)C * 8 * 3 + (size_t)K * Sn * sizeof(DirectJob) +
-                   (sizeof(DiffuseJob) + sizeof(GatherJob)) * (1 + maxdepth) + 64 * 64;
+    size_t bytes = solve_workspace_bytes(P, q, K, dm != nullptr);
     if (sx.ws.reserve(bytes)) return -1;
     Carve cv{(char *)sx.ws.base};
     float *in = cv.take<float>(PC), *stb = cv.take<float>(PC), *sbb = cv.take<float>(PC);
     float *ot = cv.take<float>(PC), *ob = cv.take<float>(PC);
     uint32_t *hits = need_hits ? cv.take<uint32_t>((size_t)P * 2 * nn) : nullptr;
     float *dsum = cv.take<float>((size_t)K * Sn * P * 2 + 1);
-    float *rho = cv.take<float>((size_t)P * B), *tau = cv.take<float>((size_t)P * B);
-    float *eps = q.eps ? cv.take<float>((size_t)P * B) : nullptr;
-    float *temp = q.temperature ? cv.take<float>(P) : nullptr;
+    // optical properties: resident device copies when the caller has them, else staged per solve
+    const float *rho = dm ? dm->rho : cv.take<float>((size_t)P * B);
+    const float *tau = dm ? dm->tau : cv.take<float>((size_t)P * B);
+    const float *eps = dm ? dm->eps : (q.eps ? cv.take<float>((size_t)P * B) : nullptr);
+    const float *temp = dm ? dm->temp : (q.temperature ? cv.take<float>(P) : nullptr);
     float *flux = cv.take<float>((size_t)K * Sn * B + 1);
     float *dflux = cv.take<float>(C), *zflux = cv.take<float>(C);
     uint8_t *masks = cv.take<uint8_t>(mk.size());
@@ -311,10 +330,12 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
     CK(hipMemsetAsync(ot, 0, PC * 4, st));
     CK(hipMemsetAsync(ob, 0, PC * 4, st));
     CK(hipMemsetAsync(err, 0, sizeof(int), st));
-    CK(hipMemcpyAsync(rho, q.rho, (size_t)P * B * 4, hipMemcpyHostToDevice, st));
-    CK(hipMemcpyAsync(tau, q.tau, (size_t)P * B * 4, hipMemcpyHostToDevice, st));
-    if (eps) CK(hipMemcpyAsync(eps, q.eps, (size_t)P * B * 4, hipMemcpyHostToDevice, st));
-    if (temp) CK(hipMemcpyAsync(temp, q.temperature, (size_t)P * 4, hipMemcpyHostToDevice, st));
+    if (!dm) {
+        CK(hipMemcpyAsync((void *)rho, q.rho, (size_t)P * B * 4, hipMemcpyHostToDevice, st));
+        CK(hipMemcpyAsync((void *)tau, q.tau, (size_t)P * B * 4, hipMemcpyHostToDevice, st));
+        if (eps) CK(hipMemcpyAsync((void *)eps, q.eps, (size_t)P * B * 4, hipMemcpyHostToDevice, st));
+        if (temp) CK(hipMemcpyAsync((void *)temp, q.temperature, (size_t)P * 4, hipMemcpyHostToDevice, st));
+    }
     if (Sn) CK(hipMemcpyAsync(flux, suns.flux, (size_t)K * Sn * B * 4, hipMemcpyHostToDevice, st));
     CK(hipMemcpyAsync(dflux, dfl.data(), (size_t)C * 4, hipMemcpyHostToDevice, st));
     CK(hipMemsetAsync(zflux, 0, (size_t)C * 4, st));

@@ -63,6 +63,22 @@ class rt_launch_params(C.Structure):
                 ("diffuse_flux", f32p), ("radiation_out_top", f32p), ("radiation_out_bottom", f32p)]
 
 
+class rt_server_cfg(C.Structure):
+    _fields_ = [("slots", C.c_uint32), ("max_batch_suns", C.c_uint32), ("memory_budget", C.c_uint64)]
+
+
+class rt_solve_sun(C.Structure):
+    _fields_ = [("source_dir", f32p), ("source_flux", f32p), ("diffuse_flux", f32p), ("direct_seed", C.c_uint32)]
+
+
+class rt_server_stats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+                ("batches", C.c_uint64), ("batched_requests", C.c_uint64), ("max_batch", C.c_uint32),
+                ("queued", C.c_uint32), ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double),
+                ("memory_budget", C.c_uint64), ("peak_workspace_bytes", C.c_uint64),
+                ("resident_model_bytes", C.c_uint64)]
+
+
 # name -> (restype, argtypes); the ABI test checks that every declared symbol is exported
 SIGNATURES = {
     "rt_last_error": (C.c_char_p, []),
@@ -104,6 +120,14 @@ SIGNATURES = {
     "rt_launch_diffuse_rays": (C.c_int, [C.c_void_p, C.POINTER(rt_launch_params)]),
     "rt_get_radiation_results": (C.c_int, [C.c_void_p, f32p, f32p, f32p, f32p, f32p]),
     "rt_get_backend_info": (C.c_int, [C.c_void_p, u32p]),
+    "rt_server_create": (C.c_int, [C.c_void_p, C.POINTER(rt_server_cfg), C.POINTER(C.c_void_p)]),
+    "rt_server_destroy": (C.c_int, [C.c_void_p]),
+    "rt_server_add_model": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(rt_band_request), u32p]),
+    "rt_server_submit": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(rt_solve_sun), f32p, C.POINTER(C.c_uint64)]),
+    "rt_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_double)]),
+    "rt_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(rt_server_stats)]),
+    "rt_server_reset_stats": (C.c_int, [C.c_void_p]),
+    "rt_solve_footprint": (C.c_int, [C.c_uint32, C.POINTER(rt_band_request), C.c_uint32, C.POINTER(C.c_uint64)]),
 }
 
 _lib = None
@@ -393,6 +417,66 @@ class Backend:
         return {"nprims": int(o[0]), "nbands_global": int(o[1]), "nbands_launch": int(o[2]), "nsources": int(o[3])}
 
 
+class Server:
+    """Request server over one context (continuous batching with memory-footprint admission)."""
+
+    def __init__(self, ctx: Context, slots: int = 4, max_batch_suns: int = 64, memory_budget: int = 0):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        cfg = rt_server_cfg(slots, max_batch_suns, memory_budget)
+        h = C.c_void_p()
+        _check(self.lib.rt_server_create(ctx._h, C.byref(cfg), C.byref(h)))
+        self._h = h
+        self._models = {}
+        self._pending = {}
+
+    def close(self):
+        if self._h:
+            self.lib.rt_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_model(self, scene: "Scene", req: "BandRequest") -> int:
+        mid = C.c_uint32()
+        cs = req.c_struct()
+        _check(self.lib.rt_server_add_model(self._h, scene._h, C.byref(cs), C.byref(mid)))
+        self._models[mid.value] = (scene.P, req.B, req.sdir.shape[0])
+        return mid.value
+
+    def submit(self, model: int, source_dirs, source_flux, diffuse_flux, direct_seed: int) -> int:
+        # an unknown model id still goes to the library, which rejects it
+        P, B, S = self._models.get(model, (1, len(np.ravel(diffuse_flux)), len(np.ravel(source_dirs)) // 3))
+        D = _f32(source_dirs).reshape(S, 3)
+        F = _f32(source_flux).reshape(S, B)
+        Q = _f32(diffuse_flux).reshape(B)
+        out = np.zeros((P, B), np.float32)
+        sun = rt_solve_sun(_pf(D), _pf(F), _pf(Q), int(direct_seed))
+        t = C.c_uint64()
+        _check(self.lib.rt_server_submit(self._h, model, C.byref(sun), _pf(out), C.byref(t)))
+        self._pending[t.value] = out
+        return t.value
+
+    def wait(self, ticket: int):
+        """-> (radiation_flux [P][B], latency ms)"""
+        lat = C.c_double()
+        out = self._pending.pop(ticket)
+        _check(self.lib.rt_server_wait(self._h, ticket, C.byref(lat)))
+        return out, lat.value
+
+    def stats(self) -> dict:
+        st = rt_server_stats()
+        _check(self.lib.rt_server_get_stats(self._h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in rt_server_stats._fields_}
+
+    def reset_stats(self):
+        _check(self.lib.rt_server_reset_stats(self._h))
+
+
 class BandRequest:
     """All inputs of one runBand call except the per-sun part (owns its numpy buffers)."""
 
@@ -454,6 +538,15 @@ def run_band_batch(scene: Scene, req: BandRequest, sun_dirs, sun_flux, sky_flux,
     _check(scene.ctx.lib.rt_run_band_batch(scene.ctx._h, scene._h, C.byref(cs), C.byref(sb),
                                            R.ctypes.data_as(C.c_void_p), 0))
     return R
+
+
+def solve_footprint(P: int, req: "BandRequest", nsuns: int) -> int:
+    """Device bytes a server slot needs for a batch of nsuns suns of req over P primitives."""
+    lib = load_library()
+    out = C.c_uint64()
+    cs = req.c_struct()
+    _check(lib.rt_solve_footprint(P, C.byref(cs), nsuns, C.byref(out)))
+    return out.value
 
 
 def energy_fp64(ctx: Context, nchannels: int) -> np.ndarray:

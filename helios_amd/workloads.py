@@ -108,3 +108,45 @@ def rays_per_solve(s, req) -> dict:
     direct = int(np.sum(traced)) * d * d * S
     diffuse = (1 + depth) * n * n * faces
     return {"direct": direct, "diffuse": diffuse, "total": direct + diffuse, "d": d, "n": n, "passes": 1 + depth}
+
+
+def serve_models(seed: int = 4, nscenes: int = 6, pmin: float = 1e4, pmax: float = 2e6, rays: int = 100,
+                 depth: int = 2):
+    """Config 4 models: seeded random canopies (mixed patches/triangles) with P log-uniform in
+    [pmin, pmax], each with 1-4 bands of its own optical properties and sky flux, one collimated
+    source, d = n = ceil(sqrt(rays)), scattering depth `depth`.  Returns [(SceneArrays, BandRequest)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(nscenes):
+        P = int(round(10 ** rng.uniform(math.log10(pmin), math.log10(pmax))))
+        s = scenes.random_canopy(100 + i, P)
+        nb = int(rng.integers(1, 5))
+        leaf = s.twosided == 1
+        rho = np.zeros((s.P, nb), np.float32)
+        tau = np.zeros((s.P, nb), np.float32)
+        spec = []
+        for b in range(nb):
+            r, t = rng.uniform(0.05, 0.45), rng.uniform(0.0, 0.4)
+            rho[:, b] = np.where(leaf, r, 0.15)
+            tau[:, b] = np.where(leaf, t, 0.0)
+            spec.append(native.BandSpec(f"band{b}", depth, False, float(rng.uniform(50, 200))))
+        req = native.BandRequest(s.P, spec, rho=rho, tau=tau, direct_rays=rays, diffuse_rays=rays,
+                                 source_dirs=[[0.0, 0.0, 1.0]], source_flux=[[500.0] * nb], direct_seed=1,
+                                 diffuse_seeds=[7000 + 10 * i + k for k in range(depth + 1)])
+        out.append((s, req))
+    return out
+
+
+def serve_requests(models, n: int, seed: int = 5):
+    """n requests: (model index, sun dir [1][3], source flux [1][B], sky flux [B], direct seed), models
+    drawn uniformly, suns uniform in zenith 10-80 deg and azimuth 0-360 deg."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        m = int(rng.integers(len(models)))
+        req = models[m][1]
+        z, a = math.radians(rng.uniform(10, 80)), math.radians(rng.uniform(0, 360))
+        d = np.array([[math.sin(z) * math.sin(a), math.sin(z) * math.cos(a), math.cos(z)]], np.float32)
+        out.append((m, d, (req.sflux * rng.uniform(0.5, 1.5)).astype(np.float32),
+                    (req.dflux * rng.uniform(0.5, 1.5)).astype(np.float32), int(rng.integers(1, 2**31))))
+    return out

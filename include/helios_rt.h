@@ -194,6 +194,55 @@ int rt_get_radiation_results(rt_ctx *ctx, float *radiation_in, float *radiation_
 int rt_get_backend_info(rt_ctx *ctx, uint32_t *out);
 
 /* ---------------------------------------------------------------------------------------------
+ * Request server (no reference counterpart: the reference solves one runBand per call).  Models
+ * (a resident scene + bands, ray counts, depths, optical properties kept in HBM, diffuse seeds) are
+ * registered once; each request is one sun for one model.  Worker slots (own stream + workspace)
+ * take the oldest request together with every queued request of the same model (FIFO, up to
+ * max_batch_suns), shrink that batch until its workspace fits the memory budget, and solve it as
+ * one batched run; slots execute concurrently.  A request's result equals rt_run_band of that sun
+ * alone with the model's settings.  A request whose single-sun footprint exceeds the budget fails.
+ * The scene must outlive the server.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct rt_server rt_server;
+typedef struct {
+    uint32_t slots;          /* concurrent batches (streams), 0 = 4, at most 16          */
+    uint32_t max_batch_suns; /* requests per batch, 0 = 64 (also capped at 256 / bands) */
+    uint64_t memory_budget;  /* device bytes for solve workspaces, 0 = half the free HBM */
+} rt_server_cfg;
+typedef struct {
+    const float *source_dir;   /* [S][3] */
+    const float *source_flux;  /* [S][B] */
+    const float *diffuse_flux; /* [B]    */
+    uint32_t direct_seed;
+} rt_solve_sun;
+typedef struct {
+    uint64_t submitted, completed, failed;
+    uint64_t batches;              /* batched solves run                                    */
+    uint64_t batched_requests;     /* requests they carried                                 */
+    uint32_t max_batch;            /* largest batch run                                     */
+    uint32_t queued;               /* requests waiting now                                  */
+    double p50_ms, p99_ms, mean_ms; /* submit -> result-on-host latency of completed requests */
+    uint64_t memory_budget;
+    uint64_t peak_workspace_bytes; /* high-water mark of slot workspaces + outputs          */
+    uint64_t resident_model_bytes; /* optical properties of the registered models           */
+} rt_server_stats;
+int rt_server_create(rt_ctx *ctx, const rt_server_cfg *cfg, rt_server **out);
+/* Finishes every queued request, then frees the server. */
+int rt_server_destroy(rt_server *srv);
+/* base: bands, ray counts, depths, emission, rho/tau/eps/temperature, diffuse seeds, schedule;
+ * its source_dir / source_flux / diffuse_flux / direct_seed are ignored (they come per request). */
+int rt_server_add_model(rt_server *srv, const rt_scene *scene, const rt_band_request *base, uint32_t *model_id);
+/* R_out: [P][B] host memory, written before the request completes; keep it alive until rt_server_wait. */
+int rt_server_submit(rt_server *srv, uint32_t model_id, const rt_solve_sun *sun, float *R_out, uint64_t *ticket);
+/* Blocks until the request is done; returns its status (and message) and releases the ticket. */
+int rt_server_wait(rt_server *srv, uint64_t ticket, double *latency_ms);
+int rt_server_get_stats(rt_server *srv, rt_server_stats *out);
+/* Device bytes (workspace + output) a server slot needs for a batch of nsuns suns of this request
+ * over nprims primitives -- the figure admission compares with the budget. */
+int rt_solve_footprint(uint32_t nprims, const rt_band_request *req, uint32_t nsuns, uint64_t *bytes);
+int rt_server_reset_stats(rt_server *srv);
+
+/* ---------------------------------------------------------------------------------------------
  * Launch tuning: rays-per-wave grouping, persistent workgroups per CU and waves per workgroup are
  * picked by rt_autotune (timing candidate launch shapes on the given scene) or set by hand.
  * ------------------------------------------------------------------------------------------- */
