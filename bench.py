@@ -165,6 +165,12 @@ def bench_diurnal(args):
     hip.hipFree.argtypes = [C.c_void_p]
     if hip.hipMalloc(C.byref(out_dev), K * P * B * 4) != 0:
         raise RuntimeError("hipMalloc failed")
+    tune = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tune = ctx.autotune(g, base, nsuns=K)
+        tune["seconds"] = round(time.perf_counter() - t0, 2)
+        log(f"[bench] autotune {tune}")
     ctx.enable_timing(True)
 
     def step():
@@ -235,6 +241,7 @@ def bench_diurnal(args):
         "kernel_ms_per_step": {k: round(v, 2) for k, v in per_step.items()},
         "rays_per_solve": rp["total"],
         "scene_build_ms": round(build_s * 1e3, 1),
+        "autotune": tune,
         "roofline": {"bound": "hbm", "kernel": f"k_{dom}", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(f"k_{dom}"),
                      "algorithmic_bytes_per_launch": round(brays * rays_per_launch),
@@ -429,6 +436,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the config-4 serving leg of the default run")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--serve-scenes", type=int, default=6)
     ap.add_argument("--serve-slots", type=int, default=4)

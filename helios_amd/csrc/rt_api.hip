@@ -250,8 +250,10 @@ int rt_set_launch_cfg(rt_ctx *ctx, int kernel, const rt_launch_cfg *cfg) {
         return -1;
     }
     if (cfg->blocks_per_cu < 1 || cfg->blocks_per_cu > 64) { set_error("blocks_per_cu must be in [1, 64]"); return -1; }
+    if (cfg->chunk > 256) { set_error("chunk must be in [0, 256] (0 = 1)"); return -1; }
     ctx->sx.cfg[kernel].waves_per_block = cfg->waves_per_block;
     ctx->sx.cfg[kernel].blocks_per_cu = cfg->blocks_per_cu;
+    ctx->sx.cfg[kernel].chunk = cfg->chunk ? cfg->chunk : 1;
     return 0;
 }
 
@@ -261,6 +263,7 @@ int rt_get_launch_cfg(rt_ctx *ctx, int kernel, rt_launch_cfg *cfg) {
     if (kernel < 0 || kernel > 2) { set_error("unknown kernel id"); return -1; }
     cfg->waves_per_block = ctx->sx.cfg[kernel].waves_per_block;
     cfg->blocks_per_cu = ctx->sx.cfg[kernel].blocks_per_cu;
+    cfg->chunk = ctx->sx.cfg[kernel].chunk;
     return 0;
 }
 

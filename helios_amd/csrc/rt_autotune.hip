@@ -1,0 +1,190 @@
+// rt_autotune.hip -- launch-shape search for the persistent ray kernels.
+//
+// Candidates (waves per workgroup x workgroups per CU x work-queue chunk) are timed with HIP events on a
+// prefix sample of the caller's scene: a direct pass towards the request's first source, a diffuse pass,
+// and a gather over the resulting hit table with the batch's channel count.  The fastest shape per kernel
+// class is installed on the context.  The sample keeps every launch within tens of milliseconds.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "rt_ctx.h"
+
+using namespace rt;
+
+namespace {
+
+struct Timer {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~Timer() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
+// Runs `launch` twice with `cfg` and returns the second run's time in ms (-1 on failure).
+template <typename F>
+double time_cfg(hipStream_t st, Timer &tm, const LaunchCfg &cfg, F launch) {
+    if (launch(cfg)) return -1.0;
+    if (hipEventRecord(tm.a, st) != hipSuccess) return -1.0;
+    if (launch(cfg)) return -1.0;
+    if (hipEventRecord(tm.b, st) != hipSuccess) return -1.0;
+    if (hipEventSynchronize(tm.b) != hipSuccess) return -1.0;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, tm.a, tm.b) != hipSuccess) return -1.0;
+    return ms;
+}
+
+template <typename F>
+int search(hipStream_t st, Timer &tm, const std::vector<LaunchCfg> &cands, F launch, LaunchCfg &best, double &best_ms,
+           double &default_ms, uint32_t &count) {
+    const LaunchCfg def{};
+    default_ms = time_cfg(st, tm, def, launch);
+    if (default_ms < 0) return -1;
+    best = def;
+    best_ms = default_ms;
+    count++;
+    for (const LaunchCfg &c : cands) {
+        double ms = time_cfg(st, tm, c, launch);
+        if (ms < 0) return -1;
+        count++;
+        if (ms < 0.97 * best_ms) {  // keep the default unless a shape is clearly faster
+            best = c;
+            best_ms = ms;
+        }
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int rt_autotune(rt_ctx *ctx, const rt_scene *scene, const rt_band_request *req, uint32_t nsuns,
+                           rt_tune_result *out) {
+    NN(ctx, "ctx");
+    NN(scene, "scene");
+    NN(req, "request");
+    NN(out, "out");
+    if (req->nbands == 0 || nsuns == 0) { set_error("rt_autotune: no bands or no suns"); return -1; }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    CKR(hipSetDevice(ctx->device));
+    const Scene &S = scene->s;
+    const uint32_t P = S.h.P;
+    if (P == 0) { set_error("rt_autotune: empty scene"); return -1; }
+    const uint32_t sample = std::min<uint32_t>(P, std::max<uint32_t>(65536u, P / 8));
+    const uint32_t C = std::min<uint32_t>(256u, nsuns * req->nbands);
+    const uint32_t d = std::max(1u, (uint32_t)std::ceil(std::sqrt((double)req->direct_rays)));
+    const uint32_t n = std::max(1u, (uint32_t)std::ceil(std::sqrt((double)req->diffuse_rays)));
+    const size_t nn = (size_t)n * n;
+    hipStream_t st = ctx->sx.stream;
+
+    // scratch: direct sums, hit table (sample rows, indexed by primitive), gather buffers
+    size_t bytes = (size_t)P * 2 * 4 + (size_t)sample * 2 * nn * 4 + 2 * (size_t)P * C * 4 + 3 * (size_t)sample * C * 4 +
+                   2 * (size_t)sample * C * 4 + (size_t)C * 8 + 4096 * 4;
+    void *mem = nullptr;
+    CKR(hipMalloc(&mem, bytes));
+    char *p = (char *)mem;
+    auto take = [&](size_t b) {
+        char *r = p;
+        p += (b + 255) & ~(size_t)255;
+        return r;
+    };
+    float *dsum = (float *)take((size_t)P * 2 * 4);
+    uint32_t *hits = (uint32_t *)take((size_t)sample * 2 * nn * 4);
+    float *otop = (float *)take((size_t)P * C * 4), *obot = (float *)take((size_t)P * C * 4);
+    float *in = (float *)take((size_t)sample * C * 4), *stb = (float *)take((size_t)sample * C * 4),
+          *sbb = (float *)take((size_t)sample * C * 4);
+    float *rho = (float *)take((size_t)sample * C * 4), *tau = (float *)take((size_t)sample * C * 4);
+    float *dfl = (float *)take((size_t)C * 4);
+    uint8_t *act = (uint8_t *)take(C);
+    DirectJob *dj = (DirectJob *)take(sizeof(DirectJob));
+    DiffuseJob *fj = (DiffuseJob *)take(sizeof(DiffuseJob));
+    GatherJob *gj = (GatherJob *)take(sizeof(GatherJob));
+    uint32_t *counter = (uint32_t *)take(64);
+    int *err = (int *)take(64);
+
+    int rc = 0;
+    Timer tm;
+    if (hipEventCreate(&tm.a) != hipSuccess || hipEventCreate(&tm.b) != hipSuccess) rc = -1;
+    if (rc == 0) {
+        DirectJob hdj{};
+        hdj.sc = S.d;
+        const float up[3] = {0.3f, 0.2f, 0.93f};
+        const float *sd = (req->nsources && req->source_dir) ? req->source_dir : up;
+        hdj.sdir[0] = sd[0]; hdj.sdir[1] = sd[1]; hdj.sdir[2] = sd[2];
+        hdj.d = d; hdj.seed = 12345u; hdj.out = dsum; hdj.item_begin = 0; hdj.P = P;
+        DiffuseJob hfj{};
+        hfj.sc = S.d; hfj.n = n; hfj.seed = 777u; hfj.hits = hits; hfj.item_begin = 0; hfj.P = P;
+        GatherJob hgj{};
+        hgj.sc = S.d; hgj.hits = hits; hgj.n = n; hgj.C = C; hgj.B = C;
+        hgj.out_top = otop; hgj.out_bot = obot; hgj.dflux = dfl; hgj.active = act; hgj.rho = rho; hgj.tau = tau;
+        hgj.in = in; hgj.st = stb; hgj.sb = sbb; hgj.item_begin = 0; hgj.P = P;
+        std::vector<float> ones((size_t)C, 1.f);
+        std::vector<uint8_t> acts((size_t)C, 1);
+        if (hipMemsetAsync(otop, 0, (size_t)P * C * 4, st) != hipSuccess ||
+            hipMemsetAsync(obot, 0, (size_t)P * C * 4, st) != hipSuccess ||
+            hipMemsetAsync(rho, 0, (size_t)sample * C * 4, st) != hipSuccess ||
+            hipMemsetAsync(tau, 0, (size_t)sample * C * 4, st) != hipSuccess ||
+            hipMemsetAsync(in, 0, (size_t)sample * C * 4, st) != hipSuccess ||
+            hipMemsetAsync(stb, 0, (size_t)sample * C * 4, st) != hipSuccess ||
+            hipMemsetAsync(sbb, 0, (size_t)sample * C * 4, st) != hipSuccess ||
+            hipMemsetAsync(err, 0, 4, st) != hipSuccess ||
+            hipMemcpyAsync(dfl, ones.data(), (size_t)C * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(act, acts.data(), C, hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(dj, &hdj, sizeof(hdj), hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(fj, &hfj, sizeof(hfj), hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(gj, &hgj, sizeof(hgj), hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess) {
+            set_error("rt_autotune: staging failed");
+            rc = -1;
+        }
+    }
+    const uint32_t depth = S.h.depth;
+    const int ncu = ctx->sx.ncu;
+    std::vector<LaunchCfg> trace_c, gather_c;
+    for (uint32_t w : {1u, 2u, 4u})
+        for (uint32_t b : {2u, 4u, 8u, 16u})
+            for (uint32_t c : {1u, 4u}) trace_c.push_back(LaunchCfg{w, b, c});
+    for (uint32_t b : {2u, 4u, 8u, 16u})
+        for (uint32_t c : {1u, 4u, 16u}) gather_c.push_back(LaunchCfg{4u, b, c});
+    LaunchCfg best[3];
+    double best_ms[3] = {0, 0, 0}, def_ms[3] = {0, 0, 0};
+    uint32_t count = 0;
+    if (rc == 0)
+        rc = search(st, tm, trace_c,
+                    [&](const LaunchCfg &c) {
+                        return launch_direct(dj, 1, sample, depth, counter, err, nullptr, c, ncu, st);
+                    },
+                    best[0], best_ms[0], def_ms[0], count);
+    if (rc == 0)
+        rc = search(st, tm, trace_c,
+                    [&](const LaunchCfg &c) {
+                        return launch_diffuse(fj, 1, sample, depth, counter, err, nullptr, c, ncu, st);
+                    },
+                    best[1], best_ms[1], def_ms[1], count);
+    if (rc == 0)
+        rc = search(st, tm, gather_c,
+                    [&](const LaunchCfg &c) { return launch_gather(gj, 1, sample, C, counter, c, ncu, st); },
+                    best[2], best_ms[2], def_ms[2], count);
+    int herr = 0;
+    if (rc == 0 && (hipMemcpy(&herr, err, 4, hipMemcpyDeviceToHost) != hipSuccess || herr != 0)) {
+        set_error("rt_autotune: traversal error code " + std::to_string(herr));
+        rc = -1;
+    }
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(mem);
+    if (rc != 0) return -1;
+    for (int k = 0; k < 3; k++) {
+        ctx->sx.cfg[k] = best[k];
+        out->cfg[k].waves_per_block = best[k].waves_per_block;
+        out->cfg[k].blocks_per_cu = best[k].blocks_per_cu;
+        out->cfg[k].chunk = best[k].chunk;
+        out->ms[k] = best_ms[k];
+        out->default_ms[k] = def_ms[k];
+    }
+    out->candidates = count;
+    out->sample_prims = sample;
+    return 0;
+}

@@ -29,6 +29,7 @@ void scene_free(Scene &S);
 struct LaunchCfg {
     uint32_t waves_per_block = 4;  // 64-lane waves per workgroup
     uint32_t blocks_per_cu = 4;    // persistent workgroups per CU
+    uint32_t chunk = 1;            // work items taken per work-queue fetch
 };
 
 // kernel launchers (rt_trace.hip)

@@ -18,6 +18,8 @@
 //                   980-1080): one thread per ray, CD predicates, face-forwarded normal.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cfloat>
 #include <string>
 
@@ -440,9 +442,9 @@ __device__ __forceinline__ bool packet_occluded(const DevScene &sc, v3f o, v3f d
 // atomic (lane 0 adds 1, the others 0) and lane 0's old value is broadcast as a scalar.  A
 // `if (lane == 0) atomicAdd` + shuffle lets the compiler split the persistent loop into a lane-0
 // loop and a lanes-1..63 loop that re-processes item 0 forever.
-__device__ __forceinline__ uint32_t wave_item(uint32_t *counter) {
+__device__ __forceinline__ uint32_t wave_item(uint32_t *counter, uint32_t chunk) {
     uint32_t lane = threadIdx.x & 63;
-    uint32_t item = atomicAdd(counter, lane == 0 ? 1u : 0u);
+    uint32_t item = atomicAdd(counter, lane == 0 ? chunk : 0u);
     return __builtin_amdgcn_readfirstlane(item);
 }
 
@@ -482,60 +484,62 @@ __device__ __forceinline__ void add_stats(unsigned long long *stats, const TCnt 
 // the packet's visits.  Both variants make the same occlusion decision for every ray.
 template <int WPB, int STK, bool COUNT>
 __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict__ jobs, uint32_t njobs, uint32_t total,
-                                                     uint32_t *counter, int *err, unsigned long long *stats) {
+                                                     uint32_t *counter, uint32_t chunk, int *err,
+                                                     unsigned long long *stats) {
     __shared__ uint32_t wstk_all[COUNT ? STK * WPB * 64 : STK * WPB];
     uint32_t *wstk = COUNT ? wstk_all + threadIdx.x : wstk_all + (threadIdx.x >> 6) * STK;
     const uint32_t lane = threadIdx.x & 63;
     TCnt cnt{0, 0};
     uint32_t nray = 0;
-    while (true) {
-        uint32_t item = wave_item(counter);
-        if (item >= total) break;
-        uint32_t j = find_job(jobs, njobs, item);
-        const DirectJob &J = jobs[j];
-        const DevScene &sc = *J.sc;
-        uint32_t i = item - J.item_begin;
-        uint32_t ty = sc.type[i];
-        float sum_out = 0.f;
-        bool top = true;
-        if (traced_type(ty)) {
-            float4 n4 = sc.nrm[i];
-            v3f n = mk3(n4.x, n4.y, n4.z);
-            uint32_t tw = __float_as_uint(n4.w);
-            v3f dir = nrm3(mk3(J.sdir[0], J.sdir[1], J.sdir[2]));
-            uint32_t d = J.d, Nrays = d * d;
-            float strength = rdiv(fabsf(dot3(n, dir)), (float)Nrays);
-            top = dot3(dir, n) > 0.0f;
-            if (!(tw == 3 || (!top && tw == 0))) {
-                bool patch = ty == 0 || ty == 3;
-                float tm[12];
-                const float *Tp = sc.T + (size_t)i * 16;
+    for (uint32_t first = wave_item(counter, chunk); first < total; first = wave_item(counter, chunk)) {
+        const uint32_t end = min(first + chunk, total);
+        for (uint32_t item = first; item < end; ++item) {
+            uint32_t j = find_job(jobs, njobs, item);
+            const DirectJob &J = jobs[j];
+            const DevScene &sc = *J.sc;
+            uint32_t i = item - J.item_begin;
+            uint32_t ty = sc.type[i];
+            float sum_out = 0.f;
+            bool top = true;
+            if (traced_type(ty)) {
+                float4 n4 = sc.nrm[i];
+                v3f n = mk3(n4.x, n4.y, n4.z);
+                uint32_t tw = __float_as_uint(n4.w);
+                v3f dir = nrm3(mk3(J.sdir[0], J.sdir[1], J.sdir[2]));
+                uint32_t d = J.d, Nrays = d * d;
+                float strength = rdiv(fabsf(dot3(n, dir)), (float)Nrays);
+                top = dot3(dir, n) > 0.0f;
+                if (!(tw == 3 || (!top && tw == 0))) {
+                    bool patch = ty == 0 || ty == 3;
+                    float tm[12];
+                    const float *Tp = sc.T + (size_t)i * 16;
 #pragma unroll
-                for (int q = 0; q < 12; q++) tm[q] = Tp[q];
-                ksum acc;
-                acc.init();
-                for (uint32_t base = 0; base < Nrays; base += 64) {
-                    uint32_t r = base + lane;
-                    bool need = r < Nrays;
-                    v3f o = mk3(0.f, 0.f, 0.f);
-                    if (need) {
-                        float spx, spy;
-                        direct_sample(i, r % d, r / d, d, J.seed, patch, spx, spy);
-                        o = xform3(tm, spx, spy, 0.0f);
-                        nray++;
+                    for (int q = 0; q < 12; q++) tm[q] = Tp[q];
+                    ksum acc;
+                    acc.init();
+                    for (uint32_t base = 0; base < Nrays; base += 64) {
+                        uint32_t r = base + lane;
+                        bool need = r < Nrays;
+                        v3f o = mk3(0.f, 0.f, 0.f);
+                        if (need) {
+                            float spx, spy;
+                            direct_sample(i, r % d, r / d, d, J.seed, patch, spx, spy);
+                            o = xform3(tm, spx, spy, 0.0f);
+                            nray++;
+                        }
+                        bool occl;
+                        if (COUNT) occl = need && occluded<STK>(sc, o, dir, 1e-4f, i, wstk, WPB * 64, err, cnt);
+                        else occl = packet_occluded<STK>(sc, o, dir, 1e-4f, i, need, wstk, err, cnt);
+                        if (need && !occl) acc.add(strength);
                     }
-                    bool occl;
-                    if (COUNT) occl = need && occluded<STK>(sc, o, dir, 1e-4f, i, wstk, WPB * 64, err, cnt);
-                    else occl = packet_occluded<STK>(sc, o, dir, 1e-4f, i, need, wstk, err, cnt);
-                    if (need && !occl) acc.add(strength);
+                    wave_reduce(acc);
+                    sum_out = acc.value();
                 }
-                wave_reduce(acc);
-                sum_out = acc.value();
             }
-        }
-        if (lane == 0) {
-            J.out[(size_t)i * 2 + 0] = top ? sum_out : 0.f;
-            J.out[(size_t)i * 2 + 1] = top ? 0.f : sum_out;
+            if (lane == 0) {
+                J.out[(size_t)i * 2 + 0] = top ? sum_out : 0.f;
+                J.out[(size_t)i * 2 + 1] = top ? 0.f : sum_out;
+            }
         }
     }
     add_stats(stats, cnt, nray);
@@ -546,56 +550,57 @@ __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict
 // ------------------------------------------------------------------------------------------------
 template <int WPB, int STK>
 __global__ __launch_bounds__(WPB * 64) void k_diffuse(const DiffuseJob *__restrict__ jobs, uint32_t njobs,
-                                                      uint32_t total, uint32_t *counter, int *err,
-                                                      unsigned long long *stats) {
+                                                      uint32_t total, uint32_t *counter, uint32_t chunk,
+                                                      int *err, unsigned long long *stats) {
     __shared__ uint32_t stk_all[STK * WPB * 64];
     uint32_t *stk = stk_all + threadIdx.x;
     const uint32_t stride = WPB * 64;
     const uint32_t lane = threadIdx.x & 63;
     TCnt cnt{0, 0};
     uint32_t nray = 0;
-    while (true) {
-        uint32_t item = wave_item(counter);
-        if (item >= total) break;
-        uint32_t j = find_job(jobs, njobs, item);
-        const DiffuseJob &J = jobs[j];
-        const DevScene &sc = *J.sc;
-        uint32_t i = item - J.item_begin;
-        uint32_t n = J.n, nn = n * n;
-        uint32_t *w = J.hits + (size_t)i * 2 * nn;
-        uint32_t ty = sc.type[i];
-        if (!traced_type(ty)) {
-            for (uint32_t r = lane; r < 2 * nn; r += 64) w[r] = RT_NOT_LAUNCHED;
-            continue;
-        }
-        uint32_t tw = __float_as_uint(sc.nrm[i].w);
-        float4 fr = sc.frame[i];
-        bool patch = ty == 0 || ty == 3;
-        uint32_t faces = J.faces ? J.faces : 3u;
-        float tm[12];
-        const float *Tp = sc.T + (size_t)i * 16;
-#pragma unroll
-        for (int q = 0; q < 12; q++) tm[q] = Tp[q];
-        for (uint32_t r = lane; r < 2 * nn; r += 64) {
-            uint32_t slot = r >= nn ? 1u : 0u;  // slot 0 = top launch (face 1), slot 1 = bottom (face 0)
-            uint32_t rr = r - slot * nn;
-            if ((slot == 1 && tw == 0) || !((faces >> slot) & 1u)) { w[r] = RT_NOT_LAUNCHED; continue; }
-            float spx, spy;
-            v3f dl;
-            diffuse_sample(i, rr % n, rr / n, n, J.seed, patch, spx, spy, dl);
-            v3f o = xform3(tm, spx, spy, 0.0f);
-            v3f dir = frame_rotate(dl, fr.x, fr.y, fr.z, fr.w);
-            if (slot == 1 && tw == 1) dir = mk3(-dir.x, -dir.y, -dir.z);
-            bool ex;
-            nray++;
-            uint32_t h = trace_loop<STK>(sc, o, dir, 1e-5f, i, ex, stk, stride, err, cnt);
-            uint32_t word = RT_MISS;
-            if (h != RT_MISS) {
-                float4 hn = sc.nrm[h];
-                bool hit_top = dot3(mk3(hn.x, hn.y, hn.z), dir) < 0.0f;
-                word = h | (hit_top ? 0u : RT_HIT_BOTTOM);
+    for (uint32_t first = wave_item(counter, chunk); first < total; first = wave_item(counter, chunk)) {
+        const uint32_t end = min(first + chunk, total);
+        for (uint32_t item = first; item < end; ++item) {
+            uint32_t j = find_job(jobs, njobs, item);
+            const DiffuseJob &J = jobs[j];
+            const DevScene &sc = *J.sc;
+            uint32_t i = item - J.item_begin;
+            uint32_t n = J.n, nn = n * n;
+            uint32_t *w = J.hits + (size_t)i * 2 * nn;
+            uint32_t ty = sc.type[i];
+            if (!traced_type(ty)) {
+                for (uint32_t r = lane; r < 2 * nn; r += 64) w[r] = RT_NOT_LAUNCHED;
+                continue;
             }
-            w[r] = word;
+            uint32_t tw = __float_as_uint(sc.nrm[i].w);
+            float4 fr = sc.frame[i];
+            bool patch = ty == 0 || ty == 3;
+            uint32_t faces = J.faces ? J.faces : 3u;
+            float tm[12];
+            const float *Tp = sc.T + (size_t)i * 16;
+#pragma unroll
+            for (int q = 0; q < 12; q++) tm[q] = Tp[q];
+            for (uint32_t r = lane; r < 2 * nn; r += 64) {
+                uint32_t slot = r >= nn ? 1u : 0u;  // slot 0 = top launch (face 1), slot 1 = bottom (face 0)
+                uint32_t rr = r - slot * nn;
+                if ((slot == 1 && tw == 0) || !((faces >> slot) & 1u)) { w[r] = RT_NOT_LAUNCHED; continue; }
+                float spx, spy;
+                v3f dl;
+                diffuse_sample(i, rr % n, rr / n, n, J.seed, patch, spx, spy, dl);
+                v3f o = xform3(tm, spx, spy, 0.0f);
+                v3f dir = frame_rotate(dl, fr.x, fr.y, fr.z, fr.w);
+                if (slot == 1 && tw == 1) dir = mk3(-dir.x, -dir.y, -dir.z);
+                bool ex;
+                nray++;
+                uint32_t h = trace_loop<STK>(sc, o, dir, 1e-5f, i, ex, stk, stride, err, cnt);
+                uint32_t word = RT_MISS;
+                if (h != RT_MISS) {
+                    float4 hn = sc.nrm[h];
+                    bool hit_top = dot3(mk3(hn.x, hn.y, hn.z), dir) < 0.0f;
+                    word = h | (hit_top ? 0u : RT_HIT_BOTTOM);
+                }
+                w[r] = word;
+            }
         }
     }
     add_stats(stats, cnt, nray);
@@ -606,66 +611,67 @@ __global__ __launch_bounds__(WPB * 64) void k_diffuse(const DiffuseJob *__restri
 // ------------------------------------------------------------------------------------------------
 template <int CM>
 __global__ __launch_bounds__(256) void k_gather_rays(const GatherJob *__restrict__ jobs, uint32_t njobs, uint32_t total,
-                                                     uint32_t *counter) {
+                                                     uint32_t *counter, uint32_t chunk) {
     const uint32_t lane = threadIdx.x & 63;
-    while (true) {
-        uint32_t item = wave_item(counter);
-        if (item >= total) break;
-        uint32_t j = find_job(jobs, njobs, item);
-        const GatherJob &J = jobs[j];
-        uint32_t i = item - J.item_begin;
-        uint32_t ty = J.sc->type[i];
-        if (!traced_type(ty)) continue;
-        uint32_t nn = J.n * J.n, C = J.C;
-        const uint32_t *w = J.hits + (size_t)i * 2 * nn;
-        ksum acc[2][CM];
+    for (uint32_t first = wave_item(counter, chunk); first < total; first = wave_item(counter, chunk)) {
+        const uint32_t end = min(first + chunk, total);
+        for (uint32_t item = first; item < end; ++item) {
+            uint32_t j = find_job(jobs, njobs, item);
+            const GatherJob &J = jobs[j];
+            uint32_t i = item - J.item_begin;
+            uint32_t ty = J.sc->type[i];
+            if (!traced_type(ty)) continue;
+            uint32_t nn = J.n * J.n, C = J.C;
+            const uint32_t *w = J.hits + (size_t)i * 2 * nn;
+            ksum acc[2][CM];
 #pragma unroll
-        for (int f = 0; f < 2; f++)
+            for (int f = 0; f < 2; f++)
 #pragma unroll
-            for (int c = 0; c < CM; c++) acc[f][c].init();
-        for (uint32_t r = lane; r < 2 * nn; r += 64) {
-            uint32_t word = w[r];
-            if (word == RT_NOT_LAUNCHED) continue;
-            uint32_t slot = r >= nn ? 1u : 0u;
-            if (word == RT_MISS) {
+                for (int c = 0; c < CM; c++) acc[f][c].init();
+            for (uint32_t r = lane; r < 2 * nn; r += 64) {
+                uint32_t word = w[r];
+                if (word == RT_NOT_LAUNCHED) continue;
+                uint32_t slot = r >= nn ? 1u : 0u;
+                if (word == RT_MISS) {
 #pragma unroll
-                for (int c = 0; c < CM; c++)
-                    if ((uint32_t)c < C) {
-                        float v = J.dflux[c];
-                        if (slot) acc[1][c].add(v); else acc[0][c].add(v);
-                    }
-            } else {
-                uint32_t h = word & ~RT_HIT_BOTTOM;
-                const float *src = ((word & RT_HIT_BOTTOM) ? J.out_bot : J.out_top) + (size_t)h * C;
+                    for (int c = 0; c < CM; c++)
+                        if ((uint32_t)c < C) {
+                            float v = J.dflux[c];
+                            if (slot) acc[1][c].add(v); else acc[0][c].add(v);
+                        }
+                } else {
+                    uint32_t h = word & ~RT_HIT_BOTTOM;
+                    const float *src = ((word & RT_HIT_BOTTOM) ? J.out_bot : J.out_top) + (size_t)h * C;
 #pragma unroll
-                for (int c = 0; c < CM; c++)
-                    if ((uint32_t)c < C) {
-                        float v = src[c];
-                        if (slot) acc[1][c].add(v); else acc[0][c].add(v);
-                    }
+                    for (int c = 0; c < CM; c++)
+                        if ((uint32_t)c < C) {
+                            float v = src[c];
+                            if (slot) acc[1][c].add(v); else acc[0][c].add(v);
+                        }
+                }
             }
-        }
-        float strength = rdiv(1.0f, (float)nn);
+            float strength = rdiv(1.0f, (float)nn);
 #pragma unroll
-        for (int c = 0; c < CM; c++) {
-            wave_reduce(acc[0][c]);
-            wave_reduce(acc[1][c]);
-        }
-        if (lane < C && lane < CM) {
-            float I1 = 0.f, I0 = 0.f;
+            for (int c = 0; c < CM; c++) {
+                wave_reduce(acc[0][c]);
+                wave_reduce(acc[1][c]);
+            }
+            if (lane < C && lane < CM) {
+                float I1 = 0.f, I0 = 0.f;
 #pragma unroll
-            for (int c = 0; c < CM; c++)
-                if ((uint32_t)c == lane) { I1 = acc[0][c].value(); I0 = acc[1][c].value(); }
-            uint32_t c = lane;
-            if (J.active[c]) {
-                I1 = I1 * strength;
-                I0 = I0 * strength;
-                uint32_t b = c % J.B;
-                float rho = J.rho[(size_t)i * J.B + b], tau = J.tau[(size_t)i * J.B + b];
-                size_t q = (size_t)i * C + c;
-                J.in[q] += (I1 + I0) * (1.0f - rho - tau);
-                J.st[q] += I1 * rho + I0 * tau;
-                J.sb[q] += I1 * tau + I0 * rho;
+                for (int c = 0; c < CM; c++)
+                    if ((uint32_t)c == lane) { I1 = acc[0][c].value(); I0 = acc[1][c].value(); }
+                uint32_t c = lane;
+                if (J.active[c]) {
+                    I1 = I1 * strength;
+                    I0 = I0 * strength;
+                    uint32_t b = c % J.B;
+                    float rho = J.rho[(size_t)i * J.B + b], tau = J.tau[(size_t)i * J.B + b];
+                    size_t q = (size_t)i * C + c;
+                    J.in[q] += (I1 + I0) * (1.0f - rho - tau);
+                    J.st[q] += I1 * rho + I0 * tau;
+                    J.sb[q] += I1 * tau + I0 * rho;
+                }
             }
         }
     }
@@ -676,63 +682,64 @@ __global__ __launch_bounds__(256) void k_gather_rays(const GatherJob *__restrict
 // ------------------------------------------------------------------------------------------------
 template <int CB>
 __global__ __launch_bounds__(256) void k_gather_chan(const GatherJob *__restrict__ jobs, uint32_t njobs, uint32_t total,
-                                                     uint32_t *counter) {
+                                                     uint32_t *counter, uint32_t chunk) {
     const uint32_t lane = threadIdx.x & 63;
-    while (true) {
-        uint32_t item = wave_item(counter);
-        if (item >= total) break;
-        uint32_t j = find_job(jobs, njobs, item);
-        const GatherJob &J = jobs[j];
-        uint32_t i = item - J.item_begin;
-        uint32_t ty = J.sc->type[i];
-        if (!traced_type(ty)) continue;
-        uint32_t nn = J.n * J.n, C = J.C;
-        const uint32_t *w = J.hits + (size_t)i * 2 * nn;
-        ksum acc[2][CB];
-        float sky[CB];
+    for (uint32_t first = wave_item(counter, chunk); first < total; first = wave_item(counter, chunk)) {
+        const uint32_t end = min(first + chunk, total);
+        for (uint32_t item = first; item < end; ++item) {
+            uint32_t j = find_job(jobs, njobs, item);
+            const GatherJob &J = jobs[j];
+            uint32_t i = item - J.item_begin;
+            uint32_t ty = J.sc->type[i];
+            if (!traced_type(ty)) continue;
+            uint32_t nn = J.n * J.n, C = J.C;
+            const uint32_t *w = J.hits + (size_t)i * 2 * nn;
+            ksum acc[2][CB];
+            float sky[CB];
 #pragma unroll
-        for (int k = 0; k < CB; k++) {
-            acc[0][k].init();
-            acc[1][k].init();
-            uint32_t c = lane + 64 * k;
-            sky[k] = c < C ? J.dflux[c] : 0.f;
-        }
-        for (uint32_t base = 0; base < 2 * nn; base += 64) {
-            uint32_t mine = base + lane < 2 * nn ? w[base + lane] : RT_NOT_LAUNCHED;
-            uint32_t cnt = min(64u, 2 * nn - base);
-            for (uint32_t q = 0; q < cnt; q++) {
-                uint32_t word = __builtin_amdgcn_readlane(mine, q);
-                if (word == RT_NOT_LAUNCHED) continue;
-                uint32_t slot = base + q >= nn ? 1u : 0u;
-                if (word == RT_MISS) {
+            for (int k = 0; k < CB; k++) {
+                acc[0][k].init();
+                acc[1][k].init();
+                uint32_t c = lane + 64 * k;
+                sky[k] = c < C ? J.dflux[c] : 0.f;
+            }
+            for (uint32_t base = 0; base < 2 * nn; base += 64) {
+                uint32_t mine = base + lane < 2 * nn ? w[base + lane] : RT_NOT_LAUNCHED;
+                uint32_t cnt = min(64u, 2 * nn - base);
+                for (uint32_t q = 0; q < cnt; q++) {
+                    uint32_t word = __builtin_amdgcn_readlane(mine, q);
+                    if (word == RT_NOT_LAUNCHED) continue;
+                    uint32_t slot = base + q >= nn ? 1u : 0u;
+                    if (word == RT_MISS) {
 #pragma unroll
-                    for (int k = 0; k < CB; k++) {
-                        if (slot) acc[1][k].add(sky[k]); else acc[0][k].add(sky[k]);
-                    }
-                } else {
-                    uint32_t h = word & ~RT_HIT_BOTTOM;
-                    const float *src = ((word & RT_HIT_BOTTOM) ? J.out_bot : J.out_top) + (size_t)h * C;
+                        for (int k = 0; k < CB; k++) {
+                            if (slot) acc[1][k].add(sky[k]); else acc[0][k].add(sky[k]);
+                        }
+                    } else {
+                        uint32_t h = word & ~RT_HIT_BOTTOM;
+                        const float *src = ((word & RT_HIT_BOTTOM) ? J.out_bot : J.out_top) + (size_t)h * C;
 #pragma unroll
-                    for (int k = 0; k < CB; k++) {
-                        uint32_t c = lane + 64 * k;
-                        float v = c < C ? src[c] : 0.f;
-                        if (slot) acc[1][k].add(v); else acc[0][k].add(v);
+                        for (int k = 0; k < CB; k++) {
+                            uint32_t c = lane + 64 * k;
+                            float v = c < C ? src[c] : 0.f;
+                            if (slot) acc[1][k].add(v); else acc[0][k].add(v);
+                        }
                     }
                 }
             }
-        }
-        float strength = rdiv(1.0f, (float)nn);
+            float strength = rdiv(1.0f, (float)nn);
 #pragma unroll
-        for (int k = 0; k < CB; k++) {
-            uint32_t c = lane + 64 * k;
-            if (c >= C || !J.active[c]) continue;
-            float I1 = acc[0][k].value() * strength, I0 = acc[1][k].value() * strength;
-            uint32_t b = c % J.B;
-            float rho = J.rho[(size_t)i * J.B + b], tau = J.tau[(size_t)i * J.B + b];
-            size_t qq = (size_t)i * C + c;
-            J.in[qq] += (I1 + I0) * (1.0f - rho - tau);
-            J.st[qq] += I1 * rho + I0 * tau;
-            J.sb[qq] += I1 * tau + I0 * rho;
+            for (int k = 0; k < CB; k++) {
+                uint32_t c = lane + 64 * k;
+                if (c >= C || !J.active[c]) continue;
+                float I1 = acc[0][k].value() * strength, I0 = acc[1][k].value() * strength;
+                uint32_t b = c % J.B;
+                float rho = J.rho[(size_t)i * J.B + b], tau = J.tau[(size_t)i * J.B + b];
+                size_t qq = (size_t)i * C + c;
+                J.in[qq] += (I1 + I0) * (1.0f - rho - tau);
+                J.st[qq] += I1 * rho + I0 * tau;
+                J.sb[qq] += I1 * tau + I0 * rho;
+            }
         }
     }
 }
@@ -863,7 +870,8 @@ int launch_debug_trace(const DevScene *sc, uint32_t n, const float *rays, const 
 // ------------------------------------------------------------------------------------------------
 static inline uint32_t grid_for(const LaunchCfg &cfg, int ncu, uint32_t total, uint32_t wpb) {
     uint64_t g = (uint64_t)cfg.blocks_per_cu * (uint64_t)ncu;
-    uint64_t need = (total + wpb - 1) / wpb;
+    uint64_t per_block = (uint64_t)wpb * std::max(cfg.chunk, 1u);
+    uint64_t need = (total + per_block - 1) / per_block;
     if (g > need) g = need;
     if (g < 1) g = 1;
     return (uint32_t)g;
@@ -888,23 +896,24 @@ static inline uint32_t grid_for(const LaunchCfg &cfg, int ncu, uint32_t total, u
 int launch_direct(const DirectJob *jobs, uint32_t njobs, uint32_t total, uint32_t depth, uint32_t *counter, int *err,
                   unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st) {
     if (total == 0) return 0;
+    const uint32_t ch = std::max(cfg.chunk, 1u);
     k_zero_u32<<<1, 1, 0, st>>>(counter);
     uint32_t wpb = cfg.waves_per_block;
     uint32_t g = grid_for(cfg, ncu, total, wpb);
     if (stats) {
         // counting variant: per-lane stacks, 4 waves per block
-        PICK_STK(depth, (k_direct<4, 32, true><<<g, 256, 0, st>>>(jobs, njobs, total, counter, err, stats)),
-                 (k_direct<4, 64, true><<<g, 256, 0, st>>>(jobs, njobs, total, counter, err, stats)));
+        PICK_STK(depth, (k_direct<4, 32, true><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)),
+                 (k_direct<4, 64, true><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)));
         LCHECK();
         return 0;
     }
     switch (wpb) {
-        case 1: PICK_STK(depth, (k_direct<1, 32, false><<<g, 64, 0, st>>>(jobs, njobs, total, counter, err, stats)),
-                         (k_direct<1, 64, false><<<g, 64, 0, st>>>(jobs, njobs, total, counter, err, stats))); break;
-        case 2: PICK_STK(depth, (k_direct<2, 32, false><<<g, 128, 0, st>>>(jobs, njobs, total, counter, err, stats)),
-                         (k_direct<2, 64, false><<<g, 128, 0, st>>>(jobs, njobs, total, counter, err, stats))); break;
-        default: PICK_STK(depth, (k_direct<4, 32, false><<<g, 256, 0, st>>>(jobs, njobs, total, counter, err, stats)),
-                          (k_direct<4, 64, false><<<g, 256, 0, st>>>(jobs, njobs, total, counter, err, stats))); break;
+        case 1: PICK_STK(depth, (k_direct<1, 32, false><<<g, 64, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)),
+                         (k_direct<1, 64, false><<<g, 64, 0, st>>>(jobs, njobs, total, counter, ch, err, stats))); break;
+        case 2: PICK_STK(depth, (k_direct<2, 32, false><<<g, 128, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)),
+                         (k_direct<2, 64, false><<<g, 128, 0, st>>>(jobs, njobs, total, counter, ch, err, stats))); break;
+        default: PICK_STK(depth, (k_direct<4, 32, false><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)),
+                          (k_direct<4, 64, false><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch, err, stats))); break;
     }
     LCHECK();
     return 0;
@@ -913,16 +922,17 @@ int launch_direct(const DirectJob *jobs, uint32_t njobs, uint32_t total, uint32_
 int launch_diffuse(const DiffuseJob *jobs, uint32_t njobs, uint32_t total, uint32_t depth, uint32_t *counter, int *err,
                    unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st) {
     if (total == 0) return 0;
+    const uint32_t ch = std::max(cfg.chunk, 1u);
     k_zero_u32<<<1, 1, 0, st>>>(counter);
     uint32_t wpb = cfg.waves_per_block;
     uint32_t g = grid_for(cfg, ncu, total, wpb);
     switch (wpb) {
-        case 1: PICK_STK(depth, (k_diffuse<1, 32><<<g, 64, 0, st>>>(jobs, njobs, total, counter, err, stats)),
-                         (k_diffuse<1, 64><<<g, 64, 0, st>>>(jobs, njobs, total, counter, err, stats))); break;
-        case 2: PICK_STK(depth, (k_diffuse<2, 32><<<g, 128, 0, st>>>(jobs, njobs, total, counter, err, stats)),
-                         (k_diffuse<2, 64><<<g, 128, 0, st>>>(jobs, njobs, total, counter, err, stats))); break;
-        default: PICK_STK(depth, (k_diffuse<4, 32><<<g, 256, 0, st>>>(jobs, njobs, total, counter, err, stats)),
-                          (k_diffuse<4, 64><<<g, 256, 0, st>>>(jobs, njobs, total, counter, err, stats))); break;
+        case 1: PICK_STK(depth, (k_diffuse<1, 32><<<g, 64, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)),
+                         (k_diffuse<1, 64><<<g, 64, 0, st>>>(jobs, njobs, total, counter, ch, err, stats))); break;
+        case 2: PICK_STK(depth, (k_diffuse<2, 32><<<g, 128, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)),
+                         (k_diffuse<2, 64><<<g, 128, 0, st>>>(jobs, njobs, total, counter, ch, err, stats))); break;
+        default: PICK_STK(depth, (k_diffuse<4, 32><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)),
+                          (k_diffuse<4, 64><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch, err, stats))); break;
     }
     LCHECK();
     return 0;
@@ -931,16 +941,17 @@ int launch_diffuse(const DiffuseJob *jobs, uint32_t njobs, uint32_t total, uint3
 int launch_gather(const GatherJob *jobs, uint32_t njobs, uint32_t total, uint32_t cmax, uint32_t *counter,
                   const LaunchCfg &cfg, int ncu, hipStream_t st) {
     if (total == 0) return 0;
+    const uint32_t ch = std::max(cfg.chunk, 1u);
     k_zero_u32<<<1, 1, 0, st>>>(counter);
     uint32_t g = grid_for(cfg, ncu, total, 4);
-    if (cmax <= 1) k_gather_rays<1><<<g, 256, 0, st>>>(jobs, njobs, total, counter);
-    else if (cmax <= 2) k_gather_rays<2><<<g, 256, 0, st>>>(jobs, njobs, total, counter);
-    else if (cmax <= 4) k_gather_rays<4><<<g, 256, 0, st>>>(jobs, njobs, total, counter);
-    else if (cmax <= 8) k_gather_rays<8><<<g, 256, 0, st>>>(jobs, njobs, total, counter);
-    else if (cmax <= 64) k_gather_chan<1><<<g, 256, 0, st>>>(jobs, njobs, total, counter);
-    else if (cmax <= 128) k_gather_chan<2><<<g, 256, 0, st>>>(jobs, njobs, total, counter);
-    else if (cmax <= 192) k_gather_chan<3><<<g, 256, 0, st>>>(jobs, njobs, total, counter);
-    else if (cmax <= 256) k_gather_chan<4><<<g, 256, 0, st>>>(jobs, njobs, total, counter);
+    if (cmax <= 1) k_gather_rays<1><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch);
+    else if (cmax <= 2) k_gather_rays<2><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch);
+    else if (cmax <= 4) k_gather_rays<4><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch);
+    else if (cmax <= 8) k_gather_rays<8><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch);
+    else if (cmax <= 64) k_gather_chan<1><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch);
+    else if (cmax <= 128) k_gather_chan<2><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch);
+    else if (cmax <= 192) k_gather_chan<3><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch);
+    else if (cmax <= 256) k_gather_chan<4><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch);
     else {
         set_error("more than 256 channels in one gather");
         return -1;

@@ -47,7 +47,12 @@ class rt_scene_info(C.Structure):
 
 
 class rt_launch_cfg(C.Structure):
-    _fields_ = [("waves_per_block", C.c_uint32), ("blocks_per_cu", C.c_uint32)]
+    _fields_ = [("waves_per_block", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("chunk", C.c_uint32)]
+
+
+class rt_tune_result(C.Structure):
+    _fields_ = [("cfg", rt_launch_cfg * 3), ("ms", C.c_double * 3), ("default_ms", C.c_double * 3),
+                ("candidates", C.c_uint32), ("sample_prims", C.c_uint32)]
 
 
 class rt_timing(C.Structure):
@@ -128,6 +133,8 @@ SIGNATURES = {
     "rt_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(rt_server_stats)]),
     "rt_server_reset_stats": (C.c_int, [C.c_void_p]),
     "rt_solve_footprint": (C.c_int, [C.c_uint32, C.POINTER(rt_band_request), C.c_uint32, C.POINTER(C.c_uint64)]),
+    "rt_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(rt_band_request), C.c_uint32,
+                              C.POINTER(rt_tune_result)]),
 }
 
 _lib = None
@@ -214,14 +221,25 @@ class Context:
     def scene(self, transforms, types, twosided, uuids=None, periodic=(False, False)) -> "Scene":
         return Scene(self, transforms, types, twosided, uuids, periodic)
 
-    def set_launch_cfg(self, kernel: int, waves_per_block: int, blocks_per_cu: int):
-        cfg = rt_launch_cfg(waves_per_block, blocks_per_cu)
+    def set_launch_cfg(self, kernel: int, waves_per_block: int, blocks_per_cu: int, chunk: int = 1):
+        cfg = rt_launch_cfg(waves_per_block, blocks_per_cu, chunk)
         _check(self.lib.rt_set_launch_cfg(self._h, kernel, C.byref(cfg)))
 
     def get_launch_cfg(self, kernel: int):
         cfg = rt_launch_cfg()
         _check(self.lib.rt_get_launch_cfg(self._h, kernel, C.byref(cfg)))
-        return cfg.waves_per_block, cfg.blocks_per_cu
+        return cfg.waves_per_block, cfg.blocks_per_cu, cfg.chunk
+
+    def autotune(self, scene: "Scene", req: "BandRequest", nsuns: int = 1) -> dict:
+        """Time candidate launch shapes on a sample of `scene` and install the fastest per kernel class."""
+        r = rt_tune_result()
+        cs = req.c_struct()
+        _check(self.lib.rt_autotune(self._h, scene._h, C.byref(cs), nsuns, C.byref(r)))
+        names = ("direct", "diffuse", "gather")
+        return {"candidates": r.candidates, "sample_prims": r.sample_prims,
+                **{names[k]: {"waves_per_block": r.cfg[k].waves_per_block, "blocks_per_cu": r.cfg[k].blocks_per_cu,
+                              "chunk": r.cfg[k].chunk, "ms": round(r.ms[k], 3),
+                              "default_ms": round(r.default_ms[k], 3)} for k in range(3)}}
 
     def enable_timing(self, on: bool = True):
         _check(self.lib.rt_enable_timing(self._h, 1 if on else 0))

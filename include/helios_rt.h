@@ -243,18 +243,31 @@ int rt_solve_footprint(uint32_t nprims, const rt_band_request *req, uint32_t nsu
 int rt_server_reset_stats(rt_server *srv);
 
 /* ---------------------------------------------------------------------------------------------
- * Launch tuning: rays-per-wave grouping, persistent workgroups per CU and waves per workgroup are
- * picked by rt_autotune (timing candidate launch shapes on the given scene) or set by hand.
+ * Launch tuning (no reference counterpart): waves per workgroup, persistent workgroups per CU and
+ * work-queue chunking (primitives taken per queue fetch) of the direct, diffuse and gather kernels,
+ * set by hand or picked by rt_autotune, which times candidate shapes on a sample of the caller's
+ * own scene and request.  The traversal stack depth is picked per launch from the BVH depth.
  * ------------------------------------------------------------------------------------------- */
 typedef struct {
-    uint32_t waves_per_block;   /* 1, 2 or 4 */
-    uint32_t blocks_per_cu;     /* persistent workgroups per CU */
+    uint32_t waves_per_block;   /* 1, 2 or 4 (the gather kernels always use 4)  */
+    uint32_t blocks_per_cu;     /* persistent workgroups per CU, 1..64          */
+    uint32_t chunk;             /* work items per queue fetch, 0 or 1 = one     */
 } rt_launch_cfg;
 #define RT_KERNEL_DIRECT 0
 #define RT_KERNEL_DIFFUSE 1
 #define RT_KERNEL_GATHER 2
 int rt_set_launch_cfg(rt_ctx *ctx, int kernel, const rt_launch_cfg *cfg);
 int rt_get_launch_cfg(rt_ctx *ctx, int kernel, rt_launch_cfg *cfg);
+typedef struct {
+    rt_launch_cfg cfg[3];   /* chosen shape per kernel class (also installed on the context)      */
+    double ms[3];           /* sample time with the chosen shape                                   */
+    double default_ms[3];   /* sample time with the default shape (4 waves, 4 blocks/CU, chunk 1) */
+    uint32_t candidates;    /* shapes timed                                                        */
+    uint32_t sample_prims;  /* primitives in the timing sample                                     */
+} rt_tune_result;
+/* Times candidate shapes for the direct, diffuse and gather kernels on a sample of `scene` with the
+ * ray counts, first source direction and band count of `req`, for batches of `nsuns` suns. */
+int rt_autotune(rt_ctx *ctx, const rt_scene *scene, const rt_band_request *req, uint32_t nsuns, rt_tune_result *out);
 
 /* ---------------------------------------------------------------------------------------------
  * Timing of the last solve's kernels (HIP events on the context stream), milliseconds.
