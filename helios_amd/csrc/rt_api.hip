@@ -114,6 +114,8 @@ int rt_scene_get_info(const rt_scene *scene, rt_scene_info *info) {
     info->nbbox = scene->s.h.nbbox;
     for (int k = 0; k < 4; k++) info->domain[k] = scene->s.h.dom[k];
     info->device_bytes = scene->s.bytes;
+    info->builder = (uint32_t)scene->s.builder;
+    info->depth4 = scene->s.h.node4 ? scene->s.h.depth4 : 0;
     return 0;
 }
 

@@ -103,7 +103,10 @@ RT_HD void sincos_2pi(float r, float &s, float &c) {
 #define RT_PATCH_BIT 0x80000000u
 
 // Radiation predicates (intersections.glsl) and CD predicates share the arithmetic; only the
-// acceptance thresholds differ.
+// acceptance thresholds differ.  Written as predicates (every quantity computed, the tests combined at the
+// end) rather than early returns: same floating-point operations and decisions, but no divergent control
+// flow, which on a wave costs exec-mask bookkeeping per test.  t_out is written in every case and is
+// meaningful only when the function returns true.
 template <bool CD>
 RT_HD bool isect_rec(const float4 r0, const float4 r1, const float4 r2, const float4 r3, bool patch, v3f o, v3f d,
                      float &t_out) {
@@ -113,40 +116,33 @@ RT_HD bool isect_rec(const float4 r0, const float4 r1, const float4 r2, const fl
     if (patch) {
         v3f n = mk3(r3.x, r3.y, r3.z);
         float denom = dot3(d, n);
-        if (CD) { if (!(fabsf(denom) > 1e-8f)) return false; }
-        else { if (fabsf(denom) < 1e-8f) return false; }
+        bool ok = CD ? (fabsf(denom) > 1e-8f) : !(fabsf(denom) < 1e-8f);
         float t = rdiv(dot3(sub3(v0, o), n), denom);
-        if (CD) { if (!(t > 1e-8f && t < 1e8f)) return false; }
-        else { if (t < 1e-8f || t > 1e8f) return false; }
+        ok = ok && (CD ? (t > 1e-8f && t < 1e8f) : !(t < 1e-8f || t > 1e8f));
         v3f p = add3(o, mul3(d, t));
         v3f dd = sub3(p, v0);
         float da = dot3(dd, e1), db = dot3(dd, e2);
         bool in;
         if (CD) in = (da >= -1e-8f && da <= r1.w + 1e-8f && db >= -1e-8f && db <= r2.w + 1e-8f);
         else in = (da > 0.0f && da < r1.w && db > 0.0f && db < r2.w);
-        if (!in) return false;
         t_out = t;
-        return true;
+        return ok && in;
     } else {
         const float EPS = CD ? 1e-5f : 1e-8f;
         v3f h = cross3(d, e2);
         float a = dot3(e1, h);
-        if (CD) { if (a > -EPS && a < EPS) return false; }
-        else { if (fabsf(a) < EPS) return false; }
+        bool ok = CD ? !(a > -EPS && a < EPS) : !(fabsf(a) < EPS);
         float f = rdiv(1.0f, a);
         v3f s = sub3(o, v0);
         float u = f * dot3(s, h);
-        if (CD) { if (u < -EPS || u > 1.0f + EPS) return false; }
-        else { if (u < 0.0f || u > 1.0f) return false; }
+        ok = ok && (CD ? !(u < -EPS || u > 1.0f + EPS) : !(u < 0.0f || u > 1.0f));
         v3f q = cross3(s, e1);
         float v = f * dot3(d, q);
-        if (CD) { if (v < -EPS || u + v > 1.0f + EPS) return false; }
-        else { if (v < 0.0f || u + v > 1.0f) return false; }
+        ok = ok && (CD ? !(v < -EPS || u + v > 1.0f + EPS) : !(v < 0.0f || u + v > 1.0f));
         float t = f * dot3(e2, q);
-        if (CD) { if (!(t > EPS)) return false; }
-        else { if (!(t > 1e-8f && t < 1e8f)) return false; }
+        ok = ok && (CD ? (t > EPS) : (t > 1e-8f && t < 1e8f));
         t_out = t;
-        return true;
+        return ok;
     }
 }
 

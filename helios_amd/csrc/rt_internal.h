@@ -19,6 +19,7 @@ struct Scene {
     uint32_t *twos = nullptr;
     std::vector<void *> own;
     size_t bytes = 0;
+    int builder = 0;  // 1 = PLOC, 0 = Karras LBVH
 };
 
 int scene_build(Scene &S, uint32_t P, const float *T, const uint32_t *type, const uint32_t *uuid,

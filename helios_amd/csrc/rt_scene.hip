@@ -246,6 +246,176 @@ __global__ void k_refit(int n, const uint32_t *__restrict__ child, const uint32_
     }
 }
 
+// ------------------------------------------------------------------------------------------------
+// PLOC: parallel locally-ordered clustering (Meister & Bittner, IEEE TVCG 2018).  Clusters start as the
+// leaves in Morton order.  Each round, every cluster finds the neighbour within +-PLOC_R positions whose
+// union box has the smallest surface area (ties to the lower position); mutual nearest neighbours merge
+// into a new internal node kept at the lower position; the array is compacted with a prefix sum.  The
+// minimal pair under (area, positions) is always mutual, so every round merges at least once.  Internal
+// nodes are numbered from the top (the last merge, the root, is node 0), and ranks come from the prefix
+// sum, so the tree is the same on every run.
+// ------------------------------------------------------------------------------------------------
+#define PLOC_R 16
+#define PLOC_BLOCK 256
+
+__device__ __forceinline__ float half_area(const float a[6]) {
+    float dx = a[3] - a[0], dy = a[4] - a[1], dz = a[5] - a[2];
+    return dx * dy + dy * dz + dz * dx;
+}
+
+__global__ void k_ploc_init(uint32_t n, const float *__restrict__ lbox, float *__restrict__ cbox,
+                            uint32_t *__restrict__ cid, uint32_t *__restrict__ ch) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    for (int q = 0; q < 6; q++) cbox[(size_t)k * 6 + q] = lbox[(size_t)k * 6 + q];
+    cid[k] = RT_LEAF_BIT | k;
+    ch[k] = 0;
+}
+
+__global__ __launch_bounds__(PLOC_BLOCK) void k_ploc_nn(uint32_t n, const float *__restrict__ cbox,
+                                                        uint32_t *__restrict__ nn) {
+    __shared__ float sb[(PLOC_BLOCK + 2 * PLOC_R) * 6];
+    const int base = (int)(blockIdx.x * PLOC_BLOCK) - PLOC_R;
+    for (int t = threadIdx.x; t < (PLOC_BLOCK + 2 * PLOC_R) * 6; t += PLOC_BLOCK) {
+        int j = base + t / 6;
+        sb[t] = (j >= 0 && j < (int)n) ? cbox[(size_t)j * 6 + t % 6] : 0.f;
+    }
+    __syncthreads();
+    const int i = (int)(blockIdx.x * PLOC_BLOCK + threadIdx.x);
+    if (i >= (int)n) return;
+    const float *bi = sb + (threadIdx.x + PLOC_R) * 6;
+    uint64_t best = ~0ull;
+    const int lo = max(0, i - PLOC_R), hi = min((int)n - 1, i + PLOC_R);
+    for (int j = lo; j <= hi; j++) {
+        if (j == i) continue;
+        const float *bj = sb + (j - base) * 6;
+        float u[6];
+        for (int a = 0; a < 3; a++) { u[a] = fminf(bi[a], bj[a]); u[3 + a] = fmaxf(bi[3 + a], bj[3 + a]); }
+        uint64_t key = ((uint64_t)__float_as_uint(half_area(u)) << 32) | (uint32_t)j;
+        best = key < best ? key : best;
+    }
+    nn[i] = (uint32_t)best;
+}
+
+// flags: high word = merges here (mutual pair, lower position), low word = cluster survives
+__global__ void k_ploc_flags(uint32_t n, const uint32_t *__restrict__ nn, uint64_t *__restrict__ flags) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t j = nn[i];
+    bool mutual = nn[j] == i;
+    uint64_t merge = (mutual && i < j) ? 1ull : 0ull;
+    uint64_t keep = (mutual && i > j) ? 0ull : 1ull;
+    flags[i] = (merge << 32) | keep;
+}
+
+__global__ void k_ploc_emit(uint32_t n, uint32_t node_base, const uint32_t *__restrict__ nn,
+                            const uint64_t *__restrict__ flags, const uint64_t *__restrict__ pref,
+                            const float *__restrict__ cbox, const uint32_t *__restrict__ cid,
+                            const uint32_t *__restrict__ ch, float *__restrict__ obox, uint32_t *__restrict__ oid,
+                            uint32_t *__restrict__ oh, uint32_t *__restrict__ child, float *__restrict__ ibox) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t f = flags[i];
+    if (!(f & 1ull)) return;
+    uint32_t pos = (uint32_t)(pref[i] & 0xFFFFFFFFull);
+    const float *bi = cbox + (size_t)i * 6;
+    if (f >> 32) {
+        uint32_t j = nn[i];
+        uint32_t node = node_base - 1u - (uint32_t)(pref[i] >> 32);
+        const float *bj = cbox + (size_t)j * 6;
+        float u[6];
+        for (int a = 0; a < 3; a++) { u[a] = fminf(bi[a], bj[a]); u[3 + a] = fmaxf(bi[3 + a], bj[3 + a]); }
+        child[2 * (size_t)node] = cid[i];
+        child[2 * (size_t)node + 1] = cid[j];
+        for (int q = 0; q < 6; q++) { ibox[(size_t)node * 6 + q] = u[q]; obox[(size_t)pos * 6 + q] = u[q]; }
+        oid[pos] = node;
+        oh[pos] = 1u + max(ch[i], ch[j]);
+    } else {
+        for (int q = 0; q < 6; q++) obox[(size_t)pos * 6 + q] = bi[q];
+        oid[pos] = cid[i];
+        oh[pos] = ch[i];
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// BVH4 collapse of the binary tree, level by level from the root.  A 4-wide node takes its binary
+// node's two children and repeatedly opens the internal child with the largest surface area until it
+// has four children or only leaves.  Nodes are numbered breadth-first (the root is 0).
+// Node layout (8 float4, 128 B): lo.x[4], hi.x[4], lo.y[4], hi.y[4], lo.z[4], hi.z[4], links[4]
+// (leaf bit 31 | leaf slot, else node index; unused slots 0xFFFFFFFF), (child count, 0, 0, 0).
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ const float *box_of(uint32_t c, const float *lbox, const float *ibox) {
+    return (c & RT_LEAF_BIT) ? lbox + (size_t)(c & ~RT_LEAF_BIT) * 6 : ibox + (size_t)c * 6;
+}
+
+__device__ int collapse_pick(uint32_t node2, const uint32_t *child, const float *ibox, uint32_t out[4]) {
+    out[0] = child[2 * (size_t)node2];
+    out[1] = child[2 * (size_t)node2 + 1];
+    int cnt = 2;
+    while (cnt < 4) {
+        int best = -1;
+        float ba = -1.f;
+        for (int k = 0; k < cnt; k++)
+            if (!(out[k] & RT_LEAF_BIT)) {
+                float a = half_area(ibox + (size_t)out[k] * 6);
+                if (a > ba) { ba = a; best = k; }
+            }
+        if (best < 0) break;
+        uint32_t c = out[best];
+        out[best] = child[2 * (size_t)c];
+        out[cnt++] = child[2 * (size_t)c + 1];
+    }
+    return cnt;
+}
+
+__global__ void k_c4_count(uint32_t count, const uint32_t *__restrict__ list, const uint32_t *__restrict__ child,
+                           const float *__restrict__ ibox, uint32_t *__restrict__ nint) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    uint32_t c[4];
+    int cnt = collapse_pick(list[t], child, ibox, c);
+    uint32_t m = 0;
+    for (int k = 0; k < cnt; k++) m += (c[k] & RT_LEAF_BIT) ? 0u : 1u;
+    nint[t] = m;
+}
+
+__global__ void k_c4_write(uint32_t count, uint32_t level_base, uint32_t next_base, const uint32_t *__restrict__ list,
+                           const uint32_t *__restrict__ child, const float *__restrict__ lbox,
+                           const float *__restrict__ ibox, const uint32_t *__restrict__ pref,
+                           uint32_t *__restrict__ next_list, float4 *__restrict__ node4) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    uint32_t c[4];
+    int cnt = collapse_pick(list[t], child, ibox, c);
+    float lo[3][4], hi[3][4];
+    uint32_t link[4];
+    uint32_t r = pref[t];
+    for (int k = 0; k < 4; k++) {
+        if (k < cnt) {
+            const float *b = box_of(c[k], lbox, ibox);
+            for (int a = 0; a < 3; a++) { lo[a][k] = b[a]; hi[a][k] = b[3 + a]; }
+            if (c[k] & RT_LEAF_BIT) {
+                link[k] = c[k];
+            } else {
+                next_list[r] = c[k];
+                link[k] = next_base + r;
+                r++;
+            }
+        } else {
+            for (int a = 0; a < 3; a++) { lo[a][k] = 0.f; hi[a][k] = 0.f; }
+            link[k] = 0xFFFFFFFFu;
+        }
+    }
+    float4 *o = node4 + (size_t)(level_base + t) * 8;
+    for (int a = 0; a < 3; a++) {
+        o[2 * a] = make_float4(lo[a][0], lo[a][1], lo[a][2], lo[a][3]);
+        o[2 * a + 1] = make_float4(hi[a][0], hi[a][1], hi[a][2], hi[a][3]);
+    }
+    o[6] = make_float4(__uint_as_float(link[0]), __uint_as_float(link[1]), __uint_as_float(link[2]),
+                       __uint_as_float(link[3]));
+    o[7] = make_float4(__uint_as_float((uint32_t)cnt), 0.f, 0.f, 0.f);
+}
+
 // node layout: (c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y), (c0.lo.z, c0.hi.z, c1.lo.x, c1.hi.x),
 //              (c1.lo.y, c1.hi.y, c1.lo.z, c1.hi.z), (c0 link, c1 link, 0, 0)
 __global__ void k_emit(int n, const uint32_t *__restrict__ child, const float *__restrict__ lbox,
@@ -317,6 +487,93 @@ static void make_periodic(DevScene &h, const float vmin[3], const float vmax[3])
     h.dom[0] = x0; h.dom[1] = x1; h.dom[2] = y0; h.dom[3] = y1;
 }
 
+// PLOC rounds on the device.  Writes child[2*(n-1)] and ibox[(n-1)*6] and the tree height.  Returns 1 on
+// success, 0 if a round made no progress (never expected; the caller then builds the Karras tree), -1 on a
+// HIP error.
+static int build_ploc(uint32_t n, const float *lbox, uint32_t *child, float *ibox, uint32_t &height, hipStream_t st) {
+    float *box[2];
+    uint32_t *id[2], *h[2], *nn;
+    uint64_t *flags, *pref;
+    CK(hipMalloc(&box[0], sizeof(float) * 6 * (size_t)n));
+    CK(hipMalloc(&box[1], sizeof(float) * 6 * (size_t)n));
+    CK(hipMalloc(&id[0], sizeof(uint32_t) * (size_t)n));
+    CK(hipMalloc(&id[1], sizeof(uint32_t) * (size_t)n));
+    CK(hipMalloc(&h[0], sizeof(uint32_t) * (size_t)n));
+    CK(hipMalloc(&h[1], sizeof(uint32_t) * (size_t)n));
+    CK(hipMalloc(&nn, sizeof(uint32_t) * (size_t)n));
+    CK(hipMalloc(&flags, sizeof(uint64_t) * (size_t)n));
+    CK(hipMalloc(&pref, sizeof(uint64_t) * (size_t)n));
+    size_t tmp_bytes = 0;
+    CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, flags, pref, (int)n, st));
+    void *tmp;
+    CK(hipMalloc(&tmp, tmp_bytes + 16));
+    k_ploc_init<<<blocks(n, 256), 256, 0, st>>>(n, lbox, box[0], id[0], h[0]);
+    uint32_t cur = 0, count = n, node_base = n - 1;
+    int ok = 1;
+    while (count > 1) {
+        k_ploc_nn<<<blocks(count, PLOC_BLOCK), PLOC_BLOCK, 0, st>>>(count, box[cur], nn);
+        k_ploc_flags<<<blocks(count, 256), 256, 0, st>>>(count, nn, flags);
+        CK(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, flags, pref, (int)count, st));
+        k_ploc_emit<<<blocks(count, 256), 256, 0, st>>>(count, node_base, nn, flags, pref, box[cur], id[cur], h[cur],
+                                                        box[cur ^ 1], id[cur ^ 1], h[cur ^ 1], child, ibox);
+        uint64_t last[2];
+        CK(hipMemcpyAsync(&last[0], pref + count - 1, 8, hipMemcpyDeviceToHost, st));
+        CK(hipMemcpyAsync(&last[1], flags + count - 1, 8, hipMemcpyDeviceToHost, st));
+        CK(hipStreamSynchronize(st));
+        uint32_t kept = (uint32_t)(last[0] & 0xFFFFFFFFull) + (uint32_t)(last[1] & 0xFFFFFFFFull);
+        uint32_t merged = (uint32_t)(last[0] >> 32) + (uint32_t)(last[1] >> 32);
+        if (merged == 0 || merged > node_base) { ok = 0; break; }
+        node_base -= merged;
+        count = kept;
+        cur ^= 1;
+    }
+    uint32_t root_h = 0;
+    if (ok) {
+        CK(hipMemcpyAsync(&root_h, h[cur], 4, hipMemcpyDeviceToHost, st));
+        CK(hipStreamSynchronize(st));
+        if (node_base != 0) ok = 0;
+    }
+    height = root_h;
+    CK(hipFree(box[0])); CK(hipFree(box[1])); CK(hipFree(id[0])); CK(hipFree(id[1]));
+    CK(hipFree(h[0])); CK(hipFree(h[1])); CK(hipFree(nn)); CK(hipFree(flags)); CK(hipFree(pref)); CK(hipFree(tmp));
+    return ok;
+}
+
+// BVH4 from the binary tree (child/ibox over n leaves with boxes lbox).  node4 holds n-1 slots.
+static int build_bvh4(uint32_t n, const uint32_t *child, const float *lbox, const float *ibox, float4 *node4,
+                      uint32_t &nnode4, uint32_t &levels, hipStream_t st) {
+    uint32_t *list[2], *nint, *pref;
+    CK(hipMalloc(&list[0], sizeof(uint32_t) * (size_t)n));
+    CK(hipMalloc(&list[1], sizeof(uint32_t) * (size_t)n));
+    CK(hipMalloc(&nint, sizeof(uint32_t) * (size_t)n));
+    CK(hipMalloc(&pref, sizeof(uint32_t) * (size_t)n));
+    size_t tmp_bytes = 0;
+    CK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, nint, pref, (int)n, st));
+    void *tmp;
+    CK(hipMalloc(&tmp, tmp_bytes + 16));
+    CK(hipMemsetAsync(list[0], 0, sizeof(uint32_t), st));  // the binary root
+    uint32_t count = 1, base = 0, cur = 0;
+    levels = 0;
+    while (count > 0) {
+        if (base + count > n - 1) { set_error("BVH4 collapse produced too many nodes"); return -1; }
+        k_c4_count<<<blocks(count, 256), 256, 0, st>>>(count, list[cur], child, ibox, nint);
+        CK(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, nint, pref, (int)count, st));
+        k_c4_write<<<blocks(count, 256), 256, 0, st>>>(count, base, base + count, list[cur], child, lbox, ibox, pref,
+                                                       list[cur ^ 1], node4);
+        uint32_t last[2];
+        CK(hipMemcpyAsync(&last[0], pref + count - 1, 4, hipMemcpyDeviceToHost, st));
+        CK(hipMemcpyAsync(&last[1], nint + count - 1, 4, hipMemcpyDeviceToHost, st));
+        CK(hipStreamSynchronize(st));
+        base += count;
+        count = last[0] + last[1];
+        cur ^= 1;
+        levels++;
+    }
+    nnode4 = base;
+    CK(hipFree(list[0])); CK(hipFree(list[1])); CK(hipFree(nint)); CK(hipFree(pref)); CK(hipFree(tmp));
+    return 0;
+}
+
 int scene_build(Scene &S, uint32_t P, const float *T, const uint32_t *type, const uint32_t *uuid,
                 const uint32_t *twos, int px, int py, hipStream_t st) {
     scene_free(S);
@@ -382,8 +639,8 @@ int scene_build(Scene &S, uint32_t P, const float *T, const uint32_t *type, cons
     for (int a = 0; a < 3; a++) { vmin[a] = o2f(hb[6 + a]); vmax[a] = o2f(hb[9 + a]); }
     make_periodic(S.h, vmin, vmax);
 
-    float4 *rec = nullptr, *node = nullptr;
-    uint32_t depth = 0;
+    float4 *rec = nullptr, *node = nullptr, *node4 = nullptr;
+    uint32_t depth = 0, nnode4 = 0, depth4 = 0;
     if (nleaf > 0) {
         float *lbox;
         CK(hipMalloc(&rec, sizeof(float4) * 4 * (size_t)nleaf));
@@ -391,35 +648,50 @@ int scene_build(Scene &S, uint32_t P, const float *T, const uint32_t *type, cons
         CK(hipMalloc(&lbox, sizeof(float) * 6 * (size_t)nleaf));
         k_leaves<<<blocks(nleaf, 256), 256, 0, st>>>(nleaf, vals2, rec_u, box, bounds, rec, lbox);
         if (nleaf > 1) {
-            uint32_t *child, *parent, *lparent, *visit, *dmax;
+            uint32_t *child;
             float *ibox;
             CK(hipMalloc(&child, sizeof(uint32_t) * 2 * (size_t)(nleaf - 1)));
-            CK(hipMalloc(&parent, sizeof(uint32_t) * (size_t)(nleaf - 1)));
-            CK(hipMalloc(&lparent, sizeof(uint32_t) * (size_t)nleaf));
-            CK(hipMalloc(&visit, sizeof(uint32_t) * (size_t)(nleaf - 1)));
-            CK(hipMalloc(&dmax, sizeof(uint32_t)));
             CK(hipMalloc(&ibox, sizeof(float) * 6 * (size_t)(nleaf - 1)));
             CK(hipMalloc(&node, sizeof(float4) * 4 * (size_t)(nleaf - 1)));
             S.own.push_back(node);
-            CK(hipMemsetAsync(visit, 0, sizeof(uint32_t) * (size_t)(nleaf - 1), st));
-            CK(hipMemsetAsync(parent, 0xFF, sizeof(uint32_t) * (size_t)(nleaf - 1), st));
-            CK(hipMemsetAsync(lparent, 0xFF, sizeof(uint32_t) * (size_t)nleaf, st));
-            CK(hipMemsetAsync(child, 0, sizeof(uint32_t) * 2 * (size_t)(nleaf - 1), st));
-            CK(hipMemsetAsync(dmax, 0, sizeof(uint32_t), st));
-            k_karras<<<blocks(nleaf - 1, 256), 256, 0, st>>>((int)nleaf, keys2, child, parent, lparent);
-            k_refit<<<blocks(nleaf, 256), 256, 0, st>>>((int)nleaf, child, parent, lparent, lbox, ibox, visit, dmax);
+            int built = build_ploc(nleaf, lbox, child, ibox, depth, st);
+            if (built < 0) return -1;
+            if (built == 0 || depth + 1 > RT_STACK) {
+                // PLOC tree too deep for the traversal stack: Karras LBVH over the same Morton order
+                uint32_t *parent, *lparent, *visit, *dmax;
+                CK(hipMalloc(&parent, sizeof(uint32_t) * (size_t)(nleaf - 1)));
+                CK(hipMalloc(&lparent, sizeof(uint32_t) * (size_t)nleaf));
+                CK(hipMalloc(&visit, sizeof(uint32_t) * (size_t)(nleaf - 1)));
+                CK(hipMalloc(&dmax, sizeof(uint32_t)));
+                CK(hipMemsetAsync(visit, 0, sizeof(uint32_t) * (size_t)(nleaf - 1), st));
+                CK(hipMemsetAsync(parent, 0xFF, sizeof(uint32_t) * (size_t)(nleaf - 1), st));
+                CK(hipMemsetAsync(lparent, 0xFF, sizeof(uint32_t) * (size_t)nleaf, st));
+                CK(hipMemsetAsync(child, 0, sizeof(uint32_t) * 2 * (size_t)(nleaf - 1), st));
+                CK(hipMemsetAsync(dmax, 0, sizeof(uint32_t), st));
+                k_karras<<<blocks(nleaf - 1, 256), 256, 0, st>>>((int)nleaf, keys2, child, parent, lparent);
+                k_refit<<<blocks(nleaf, 256), 256, 0, st>>>((int)nleaf, child, parent, lparent, lbox, ibox, visit,
+                                                             dmax);
+                CK(hipMemcpyAsync(&depth, dmax, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+                CK(hipStreamSynchronize(st));
+                CK(hipFree(parent)); CK(hipFree(lparent)); CK(hipFree(visit)); CK(hipFree(dmax));
+                S.builder = 0;
+            } else {
+                S.builder = 1;
+            }
             k_emit<<<blocks(nleaf - 1, 256), 256, 0, st>>>((int)nleaf, child, lbox, ibox, node);
-            CK(hipMemcpyAsync(&depth, dmax, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+            CK(hipMalloc(&node4, sizeof(float4) * 8 * (size_t)(nleaf - 1)));
+            S.own.push_back(node4);
+            if (build_bvh4(nleaf, child, lbox, ibox, node4, nnode4, depth4, st)) return -1;
             CK(hipStreamSynchronize(st));
-            CK(hipFree(child)); CK(hipFree(parent)); CK(hipFree(lparent)); CK(hipFree(visit));
-            CK(hipFree(dmax)); CK(hipFree(ibox));
+            CK(hipFree(child)); CK(hipFree(ibox));
         } else {
             CK(hipStreamSynchronize(st));
         }
         CK(hipFree(lbox));
     }
     CK(hipFree(rec_u)); CK(hipFree(box)); CK(hipFree(bounds)); CK(hipFree(keys)); CK(hipFree(keys2));
-    CK(hipFree(vals)); CK(hipFree(vals2)); CK(hipFree(tmp));
+    CK(hipFree(vals)); CK(hipFree(tmp));
+    S.own.push_back(vals2);  // traced primitives in leaf order, then the rest: the solve's work order
     if (depth + 1 > RT_STACK) {
         set_error("LBVH depth " + std::to_string(depth) + " exceeds the traversal stack (" + std::to_string(RT_STACK) + ")");
         scene_free(S);
@@ -432,13 +704,19 @@ int scene_build(Scene &S, uint32_t P, const float *T, const uint32_t *type, cons
     S.h.T = dT;
     S.h.type = dtype;
     S.h.uuid = duuid;
+    S.h.order = vals2;
+    // a BVH4 traversal pushes at most 3 entries per level: use it only when that fits the wave stack (128)
+    S.h.node4 = (node4 && 3 * depth4 + 1 <= 128) ? node4 : nullptr;
+    S.h.nnode4 = nnode4;
+    S.h.depth4 = depth4;
     S.h.nleaf = nleaf;
     S.h.depth = depth;
     S.twos = dtwos;
     CK(hipMalloc(&S.d, sizeof(DevScene)));
     CK(hipMemcpyAsync(S.d, &S.h, sizeof(DevScene), hipMemcpyHostToDevice, st));
     CK(hipStreamSynchronize(st));
-    S.bytes = (size_t)P * (64 + 4 * 3 + 16 * 2) + (size_t)nleaf * 64 + (nleaf > 1 ? (size_t)(nleaf - 1) * 64 : 0);
+    S.bytes = (size_t)P * (64 + 4 * 4 + 16 * 2) + (size_t)nleaf * 64 + (nleaf > 1 ? (size_t)(nleaf - 1) * 64 : 0) +
+              (size_t)(nleaf > 1 ? nleaf - 1 : 0) * 128;
     return 0;
 }
 

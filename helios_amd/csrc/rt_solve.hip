@@ -357,6 +357,7 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
                 j.out = dsum + ((size_t)k * Sn + s) * P * 2;
                 j.item_begin = items;
                 j.P = P;
+                j.order = S.h.order;  // primitives in leaf order: concurrent waves trace nearby rays
                 items += P;
                 hd.push_back(j);
             }
@@ -367,6 +368,7 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
     for (uint32_t g = 0; g < hf.size(); g++) {
         DiffuseJob &fj = hf[g];
         fj.sc = S.d; fj.n = n; fj.seed = q.diffuse_seed[g]; fj.hits = hits; fj.item_begin = 0; fj.P = P;
+        fj.order = S.h.order;
     }
     CK(hipMemcpyAsync(fjobs, hf.data(), hf.size() * sizeof(DiffuseJob), hipMemcpyHostToDevice, st));
     std::vector<GatherJob> hg(1 + maxdepth);
@@ -378,6 +380,7 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
         J.active = g == 0 ? masks + (size_t)M_PRIM * C : masks + (size_t)(M_BASE + 4 * (g - 1) + 3) * C;
         J.rho = rho; J.tau = tau; J.in = in; J.st = stb; J.sb = sbb;
         J.item_begin = 0; J.P = P;
+        J.order = S.h.order;
     }
     CK(hipMemcpyAsync(gjobs, hg.data(), hg.size() * sizeof(GatherJob), hipMemcpyHostToDevice, st));
     // the host vectors above must outlive the async copies

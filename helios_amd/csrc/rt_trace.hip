@@ -81,11 +81,10 @@ __device__ __forceinline__ bool leaf_hit_p(const float4 *recs, uint32_t k, v3f o
     float4 r0 = ldc(r);
     uint32_t tag = __float_as_uint(r0.w);
     id = tag & ~RT_PATCH_BIT;
-    if (id == skip) return false;
     bool patch = (tag & RT_PATCH_BIT) != 0;
     float4 r1 = ldc(r + 1), r2 = ldc(r + 2);
     float4 r3 = patch ? ldc(r + 3) : make_float4(0.f, 0.f, 0.f, 0.f);
-    return isect_rec<CD>(r0, r1, r2, r3, patch, o, d, t);
+    return isect_rec<CD>(r0, r1, r2, r3, patch, o, d, t) && id != skip;
 }
 
 template <bool CD>
@@ -95,11 +94,10 @@ __device__ __forceinline__ bool leaf_hit(const DevScene &sc, uint32_t k, v3f o, 
     float4 r0 = ldc(r);
     uint32_t tag = __float_as_uint(r0.w);
     id = tag & ~RT_PATCH_BIT;
-    if (id == skip) return false;
     bool patch = (tag & RT_PATCH_BIT) != 0;
     float4 r1 = ldc(r + 1), r2 = ldc(r + 2);
     float4 r3 = patch ? ldc(r + 3) : make_float4(0.f, 0.f, 0.f, 0.f);
-    return isect_rec<CD>(r0, r1, r2, r3, patch, o, d, t);
+    return isect_rec<CD>(r0, r1, r2, r3, patch, o, d, t) && id != skip;
 }
 
 template <bool CD>
@@ -338,6 +336,128 @@ __device__ bool packet_any(const DevScene &sc, v3f o, v3f d, float tmin, uint32_
     return found;
 }
 
+// Slab test of one child box of a BVH4 node.
+__device__ __forceinline__ void slab4(float lx, float hx, float ly, float hy, float lz, float hz, const RayPre &rp,
+                                      float &tn, float &tf) {
+    float ax = fmaf(lx, rp.ix, -rp.ox), bx = fmaf(hx, rp.ix, -rp.ox);
+    float ay = fmaf(ly, rp.iy, -rp.oy), by = fmaf(hy, rp.iy, -rp.oy);
+    float az = fmaf(lz, rp.iz, -rp.oz), bz = fmaf(hz, rp.iz, -rp.oz);
+    tn = max3f(fminf(ax, bx), fminf(ay, by), fminf(az, bz));
+    tf = min3f(fmaxf(ax, bx), fmaxf(ay, by), fmaxf(az, bz));
+}
+
+// Ascending compare-exchange of two (key, link) pairs held in scalar registers.
+__device__ __forceinline__ void cswap(float &ka, uint32_t &la, float &kb, uint32_t &lb) {
+    bool sw = kb < ka;
+    float k = sw ? kb : ka;
+    uint32_t l = sw ? lb : la;
+    kb = sw ? ka : kb;
+    lb = sw ? la : lb;
+    ka = k;
+    la = l;
+}
+
+// packet_any() on the BVH4: one 128-byte node per step (two scalar loads), four child boxes tested by every
+// lane.  Leaf children are tested by the lanes whose box test passed; internal children hit by any still
+// searching lane are visited nearest-first as seen by the wave's first searching lane (a sorting network on
+// at most four scalar keys), the others pushed on the per-wave LDS stack.  A lane still reaches every node
+// whose box chain it intersects, so each lane's answer equals the single-ray traversal's.
+#define RT_WSTK4 128
+__device__ bool packet_any4(const DevScene &sc, v3f o, v3f d, float tmin, uint32_t skip, float tmax, bool active,
+                            uint32_t *wstk, int *err, TCnt &cnt) {
+    bool act = active, found = false;
+    uint32_t id;
+    float t;
+    if (sc.nleaf <= 1) {
+        if (sc.nleaf == 1 && act) {
+            cnt.prims++;
+            found = leaf_hit<false>(sc, 0, o, d, skip, id, t) && t > tmin && t <= tmax;
+        }
+        return found;
+    }
+    RayPre rp = ray_pre(o, d);
+    const float4 *nodes = uptr(sc.node4), *recs = uptr(sc.rec);
+    const uint32_t nn4 = __builtin_amdgcn_readfirstlane(sc.nnode4);
+    uint32_t ni = 0, sp = 0;
+    uint32_t budget = nn4 + 8;
+    while (__ballot(act) != 0) {
+        ni = __builtin_amdgcn_readfirstlane(ni);
+        if (budget-- == 0 || ni >= nn4) {
+            atomicOr(err, 2);
+            break;
+        }
+        if (act) cnt.nodes++;
+        const float4 *nd = nodes + (size_t)ni * 8;
+        const float4 lx = ldc(nd), hx = ldc(nd + 1), ly = ldc(nd + 2), hy = ldc(nd + 3);
+        const float4 lz = ldc(nd + 4), hz = ldc(nd + 5), lk = ldc(nd + 6), mt = ldc(nd + 7);
+        const uint32_t nc = __builtin_amdgcn_readfirstlane(__float_as_uint(mt.x));
+        uint32_t lnk[4];
+        lnk[0] = __builtin_amdgcn_readfirstlane(__float_as_uint(lk.x));
+        lnk[1] = __builtin_amdgcn_readfirstlane(__float_as_uint(lk.y));
+        lnk[2] = __builtin_amdgcn_readfirstlane(__float_as_uint(lk.z));
+        lnk[3] = __builtin_amdgcn_readfirstlane(__float_as_uint(lk.w));
+        float tn[4], tf[4];
+        slab4(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, rp, tn[0], tf[0]);
+        slab4(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, rp, tn[1], tf[1]);
+        slab4(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, rp, tn[2], tf[2]);
+        slab4(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, rp, tn[3], tf[3]);
+        bool h[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) h[k] = (uint32_t)k < nc && tn[k] <= tf[k] && tf[k] >= 0.f && tn[k] <= tmax;
+        // leaf children: a wave-uniform branch per leaf, the test itself predicated per lane
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if ((uint32_t)k < nc && (lnk[k] & RT_LEAF_BIT)) {
+                const bool want = h[k] && act;
+                if (__ballot(want) != 0) {
+                    cnt.prims += want ? 1u : 0u;
+                    const bool hit = leaf_hit_p<false>(recs, lnk[k] & ~RT_LEAF_BIT, o, d, skip, id, t) && t > tmin &&
+                                     t <= tmax;
+                    const bool take = want && hit;
+                    found = found || take;
+                    act = act && !take;
+                }
+            }
+        }
+        const uint64_t am = __ballot(act);
+        if (am == 0) break;
+        const uint32_t leader = (uint32_t)__builtin_ctzll(am);
+        // internal children some searching lane hits, keyed by the leader's entry distance
+        float key[4];
+        uint32_t lk4[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            bool inner = (uint32_t)k < nc && !(lnk[k] & RT_LEAF_BIT);
+            bool any = inner && __ballot(h[k] && act) != 0;
+            const int kk = __builtin_amdgcn_readlane(__float_as_int(h[k] ? tn[k] : 3.0e38f), (int)leader);
+            key[k] = any ? __int_as_float(kk) : INFINITY;
+            lk4[k] = any ? lnk[k] : 0xFFFFFFFFu;
+        }
+        cswap(key[0], lk4[0], key[1], lk4[1]);
+        cswap(key[2], lk4[2], key[3], lk4[3]);
+        cswap(key[0], lk4[0], key[2], lk4[2]);
+        cswap(key[1], lk4[1], key[3], lk4[3]);
+        cswap(key[1], lk4[1], key[2], lk4[2]);
+        const uint32_t m = (lk4[0] != 0xFFFFFFFFu) + (lk4[1] != 0xFFFFFFFFu) + (lk4[2] != 0xFFFFFFFFu) +
+                           (lk4[3] != 0xFFFFFFFFu);
+        if (m == 0) {
+            if (sp == 0) break;
+            sp--;
+            ni = wstk[sp];
+        } else {
+            if (sp + m - 1 > RT_WSTK4) {
+                atomicOr(err, 1);
+                break;
+            }
+            if (m >= 4) wstk[sp++] = lk4[3];
+            if (m >= 3) wstk[sp++] = lk4[2];
+            if (m >= 2) wstk[sp++] = lk4[1];
+            ni = lk4[0];
+        }
+    }
+    return found;
+}
+
 // R/shaders/common/periodic_boundary.glsl:28-59
 __device__ __forceinline__ float isect_bbox_face(const BBoxFace &f, v3f o, v3f d) {
     v3f n = mk3(f.n[0], f.n[1], f.n[2]);
@@ -418,17 +538,19 @@ __device__ __forceinline__ bool occluded(const DevScene &sc, v3f o, v3f d, float
     return false;
 }
 
-// occluded() for a whole wave with packet_any(); each lane keeps its own wrap state.
+// occluded() for a whole wave with packet_any4() (packet_any() when the scene has no usable BVH4); each
+// lane keeps its own wrap state.
 template <int STK>
 __device__ __forceinline__ bool packet_occluded(const DevScene &sc, v3f o, v3f d, float tmin, uint32_t skip,
-                                                bool active, uint32_t *wstk, int *err, TCnt &cnt) {
+                                                bool active, uint32_t *wstk, uint32_t *wstk4, int *err, TCnt &cnt) {
     v3f cur = o;
     bool need = active, occl = false;
     for (int step = 0; step < 32; ++step) {
         if (__ballot(need) == 0) break;
         float tf = (need && sc.nbbox) ? nearest_face(sc, cur, d, 1e30f) : -1.0f;
         float tmax = tf > 0.f ? tf : 1e30f;
-        bool f = packet_any<STK>(sc, cur, d, tmin, skip, tmax, need, wstk, err, cnt);
+        bool f = sc.node4 ? packet_any4(sc, cur, d, tmin, skip, tmax, need, wstk4, err, cnt)
+                          : packet_any<STK>(sc, cur, d, tmin, skip, tmax, need, wstk, err, cnt);
         if (need) {
             if (f) { occl = true; need = false; }
             else if (tf <= 0.f) need = false;
@@ -487,7 +609,9 @@ __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict
                                                      uint32_t *counter, uint32_t chunk, int *err,
                                                      unsigned long long *stats) {
     __shared__ uint32_t wstk_all[COUNT ? STK * WPB * 64 : STK * WPB];
+    __shared__ uint32_t wstk4_all[COUNT ? 1 : RT_WSTK4 * WPB];
     uint32_t *wstk = COUNT ? wstk_all + threadIdx.x : wstk_all + (threadIdx.x >> 6) * STK;
+    uint32_t *wstk4 = COUNT ? wstk4_all : wstk4_all + (threadIdx.x >> 6) * RT_WSTK4;
     const uint32_t lane = threadIdx.x & 63;
     TCnt cnt{0, 0};
     uint32_t nray = 0;
@@ -498,6 +622,7 @@ __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict
             const DirectJob &J = jobs[j];
             const DevScene &sc = *J.sc;
             uint32_t i = item - J.item_begin;
+            if (J.order) i = J.order[i];
             uint32_t ty = sc.type[i];
             float sum_out = 0.f;
             bool top = true;
@@ -529,7 +654,7 @@ __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict
                         }
                         bool occl;
                         if (COUNT) occl = need && occluded<STK>(sc, o, dir, 1e-4f, i, wstk, WPB * 64, err, cnt);
-                        else occl = packet_occluded<STK>(sc, o, dir, 1e-4f, i, need, wstk, err, cnt);
+                        else occl = packet_occluded<STK>(sc, o, dir, 1e-4f, i, need, wstk, wstk4, err, cnt);
                         if (need && !occl) acc.add(strength);
                     }
                     wave_reduce(acc);
@@ -565,6 +690,7 @@ __global__ __launch_bounds__(WPB * 64) void k_diffuse(const DiffuseJob *__restri
             const DiffuseJob &J = jobs[j];
             const DevScene &sc = *J.sc;
             uint32_t i = item - J.item_begin;
+            if (J.order) i = J.order[i];
             uint32_t n = J.n, nn = n * n;
             uint32_t *w = J.hits + (size_t)i * 2 * nn;
             uint32_t ty = sc.type[i];
@@ -619,6 +745,7 @@ __global__ __launch_bounds__(256) void k_gather_rays(const GatherJob *__restrict
             uint32_t j = find_job(jobs, njobs, item);
             const GatherJob &J = jobs[j];
             uint32_t i = item - J.item_begin;
+            if (J.order) i = J.order[i];
             uint32_t ty = J.sc->type[i];
             if (!traced_type(ty)) continue;
             uint32_t nn = J.n * J.n, C = J.C;
@@ -690,6 +817,7 @@ __global__ __launch_bounds__(256) void k_gather_chan(const GatherJob *__restrict
             uint32_t j = find_job(jobs, njobs, item);
             const GatherJob &J = jobs[j];
             uint32_t i = item - J.item_begin;
+            if (J.order) i = J.order[i];
             uint32_t ty = J.sc->type[i];
             if (!traced_type(ty)) continue;
             uint32_t nn = J.n * J.n, C = J.C;

@@ -25,14 +25,18 @@ struct BBoxFace {
 struct DevScene {
     const float4 *rec;     // [P][4] primitive records in leaf (Morton) order
     const float4 *node;    // [P-1][4] BVH2 nodes: both child boxes + child links
+    const float4 *node4;   // [nnode4][8] BVH4 nodes (same tree collapsed 4-wide), breadth-first
     const float4 *nrm;     // [P] world normal .xyz, .w = bits(twosided flag)
     const float4 *frame;   // [P] (cos elev, sin elev, cos az, sin az) of the normal
     const float *T;        // [P][16] row-major transforms
     const uint32_t *type;  // [P] 0 patch, 1 triangle, 3 tile, other = not traced
     const uint32_t *uuid;  // [P]
+    const uint32_t *order; // [P] traced primitives in leaf (Morton) order, then the others
     uint32_t P;            // primitives (original order)
     uint32_t nleaf;        // traced primitives (leaves)
     uint32_t depth;        // max leaf depth
+    uint32_t nnode4;       // BVH4 nodes
+    uint32_t depth4;       // BVH4 levels
     uint32_t nbbox;        // periodic faces (0, 2 or 4)
     int px, py;
     float dom[4];          // xmin, xmax, ymin, ymax of the periodic faces
@@ -48,6 +52,7 @@ struct DirectJob {
     float *out;            // [P][2] strength sums lit-from-top / lit-from-bottom
     uint32_t item_begin;   // first work item of this job
     uint32_t P;
+    const uint32_t *order; // item -> primitive map (null: item k is primitive k)
 };
 
 // one diffuse trace pass: writes hit words [P][2][n*n]
@@ -60,6 +65,7 @@ struct DiffuseJob {
     uint32_t P;
     uint32_t faces;        // bit 0: top launch (face 1), bit 1: bottom launch (face 0); 0 means both
     uint32_t pad;
+    const uint32_t *order; // item -> primitive map (null: item k is primitive k)
 };
 
 // one gather over a hit table: C channels, band of channel c is c % B
@@ -79,4 +85,5 @@ struct GatherJob {
     float *sb;             // [P][C]
     uint32_t item_begin;
     uint32_t P;
+    const uint32_t *order; // item -> primitive map (null: item k is primitive k)
 };

@@ -43,7 +43,8 @@ class rt_sun_batch(C.Structure):
 
 class rt_scene_info(C.Structure):
     _fields_ = [("nprims", C.c_uint32), ("nleaves", C.c_uint32), ("depth", C.c_uint32), ("nbbox", C.c_uint32),
-                ("domain", C.c_float * 4), ("device_bytes", C.c_uint64)]
+                ("domain", C.c_float * 4), ("device_bytes", C.c_uint64), ("builder", C.c_uint32),
+                ("depth4", C.c_uint32)]
 
 
 class rt_launch_cfg(C.Structure):
@@ -292,7 +293,8 @@ class Scene:
         i = rt_scene_info()
         _check(self.ctx.lib.rt_scene_get_info(self._h, C.byref(i)))
         return {"nprims": i.nprims, "nleaves": i.nleaves, "depth": i.depth, "nbbox": i.nbbox,
-                "domain": list(i.domain), "device_bytes": i.device_bytes}
+                "domain": list(i.domain), "device_bytes": i.device_bytes,
+                "builder": "ploc" if i.builder == 1 else "lbvh", "depth4": i.depth4}
 
     # ---- CollisionDetection::castRaysSoA ----
     def cast_rays(self, origins, dirs, max_dist: float = -1.0):

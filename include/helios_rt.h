@@ -64,6 +64,8 @@ typedef struct {
     uint32_t nbbox;        /* periodic boundary faces                     */
     float domain[4];       /* xmin, xmax, ymin, ymax of the periodic faces */
     uint64_t device_bytes; /* resident bytes                              */
+    uint32_t builder;      /* 1 = PLOC tree, 0 = Morton LBVH (Karras)     */
+    uint32_t depth4;       /* levels of the 4-wide tree, 0 if not used    */
 } rt_scene_info;
 int rt_scene_get_info(const rt_scene *scene, rt_scene_info *info);
 

@@ -25,6 +25,8 @@ def test_scene_info(ctx):
     info = g.info()
     assert info["nprims"] == 1601 and info["nleaves"] == 1601
     assert 0 < info["depth"] < 64
+    assert info["builder"] == "ploc"
+    assert 0 < info["depth4"] < info["depth"]  # the direct pass traverses the 4-wide tree
 
 
 def _diagnose(g, o, hg, ho, n, seed):
