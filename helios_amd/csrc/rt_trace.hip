@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include <cfloat>
 #include <string>
@@ -33,6 +34,14 @@ __device__ __forceinline__ float max3f(float a, float b, float c) { return fmaxf
 // traversal work counters (node fetches, primitive tests) of one lane
 struct TCnt {
     uint32_t nodes, prims;
+};
+// the same interface with every update compiled away (timed kernels)
+struct NopCounter {
+    __device__ __forceinline__ NopCounter operator++(int) { return *this; }
+    __device__ __forceinline__ NopCounter &operator+=(uint32_t) { return *this; }
+};
+struct NoCnt {
+    NopCounter nodes, prims;
 };
 
 struct RayPre {
@@ -143,18 +152,19 @@ __device__ __forceinline__ NodeTest node_test_p(const float4 *nodes, uint32_t ni
     NodeTest r;
     r.tn0 = tn0;
     r.tn1 = tn1;
-    r.h0 = tn0 <= tf0 && tf0 >= 0.f && tn0 <= tlim;
-    r.h1 = tn1 <= tf1 && tf1 >= 0.f && tn1 <= tlim;
+    // tn <= tf && tf >= 0 && tn <= tlim, as one comparison (tlim >= 0; NaN only ever widens the test)
+    r.h0 = fmaxf(tn0, 0.f) <= fminf(tf0, tlim);
+    r.h1 = fmaxf(tn1, 0.f) <= fminf(tf1, tlim);
     r.c0 = __float_as_uint(n3.x);
     r.c1 = __float_as_uint(n3.y);
     return r;
 }
 
 // Closest accepted hit (t > tmin, t < tstart, ties -> lower primitive index).
-template <bool CD, int STK>
+template <bool CD, int STK, typename C>
 __device__ uint32_t trace_closest(const DevScene &sc, v3f o, v3f d, float tmin, uint32_t skip, float tstart,
                                   float &t_out, uint32_t &slot_out, uint32_t *stk, uint32_t stride, int *err,
-                                  TCnt &cnt) {
+                                  C &cnt) {
     float best_t = tstart;
     uint32_t best = RT_MISS, best_slot = RT_MISS;
     if (sc.nleaf <= 1) {
@@ -204,9 +214,9 @@ __device__ uint32_t trace_closest(const DevScene &sc, v3f o, v3f d, float tmin, 
 }
 
 // Any accepted radiation-predicate hit with tmin < t <= tmax?
-template <int STK>
+template <int STK, typename C>
 __device__ bool trace_any(const DevScene &sc, v3f o, v3f d, float tmin, uint32_t skip, float tmax, uint32_t *stk,
-                          uint32_t stride, int *err, TCnt &cnt) {
+                          uint32_t stride, int *err, C &cnt) {
     uint32_t id;
     float t;
     if (sc.nleaf <= 1) {
@@ -263,9 +273,9 @@ __device__ bool trace_any(const DevScene &sc, v3f o, v3f d, float tmin, uint32_t
 // (node fetched once per wave through scalar loads, stack in a per-wave LDS column); every lane tests
 // the child boxes and primitives itself and stops searching at its first accepted hit.  A lane reaches
 // every node whose box chain it intersects, so its answer equals the per-lane traversal's.
-template <int STK>
+template <int STK, typename C>
 __device__ bool packet_any(const DevScene &sc, v3f o, v3f d, float tmin, uint32_t skip, float tmax, bool active,
-                           uint32_t *wstk, int *err, TCnt &cnt) {
+                           uint32_t *wstk, int *err, C &cnt) {
     bool act = active, found = false;
     uint32_t id;
     float t;
@@ -363,8 +373,9 @@ __device__ __forceinline__ void cswap(float &ka, uint32_t &la, float &kb, uint32
 // at most four scalar keys), the others pushed on the per-wave LDS stack.  A lane still reaches every node
 // whose box chain it intersects, so each lane's answer equals the single-ray traversal's.
 #define RT_WSTK4 128
+template <typename C>
 __device__ bool packet_any4(const DevScene &sc, v3f o, v3f d, float tmin, uint32_t skip, float tmax, bool active,
-                            uint32_t *wstk, int *err, TCnt &cnt) {
+                            uint32_t *wstk, int *err, C &cnt) {
     bool act = active, found = false;
     uint32_t id;
     float t;
@@ -403,7 +414,7 @@ __device__ bool packet_any4(const DevScene &sc, v3f o, v3f d, float tmin, uint32
         slab4(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, rp, tn[3], tf[3]);
         bool h[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) h[k] = (uint32_t)k < nc && tn[k] <= tf[k] && tf[k] >= 0.f && tn[k] <= tmax;
+        for (int k = 0; k < 4; k++) h[k] = (uint32_t)k < nc && fmaxf(tn[k], 0.f) <= fminf(tf[k], tmax);
         // leaf children: a wave-uniform branch per leaf, the test itself predicated per lane
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -500,9 +511,9 @@ __device__ __forceinline__ v3f wrap_at(const DevScene &sc, v3f o, v3f d, float b
 
 // Closest-hit trace with up to 32 periodic wraps (diffuse_raygen.comp:498-560 without translucent
 // covers).  Returns the last traversal result.
-template <int STK>
+template <int STK, typename C>
 __device__ __forceinline__ uint32_t trace_loop(const DevScene &sc, v3f o, v3f d, float tmin, uint32_t skip,
-                                               bool &exhausted, uint32_t *stk, uint32_t stride, int *err, TCnt &cnt) {
+                                               bool &exhausted, uint32_t *stk, uint32_t stride, int *err, C &cnt) {
     uint32_t hit = RT_MISS;
     v3f cur = o;
     exhausted = true;
@@ -524,9 +535,9 @@ __device__ __forceinline__ uint32_t trace_loop(const DevScene &sc, v3f o, v3f d,
 // translucent covers).  The reference wraps when the nearest periodic face is strictly closer than
 // the closest hit, and is occluded otherwise if anything was hit; "some hit with t <= t_face" is the
 // same decision, so an any-hit query suffices.  32 wraps without resolution count as unoccluded.
-template <int STK>
+template <int STK, typename C>
 __device__ __forceinline__ bool occluded(const DevScene &sc, v3f o, v3f d, float tmin, uint32_t skip, uint32_t *stk,
-                                         uint32_t stride, int *err, TCnt &cnt) {
+                                         uint32_t stride, int *err, C &cnt) {
     v3f cur = o;
     for (int step = 0; step < 32; ++step) {
         float tf = sc.nbbox ? nearest_face(sc, cur, d, 1e30f) : -1.0f;
@@ -540,9 +551,9 @@ __device__ __forceinline__ bool occluded(const DevScene &sc, v3f o, v3f d, float
 
 // occluded() for a whole wave with packet_any4() (packet_any() when the scene has no usable BVH4); each
 // lane keeps its own wrap state.
-template <int STK>
+template <int STK, typename C>
 __device__ __forceinline__ bool packet_occluded(const DevScene &sc, v3f o, v3f d, float tmin, uint32_t skip,
-                                                bool active, uint32_t *wstk, uint32_t *wstk4, int *err, TCnt &cnt) {
+                                                bool active, uint32_t *wstk, uint32_t *wstk4, int *err, C &cnt) {
     v3f cur = o;
     bool need = active, occl = false;
     for (int step = 0; step < 32; ++step) {
@@ -590,6 +601,7 @@ __device__ __forceinline__ void wave_reduce(ksum &k) {
 
 __device__ __forceinline__ bool traced_type(uint32_t t) { return t == 0 || t == 1 || t == 3; }
 
+__device__ __forceinline__ void add_stats(unsigned long long *, const NoCnt &, uint32_t) {}
 __device__ __forceinline__ void add_stats(unsigned long long *stats, const TCnt &c, uint32_t rays) {
     if (stats) {
         atomicAdd(&stats[0], (unsigned long long)c.nodes);
@@ -613,7 +625,7 @@ __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict
     uint32_t *wstk = COUNT ? wstk_all + threadIdx.x : wstk_all + (threadIdx.x >> 6) * STK;
     uint32_t *wstk4 = COUNT ? wstk4_all : wstk4_all + (threadIdx.x >> 6) * RT_WSTK4;
     const uint32_t lane = threadIdx.x & 63;
-    TCnt cnt{0, 0};
+    typename std::conditional<COUNT, TCnt, NoCnt>::type cnt{};
     uint32_t nray = 0;
     for (uint32_t first = wave_item(counter, chunk); first < total; first = wave_item(counter, chunk)) {
         const uint32_t end = min(first + chunk, total);
@@ -673,7 +685,7 @@ __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict
 // ------------------------------------------------------------------------------------------------
 // diffuse / scattering rays -> hit words
 // ------------------------------------------------------------------------------------------------
-template <int WPB, int STK>
+template <int WPB, int STK, bool COUNT>
 __global__ __launch_bounds__(WPB * 64) void k_diffuse(const DiffuseJob *__restrict__ jobs, uint32_t njobs,
                                                       uint32_t total, uint32_t *counter, uint32_t chunk,
                                                       int *err, unsigned long long *stats) {
@@ -681,7 +693,7 @@ __global__ __launch_bounds__(WPB * 64) void k_diffuse(const DiffuseJob *__restri
     uint32_t *stk = stk_all + threadIdx.x;
     const uint32_t stride = WPB * 64;
     const uint32_t lane = threadIdx.x & 63;
-    TCnt cnt{0, 0};
+    typename std::conditional<COUNT, TCnt, NoCnt>::type cnt{};
     uint32_t nray = 0;
     for (uint32_t first = wave_item(counter, chunk); first < total; first = wave_item(counter, chunk)) {
         const uint32_t end = min(first + chunk, total);
@@ -875,7 +887,7 @@ __global__ __launch_bounds__(256) void k_gather_chan(const GatherJob *__restrict
 // ------------------------------------------------------------------------------------------------
 // castRaysSoA
 // ------------------------------------------------------------------------------------------------
-template <int STK>
+template <int STK, bool COUNT>
 __global__ __launch_bounds__(256) void k_cast(const DevScene *__restrict__ scp, uint64_t n, const float *__restrict__ O,
                                               const float *__restrict__ D, float maxd, float *__restrict__ T,
                                               float *__restrict__ N, uint32_t *__restrict__ U, int *err,
@@ -883,7 +895,7 @@ __global__ __launch_bounds__(256) void k_cast(const DevScene *__restrict__ scp, 
     __shared__ uint32_t stk_all[STK * 256];
     uint32_t *stk = stk_all + threadIdx.x;
     const DevScene &sc = *scp;
-    TCnt cnt{0, 0};
+    typename std::conditional<COUNT, TCnt, NoCnt>::type cnt{};
     uint32_t nray = 0;
     for (uint64_t r = (uint64_t)blockIdx.x * 256 + threadIdx.x; r < n; r += (uint64_t)gridDim.x * 256) {
         v3f o = mk3(O[r * 3], O[r * 3 + 1], O[r * 3 + 2]);
@@ -1054,13 +1066,20 @@ int launch_diffuse(const DiffuseJob *jobs, uint32_t njobs, uint32_t total, uint3
     k_zero_u32<<<1, 1, 0, st>>>(counter);
     uint32_t wpb = cfg.waves_per_block;
     uint32_t g = grid_for(cfg, ncu, total, wpb);
+    if (stats) {
+        // counting variant: 4 waves per block
+        PICK_STK(depth, (k_diffuse<4, 32, true><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)),
+                 (k_diffuse<4, 64, true><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)));
+        LCHECK();
+        return 0;
+    }
     switch (wpb) {
-        case 1: PICK_STK(depth, (k_diffuse<1, 32><<<g, 64, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)),
-                         (k_diffuse<1, 64><<<g, 64, 0, st>>>(jobs, njobs, total, counter, ch, err, stats))); break;
-        case 2: PICK_STK(depth, (k_diffuse<2, 32><<<g, 128, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)),
-                         (k_diffuse<2, 64><<<g, 128, 0, st>>>(jobs, njobs, total, counter, ch, err, stats))); break;
-        default: PICK_STK(depth, (k_diffuse<4, 32><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)),
-                          (k_diffuse<4, 64><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch, err, stats))); break;
+        case 1: PICK_STK(depth, (k_diffuse<1, 32, false><<<g, 64, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)),
+                         (k_diffuse<1, 64, false><<<g, 64, 0, st>>>(jobs, njobs, total, counter, ch, err, stats))); break;
+        case 2: PICK_STK(depth, (k_diffuse<2, 32, false><<<g, 128, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)),
+                         (k_diffuse<2, 64, false><<<g, 128, 0, st>>>(jobs, njobs, total, counter, ch, err, stats))); break;
+        default: PICK_STK(depth, (k_diffuse<4, 32, false><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)),
+                          (k_diffuse<4, 64, false><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch, err, stats))); break;
     }
     LCHECK();
     return 0;
@@ -1093,8 +1112,12 @@ int launch_cast(const DevScene *sc, uint32_t depth, uint64_t n, const float *o, 
     if (n == 0) return 0;
     uint64_t need = (n + 255) / 256;
     uint32_t g = (uint32_t)(need < 65536ull * 16 ? need : 65536ull * 16);
-    PICK_STK(depth, (k_cast<32><<<g, 256, 0, st>>>(sc, n, o, d, maxd, t, nrm, uuid, err, stats)),
-             (k_cast<64><<<g, 256, 0, st>>>(sc, n, o, d, maxd, t, nrm, uuid, err, stats)));
+    if (stats)
+        PICK_STK(depth, (k_cast<32, true><<<g, 256, 0, st>>>(sc, n, o, d, maxd, t, nrm, uuid, err, stats)),
+                 (k_cast<64, true><<<g, 256, 0, st>>>(sc, n, o, d, maxd, t, nrm, uuid, err, stats)));
+    else
+        PICK_STK(depth, (k_cast<32, false><<<g, 256, 0, st>>>(sc, n, o, d, maxd, t, nrm, uuid, err, stats)),
+                 (k_cast<64, false><<<g, 256, 0, st>>>(sc, n, o, d, maxd, t, nrm, uuid, err, stats)));
     LCHECK();
     return 0;
 }
