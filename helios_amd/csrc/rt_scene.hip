@@ -342,7 +342,7 @@ __global__ void k_ploc_emit(uint32_t n, uint32_t node_base, const uint32_t *__re
 // node's two children and repeatedly opens the internal child with the largest surface area until it
 // has four children or only leaves.  Nodes are numbered breadth-first (the root is 0).
 // Node layout (8 float4, 128 B): lo.x[4], hi.x[4], lo.y[4], hi.y[4], lo.z[4], hi.z[4], links[4]
-// (leaf bit 31 | leaf slot, else node index; unused slots 0xFFFFFFFF), (child count, 0, 0, 0).
+// (leaf bit 31 | leaf slot, else node index; unused slots 0x7FFFFFFF with a box at +inf), (child count, 0, 0, 0).
 // ------------------------------------------------------------------------------------------------
 __device__ __forceinline__ const float *box_of(uint32_t c, const float *lbox, const float *ibox) {
     return (c & RT_LEAF_BIT) ? lbox + (size_t)(c & ~RT_LEAF_BIT) * 6 : ibox + (size_t)c * 6;
@@ -402,8 +402,11 @@ __global__ void k_c4_write(uint32_t count, uint32_t level_base, uint32_t next_ba
                 r++;
             }
         } else {
-            for (int a = 0; a < 3; a++) { lo[a][k] = 0.f; hi[a][k] = 0.f; }
-            link[k] = 0xFFFFFFFFu;
+            // unused slot: a point box at +inf, which every slab test rejects (each axis gives +-inf entry and
+            // exit distances: mixed signs make entry > exit, all +inf is beyond any tmax, all -inf is behind
+            // the origin), and a link without the leaf bit that no traversal follows
+            for (int a = 0; a < 3; a++) { lo[a][k] = INFINITY; hi[a][k] = INFINITY; }
+            link[k] = 0x7FFFFFFFu;
         }
     }
     float4 *o = node4 + (size_t)(level_base + t) * 8;

@@ -400,8 +400,7 @@ __device__ bool packet_any4(const DevScene &sc, v3f o, v3f d, float tmin, uint32
         if (act) cnt.nodes++;
         const float4 *nd = nodes + (size_t)ni * 8;
         const float4 lx = ldc(nd), hx = ldc(nd + 1), ly = ldc(nd + 2), hy = ldc(nd + 3);
-        const float4 lz = ldc(nd + 4), hz = ldc(nd + 5), lk = ldc(nd + 6), mt = ldc(nd + 7);
-        const uint32_t nc = __builtin_amdgcn_readfirstlane(__float_as_uint(mt.x));
+        const float4 lz = ldc(nd + 4), hz = ldc(nd + 5), lk = ldc(nd + 6);
         uint32_t lnk[4];
         lnk[0] = __builtin_amdgcn_readfirstlane(__float_as_uint(lk.x));
         lnk[1] = __builtin_amdgcn_readfirstlane(__float_as_uint(lk.y));
@@ -414,11 +413,11 @@ __device__ bool packet_any4(const DevScene &sc, v3f o, v3f d, float tmin, uint32
         slab4(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, rp, tn[3], tf[3]);
         bool h[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) h[k] = (uint32_t)k < nc && fmaxf(tn[k], 0.f) <= fminf(tf[k], tmax);
+        for (int k = 0; k < 4; k++) h[k] = fmaxf(tn[k], 0.f) <= fminf(tf[k], tmax);  // unused slots never pass
         // leaf children: a wave-uniform branch per leaf, the test itself predicated per lane
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            if ((uint32_t)k < nc && (lnk[k] & RT_LEAF_BIT)) {
+            if (lnk[k] & RT_LEAF_BIT) {
                 const bool want = h[k] && act;
                 if (__ballot(want) != 0) {
                     cnt.prims += want ? 1u : 0u;
@@ -438,7 +437,7 @@ __device__ bool packet_any4(const DevScene &sc, v3f o, v3f d, float tmin, uint32
         uint32_t lk4[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            bool inner = (uint32_t)k < nc && !(lnk[k] & RT_LEAF_BIT);
+            bool inner = !(lnk[k] & RT_LEAF_BIT);
             bool any = inner && __ballot(h[k] && act) != 0;
             const int kk = __builtin_amdgcn_readlane(__float_as_int(h[k] ? tn[k] : 3.0e38f), (int)leader);
             key[k] = any ? __int_as_float(kk) : INFINITY;
