@@ -110,11 +110,11 @@ __device__ __forceinline__ bool leaf_hit(const DevScene &sc, uint32_t k, v3f o, 
 }
 
 template <bool CD>
-__device__ __forceinline__ void test_leaf(const DevScene &sc, uint32_t k, v3f o, v3f d, float tmin, uint32_t skip,
+__device__ __forceinline__ void test_leaf(const float4 *recs, uint32_t k, v3f o, v3f d, float tmin, uint32_t skip,
                                           float &best_t, uint32_t &best, uint32_t &best_slot) {
     uint32_t id;
     float t;
-    if (leaf_hit<CD>(sc, k, o, d, skip, id, t) && t > tmin) {
+    if (leaf_hit_p<CD>(recs, k, o, d, skip, id, t) && t > tmin) {
         if (t < best_t || (t == best_t && best != RT_MISS && id < best)) {
             best_t = t;
             best = id;
@@ -167,8 +167,12 @@ __device__ uint32_t trace_closest(const DevScene &sc, v3f o, v3f d, float tmin, 
                                   C &cnt) {
     float best_t = tstart;
     uint32_t best = RT_MISS, best_slot = RT_MISS;
-    if (sc.nleaf <= 1) {
-        if (sc.nleaf == 1) { cnt.prims++; test_leaf<CD>(sc, 0, o, d, tmin, skip, best_t, best, best_slot); }
+    // scene fields in registers: the stack stores below go through a generic pointer, and the compiler
+    // would otherwise reload them from the descriptor on every step
+    const float4 *nodes = sc.node, *recs = sc.rec;
+    const uint32_t nleaf = sc.nleaf;
+    if (nleaf <= 1) {
+        if (nleaf == 1) { cnt.prims++; test_leaf<CD>(recs, 0, o, d, tmin, skip, best_t, best, best_slot); }
         t_out = best_t;
         slot_out = best_slot;
         return best;
@@ -177,17 +181,17 @@ __device__ uint32_t trace_closest(const DevScene &sc, v3f o, v3f d, float tmin, 
     uint32_t ni = 0;
     int sp = 0;
     // every internal node is entered at most once per trace; the cap only guards malformed trees
-    uint32_t budget = sc.nleaf + 8;
+    uint32_t budget = nleaf + 8;
     while (true) {
-        if (budget-- == 0 || ni >= sc.nleaf - 1) {
+        if (budget-- == 0 || ni >= nleaf - 1) {
             atomicOr(err, 2);
             break;
         }
         cnt.nodes++;
-        NodeTest nt = node_test(sc, ni, rp, best_t);
+        NodeTest nt = node_test_p(nodes, ni, rp, best_t);
         bool l0 = (nt.c0 & RT_LEAF_BIT) != 0, l1 = (nt.c1 & RT_LEAF_BIT) != 0;
-        if (nt.h0 && l0) { cnt.prims++; test_leaf<CD>(sc, nt.c0 & ~RT_LEAF_BIT, o, d, tmin, skip, best_t, best, best_slot); }
-        if (nt.h1 && l1) { cnt.prims++; test_leaf<CD>(sc, nt.c1 & ~RT_LEAF_BIT, o, d, tmin, skip, best_t, best, best_slot); }
+        if (nt.h0 && l0) { cnt.prims++; test_leaf<CD>(recs, nt.c0 & ~RT_LEAF_BIT, o, d, tmin, skip, best_t, best, best_slot); }
+        if (nt.h1 && l1) { cnt.prims++; test_leaf<CD>(recs, nt.c1 & ~RT_LEAF_BIT, o, d, tmin, skip, best_t, best, best_slot); }
         bool g0 = nt.h0 && !l0, g1 = nt.h1 && !l1;
         if (g0 && g1) {
             bool first0 = nt.tn0 <= nt.tn1;
@@ -219,32 +223,34 @@ __device__ bool trace_any(const DevScene &sc, v3f o, v3f d, float tmin, uint32_t
                           uint32_t stride, int *err, C &cnt) {
     uint32_t id;
     float t;
-    if (sc.nleaf <= 1) {
-        if (sc.nleaf == 1) {
+    const float4 *nodes = sc.node, *recs = sc.rec;
+    const uint32_t nleaf = sc.nleaf;
+    if (nleaf <= 1) {
+        if (nleaf == 1) {
             cnt.prims++;
-            return leaf_hit<false>(sc, 0, o, d, skip, id, t) && t > tmin && t <= tmax;
+            return leaf_hit_p<false>(recs, 0, o, d, skip, id, t) && t > tmin && t <= tmax;
         }
         return false;
     }
     RayPre rp = ray_pre(o, d);
     uint32_t ni = 0;
     int sp = 0;
-    uint32_t budget = sc.nleaf + 8;
+    uint32_t budget = nleaf + 8;
     while (true) {
-        if (budget-- == 0 || ni >= sc.nleaf - 1) {
+        if (budget-- == 0 || ni >= nleaf - 1) {
             atomicOr(err, 2);
             return false;
         }
         cnt.nodes++;
-        NodeTest nt = node_test(sc, ni, rp, tmax);
+        NodeTest nt = node_test_p(nodes, ni, rp, tmax);
         bool l0 = (nt.c0 & RT_LEAF_BIT) != 0, l1 = (nt.c1 & RT_LEAF_BIT) != 0;
         if (nt.h0 && l0) {
             cnt.prims++;
-            if (leaf_hit<false>(sc, nt.c0 & ~RT_LEAF_BIT, o, d, skip, id, t) && t > tmin && t <= tmax) return true;
+            if (leaf_hit_p<false>(recs, nt.c0 & ~RT_LEAF_BIT, o, d, skip, id, t) && t > tmin && t <= tmax) return true;
         }
         if (nt.h1 && l1) {
             cnt.prims++;
-            if (leaf_hit<false>(sc, nt.c1 & ~RT_LEAF_BIT, o, d, skip, id, t) && t > tmin && t <= tmax) return true;
+            if (leaf_hit_p<false>(recs, nt.c1 & ~RT_LEAF_BIT, o, d, skip, id, t) && t > tmin && t <= tmax) return true;
         }
         bool g0 = nt.h0 && !l0, g1 = nt.h1 && !l1;
         if (g0 && g1) {
@@ -978,7 +984,7 @@ __global__ __launch_bounds__(256) void k_debug_trace(const DevScene *__restrict_
     float t = 1e30f;
     uint32_t h = RT_MISS, slot;
     if (brute) {
-        for (uint32_t k = 0; k < sc.nleaf; k++) test_leaf<false>(sc, k, o, d, tmin, skip[r], t, h, slot);
+        for (uint32_t k = 0; k < sc.nleaf; k++) test_leaf<false>(sc.rec, k, o, d, tmin, skip[r], t, h, slot);
     } else {
         h = trace_closest<false, RT_STACK>(sc, o, d, tmin, skip[r], 1e30f, t, slot, stk, 256, err, cnt);
     }
