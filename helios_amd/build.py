@@ -20,7 +20,7 @@ CSRC = os.path.join(HERE, "csrc")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_SOURCES = ["rt_scene.hip", "rt_trace.hip", "rt_solve.hip", "rt_api.hip", "rt_server.hip", "rt_backend.hip",
-               "rt_autotune.hip"]
+               "rt_autotune.hip", "rt_grid.hip"]
 HIP_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",

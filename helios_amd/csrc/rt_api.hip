@@ -58,6 +58,7 @@ int rt_destroy(rt_ctx *ctx) {
     (void)hipStreamSynchronize(ctx->sx.stream);
     for (auto &e : ctx->sx.ev) { if (e.a) (void)hipEventDestroy(e.a); if (e.b) (void)hipEventDestroy(e.b); }
     ctx->sx.ws.release();
+    ctx->sx.grids.release();
     backend_release(ctx->be);
     if (ctx->sx.stats) (void)hipFree(ctx->sx.stats);
     (void)hipStreamDestroy(ctx->sx.stream);
@@ -300,6 +301,7 @@ int rt_get_timing(rt_ctx *ctx, rt_timing *out) {
     NN(ctx, "ctx");
     NN(out, "out");
     *out = ctx->sx.timing_info;
+    out->direct_grid = ctx->sx.last_direct_grid ? 1u : 0u;
     return 0;
 }
 

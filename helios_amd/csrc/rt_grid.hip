@@ -1,0 +1,261 @@
+// rt_grid.hip -- occlusion grids for collimated sources.
+//
+// All shadow rays towards one collimated source are parallel, so a ray from origin o can only hit a
+// primitive whose projection onto the plane perpendicular to the direction contains the projection of
+// o.  For each direction this file bins every traced primitive's projected rectangle (padded) into a
+// uniform grid over that plane; the direct kernel then tests only the primitives of the ray's cell, with
+// the same intersection predicate as the BVH traversal, and stops at the first accepted hit.
+//
+// Exactness: a primitive the FP32 predicate accepts is hit at a point whose exact projection lies within
+// the primitive's exact projection up to rounding of a few ulps of the scene's coordinates (t is bounded
+// by the scene diameter, the basis is orthogonal to the direction to FP32 precision).  The rectangles are
+// padded by 2e-4 of the primitive's extent plus 1e-5 of the largest scene coordinate, far above those
+// errors, so the rectangle contains the ray origin's FP32 projection.  Cells are found with the same
+// monotone formula for rectangles and rays, so the ray's cell lies in the rectangle's cell range.  The
+// candidate set is therefore a superset of the primitives the predicate accepts, and every occlusion
+// decision equals the BVH traversal's (order within a cell does not matter for an any-hit query).
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rt_solve.h"
+
+namespace rt {
+
+namespace {
+
+__device__ __forceinline__ uint32_t f2o_g(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+float o2f_g(uint32_t u) {
+    uint32_t v = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+    float f;
+    std::memcpy(&f, &v, 4);
+    return f;
+}
+
+// padded projected rectangle of the primitive in leaf slot k
+__device__ __forceinline__ void proj_rect(const float4 *__restrict__ rec, uint32_t k, const SunGrid &G, float smax,
+                                          float &u0, float &u1, float &v0, float &v1) {
+    const float4 r0 = rec[(size_t)k * 4], r1 = rec[(size_t)k * 4 + 1], r2 = rec[(size_t)k * 4 + 2];
+    const bool patch = (__float_as_uint(r0.w) & RT_PATCH_BIT) != 0;
+    float cx[4], cy[4], cz[4];
+    cx[0] = r0.x; cy[0] = r0.y; cz[0] = r0.z;
+    cx[1] = r0.x + r1.x; cy[1] = r0.y + r1.y; cz[1] = r0.z + r1.z;
+    cx[2] = r0.x + r2.x; cy[2] = r0.y + r2.y; cz[2] = r0.z + r2.z;
+    if (patch) { cx[3] = cx[1] + r2.x; cy[3] = cy[1] + r2.y; cz[3] = cz[1] + r2.z; }
+    else { cx[3] = cx[0]; cy[3] = cy[0]; cz[3] = cz[0]; }
+    u0 = v0 = INFINITY;
+    u1 = v1 = -INFINITY;
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        float pu = cx[c] * G.u[0] + cy[c] * G.u[1] + cz[c] * G.u[2];
+        float pv = cx[c] * G.v[0] + cy[c] * G.v[1] + cz[c] * G.v[2];
+        u0 = fminf(u0, pu); u1 = fmaxf(u1, pu);
+        v0 = fminf(v0, pv); v1 = fmaxf(v1, pv);
+        lo[0] = fminf(lo[0], cx[c]); hi[0] = fmaxf(hi[0], cx[c]);
+        lo[1] = fminf(lo[1], cy[c]); hi[1] = fmaxf(hi[1], cy[c]);
+        lo[2] = fminf(lo[2], cz[c]); hi[2] = fmaxf(hi[2], cz[c]);
+    }
+    const float ext = fmaxf(fmaxf(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]);
+    const float pad = ext * 2e-4f + smax * 1e-5f + 1e-30f;
+    u0 -= pad; u1 += pad; v0 -= pad; v1 += pad;
+}
+
+__device__ __forceinline__ int cell_of(float q, float g0, float inv_cs, uint32_t n) {
+    float f = floorf((q - g0) * inv_cs);
+    int c = f < 0.f ? 0 : (f > (float)(n - 1) ? (int)(n - 1) : (int)f);
+    return c;
+}
+
+__global__ void k_grid_bounds(uint32_t nleaf, const float4 *__restrict__ rec, float smax,
+                              const SunGrid *__restrict__ grids, uint32_t *__restrict__ bounds /*[n][4]*/) {
+    const SunGrid G = grids[blockIdx.y];
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    float u0 = INFINITY, u1 = -INFINITY, v0 = INFINITY, v1 = -INFINITY;
+    if (k < nleaf) proj_rect(rec, k, G, smax, u0, u1, v0, v1);
+    // block reduction, then one atomic per block
+    __shared__ float s[4][256];
+    s[0][threadIdx.x] = u0; s[1][threadIdx.x] = -u1; s[2][threadIdx.x] = v0; s[3][threadIdx.x] = -v1;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int q = 0; q < 4; q++) s[q][threadIdx.x] = fminf(s[q][threadIdx.x], s[q][threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        uint32_t *b = bounds + (size_t)blockIdx.y * 4;
+        atomicMin(&b[0], f2o_g(s[0][0]));
+        atomicMin(&b[1], f2o_g(s[1][0]));
+        atomicMin(&b[2], f2o_g(s[2][0]));
+        atomicMin(&b[3], f2o_g(s[3][0]));
+    }
+}
+
+__global__ void k_grid_count(uint32_t nleaf, const float4 *__restrict__ rec, float smax,
+                             const SunGrid *__restrict__ grids, const uint32_t *__restrict__ cell_base,
+                             uint32_t *__restrict__ counts, unsigned long long *__restrict__ entries) {
+    const SunGrid G = grids[blockIdx.y];
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nleaf) return;
+    float u0, u1, v0, v1;
+    proj_rect(rec, k, G, smax, u0, u1, v0, v1);
+    const int x0 = cell_of(u0, G.g0u, G.inv_cs, G.nx), x1 = cell_of(u1, G.g0u, G.inv_cs, G.nx);
+    const int y0 = cell_of(v0, G.g0v, G.inv_cs, G.ny), y1 = cell_of(v1, G.g0v, G.inv_cs, G.ny);
+    atomicAdd(entries, (unsigned long long)(x1 - x0 + 1) * (unsigned long long)(y1 - y0 + 1));
+    uint32_t *c = counts + cell_base[blockIdx.y];
+    for (int y = y0; y <= y1; y++)
+        for (int x = x0; x <= x1; x++) atomicAdd(&c[(size_t)y * G.nx + x], 1u);
+}
+
+__global__ void k_grid_fill(uint32_t nleaf, const float4 *__restrict__ rec, float smax,
+                            const SunGrid *__restrict__ grids, const uint32_t *__restrict__ cell_base,
+                            const uint32_t *__restrict__ start, uint32_t *__restrict__ counts,
+                            uint32_t *__restrict__ items, uint32_t total) {
+    const SunGrid G = grids[blockIdx.y];
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nleaf) return;
+    float u0, u1, v0, v1;
+    proj_rect(rec, k, G, smax, u0, u1, v0, v1);
+    const int x0 = cell_of(u0, G.g0u, G.inv_cs, G.nx), x1 = cell_of(u1, G.g0u, G.inv_cs, G.nx);
+    const int y0 = cell_of(v0, G.g0v, G.inv_cs, G.ny), y1 = cell_of(v1, G.g0v, G.inv_cs, G.ny);
+    const size_t base = cell_base[blockIdx.y];
+    for (int y = y0; y <= y1; y++)
+        for (int x = x0; x <= x1; x++) {
+            const size_t c = base + (size_t)y * G.nx + x;
+            // the ranges equal the counting kernel's (same code, same FP32 operations); the guards only
+            // keep a mismatch from writing outside the list
+            const uint32_t left = atomicSub(&counts[c], 1u);
+            if (left == 0u) continue;
+            const uint32_t at = start[c] + left - 1u;
+            if (at < total) items[at] = k;
+        }
+}
+
+inline uint32_t nblk(size_t n) { return (uint32_t)((n + 255) / 256); }
+
+#define GK(x)                                                                     \
+    do {                                                                          \
+        hipError_t e_ = (x);                                                      \
+        if (e_ != hipSuccess) {                                                   \
+            set_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " #x); \
+            return -1;                                                            \
+        }                                                                         \
+    } while (0)
+
+}  // namespace
+
+int build_sun_grids(GridSet &gs, const Scene &S, const float *dirs, uint32_t n, hipStream_t st) {
+    gs.n = 0;
+    const uint32_t nleaf = S.h.nleaf;
+    if (n == 0 || nleaf == 0 || S.h.nbbox != 0) return 1;
+    // host: basis per direction (double precision, then rounded)
+    std::vector<SunGrid> hg(n);
+    for (uint32_t j = 0; j < n; j++) {
+        double d[3] = {dirs[j * 3], dirs[j * 3 + 1], dirs[j * 3 + 2]};
+        double m = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        if (!(m > 0.0)) return 1;
+        for (double &x : d) x /= m;
+        int a = 0;
+        for (int q = 1; q < 3; q++)
+            if (std::fabs(d[q]) < std::fabs(d[a])) a = q;
+        double e[3] = {0, 0, 0};
+        e[a] = 1.0;
+        double u[3] = {d[1] * e[2] - d[2] * e[1], d[2] * e[0] - d[0] * e[2], d[0] * e[1] - d[1] * e[0]};
+        double um = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        for (double &x : u) x /= um;
+        double v[3] = {d[1] * u[2] - d[2] * u[1], d[2] * u[0] - d[0] * u[2], d[0] * u[1] - d[1] * u[0]};
+        SunGrid &G = hg[j];
+        for (int q = 0; q < 3; q++) { G.u[q] = (float)u[q]; G.v[q] = (float)v[q]; }
+        G.g0u = G.g0v = 0.f;
+        G.inv_cs = 1.f;
+        G.nx = G.ny = 1;
+        G.start = nullptr;
+        G.items = nullptr;
+    }
+    // 1. projected extents
+    if (gs.small.reserve(n * sizeof(SunGrid) + n * 16 + n * 8 + 2048)) return -1;
+    char *p = (char *)gs.small.base;
+    SunGrid *dgrids = (SunGrid *)p;
+    uint32_t *bounds = (uint32_t *)(p + ((n * sizeof(SunGrid) + 255) & ~(size_t)255));
+    uint32_t *dbase = bounds + 4 * n;
+    unsigned long long *dentries = (unsigned long long *)(dbase + ((n + 63) & ~(size_t)63));
+    GK(hipMemcpyAsync(dgrids, hg.data(), n * sizeof(SunGrid), hipMemcpyHostToDevice, st));
+    GK(hipMemsetAsync(bounds, 0xFF, n * 16, st));
+    dim3 grid(nblk(nleaf), n);
+    k_grid_bounds<<<grid, 256, 0, st>>>(nleaf, S.h.rec, S.h.smax, dgrids, bounds);
+    GK(hipGetLastError());
+    std::vector<uint32_t> hb((size_t)n * 4);
+    GK(hipMemcpyAsync(hb.data(), bounds, n * 16, hipMemcpyDeviceToHost, st));
+    GK(hipStreamSynchronize(st));
+    // 2. grid shapes: about two cells per primitive over the projected extent
+    std::vector<uint32_t> base(n + 1, 0);
+    size_t total_cells = 0;
+    const double target = std::min(std::max(2.0 * nleaf, 1024.0), (double)(1u << 24));
+    for (uint32_t j = 0; j < n; j++) {
+        SunGrid &G = hg[j];
+        float u0 = o2f_g(hb[j * 4]), u1 = -o2f_g(hb[j * 4 + 1]), v0 = o2f_g(hb[j * 4 + 2]), v1 = -o2f_g(hb[j * 4 + 3]);
+        if (!(u1 >= u0) || !(v1 >= v0) || !std::isfinite(u0) || !std::isfinite(v1)) return 1;
+        double W = (double)u1 - u0, H = (double)v1 - v0;
+        double cs = std::sqrt(W * H / target);
+        if (!(cs > 0.0) || !std::isfinite(cs)) cs = std::max(std::max(W, H), 1e-6) / std::sqrt(target);
+        uint32_t nx = (uint32_t)std::min(16384.0, std::max(1.0, std::ceil(W / cs) + 1.0));
+        uint32_t ny = (uint32_t)std::min(16384.0, std::max(1.0, std::ceil(H / cs) + 1.0));
+        G.g0u = u0;
+        G.g0v = v0;
+        G.inv_cs = (float)(1.0 / cs);
+        G.nx = nx;
+        G.ny = ny;
+        base[j] = (uint32_t)total_cells;
+        total_cells += (size_t)nx * ny;
+        if (total_cells + 1 > (1ull << 30)) return 1;  // scans index with int
+    }
+    // 3. counts -> starts -> items
+    size_t scan_bytes = 0;
+    GK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                        (int)(total_cells + 1), st));
+    const size_t cell_bytes = 2 * (total_cells + 1) * 4 + scan_bytes + 1024;
+    if (gs.cap && cell_bytes > gs.cap) return 1;
+    if (gs.cells.reserve(cell_bytes)) return -1;
+    uint32_t *counts = (uint32_t *)gs.cells.base;
+    uint32_t *starts = counts + ((total_cells + 1 + 63) & ~(size_t)63);
+    void *tmp = starts + ((total_cells + 1 + 63) & ~(size_t)63);
+    GK(hipMemcpyAsync(dbase, base.data(), n * 4, hipMemcpyHostToDevice, st));
+    GK(hipMemcpyAsync(dgrids, hg.data(), n * sizeof(SunGrid), hipMemcpyHostToDevice, st));
+    GK(hipMemsetAsync(counts, 0, (total_cells + 1) * 4, st));
+    GK(hipMemsetAsync(dentries, 0, 8, st));
+    k_grid_count<<<grid, 256, 0, st>>>(nleaf, S.h.rec, S.h.smax, dgrids, dbase, counts, dentries);
+    GK(hipGetLastError());
+    unsigned long long entries = 0;
+    GK(hipMemcpyAsync(&entries, dentries, 8, hipMemcpyDeviceToHost, st));
+    GK(hipStreamSynchronize(st));
+    if (entries + 1 > (1ull << 31)) return 1;  // uint32 positions: traverse the BVH instead
+    if (gs.cap && cell_bytes + entries * 4 + 256 > gs.cap) return 1;
+    GK(hipcub::DeviceScan::ExclusiveSum(tmp, scan_bytes, counts, starts, (int)(total_cells + 1), st));
+    const uint32_t total = (uint32_t)entries;
+    if (gs.items.reserve((size_t)total * 4 + 256)) return -1;
+    uint32_t *items = (uint32_t *)gs.items.base;
+    GK(hipMemsetAsync(items, 0xFF, (size_t)total * 4, st));  // an unfilled slot reads as "no primitive"
+    k_grid_fill<<<grid, 256, 0, st>>>(nleaf, S.h.rec, S.h.smax, dgrids, dbase, starts, counts, items, total);
+    GK(hipGetLastError());
+    for (uint32_t j = 0; j < n; j++) {
+        hg[j].start = starts + base[j];
+        hg[j].items = items;
+    }
+    GK(hipMemcpyAsync(dgrids, hg.data(), n * sizeof(SunGrid), hipMemcpyHostToDevice, st));
+    GK(hipStreamSynchronize(st));
+    gs.dev = dgrids;
+    gs.n = n;
+    gs.entries = total;
+    gs.cells_total = total_cells;
+    return 0;
+}
+
+}  // namespace rt

@@ -641,6 +641,8 @@ int scene_build(Scene &S, uint32_t P, const float *T, const uint32_t *type, cons
     float vmin[3], vmax[3];
     for (int a = 0; a < 3; a++) { vmin[a] = o2f(hb[6 + a]); vmax[a] = o2f(hb[9 + a]); }
     make_periodic(S.h, vmin, vmax);
+    S.h.smax = 0.f;
+    for (int a = 0; a < 3; a++) S.h.smax = fmaxf(S.h.smax, fmaxf(fabsf(vmin[a]), fabsf(vmax[a])));
 
     float4 *rec = nullptr, *node = nullptr, *node4 = nullptr;
     uint32_t depth = 0, nnode4 = 0, depth4 = 0;

@@ -95,6 +95,7 @@ namespace {
 
 void release_slot(rt_server &srv, Slot &sl) {
     sl.sx.ws.release();
+    sl.sx.grids.release();
     if (sl.dout) (void)hipFree(sl.dout);
     sl.dout = nullptr;
     sl.dout_cap = 0;
@@ -141,15 +142,18 @@ void rt_server::worker(uint32_t w) {
         bool fits = false;
         while (!batch.empty()) {
             const uint32_t K = (uint32_t)batch.size();
-            need_ws = Workspace::capacity_for(solve_workspace_bytes(m.P, m.q, K, true));
+            need_ws = Workspace::capacity_for(solve_workspace_bytes(m.P, m.q, K, true)) +
+                      Workspace::capacity_for(GridSet::budget(m.P, K * m.Sn)) + (3u << 20);
             need_out = (size_t)K * m.P * m.B * sizeof(float);
-            new_acc = std::max(need_ws, sl.sx.ws.cap) + std::max(need_out, sl.dout_cap);
+            const size_t have_ws = sl.sx.ws.cap + sl.sx.grids.small.cap + sl.sx.grids.cells.cap + sl.sx.grids.items.cap;
+            new_acc = std::max(need_ws, have_ws) + std::max(need_out, sl.dout_cap);
             if (allocated - sl.accounted + new_acc <= budget) { fits = true; break; }
             if (K == 1) break;
             batch.pop_back();
         }
         if (!fits) {
             size_t alone = Workspace::capacity_for(solve_workspace_bytes(m.P, m.q, 1, true)) +
+                           Workspace::capacity_for(GridSet::budget(m.P, m.Sn)) + (3u << 20) +
                            (size_t)m.P * m.B * sizeof(float);
             if (alone > budget) {
                 queue.pop_front();
@@ -195,6 +199,7 @@ void rt_server::worker(uint32_t w) {
             std::copy(r.dflux.begin(), r.dflux.end(), dfl.begin() + (size_t)k * B);
             seeds[k] = r.seed;
         }
+        sl.sx.grids.cap = GridSet::budget(m.P, K * m.Sn);
         if (sl.sx.ws.reserve(solve_workspace_bytes(m.P, m.q, K, true))) rc = -1;
         if (rc == 0 && need_out > sl.dout_cap) {
             if (sl.dout) (void)hipFree(sl.dout);
@@ -449,6 +454,7 @@ int rt_solve_footprint(uint32_t nprims, const rt_band_request *req, uint32_t nsu
     NN(req->band_depth, "band_depth");
     if (nsuns == 0 || req->nbands == 0) { set_error("rt_solve_footprint: no suns or no bands"); return -1; }
     *bytes = Workspace::capacity_for(solve_workspace_bytes(nprims, *req, nsuns, true)) +
+             Workspace::capacity_for(GridSet::budget(nprims, nsuns * req->nsources)) + (3u << 20) +
              (uint64_t)nsuns * nprims * req->nbands * sizeof(float);
     return 0;
 }

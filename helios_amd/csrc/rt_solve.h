@@ -24,6 +24,22 @@ struct EvRec {
     hipEvent_t a, b;
 };
 
+// Occlusion grids of the collimated-source directions of one solve (rt_grid.hip).
+struct GridSet {
+    Workspace small, cells, items;
+    SunGrid *dev = nullptr;  // [n] device headers
+    uint32_t n = 0;
+    uint32_t entries = 0;
+    size_t cells_total = 0;
+    size_t cap = 0;          // byte limit for cells + items (0 = none); above it the BVH is traversed
+    // bytes a server reserves for the grids of one solve of `njobs` directions over P primitives
+    static size_t budget(uint32_t P, uint32_t njobs) { return (size_t)96 * P * njobs + (1u << 20); }
+    void release() { small.release(); cells.release(); items.release(); dev = nullptr; n = 0; }
+};
+// Builds grids for n directions (host [n][3]).  Returns 0 on success, 1 when grids do not apply (periodic
+// scene, degenerate input, too large) and the BVH is to be traversed instead, -1 on a HIP error.
+int build_sun_grids(GridSet &gs, const Scene &S, const float *dirs, uint32_t n, hipStream_t st);
+
 // Everything one stream needs to run solves.
 struct SolveCtx {
     hipStream_t stream = nullptr;
@@ -37,6 +53,9 @@ struct SolveCtx {
     // traversal counters: [0..2] direct, [3..5] diffuse, [6..8] castRays (node fetches, prim tests, rays)
     bool counting = false;
     unsigned long long *stats = nullptr;  // device, 9 counters
+    bool use_grid = true;                 // occlusion grids for direct passes of non-periodic scenes
+    bool last_direct_grid = false;        // the last solve's direct pass used them
+    GridSet grids;
     unsigned long long *stat_ptr(int k) { return counting ? stats + 3 * k : nullptr; }
 };
 

@@ -242,6 +242,7 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
     for (auto &e : sx.ev) { if (e.a) (void)hipEventDestroy(e.a); if (e.b) (void)hipEventDestroy(e.b); }
     sx.ev.clear();
     sx.timing_info = rt_timing{};
+    sx.last_direct_grid = false;
     uint32_t maxdepth = 0;
     int scattering = 0, emission = 0;
     for (uint32_t b = 0; b < B; b++) {
@@ -362,6 +363,23 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
                 hd.push_back(j);
             }
         }
+        // occlusion grids, one per (sun, source) direction, when the scene is not periodic
+        sx.last_direct_grid = false;
+        if (sx.use_grid) {
+            std::vector<float> jd(hd.size() * 3);
+            for (size_t j = 0; j < hd.size(); j++)
+                for (int a = 0; a < 3; a++) jd[j * 3 + a] = hd[j].sdir[a];
+            hipEvent_t ga, gb;
+            if (ev_pair(sx, ga, gb)) return -1;
+            if (ga) CK(hipEventRecord(ga, st));
+            int g = build_sun_grids(sx.grids, S, jd.data(), (uint32_t)hd.size(), st);
+            if (g < 0) return -1;
+            if (gb) { CK(hipEventRecord(gb, st)); tick(sx, 3, ga, gb); }
+            if (g == 0) {
+                for (size_t j = 0; j < hd.size(); j++) hd[j].grid = sx.grids.dev + j;
+                sx.last_direct_grid = true;
+            }
+        }
         CK(hipMemcpyAsync(djobs, hd.data(), hd.size() * sizeof(DirectJob), hipMemcpyHostToDevice, st));
     }
     std::vector<DiffuseJob> hf(1 + maxdepth);
@@ -469,7 +487,8 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
         CK(hipEventElapsedTime(&ms, e.a, e.b));
         if (e.cls == 0) sx.timing_info.direct_ms += ms;
         else if (e.cls == 1) sx.timing_info.diffuse_ms += ms;
-        else sx.timing_info.gather_ms += ms;
+        else if (e.cls == 2) sx.timing_info.gather_ms += ms;
+        else sx.timing_info.other_ms += ms;
     }
     return 0;
 }
@@ -514,6 +533,12 @@ int debug_direct_sums(SolveCtx &sx, const Scene &S, uint32_t d, uint32_t seed, u
         j.sc = S.d;
         j.sdir[0] = src_dir[s * 3]; j.sdir[1] = src_dir[s * 3 + 1]; j.sdir[2] = src_dir[s * 3 + 2];
         j.d = d; j.seed = seed; j.out = sums + (size_t)s * P * 2; j.item_begin = s * P; j.P = P;
+    }
+    if (sx.use_grid) {
+        int g = build_sun_grids(sx.grids, S, src_dir, nsrc, st);
+        if (g < 0) return -1;
+        if (g == 0)
+            for (uint32_t s = 0; s < nsrc; s++) hj[s].grid = sx.grids.dev + s;
     }
     CK(hipMemsetAsync(err, 0, 4, st));
     CK(hipMemcpyAsync(jobs, hj.data(), nsrc * sizeof(DirectJob), hipMemcpyHostToDevice, st));

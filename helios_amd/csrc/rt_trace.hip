@@ -615,6 +615,31 @@ __device__ __forceinline__ void add_stats(unsigned long long *stats, const TCnt 
     }
 }
 
+// Occlusion of one shadow ray through the source's grid: test the primitives binned in the cell of the
+// origin's projection with the BVH traversal's predicate (t in (1e-4, 1e30], not the origin primitive) and
+// stop at the first hit.  Non-periodic scenes only (the grid is not built otherwise).
+template <typename C>
+__device__ __forceinline__ bool grid_occluded(const SunGrid &G, const float4 *recs, uint32_t nleaf, v3f o, v3f d,
+                                              uint32_t skip, C &cnt) {
+    cnt.nodes++;
+    const float qu = o.x * G.u[0] + o.y * G.u[1] + o.z * G.u[2];
+    const float qv = o.x * G.v[0] + o.y * G.v[1] + o.z * G.v[2];
+    float fx = floorf((qu - G.g0u) * G.inv_cs), fy = floorf((qv - G.g0v) * G.inv_cs);
+    const int cx = fx < 0.f ? 0 : (fx > (float)(G.nx - 1) ? (int)(G.nx - 1) : (int)fx);
+    const int cy = fy < 0.f ? 0 : (fy > (float)(G.ny - 1) ? (int)(G.ny - 1) : (int)fy);
+    const uint32_t c = (uint32_t)cy * G.nx + (uint32_t)cx;
+    const uint32_t b = G.start[c], e = G.start[c + 1];
+    for (uint32_t p = b; p < e; p++) {
+        const uint32_t k = G.items[p];
+        if (k >= nleaf) continue;
+        uint32_t id;
+        float t;
+        cnt.prims++;
+        if (leaf_hit_p<false>(recs, k, o, d, skip, id, t) && t > 1e-4f && t <= 1e30f) return true;
+    }
+    return false;
+}
+
 // ------------------------------------------------------------------------------------------------
 // direct rays
 // ------------------------------------------------------------------------------------------------
@@ -643,6 +668,8 @@ __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict
             uint32_t ty = sc.type[i];
             float sum_out = 0.f;
             bool top = true;
+            const SunGrid *grid = J.grid;
+            const float4 *recs = sc.rec;
             if (traced_type(ty)) {
                 float4 n4 = sc.nrm[i];
                 v3f n = mk3(n4.x, n4.y, n4.z);
@@ -670,7 +697,8 @@ __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict
                             nray++;
                         }
                         bool occl;
-                        if (COUNT) occl = need && occluded<STK>(sc, o, dir, 1e-4f, i, wstk, WPB * 64, err, cnt);
+                        if (grid) occl = need && grid_occluded(*grid, recs, sc.nleaf, o, dir, i, cnt);
+                        else if (COUNT) occl = need && occluded<STK>(sc, o, dir, 1e-4f, i, wstk, WPB * 64, err, cnt);
                         else occl = packet_occluded<STK>(sc, o, dir, 1e-4f, i, need, wstk, wstk4, err, cnt);
                         if (need && !occl) acc.add(strength);
                     }

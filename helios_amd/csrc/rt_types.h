@@ -41,6 +41,20 @@ struct DevScene {
     int px, py;
     float dom[4];          // xmin, xmax, ymin, ymax of the periodic faces
     BBoxFace bbox[4];
+    float smax;            // largest |coordinate| of any vertex (scale of FP32 rounding in the scene)
+};
+
+// Occlusion grid of one collimated-source direction: every traced primitive's padded projection onto the
+// plane perpendicular to the direction, binned into a uniform grid (CSR: start[cell] .. start[cell + 1]
+// index items[], which hold leaf slots).  A shadow ray can only hit primitives binned in the cell of its
+// origin's projection.
+struct SunGrid {
+    float u[3], v[3];      // orthonormal basis of the projection plane
+    float g0u, g0v;        // grid origin
+    float inv_cs;          // 1 / cell size
+    uint32_t nx, ny;
+    const uint32_t *start; // [nx*ny + 1] into items
+    const uint32_t *items;
 };
 
 // one direct trace job: one scene, one collimated source
@@ -53,6 +67,7 @@ struct DirectJob {
     uint32_t item_begin;   // first work item of this job
     uint32_t P;
     const uint32_t *order; // item -> primitive map (null: item k is primitive k)
+    const SunGrid *grid;   // occlusion grid of this direction (null: traverse the BVH)
 };
 
 // one diffuse trace pass: writes hit words [P][2][n*n]

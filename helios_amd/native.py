@@ -59,7 +59,8 @@ class rt_tune_result(C.Structure):
 class rt_timing(C.Structure):
     _fields_ = [("direct_ms", C.c_double), ("diffuse_ms", C.c_double), ("gather_ms", C.c_double),
                 ("other_ms", C.c_double), ("direct_rays", C.c_uint64), ("diffuse_rays", C.c_uint64),
-                ("direct_launches", C.c_uint32), ("diffuse_launches", C.c_uint32)]
+                ("direct_launches", C.c_uint32), ("diffuse_launches", C.c_uint32), ("direct_grid", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 class rt_launch_params(C.Structure):

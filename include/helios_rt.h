@@ -278,11 +278,13 @@ typedef struct {
     double direct_ms;   /* direct trace kernels            */
     double diffuse_ms;  /* diffuse/scatter trace kernels   */
     double gather_ms;   /* gather kernels                  */
-    double other_ms;    /* deposit/copy/emission/final     */
+    double other_ms;    /* occlusion-grid builds           */
     uint64_t direct_rays;
     uint64_t diffuse_rays;
     uint32_t direct_launches;
     uint32_t diffuse_launches;
+    uint32_t direct_grid;   /* 1: the direct pass tested occlusion through per-direction grids, 0: BVH */
+    uint32_t reserved;
 } rt_timing;
 int rt_enable_timing(rt_ctx *ctx, int on);
 /* Traversal work counters (zeroed by rt_set_counting): out[9] = direct (BVH node fetches, primitive
