@@ -71,9 +71,16 @@ def pmc_traffic(kernel, profile="pmc_traffic_config3_diurnal.json"):
     return None
 
 
-def bytes_per_ray(c):
+# occlusion-grid direct pass: 8 B cell range per lookup ("nodes" counts lookups), 4 B list entry + 48 B
+# primitive record per candidate tested ("prims")
+B_CELL, B_CAND = 8, 52
+
+
+def bytes_per_ray(c, grid=False):
     if not c["rays"]:
         return None
+    if grid:
+        return B_CELL * c["nodes"] / c["rays"] + B_CAND * c["prims"] / c["rays"] + B_IO
     return B_NODE * c["nodes"] / c["rays"] + B_PRIM * c["prims"] / c["rays"] + B_IO
 
 
@@ -187,7 +194,9 @@ def bench_diurnal(args):
     log(f"[bench] counters {counts}")
     device_sync()
     lat = []
-    kt = {"direct_ms": 0.0, "diffuse_ms": 0.0, "gather_ms": 0.0, "direct_launches": 0, "diffuse_launches": 0}
+    kt = {"direct_ms": 0.0, "diffuse_ms": 0.0, "gather_ms": 0.0, "other_ms": 0.0, "direct_launches": 0,
+          "diffuse_launches": 0}
+    grid, grid_entries = False, 0.0
     t_start = time.perf_counter()
     for k in range(args.steps):
         t = time.perf_counter()
@@ -196,6 +205,8 @@ def bench_diurnal(args):
         tm = ctx.timing()
         for key in kt:
             kt[key] += tm[key]
+        grid = bool(tm["direct_grid"])
+        grid_entries = tm["grid_entries"] / 100.0
         log(f"[bench] step {k}: {lat[-1]:.1f} ms  {tm}")
     device_sync()
     wall = time.perf_counter() - t_start
@@ -215,11 +226,11 @@ def bench_diurnal(args):
     plain_solves_s = 1e3 / float(np.mean(plain_lat))
 
     # roofline of the dominant kernel class (HIP events on the solve stream)
-    per_step = {c: kt[f"{c}_ms"] / args.steps for c in ("direct", "diffuse", "gather")}
+    per_step = {c: kt[f"{c}_ms"] / args.steps for c in ("direct", "diffuse", "gather", "other")}
     dom = max(("direct", "diffuse"), key=lambda c: per_step[c])
     launches = kt[f"{dom}_launches"] / args.steps
     avg_launch_ms = per_step[dom] / max(launches, 1)
-    brays = bytes_per_ray(counts[dom])
+    brays = bytes_per_ray(counts[dom], grid=(dom == "direct" and grid))
     rays_per_launch = counts[dom]["rays"] / max(launches, 1)
     achieved = brays * rays_per_launch / (avg_launch_ms / 1e3) / 1e9
     traced_rays = counts["direct"]["rays"] + counts["diffuse"]["rays"]
@@ -238,7 +249,11 @@ def bench_diurnal(args):
         "speedup_batched_vs_plain": round(solves_s / plain_solves_s, 2),
         "latency_ms": {"series_p50": pct(lat, 50), "series_p99": pct(lat, 99),
                        "plain_solve_p50": pct(plain_lat, 50), "plain_solve_p99": pct(plain_lat, 99)},
-        "kernel_ms_per_step": {k: round(v, 2) for k, v in per_step.items()},
+        "kernel_ms_per_step": {("grid_build" if k == "other" else k): round(v, 2) for k, v in per_step.items()},
+        "direct_occlusion": "grid" if grid else "bvh4-packet",
+        "grid_entries_per_prim": round(grid_entries, 2) if grid else None,
+        "direct_bytes_per_ray": round(bytes_per_ray(counts["direct"], grid=grid), 1),
+        "direct_candidates_per_ray": round(counts["direct"]["prims"] / counts["direct"]["rays"], 2),
         "rays_per_solve": rp["total"],
         "scene_build_ms": round(build_s * 1e3, 1),
         "autotune": tune,

@@ -254,9 +254,11 @@ int rt_set_launch_cfg(rt_ctx *ctx, int kernel, const rt_launch_cfg *cfg) {
     }
     if (cfg->blocks_per_cu < 1 || cfg->blocks_per_cu > 64) { set_error("blocks_per_cu must be in [1, 64]"); return -1; }
     if (cfg->chunk > 256) { set_error("chunk must be in [0, 256] (0 = 1)"); return -1; }
+    if (cfg->grid_density > 64) { set_error("grid_density must be in [0, 64] (0 = 2)"); return -1; }
     ctx->sx.cfg[kernel].waves_per_block = cfg->waves_per_block;
     ctx->sx.cfg[kernel].blocks_per_cu = cfg->blocks_per_cu;
     ctx->sx.cfg[kernel].chunk = cfg->chunk ? cfg->chunk : 1;
+    ctx->sx.cfg[kernel].grid_density = cfg->grid_density ? cfg->grid_density : 2;
     return 0;
 }
 
@@ -267,6 +269,7 @@ int rt_get_launch_cfg(rt_ctx *ctx, int kernel, rt_launch_cfg *cfg) {
     cfg->waves_per_block = ctx->sx.cfg[kernel].waves_per_block;
     cfg->blocks_per_cu = ctx->sx.cfg[kernel].blocks_per_cu;
     cfg->chunk = ctx->sx.cfg[kernel].chunk;
+    cfg->grid_density = ctx->sx.cfg[kernel].grid_density;
     return 0;
 }
 
@@ -302,6 +305,7 @@ int rt_get_timing(rt_ctx *ctx, rt_timing *out) {
     NN(out, "out");
     *out = ctx->sx.timing_info;
     out->direct_grid = ctx->sx.last_direct_grid ? 1u : 0u;
+    out->grid_entries = ctx->sx.last_direct_grid ? ctx->sx.last_grid_entries : 0u;
     return 0;
 }
 

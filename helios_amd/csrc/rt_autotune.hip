@@ -7,11 +7,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <string>
 #include <vector>
 
 #include "rt_ctx.h"
+#include "rt_solve.h"
 
 using namespace rt;
 
@@ -111,8 +113,11 @@ extern "C" int rt_autotune(rt_ctx *ctx, const rt_scene *scene, const rt_band_req
     if (rc == 0) {
         DirectJob hdj{};
         hdj.sc = S.d;
+        // the request's first source direction, or an oblique default when it has none (a batch's base
+        // request carries no sun of its own)
         const float up[3] = {0.3f, 0.2f, 0.93f};
         const float *sd = (req->nsources && req->source_dir) ? req->source_dir : up;
+        if (!(sd[0] * sd[0] + sd[1] * sd[1] + sd[2] * sd[2] > 1e-12f)) sd = up;
         hdj.sdir[0] = sd[0]; hdj.sdir[1] = sd[1]; hdj.sdir[2] = sd[2];
         hdj.d = d; hdj.seed = 12345u; hdj.out = dsum; hdj.item_begin = 0; hdj.P = P;
         DiffuseJob hfj{};
@@ -152,16 +157,58 @@ extern "C" int rt_autotune(rt_ctx *ctx, const rt_scene *scene, const rt_band_req
     LaunchCfg best[3];
     double best_ms[3] = {0, 0, 0}, def_ms[3] = {0, 0, 0};
     uint32_t count = 0;
+    // occlusion-grid density of the direct pass (non-periodic scenes): grids over the whole scene for the
+    // sample direction, traced with the default shape; a density costs its build plus the sample trace
+    // scaled to all P primitives.  The shapes are then searched at the chosen density.
+    GridSet gs;
+    uint32_t density = 2;
+    bool use_grid = false;
+    if (rc == 0 && ctx->sx.use_grid) {
+        DirectJob hdj{};
+        if (hipMemcpy(&hdj, dj, sizeof(hdj), hipMemcpyDeviceToHost) != hipSuccess) rc = -1;
+        double best_d = -1.0;
+        for (uint32_t dens : {1u, 2u, 4u, 8u}) {
+            if (rc) break;
+            gs.density = (float)dens;
+            if (hipStreamSynchronize(st) != hipSuccess) { rc = -1; break; }
+            const auto t0 = std::chrono::steady_clock::now();
+            int g = build_sun_grids(gs, S, hdj.sdir, 1, st);
+            if (g < 0) { rc = -1; break; }
+            if (g > 0) break;  // grids do not apply to this scene (or not at this density)
+            const double build_ms =
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            DirectJob j = hdj;
+            j.grid = gs.dev;
+            if (hipMemcpy(dj, &j, sizeof(j), hipMemcpyHostToDevice) != hipSuccess) { rc = -1; break; }
+            const LaunchCfg def{};
+            double ms = time_cfg(st, tm, def, [&](const LaunchCfg &c) {
+                return launch_direct(dj, 1, sample, depth, counter, err, nullptr, c, ncu, st);
+            });
+            count++;
+            if (ms < 0) { rc = -1; break; }
+            const double cost = ms * ((double)P / sample) + build_ms;
+            if (best_d < 0 || cost < 0.97 * best_d) { best_d = cost; density = dens; }
+            use_grid = true;
+        }
+        if (rc == 0 && use_grid) {
+            gs.density = (float)density;
+            if (build_sun_grids(gs, S, hdj.sdir, 1, st) != 0) rc = -1;
+            DirectJob j = hdj;
+            j.grid = gs.dev;
+            if (rc == 0 && hipMemcpy(dj, &j, sizeof(j), hipMemcpyHostToDevice) != hipSuccess) rc = -1;
+        }
+    }
     if (rc == 0)
         rc = search(st, tm, trace_c,
                     [&](const LaunchCfg &c) {
                         return launch_direct(dj, 1, sample, depth, counter, err, nullptr, c, ncu, st);
                     },
                     best[0], best_ms[0], def_ms[0], count);
+    best[0].grid_density = density;
     if (rc == 0)
         rc = search(st, tm, trace_c,
                     [&](const LaunchCfg &c) {
-                        return launch_diffuse(fj, 1, sample, depth, counter, err, nullptr, c, ncu, st);
+                        return launch_diffuse(fj, sample, depth, counter, err, nullptr, c, ncu, st);
                     },
                     best[1], best_ms[1], def_ms[1], count);
     if (rc == 0)
@@ -175,12 +222,14 @@ extern "C" int rt_autotune(rt_ctx *ctx, const rt_scene *scene, const rt_band_req
     }
     (void)hipStreamSynchronize(st);
     (void)hipFree(mem);
+    gs.release();
     if (rc != 0) return -1;
     for (int k = 0; k < 3; k++) {
         ctx->sx.cfg[k] = best[k];
         out->cfg[k].waves_per_block = best[k].waves_per_block;
         out->cfg[k].blocks_per_cu = best[k].blocks_per_cu;
         out->cfg[k].chunk = best[k].chunk;
+        out->cfg[k].grid_density = best[k].grid_density;
         out->ms[k] = best_ms[k];
         out->default_ms[k] = def_ms[k];
     }

@@ -346,6 +346,14 @@ int rt_launch_direct_rays(rt_ctx *ctx, const rt_launch_params *p) {
         j.item_begin = s * p->launch_count + range_begin(p->launch_offset);
         j.P = P;
     }
+    // occlusion grids per source direction (non-periodic scenes), as in rt_run_band
+    if (ctx->sx.use_grid) {
+        ctx->sx.grids.density = (float)ctx->sx.cfg[0].grid_density;
+        int g = build_sun_grids(ctx->sx.grids, be.scene, be.src_dir.data(), S, st);
+        if (g < 0) return -1;
+        if (g == 0)
+            for (uint32_t s = 0; s < S; s++) hj[s].grid = ctx->sx.grids.dev + s;
+    }
     CKR(hipMemsetAsync(dsum, 0, (size_t)S * P * 2 * 4, st));
     CKR(hipMemsetAsync(err, 0, sizeof(int), st));
     CKR(hipMemcpyAsync(dmap, map.data(), Bl * 4, hipMemcpyHostToDevice, st));
@@ -453,7 +461,7 @@ int rt_launch_diffuse_rays(rt_ctx *ctx, const rt_launch_params *p) {
     CKR(hipStreamSynchronize(st));
     k_select_bands<<<nblk(PB), 256, 0, st>>>(P, be.Bg, Bl, dmap, be.rho(), be.tau(), rl, tl);
     CKR(hipGetLastError());
-    if (launch_diffuse(fjob, 1, p->launch_count, be.scene.h.depth, counter, err, ctx->sx.stat_ptr(1), ctx->sx.cfg[1],
+    if (launch_diffuse(fjob, p->launch_count, be.scene.h.depth, counter, err, ctx->sx.stat_ptr(1), ctx->sx.cfg[1],
                        ctx->sx.ncu, st))
         return -1;
     if (launch_gather(gjob, 1, p->launch_count, Bl, counter, ctx->sx.cfg[2], ctx->sx.ncu, st)) return -1;

@@ -101,7 +101,7 @@ __global__ void k_grid_bounds(uint32_t nleaf, const float4 *__restrict__ rec, fl
 
 __global__ void k_grid_count(uint32_t nleaf, const float4 *__restrict__ rec, float smax,
                              const SunGrid *__restrict__ grids, const uint32_t *__restrict__ cell_base,
-                             uint32_t *__restrict__ counts, unsigned long long *__restrict__ entries) {
+                             uint32_t *__restrict__ counts, unsigned long long *__restrict__ entries /*[n]*/) {
     const SunGrid G = grids[blockIdx.y];
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nleaf) return;
@@ -109,16 +109,20 @@ __global__ void k_grid_count(uint32_t nleaf, const float4 *__restrict__ rec, flo
     proj_rect(rec, k, G, smax, u0, u1, v0, v1);
     const int x0 = cell_of(u0, G.g0u, G.inv_cs, G.nx), x1 = cell_of(u1, G.g0u, G.inv_cs, G.nx);
     const int y0 = cell_of(v0, G.g0v, G.inv_cs, G.ny), y1 = cell_of(v1, G.g0v, G.inv_cs, G.ny);
-    atomicAdd(entries, (unsigned long long)(x1 - x0 + 1) * (unsigned long long)(y1 - y0 + 1));
+    atomicAdd(&entries[blockIdx.y], (unsigned long long)(x1 - x0 + 1) * (unsigned long long)(y1 - y0 + 1));
     uint32_t *c = counts + cell_base[blockIdx.y];
     for (int y = y0; y <= y1; y++)
         for (int x = x0; x <= x1; x++) atomicAdd(&c[(size_t)y * G.nx + x], 1u);
 }
 
+// start[] is one exclusive scan over the cells of all directions in uint32 arithmetic, so it holds the
+// true offsets modulo 2^32; a direction's own list starts at the 64-bit offset item_base[j] and its
+// offsets count from sbase = item_base[j] mod 2^32 (each direction has fewer than 2^32 entries).
 __global__ void k_grid_fill(uint32_t nleaf, const float4 *__restrict__ rec, float smax,
                             const SunGrid *__restrict__ grids, const uint32_t *__restrict__ cell_base,
-                            const uint32_t *__restrict__ start, uint32_t *__restrict__ counts,
-                            uint32_t *__restrict__ items, uint32_t total) {
+                            const unsigned long long *__restrict__ item_base,
+                            const unsigned long long *__restrict__ item_count, const uint32_t *__restrict__ start,
+                            uint32_t *__restrict__ counts, uint32_t *__restrict__ items) {
     const SunGrid G = grids[blockIdx.y];
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nleaf) return;
@@ -127,15 +131,17 @@ __global__ void k_grid_fill(uint32_t nleaf, const float4 *__restrict__ rec, floa
     const int x0 = cell_of(u0, G.g0u, G.inv_cs, G.nx), x1 = cell_of(u1, G.g0u, G.inv_cs, G.nx);
     const int y0 = cell_of(v0, G.g0v, G.inv_cs, G.ny), y1 = cell_of(v1, G.g0v, G.inv_cs, G.ny);
     const size_t base = cell_base[blockIdx.y];
+    const unsigned long long ib = item_base[blockIdx.y], ni = item_count[blockIdx.y];
+    const uint32_t sbase = (uint32_t)ib;
     for (int y = y0; y <= y1; y++)
         for (int x = x0; x <= x1; x++) {
             const size_t c = base + (size_t)y * G.nx + x;
             // the ranges equal the counting kernel's (same code, same FP32 operations); the guards only
-            // keep a mismatch from writing outside the list
+            // keep a mismatch from writing outside the direction's list
             const uint32_t left = atomicSub(&counts[c], 1u);
             if (left == 0u) continue;
-            const uint32_t at = start[c] + left - 1u;
-            if (at < total) items[at] = k;
+            const unsigned long long at = (unsigned long long)(uint32_t)(start[c] - sbase) + left - 1u;
+            if (at < ni) items[ib + at] = k;
         }
 }
 
@@ -177,16 +183,18 @@ int build_sun_grids(GridSet &gs, const Scene &S, const float *dirs, uint32_t n, 
         G.g0u = G.g0v = 0.f;
         G.inv_cs = 1.f;
         G.nx = G.ny = 1;
+        G.sbase = 0;
         G.start = nullptr;
         G.items = nullptr;
     }
     // 1. projected extents
-    if (gs.small.reserve(n * sizeof(SunGrid) + n * 16 + n * 8 + 2048)) return -1;
+    if (gs.small.reserve(n * sizeof(SunGrid) + n * 16 + n * 4 + n * 16 + 2048)) return -1;
     char *p = (char *)gs.small.base;
     SunGrid *dgrids = (SunGrid *)p;
     uint32_t *bounds = (uint32_t *)(p + ((n * sizeof(SunGrid) + 255) & ~(size_t)255));
     uint32_t *dbase = bounds + 4 * n;
-    unsigned long long *dentries = (unsigned long long *)(dbase + ((n + 63) & ~(size_t)63));
+    unsigned long long *dentries = (unsigned long long *)(dbase + ((n + 63) & ~(size_t)63));  // [n]
+    unsigned long long *dibase = dentries + n;                                                // [n]
     GK(hipMemcpyAsync(dgrids, hg.data(), n * sizeof(SunGrid), hipMemcpyHostToDevice, st));
     GK(hipMemsetAsync(bounds, 0xFF, n * 16, st));
     dim3 grid(nblk(nleaf), n);
@@ -195,10 +203,11 @@ int build_sun_grids(GridSet &gs, const Scene &S, const float *dirs, uint32_t n, 
     std::vector<uint32_t> hb((size_t)n * 4);
     GK(hipMemcpyAsync(hb.data(), bounds, n * 16, hipMemcpyDeviceToHost, st));
     GK(hipStreamSynchronize(st));
-    // 2. grid shapes: about two cells per primitive over the projected extent
+    // 2. grid shapes: `density` cells per primitive (default 2) over the projected extent
     std::vector<uint32_t> base(n + 1, 0);
     size_t total_cells = 0;
-    const double target = std::min(std::max(2.0 * nleaf, 1024.0), (double)(1u << 24));
+    const double density = gs.density > 0.f ? (double)gs.density : 2.0;
+    const double target = std::min(std::max(density * nleaf, 1024.0), (double)(1u << 24));
     for (uint32_t j = 0; j < n; j++) {
         SunGrid &G = hg[j];
         float u0 = o2f_g(hb[j * 4]), u1 = -o2f_g(hb[j * 4 + 1]), v0 = o2f_g(hb[j * 4 + 2]), v1 = -o2f_g(hb[j * 4 + 3]);
@@ -230,24 +239,31 @@ int build_sun_grids(GridSet &gs, const Scene &S, const float *dirs, uint32_t n, 
     GK(hipMemcpyAsync(dbase, base.data(), n * 4, hipMemcpyHostToDevice, st));
     GK(hipMemcpyAsync(dgrids, hg.data(), n * sizeof(SunGrid), hipMemcpyHostToDevice, st));
     GK(hipMemsetAsync(counts, 0, (total_cells + 1) * 4, st));
-    GK(hipMemsetAsync(dentries, 0, 8, st));
+    GK(hipMemsetAsync(dentries, 0, (size_t)n * 8, st));
     k_grid_count<<<grid, 256, 0, st>>>(nleaf, S.h.rec, S.h.smax, dgrids, dbase, counts, dentries);
     GK(hipGetLastError());
-    unsigned long long entries = 0;
-    GK(hipMemcpyAsync(&entries, dentries, 8, hipMemcpyDeviceToHost, st));
+    std::vector<unsigned long long> ent(n), ibase(n);
+    GK(hipMemcpyAsync(ent.data(), dentries, (size_t)n * 8, hipMemcpyDeviceToHost, st));
     GK(hipStreamSynchronize(st));
-    if (entries + 1 > (1ull << 31)) return 1;  // uint32 positions: traverse the BVH instead
-    if (gs.cap && cell_bytes + entries * 4 + 256 > gs.cap) return 1;
+    unsigned long long total = 0;
+    for (uint32_t j = 0; j < n; j++) {
+        if (ent[j] >= (1ull << 32) - 1) return 1;  // a direction's offsets are uint32: traverse the BVH
+        ibase[j] = total;
+        total += ent[j];
+    }
+    if (gs.cap && cell_bytes + total * 4 + 256 > gs.cap) return 1;
     GK(hipcub::DeviceScan::ExclusiveSum(tmp, scan_bytes, counts, starts, (int)(total_cells + 1), st));
-    const uint32_t total = (uint32_t)entries;
     if (gs.items.reserve((size_t)total * 4 + 256)) return -1;
     uint32_t *items = (uint32_t *)gs.items.base;
     GK(hipMemsetAsync(items, 0xFF, (size_t)total * 4, st));  // an unfilled slot reads as "no primitive"
-    k_grid_fill<<<grid, 256, 0, st>>>(nleaf, S.h.rec, S.h.smax, dgrids, dbase, starts, counts, items, total);
+    GK(hipMemcpyAsync(dibase, ibase.data(), (size_t)n * 8, hipMemcpyHostToDevice, st));
+    k_grid_fill<<<grid, 256, 0, st>>>(nleaf, S.h.rec, S.h.smax, dgrids, dbase, dibase, dentries, starts, counts,
+                                      items);
     GK(hipGetLastError());
     for (uint32_t j = 0; j < n; j++) {
         hg[j].start = starts + base[j];
-        hg[j].items = items;
+        hg[j].items = items + ibase[j];
+        hg[j].sbase = (uint32_t)ibase[j];
     }
     GK(hipMemcpyAsync(dgrids, hg.data(), n * sizeof(SunGrid), hipMemcpyHostToDevice, st));
     GK(hipStreamSynchronize(st));

@@ -31,14 +31,16 @@ struct LaunchCfg {
     uint32_t waves_per_block = 4;  // 64-lane waves per workgroup
     uint32_t blocks_per_cu = 4;    // persistent workgroups per CU
     uint32_t chunk = 1;            // work items taken per work-queue fetch
+    uint32_t grid_density = 2;     // direct pass: occlusion-grid cells per primitive
 };
 
 // kernel launchers (rt_trace.hip)
 // stats (may be null): [0] BVH node fetches, [1] primitive tests, [2] rays traced
 int launch_direct(const DirectJob *jobs_dev, uint32_t njobs, uint32_t total_items, uint32_t depth, uint32_t *counter,
                   int *err, unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st);
-int launch_diffuse(const DiffuseJob *jobs_dev, uint32_t njobs, uint32_t total_items, uint32_t depth, uint32_t *counter,
-                   int *err, unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st);
+// one diffuse job (device pointer) per launch: items [0, total_items) of that job
+int launch_diffuse(const DiffuseJob *job_dev, uint32_t total_items, uint32_t depth, uint32_t *counter, int *err,
+                   unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st);
 int launch_gather(const GatherJob *jobs_dev, uint32_t njobs, uint32_t total_items, uint32_t cmax, uint32_t *counter,
                   const LaunchCfg &cfg, int ncu, hipStream_t st);
 int launch_cast(const DevScene *sc, uint32_t depth, uint64_t n, const float *o, const float *d, float maxd, float *t,

@@ -29,9 +29,10 @@ struct GridSet {
     Workspace small, cells, items;
     SunGrid *dev = nullptr;  // [n] device headers
     uint32_t n = 0;
-    uint32_t entries = 0;
+    size_t entries = 0;      // list entries over all directions
     size_t cells_total = 0;
     size_t cap = 0;          // byte limit for cells + items (0 = none); above it the BVH is traversed
+    float density = 2.f;     // grid cells per traced primitive
     // bytes a server reserves for the grids of one solve of `njobs` directions over P primitives
     static size_t budget(uint32_t P, uint32_t njobs) { return (size_t)96 * P * njobs + (1u << 20); }
     void release() { small.release(); cells.release(); items.release(); dev = nullptr; n = 0; }
@@ -55,6 +56,7 @@ struct SolveCtx {
     unsigned long long *stats = nullptr;  // device, 9 counters
     bool use_grid = true;                 // occlusion grids for direct passes of non-periodic scenes
     bool last_direct_grid = false;        // the last solve's direct pass used them
+    uint32_t last_grid_entries = 0;       // their list entries per direction per primitive, x100
     GridSet grids;
     unsigned long long *stat_ptr(int k) { return counting ? stats + 3 * k : nullptr; }
 };

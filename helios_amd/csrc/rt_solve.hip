@@ -372,12 +372,15 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
             hipEvent_t ga, gb;
             if (ev_pair(sx, ga, gb)) return -1;
             if (ga) CK(hipEventRecord(ga, st));
+            sx.grids.density = (float)sx.cfg[0].grid_density;
             int g = build_sun_grids(sx.grids, S, jd.data(), (uint32_t)hd.size(), st);
             if (g < 0) return -1;
             if (gb) { CK(hipEventRecord(gb, st)); tick(sx, 3, ga, gb); }
             if (g == 0) {
                 for (size_t j = 0; j < hd.size(); j++) hd[j].grid = sx.grids.dev + j;
                 sx.last_direct_grid = true;
+                const double per = (double)sx.grids.entries / ((double)hd.size() * std::max<uint32_t>(1u, S.h.nleaf));
+                sx.last_grid_entries = (uint32_t)std::min(4.0e9, std::round(per * 100.0));
             }
         }
         CK(hipMemcpyAsync(djobs, hd.data(), hd.size() * sizeof(DirectJob), hipMemcpyHostToDevice, st));
@@ -428,7 +431,7 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
         hipEvent_t a, b;
         if (ev_pair(sx, a, b)) return -1;
         if (a) CK(hipEventRecord(a, st));
-        if (launch_diffuse(fjobs + g, 1, P, S.h.depth, counter, err, sx.stat_ptr(1), sx.cfg[1], sx.ncu, st)) return -1;
+        if (launch_diffuse(fjobs + g, P, S.h.depth, counter, err, sx.stat_ptr(1), sx.cfg[1], sx.ncu, st)) return -1;
         if (b) { CK(hipEventRecord(b, st)); tick(sx, 1, a, b); }
         sx.timing_info.diffuse_rays += (uint64_t)P * 2 * nn;
         sx.timing_info.diffuse_launches++;
@@ -507,7 +510,7 @@ int debug_diffuse_hits(SolveCtx &sx, const Scene &S, uint32_t n, uint32_t seed, 
     fj.sc = S.d; fj.n = n; fj.seed = seed; fj.hits = hits; fj.item_begin = 0; fj.P = P;
     CK(hipMemsetAsync(err, 0, 4, st));
     CK(hipMemcpyAsync(fjob, &fj, sizeof(fj), hipMemcpyHostToDevice, st));
-    if (launch_diffuse(fjob, 1, P, S.h.depth, counter, err, sx.stat_ptr(1), sx.cfg[1], sx.ncu, st)) return -1;
+    if (launch_diffuse(fjob, P, S.h.depth, counter, err, sx.stat_ptr(1), sx.cfg[1], sx.ncu, st)) return -1;
     CK(hipMemcpyAsync(out, hits, (size_t)P * 2 * nn * 4, hipMemcpyDeviceToHost, st));
     CK(hipStreamSynchronize(st));
     int herr = 0;
@@ -535,6 +538,7 @@ int debug_direct_sums(SolveCtx &sx, const Scene &S, uint32_t d, uint32_t seed, u
         j.d = d; j.seed = seed; j.out = sums + (size_t)s * P * 2; j.item_begin = s * P; j.P = P;
     }
     if (sx.use_grid) {
+        sx.grids.density = (float)sx.cfg[0].grid_density;
         int g = build_sun_grids(sx.grids, S, src_dir, nsrc, st);
         if (g < 0) return -1;
         if (g == 0)

@@ -82,6 +82,17 @@ __device__ __forceinline__ T *uptr(T *p) {
     return (T *)(((uint64_t)hi << 32) | lo);
 }
 
+// Load of a value at a wave-uniform address through the constant address space (scalar load).
+template <typename T>
+__device__ __forceinline__ T ldu(const T *p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    typedef __attribute__((address_space(4))) const T cT;
+    return *(const cT *)p;
+#else
+    return *p;
+#endif
+}
+
 // Intersect the primitive in leaf slot k; returns its index (RT_MISS if skipped) and the hit.
 template <bool CD>
 __device__ __forceinline__ bool leaf_hit_p(const float4 *recs, uint32_t k, v3f o, v3f d, uint32_t skip, uint32_t &id,
@@ -628,7 +639,7 @@ __device__ __forceinline__ bool grid_occluded(const SunGrid &G, const float4 *re
     const int cx = fx < 0.f ? 0 : (fx > (float)(G.nx - 1) ? (int)(G.nx - 1) : (int)fx);
     const int cy = fy < 0.f ? 0 : (fy > (float)(G.ny - 1) ? (int)(G.ny - 1) : (int)fy);
     const uint32_t c = (uint32_t)cy * G.nx + (uint32_t)cx;
-    const uint32_t b = G.start[c], e = G.start[c + 1];
+    const uint32_t b = G.start[c] - G.sbase, e = G.start[c + 1] - G.sbase;
     for (uint32_t p = b; p < e; p++) {
         const uint32_t k = G.items[p];
         if (k >= nleaf) continue;
@@ -715,63 +726,209 @@ __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict
     add_stats(stats, cnt, nray);
 }
 
+// Resumable closest-hit traversal state of one lane (BVH2), for kernels that refill finished lanes.
+struct LaneRay {
+    v3f o, d;
+    RayPre rp;
+    float best_t;
+    uint32_t best, ni, budget;
+    int sp;
+};
+
+__device__ __forceinline__ void lane_start(LaneRay &L, uint32_t nleaf) {
+    L.rp = ray_pre(L.o, L.d);
+    L.best_t = 1e30f;
+    L.best = RT_MISS;
+    L.ni = 0;
+    L.sp = 0;
+    L.budget = nleaf + 8;
+}
+
+// Starts the closest-hit trace of L.o, L.d; returns false when it is already complete (a scene with at
+// most one traced primitive needs no node steps), as trace_closest() with tstart = 1e30.
+template <typename C>
+__device__ __forceinline__ bool lane_begin(LaneRay &L, const float4 *recs, uint32_t nleaf, float tmin, uint32_t skip,
+                                           C &cnt) {
+    lane_start(L, nleaf);
+    if (nleaf > 1) return true;
+    uint32_t slot_unused;
+    if (nleaf == 1) { cnt.prims++; test_leaf<false>(recs, 0, L.o, L.d, tmin, skip, L.best_t, L.best, slot_unused); }
+    return false;
+}
+
+// One node visit of trace_closest() (radiation predicates, no tstart); returns true when the traversal
+// has ended.  The visited nodes, tested primitives and the result equal trace_closest()'s.
+template <int STK, typename C>
+__device__ __forceinline__ bool lane_step(const float4 *nodes, const float4 *recs, uint32_t nleaf, LaneRay &L,
+                                          float tmin, uint32_t skip, uint32_t *stk, uint32_t stride, int *err,
+                                          C &cnt) {
+    if (L.budget-- == 0 || L.ni >= nleaf - 1) {
+        atomicOr(err, 2);
+        return true;
+    }
+    cnt.nodes++;
+    uint32_t slot_unused;
+    NodeTest nt = node_test_p(nodes, L.ni, L.rp, L.best_t);
+    bool l0 = (nt.c0 & RT_LEAF_BIT) != 0, l1 = (nt.c1 & RT_LEAF_BIT) != 0;
+    if (nt.h0 && l0) { cnt.prims++; test_leaf<false>(recs, nt.c0 & ~RT_LEAF_BIT, L.o, L.d, tmin, skip, L.best_t, L.best, slot_unused); }
+    if (nt.h1 && l1) { cnt.prims++; test_leaf<false>(recs, nt.c1 & ~RT_LEAF_BIT, L.o, L.d, tmin, skip, L.best_t, L.best, slot_unused); }
+    bool g0 = nt.h0 && !l0, g1 = nt.h1 && !l1;
+    if (g0 && g1) {
+        bool first0 = nt.tn0 <= nt.tn1;
+        if (L.sp >= STK) {
+            atomicOr(err, 1);
+            return true;
+        }
+        stk[(uint32_t)L.sp * stride] = first0 ? nt.c1 : nt.c0;
+        L.sp++;
+        L.ni = first0 ? nt.c0 : nt.c1;
+    } else if (g0) {
+        L.ni = nt.c0;
+    } else if (g1) {
+        L.ni = nt.c1;
+    } else {
+        if (L.sp == 0) return true;
+        L.sp--;
+        L.ni = stk[(uint32_t)L.sp * stride];
+    }
+    return false;
+}
+
 // ------------------------------------------------------------------------------------------------
 // diffuse / scattering rays -> hit words
 // ------------------------------------------------------------------------------------------------
+// A wave keeps 64 rays in flight.  Rays of one primitive are handed to idle lanes in ray order; a lane
+// whose traversal ends writes its hit word and takes the primitive's next ray, and the wave opens the
+// next primitive of the work queue when the current one has no rays left.  The traversal loop runs until
+// fewer than RT_REFILL_BELOW lanes are still traversing (all of them once the queue is empty), so lanes do
+// not idle for the longest ray of a fixed group of 64.  Every ray is traced exactly as a plain
+// per-ray trace_loop() would trace it; only the assignment of rays to lanes and steps differs.
+#define RT_REFILL_BELOW 48
 template <int WPB, int STK, bool COUNT>
-__global__ __launch_bounds__(WPB * 64) void k_diffuse(const DiffuseJob *__restrict__ jobs, uint32_t njobs,
-                                                      uint32_t total, uint32_t *counter, uint32_t chunk,
-                                                      int *err, unsigned long long *stats) {
+__global__ __launch_bounds__(WPB * 64) void k_diffuse(const DiffuseJob *__restrict__ job, uint32_t total,
+                                                      uint32_t *counter, uint32_t chunk, int *err,
+                                                      unsigned long long *stats) {
     __shared__ uint32_t stk_all[STK * WPB * 64];
     uint32_t *stk = stk_all + threadIdx.x;
     const uint32_t stride = WPB * 64;
     const uint32_t lane = threadIdx.x & 63;
     typename std::conditional<COUNT, TCnt, NoCnt>::type cnt{};
     uint32_t nray = 0;
-    for (uint32_t first = wave_item(counter, chunk); first < total; first = wave_item(counter, chunk)) {
-        const uint32_t end = min(first + chunk, total);
-        for (uint32_t item = first; item < end; ++item) {
-            uint32_t j = find_job(jobs, njobs, item);
-            const DiffuseJob &J = jobs[j];
-            const DevScene &sc = *J.sc;
-            uint32_t i = item - J.item_begin;
-            if (J.order) i = J.order[i];
-            uint32_t n = J.n, nn = n * n;
-            uint32_t *w = J.hits + (size_t)i * 2 * nn;
-            uint32_t ty = sc.type[i];
-            if (!traced_type(ty)) {
-                for (uint32_t r = lane; r < 2 * nn; r += 64) w[r] = RT_NOT_LAUNCHED;
+    // job and scene (wave-uniform, scalar loads)
+    const DevScene &sc = *ldu(&job->sc);
+    const float4 *nodes = ldu(&sc.node), *recs = ldu(&sc.rec), *nrm = ldu(&sc.nrm), *frame = ldu(&sc.frame);
+    const float *T = ldu(&sc.T);
+    const uint32_t *type = ldu(&sc.type);
+    const uint32_t nleaf = ldu(&sc.nleaf), nbbox = ldu(&sc.nbbox);
+    const uint32_t n = ldu(&job->n), nn = n * n, seed = ldu(&job->seed);
+    const uint32_t faces = ldu(&job->faces) ? ldu(&job->faces) : 3u;
+    const uint32_t item_begin = ldu(&job->item_begin);
+    const uint32_t *order = ldu(&job->order);
+    uint32_t *hits = ldu(&job->hits);
+    // work queue and open primitive (wave-uniform); the primitive's frame and transform are reloaded
+    // (scalar) whenever rays are generated, so they hold no registers across the traversal
+    uint32_t it = 0, it_end = 0;
+    bool drained = false;
+    uint32_t op_i = 0, op_next = 0, op_cnt = 0, op_base = 0;
+    bool patch = false, flip = false;
+    // lane state: busy = holds a ray, trav = its traversal has not ended
+    bool busy = false, trav = false;
+    LaneRay L;
+    L.o = L.d = mk3(0.f, 0.f, 0.f);
+    lane_start(L, nleaf);
+    uint32_t my_i = 0, my_r = 0, wraps = 0;
+    while (true) {
+        // finish lanes whose traversal ended: periodic wrap (as trace_loop: at most 32 traces), else write
+        if (busy && !trav) {
+            bool done = true;
+            if (nbbox) {
+                float bt = nearest_face(sc, L.o, L.d, L.best_t);
+                if (bt > 0.f && wraps < 31) {
+                    L.o = wrap_at(sc, L.o, L.d, bt);
+                    wraps++;
+                    trav = lane_begin(L, recs, nleaf, 1e-5f, my_i, cnt);
+                    done = false;
+                }
+            }
+            if (done) {
+                uint32_t word = RT_MISS;
+                if (L.best != RT_MISS) {
+                    float4 hn = nrm[L.best];
+                    bool hit_top = dot3(mk3(hn.x, hn.y, hn.z), L.d) < 0.0f;
+                    word = L.best | (hit_top ? 0u : RT_HIT_BOTTOM);
+                }
+                hits[(size_t)my_i * 2 * nn + my_r] = word;
+                busy = false;
+            }
+        }
+        // refill idle lanes
+        while (true) {
+            const uint64_t idle = __ballot(!busy);
+            if (idle == 0) break;
+            if (op_next == op_cnt) {
+                // open the next primitive of the queue
+                if (it == it_end) {
+                    if (drained) break;
+                    it = wave_item(counter, chunk);
+                    if (it >= total) { drained = true; it = it_end = 0; break; }
+                    it_end = min(it + chunk, total);
+                }
+                uint32_t i = it - item_begin;
+                if (order) i = ldu(order + i);
+                i = __builtin_amdgcn_readfirstlane(i);
+                it++;
+                uint32_t *w = hits + (size_t)i * 2 * nn;
+                const uint32_t ty = ldu(type + i);
+                if (!traced_type(ty)) {
+                    for (uint32_t r = lane; r < 2 * nn; r += 64) w[r] = RT_NOT_LAUNCHED;
+                    continue;
+                }
+                const uint32_t tw = __float_as_uint(ldu(&nrm[i].w));
+                const bool s0 = (faces & 1u) != 0, s1 = ((faces >> 1) & 1u) != 0 && tw != 0;
+                if (!s0) for (uint32_t r = lane; r < nn; r += 64) w[r] = RT_NOT_LAUNCHED;
+                if (!s1) for (uint32_t r = nn + lane; r < 2 * nn; r += 64) w[r] = RT_NOT_LAUNCHED;
+                op_i = i;
+                op_next = 0;
+                op_cnt = ((s0 ? 1u : 0u) + (s1 ? 1u : 0u)) * nn;
+                op_base = s0 ? 0u : nn;
+                patch = ty == 0 || ty == 3;
+                flip = tw == 1;
                 continue;
             }
-            uint32_t tw = __float_as_uint(sc.nrm[i].w);
-            float4 fr = sc.frame[i];
-            bool patch = ty == 0 || ty == 3;
-            uint32_t faces = J.faces ? J.faces : 3u;
-            float tm[12];
-            const float *Tp = sc.T + (size_t)i * 16;
-#pragma unroll
-            for (int q = 0; q < 12; q++) tm[q] = Tp[q];
-            for (uint32_t r = lane; r < 2 * nn; r += 64) {
-                uint32_t slot = r >= nn ? 1u : 0u;  // slot 0 = top launch (face 1), slot 1 = bottom (face 0)
-                uint32_t rr = r - slot * nn;
-                if ((slot == 1 && tw == 0) || !((faces >> slot) & 1u)) { w[r] = RT_NOT_LAUNCHED; continue; }
+            const uint32_t rank =
+                __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+            const uint32_t take = min((uint32_t)__popcll(idle), op_cnt - op_next);
+            if (!busy && rank < take) {
+                const uint32_t r = op_base + op_next + rank;
+                const uint32_t slot = r >= nn ? 1u : 0u;  // slot 0 = top launch (face 1), slot 1 = bottom (face 0)
+                const uint32_t rr = r - slot * nn;
                 float spx, spy;
                 v3f dl;
-                diffuse_sample(i, rr % n, rr / n, n, J.seed, patch, spx, spy, dl);
-                v3f o = xform3(tm, spx, spy, 0.0f);
+                diffuse_sample(op_i, rr % n, rr / n, n, seed, patch, spx, spy, dl);
+                float tm[12];
+                const float *Tp = T + (size_t)op_i * 16;
+#pragma unroll
+                for (int q = 0; q < 12; q++) tm[q] = ldu(Tp + q);
+                const float4 fr = ldu(frame + op_i);
+                L.o = xform3(tm, spx, spy, 0.0f);
                 v3f dir = frame_rotate(dl, fr.x, fr.y, fr.z, fr.w);
-                if (slot == 1 && tw == 1) dir = mk3(-dir.x, -dir.y, -dir.z);
-                bool ex;
+                if (slot == 1 && flip) dir = mk3(-dir.x, -dir.y, -dir.z);
+                L.d = dir;
+                my_i = op_i;
+                my_r = r;
+                wraps = 0;
                 nray++;
-                uint32_t h = trace_loop<STK>(sc, o, dir, 1e-5f, i, ex, stk, stride, err, cnt);
-                uint32_t word = RT_MISS;
-                if (h != RT_MISS) {
-                    float4 hn = sc.nrm[h];
-                    bool hit_top = dot3(mk3(hn.x, hn.y, hn.z), dir) < 0.0f;
-                    word = h | (hit_top ? 0u : RT_HIT_BOTTOM);
-                }
-                w[r] = word;
+                busy = true;
+                trav = lane_begin(L, recs, nleaf, 1e-5f, my_i, cnt);
             }
+            op_next += take;
+        }
+        if (__ballot(busy) == 0) break;  // queue empty and nothing in flight
+        // traverse until fewer than RT_REFILL_BELOW lanes (once the queue is empty: no lane) still traverse
+        const uint32_t floor_lanes = drained ? 1u : RT_REFILL_BELOW;
+        while (true) {
+            if (trav && lane_step<STK>(nodes, recs, nleaf, L, 1e-5f, my_i, stk, stride, err, cnt)) trav = false;
+            if ((uint32_t)__popcll(__ballot(trav)) < floor_lanes) break;
         }
     }
     add_stats(stats, cnt, nray);
@@ -1092,7 +1249,7 @@ int launch_direct(const DirectJob *jobs, uint32_t njobs, uint32_t total, uint32_
     return 0;
 }
 
-int launch_diffuse(const DiffuseJob *jobs, uint32_t njobs, uint32_t total, uint32_t depth, uint32_t *counter, int *err,
+int launch_diffuse(const DiffuseJob *job, uint32_t total, uint32_t depth, uint32_t *counter, int *err,
                    unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st) {
     if (total == 0) return 0;
     const uint32_t ch = std::max(cfg.chunk, 1u);
@@ -1101,18 +1258,18 @@ int launch_diffuse(const DiffuseJob *jobs, uint32_t njobs, uint32_t total, uint3
     uint32_t g = grid_for(cfg, ncu, total, wpb);
     if (stats) {
         // counting variant: 4 waves per block
-        PICK_STK(depth, (k_diffuse<4, 32, true><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)),
-                 (k_diffuse<4, 64, true><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)));
+        PICK_STK(depth, (k_diffuse<4, 32, true><<<g, 256, 0, st>>>(job, total, counter, ch, err, stats)),
+                 (k_diffuse<4, 64, true><<<g, 256, 0, st>>>(job, total, counter, ch, err, stats)));
         LCHECK();
         return 0;
     }
     switch (wpb) {
-        case 1: PICK_STK(depth, (k_diffuse<1, 32, false><<<g, 64, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)),
-                         (k_diffuse<1, 64, false><<<g, 64, 0, st>>>(jobs, njobs, total, counter, ch, err, stats))); break;
-        case 2: PICK_STK(depth, (k_diffuse<2, 32, false><<<g, 128, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)),
-                         (k_diffuse<2, 64, false><<<g, 128, 0, st>>>(jobs, njobs, total, counter, ch, err, stats))); break;
-        default: PICK_STK(depth, (k_diffuse<4, 32, false><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)),
-                          (k_diffuse<4, 64, false><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch, err, stats))); break;
+        case 1: PICK_STK(depth, (k_diffuse<1, 32, false><<<g, 64, 0, st>>>(job, total, counter, ch, err, stats)),
+                         (k_diffuse<1, 64, false><<<g, 64, 0, st>>>(job, total, counter, ch, err, stats))); break;
+        case 2: PICK_STK(depth, (k_diffuse<2, 32, false><<<g, 128, 0, st>>>(job, total, counter, ch, err, stats)),
+                         (k_diffuse<2, 64, false><<<g, 128, 0, st>>>(job, total, counter, ch, err, stats))); break;
+        default: PICK_STK(depth, (k_diffuse<4, 32, false><<<g, 256, 0, st>>>(job, total, counter, ch, err, stats)),
+                          (k_diffuse<4, 64, false><<<g, 256, 0, st>>>(job, total, counter, ch, err, stats))); break;
     }
     LCHECK();
     return 0;

@@ -53,8 +53,10 @@ struct SunGrid {
     float g0u, g0v;        // grid origin
     float inv_cs;          // 1 / cell size
     uint32_t nx, ny;
-    const uint32_t *start; // [nx*ny + 1] into items
-    const uint32_t *items;
+    uint32_t sbase;        // start[] values count from sbase (modulo 2^32): cell c lists
+                           // items[start[c] - sbase, start[c + 1] - sbase)
+    const uint32_t *start; // [nx*ny + 1]
+    const uint32_t *items; // this direction's list
 };
 
 // one direct trace job: one scene, one collimated source

@@ -48,7 +48,8 @@ class rt_scene_info(C.Structure):
 
 
 class rt_launch_cfg(C.Structure):
-    _fields_ = [("waves_per_block", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("chunk", C.c_uint32)]
+    _fields_ = [("waves_per_block", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("chunk", C.c_uint32),
+                ("grid_density", C.c_uint32)]
 
 
 class rt_tune_result(C.Structure):
@@ -60,7 +61,7 @@ class rt_timing(C.Structure):
     _fields_ = [("direct_ms", C.c_double), ("diffuse_ms", C.c_double), ("gather_ms", C.c_double),
                 ("other_ms", C.c_double), ("direct_rays", C.c_uint64), ("diffuse_rays", C.c_uint64),
                 ("direct_launches", C.c_uint32), ("diffuse_launches", C.c_uint32), ("direct_grid", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("grid_entries", C.c_uint32)]
 
 
 class rt_launch_params(C.Structure):
@@ -223,14 +224,20 @@ class Context:
     def scene(self, transforms, types, twosided, uuids=None, periodic=(False, False)) -> "Scene":
         return Scene(self, transforms, types, twosided, uuids, periodic)
 
-    def set_launch_cfg(self, kernel: int, waves_per_block: int, blocks_per_cu: int, chunk: int = 1):
-        cfg = rt_launch_cfg(waves_per_block, blocks_per_cu, chunk)
+    def set_launch_cfg(self, kernel: int, waves_per_block: int, blocks_per_cu: int, chunk: int = 1,
+                       grid_density: int = 0):
+        cfg = rt_launch_cfg(waves_per_block, blocks_per_cu, chunk, grid_density)
         _check(self.lib.rt_set_launch_cfg(self._h, kernel, C.byref(cfg)))
 
     def get_launch_cfg(self, kernel: int):
         cfg = rt_launch_cfg()
         _check(self.lib.rt_get_launch_cfg(self._h, kernel, C.byref(cfg)))
         return cfg.waves_per_block, cfg.blocks_per_cu, cfg.chunk
+
+    def grid_density(self) -> int:
+        cfg = rt_launch_cfg()
+        _check(self.lib.rt_get_launch_cfg(self._h, 0, C.byref(cfg)))
+        return cfg.grid_density
 
     def autotune(self, scene: "Scene", req: "BandRequest", nsuns: int = 1) -> dict:
         """Time candidate launch shapes on a sample of `scene` and install the fastest per kernel class."""
@@ -241,7 +248,8 @@ class Context:
         return {"candidates": r.candidates, "sample_prims": r.sample_prims,
                 **{names[k]: {"waves_per_block": r.cfg[k].waves_per_block, "blocks_per_cu": r.cfg[k].blocks_per_cu,
                               "chunk": r.cfg[k].chunk, "ms": round(r.ms[k], 3),
-                              "default_ms": round(r.default_ms[k], 3)} for k in range(3)}}
+                              "default_ms": round(r.default_ms[k], 3)} for k in range(3)},
+                "grid_density": r.cfg[0].grid_density}
 
     def enable_timing(self, on: bool = True):
         _check(self.lib.rt_enable_timing(self._h, 1 if on else 0))

@@ -247,13 +247,15 @@ int rt_server_reset_stats(rt_server *srv);
 /* ---------------------------------------------------------------------------------------------
  * Launch tuning (no reference counterpart): waves per workgroup, persistent workgroups per CU and
  * work-queue chunking (primitives taken per queue fetch) of the direct, diffuse and gather kernels,
- * set by hand or picked by rt_autotune, which times candidate shapes on a sample of the caller's
- * own scene and request.  The traversal stack depth is picked per launch from the BVH depth.
+ * and the occlusion-grid density of the direct pass, set by hand or picked by rt_autotune, which
+ * times candidates on a sample of the caller's own scene and request.  The traversal stack depth is
+ * picked per launch from the BVH depth.
  * ------------------------------------------------------------------------------------------- */
 typedef struct {
     uint32_t waves_per_block;   /* 1, 2 or 4 (the gather kernels always use 4)  */
     uint32_t blocks_per_cu;     /* persistent workgroups per CU, 1..64          */
     uint32_t chunk;             /* work items per queue fetch, 0 or 1 = one     */
+    uint32_t grid_density;      /* direct kernel: occlusion-grid cells per primitive, 1..64 (0 = 2) */
 } rt_launch_cfg;
 #define RT_KERNEL_DIRECT 0
 #define RT_KERNEL_DIFFUSE 1
@@ -268,7 +270,8 @@ typedef struct {
     uint32_t sample_prims;  /* primitives in the timing sample                                     */
 } rt_tune_result;
 /* Times candidate shapes for the direct, diffuse and gather kernels on a sample of `scene` with the
- * ray counts, first source direction and band count of `req`, for batches of `nsuns` suns. */
+ * ray counts, first source direction (an oblique default when it has none or it is zero) and band
+ * count of `req`, for batches of `nsuns` suns. */
 int rt_autotune(rt_ctx *ctx, const rt_scene *scene, const rt_band_request *req, uint32_t nsuns, rt_tune_result *out);
 
 /* ---------------------------------------------------------------------------------------------
@@ -284,7 +287,7 @@ typedef struct {
     uint32_t direct_launches;
     uint32_t diffuse_launches;
     uint32_t direct_grid;   /* 1: the direct pass tested occlusion through per-direction grids, 0: BVH */
-    uint32_t reserved;
+    uint32_t grid_entries;  /* list entries per direction per traced primitive (x100) of those grids */
 } rt_timing;
 int rt_enable_timing(rt_ctx *ctx, int on);
 /* Traversal work counters (zeroed by rt_set_counting): out[9] = direct (BVH node fetches, primitive

@@ -41,12 +41,18 @@ def test_autotune_installs_fastest_shapes(ctx):
     R0 = native.run_band(g, req)
     try:
         res = ctx.autotune(g, req, nsuns=8)
-        assert res["candidates"] >= 60 and res["sample_prims"] > 0
+        # 4 grid densities + 25 direct + 25 diffuse + 13 gather shapes (non-periodic scene: grids apply)
+        assert res["candidates"] == 67 and res["sample_prims"] > 0
+        assert res["grid_density"] in (1, 2, 4, 8) and ctx.grid_density() == res["grid_density"]
         for k, name in enumerate(("direct", "diffuse", "gather")):
             r = res[name]
             assert 0 < r["ms"] <= r["default_ms"] + 1e-9
             assert ctx.get_launch_cfg(k) == (r["waves_per_block"], r["blocks_per_cu"], r["chunk"])
         R1 = native.run_band(g, req)
+        # a batch base without a sun of its own is tuned on an oblique default direction (grids still apply)
+        zero_dir = native.BandRequest(s.P, req.bands, rho=req.rho, tau=req.tau, direct_rays=36, diffuse_rays=36,
+                                      source_dirs=[[0.0, 0.0, 0.0]], source_flux=[[0.0, 0.0]])
+        assert ctx.autotune(g, zero_dir, nsuns=8)["candidates"] == 67
     finally:
         _restore(ctx)
     assert np.array_equal(R0, R1)
