@@ -640,13 +640,25 @@ __device__ __forceinline__ bool grid_occluded(const SunGrid &G, const float4 *re
     const int cy = fy < 0.f ? 0 : (fy > (float)(G.ny - 1) ? (int)(G.ny - 1) : (int)fy);
     const uint32_t c = (uint32_t)cy * G.nx + (uint32_t)cx;
     const uint32_t b = G.start[c] - G.sbase, e = G.start[c + 1] - G.sbase;
-    for (uint32_t p = b; p < e; p++) {
-        const uint32_t k = G.items[p];
-        if (k >= nleaf) continue;
-        uint32_t id;
-        float t;
-        cnt.prims++;
-        if (leaf_hit_p<false>(recs, k, o, d, skip, id, t) && t > 1e-4f && t <= 1e30f) return true;
+    const uint32_t *items = G.items;
+    // two candidates per iteration, all their loads issued before either test (the list entry -> record
+    // chain is latency-bound); a slot that is out of the list or unfilled loads record 0 and is ignored
+    for (uint32_t p = b; p < e; p += 2) {
+        const bool two = p + 1 < e;
+        const uint32_t k0 = items[p];
+        const uint32_t k1 = two ? items[p + 1] : RT_MISS;
+        const bool v0 = k0 < nleaf, v1 = k1 < nleaf;
+        const float4 *q0 = recs + (size_t)(v0 ? k0 : 0u) * 4, *q1 = recs + (size_t)(v1 ? k1 : 0u) * 4;
+        const float4 a0 = ldc(q0), a1 = ldc(q0 + 1), a2 = ldc(q0 + 2), a3 = ldc(q0 + 3);
+        const float4 b0 = ldc(q1), b1 = ldc(q1 + 1), b2 = ldc(q1 + 2), b3 = ldc(q1 + 3);
+        const uint32_t ta = __float_as_uint(a0.w), tb = __float_as_uint(b0.w);
+        float t0, t1;
+        const bool h0 = v0 && isect_rec<false>(a0, a1, a2, a3, (ta & RT_PATCH_BIT) != 0, o, d, t0) &&
+                        (ta & ~RT_PATCH_BIT) != skip && t0 > 1e-4f && t0 <= 1e30f;
+        const bool h1 = v1 && isect_rec<false>(b0, b1, b2, b3, (tb & RT_PATCH_BIT) != 0, o, d, t1) &&
+                        (tb & ~RT_PATCH_BIT) != skip && t1 > 1e-4f && t1 <= 1e30f;
+        cnt.prims += (v0 ? 1u : 0u) + ((v1 && !h0) ? 1u : 0u);  // the tests a one-at-a-time scan makes
+        if (h0 || h1) return true;
     }
     return false;
 }
@@ -671,30 +683,33 @@ __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict
     for (uint32_t first = wave_item(counter, chunk); first < total; first = wave_item(counter, chunk)) {
         const uint32_t end = min(first + chunk, total);
         for (uint32_t item = first; item < end; ++item) {
+            // job, scene and primitive data are wave-uniform: scalar loads keep them out of VGPRs
             uint32_t j = find_job(jobs, njobs, item);
             const DirectJob &J = jobs[j];
-            const DevScene &sc = *J.sc;
-            uint32_t i = item - J.item_begin;
-            if (J.order) i = J.order[i];
-            uint32_t ty = sc.type[i];
+            const DevScene &sc = *ldu(&J.sc);
+            uint32_t i = item - ldu(&J.item_begin);
+            const uint32_t *jorder = ldu(&J.order);
+            if (jorder) i = ldu(jorder + i);
+            i = __builtin_amdgcn_readfirstlane(i);
+            uint32_t ty = ldu(ldu(&sc.type) + i);
             float sum_out = 0.f;
             bool top = true;
-            const SunGrid *grid = J.grid;
-            const float4 *recs = sc.rec;
+            const SunGrid *grid = ldu(&J.grid);
+            const float4 *recs = ldu(&sc.rec);
             if (traced_type(ty)) {
-                float4 n4 = sc.nrm[i];
+                float4 n4 = ldu(ldu(&sc.nrm) + i);
                 v3f n = mk3(n4.x, n4.y, n4.z);
                 uint32_t tw = __float_as_uint(n4.w);
-                v3f dir = nrm3(mk3(J.sdir[0], J.sdir[1], J.sdir[2]));
-                uint32_t d = J.d, Nrays = d * d;
+                v3f dir = nrm3(mk3(ldu(&J.sdir[0]), ldu(&J.sdir[1]), ldu(&J.sdir[2])));
+                uint32_t d = ldu(&J.d), Nrays = d * d;
                 float strength = rdiv(fabsf(dot3(n, dir)), (float)Nrays);
                 top = dot3(dir, n) > 0.0f;
                 if (!(tw == 3 || (!top && tw == 0))) {
                     bool patch = ty == 0 || ty == 3;
                     float tm[12];
-                    const float *Tp = sc.T + (size_t)i * 16;
+                    const float *Tp = ldu(&sc.T) + (size_t)i * 16;
 #pragma unroll
-                    for (int q = 0; q < 12; q++) tm[q] = Tp[q];
+                    for (int q = 0; q < 12; q++) tm[q] = ldu(Tp + q);
                     ksum acc;
                     acc.init();
                     for (uint32_t base = 0; base < Nrays; base += 64) {
@@ -703,12 +718,12 @@ __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict
                         v3f o = mk3(0.f, 0.f, 0.f);
                         if (need) {
                             float spx, spy;
-                            direct_sample(i, r % d, r / d, d, J.seed, patch, spx, spy);
+                            direct_sample(i, r % d, r / d, d, ldu(&J.seed), patch, spx, spy);
                             o = xform3(tm, spx, spy, 0.0f);
                             nray++;
                         }
                         bool occl;
-                        if (grid) occl = need && grid_occluded(*grid, recs, sc.nleaf, o, dir, i, cnt);
+                        if (grid) occl = need && grid_occluded(*grid, recs, ldu(&sc.nleaf), o, dir, i, cnt);
                         else if (COUNT) occl = need && occluded<STK>(sc, o, dir, 1e-4f, i, wstk, WPB * 64, err, cnt);
                         else occl = packet_occluded<STK>(sc, o, dir, 1e-4f, i, need, wstk, wstk4, err, cnt);
                         if (need && !occl) acc.add(strength);
@@ -718,8 +733,9 @@ __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict
                 }
             }
             if (lane == 0) {
-                J.out[(size_t)i * 2 + 0] = top ? sum_out : 0.f;
-                J.out[(size_t)i * 2 + 1] = top ? 0.f : sum_out;
+                float *out = ldu(&J.out);
+                out[(size_t)i * 2 + 0] = top ? sum_out : 0.f;
+                out[(size_t)i * 2 + 1] = top ? 0.f : sum_out;
             }
         }
     }
@@ -756,43 +772,71 @@ __device__ __forceinline__ bool lane_begin(LaneRay &L, const float4 *recs, uint3
     return false;
 }
 
-// One node visit of trace_closest() (radiation predicates, no tstart); returns true when the traversal
-// has ended.  The visited nodes, tested primitives and the result equal trace_closest()'s.
-template <int STK, typename C>
-__device__ __forceinline__ bool lane_step(const float4 *nodes, const float4 *recs, uint32_t nleaf, LaneRay &L,
-                                          float tmin, uint32_t skip, uint32_t *stk, uint32_t stride, int *err,
-                                          C &cnt) {
+// LDS pointer: accesses compile to ds_read/ds_write (a generic pointer chosen between LDS and private
+// memory would compile to slower flat accesses that also wait on vector memory)
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+#else
+typedef uint32_t lds_u32;
+#endif
+
+// Traversal entries are inner nodes or leaves (RT_LEAF_BIT).  The stack holds its first LSTK entries in
+// LDS (stk, lane stride `stride`) and the rest, up to STK, in the lane's private array ovf (deep stacks
+// are rare; LDS per wave bounds the occupancy).  lane_pop() returns true when the stack is empty.
+template <int STK, int LSTK>
+__device__ __forceinline__ bool lane_pop(LaneRay &L, const lds_u32 *stk, const uint32_t *ovf, uint32_t stride) {
+    if (L.sp == 0) return true;
+    L.sp--;
+    if (LSTK >= STK || L.sp < LSTK) L.ni = stk[(uint32_t)L.sp * stride];
+    else L.ni = ovf[L.sp - LSTK];
+    return false;
+}
+
+// Visit the inner node L.ni: both child boxes against [0, best_t]; the nearer hit child (inner or leaf)
+// becomes the next entry and the farther one is pushed.  Returns true when the traversal has ended.
+template <int STK, int LSTK, typename C>
+__device__ __forceinline__ bool lane_node(const float4 *nodes, uint32_t nleaf, LaneRay &L, lds_u32 *stk, uint32_t *ovf,
+                                          uint32_t stride, int *err, C &cnt) {
     if (L.budget-- == 0 || L.ni >= nleaf - 1) {
         atomicOr(err, 2);
         return true;
     }
     cnt.nodes++;
-    uint32_t slot_unused;
     NodeTest nt = node_test_p(nodes, L.ni, L.rp, L.best_t);
-    bool l0 = (nt.c0 & RT_LEAF_BIT) != 0, l1 = (nt.c1 & RT_LEAF_BIT) != 0;
-    if (nt.h0 && l0) { cnt.prims++; test_leaf<false>(recs, nt.c0 & ~RT_LEAF_BIT, L.o, L.d, tmin, skip, L.best_t, L.best, slot_unused); }
-    if (nt.h1 && l1) { cnt.prims++; test_leaf<false>(recs, nt.c1 & ~RT_LEAF_BIT, L.o, L.d, tmin, skip, L.best_t, L.best, slot_unused); }
-    bool g0 = nt.h0 && !l0, g1 = nt.h1 && !l1;
-    if (g0 && g1) {
+    if (nt.h0 && nt.h1) {
         bool first0 = nt.tn0 <= nt.tn1;
         if (L.sp >= STK) {
             atomicOr(err, 1);
             return true;
         }
-        stk[(uint32_t)L.sp * stride] = first0 ? nt.c1 : nt.c0;
+        const uint32_t far = first0 ? nt.c1 : nt.c0;
+        if (LSTK >= STK || L.sp < LSTK) stk[(uint32_t)L.sp * stride] = far;
+        else ovf[L.sp - LSTK] = far;
         L.sp++;
         L.ni = first0 ? nt.c0 : nt.c1;
-    } else if (g0) {
+    } else if (nt.h0) {
         L.ni = nt.c0;
-    } else if (g1) {
+    } else if (nt.h1) {
         L.ni = nt.c1;
     } else {
-        if (L.sp == 0) return true;
-        L.sp--;
-        L.ni = stk[(uint32_t)L.sp * stride];
+        return lane_pop<STK, LSTK>(L, stk, ovf, stride);
     }
     return false;
 }
+
+// Test the leaf entry L.ni (closest accepted hit, ties to the lower primitive index), then pop.
+template <int STK, int LSTK, typename C>
+__device__ __forceinline__ bool lane_leaf(const float4 *recs, LaneRay &L, float tmin, uint32_t skip,
+                                          const lds_u32 *stk, const uint32_t *ovf, uint32_t stride, C &cnt) {
+    cnt.prims++;
+    uint32_t slot_unused;
+    test_leaf<false>(recs, L.ni & ~RT_LEAF_BIT, L.o, L.d, tmin, skip, L.best_t, L.best, slot_unused);
+    return lane_pop<STK, LSTK>(L, stk, ovf, stride);
+}
+
+// The result does not depend on the visiting order: a primitive hit at t <= the final closest t lies in
+// boxes whose entry distance is <= t <= every intermediate best_t, so no step culls it, and the tie rule
+// is order-free.  The entries visited equal trace_closest()'s up to that order.
 
 // ------------------------------------------------------------------------------------------------
 // diffuse / scattering rays -> hit words
@@ -803,13 +847,27 @@ __device__ __forceinline__ bool lane_step(const float4 *nodes, const float4 *rec
 // fewer than RT_REFILL_BELOW lanes are still traversing (all of them once the queue is empty), so lanes do
 // not idle for the longest ray of a fixed group of 64.  Every ray is traced exactly as a plain
 // per-ray trace_loop() would trace it; only the assignment of rays to lanes and steps differs.
-#define RT_REFILL_BELOW 48
+// The traversal stack keeps its first RT_DIFFUSE_LDS_STACK entries in LDS and deeper ones in private
+// memory: 4 KB of LDS per wave leaves the register file (about 70 VGPRs) as the occupancy limit.
+#ifndef RT_REFILL_BELOW
+#define RT_REFILL_BELOW 40
+#endif
+#ifndef RT_LEAF_BATCH
+#define RT_LEAF_BATCH 16
+#endif
+#ifndef RT_DIFFUSE_LDS_STACK
+#define RT_DIFFUSE_LDS_STACK 16
+#endif
 template <int WPB, int STK, bool COUNT>
 __global__ __launch_bounds__(WPB * 64) void k_diffuse(const DiffuseJob *__restrict__ job, uint32_t total,
                                                       uint32_t *counter, uint32_t chunk, int *err,
                                                       unsigned long long *stats) {
-    __shared__ uint32_t stk_all[STK * WPB * 64];
-    uint32_t *stk = stk_all + threadIdx.x;
+    constexpr int LSTK = STK < RT_DIFFUSE_LDS_STACK ? STK : RT_DIFFUSE_LDS_STACK;
+    __shared__ uint32_t stk_all[LSTK * WPB * 64];
+    // overflow entries: sized for the deepest stack (RT_STACK) so the array stays in private memory
+    // instead of being promoted to registers, which would cost the occupancy the small LDS part buys
+    uint32_t ovf[STK > LSTK ? RT_STACK - LSTK : 1];
+    lds_u32 *stk = (lds_u32 *)stk_all + threadIdx.x;
     const uint32_t stride = WPB * 64;
     const uint32_t lane = threadIdx.x & 63;
     typename std::conditional<COUNT, TCnt, NoCnt>::type cnt{};
@@ -925,9 +983,22 @@ __global__ __launch_bounds__(WPB * 64) void k_diffuse(const DiffuseJob *__restri
         }
         if (__ballot(busy) == 0) break;  // queue empty and nothing in flight
         // traverse until fewer than RT_REFILL_BELOW lanes (once the queue is empty: no lane) still traverse
+        // node phase: inner-node steps until no lane needs one or RT_LEAF_BATCH lanes wait at a leaf; leaf
+        // phase: the waiting lanes test their leaves together (the intersection code runs with many lanes
+        // active instead of once per node step for the few lanes that reached a leaf)
         const uint32_t floor_lanes = drained ? 1u : RT_REFILL_BELOW;
         while (true) {
-            if (trav && lane_step<STK>(nodes, recs, nleaf, L, 1e-5f, my_i, stk, stride, err, cnt)) trav = false;
+            while (true) {
+                const bool inner = trav && !(L.ni & RT_LEAF_BIT);
+                const uint64_t mi = __ballot(inner);
+                if (mi == 0 || (uint32_t)__popcll(__ballot(trav && !inner)) >= RT_LEAF_BATCH) break;
+                if (inner && lane_node<STK, LSTK>(nodes, nleaf, L, stk, ovf, stride, err, cnt)) trav = false;
+            }
+            while (true) {
+                const bool leaf = trav && (L.ni & RT_LEAF_BIT);
+                if (__ballot(leaf) == 0) break;
+                if (leaf && lane_leaf<STK, LSTK>(recs, L, 1e-5f, my_i, stk, ovf, stride, cnt)) trav = false;
+            }
             if ((uint32_t)__popcll(__ballot(trav)) < floor_lanes) break;
         }
     }
