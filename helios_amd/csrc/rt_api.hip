@@ -384,7 +384,20 @@ int rt_debug_direct_sums(rt_ctx *ctx, const rt_scene *scene, uint32_t d, uint32_
     if (d == 0 || nsrc == 0) { set_error("d and nsrc must be positive"); return -1; }
     std::lock_guard<std::mutex> lk(ctx->mu);
     CKR(hipSetDevice(ctx->device));
-    return debug_direct_sums(ctx->sx, scene->s, d, seed, nsrc, src_dir, out);
+    return debug_direct_sums(ctx->sx, scene->s, d, seed, nsrc, src_dir, nullptr, nullptr, nullptr, out);
+}
+
+int rt_debug_direct_sums_src(rt_ctx *ctx, const rt_scene *scene, uint32_t d, uint32_t seed, uint32_t nsrc,
+                             const float *src_pos, const uint32_t *types, const float *widths,
+                             const float *rotations, float *out) {
+    NN(ctx, "ctx");
+    NN(scene, "scene");
+    NN(out, "out");
+    NN(src_pos, "src_pos");
+    if (d == 0 || nsrc == 0) { set_error("d and nsrc must be positive"); return -1; }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    CKR(hipSetDevice(ctx->device));
+    return debug_direct_sums(ctx->sx, scene->s, d, seed, nsrc, src_pos, types, widths, rotations, out);
 }
 
 }  // extern "C"

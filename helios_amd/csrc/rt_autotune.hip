@@ -116,7 +116,8 @@ extern "C" int rt_autotune(rt_ctx *ctx, const rt_scene *scene, const rt_band_req
         // the request's first source direction, or an oblique default when it has none (a batch's base
         // request carries no sun of its own)
         const float up[3] = {0.3f, 0.2f, 0.93f};
-        const float *sd = (req->nsources && req->source_dir) ? req->source_dir : up;
+        const bool col0 = !req->source_type || (req->nsources && req->source_type[0] == 0u);
+        const float *sd = (req->nsources && req->source_dir && col0) ? req->source_dir : up;
         if (!(sd[0] * sd[0] + sd[1] * sd[1] + sd[2] * sd[2] > 1e-12f)) sd = up;
         hdj.sdir[0] = sd[0]; hdj.sdir[1] = sd[1]; hdj.sdir[2] = sd[2];
         hdj.d = d; hdj.seed = 12345u; hdj.out = dsum; hdj.item_begin = 0; hdj.P = P;

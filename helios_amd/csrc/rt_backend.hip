@@ -208,20 +208,18 @@ int rt_update_materials(rt_ctx *ctx, uint32_t nsources, uint32_t nprims, uint32_
     return 0;
 }
 
-int rt_update_sources(rt_ctx *ctx, uint32_t nsources, const uint32_t *types, const float *positions) {
+int rt_update_sources(rt_ctx *ctx, uint32_t nsources, const uint32_t *types, const float *positions,
+                      const float *widths, const float *rotations) {
     NN(ctx, "ctx");
     if (nsources) { NN(types, "types"); NN(positions, "positions"); }
+    std::vector<SourceShape> shapes;
+    if (source_shapes(nsources, types, widths, rotations, shapes, "rt_update_sources")) return -1;
     std::lock_guard<std::mutex> lk(ctx->mu);
     BackendState &be = ctx->be;
-    for (uint32_t s = 0; s < nsources; s++)
-        if (types[s] != 0) {
-            set_error("rt_update_sources: source " + std::to_string(s) + " has type " + std::to_string(types[s]) +
-                      "; this backend traces collimated sources (type 0) only");
-            return -1;
-        }
     if (nsources != be.S) be.has_fluxes = false;
     be.S = nsources;
     be.src_dir.assign(positions, positions + (size_t)nsources * 3);
+    be.shapes = std::move(shapes);
     return 0;
 }
 
@@ -335,24 +333,32 @@ int rt_launch_direct_rays(rt_ctx *ctx, const rt_launch_params *p) {
     DirectJob *jobs = cv.take<DirectJob>(S);
     uint32_t *counter = cv.take<uint32_t>(16);
     int *err = reinterpret_cast<int *>(counter + 8);
-    std::vector<DirectJob> hj(S);
-    for (uint32_t s = 0; s < S; s++) {
-        DirectJob &j = hj[s];
-        j.sc = be.scene.d;
-        j.sdir[0] = be.src_dir[s * 3]; j.sdir[1] = be.src_dir[s * 3 + 1]; j.sdir[2] = be.src_dir[s * 3 + 2];
-        j.d = d;
-        j.seed = p->random_seed;
-        j.out = dsum + (size_t)s * P * 2;
-        j.item_begin = s * p->launch_count + range_begin(p->launch_offset);
-        j.P = P;
-    }
-    // occlusion grids per source direction (non-periodic scenes), as in rt_run_band
-    if (ctx->sx.use_grid) {
+    // one job per source, collimated sources first (k_direct), then the other types (k_direct_src)
+    std::vector<DirectJob> hj;
+    std::vector<float> coldir;
+    for (int pass = 0; pass < 2; pass++)
+        for (uint32_t s = 0; s < S; s++) {
+            if ((be.shapes[s].type == 0u) != (pass == 0)) continue;
+            DirectJob j{};
+            j.sc = be.scene.d;
+            j.sdir[0] = be.src_dir[s * 3]; j.sdir[1] = be.src_dir[s * 3 + 1]; j.sdir[2] = be.src_dir[s * 3 + 2];
+            j.d = d;
+            j.seed = p->random_seed;
+            j.out = dsum + (size_t)s * P * 2;
+            j.item_begin = (uint32_t)hj.size() * p->launch_count + range_begin(p->launch_offset);
+            j.P = P;
+            set_job_source(j, be.shapes, s);
+            if (pass == 0) coldir.insert(coldir.end(), j.sdir, j.sdir + 3);
+            hj.push_back(j);
+        }
+    const uint32_t ncol = (uint32_t)(coldir.size() / 3);
+    // occlusion grids per collimated source direction (non-periodic scenes), as in rt_run_band
+    if (ctx->sx.use_grid && ncol) {
         ctx->sx.grids.density = (float)ctx->sx.cfg[0].grid_density;
-        int g = build_sun_grids(ctx->sx.grids, be.scene, be.src_dir.data(), S, st);
+        int g = build_sun_grids(ctx->sx.grids, be.scene, coldir.data(), ncol, st);
         if (g < 0) return -1;
         if (g == 0)
-            for (uint32_t s = 0; s < S; s++) hj[s].grid = ctx->sx.grids.dev + s;
+            for (uint32_t s = 0; s < ncol; s++) hj[s].grid = ctx->sx.grids.dev + s;
     }
     CKR(hipMemsetAsync(dsum, 0, (size_t)S * P * 2 * 4, st));
     CKR(hipMemsetAsync(err, 0, sizeof(int), st));
@@ -365,8 +371,8 @@ int rt_launch_direct_rays(rt_ctx *ctx, const rt_launch_params *p) {
     // Sources are jobs over items [s*count, (s+1)*count).  find_job needs ascending item_begin, which
     // the modular offset of a sub-range breaks for S > 1: such launches go one source at a time.
     if (p->launch_offset == 0) {
-        if (launch_direct(jobs, S, S * p->launch_count, be.scene.h.depth, counter, err, ctx->sx.stat_ptr(0),
-                          ctx->sx.cfg[0], ctx->sx.ncu, st))
+        if (launch_direct_jobs(hj.data(), jobs, S, S * p->launch_count, be.scene.h.depth, counter, err,
+                               ctx->sx.stat_ptr(0), ctx->sx.cfg[0], ctx->sx.ncu, st))
             return -1;
     } else {
         for (uint32_t s = 0; s < S; s++) {
@@ -374,8 +380,8 @@ int rt_launch_direct_rays(rt_ctx *ctx, const rt_launch_params *p) {
             j.item_begin = range_begin(p->launch_offset);
             CKR(hipMemcpyAsync(jobs + s, &j, sizeof(DirectJob), hipMemcpyHostToDevice, st));
             CKR(hipStreamSynchronize(st));
-            if (launch_direct(jobs + s, 1, p->launch_count, be.scene.h.depth, counter, err, ctx->sx.stat_ptr(0),
-                              ctx->sx.cfg[0], ctx->sx.ncu, st))
+            if (launch_direct_jobs(&j, jobs + s, 1, p->launch_count, be.scene.h.depth, counter, err,
+                                   ctx->sx.stat_ptr(0), ctx->sx.cfg[0], ctx->sx.ncu, st))
                 return -1;
         }
     }

@@ -26,7 +26,8 @@ struct BackendState {
     uint32_t Bl = 0;  // launch bands (set by zeroRadiationBuffers)
     uint32_t S = 0;   // sources
     bool has_materials = false, has_fluxes = false;
-    std::vector<float> src_dir;  // [S][3]
+    std::vector<float> src_dir;  // [S][3] direction (collimated) or position
+    std::vector<rt::SourceShape> shapes;  // [S]
     void *mat = nullptr;         // rho [P][Bg] then tau [P][Bg] (source-0 slice)
     size_t mat_cap = 0;
     void *flux = nullptr;        // [S][Bl]

@@ -161,6 +161,106 @@ RT_HD void direct_sample(uint32_t prim, uint32_t lx, uint32_t ly, uint32_t d, ui
     }
 }
 
+// direct_sample() that also returns the RNG state after its two draws: the next draws belong to the
+// non-collimated sources of the request, in source order (direct_raygen.comp:482-561)
+RT_HD void direct_sample_st(uint32_t prim, uint32_t lx, uint32_t ly, uint32_t d, uint32_t seed, bool patch, float &spx,
+                            float &spy, uint32_t &st) {
+    uint32_t Nrays = d * d;
+    st = tea16(d * ly + lx + Nrays * prim, seed);
+    float Rx = rnd24(st), Ry = rnd24(st);
+    if (patch) {
+        float fd = (float)d;
+        spx = (-0.5f + rdiv((float)lx, fd)) + rdiv(Rx, fd);
+        spy = (-0.5f + rdiv((float)ly, fd)) + rdiv(Ry, fd);
+    } else {
+        if (Rx < Ry) { spx = Rx; spy = Ry; } else { spx = Ry; spy = Rx; }
+    }
+}
+
+// ---- non-collimated sources (direct_raygen.comp:495-561) ----
+// Source types: 0 collimated, 1 sphere, 2 sun sphere (as 1), 3 rectangle, 4 disk.  A source carries its
+// position, width (sphere diameter | rectangle w, h | disk radius) and a rotation matrix m[9] (rows) built
+// on the host from the ZYX Euler angles (transforms.glsl:49-64).  sin(acos(x)) is evaluated as
+// sqrt(1 - x^2) and sin/cos of angles that are fractions of 2*pi with sincos_2pi(), so the FP32 rays are
+// reproducible bit for bit; mathematically they are the reference's expressions.
+#define RT_PI_F 3.14159265359f
+
+// calc_sphere_strength (direct_raygen.comp:305-320): 10x10 quadrature over the sphere's visible hemisphere
+RT_HD float sphere_strength(v3f n, v3f rd, float mag, float diam, uint32_t Nrays) {
+    const float inv_n = rdiv(1.0f, (float)Nrays);
+    const float ndr = fabsf(dot3(n, rd));
+    const float mag2 = mag * mag;
+    float s = 0.0f;
+    for (int j = 0; j < 10; j++) {
+        float sp, cp;
+        sincos_2pi(rdiv((float)j + 0.5f, 10.0f), sp, cp);
+        for (int i = 0; i < 10; i++) {
+            float ct = 1.0f - rdiv(2.0f * ((float)i + 0.5f), 10.0f);
+            float sth = rsqrt_cr(fmaxf(0.0f, 1.0f - ct * ct));
+            v3f L = mk3(sth * cp, sth * sp, ct);
+            float ld = dot3(L, rd);
+            if (ld < 0.0f) s += rdiv(rdiv((inv_n * ndr) * fabsf(ld), mag2), 100.0f) * diam * diam;
+        }
+    }
+    return s;
+}
+
+// One direct ray from origin o (primitive normal n) towards a non-collimated source: draws from st, sets the
+// unit direction and the strength; false when the rectangle/disk back-face test rejects the ray.
+RT_HD bool source_ray(uint32_t type, const float *pos, const float *width, const float *m, v3f o, v3f n,
+                      uint32_t Nrays, uint32_t &st, v3f &dir, float &strength) {
+    const v3f p = mk3(pos[0], pos[1], pos[2]);
+    const float inv_n = rdiv(1.0f, (float)Nrays);
+    if (type == 1u || type == 2u) {
+        float u1 = rnd24(st), u2 = rnd24(st);
+        float ct = 1.0f - 2.0f * u1;
+        float sth = rsqrt_cr(fmaxf(0.0f, 1.0f - ct * ct));
+        float sp, cp;
+        sincos_2pi(u2, sp, cp);
+        float radius = width[0] * 0.5f;
+        v3f sph = mk3(radius * (sth * cp), radius * (sth * sp), radius * ct);
+        v3f rd = sub3(add3(sph, p), o);
+        float mag = rsqrt_cr(dot3(rd, rd));
+        dir = nrm3(rd);
+        strength = sphere_strength(n, dir, mag, width[0], Nrays);
+        return true;
+    }
+    float lx, ly;
+    float scale = 1.0f;
+    if (type == 3u) {
+        float Rx = rnd24(st), Ry = rnd24(st);
+        lx = (-0.5f + Rx) * width[0];
+        ly = (-0.5f + Ry) * width[1];
+    } else {
+        // concentric square-to-disk map (direct_raygen.comp:227-266); phi = k * pi/4 = 2*pi * (k/8)
+        float Rx = rnd24(st), Ry = rnd24(st);
+        float x = -1.0f + 2.0f * Rx, y = -1.0f + 2.0f * Ry;
+        float r, k;
+        if (x > -y) {
+            if (x > y) { r = x; k = rdiv(y, x); }
+            else { r = y; k = 2.0f - rdiv(x, y); }
+        } else {
+            if (x < y) { r = -x; k = 4.0f + rdiv(y, x); }
+            else { r = -y; k = (y != 0.0f) ? 6.0f - rdiv(x, y) : 0.0f; }
+        }
+        float sk, ck;
+        sincos_2pi(k * 0.125f, sk, ck);
+        lx = r * ck;
+        ly = r * sk;
+        scale = width[0];
+    }
+    // light point = R * (lx, ly, 0); light normal = R * (0, 0, 1), normalised
+    v3f lp = mk3(m[0] * lx + m[1] * ly, m[3] * lx + m[4] * ly, m[6] * lx + m[7] * ly);
+    v3f ln = nrm3(mk3(m[2], m[5], m[8]));
+    v3f rd = type == 3u ? sub3(add3(lp, p), o) : sub3(add3(mul3(lp, scale), p), o);
+    if (dot3(rd, ln) > 0.0f) return false;
+    float mag = rsqrt_cr(dot3(rd, rd));
+    dir = nrm3(rd);
+    float g = rdiv((inv_n * fabsf(dot3(n, dir))) * fabsf(dot3(ln, dir)), mag * mag);
+    strength = type == 3u ? rdiv(g * (width[0] * width[1]), RT_PI_F) : (g * scale) * scale;
+    return true;
+}
+
 // Diffuse-pass sample point and local hemisphere direction (diffuse_raygen.comp:375-482)
 RT_HD void diffuse_sample(uint32_t prim, uint32_t lx, uint32_t ly, uint32_t n, uint32_t seed, bool patch, float &spx,
                           float &spy, v3f &dl) {

@@ -38,6 +38,14 @@ struct LaunchCfg {
 // stats (may be null): [0] BVH node fetches, [1] primitive tests, [2] rays traced
 int launch_direct(const DirectJob *jobs_dev, uint32_t njobs, uint32_t total_items, uint32_t depth, uint32_t *counter,
                   int *err, unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st);
+// Non-collimated sources (k_direct_src): items [item_lo, item_lo + nitems) of jobs with ascending item_begin.
+int launch_direct_src(const DirectJob *jobs_dev, uint32_t njobs, uint32_t item_lo, uint32_t nitems, uint32_t depth,
+                      uint32_t *counter, int *err, unsigned long long *stats, int ncu, hipStream_t st);
+// Any mix of source types: hjobs is the host copy of jobs_dev, collimated jobs first (their items precede
+// the others'); collimated jobs go to k_direct, the rest to k_direct_src.
+int launch_direct_jobs(const DirectJob *hjobs, const DirectJob *jobs_dev, uint32_t njobs, uint32_t total_items,
+                       uint32_t depth, uint32_t *counter, int *err, unsigned long long *stats, const LaunchCfg &cfg,
+                       int ncu, hipStream_t st);
 // one diffuse job (device pointer) per launch: items [0, total_items) of that job
 int launch_diffuse(const DiffuseJob *job_dev, uint32_t total_items, uint32_t depth, uint32_t *counter, int *err,
                    unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st);

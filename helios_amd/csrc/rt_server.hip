@@ -36,7 +36,8 @@ struct Model {
     const Scene *S = nullptr;
     uint32_t P = 0, B = 0, Sn = 0;
     rt_band_request q{};  // host copy; pointers aim at the vectors below (rho/tau/eps/temp are null)
-    std::vector<uint32_t> depth, emission, dseed;
+    std::vector<uint32_t> depth, emission, dseed, stype;
+    std::vector<float> swidth, srot;
     void *dmem = nullptr;  // resident rho, tau[, eps][, temp]
     DevMaterials dm{};
     size_t resident_bytes = 0;
@@ -320,6 +321,27 @@ int rt_server_add_model(rt_server *srv, const rt_scene *scene, const rt_band_req
     m->q.source_dir = nullptr;
     m->q.source_flux = nullptr;
     m->q.diffuse_flux = nullptr;
+    {
+        std::vector<SourceShape> shapes;  // validated here so that a bad shape fails at registration
+        if (source_shapes(q->nsources, q->source_type, q->source_width, q->source_rotation, shapes,
+                          "rt_server_add_model"))
+            return -1;
+        m->q.source_type = nullptr;
+        m->q.source_width = nullptr;
+        m->q.source_rotation = nullptr;
+        if (q->source_type) {
+            m->stype.assign(q->source_type, q->source_type + q->nsources);
+            m->q.source_type = m->stype.data();
+            if (q->source_width) {
+                m->swidth.assign(q->source_width, q->source_width + (size_t)2 * q->nsources);
+                m->q.source_width = m->swidth.data();
+            }
+            if (q->source_rotation) {
+                m->srot.assign(q->source_rotation, q->source_rotation + (size_t)3 * q->nsources);
+                m->q.source_rotation = m->srot.data();
+            }
+        }
+    }
     m->q.rho = m->q.tau = m->q.eps = m->q.temperature = nullptr;
     // resident optical properties
     const size_t PB = (size_t)P * B;

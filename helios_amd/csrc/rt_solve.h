@@ -62,6 +62,20 @@ struct SolveCtx {
 };
 
 // The per-sun part of a solve (K suns share everything else).
+// Shape of one radiation source as the direct kernels take it.
+struct SourceShape {
+    uint32_t type = 0;     // RT_SOURCE_*
+    float width[2] = {0.f, 0.f};
+    float rot[9] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};
+};
+// Validates S sources (types may be null = all collimated; widths [S][2], rotations [S][3]) and builds
+// their shapes, rotation matrices from ZYX Euler angles (transforms.glsl:49-64) evaluated in double and
+// rounded.  Returns -1 with the error set on invalid input.
+int source_shapes(uint32_t S, const uint32_t *types, const float *widths, const float *rotations,
+                  std::vector<SourceShape> &out, const char *who);
+// Fills a direct job's source fields from shapes[s]; skip_draws counts the earlier non-collimated sources.
+void set_job_source(DirectJob &j, const std::vector<SourceShape> &shapes, uint32_t s);
+
 struct SunSet {
     uint32_t K;
     const float *dir;     // [K][S][3]
@@ -86,6 +100,6 @@ int launch_deposit_direct(uint32_t P, uint32_t C, uint32_t B, uint32_t S, const 
                           hipStream_t s);
 int debug_diffuse_hits(SolveCtx &sx, const Scene &S, uint32_t n, uint32_t seed, uint32_t *out);
 int debug_direct_sums(SolveCtx &sx, const Scene &S, uint32_t d, uint32_t seed, uint32_t nsrc, const float *src_dir,
-                      float *out);
+                      const uint32_t *types, const float *widths, const float *rotations, float *out);
 
 }  // namespace rt

@@ -173,6 +173,59 @@ int launch_deposit_direct(uint32_t P, uint32_t C, uint32_t B, uint32_t S, const 
     return 0;
 }
 
+int source_shapes(uint32_t S, const uint32_t *types, const float *widths, const float *rotations,
+                  std::vector<SourceShape> &out, const char *who) {
+    out.assign(S, SourceShape{});
+    if (!types) return 0;
+    for (uint32_t s = 0; s < S; s++) {
+        SourceShape &sh = out[s];
+        sh.type = types[s];
+        if (sh.type > 4u) {
+            set_error(std::string(who) + ": source " + std::to_string(s) + " has unknown type " +
+                      std::to_string(sh.type) + " (0 collimated, 1 sphere, 2 sun sphere, 3 rectangle, 4 disk)");
+            return -1;
+        }
+        if (sh.type == 0u) continue;
+        if (!widths) {
+            set_error(std::string(who) + ": source widths are required for non-collimated sources");
+            return -1;
+        }
+        sh.width[0] = widths[2 * s];
+        sh.width[1] = widths[2 * s + 1];
+        const bool bad = !(sh.width[0] > 0.f) || !std::isfinite(sh.width[0]) ||
+                         (sh.type == 3u && (!(sh.width[1] > 0.f) || !std::isfinite(sh.width[1])));
+        if (bad) {
+            set_error(std::string(who) + ": source " + std::to_string(s) + " needs a positive finite width");
+            return -1;
+        }
+        if (sh.type == 3u || sh.type == 4u) {
+            if (!rotations) {
+                set_error(std::string(who) + ": source rotations are required for rectangle and disk sources");
+                return -1;
+            }
+            const double rx = rotations[3 * s], ry = rotations[3 * s + 1], rz = rotations[3 * s + 2];
+            const double cx = std::cos(rx), sx = std::sin(rx), cy = std::cos(ry), sy = std::sin(ry);
+            const double cz = std::cos(rz), sz = std::sin(rz);
+            const double m[9] = {cz * cy, cz * sy * sx - sz * cx, cz * sy * cx + sz * sx,
+                                 sz * cy, sz * sy * sx + cz * cx, sz * sy * cx - cz * sx,
+                                 -sy,     cy * sx,                cy * cx};
+            for (int k = 0; k < 9; k++) sh.rot[k] = (float)m[k];
+        }
+    }
+    return 0;
+}
+
+void set_job_source(DirectJob &j, const std::vector<SourceShape> &shapes, uint32_t s) {
+    const SourceShape &sh = shapes[s];
+    j.stype = sh.type;
+    j.width[0] = sh.width[0];
+    j.width[1] = sh.width[1];
+    for (int k = 0; k < 9; k++) j.rot[k] = sh.rot[k];
+    uint32_t skip = 0;
+    for (uint32_t q = 0; q < s; q++) skip += shapes[q].type != 0u ? 2u : 0u;
+    j.skip_draws = skip;
+}
+
 int Workspace::reserve(size_t bytes) {
     if (bytes <= cap) return 0;
     if (base) (void)hipFree(base);
@@ -344,42 +397,49 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
     CK(hipMemcpyAsync(depth, q.band_depth, (size_t)B * 4, hipMemcpyHostToDevice, st));
 
     std::vector<DirectJob> hd;
-    uint32_t items = 0;
+    uint32_t items = 0, ncol = 0;
+    std::vector<SourceShape> shapes;
+    if (source_shapes(Sn, q.source_type, q.source_width, q.source_rotation, shapes, "run_band")) return -1;
     if (any_direct) {
-        for (uint32_t k = 0; k < K; k++) {
-            if (!rundirect[k]) continue;
-            for (uint32_t s = 0; s < Sn; s++) {
-                DirectJob j{};
-                j.sc = S.d;
-                const float *sd = suns.dir + ((size_t)k * Sn + s) * 3;
-                j.sdir[0] = sd[0]; j.sdir[1] = sd[1]; j.sdir[2] = sd[2];
-                j.d = d;
-                j.seed = suns.seed[k];
-                j.out = dsum + ((size_t)k * Sn + s) * P * 2;
-                j.item_begin = items;
-                j.P = P;
-                j.order = S.h.order;  // primitives in leaf order: concurrent waves trace nearby rays
-                items += P;
-                hd.push_back(j);
+        // collimated sources first (k_direct), then the other types (k_direct_src)
+        for (int pass = 0; pass < 2; pass++)
+            for (uint32_t k = 0; k < K; k++) {
+                if (!rundirect[k]) continue;
+                for (uint32_t s = 0; s < Sn; s++) {
+                    if ((shapes[s].type == 0u) != (pass == 0)) continue;
+                    DirectJob j{};
+                    j.sc = S.d;
+                    const float *sd = suns.dir + ((size_t)k * Sn + s) * 3;
+                    j.sdir[0] = sd[0]; j.sdir[1] = sd[1]; j.sdir[2] = sd[2];
+                    j.d = d;
+                    j.seed = suns.seed[k];
+                    j.out = dsum + ((size_t)k * Sn + s) * P * 2;
+                    j.item_begin = items;
+                    j.P = P;
+                    j.order = S.h.order;  // primitives in leaf order: concurrent waves trace nearby rays
+                    set_job_source(j, shapes, s);
+                    items += P;
+                    hd.push_back(j);
+                    if (pass == 0) ncol++;
+                }
             }
-        }
-        // occlusion grids, one per (sun, source) direction, when the scene is not periodic
+        // occlusion grids, one per collimated (sun, source) direction, when the scene is not periodic
         sx.last_direct_grid = false;
-        if (sx.use_grid) {
-            std::vector<float> jd(hd.size() * 3);
-            for (size_t j = 0; j < hd.size(); j++)
+        if (sx.use_grid && ncol) {
+            std::vector<float> jd((size_t)ncol * 3);
+            for (size_t j = 0; j < ncol; j++)
                 for (int a = 0; a < 3; a++) jd[j * 3 + a] = hd[j].sdir[a];
             hipEvent_t ga, gb;
             if (ev_pair(sx, ga, gb)) return -1;
             if (ga) CK(hipEventRecord(ga, st));
             sx.grids.density = (float)sx.cfg[0].grid_density;
-            int g = build_sun_grids(sx.grids, S, jd.data(), (uint32_t)hd.size(), st);
+            int g = build_sun_grids(sx.grids, S, jd.data(), ncol, st);
             if (g < 0) return -1;
             if (gb) { CK(hipEventRecord(gb, st)); tick(sx, 3, ga, gb); }
             if (g == 0) {
-                for (size_t j = 0; j < hd.size(); j++) hd[j].grid = sx.grids.dev + j;
+                for (size_t j = 0; j < ncol; j++) hd[j].grid = sx.grids.dev + j;
                 sx.last_direct_grid = true;
-                const double per = (double)sx.grids.entries / ((double)hd.size() * std::max<uint32_t>(1u, S.h.nleaf));
+                const double per = (double)sx.grids.entries / ((double)ncol * std::max<uint32_t>(1u, S.h.nleaf));
                 sx.last_grid_entries = (uint32_t)std::min(4.0e9, std::round(per * 100.0));
             }
         }
@@ -415,7 +475,9 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
         hipEvent_t a, b;
         if (ev_pair(sx, a, b)) return -1;
         if (a) CK(hipEventRecord(a, st));
-        if (launch_direct(djobs, (uint32_t)hd.size(), items, S.h.depth, counter, err, sx.stat_ptr(0), sx.cfg[0], sx.ncu, st)) return -1;
+        if (launch_direct_jobs(hd.data(), djobs, (uint32_t)hd.size(), items, S.h.depth, counter, err, sx.stat_ptr(0),
+                               sx.cfg[0], sx.ncu, st))
+            return -1;
         if (b) { CK(hipEventRecord(b, st)); tick(sx, 0, a, b); }
         sx.timing_info.direct_rays += (uint64_t)items * d * d;
         sx.timing_info.direct_launches++;
@@ -520,9 +582,11 @@ int debug_diffuse_hits(SolveCtx &sx, const Scene &S, uint32_t n, uint32_t seed, 
 }
 
 int debug_direct_sums(SolveCtx &sx, const Scene &S, uint32_t d, uint32_t seed, uint32_t nsrc, const float *src_dir,
-                      float *out) {
+                      const uint32_t *types, const float *widths, const float *rotations, float *out) {
     hipStream_t st = sx.stream;
     const uint32_t P = S.h.P;
+    std::vector<SourceShape> shapes;
+    if (source_shapes(nsrc, types, widths, rotations, shapes, "rt_debug_direct_sums")) return -1;
     size_t bytes = (size_t)nsrc * P * 2 * 4 + nsrc * sizeof(DirectJob) + 4096;
     if (sx.ws.reserve(bytes)) return -1;
     Carve cv{(char *)sx.ws.base};
@@ -530,23 +594,33 @@ int debug_direct_sums(SolveCtx &sx, const Scene &S, uint32_t d, uint32_t seed, u
     DirectJob *jobs = cv.take<DirectJob>(nsrc);
     uint32_t *counter = cv.take<uint32_t>(16);
     int *err = reinterpret_cast<int *>(counter + 8);
-    std::vector<DirectJob> hj(nsrc);
-    for (uint32_t s = 0; s < nsrc; s++) {
-        DirectJob &j = hj[s];
-        j.sc = S.d;
-        j.sdir[0] = src_dir[s * 3]; j.sdir[1] = src_dir[s * 3 + 1]; j.sdir[2] = src_dir[s * 3 + 2];
-        j.d = d; j.seed = seed; j.out = sums + (size_t)s * P * 2; j.item_begin = s * P; j.P = P;
-    }
-    if (sx.use_grid) {
+    // collimated sources first (their items precede the others')
+    std::vector<DirectJob> hj;
+    std::vector<float> coldir;
+    for (int pass = 0; pass < 2; pass++)
+        for (uint32_t s = 0; s < nsrc; s++) {
+            if ((shapes[s].type == 0u) != (pass == 0)) continue;
+            DirectJob j{};
+            j.sc = S.d;
+            j.sdir[0] = src_dir[s * 3]; j.sdir[1] = src_dir[s * 3 + 1]; j.sdir[2] = src_dir[s * 3 + 2];
+            j.d = d; j.seed = seed; j.out = sums + (size_t)s * P * 2; j.item_begin = (uint32_t)hj.size() * P; j.P = P;
+            set_job_source(j, shapes, s);
+            if (pass == 0) coldir.insert(coldir.end(), j.sdir, j.sdir + 3);
+            hj.push_back(j);
+        }
+    const uint32_t ncol = (uint32_t)(coldir.size() / 3);
+    if (sx.use_grid && ncol) {
         sx.grids.density = (float)sx.cfg[0].grid_density;
-        int g = build_sun_grids(sx.grids, S, src_dir, nsrc, st);
+        int g = build_sun_grids(sx.grids, S, coldir.data(), ncol, st);
         if (g < 0) return -1;
         if (g == 0)
-            for (uint32_t s = 0; s < nsrc; s++) hj[s].grid = sx.grids.dev + s;
+            for (uint32_t s = 0; s < ncol; s++) hj[s].grid = sx.grids.dev + s;
     }
     CK(hipMemsetAsync(err, 0, 4, st));
     CK(hipMemcpyAsync(jobs, hj.data(), nsrc * sizeof(DirectJob), hipMemcpyHostToDevice, st));
-    if (launch_direct(jobs, nsrc, nsrc * P, S.h.depth, counter, err, sx.stat_ptr(0), sx.cfg[0], sx.ncu, st)) return -1;
+    if (launch_direct_jobs(hj.data(), jobs, nsrc, nsrc * P, S.h.depth, counter, err, sx.stat_ptr(0), sx.cfg[0], sx.ncu,
+                           st))
+        return -1;
     std::vector<float> tmp((size_t)nsrc * P * 2);
     CK(hipMemcpyAsync(tmp.data(), sums, tmp.size() * 4, hipMemcpyDeviceToHost, st));
     CK(hipStreamSynchronize(st));

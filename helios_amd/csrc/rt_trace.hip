@@ -742,6 +742,91 @@ __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict
     add_stats(stats, cnt, nray);
 }
 
+// Occlusion of a ray towards a non-collimated source, literally as the reference loop
+// (direct_raygen.comp:603-628 without glass covers): closest hit, periodic wrap when a face is strictly
+// nearer, and a hit at or beyond the distance from the original origin to the source centre does not block.
+template <int STK, typename C>
+__device__ __forceinline__ bool source_occluded(const DevScene &sc, v3f o, v3f d, uint32_t skip, float dist0,
+                                                uint32_t *stk, uint32_t stride, int *err, C &cnt) {
+    v3f cur = o;
+    for (int step = 0; step < 32; ++step) {
+        float t;
+        uint32_t slot;
+        uint32_t h = trace_closest<false, STK>(sc, cur, d, 1e-4f, skip, 1e30f, t, slot, stk, stride, err, cnt);
+        if (sc.nbbox) {
+            float bt = nearest_face(sc, cur, d, t);
+            if (bt > 0.f) { cur = wrap_at(sc, cur, d, bt); continue; }
+        }
+        if (h == RT_MISS) return false;
+        return t < dist0;
+    }
+    return false;
+}
+
+// Direct rays towards sphere, sun-sphere, rectangle and disk sources (direct_raygen.comp:482-561): one
+// wave per (job, primitive), d*d stratified origins as k_direct, per-ray direction, strength and lit face,
+// per-lane closest-hit traversal.  Items [item_lo, item_lo + nitems) of jobs whose item_begin ascend.
+template <int STK, bool COUNT>
+__global__ __launch_bounds__(256) void k_direct_src(const DirectJob *__restrict__ jobs, uint32_t njobs, uint32_t item_lo,
+                                                    uint32_t nitems, uint32_t *counter, int *err,
+                                                    unsigned long long *stats) {
+    __shared__ uint32_t stk_all[STK * 256];
+    uint32_t *stk = stk_all + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63;
+    typename std::conditional<COUNT, TCnt, NoCnt>::type cnt{};
+    uint32_t nray = 0;
+    for (uint32_t first = wave_item(counter, 1u); first < nitems; first = wave_item(counter, 1u)) {
+        const uint32_t item = item_lo + first;
+        const DirectJob &J = jobs[find_job(jobs, njobs, item)];
+        const DevScene &sc = *J.sc;
+        uint32_t i = item - J.item_begin;
+        if (J.order) i = J.order[i];
+        const uint32_t ty = sc.type[i];
+        ksum top_acc, bot_acc;
+        top_acc.init();
+        bot_acc.init();
+        if (traced_type(ty)) {
+            const float4 n4 = sc.nrm[i];
+            const v3f n = mk3(n4.x, n4.y, n4.z);
+            const uint32_t tw = __float_as_uint(n4.w);
+            const bool patch = ty == 0 || ty == 3;
+            const uint32_t d = J.d, Nrays = d * d;
+            float tm[12];
+            const float *Tp = sc.T + (size_t)i * 16;
+#pragma unroll
+            for (int q = 0; q < 12; q++) tm[q] = Tp[q];
+            const v3f sp3 = mk3(J.sdir[0], J.sdir[1], J.sdir[2]);
+            for (uint32_t base = 0; base < Nrays; base += 64) {
+                const uint32_t r = base + lane;
+                if (r >= Nrays) continue;
+                float spx, spy;
+                uint32_t st;
+                direct_sample_st(i, r % d, r / d, d, J.seed, patch, spx, spy, st);
+                const v3f o = xform3(tm, spx, spy, 0.0f);
+                for (uint32_t k = 0; k < J.skip_draws; k++) (void)rnd24(st);
+                nray++;
+                v3f dir;
+                float strength;
+                if (!source_ray(J.stype, J.sdir, J.width, J.rot, o, n, Nrays, st, dir, strength)) continue;
+                const bool top = dot3(dir, n) > 0.0f;
+                if (tw == 3 || (!top && tw == 0)) continue;
+                const v3f dv = sub3(sp3, o);
+                const float dist0 = rsqrt_cr(dot3(dv, dv));
+                if (source_occluded<STK>(sc, o, dir, i, dist0, stk, 256u, err, cnt)) continue;
+                if (top) top_acc.add(strength);
+                else bot_acc.add(strength);
+            }
+        }
+        wave_reduce(top_acc);
+        wave_reduce(bot_acc);
+        if (lane == 0) {
+            J.out[(size_t)i * 2 + 0] = top_acc.value();
+            J.out[(size_t)i * 2 + 1] = bot_acc.value();
+        }
+    }
+    add_stats(stats, cnt, nray);
+}
+
 // Resumable closest-hit traversal state of one lane (BVH2), for kernels that refill finished lanes.
 struct LaneRay {
     v3f o, d;
@@ -1318,6 +1403,43 @@ int launch_direct(const DirectJob *jobs, uint32_t njobs, uint32_t total, uint32_
     }
     LCHECK();
     return 0;
+}
+
+int launch_direct_src(const DirectJob *jobs, uint32_t njobs, uint32_t item_lo, uint32_t nitems, uint32_t depth,
+                      uint32_t *counter, int *err, unsigned long long *stats, int ncu, hipStream_t st) {
+    if (nitems == 0) return 0;
+    k_zero_u32<<<1, 1, 0, st>>>(counter);
+    uint64_t g64 = std::min<uint64_t>((uint64_t)4 * ncu, ((uint64_t)nitems + 3) / 4);
+    uint32_t g = (uint32_t)std::max<uint64_t>(g64, 1);
+    if (stats)
+        PICK_STK(depth, (k_direct_src<32, true><<<g, 256, 0, st>>>(jobs, njobs, item_lo, nitems, counter, err, stats)),
+                 (k_direct_src<64, true><<<g, 256, 0, st>>>(jobs, njobs, item_lo, nitems, counter, err, stats)));
+    else
+        PICK_STK(depth, (k_direct_src<32, false><<<g, 256, 0, st>>>(jobs, njobs, item_lo, nitems, counter, err, stats)),
+                 (k_direct_src<64, false><<<g, 256, 0, st>>>(jobs, njobs, item_lo, nitems, counter, err, stats)));
+    LCHECK();
+    return 0;
+}
+
+int launch_direct_jobs(const DirectJob *hjobs, const DirectJob *jobs, uint32_t njobs, uint32_t total, uint32_t depth,
+                       uint32_t *counter, int *err, unsigned long long *stats, const LaunchCfg &cfg, int ncu,
+                       hipStream_t st) {
+    uint32_t ncol = 0;
+    while (ncol < njobs && hjobs[ncol].stype == 0) ncol++;
+    for (uint32_t k = ncol; k < njobs; k++)
+        if (hjobs[k].stype == 0) {
+            set_error("internal: collimated direct jobs must precede the other source types");
+            return -1;
+        }
+    if (ncol == njobs) return launch_direct(jobs, njobs, total, depth, counter, err, stats, cfg, ncu, st);
+    if (ncol == 0) return launch_direct_src(jobs, njobs, 0, total, depth, counter, err, stats, ncu, st);
+    const uint32_t split = hjobs[ncol].item_begin;
+    if (split > total) {
+        set_error("internal: direct job items out of order");
+        return -1;
+    }
+    if (launch_direct(jobs, ncol, split, depth, counter, err, stats, cfg, ncu, st)) return -1;
+    return launch_direct_src(jobs + ncol, njobs - ncol, split, total - split, depth, counter, err, stats, ncu, st);
 }
 
 int launch_diffuse(const DiffuseJob *job, uint32_t total, uint32_t depth, uint32_t *counter, int *err,

@@ -62,7 +62,8 @@ struct SunGrid {
 // one direct trace job: one scene, one collimated source
 struct DirectJob {
     const DevScene *sc;
-    float sdir[3];         // raw direction towards the source (normalised in-kernel)
+    float sdir[3];         // collimated: raw direction towards the source (normalised in-kernel);
+                           // other types: source position
     uint32_t d;            // rays per side (d*d rays per primitive)
     uint32_t seed;
     float *out;            // [P][2] strength sums lit-from-top / lit-from-bottom
@@ -70,6 +71,10 @@ struct DirectJob {
     uint32_t P;
     const uint32_t *order; // item -> primitive map (null: item k is primitive k)
     const SunGrid *grid;   // occlusion grid of this direction (null: traverse the BVH)
+    uint32_t stype;        // source type: 0 collimated, 1 sphere, 2 sun sphere, 3 rectangle, 4 disk
+    uint32_t skip_draws;   // RNG draws of the request's earlier non-collimated sources (2 each)
+    float width[2];        // sphere diameter | rectangle width, height | disk radius
+    float rot[9];          // rotation matrix (rows) of a rectangle or disk source
 };
 
 // one diffuse trace pass: writes hit words [P][2][n*n]

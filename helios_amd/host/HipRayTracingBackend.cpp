@@ -83,14 +83,21 @@ namespace helios {
 
     void HipRayTracingBackend::updateSources(const std::vector<RayTracingSource> &sources) {
         std::vector<uint32_t> types(sources.size());
-        std::vector<float> positions(3 * sources.size());
+        std::vector<float> positions(3 * sources.size()), widths(2 * sources.size()), rotations(3 * sources.size());
         for (size_t s = 0; s < sources.size(); s++) {
             types[s] = sources[s].type;
             positions[3 * s] = sources[s].position.x;
             positions[3 * s + 1] = sources[s].position.y;
             positions[3 * s + 2] = sources[s].position.z;
+            widths[2 * s] = sources[s].width.x;
+            widths[2 * s + 1] = sources[s].width.y;
+            rotations[3 * s] = sources[s].rotation.x;
+            rotations[3 * s + 1] = sources[s].rotation.y;
+            rotations[3 * s + 2] = sources[s].rotation.z;
         }
-        check(rt_update_sources(ctx_, static_cast<uint32_t>(sources.size()), types.data(), positions.data()), "updateSources");
+        check(rt_update_sources(ctx_, static_cast<uint32_t>(sources.size()), types.data(), positions.data(), widths.data(),
+                                rotations.data()),
+              "updateSources");
     }
 
     void HipRayTracingBackend::updateDiffuseRadiation(const std::vector<float> &flux, const std::vector<float> &extinction, const std::vector<helios::vec3> &peak_dir, const std::vector<float> &dist_norm,

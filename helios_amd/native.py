@@ -25,6 +25,10 @@ class RTError(RuntimeError):
     pass
 
 
+# radiation source types (RayTracingTypes.h:189-196)
+SOURCE_COLLIMATED, SOURCE_SPHERE, SOURCE_SUN_SPHERE, SOURCE_RECTANGLE, SOURCE_DISK = 0, 1, 2, 3, 4
+
+
 class rt_band_request(C.Structure):
     _fields_ = [
         ("nbands", C.c_uint32), ("nsources", C.c_uint32),
@@ -33,6 +37,7 @@ class rt_band_request(C.Structure):
         ("source_dir", f32p), ("source_flux", f32p), ("diffuse_flux", f32p),
         ("rho", f32p), ("tau", f32p), ("eps", f32p), ("temperature", f32p),
         ("direct_seed", C.c_uint32), ("diffuse_seed", u32p), ("schedule", C.c_uint32),
+        ("source_type", u32p), ("source_width", f32p), ("source_rotation", f32p),
     ]
 
 
@@ -113,12 +118,14 @@ SIGNATURES = {
     "rt_get_timing": (C.c_int, [C.c_void_p, C.POINTER(rt_timing)]),
     "rt_debug_diffuse_hits": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, u32p]),
     "rt_debug_direct_sums": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, f32p, f32p]),
+    "rt_debug_direct_sums_src": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, f32p, u32p, f32p,
+                                           f32p, f32p]),
     "rt_debug_diffuse_rays": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, f32p]),
     "rt_debug_trace": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, f32p, u32p, C.c_float, C.c_int, u32p, f32p]),
     "rt_update_geometry": (C.c_int, [C.c_void_p, C.c_uint32, f32p, u32p, u32p, u32p, C.c_int, C.c_int]),
     "rt_build_acceleration_structure": (C.c_int, [C.c_void_p]),
     "rt_update_materials": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, f32p, f32p]),
-    "rt_update_sources": (C.c_int, [C.c_void_p, C.c_uint32, u32p, f32p]),
+    "rt_update_sources": (C.c_int, [C.c_void_p, C.c_uint32, u32p, f32p, f32p, f32p]),
     "rt_zero_radiation_buffers": (C.c_int, [C.c_void_p, C.c_uint32]),
     "rt_zero_scatter_buffers": (C.c_int, [C.c_void_p]),
     "rt_copy_scatter_to_radiation": (C.c_int, [C.c_void_p]),
@@ -341,6 +348,18 @@ class Scene:
         _check(self.ctx.lib.rt_debug_trace(self.ctx._h, self._h, n, _pf(R), _pu(S), tmin, int(brute), _pu(h), _pf(t)))
         return h, t
 
+    def direct_sums_src(self, d: int, seed: int, positions, types=None, widths=None, rotations=None) -> np.ndarray:
+        """One direct pass for sources of any type -> [P][S][2] strength sums (debug/parity access)."""
+        X = _f32(positions).reshape(-1, 3)
+        S = X.shape[0]
+        ty = None if types is None else _u32(types).reshape(S)
+        W = None if widths is None else _f32(widths).reshape(S, 2)
+        R = None if rotations is None else _f32(rotations).reshape(S, 3)
+        out = np.zeros((self.P, S, 2), np.float32)
+        _check(self.ctx.lib.rt_debug_direct_sums_src(self.ctx._h, self._h, d, seed, S, _pf(X), _pu(ty), _pf(W), _pf(R),
+                                                      _pf(out)))
+        return out
+
     def direct_sums(self, d: int, seed: int, src_dirs) -> np.ndarray:
         S = _f32(src_dirs).reshape(-1, 3)
         out = np.zeros((self.P, S.shape[0], 2), np.float32)
@@ -378,10 +397,15 @@ class Backend:
         S, P, B = R.shape
         _check(self.lib.rt_update_materials(self.ctx._h, S, P, B, _pf(R), _pf(Tt)))
 
-    def update_sources(self, directions, types=None):
-        D = _f32(directions).reshape(-1, 3)
-        ty = _u32(np.zeros(D.shape[0]) if types is None else types)
-        _check(self.lib.rt_update_sources(self.ctx._h, D.shape[0], _pu(ty), _pf(D)))
+    def update_sources(self, positions, types=None, widths=None, rotations=None):
+        """positions [S][3]: direction towards a collimated source, else the source position; widths [S][2]
+        and rotations [S][3] as RayTracingSource::width / ::rotation (needed for non-collimated types)."""
+        D = _f32(positions).reshape(-1, 3)
+        S = D.shape[0]
+        ty = _u32(np.zeros(S) if types is None else types)
+        W = None if widths is None else _f32(widths).reshape(S, 2)
+        R = None if rotations is None else _f32(rotations).reshape(S, 3)
+        _check(self.lib.rt_update_sources(self.ctx._h, S, _pu(ty), _pf(D), _pf(W), _pf(R)))
 
     def zero_radiation_buffers(self, launch_band_count: int):
         _check(self.lib.rt_zero_radiation_buffers(self.ctx._h, launch_band_count))
@@ -511,7 +535,11 @@ class BandRequest:
 
     def __init__(self, P: int, bands: Sequence[BandSpec], rho, tau, direct_rays: int, diffuse_rays: int,
                  source_dirs=None, source_flux=None, direct_seed: int = 1, diffuse_seeds=None, eps=None,
-                 temperature=None, schedule: int = 0):
+                 temperature=None, schedule: int = 0, source_types=None, source_widths=None,
+                 source_rotations=None):
+        """source_dirs [S][3]: direction towards a collimated source, else the source position;
+        source_types [S] (SOURCE_*; None = all collimated), source_widths [S][2] and source_rotations [S][3]
+        as RayTracingSource::width / ::rotation."""
         B = len(bands)
         self.P, self.B = P, B
         self.bands = list(bands)
@@ -532,12 +560,16 @@ class BandRequest:
         self.direct_seed = int(direct_seed)
         self.direct_rays, self.diffuse_rays = int(direct_rays), int(diffuse_rays)
         self.schedule = int(schedule)
+        self.stype = None if source_types is None else _u32(source_types).reshape(S)
+        self.swidth = None if source_widths is None else _f32(source_widths).reshape(S, 2)
+        self.srot = None if source_rotations is None else _f32(source_rotations).reshape(S, 3)
 
     def c_struct(self) -> rt_band_request:
         return rt_band_request(
             self.B, self.sdir.shape[0], self.direct_rays, self.diffuse_rays, _pu(self.depth), _pu(self.emission),
             _pf(self.sdir), _pf(self.sflux), _pf(self.dflux), _pf(self.rho), _pf(self.tau), _pf(self.eps),
-            _pf(self.temp), self.direct_seed, _pu(self.dseeds), self.schedule)
+            _pf(self.temp), self.direct_seed, _pu(self.dseeds), self.schedule, _pu(self.stype), _pf(self.swidth),
+            _pf(self.srot))
 
 
 def run_band(scene: Scene, req: BandRequest) -> np.ndarray:

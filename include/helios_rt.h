@@ -90,14 +90,21 @@ int rt_cast_rays_soa_device(rt_ctx *ctx, const rt_scene *scene, const float *d_o
 #define RT_SCHEDULE_REFERENCE 0u  /* pass schedule exactly as RadiationModel.cpp:4236-4356          */
 #define RT_SCHEDULE_CONSERVING 1u /* also propagates the primary pass's scatter when emission is off */
 
+#define RT_SOURCE_COLLIMATED 0u
+#define RT_SOURCE_SPHERE 1u
+#define RT_SOURCE_SUN_SPHERE 2u
+#define RT_SOURCE_RECTANGLE 3u
+#define RT_SOURCE_DISK 4u
+
 typedef struct {
     uint32_t nbands;             /* B, launched bands (alphabetical order)                         */
-    uint32_t nsources;           /* S, collimated sources                                          */
+    uint32_t nsources;           /* S, radiation sources                                           */
     uint32_t direct_rays;        /* max directRayCount over the bands (d = ceil(sqrt))             */
     uint32_t diffuse_rays;       /* max diffuseRayCount over the bands (n = ceil(sqrt))            */
     const uint32_t *band_depth;  /* [B] scatteringDepth                                            */
     const uint32_t *emission;    /* [B] emission flag                                              */
-    const float *source_dir;     /* [S][3] direction towards each collimated source                */
+    const float *source_dir;     /* [S][3] collimated: direction towards the source; other types:  */
+                                 /* source position (RayTracingSource::position)                   */
     const float *source_flux;    /* [S][B]                                                         */
     const float *diffuse_flux;   /* [B]                                                            */
     const float *rho;            /* [P][B] reflectivity (validated, source-0 slice)                */
@@ -107,6 +114,12 @@ typedef struct {
     uint32_t direct_seed;
     const uint32_t *diffuse_seed;/* [1 + max depth]: primary diffuse pass, then each scattering pass */
     uint32_t schedule;           /* RT_SCHEDULE_*                                                   */
+    /* Source shapes (RayTracingTypes.h:189-196; direct_raygen.comp:491-561).  source_type NULL = every
+     * source collimated.  Sphere / sun sphere: width[0] = diameter; rectangle: width = (w, h); disk:
+     * width[0] = radius; rotation = ZYX Euler angles (radians) of a rectangle or disk. */
+    const uint32_t *source_type; /* [S] RT_SOURCE_*, or NULL                                        */
+    const float *source_width;   /* [S][2], required when a source is not collimated               */
+    const float *source_rotation;/* [S][3], required when a source is a rectangle or disk          */
 } rt_band_request;
 
 /* R_out: [P][B] radiation_flux_<band> (host memory). */
@@ -151,9 +164,11 @@ int rt_build_acceleration_structure(rt_ctx *ctx);
  * (R/shaders/direct_raygen.comp:684-687, diffuse_raygen.comp:575-579). */
 int rt_update_materials(rt_ctx *ctx, uint32_t nsources, uint32_t nprims, uint32_t nbands_global, const float *rho,
                         const float *tau);
-/* updateSources (RayTracingBackend.h:99; RayTracingTypes.h:189-196): types[S] (0 = collimated;
- * other types are rejected), positions[S][3] = direction towards each collimated source. */
-int rt_update_sources(rt_ctx *ctx, uint32_t nsources, const uint32_t *types, const float *positions);
+/* updateSources (RayTracingBackend.h:99; RayTracingTypes.h:189-196): types[S] (RT_SOURCE_*),
+ * positions[S][3] (direction towards a collimated source, else the source position), widths[S][2] and
+ * rotations[S][3] (RayTracingSource::width / ::rotation; NULL when every source is collimated). */
+int rt_update_sources(rt_ctx *ctx, uint32_t nsources, const uint32_t *types, const float *positions,
+                      const float *widths, const float *rotations);
 /* zeroRadiationBuffers (RayTracingBackend.h:194; VulkanComputeBackend.cpp:1935): sets B_launch
  * (1..256) and zeroes all five radiation buffers. */
 int rt_zero_radiation_buffers(rt_ctx *ctx, uint32_t launch_band_count);
@@ -302,6 +317,11 @@ int rt_get_timing(rt_ctx *ctx, rt_timing *out);
 int rt_debug_diffuse_hits(rt_ctx *ctx, const rt_scene *scene, uint32_t n, uint32_t seed, uint32_t *out);
 int rt_debug_direct_sums(rt_ctx *ctx, const rt_scene *scene, uint32_t d, uint32_t seed, uint32_t nsrc,
                          const float *src_dir, float *out);
+/* The same for sources of any type (src_pos as rt_band_request.source_dir; types/widths/rotations as
+ * rt_band_request.source_type/_width/_rotation). */
+int rt_debug_direct_sums_src(rt_ctx *ctx, const rt_scene *scene, uint32_t d, uint32_t seed, uint32_t nsrc,
+                             const float *src_pos, const uint32_t *types, const float *widths,
+                             const float *rotations, float *out);
 /* Generated diffuse rays of one pass, [P][2][n*n][6] = origin.xyz, direction.xyz (no tracing). */
 int rt_debug_diffuse_rays(rt_ctx *ctx, const rt_scene *scene, uint32_t n, uint32_t seed, float *out);
 /* Closest hit of arbitrary rays [n][6] with the radiation predicates, skipping skip[r]; brute != 0

@@ -21,7 +21,7 @@ class orc_band_params(C.Structure):
                 ("diffuse_rays", C.c_uint32), ("band_depth", u32p), ("emission", u32p), ("source_dir", f32p),
                 ("source_flux", f32p), ("diffuse_flux", f32p), ("rho", f32p), ("tau", f32p), ("eps", f32p),
                 ("temperature", f32p), ("direct_seed", C.c_uint32), ("diffuse_seed", u32p),
-                ("schedule", C.c_uint32)]
+                ("schedule", C.c_uint32), ("source_type", u32p), ("source_width", f32p), ("source_rotation", f32p)]
 
 
 _lib = None
@@ -62,6 +62,9 @@ def lib():
         L.orc_cast_rays.argtypes = [C.c_void_p, C.c_uint64, f32p, f32p, C.c_float, f32p, f32p, u32p, C.c_int]
         L.orc_run_band.restype = C.c_int
         L.orc_run_band.argtypes = [C.c_void_p, C.POINTER(orc_band_params), f64p, f64p]
+        L.orc_direct_sums_src.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, f32p, u32p, f32p, f32p,
+                                          f64p]
+        L.orc_rotation_matrix.argtypes = [f32p, f32p]
         L.orc_areas.argtypes = [C.c_void_p, f64p]
         _lib = L
     return _lib
@@ -173,6 +176,20 @@ class OracleScene:
         lib().orc_direct_sums(self.h, d, seed, S.shape[0], S.ctypes.data_as(f32p), out.ctypes.data_as(f64p))
         return out
 
+    def direct_sums_src(self, d, seed, positions, types=None, widths=None, rotations=None):
+        """[P][S][2] direct strength sums for sources of any type (FP64 accumulation)."""
+        X = _f(positions).reshape(-1, 3)
+        S = X.shape[0]
+        ty = None if types is None else _u(types).reshape(S)
+        W = None if widths is None else _f(widths).reshape(S, 2)
+        R = None if rotations is None else _f(rotations).reshape(S, 3)
+        out = np.zeros((self.P, S, 2), np.float64)
+        lib().orc_direct_sums_src(self.h, d, seed, S, X.ctypes.data_as(f32p),
+                                  None if ty is None else ty.ctypes.data_as(u32p),
+                                  None if W is None else W.ctypes.data_as(f32p),
+                                  None if R is None else R.ctypes.data_as(f32p), out.ctypes.data_as(f64p))
+        return out
+
     def cast_rays(self, O, D, max_dist=-1.0, brute=False):
         O = _f(O).reshape(-1, 3)
         D = _f(D).reshape(-1, 3)
@@ -190,7 +207,10 @@ class OracleScene:
         keep = dict(depth=_u(req.depth), emission=_u(req.emission), sdir=_f(req.sdir), sflux=_f(req.sflux),
                     dflux=_f(req.dflux), rho=_f(req.rho), tau=_f(req.tau),
                     eps=None if req.eps is None else _f(req.eps),
-                    temp=None if req.temp is None else _f(req.temp), dseeds=_u(req.dseeds))
+                    temp=None if req.temp is None else _f(req.temp), dseeds=_u(req.dseeds),
+                    stype=None if getattr(req, "stype", None) is None else _u(req.stype),
+                    swidth=None if getattr(req, "swidth", None) is None else _f(req.swidth),
+                    srot=None if getattr(req, "srot", None) is None else _f(req.srot))
         p = orc_band_params(
             B, keep["sdir"].reshape(-1, 3).shape[0], req.direct_rays, req.diffuse_rays,
             keep["depth"].ctypes.data_as(u32p), keep["emission"].ctypes.data_as(u32p),
@@ -198,7 +218,10 @@ class OracleScene:
             keep["rho"].ctypes.data_as(f32p), keep["tau"].ctypes.data_as(f32p),
             None if keep["eps"] is None else keep["eps"].ctypes.data_as(f32p),
             None if keep["temp"] is None else keep["temp"].ctypes.data_as(f32p),
-            req.direct_seed, keep["dseeds"].ctypes.data_as(u32p), req.schedule)
+            req.direct_seed, keep["dseeds"].ctypes.data_as(u32p), req.schedule,
+            None if keep["stype"] is None else keep["stype"].ctypes.data_as(u32p),
+            None if keep["swidth"] is None else keep["swidth"].ctypes.data_as(f32p),
+            None if keep["srot"] is None else keep["srot"].ctypes.data_as(f32p))
         R = np.zeros((P, B), np.float64)
         dg = np.zeros((3, B), np.float64)
         rc = lib().orc_run_band(self.h, C.byref(p), R.ctypes.data_as(f64p), dg.ctypes.data_as(f64p))

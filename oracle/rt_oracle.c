@@ -770,6 +770,164 @@ void orc_direct_sums(const orc_scene *s, uint32_t d, uint32_t seed, uint32_t nsr
 }
 
 /* ------------------------------------------------------------------------------------------------ */
+/* non-collimated sources (direct_raygen.comp:482-628)                                               */
+/* ------------------------------------------------------------------------------------------------ */
+/* make_rotation_matrix (transforms.glsl:49-64), ZYX Euler angles; rows of the column-major GLSL matrix,
+ * evaluated in double and rounded to float */
+void orc_rotation_matrix(const float rot[3], float m[9]) {
+    double cx = cos((double)rot[0]), sx = sin((double)rot[0]);
+    double cy = cos((double)rot[1]), sy = sin((double)rot[1]);
+    double cz = cos((double)rot[2]), sz = sin((double)rot[2]);
+    m[0] = (float)(cz * cy); m[1] = (float)(cz * sy * sx - sz * cx); m[2] = (float)(cz * sy * cx + sz * sx);
+    m[3] = (float)(sz * cy); m[4] = (float)(sz * sy * sx + cz * cx); m[5] = (float)(sz * sy * cx - cz * sx);
+    m[6] = (float)(-sy);     m[7] = (float)(cy * sx);                m[8] = (float)(cy * cx);
+}
+
+/* calc_sphere_strength (direct_raygen.comp:305-320); sin(acos(c)) = sqrt(1 - c^2) */
+static float sphere_strength_orc(v3 n, v3 rd, float mag, float diam, uint32_t Nrays) {
+    float sum = 0.0f;
+    for (int j = 0; j < 10; j++) {
+        float sp, cp;
+        orc_sincos_2pi(((float)j + 0.5f) / 10.0f, &sp, &cp);
+        for (int i = 0; i < 10; i++) {
+            float c = 1.0f - (2.0f * ((float)i + 0.5f)) / 10.0f;
+            float sn = sqrtf(fmaxf(0.0f, 1.0f - c * c));
+            v3 L = mk(sn * cp, sn * sp, c);
+            float ldr = dot(L, rd);
+            if (ldr < 0.0f)
+                sum += ((((1.0f / (float)Nrays) * fabsf(dot(n, rd))) * fabsf(ldr)) / (mag * mag)) / 100.0f * diam * diam;
+        }
+    }
+    return sum;
+}
+
+/* direction and strength of one ray from o towards source (type, pos, width, m); returns 0 when the
+ * rectangle/disk back-face test rejects it.  Draws from *st. */
+static int source_sample_orc(uint32_t type, v3 pos, const float *width, const float *m, v3 o, v3 n, uint32_t Nrays,
+                             uint32_t *st, v3 *dir, float *strength) {
+    if (type == 1 || type == 2) {
+        float u1 = orc_rnd(st), u2 = orc_rnd(st);
+        float c = 1.0f - 2.0f * u1;
+        float sn = sqrtf(fmaxf(0.0f, 1.0f - c * c));
+        float sp, cp;
+        orc_sincos_2pi(u2, &sp, &cp);
+        float r = width[0] * 0.5f;
+        v3 q = mk(r * (sn * cp), r * (sn * sp), r * c);
+        v3 rd = sub(add(q, pos), o);
+        float mag = sqrtf(dot(rd, rd));
+        *dir = nrm(rd);
+        *strength = sphere_strength_orc(n, *dir, mag, width[0], Nrays);
+        return 1;
+    }
+    float px, py, scale = 1.0f;
+    if (type == 3) {
+        float Rx = orc_rnd(st), Ry = orc_rnd(st);
+        px = (-0.5f + Rx) * width[0];
+        py = (-0.5f + Ry) * width[1];
+    } else { /* disk: concentric map (direct_raygen.comp:227-266) */
+        float Rx = orc_rnd(st), Ry = orc_rnd(st);
+        float x = -1.0f + 2.0f * Rx, y = -1.0f + 2.0f * Ry, r, phi8;
+        if (x > -y) {
+            if (x > y) { r = x; phi8 = y / x; } else { r = y; phi8 = 2.0f - x / y; }
+        } else {
+            if (x < y) { r = -x; phi8 = 4.0f + y / x; } else { r = -y; phi8 = (y != 0.0f) ? 6.0f - x / y : 0.0f; }
+        }
+        float sp, cp;
+        orc_sincos_2pi(phi8 * 0.125f, &sp, &cp); /* phi = phi8 * pi/4 */
+        px = r * cp;
+        py = r * sp;
+        scale = width[0];
+    }
+    v3 lp = mk(m[0] * px + m[1] * py, m[3] * px + m[4] * py, m[6] * px + m[7] * py);
+    v3 ln = nrm(mk(m[2], m[5], m[8]));
+    v3 rd = (type == 3) ? sub(add(lp, pos), o) : sub(add(scl(lp, scale), pos), o);
+    if (dot(rd, ln) > 0.0f) return 0;
+    float mag = sqrtf(dot(rd, rd));
+    *dir = nrm(rd);
+    float g = (((1.0f / (float)Nrays) * fabsf(dot(n, *dir))) * fabsf(dot(ln, *dir))) / (mag * mag);
+    *strength = (type == 3) ? (g * (width[0] * width[1])) / 3.14159265359f : (g * scale) * scale;
+    return 1;
+}
+
+/* occlusion towards a non-collimated source: the reference loop with the "hit beyond the source" rule */
+static int source_blocked_orc(const orc_scene *s, v3 o, v3 d, uint32_t skip, float dist0) {
+    v3 cur = o;
+    for (int step = 0; step < 32; ++step) {
+        float t;
+        uint32_t h = closest(s, cur, d, 1e-4f, skip, 0, 0, 1e30f, &t);
+        if (s->nbbox) {
+            v3 w;
+            if (periodic_wrap(s, cur, d, t, &w)) { cur = w; continue; }
+        }
+        if (h == MISS) return 0;
+        return t < dist0;
+    }
+    return 0;
+}
+
+void orc_direct_sums_src_range(const orc_scene *s, uint32_t d, uint32_t seed, uint32_t nsrc, const float *src_pos,
+                               const uint32_t *types, const float *widths, const float *rotations, uint32_t p0,
+                               uint32_t p1, double *out) {
+    uint32_t Nrays = d * d;
+    float *mats = (float *)calloc((size_t)nsrc * 9 + 1, sizeof(float));
+    for (uint32_t q = 0; q < nsrc; q++) {
+        uint32_t ty = types ? types[q] : 0;
+        if (ty == 3 || ty == 4) orc_rotation_matrix(rotations + 3 * q, mats + 9 * q);
+    }
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long long ii = (long long)p0; ii < (long long)p1; ii++) {
+        uint32_t i = (uint32_t)ii;
+        double *o_i = out + (size_t)(i - p0) * nsrc * 2;
+        for (uint32_t k = 0; k < nsrc * 2; k++) o_i[k] = 0.0;
+        if (!traced_type(s->type[i])) continue;
+        v3 n = s->normal[i];
+        uint32_t tw = s->twos[i];
+        for (uint32_t r = 0; r < Nrays; r++) {
+            uint32_t lx = r % d, ly = r / d;
+            /* origin sample (two draws), then each source in order draws from the same state */
+            uint32_t st = orc_tea16(d * ly + lx + Nrays * i, seed);
+            float Rx = orc_rnd(&st), Ry = orc_rnd(&st), spx, spy;
+            if (is_patch(s->type[i])) {
+                spx = (-0.5f + (float)lx / (float)d) + Rx / (float)d;
+                spy = (-0.5f + (float)ly / (float)d) + Ry / (float)d;
+            } else {
+                if (Rx < Ry) { spx = Rx; spy = Ry; } else { spx = Ry; spy = Rx; }
+            }
+            v3 o = xform(s->T + (size_t)i * 16, spx, spy, 0.0f);
+            for (uint32_t src = 0; src < nsrc; src++) {
+                uint32_t ty = types ? types[src] : 0;
+                v3 dir;
+                float strength;
+                if (ty == 0) {
+                    dir = nrm(ld3(src_pos + src * 3));
+                    strength = fabsf(dot(n, dir)) / (float)Nrays;
+                } else if (!source_sample_orc(ty, ld3(src_pos + src * 3), widths + 2 * src, mats + 9 * src, o, n, Nrays,
+                                              &st, &dir, &strength)) {
+                    continue;
+                }
+                int top = dot(dir, n) > 0.0f;
+                if (tw == 3 || (!top && tw == 0)) continue;
+                if (ty == 0) {
+                    int ex;
+                    uint32_t h = trace_loop(s, o, dir, 1e-4f, i, &ex);
+                    if (h != MISS && !ex) continue;
+                } else {
+                    v3 dv = sub(ld3(src_pos + src * 3), o);
+                    if (source_blocked_orc(s, o, dir, i, sqrtf(dot(dv, dv)))) continue;
+                }
+                o_i[src * 2 + (top ? 0 : 1)] += (double)strength;
+            }
+        }
+    }
+    free(mats);
+}
+
+void orc_direct_sums_src(const orc_scene *s, uint32_t d, uint32_t seed, uint32_t nsrc, const float *src_pos,
+                         const uint32_t *types, const float *widths, const float *rotations, double *out) {
+    orc_direct_sums_src_range(s, d, seed, nsrc, src_pos, types, widths, rotations, 0, s->P, out);
+}
+
+/* ------------------------------------------------------------------------------------------------ */
 /* castRaysSoA (CD/src/CollisionDetection_RayTracing.cpp:551-680, 980-1080, 1555-1620)               */
 /* ------------------------------------------------------------------------------------------------ */
 void orc_cast_rays(const orc_scene *s, uint64_t n, const float *origins, const float *dirs, float max_dist,
@@ -880,7 +1038,8 @@ int orc_run_band(const orc_scene *s, const orc_band_params *p, double *R, double
     /* DIRECT (:3847-3982) */
     if (S > 0 && rundirect && d > 0) {
         double *sums = (double *)calloc((size_t)P * S * 2, sizeof(double));
-        orc_direct_sums(s, d, p->direct_seed, S, p->source_dir, sums);
+        orc_direct_sums_src(s, d, p->direct_seed, S, p->source_dir, p->source_type, p->source_width, p->source_rotation,
+                            sums);
         for (uint32_t i = 0; i < P; i++)
             for (uint32_t k = 0; k < S; k++) {
                 double stp = sums[((size_t)i * S + k) * 2], sbt = sums[((size_t)i * S + k) * 2 + 1];

@@ -71,6 +71,16 @@ void orc_diffuse_hits(const orc_scene *s, uint32_t n, uint32_t seed, uint32_t *o
 /* Direct visibility sums: out[P][S][2] = (sum strength lit-from-top, sum strength lit-from-bottom) */
 void orc_direct_sums(const orc_scene *s, uint32_t d, uint32_t seed, uint32_t nsrc, const float *src_dir, double *out);
 
+/* Direct sums for sources of any type (direct_raygen.comp:482-628): src_pos [S][3] (collimated: direction),
+ * types [S] (NULL = all collimated; 1 sphere, 2 sun sphere, 3 rectangle, 4 disk), widths [S][2],
+ * rotations [S][3] (ZYX Euler angles of rectangles and disks). */
+void orc_direct_sums_src(const orc_scene *s, uint32_t d, uint32_t seed, uint32_t nsrc, const float *src_pos,
+                         const uint32_t *types, const float *widths, const float *rotations, double *out);
+void orc_direct_sums_src_range(const orc_scene *s, uint32_t d, uint32_t seed, uint32_t nsrc, const float *src_pos,
+                               const uint32_t *types, const float *widths, const float *rotations, uint32_t p0,
+                               uint32_t p1, double *out);
+void orc_rotation_matrix(const float rot[3], float m[9]);
+
 /* Range versions (primitives [p0, p1) only; output indexed from p0) -- used to time bounded samples. */
 void orc_diffuse_hits_range(const orc_scene *s, uint32_t n, uint32_t seed, uint32_t p0, uint32_t p1, uint32_t *out);
 void orc_direct_sums_range(const orc_scene *s, uint32_t d, uint32_t seed, uint32_t nsrc, const float *src_dir,
@@ -98,6 +108,9 @@ typedef struct {
     uint32_t direct_seed;
     const uint32_t *diffuse_seed; /* [1 + max depth]: primary diffuse pass, then each scattering pass */
     uint32_t schedule;         /* 0 = reference pass schedule, 1 = energy-conserving    */
+    const uint32_t *source_type;   /* [S] or NULL (all collimated)                       */
+    const float *source_width;     /* [S][2]                                             */
+    const float *source_rotation;  /* [S][3]                                             */
 } orc_band_params;
 
 /* R[P][B] = radiation_flux_<band>; diag (may be NULL) receives [B] FP64 energy totals:

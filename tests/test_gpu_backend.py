@@ -125,7 +125,9 @@ def _check_errors(be, s):
     with pytest.raises(native.RTError, match="no acceleration structure"):
         be.zero_radiation_buffers(1)
     be.build_acceleration_structure()
-    with pytest.raises(native.RTError, match="collimated"):
+    with pytest.raises(native.RTError, match="unknown type 5"):
+        be.update_sources([[0, 0, 1]], types=[5])
+    with pytest.raises(native.RTError, match="widths are required"):
         be.update_sources([[0, 0, 1]], types=[1])
     be.update_sources([[0, 0, 1]])
     with pytest.raises(native.RTError, match="not sized"):
