@@ -142,24 +142,69 @@ void RadiationModel::setScatteringDepth(const std::string &label, uint32_t depth
 
 uint32_t RadiationModel::addCollimatedRadiationSource() { return addCollimatedRadiationSource(vec3{0.f, 0.f, 1.f}); }
 
-uint32_t RadiationModel::addCollimatedRadiationSource(const vec3 &direction) {
-    // RadiationModel.cpp:976-1010
-    if (direction.x == 0.f && direction.y == 0.f && direction.z == 0.f)
-        fail("ERROR (RadiationModel::addCollimatedRadiationSource): direction has zero length");
-    if (source_dir_.size() + 1 > 256)
-        fail("ERROR (RadiationModel::addCollimatedRadiationSource): at most 256 radiation sources");
-    source_dir_.push_back(direction);
+uint32_t RadiationModel::addSource(uint32_t type, const vec3 &position, const vec2 &width, const vec3 &rotation,
+                                   float flux_scale, const char *who) {
+    if (source_dir_.size() + 1 > 256) fail(std::string("ERROR (RadiationModel::") + who + "): at most 256 radiation sources");
+    source_dir_.push_back(position);
+    source_type_.push_back(type);
+    source_width_.push_back(width);
+    source_rot_.push_back(rotation);
+    source_flux_scale_.push_back(flux_scale);
     std::map<std::string, float> f;
     for (auto &b : bands_) f[b.first] = -1.f;
     source_flux_.push_back(f);
     return (uint32_t)(source_dir_.size() - 1);
 }
 
+uint32_t RadiationModel::addCollimatedRadiationSource(const vec3 &direction) {
+    // RadiationModel.cpp:976-1010
+    if (direction.x == 0.f && direction.y == 0.f && direction.z == 0.f)
+        fail("ERROR (RadiationModel::addCollimatedRadiationSource): direction has zero length");
+    return addSource(RT_SOURCE_COLLIMATED, direction, vec2{0.f, 0.f}, vec3{0.f, 0.f, 0.f}, 1.f,
+                     "addCollimatedRadiationSource");
+}
+
+uint32_t RadiationModel::addSphereRadiationSource(const vec3 &position, float radius) {
+    // RadiationModel.cpp:1011-1040: width = diameter
+    if (!(radius > 0.f)) fail("ERROR (RadiationModel::addSphereRadiationSource): Spherical radiation source radius must be positive.");
+    const float w = 2.f * std::fabs(radius);
+    return addSource(RT_SOURCE_SPHERE, position, vec2{w, w}, vec3{0.f, 0.f, 0.f}, 1.f, "addSphereRadiationSource");
+}
+
+uint32_t RadiationModel::addSunSphereRadiationSource() { return addSunSphereRadiationSource(vec3{0.f, 0.f, 1.f}); }
+
+uint32_t RadiationModel::addSunSphereRadiationSource(const vec3 &sun_direction) {
+    // RadiationModel.cpp:1050-1079: the sun as a sphere of diameter 2 x 695.5e6 m at 150e9 m, fluxes set with
+    // setSourceFlux scaled by sigma * 5700^4 / 1288.437
+    const float m = std::sqrt(sun_direction.x * sun_direction.x + sun_direction.y * sun_direction.y +
+                              sun_direction.z * sun_direction.z);
+    if (!(m > 0.f)) fail("ERROR (RadiationModel::addSunSphereRadiationSource): direction has zero length");
+    const vec3 pos{150e9f * sun_direction.x / m, 150e9f * sun_direction.y / m, 150e9f * sun_direction.z / m};
+    const float w = 2.f * 695.5e6f;
+    const float sigma = 5.6703744E-8f;
+    return addSource(RT_SOURCE_SUN_SPHERE, pos, vec2{w, w}, vec3{0.f, 0.f, 0.f}, sigma * std::pow(5700.f, 4.f) / 1288.437f,
+                     "addSunSphereRadiationSource");
+}
+
+uint32_t RadiationModel::addRectangleRadiationSource(const vec3 &position, const vec2 &size, const vec3 &rotation_rad) {
+    // RadiationModel.cpp:1081-1110
+    if (!(size.x > 0.f) || !(size.y > 0.f))
+        fail("ERROR (RadiationModel::addRectangleRadiationSource): Radiation source size must be positive.");
+    return addSource(RT_SOURCE_RECTANGLE, position, size, rotation_rad, 1.f, "addRectangleRadiationSource");
+}
+
+uint32_t RadiationModel::addDiskRadiationSource(const vec3 &position, float radius, const vec3 &rotation_rad) {
+    // RadiationModel.cpp:1112-1140: width = (radius, radius)
+    if (!(radius > 0.f)) fail("ERROR (RadiationModel::addDiskRadiationSource): Disk radiation source radius must be positive.");
+    return addSource(RT_SOURCE_DISK, position, vec2{radius, radius}, rotation_rad, 1.f, "addDiskRadiationSource");
+}
+
 void RadiationModel::setSourceFlux(uint32_t source_ID, const std::string &band_label, float flux) {
     band(band_label, "setSourceFlux");
     if (source_ID >= source_dir_.size())
         fail("ERROR (RadiationModel::setSourceFlux): source ID " + std::to_string(source_ID) + " out of bounds");
-    source_flux_[source_ID][band_label] = flux;
+    // RadiationModel.cpp:1199: a sun-sphere source scales the flux it is given
+    source_flux_[source_ID][band_label] = flux * source_flux_scale_[source_ID];
 }
 
 float RadiationModel::getSourceFlux(uint32_t source_ID, const std::string &band_label) const {
@@ -228,11 +273,16 @@ void RadiationModel::runBand(const std::vector<std::string> &labels) {
     std::vector<float> temperature;
     if (any_emission && doesPrimitiveDataExist("temperature")) temperature = data_.at("temperature");
 
-    std::vector<float> sdir(S * 3), sflux(S * B), dflux(B);
+    std::vector<float> sdir(S * 3), sflux(S * B), dflux(B), swidth(S * 2), srot(S * 3);
     for (size_t s = 0; s < S; s++) {
         sdir[s * 3] = source_dir_[s].x;
         sdir[s * 3 + 1] = source_dir_[s].y;
         sdir[s * 3 + 2] = source_dir_[s].z;
+        swidth[s * 2] = source_width_[s].x;
+        swidth[s * 2 + 1] = source_width_[s].y;
+        srot[s * 3] = source_rot_[s].x;
+        srot[s * 3 + 1] = source_rot_[s].y;
+        srot[s * 3 + 2] = source_rot_[s].z;
         for (size_t b = 0; b < B; b++) sflux[s * B + b] = getSourceFlux((uint32_t)s, launched[b]);
     }
     for (size_t b = 0; b < B; b++) dflux[b] = getDiffuseFlux(launched[b]);
@@ -258,6 +308,9 @@ void RadiationModel::runBand(const std::vector<std::string> &labels) {
     q.emission = emission.data();
     q.source_dir = S ? sdir.data() : nullptr;
     q.source_flux = S ? sflux.data() : nullptr;
+    q.source_type = S ? source_type_.data() : nullptr;
+    q.source_width = S ? swidth.data() : nullptr;
+    q.source_rotation = S ? srot.data() : nullptr;
     q.diffuse_flux = dflux.data();
     q.rho = rho.data();
     q.tau = tau.data();

@@ -21,6 +21,9 @@ namespace hiprt {
 struct vec3 {
     float x, y, z;
 };
+struct vec2 {
+    float x, y;
+};
 
 // Per-band settings (RadiationModel.h RadiationBand; defaults RadiationModel.cpp:40-55)
 struct RadiationBand {
@@ -72,6 +75,11 @@ public:
     // ---- sources (RadiationModel.h:884-992) ----
     uint32_t addCollimatedRadiationSource();
     uint32_t addCollimatedRadiationSource(const vec3 &direction);
+    uint32_t addSphereRadiationSource(const vec3 &position, float radius);
+    uint32_t addSunSphereRadiationSource();
+    uint32_t addSunSphereRadiationSource(const vec3 &sun_direction);
+    uint32_t addRectangleRadiationSource(const vec3 &position, const vec2 &size, const vec3 &rotation_rad);
+    uint32_t addDiskRadiationSource(const vec3 &position, float radius, const vec3 &rotation_rad);
     void setSourceFlux(uint32_t source_ID, const std::string &band_label, float flux);
     float getSourceFlux(uint32_t source_ID, const std::string &band_label) const;
 
@@ -99,7 +107,14 @@ private:
     bool periodic_x_ = false, periodic_y_ = false;
     bool geometry_dirty_ = true;
     std::map<std::string, RadiationBand> bands_;             // alphabetical = global band order
-    std::vector<vec3> source_dir_;
+    uint32_t addSource(uint32_t type, const vec3 &position, const vec2 &width, const vec3 &rotation,
+                       float flux_scale, const char *who);
+
+    std::vector<vec3> source_dir_;                           // direction (collimated) or position
+    std::vector<uint32_t> source_type_;                      // RT_SOURCE_*
+    std::vector<vec2> source_width_;
+    std::vector<vec3> source_rot_;
+    std::vector<float> source_flux_scale_;                   // setSourceFlux multiplies by it (sun sphere)
     std::vector<std::map<std::string, float>> source_flux_;  // -1 = not set
     std::map<std::string, std::vector<float>> data_;
     uint32_t schedule_ = RT_SCHEDULE_REFERENCE;

@@ -19,6 +19,7 @@
 extern "C" uint32_t sg_config1(uint32_t seed, float *T, uint32_t *type, uint32_t *twos);
 
 using hiprt::RadiationModel;
+using hiprt::vec2;
 using hiprt::vec3;
 
 namespace {
@@ -229,6 +230,76 @@ TEST("RadiationModel config-1 canopy against the FP64 oracle (1e-4)", true) {
     CHECK(worst <= 1e-4);
 }
 
+TEST("RadiationModel sphere, rectangle and disk sources vs oracle", true) {
+    const uint32_t P = 1601;
+    std::vector<float> T(P * 16);
+    std::vector<uint32_t> type(P), twos(P);
+    sg_config1(2, T.data(), type.data(), twos.data());
+    std::vector<float> rho(P, 0.10f), tau(P, 0.05f);
+    rho[P - 1] = 0.15f;
+    tau[P - 1] = 0.f;
+
+    RadiationModel radiation;
+    radiation.setGeometry(T, type, {}, twos);
+    radiation.setPrimitiveData("reflectivity_PAR", rho);
+    radiation.setPrimitiveData("transmissivity_PAR", tau);
+    radiation.addRadiationBand("PAR");
+    radiation.disableEmission("PAR");
+    const vec3 sph{0.5f, -0.3f, 3.f}, rect{0.f, 0.f, 2.5f}, disk{-0.6f, 0.4f, 2.f};
+    const vec2 rsize{1.5f, 1.f};
+    const vec3 rrot{(float)M_PI, 0.2f, 0.f}, drot{(float)M_PI - 0.2f, 0.f, 0.3f};
+    uint32_t a = radiation.addSphereRadiationSource(sph, 0.25f);
+    uint32_t b = radiation.addRectangleRadiationSource(rect, rsize, rrot);
+    uint32_t c = radiation.addDiskRadiationSource(disk, 0.4f, drot);
+    radiation.setSourceFlux(a, "PAR", 2000.f);
+    radiation.setSourceFlux(b, "PAR", 800.f);
+    radiation.setSourceFlux(c, "PAR", 1200.f);
+    radiation.setDiffuseRadiationFlux("PAR", 50.f);
+    radiation.setDirectRayCount("PAR", 64);
+    radiation.setDiffuseRayCount("PAR", 64);
+    radiation.setScatteringDepth("PAR", 2);
+    const std::vector<uint32_t> dseeds{21, 22, 23};
+    radiation.setRandomSeeds(9, dseeds);
+    radiation.runBand("PAR");
+    const std::vector<float> &R = radiation.getPrimitiveData("radiation_flux_PAR");
+
+    orc_scene *os = orc_scene_create(P, T.data(), type.data(), twos.data(), nullptr, 0, 0);
+    const uint32_t depth = 2, emission = 0;
+    const float sdir[9] = {sph.x, sph.y, sph.z, rect.x, rect.y, rect.z, disk.x, disk.y, disk.z};
+    const float sflux[3] = {2000.f, 800.f, 1200.f}, dflux = 50.f;
+    const uint32_t stype[3] = {RT_SOURCE_SPHERE, RT_SOURCE_RECTANGLE, RT_SOURCE_DISK};
+    const float swidth[6] = {0.5f, 0.5f, rsize.x, rsize.y, 0.4f, 0.4f};
+    const float srot[9] = {0.f, 0.f, 0.f, rrot.x, rrot.y, rrot.z, drot.x, drot.y, drot.z};
+    orc_band_params q{};
+    q.nbands = 1;
+    q.nsources = 3;
+    q.direct_rays = 64;
+    q.diffuse_rays = 64;
+    q.band_depth = &depth;
+    q.emission = &emission;
+    q.source_dir = sdir;
+    q.source_flux = sflux;
+    q.diffuse_flux = &dflux;
+    q.rho = rho.data();
+    q.tau = tau.data();
+    q.direct_seed = 9;
+    q.diffuse_seed = dseeds.data();
+    q.source_type = stype;
+    q.source_width = swidth;
+    q.source_rotation = srot;
+    std::vector<double> Ro(P);
+    CHECK(orc_run_band(os, &q, Ro.data(), nullptr) == 0);
+    orc_scene_destroy(os);
+    double worst = 0, lit = 0;
+    for (uint32_t i = 0; i < P; i++) {
+        worst = std::max(worst, std::fabs(R[i] - Ro[i]) / std::max(std::fabs(Ro[i]), 1.0));
+        lit = std::max(lit, Ro[i]);
+    }
+    std::printf("    max rel err vs oracle %.3e over %u primitives (max flux %.1f)\n", worst, P, lit);
+    CHECK(lit > 100.0);
+    CHECK(worst <= 1e-4);
+}
+
 TEST("RadiationModel errors", true) {
     RadiationModel radiation;
     radiation.setGeometry(patch_transform({0, 0, 0}, 1, 1, 0, 0), {0}, {0}, {0});
@@ -238,6 +309,9 @@ TEST("RadiationModel errors", true) {
     CHECK_THROWS(radiation.setScatteringDepth("NIR", 1));
     CHECK_THROWS(radiation.setPrimitiveData("reflectivity_SW", {0.1f, 0.2f}));
     CHECK_THROWS(radiation.addCollimatedRadiationSource(vec3{0, 0, 0}));
+    CHECK_THROWS(radiation.addSphereRadiationSource(vec3{0, 0, 1}, -1.f));
+    CHECK_THROWS(radiation.addRectangleRadiationSource(vec3{0, 0, 1}, vec2{0.f, 1.f}, vec3{0, 0, 0}));
+    CHECK_THROWS(radiation.addDiskRadiationSource(vec3{0, 0, 1}, 0.f, vec3{0, 0, 0}));
     CHECK_THROWS(radiation.getPrimitiveData("radiation_flux_SW"));
     CHECK(radiation.getBackendName().find("gfx950") != std::string::npos);
 }
