@@ -300,6 +300,13 @@ int rt_enable_timing(rt_ctx *ctx, int on) {
     return 0;
 }
 
+int rt_set_direct_grids(rt_ctx *ctx, int on) {
+    NN(ctx, "ctx");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->sx.use_grid = on != 0;
+    return 0;
+}
+
 int rt_get_timing(rt_ctx *ctx, rt_timing *out) {
     NN(ctx, "ctx");
     NN(out, "out");

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <string>
@@ -158,40 +159,59 @@ extern "C" int rt_autotune(rt_ctx *ctx, const rt_scene *scene, const rt_band_req
     LaunchCfg best[3];
     double best_ms[3] = {0, 0, 0}, def_ms[3] = {0, 0, 0};
     uint32_t count = 0;
-    // occlusion-grid density of the direct pass (non-periodic scenes): grids over the whole scene for the
-    // sample direction, traced with the default shape; a density costs its build plus the sample trace
-    // scaled to all P primitives.  The shapes are then searched at the chosen density.
+    // occlusion-grid density of the direct pass (non-periodic scenes).  A density costs, summed over a few
+    // sun directions (the request's first collimated source, if any, plus zenith 20, 50 and 75 degrees:
+    // the lists grow as the sun gets lower), the grid build plus a direct pass over all P primitives with
+    // the default shape (a full pass, not the sample: the list lengths depend on the whole scene and a short
+    // launch is dominated by its tail).  The shapes are then searched at the chosen density.
     GridSet gs;
     uint32_t density = 2;
     bool use_grid = false;
     if (rc == 0 && ctx->sx.use_grid) {
         DirectJob hdj{};
         if (hipMemcpy(&hdj, dj, sizeof(hdj), hipMemcpyDeviceToHost) != hipSuccess) rc = -1;
+        std::vector<std::array<float, 3>> dirs;
+        const bool own = req->nsources && req->source_dir && (!req->source_type || req->source_type[0] == 0u) &&
+                         (req->source_dir[0] * req->source_dir[0] + req->source_dir[1] * req->source_dir[1] +
+                              req->source_dir[2] * req->source_dir[2] > 1e-12f);
+        if (own) dirs.push_back({req->source_dir[0], req->source_dir[1], req->source_dir[2]});
+        const double zen[3] = {20.0, 50.0, 75.0}, az[3] = {30.0, 150.0, 260.0};
+        for (int k = 0; k < 3; k++) {
+            const double z = zen[k] * M_PI / 180.0, a = az[k] * M_PI / 180.0;
+            dirs.push_back({(float)(std::sin(z) * std::sin(a)), (float)(std::sin(z) * std::cos(a)), (float)std::cos(z)});
+        }
         double best_d = -1.0;
         for (uint32_t dens : {2u, 1u, 4u, 8u}) {  // the default first: others must beat it by 3%
             if (rc) break;
             gs.density = (float)dens;
-            if (hipStreamSynchronize(st) != hipSuccess) { rc = -1; break; }
-            const auto t0 = std::chrono::steady_clock::now();
-            int g = build_sun_grids(gs, S, hdj.sdir, 1, st);
-            if (g < 0) { rc = -1; break; }
-            if (g > 0) break;  // grids do not apply to this scene (or not at this density)
-            const double build_ms =
-                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-            DirectJob j = hdj;
-            j.grid = gs.dev;
-            if (hipMemcpy(dj, &j, sizeof(j), hipMemcpyHostToDevice) != hipSuccess) { rc = -1; break; }
-            const LaunchCfg def{};
-            double ms = time_cfg(st, tm, def, [&](const LaunchCfg &c) {
-                return launch_direct(dj, 1, sample, depth, counter, err, nullptr, c, ncu, st);
-            });
+            double cost = 0.0;
+            bool applies = true;
+            for (const auto &dv : dirs) {
+                if (hipStreamSynchronize(st) != hipSuccess) { rc = -1; break; }
+                const auto t0 = std::chrono::steady_clock::now();
+                int g = build_sun_grids(gs, S, dv.data(), 1, st);
+                if (g < 0) { rc = -1; break; }
+                if (g > 0) { applies = false; break; }  // grids do not apply (or not at this density)
+                const double build_ms =
+                    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                DirectJob j = hdj;
+                j.sdir[0] = dv[0]; j.sdir[1] = dv[1]; j.sdir[2] = dv[2];
+                j.grid = gs.dev;
+                if (hipMemcpy(dj, &j, sizeof(j), hipMemcpyHostToDevice) != hipSuccess) { rc = -1; break; }
+                const LaunchCfg def{};
+                double ms = time_cfg(st, tm, def, [&](const LaunchCfg &c) {
+                    return launch_direct(dj, 1, P, depth, counter, err, nullptr, c, ncu, st);
+                });
+                if (ms < 0) { rc = -1; break; }
+                cost += ms + build_ms;
+            }
+            if (rc || !applies) break;
             count++;
-            if (ms < 0) { rc = -1; break; }
-            const double cost = ms * ((double)P / sample) + build_ms;
             if (best_d < 0 || cost < 0.97 * best_d) { best_d = cost; density = dens; }
             use_grid = true;
         }
         if (rc == 0 && use_grid) {
+            // the shape search below traces the request's (or the default) direction at the chosen density
             gs.density = (float)density;
             if (build_sun_grids(gs, S, hdj.sdir, 1, st) != 0) rc = -1;
             DirectJob j = hdj;

@@ -113,6 +113,7 @@ SIGNATURES = {
     "rt_set_launch_cfg": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(rt_launch_cfg)]),
     "rt_get_launch_cfg": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(rt_launch_cfg)]),
     "rt_enable_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "rt_set_direct_grids": (C.c_int, [C.c_void_p, C.c_int]),
     "rt_set_counting": (C.c_int, [C.c_void_p, C.c_int]),
     "rt_get_counters": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "rt_get_timing": (C.c_int, [C.c_void_p, C.POINTER(rt_timing)]),
@@ -260,6 +261,10 @@ class Context:
 
     def enable_timing(self, on: bool = True):
         _check(self.lib.rt_enable_timing(self._h, 1 if on else 0))
+
+    def set_direct_grids(self, on: bool):
+        """Occlusion grids for collimated direct rays (default on) or BVH traversal only; same results."""
+        _check(self.lib.rt_set_direct_grids(self._h, 1 if on else 0))
 
     def timing(self) -> dict:
         t = rt_timing()

@@ -1,10 +1,12 @@
 """Summarise rocprofv3 SQLite results (``*_results.db``) into short text: per-kernel timing and,
 for PMC runs, per-dispatch counter values of the ray kernels and of the byte-counter calibration copy.
 
-usage: python -m helios_amd.rocprof_summary KERNEL_TRACE_DB [PMC_DB ...] [--json OUT.json]
+usage: python -m helios_amd.rocprof_summary KERNEL_TRACE_DB [PMC_DB ...] [--json OUT.json] [--brief]
 
 With --json, also writes HBM bytes per dispatch of each ray kernel: FETCH_SIZE and WRITE_SIZE
-(KiB) scaled by what the same counters read for the calibration copy of known size.
+(KiB) scaled by what the same counters read for the calibration copy of known size.  A PMC run with
+TCC_HIT_sum / TCC_MISS_sum adds each ray kernel's L2 hit rate (its largest dispatch).  --brief leaves
+out the per-dispatch counter listing.
 """
 from __future__ import annotations
 
@@ -33,7 +35,7 @@ def counter_rows(db):
                             "order by dispatch_id"))
 
 
-def summarize(kt_db, pmc_dbs, calib_bytes=1 << 30):
+def summarize(kt_db, pmc_dbs, calib_bytes=1 << 30, brief=False):
     out = []
     rows = kernel_rows(kt_db)
     agg = collections.OrderedDict()
@@ -54,17 +56,32 @@ def summarize(kt_db, pmc_dbs, calib_bytes=1 << 30):
         out.append(f"resources {k}: grid {g} workgroup {w} vgpr {v} sgpr {sg} lds {lds}")
     traffic = collections.defaultdict(lambda: collections.defaultdict(list))
     scale = {}
+    l2 = collections.defaultdict(dict)  # kernel -> dispatch -> {counter: value}
     for db in pmc_dbs:
         rows = counter_rows(db)
-        out.append(f"\n== {db.split('/')[-1]}: counter value per dispatch (KiB)")
+        if not brief:
+            out.append(f"\n== {db.split('/')[-1]}: counter value per dispatch (KiB)")
         for d, n, c, val in rows:
             sn = short(n)
+            if c.startswith("TCC_HIT") or c.startswith("TCC_MISS"):
+                if any(t in sn for t in RAY_KERNELS):
+                    l2[sn].setdefault(d, {})[c] = val
+                continue
             if "copyBuffer" in sn and val >= calib_bytes / 1024 / 4:
                 scale[c] = calib_bytes / 1024.0 / val
                 out.append(f"dispatch {d:6d} {sn:60s} {c:12s} {val:16.1f}  <- calibration copy of {calib_bytes} B")
             elif any(t in sn for t in RAY_KERNELS):
                 traffic[sn][c].append(val)
-                out.append(f"dispatch {d:6d} {sn:60s} {c:12s} {val:16.1f}")
+                if not brief:
+                    out.append(f"dispatch {d:6d} {sn:60s} {c:12s} {val:16.1f}")
+    if l2:
+        out.append("\n== L2 (TCC) hit rate of each ray kernel's largest dispatch")
+        for k, ds in l2.items():
+            d, cs = max(ds.items(), key=lambda kv: sum(kv[1].values()))
+            hit = sum(v for c, v in cs.items() if c.startswith("TCC_HIT"))
+            miss = sum(v for c, v in cs.items() if c.startswith("TCC_MISS"))
+            if hit + miss > 0:
+                out.append(f"{k}: dispatch {d} hits {hit:.3e} misses {miss:.3e} hit rate {hit / (hit + miss):.3f}")
     summary = {}
     if scale:
         out.append("\n== HBM bytes per dispatch (counter KiB x 1024 x calibration factor)")
@@ -86,7 +103,9 @@ def main(argv):
         i = argv.index("--json")
         js = argv[i + 1]
         argv = argv[:i] + argv[i + 2:]
-    text, summary, scale = summarize(argv[0], argv[1:])
+    brief = "--brief" in argv
+    argv = [a for a in argv if a != "--brief"]
+    text, summary, scale = summarize(argv[0], argv[1:], brief=brief)
     print(text)
     if js:
         with open(js, "w") as f:

@@ -305,6 +305,10 @@ typedef struct {
     uint32_t grid_entries;  /* list entries per direction per traced primitive (x100) of those grids */
 } rt_timing;
 int rt_enable_timing(rt_ctx *ctx, int on);
+/* Occlusion test of collimated direct rays: 1 (default) = per-direction occlusion grids when the scene is
+ * not periodic and they fit, 0 = always traverse the BVH.  Results are identical either way; grids cost
+ * memory (about 100 B per primitive per direction) and are faster. */
+int rt_set_direct_grids(rt_ctx *ctx, int on);
 /* Traversal work counters (zeroed by rt_set_counting): out[9] = direct (BVH node fetches, primitive
  * tests, rays), diffuse/scatter (same three), castRays (same three).  They define the counted
  * algorithmic bytes per ray used for the roofline. */

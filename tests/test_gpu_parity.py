@@ -94,6 +94,28 @@ def test_direct_sums_config1(ctx):
     assert np.max(np.abs(sg - so)) <= 1e-5 * max(1.0, np.abs(so).max())
 
 
+def test_direct_grid_equals_bvh(ctx):
+    """Occlusion grids and BVH traversal make the same decision for every shadow ray: identical sums and
+    fluxes, on a scene with many overlapping primitives and several sun directions."""
+    s = small_canopy(23, 4000)
+    g = ctx.scene(s.transforms, s.types, s.twosided)
+    src = [native.sun_direction(z, a) for z, a in ((5, 0), (35, 120), (70, 250), (85, 10))]
+    req = two_band_request(s, depth=2, nrays=36)
+    try:
+        ctx.set_direct_grids(True)
+        sg = g.direct_sums(12, 5, src)
+        Rg = native.run_band(g, req)
+        assert ctx.timing()["direct_grid"] == 1
+        ctx.set_direct_grids(False)
+        sb = g.direct_sums(12, 5, src)
+        Rb = native.run_band(g, req)
+        assert ctx.timing()["direct_grid"] == 0
+    finally:
+        ctx.set_direct_grids(True)
+    assert np.array_equal(sg, sb)
+    assert np.array_equal(Rg, Rb)
+
+
 @pytest.mark.parametrize("schedule", [0, 1])
 def test_run_band_config1(ctx, schedule):
     s = scenes.config1(1)
