@@ -629,38 +629,39 @@ __device__ __forceinline__ void add_stats(unsigned long long *stats, const TCnt 
 // Occlusion of one shadow ray through the source's grid: test the primitives binned in the cell of the
 // origin's projection with the BVH traversal's predicate (t in (1e-4, 1e30], not the origin primitive) and
 // stop at the first hit.  Non-periodic scenes only (the grid is not built otherwise).
-template <typename C>
-__device__ __forceinline__ bool grid_occluded(const SunGrid &G, const float4 *recs, uint32_t nleaf, v3f o, v3f d,
-                                              uint32_t skip, C &cnt) {
-    cnt.nodes++;
+// Candidate range [b, e) of the grid cell holding origin o's projection.
+__device__ __forceinline__ void grid_cell(const SunGrid &G, v3f o, uint32_t &b, uint32_t &e) {
     const float qu = o.x * G.u[0] + o.y * G.u[1] + o.z * G.u[2];
     const float qv = o.x * G.v[0] + o.y * G.v[1] + o.z * G.v[2];
     float fx = floorf((qu - G.g0u) * G.inv_cs), fy = floorf((qv - G.g0v) * G.inv_cs);
     const int cx = fx < 0.f ? 0 : (fx > (float)(G.nx - 1) ? (int)(G.nx - 1) : (int)fx);
     const int cy = fy < 0.f ? 0 : (fy > (float)(G.ny - 1) ? (int)(G.ny - 1) : (int)fy);
     const uint32_t c = (uint32_t)cy * G.nx + (uint32_t)cx;
-    const uint32_t b = G.start[c] - G.sbase, e = G.start[c + 1] - G.sbase;
-    const uint32_t *items = G.items;
-    // two candidates per iteration, all their loads issued before either test (the list entry -> record
-    // chain is latency-bound); a slot that is out of the list or unfilled loads record 0 and is ignored
-    for (uint32_t p = b; p < e; p += 2) {
-        const bool two = p + 1 < e;
-        const uint32_t k0 = items[p];
-        const uint32_t k1 = two ? items[p + 1] : RT_MISS;
-        const bool v0 = k0 < nleaf, v1 = k1 < nleaf;
-        const float4 *q0 = recs + (size_t)(v0 ? k0 : 0u) * 4, *q1 = recs + (size_t)(v1 ? k1 : 0u) * 4;
-        const float4 a0 = ldc(q0), a1 = ldc(q0 + 1), a2 = ldc(q0 + 2), a3 = ldc(q0 + 3);
-        const float4 b0 = ldc(q1), b1 = ldc(q1 + 1), b2 = ldc(q1 + 2), b3 = ldc(q1 + 3);
-        const uint32_t ta = __float_as_uint(a0.w), tb = __float_as_uint(b0.w);
-        float t0, t1;
-        const bool h0 = v0 && isect_rec<false>(a0, a1, a2, a3, (ta & RT_PATCH_BIT) != 0, o, d, t0) &&
-                        (ta & ~RT_PATCH_BIT) != skip && t0 > 1e-4f && t0 <= 1e30f;
-        const bool h1 = v1 && isect_rec<false>(b0, b1, b2, b3, (tb & RT_PATCH_BIT) != 0, o, d, t1) &&
-                        (tb & ~RT_PATCH_BIT) != skip && t1 > 1e-4f && t1 <= 1e30f;
-        cnt.prims += (v0 ? 1u : 0u) + ((v1 && !h0) ? 1u : 0u);  // the tests a one-at-a-time scan makes
-        if (h0 || h1) return true;
-    }
-    return false;
+    b = G.start[c] - G.sbase;
+    e = G.start[c + 1] - G.sbase;
+}
+
+// Candidates p and p + 1 (when p + 1 < e) of a cell list: is either accepted as an occluder?  Both
+// candidates' loads are issued before either test (the list entry -> record chain is latency-bound); a slot
+// that is out of the list or unfilled loads record 0 and is ignored.
+template <typename C>
+__device__ __forceinline__ bool grid_pair(const uint32_t *items, const float4 *recs, uint32_t nleaf, uint32_t p,
+                                          uint32_t e, v3f o, v3f d, uint32_t skip, C &cnt) {
+    const bool two = p + 1 < e;
+    const uint32_t k0 = items[p];
+    const uint32_t k1 = two ? items[p + 1] : RT_MISS;
+    const bool v0 = k0 < nleaf, v1 = k1 < nleaf;
+    const float4 *q0 = recs + (size_t)(v0 ? k0 : 0u) * 4, *q1 = recs + (size_t)(v1 ? k1 : 0u) * 4;
+    const float4 a0 = ldc(q0), a1 = ldc(q0 + 1), a2 = ldc(q0 + 2), a3 = ldc(q0 + 3);
+    const float4 b0 = ldc(q1), b1 = ldc(q1 + 1), b2 = ldc(q1 + 2), b3 = ldc(q1 + 3);
+    const uint32_t ta = __float_as_uint(a0.w), tb = __float_as_uint(b0.w);
+    float t0, t1;
+    const bool h0 = v0 && isect_rec<false>(a0, a1, a2, a3, (ta & RT_PATCH_BIT) != 0, o, d, t0) &&
+                    (ta & ~RT_PATCH_BIT) != skip && t0 > 1e-4f && t0 <= 1e30f;
+    const bool h1 = v1 && isect_rec<false>(b0, b1, b2, b3, (tb & RT_PATCH_BIT) != 0, o, d, t1) &&
+                    (tb & ~RT_PATCH_BIT) != skip && t1 > 1e-4f && t1 <= 1e30f;
+    cnt.prims += (v0 ? 1u : 0u) + ((v1 && !h0) ? 1u : 0u);  // the tests a one-at-a-time scan makes
+    return h0 || h1;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -669,6 +670,9 @@ __device__ __forceinline__ bool grid_occluded(const SunGrid &G, const float4 *re
 // COUNT: the counting variant (stats != nullptr) traces every ray on its own (per-lane LDS stack) so the
 // counters hold single-ray node visits and primitive tests -- the algorithmic work per ray -- rather than
 // the packet's visits.  Both variants make the same occlusion decision for every ray.
+#ifndef RT_DIRECT_REFILL
+#define RT_DIRECT_REFILL 24
+#endif
 template <int WPB, int STK, bool COUNT>
 __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict__ jobs, uint32_t njobs, uint32_t total,
                                                      uint32_t *counter, uint32_t chunk, int *err,
@@ -712,21 +716,79 @@ __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict
                     for (int q = 0; q < 12; q++) tm[q] = ldu(Tp + q);
                     ksum acc;
                     acc.init();
-                    for (uint32_t base = 0; base < Nrays; base += 64) {
-                        uint32_t r = base + lane;
-                        bool need = r < Nrays;
+                    const uint32_t seed = ldu(&J.seed);
+                    if (grid) {
+                        // Occlusion grid: each lane scans its ray's cell list two candidates per step.  A lane
+                        // whose ray ends (occluder found, or list exhausted: lit) takes the primitive's next
+                        // ray; idle lanes are refilled together once fewer than RT_DIRECT_REFILL lanes still
+                        // scan (every ray is tested exactly as on its own; only the ray -> lane assignment
+                        // and the summation grouping differ).
+                        const SunGrid &G = *grid;
+                        const uint32_t *items = G.items;
+                        const uint32_t nleaf = ldu(&sc.nleaf);
+                        uint32_t r = lane, pc = 0, pe = 0;
                         v3f o = mk3(0.f, 0.f, 0.f);
-                        if (need) {
+                        bool act = r < Nrays;
+                        if (act) {
                             float spx, spy;
-                            direct_sample(i, r % d, r / d, d, ldu(&J.seed), patch, spx, spy);
+                            direct_sample(i, r % d, r / d, d, seed, patch, spx, spy);
                             o = xform3(tm, spx, spy, 0.0f);
+                            grid_cell(G, o, pc, pe);
+                            cnt.nodes++;
                             nray++;
                         }
-                        bool occl;
-                        if (grid) occl = need && grid_occluded(*grid, recs, ldu(&sc.nleaf), o, dir, i, cnt);
-                        else if (COUNT) occl = need && occluded<STK>(sc, o, dir, 1e-4f, i, wstk, WPB * 64, err, cnt);
-                        else occl = packet_occluded<STK>(sc, o, dir, 1e-4f, i, need, wstk, wstk4, err, cnt);
-                        if (need && !occl) acc.add(strength);
+                        uint32_t qnext = min(64u, Nrays);
+                        while (true) {
+                            const uint32_t floor_lanes = qnext < Nrays ? RT_DIRECT_REFILL : 1u;
+                            while (true) {
+                                if (act) {
+                                    if (pc >= pe) {
+                                        acc.add(strength);
+                                        act = false;
+                                    } else if (grid_pair(items, recs, nleaf, pc, pe, o, dir, i, cnt)) {
+                                        act = false;
+                                    } else {
+                                        pc += 2;
+                                    }
+                                }
+                                if ((uint32_t)__popcll(__ballot(act)) < floor_lanes) break;
+                            }
+                            const uint64_t idle = __ballot(!act);
+                            if (qnext >= Nrays) {
+                                if (idle == ~0ull) break;
+                                continue;
+                            }
+                            const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
+                                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+                            const uint32_t take = min((uint32_t)__popcll(idle), Nrays - qnext);
+                            if (!act && rank < take) {
+                                r = qnext + rank;
+                                float spx, spy;
+                                direct_sample(i, r % d, r / d, d, seed, patch, spx, spy);
+                                o = xform3(tm, spx, spy, 0.0f);
+                                grid_cell(G, o, pc, pe);
+                                cnt.nodes++;
+                                nray++;
+                                act = true;
+                            }
+                            qnext += take;
+                        }
+                    } else {
+                        for (uint32_t base = 0; base < Nrays; base += 64) {
+                            uint32_t r = base + lane;
+                            bool need = r < Nrays;
+                            v3f o = mk3(0.f, 0.f, 0.f);
+                            if (need) {
+                                float spx, spy;
+                                direct_sample(i, r % d, r / d, d, seed, patch, spx, spy);
+                                o = xform3(tm, spx, spy, 0.0f);
+                                nray++;
+                            }
+                            bool occl;
+                            if (COUNT) occl = need && occluded<STK>(sc, o, dir, 1e-4f, i, wstk, WPB * 64, err, cnt);
+                            else occl = packet_occluded<STK>(sc, o, dir, 1e-4f, i, need, wstk, wstk4, err, cnt);
+                            if (need && !occl) acc.add(strength);
+                        }
                     }
                     wave_reduce(acc);
                     sum_out = acc.value();
@@ -1165,6 +1227,9 @@ __global__ __launch_bounds__(256) void k_gather_rays(const GatherJob *__restrict
 // ------------------------------------------------------------------------------------------------
 // gather: lanes over channels (many channels, e.g. suns x bands of a diurnal batch)
 // ------------------------------------------------------------------------------------------------
+#ifndef RT_GATHER_UNROLL
+#define RT_GATHER_UNROLL 4
+#endif
 template <int CB>
 __global__ __launch_bounds__(256) void k_gather_chan(const GatherJob *__restrict__ jobs, uint32_t njobs, uint32_t total,
                                                      uint32_t *counter, uint32_t chunk) {
@@ -1181,34 +1246,40 @@ __global__ __launch_bounds__(256) void k_gather_chan(const GatherJob *__restrict
             uint32_t nn = J.n * J.n, C = J.C;
             const uint32_t *w = J.hits + (size_t)i * 2 * nn;
             ksum acc[2][CB];
-            float sky[CB];
 #pragma unroll
             for (int k = 0; k < CB; k++) {
                 acc[0][k].init();
                 acc[1][k].init();
-                uint32_t c = lane + 64 * k;
-                sky[k] = c < C ? J.dflux[c] : 0.f;
             }
+            const float *dfl = J.dflux, *otop = J.out_top, *obot = J.out_bot;
             for (uint32_t base = 0; base < 2 * nn; base += 64) {
                 uint32_t mine = base + lane < 2 * nn ? w[base + lane] : RT_NOT_LAUNCHED;
                 uint32_t cnt = min(64u, 2 * nn - base);
-                for (uint32_t q = 0; q < cnt; q++) {
-                    uint32_t word = __builtin_amdgcn_readlane(mine, q);
-                    if (word == RT_NOT_LAUNCHED) continue;
-                    uint32_t slot = base + q >= nn ? 1u : 0u;
-                    if (word == RT_MISS) {
+                // RT_GATHER_UNROLL hit words per step: every row (the hit primitive's radiation_out C-vector,
+                // or the sky fluxes for a miss) is loaded before any is summed, so that many rows are in
+                // flight per wave; the sums take them in ray order.  A face slot that was not launched adds
+                // nothing (its value is 0, which leaves a compensated sum unchanged).
+                for (uint32_t q0 = 0; q0 < cnt; q0 += RT_GATHER_UNROLL) {
+                    float v[RT_GATHER_UNROLL][CB];
 #pragma unroll
-                        for (int k = 0; k < CB; k++) {
-                            if (slot) acc[1][k].add(sky[k]); else acc[0][k].add(sky[k]);
-                        }
-                    } else {
-                        uint32_t h = word & ~RT_HIT_BOTTOM;
-                        const float *src = ((word & RT_HIT_BOTTOM) ? J.out_bot : J.out_top) + (size_t)h * C;
+                    for (int u = 0; u < RT_GATHER_UNROLL; u++) {
+                        const uint32_t q = q0 + u;
+                        const uint32_t word = q < cnt ? __builtin_amdgcn_readlane(mine, q) : RT_NOT_LAUNCHED;
+                        const bool skip = word == RT_NOT_LAUNCHED;
+                        const float *src = word == RT_MISS ? dfl
+                                           : ((word & RT_HIT_BOTTOM) ? obot : otop) + (size_t)(word & ~RT_HIT_BOTTOM) * C;
 #pragma unroll
                         for (int k = 0; k < CB; k++) {
                             uint32_t c = lane + 64 * k;
-                            float v = c < C ? src[c] : 0.f;
-                            if (slot) acc[1][k].add(v); else acc[0][k].add(v);
+                            v[u][k] = (!skip && c < C) ? src[c] : 0.f;
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < RT_GATHER_UNROLL; u++) {
+                        const uint32_t slot = base + q0 + u >= nn ? 1u : 0u;
+#pragma unroll
+                        for (int k = 0; k < CB; k++) {
+                            if (slot) acc[1][k].add(v[u][k]); else acc[0][k].add(v[u][k]);
                         }
                     }
                 }

@@ -95,8 +95,10 @@ def test_direct_sums_config1(ctx):
 
 
 def test_direct_grid_equals_bvh(ctx):
-    """Occlusion grids and BVH traversal make the same decision for every shadow ray: identical sums and
-    fluxes, on a scene with many overlapping primitives and several sun directions."""
+    """Occlusion grids and BVH traversal make the same decision for every shadow ray, on a scene with many
+    overlapping primitives and several sun directions.  The grid path hands rays to lanes dynamically, so
+    its compensated sums group the (equal) per-ray strengths differently: they may differ in the last bit,
+    while a single differing decision would move a sum by 1/144 relative."""
     s = small_canopy(23, 4000)
     g = ctx.scene(s.transforms, s.types, s.twosided)
     src = [native.sun_direction(z, a) for z, a in ((5, 0), (35, 120), (70, 250), (85, 10))]
@@ -112,8 +114,8 @@ def test_direct_grid_equals_bvh(ctx):
         assert ctx.timing()["direct_grid"] == 0
     finally:
         ctx.set_direct_grids(True)
-    assert np.array_equal(sg, sb)
-    assert np.array_equal(Rg, Rb)
+    assert np.max(np.abs(sg - sb) / np.maximum(np.abs(sb), 1e-30)) <= 1e-6
+    assert rel_err(Rg, Rb, floor=1.0).max() <= 1e-6
 
 
 @pytest.mark.parametrize("schedule", [0, 1])
