@@ -40,9 +40,10 @@ float o2f_g(uint32_t u) {
     return f;
 }
 
-// padded projected rectangle of the primitive in leaf slot k
+// padded projected rectangle of the primitive in leaf slot k, and its depth key: the largest depth of its
+// vertices along the source direction w, plus the same padding (an upper bound of any of its points' depth)
 __device__ __forceinline__ void proj_rect(const float4 *__restrict__ rec, uint32_t k, const SunGrid &G, float smax,
-                                          float &u0, float &u1, float &v0, float &v1) {
+                                          float &u0, float &u1, float &v0, float &v1, float &key) {
     const float4 r0 = rec[(size_t)k * 4], r1 = rec[(size_t)k * 4 + 1], r2 = rec[(size_t)k * 4 + 2];
     const bool patch = (__float_as_uint(r0.w) & RT_PATCH_BIT) != 0;
     float cx[4], cy[4], cz[4];
@@ -53,13 +54,16 @@ __device__ __forceinline__ void proj_rect(const float4 *__restrict__ rec, uint32
     else { cx[3] = cx[0]; cy[3] = cy[0]; cz[3] = cz[0]; }
     u0 = v0 = INFINITY;
     u1 = v1 = -INFINITY;
+    float kmax = -INFINITY;
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
 #pragma unroll
     for (int c = 0; c < 4; c++) {
         float pu = cx[c] * G.u[0] + cy[c] * G.u[1] + cz[c] * G.u[2];
         float pv = cx[c] * G.v[0] + cy[c] * G.v[1] + cz[c] * G.v[2];
+        float pw = cx[c] * G.w[0] + cy[c] * G.w[1] + cz[c] * G.w[2];
         u0 = fminf(u0, pu); u1 = fmaxf(u1, pu);
         v0 = fminf(v0, pv); v1 = fmaxf(v1, pv);
+        kmax = fmaxf(kmax, pw);
         lo[0] = fminf(lo[0], cx[c]); hi[0] = fmaxf(hi[0], cx[c]);
         lo[1] = fminf(lo[1], cy[c]); hi[1] = fmaxf(hi[1], cy[c]);
         lo[2] = fminf(lo[2], cz[c]); hi[2] = fmaxf(hi[2], cz[c]);
@@ -67,6 +71,7 @@ __device__ __forceinline__ void proj_rect(const float4 *__restrict__ rec, uint32
     const float ext = fmaxf(fmaxf(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]);
     const float pad = ext * 2e-4f + smax * 1e-5f + 1e-30f;
     u0 -= pad; u1 += pad; v0 -= pad; v1 += pad;
+    key = kmax + pad;
 }
 
 __device__ __forceinline__ int cell_of(float q, float g0, float inv_cs, uint32_t n) {
@@ -79,8 +84,8 @@ __global__ void k_grid_bounds(uint32_t nleaf, const float4 *__restrict__ rec, fl
                               const SunGrid *__restrict__ grids, uint32_t *__restrict__ bounds /*[n][4]*/) {
     const SunGrid G = grids[blockIdx.y];
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    float u0 = INFINITY, u1 = -INFINITY, v0 = INFINITY, v1 = -INFINITY;
-    if (k < nleaf) proj_rect(rec, k, G, smax, u0, u1, v0, v1);
+    float u0 = INFINITY, u1 = -INFINITY, v0 = INFINITY, v1 = -INFINITY, key;
+    if (k < nleaf) proj_rect(rec, k, G, smax, u0, u1, v0, v1, key);
     // block reduction, then one atomic per block
     __shared__ float s[4][256];
     s[0][threadIdx.x] = u0; s[1][threadIdx.x] = -u1; s[2][threadIdx.x] = v0; s[3][threadIdx.x] = -v1;
@@ -105,14 +110,101 @@ __global__ void k_grid_count(uint32_t nleaf, const float4 *__restrict__ rec, flo
     const SunGrid G = grids[blockIdx.y];
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nleaf) return;
-    float u0, u1, v0, v1;
-    proj_rect(rec, k, G, smax, u0, u1, v0, v1);
+    float u0, u1, v0, v1, key;
+    proj_rect(rec, k, G, smax, u0, u1, v0, v1, key);
     const int x0 = cell_of(u0, G.g0u, G.inv_cs, G.nx), x1 = cell_of(u1, G.g0u, G.inv_cs, G.nx);
     const int y0 = cell_of(v0, G.g0v, G.inv_cs, G.ny), y1 = cell_of(v1, G.g0v, G.inv_cs, G.ny);
     atomicAdd(&entries[blockIdx.y], (unsigned long long)(x1 - x0 + 1) * (unsigned long long)(y1 - y0 + 1));
     uint32_t *c = counts + cell_base[blockIdx.y];
     for (int y = y0; y <= y1; y++)
         for (int x = x0; x <= x1; x++) atomicAdd(&c[(size_t)y * G.nx + x], 1u);
+}
+
+// Composite sort key (direction, depth key) of every (direction, primitive) pair: sorting them puts each
+// direction's primitives in ascending key order, the order k_grid_fill takes them in.
+__global__ void k_grid_sortkeys(uint32_t nleaf, const float4 *__restrict__ rec, float smax,
+                                const SunGrid *__restrict__ grids, unsigned long long *__restrict__ skey,
+                                uint32_t *__restrict__ sval) {
+    const SunGrid G = grids[blockIdx.y];
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nleaf) return;
+    float u0, u1, v0, v1, key;
+    proj_rect(rec, k, G, smax, u0, u1, v0, v1, key);
+    const size_t g = (size_t)blockIdx.y * nleaf + k;
+    skey[g] = ((unsigned long long)blockIdx.y << 32) | f2o_g(key);
+    sval[g] = k;
+}
+
+// Insertion sort of each cell's list into descending key order (ties: ascending leaf slot), so the list
+// order is deterministic and a scan can stop at the first key below the ray origin's depth.  A workgroup
+// takes RT_GRID_SORT_CELLS consecutive cells, whose lists are contiguous: when they hold at most
+// RT_GRID_SORT_LDS entries it stages them in LDS and each thread sorts its cell there; otherwise each thread
+// sorts in global memory.
+#define RT_GRID_SORT_LDS 8192
+#ifndef RT_GRID_PRESORT
+#define RT_GRID_PRESORT 1
+#endif
+#define RT_GRID_SORT_CELLS 128
+__device__ __forceinline__ bool grid_before(float ka, uint32_t ia, float kb, uint32_t ib) {
+    return ka > kb || (ka == kb && ia <= ib);
+}
+
+template <typename K, typename I>
+__device__ __forceinline__ void grid_isort(K *keys, I *items, uint32_t b, uint32_t e) {
+    for (uint32_t x = b + 1; x < e; x++) {
+        const float kx = keys[x];
+        const uint32_t ix = items[x];
+        uint32_t y = x;
+        while (y > b) {
+            const float ky = keys[y - 1];
+            const uint32_t iy = items[y - 1];
+            if (grid_before(ky, iy, kx, ix)) break;
+            keys[y] = ky;
+            items[y] = iy;
+            y--;
+        }
+        keys[y] = kx;
+        items[y] = ix;
+    }
+}
+
+__global__ __launch_bounds__(RT_GRID_SORT_CELLS) void k_grid_sort(const SunGrid *__restrict__ grids,
+                                                   const uint32_t *__restrict__ cell_base,
+                                                   const unsigned long long *__restrict__ item_base,
+                                                   const uint32_t *__restrict__ start, uint32_t *__restrict__ items,
+                                                   float *__restrict__ keys) {
+    __shared__ float lk[RT_GRID_SORT_LDS];
+    __shared__ uint32_t li[RT_GRID_SORT_LDS];
+    const SunGrid G = grids[blockIdx.y];
+    const uint32_t ncell = G.nx * G.ny;
+    const uint32_t c0 = blockIdx.x * RT_GRID_SORT_CELLS;
+    if (c0 >= ncell) return;  // whole workgroup
+    const uint32_t c1 = min(c0 + RT_GRID_SORT_CELLS, ncell);
+    const size_t cb = cell_base[blockIdx.y];
+    const unsigned long long ib = item_base[blockIdx.y];
+    const uint32_t sb = (uint32_t)ib;
+    const uint32_t s0 = start[cb + c0] - sb, s1 = start[cb + c1] - sb;  // this workgroup's entries
+    const uint32_t c = c0 + threadIdx.x;
+    uint32_t b = 0, e = 0;
+    if (c < c1) {
+        b = start[cb + c] - sb;
+        e = start[cb + c + 1] - sb;
+    }
+    if (s1 - s0 <= RT_GRID_SORT_LDS) {
+        for (uint32_t x = threadIdx.x; x < s1 - s0; x += RT_GRID_SORT_CELLS) {
+            lk[x] = keys[ib + s0 + x];
+            li[x] = items[ib + s0 + x];
+        }
+        __syncthreads();
+        if (c < c1) grid_isort(lk, li, b - s0, e - s0);
+        __syncthreads();
+        for (uint32_t x = threadIdx.x; x < s1 - s0; x += RT_GRID_SORT_CELLS) {
+            keys[ib + s0 + x] = lk[x];
+            items[ib + s0 + x] = li[x];
+        }
+    } else if (c < c1) {
+        grid_isort(keys + ib, items + ib, b, e);
+    }
 }
 
 // start[] is one exclusive scan over the cells of all directions in uint32 arithmetic, so it holds the
@@ -122,12 +214,14 @@ __global__ void k_grid_fill(uint32_t nleaf, const float4 *__restrict__ rec, floa
                             const SunGrid *__restrict__ grids, const uint32_t *__restrict__ cell_base,
                             const unsigned long long *__restrict__ item_base,
                             const unsigned long long *__restrict__ item_count, const uint32_t *__restrict__ start,
-                            uint32_t *__restrict__ counts, uint32_t *__restrict__ items) {
+                            uint32_t *__restrict__ counts, uint32_t *__restrict__ items, float *__restrict__ keys,
+                            const uint32_t *__restrict__ order) {
     const SunGrid G = grids[blockIdx.y];
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nleaf) return;
-    float u0, u1, v0, v1;
-    proj_rect(rec, k, G, smax, u0, u1, v0, v1);
+    if (order) k = order[(size_t)blockIdx.y * nleaf + k];  // this direction's primitives by ascending key
+    float u0, u1, v0, v1, key;
+    proj_rect(rec, k, G, smax, u0, u1, v0, v1, key);
     const int x0 = cell_of(u0, G.g0u, G.inv_cs, G.nx), x1 = cell_of(u1, G.g0u, G.inv_cs, G.nx);
     const int y0 = cell_of(v0, G.g0v, G.inv_cs, G.ny), y1 = cell_of(v1, G.g0v, G.inv_cs, G.ny);
     const size_t base = cell_base[blockIdx.y];
@@ -141,7 +235,10 @@ __global__ void k_grid_fill(uint32_t nleaf, const float4 *__restrict__ rec, floa
             const uint32_t left = atomicSub(&counts[c], 1u);
             if (left == 0u) continue;
             const unsigned long long at = (unsigned long long)(uint32_t)(start[c] - sbase) + left - 1u;
-            if (at < ni) items[ib + at] = k;
+            if (at < ni) {
+                items[ib + at] = k;
+                keys[ib + at] = key;
+            }
         }
 }
 
@@ -179,13 +276,15 @@ int build_sun_grids(GridSet &gs, const Scene &S, const float *dirs, uint32_t n, 
         for (double &x : u) x /= um;
         double v[3] = {d[1] * u[2] - d[2] * u[1], d[2] * u[0] - d[0] * u[2], d[0] * u[1] - d[1] * u[0]};
         SunGrid &G = hg[j];
-        for (int q = 0; q < 3; q++) { G.u[q] = (float)u[q]; G.v[q] = (float)v[q]; }
+        for (int q = 0; q < 3; q++) { G.u[q] = (float)u[q]; G.v[q] = (float)v[q]; G.w[q] = (float)d[q]; }
         G.g0u = G.g0v = 0.f;
         G.inv_cs = 1.f;
         G.nx = G.ny = 1;
         G.sbase = 0;
+        G.sorted = 0;
         G.start = nullptr;
         G.items = nullptr;
+        G.keys = nullptr;
     }
     // 1. projected extents
     if (gs.small.reserve(n * sizeof(SunGrid) + n * 16 + n * 4 + n * 16 + 2048)) return -1;
@@ -251,19 +350,57 @@ int build_sun_grids(GridSet &gs, const Scene &S, const float *dirs, uint32_t n, 
         ibase[j] = total;
         total += ent[j];
     }
-    if (gs.cap && cell_bytes + total * 4 + 256 > gs.cap) return 1;
+    if (gs.cap && cell_bytes + total * 8 + 512 > gs.cap) return 1;
     GK(hipcub::DeviceScan::ExclusiveSum(tmp, scan_bytes, counts, starts, (int)(total_cells + 1), st));
-    if (gs.items.reserve((size_t)total * 4 + 256)) return -1;
+    if (gs.items.reserve((size_t)total * 4 + 256) || gs.keys.reserve((size_t)total * 4 + 256)) return -1;
     uint32_t *items = (uint32_t *)gs.items.base;
-    GK(hipMemsetAsync(items, 0xFF, (size_t)total * 4, st));  // an unfilled slot reads as "no primitive"
+    float *keys = (float *)gs.keys.base;
+    // an unfilled slot reads as "no primitive" with a huge key (sorted first, never ends a scan)
+    GK(hipMemsetAsync(items, 0xFF, (size_t)total * 4, st));
+    GK(hipMemsetAsync(keys, 0x7F, (size_t)total * 4, st));
     GK(hipMemcpyAsync(dibase, ibase.data(), (size_t)n * 8, hipMemcpyHostToDevice, st));
+    // primitives of each direction in ascending key order (one radix sort over all directions), so the
+    // cell lists come out closer to sorted and the per-cell sort below has less to move; left out when the
+    // pairs exceed the radix sort's int count
+    const size_t npairs = (size_t)n * nleaf;
+    const uint32_t *order = nullptr;
+    if (RT_GRID_PRESORT && npairs < (1ull << 31)) {
+        int jbits = 1;
+        while ((1ull << jbits) < n) jbits++;
+        size_t sort_tmp = 0;
+        GK(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_tmp, (const unsigned long long *)nullptr,
+                                              (unsigned long long *)nullptr, (const uint32_t *)nullptr,
+                                              (uint32_t *)nullptr, (int)npairs, 0, 32 + jbits, st));
+        const size_t sb = ((npairs * 8 + 255) & ~(size_t)255);
+        const size_t vb = ((npairs * 4 + 255) & ~(size_t)255);
+        if (gs.sortbuf.reserve(2 * sb + 2 * vb + sort_tmp + 256)) return -1;
+        char *q = (char *)gs.sortbuf.base;
+        unsigned long long *k_in = (unsigned long long *)q, *k_out = (unsigned long long *)(q + sb);
+        uint32_t *v_in = (uint32_t *)(q + 2 * sb), *v_out = (uint32_t *)(q + 2 * sb + vb);
+        void *stmp = q + 2 * sb + 2 * vb;
+        k_grid_sortkeys<<<grid, 256, 0, st>>>(nleaf, S.h.rec, S.h.smax, dgrids, k_in, v_in);
+        GK(hipGetLastError());
+        GK(hipcub::DeviceRadixSort::SortPairs(stmp, sort_tmp, k_in, k_out, v_in, v_out, (int)npairs, 0, 32 + jbits,
+                                              st));
+        order = v_out;
+    }
     k_grid_fill<<<grid, 256, 0, st>>>(nleaf, S.h.rec, S.h.smax, dgrids, dbase, dibase, dentries, starts, counts,
-                                      items);
+                                      items, keys, order);
     GK(hipGetLastError());
+    {
+        uint32_t maxcells = 0;
+        for (uint32_t j = 0; j < n; j++) maxcells = std::max(maxcells, hg[j].nx * hg[j].ny);
+        k_grid_sort<<<dim3((maxcells + RT_GRID_SORT_CELLS - 1) / RT_GRID_SORT_CELLS, n), RT_GRID_SORT_CELLS, 0, st>>>(
+            dgrids, dbase, dibase, starts, items, keys);
+        GK(hipGetLastError());
+    }
+    const bool sorted = true;
     for (uint32_t j = 0; j < n; j++) {
         hg[j].start = starts + base[j];
         hg[j].items = items + ibase[j];
+        hg[j].keys = keys + ibase[j];
         hg[j].sbase = (uint32_t)ibase[j];
+        hg[j].sorted = sorted ? 1u : 0u;
     }
     GK(hipMemcpyAsync(dgrids, hg.data(), n * sizeof(SunGrid), hipMemcpyHostToDevice, st));
     GK(hipStreamSynchronize(st));

@@ -26,16 +26,19 @@ struct EvRec {
 
 // Occlusion grids of the collimated-source directions of one solve (rt_grid.hip).
 struct GridSet {
-    Workspace small, cells, items;
+    Workspace small, cells, items, keys, sortbuf;
     SunGrid *dev = nullptr;  // [n] device headers
     uint32_t n = 0;
     size_t entries = 0;      // list entries over all directions
     size_t cells_total = 0;
-    size_t cap = 0;          // byte limit for cells + items (0 = none); above it the BVH is traversed
+    size_t cap = 0;          // byte limit for cells + lists (0 = none); above it the BVH is traversed
     float density = 2.f;     // grid cells per traced primitive
     // bytes a server reserves for the grids of one solve of `njobs` directions over P primitives
-    static size_t budget(uint32_t P, uint32_t njobs) { return (size_t)96 * P * njobs + (1u << 20); }
-    void release() { small.release(); cells.release(); items.release(); dev = nullptr; n = 0; }
+    static size_t budget(uint32_t P, uint32_t njobs) { return (size_t)320 * P * njobs + (1u << 20); }
+    void release() {
+        small.release(); cells.release(); items.release(); keys.release(); sortbuf.release();
+        dev = nullptr; n = 0;
+    }
 };
 // Builds grids for n directions (host [n][3]).  Returns 0 on success, 1 when grids do not apply (periodic
 // scene, degenerate input, too large) and the BVH is to be traversed instead, -1 on a HIP error.

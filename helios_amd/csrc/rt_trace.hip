@@ -643,14 +643,23 @@ __device__ __forceinline__ void grid_cell(const SunGrid &G, v3f o, uint32_t &b, 
 
 // Candidates p and p + 1 (when p + 1 < e) of a cell list: is either accepted as an occluder?  Both
 // candidates' loads are issued before either test (the list entry -> record chain is latency-bound); a slot
-// that is out of the list or unfilled loads record 0 and is ignored.
+// that is out of the list or unfilled loads record 0 and is ignored.  A candidate whose depth key (an upper
+// bound of the depth of its points along the source direction, padded by 1e-5 of the scene's largest
+// coordinate) is below the ray origin's depth lies wholly behind the origin: no FP32 hit of it can have
+// t > 1e-4 (the FP32 errors of t, of the key and of the depth are a few ulps of that coordinate), so it is
+// skipped without loading its record.
+// Returns 1 when a candidate occludes the ray, 2 when the list is known to hold no further occluder (a
+// sorted list whose next key is below the origin's depth), else 0.
 template <typename C>
-__device__ __forceinline__ bool grid_pair(const uint32_t *items, const float4 *recs, uint32_t nleaf, uint32_t p,
-                                          uint32_t e, v3f o, v3f d, uint32_t skip, C &cnt) {
+__device__ __forceinline__ int grid_pair(const uint32_t *items, const float *keys, bool sorted, float depth,
+                                         const float4 *recs, uint32_t nleaf, uint32_t p, uint32_t e, v3f o, v3f d,
+                                         uint32_t skip, C &cnt) {
     const bool two = p + 1 < e;
     const uint32_t k0 = items[p];
     const uint32_t k1 = two ? items[p + 1] : RT_MISS;
-    const bool v0 = k0 < nleaf, v1 = k1 < nleaf;
+    const float key0 = keys[p];
+    const float key1 = two ? keys[p + 1] : -INFINITY;
+    const bool v0 = k0 < nleaf && !(key0 < depth), v1 = k1 < nleaf && !(key1 < depth);
     const float4 *q0 = recs + (size_t)(v0 ? k0 : 0u) * 4, *q1 = recs + (size_t)(v1 ? k1 : 0u) * 4;
     const float4 a0 = ldc(q0), a1 = ldc(q0 + 1), a2 = ldc(q0 + 2), a3 = ldc(q0 + 3);
     const float4 b0 = ldc(q1), b1 = ldc(q1 + 1), b2 = ldc(q1 + 2), b3 = ldc(q1 + 3);
@@ -661,7 +670,8 @@ __device__ __forceinline__ bool grid_pair(const uint32_t *items, const float4 *r
     const bool h1 = v1 && isect_rec<false>(b0, b1, b2, b3, (tb & RT_PATCH_BIT) != 0, o, d, t1) &&
                     (tb & ~RT_PATCH_BIT) != skip && t1 > 1e-4f && t1 <= 1e30f;
     cnt.prims += (v0 ? 1u : 0u) + ((v1 && !h0) ? 1u : 0u);  // the tests a one-at-a-time scan makes
-    return h0 || h1;
+    if (h0 || h1) return 1;
+    return (sorted && (!(key0 >= depth) || (two && !(key1 >= depth)))) ? 2 : 0;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -725,15 +735,20 @@ __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict
                         // and the summation grouping differ).
                         const SunGrid &G = *grid;
                         const uint32_t *items = G.items;
+                        const float *keys = G.keys;
+                        const bool sorted = G.sorted != 0;
                         const uint32_t nleaf = ldu(&sc.nleaf);
+                        const float gw0 = G.w[0], gw1 = G.w[1], gw2 = G.w[2];
                         uint32_t r = lane, pc = 0, pe = 0;
                         v3f o = mk3(0.f, 0.f, 0.f);
+                        float depth = 0.f;  // origin depth along the source direction
                         bool act = r < Nrays;
                         if (act) {
                             float spx, spy;
                             direct_sample(i, r % d, r / d, d, seed, patch, spx, spy);
                             o = xform3(tm, spx, spy, 0.0f);
                             grid_cell(G, o, pc, pe);
+                            depth = o.x * gw0 + o.y * gw1 + o.z * gw2;
                             cnt.nodes++;
                             nray++;
                         }
@@ -742,14 +757,12 @@ __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict
                             const uint32_t floor_lanes = qnext < Nrays ? RT_DIRECT_REFILL : 1u;
                             while (true) {
                                 if (act) {
-                                    if (pc >= pe) {
-                                        acc.add(strength);
-                                        act = false;
-                                    } else if (grid_pair(items, recs, nleaf, pc, pe, o, dir, i, cnt)) {
-                                        act = false;
-                                    } else {
-                                        pc += 2;
-                                    }
+                                    const int res = pc >= pe ? 2
+                                                             : grid_pair(items, keys, sorted, depth, recs, nleaf, pc, pe,
+                                                                         o, dir, i, cnt);
+                                    if (res == 2) acc.add(strength);  // lit
+                                    if (res != 0) act = false;
+                                    else pc += 2;
                                 }
                                 if ((uint32_t)__popcll(__ballot(act)) < floor_lanes) break;
                             }
@@ -767,6 +780,7 @@ __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict
                                 direct_sample(i, r % d, r / d, d, seed, patch, spx, spy);
                                 o = xform3(tm, spx, spy, 0.0f);
                                 grid_cell(G, o, pc, pe);
+                                depth = o.x * gw0 + o.y * gw1 + o.z * gw2;
                                 cnt.nodes++;
                                 nray++;
                                 act = true;

@@ -50,13 +50,17 @@ struct DevScene {
 // origin's projection.
 struct SunGrid {
     float u[3], v[3];      // orthonormal basis of the projection plane
+    float w[3];            // unit direction towards the source (depth axis of the keys)
     float g0u, g0v;        // grid origin
     float inv_cs;          // 1 / cell size
     uint32_t nx, ny;
+    uint32_t sorted;       // 1: each cell's list is in descending key order (a scan may stop at the first
+                           // key below the ray origin's depth)
     uint32_t sbase;        // start[] values count from sbase (modulo 2^32): cell c lists
                            // items[start[c] - sbase, start[c + 1] - sbase)
     const uint32_t *start; // [nx*ny + 1]
     const uint32_t *items; // this direction's list
+    const float *keys;     // [entry] the primitive's largest depth along w (max over its vertices of v.w)
 };
 
 // one direct trace job: one scene, one collimated source
