@@ -120,90 +120,38 @@ __global__ void k_grid_count(uint32_t nleaf, const float4 *__restrict__ rec, flo
         for (int x = x0; x <= x1; x++) atomicAdd(&c[(size_t)y * G.nx + x], 1u);
 }
 
-// Composite sort key (direction, depth key) of every (direction, primitive) pair: sorting them puts each
-// direction's primitives in ascending key order, the order k_grid_fill takes them in.
-__global__ void k_grid_sortkeys(uint32_t nleaf, const float4 *__restrict__ rec, float smax,
-                                const SunGrid *__restrict__ grids, unsigned long long *__restrict__ skey,
-                                uint32_t *__restrict__ sval) {
-    const SunGrid G = grids[blockIdx.y];
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nleaf) return;
-    float u0, u1, v0, v1, key;
-    proj_rect(rec, k, G, smax, u0, u1, v0, v1, key);
-    const size_t g = (size_t)blockIdx.y * nleaf + k;
-    skey[g] = ((unsigned long long)blockIdx.y << 32) | f2o_g(key);
-    sval[g] = k;
-}
-
-// Insertion sort of each cell's list into descending key order (ties: ascending leaf slot), so the list
-// order is deterministic and a scan can stop at the first key below the ray origin's depth.  A workgroup
-// takes RT_GRID_SORT_CELLS consecutive cells, whose lists are contiguous: when they hold at most
-// RT_GRID_SORT_LDS entries it stages them in LDS and each thread sorts its cell there; otherwise each thread
-// sorts in global memory.
-#define RT_GRID_SORT_LDS 8192
-#ifndef RT_GRID_PRESORT
-#define RT_GRID_PRESORT 1
-#endif
-#define RT_GRID_SORT_CELLS 128
-__device__ __forceinline__ bool grid_before(float ka, uint32_t ia, float kb, uint32_t ib) {
-    return ka > kb || (ka == kb && ia <= ib);
-}
-
-template <typename K, typename I>
-__device__ __forceinline__ void grid_isort(K *keys, I *items, uint32_t b, uint32_t e) {
-    for (uint32_t x = b + 1; x < e; x++) {
-        const float kx = keys[x];
-        const uint32_t ix = items[x];
-        uint32_t y = x;
-        while (y > b) {
-            const float ky = keys[y - 1];
-            const uint32_t iy = items[y - 1];
-            if (grid_before(ky, iy, kx, ix)) break;
-            keys[y] = ky;
-            items[y] = iy;
-            y--;
-        }
-        keys[y] = kx;
-        items[y] = ix;
-    }
-}
-
-__global__ __launch_bounds__(RT_GRID_SORT_CELLS) void k_grid_sort(const SunGrid *__restrict__ grids,
-                                                   const uint32_t *__restrict__ cell_base,
-                                                   const unsigned long long *__restrict__ item_base,
-                                                   const uint32_t *__restrict__ start, uint32_t *__restrict__ items,
-                                                   float *__restrict__ keys) {
-    __shared__ float lk[RT_GRID_SORT_LDS];
-    __shared__ uint32_t li[RT_GRID_SORT_LDS];
-    const SunGrid G = grids[blockIdx.y];
-    const uint32_t ncell = G.nx * G.ny;
-    const uint32_t c0 = blockIdx.x * RT_GRID_SORT_CELLS;
-    if (c0 >= ncell) return;  // whole workgroup
-    const uint32_t c1 = min(c0 + RT_GRID_SORT_CELLS, ncell);
-    const size_t cb = cell_base[blockIdx.y];
+// Each cell's list in descending key order (ties: ascending leaf slot), so the list order is deterministic
+// and a scan can stop at the first key below the ray origin's depth.  One thread per entry counts the entries of its cell that precede it and
+// moves the entry to that rank: the work is the sum of squared list lengths, spread over all entries.
+__global__ void k_grid_rank(const SunGrid *__restrict__ grids, const uint32_t *__restrict__ cell_base,
+                            const unsigned long long *__restrict__ item_base,
+                            const unsigned long long *__restrict__ item_count, const uint32_t *__restrict__ start,
+                            const uint32_t *__restrict__ t_items, const float *__restrict__ t_keys,
+                            const uint32_t *__restrict__ t_cell, uint32_t *__restrict__ items,
+                            float *__restrict__ keys) {
+    const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= item_count[blockIdx.y]) return;
     const unsigned long long ib = item_base[blockIdx.y];
     const uint32_t sb = (uint32_t)ib;
-    const uint32_t s0 = start[cb + c0] - sb, s1 = start[cb + c1] - sb;  // this workgroup's entries
-    const uint32_t c = c0 + threadIdx.x;
-    uint32_t b = 0, e = 0;
-    if (c < c1) {
-        b = start[cb + c] - sb;
-        e = start[cb + c + 1] - sb;
+    const size_t e0 = ib + x;
+    const uint32_t cl = t_cell[e0];
+    // a slot the fill left empty (none when the fill matches the count, as it does) has no cell: it is not
+    // placed, and its NaN key never counts as preceding anything; the final list keeps its trailing slots
+    // as "no primitive" with a NaN key, which ends a scan
+    if (cl >= grids[blockIdx.y].nx * grids[blockIdx.y].ny) return;
+    const size_t cc = cell_base[blockIdx.y] + cl;
+    const uint32_t b = start[cc] - sb, e = start[cc + 1] - sb;
+    const float kx = t_keys[e0];
+    const uint32_t ix = t_items[e0];
+    uint32_t rank = 0;
+    for (uint32_t y = b; y < e; y++) {
+        const float ky = t_keys[ib + y];
+        const uint32_t iy = t_items[ib + y];
+        rank += (ky > kx || (ky == kx && (iy < ix || (iy == ix && y < (uint32_t)x)))) ? 1u : 0u;
     }
-    if (s1 - s0 <= RT_GRID_SORT_LDS) {
-        for (uint32_t x = threadIdx.x; x < s1 - s0; x += RT_GRID_SORT_CELLS) {
-            lk[x] = keys[ib + s0 + x];
-            li[x] = items[ib + s0 + x];
-        }
-        __syncthreads();
-        if (c < c1) grid_isort(lk, li, b - s0, e - s0);
-        __syncthreads();
-        for (uint32_t x = threadIdx.x; x < s1 - s0; x += RT_GRID_SORT_CELLS) {
-            keys[ib + s0 + x] = lk[x];
-            items[ib + s0 + x] = li[x];
-        }
-    } else if (c < c1) {
-        grid_isort(keys + ib, items + ib, b, e);
+    if (b + rank < e) {
+        items[ib + b + rank] = ix;
+        keys[ib + b + rank] = kx;
     }
 }
 
@@ -215,11 +163,10 @@ __global__ void k_grid_fill(uint32_t nleaf, const float4 *__restrict__ rec, floa
                             const unsigned long long *__restrict__ item_base,
                             const unsigned long long *__restrict__ item_count, const uint32_t *__restrict__ start,
                             uint32_t *__restrict__ counts, uint32_t *__restrict__ items, float *__restrict__ keys,
-                            const uint32_t *__restrict__ order) {
+                            uint32_t *__restrict__ cells) {
     const SunGrid G = grids[blockIdx.y];
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nleaf) return;
-    if (order) k = order[(size_t)blockIdx.y * nleaf + k];  // this direction's primitives by ascending key
     float u0, u1, v0, v1, key;
     proj_rect(rec, k, G, smax, u0, u1, v0, v1, key);
     const int x0 = cell_of(u0, G.g0u, G.inv_cs, G.nx), x1 = cell_of(u1, G.g0u, G.inv_cs, G.nx);
@@ -238,6 +185,7 @@ __global__ void k_grid_fill(uint32_t nleaf, const float4 *__restrict__ rec, floa
             if (at < ni) {
                 items[ib + at] = k;
                 keys[ib + at] = key;
+                cells[ib + at] = (uint32_t)(c - base);
             }
         }
 }
@@ -350,49 +298,34 @@ int build_sun_grids(GridSet &gs, const Scene &S, const float *dirs, uint32_t n, 
         ibase[j] = total;
         total += ent[j];
     }
-    if (gs.cap && cell_bytes + total * 8 + 512 > gs.cap) return 1;
+    if (gs.cap && cell_bytes + total * 20 + 1024 > gs.cap) return 1;  // lists (8 B) + sort scratch (12 B)
     GK(hipcub::DeviceScan::ExclusiveSum(tmp, scan_bytes, counts, starts, (int)(total_cells + 1), st));
     if (gs.items.reserve((size_t)total * 4 + 256) || gs.keys.reserve((size_t)total * 4 + 256)) return -1;
     uint32_t *items = (uint32_t *)gs.items.base;
     float *keys = (float *)gs.keys.base;
-    // an unfilled slot reads as "no primitive" with a huge key (sorted first, never ends a scan)
+    // the fill writes unsorted lists (entry, key, cell) to scratch; k_grid_rank moves each entry to its
+    // place.  Slots nothing is written to read as "no primitive" with a NaN key.
+    const size_t tb = ((size_t)total * 4 + 255) & ~(size_t)255;
+    if (gs.sortbuf.reserve(3 * tb + 256)) return -1;
+    uint32_t *t_items = (uint32_t *)gs.sortbuf.base;
+    float *t_keys = (float *)((char *)gs.sortbuf.base + tb);
+    uint32_t *t_cell = (uint32_t *)((char *)gs.sortbuf.base + 2 * tb);
+    GK(hipMemsetAsync(gs.sortbuf.base, 0xFF, 3 * tb, st));
     GK(hipMemsetAsync(items, 0xFF, (size_t)total * 4, st));
-    GK(hipMemsetAsync(keys, 0x7F, (size_t)total * 4, st));
+    GK(hipMemsetAsync(keys, 0xFF, (size_t)total * 4, st));
     GK(hipMemcpyAsync(dibase, ibase.data(), (size_t)n * 8, hipMemcpyHostToDevice, st));
-    // primitives of each direction in ascending key order (one radix sort over all directions), so the
-    // cell lists come out closer to sorted and the per-cell sort below has less to move; left out when the
-    // pairs exceed the radix sort's int count
-    const size_t npairs = (size_t)n * nleaf;
-    const uint32_t *order = nullptr;
-    if (RT_GRID_PRESORT && npairs < (1ull << 31)) {
-        int jbits = 1;
-        while ((1ull << jbits) < n) jbits++;
-        size_t sort_tmp = 0;
-        GK(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_tmp, (const unsigned long long *)nullptr,
-                                              (unsigned long long *)nullptr, (const uint32_t *)nullptr,
-                                              (uint32_t *)nullptr, (int)npairs, 0, 32 + jbits, st));
-        const size_t sb = ((npairs * 8 + 255) & ~(size_t)255);
-        const size_t vb = ((npairs * 4 + 255) & ~(size_t)255);
-        if (gs.sortbuf.reserve(2 * sb + 2 * vb + sort_tmp + 256)) return -1;
-        char *q = (char *)gs.sortbuf.base;
-        unsigned long long *k_in = (unsigned long long *)q, *k_out = (unsigned long long *)(q + sb);
-        uint32_t *v_in = (uint32_t *)(q + 2 * sb), *v_out = (uint32_t *)(q + 2 * sb + vb);
-        void *stmp = q + 2 * sb + 2 * vb;
-        k_grid_sortkeys<<<grid, 256, 0, st>>>(nleaf, S.h.rec, S.h.smax, dgrids, k_in, v_in);
-        GK(hipGetLastError());
-        GK(hipcub::DeviceRadixSort::SortPairs(stmp, sort_tmp, k_in, k_out, v_in, v_out, (int)npairs, 0, 32 + jbits,
-                                              st));
-        order = v_out;
-    }
     k_grid_fill<<<grid, 256, 0, st>>>(nleaf, S.h.rec, S.h.smax, dgrids, dbase, dibase, dentries, starts, counts,
-                                      items, keys, order);
+                                      t_items, t_keys, t_cell);
     GK(hipGetLastError());
     {
-        uint32_t maxcells = 0;
-        for (uint32_t j = 0; j < n; j++) maxcells = std::max(maxcells, hg[j].nx * hg[j].ny);
-        k_grid_sort<<<dim3((maxcells + RT_GRID_SORT_CELLS - 1) / RT_GRID_SORT_CELLS, n), RT_GRID_SORT_CELLS, 0, st>>>(
-            dgrids, dbase, dibase, starts, items, keys);
-        GK(hipGetLastError());
+        unsigned long long maxent = 0;
+        for (uint32_t j = 0; j < n; j++) maxent = std::max(maxent, ent[j]);
+        if (maxent) {
+            k_grid_rank<<<dim3((uint32_t)((maxent + 255) / 256), n), 256, 0, st>>>(dgrids, dbase, dibase, dentries,
+                                                                                   starts, t_items, t_keys, t_cell,
+                                                                                   items, keys);
+            GK(hipGetLastError());
+        }
     }
     const bool sorted = true;
     for (uint32_t j = 0; j < n; j++) {

@@ -34,7 +34,7 @@ struct GridSet {
     size_t cap = 0;          // byte limit for cells + lists (0 = none); above it the BVH is traversed
     float density = 2.f;     // grid cells per traced primitive
     // bytes a server reserves for the grids of one solve of `njobs` directions over P primitives
-    static size_t budget(uint32_t P, uint32_t njobs) { return (size_t)320 * P * njobs + (1u << 20); }
+    static size_t budget(uint32_t P, uint32_t njobs) { return (size_t)512 * P * njobs + (1u << 20); }
     void release() {
         small.release(); cells.release(); items.release(); keys.release(); sortbuf.release();
         dev = nullptr; n = 0;
