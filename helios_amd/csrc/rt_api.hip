@@ -131,10 +131,13 @@ int rt_cast_rays_soa_device(rt_ctx *ctx, const rt_scene *scene, const float *o, 
     hipStream_t st = ctx->sx.stream;
     if (ctx->sx.ws.reserve(64)) return -1;
     int *err = (int *)ctx->sx.ws.base;
+    unsigned long long *counter = (unsigned long long *)((char *)ctx->sx.ws.base + 8);
     CKR(hipMemsetAsync(err, 0, sizeof(int), st));
     hipEvent_t a = nullptr, b = nullptr;
     if (ctx->sx.timing) { CKR(hipEventCreate(&a)); CKR(hipEventCreate(&b)); CKR(hipEventRecord(a, st)); }
-    if (launch_cast(scene->s.d, scene->s.h.depth, n, o, d, max_dist, t, nrm, uuid, err, ctx->sx.stat_ptr(2), st)) return -1;
+    if (launch_cast(scene->s.d, scene->s.h.depth, n, o, d, max_dist, t, nrm, uuid, counter, err, ctx->sx.stat_ptr(2),
+                    ctx->sx.ncu, st))
+        return -1;
     if (b) CKR(hipEventRecord(b, st));
     int herr = 0;
     CKR(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, st));

@@ -51,8 +51,10 @@ int launch_diffuse(const DiffuseJob *job_dev, uint32_t total_items, uint32_t dep
                    unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st);
 int launch_gather(const GatherJob *jobs_dev, uint32_t njobs, uint32_t total_items, uint32_t cmax, uint32_t *counter,
                   const LaunchCfg &cfg, int ncu, hipStream_t st);
+// counter: device scratch for the work queue (zeroed by the launch)
 int launch_cast(const DevScene *sc, uint32_t depth, uint64_t n, const float *o, const float *d, float maxd, float *t,
-                float *nrm, uint32_t *uuid, int *err, unsigned long long *stats, hipStream_t st);
+                float *nrm, uint32_t *uuid, unsigned long long *counter, int *err, unsigned long long *stats,
+                int ncu, hipStream_t st);
 int launch_debug_rays(const DevScene *sc, uint32_t P, uint32_t n, uint32_t seed, float *out, hipStream_t st);
 int launch_debug_trace(const DevScene *sc, uint32_t n, const float *rays, const uint32_t *skip, float tmin, int brute,
                        uint32_t *hit, float *t, int *err, hipStream_t st);

@@ -1412,56 +1412,142 @@ __global__ __launch_bounds__(256) void k_gather_chan(const GatherJob *__restrict
 // ------------------------------------------------------------------------------------------------
 // castRaysSoA
 // ------------------------------------------------------------------------------------------------
+// One closest-hit leaf test with the CollisionDetection predicates (tmin 1e-6, nothing skipped), keeping the
+// hit's leaf slot for the normal, then pop.
+template <int STK, int LSTK, typename C>
+__device__ __forceinline__ bool lane_leaf_cd(const float4 *recs, LaneRay &L, uint32_t &slot, const lds_u32 *stk,
+                                             const uint32_t *ovf, uint32_t stride, C &cnt) {
+    cnt.prims++;
+    test_leaf<true>(recs, L.ni & ~RT_LEAF_BIT, L.o, L.d, 1e-6f, RT_MISS, L.best_t, L.best, slot);
+    return lane_pop<STK, LSTK>(L, stk, ovf, stride);
+}
+
+// Next 64-ray chunk of a wave (wave-uniform 64-bit work counter).
+__device__ __forceinline__ unsigned long long wave_chunk64(unsigned long long *counter) {
+    const uint32_t lane = threadIdx.x & 63;
+    unsigned long long f = atomicAdd(counter, lane == 0 ? 64ull : 0ull);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)f);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(f >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// castRaysSoA (CD/src/CollisionDetection_RayTracing.cpp:551-680, 980-1080): closest hit of each ray with
+// the CD predicates.  Persistent waves take 64-ray chunks from a counter and keep 64 rays in flight: a lane
+// whose trace ends writes its result and takes the chunk's next ray; traversal runs in node and leaf phases
+// as k_diffuse's.  Each ray is traced exactly as on its own.
 template <int STK, bool COUNT>
 __global__ __launch_bounds__(256) void k_cast(const DevScene *__restrict__ scp, uint64_t n, const float *__restrict__ O,
                                               const float *__restrict__ D, float maxd, float *__restrict__ T,
-                                              float *__restrict__ N, uint32_t *__restrict__ U, int *err,
-                                              unsigned long long *stats) {
-    __shared__ uint32_t stk_all[STK * 256];
-    uint32_t *stk = stk_all + threadIdx.x;
+                                              float *__restrict__ N, uint32_t *__restrict__ U,
+                                              unsigned long long *counter, int *err, unsigned long long *stats) {
+    constexpr int LSTK = STK < RT_DIFFUSE_LDS_STACK ? STK : RT_DIFFUSE_LDS_STACK;
+    __shared__ uint32_t stk_all[LSTK * 256];
+    uint32_t ovf[STK > LSTK ? RT_STACK - LSTK : 1];
+    lds_u32 *stk = (lds_u32 *)stk_all + threadIdx.x;
+    const uint32_t stride = 256;
     const DevScene &sc = *scp;
+    const float4 *nodes = ldu(&sc.node), *recs = ldu(&sc.rec);
+    const uint32_t *uuid = ldu(&sc.uuid);
+    const uint32_t nleaf = ldu(&sc.nleaf);
+    const float tstart = maxd > 0.f ? maxd : FLT_MAX;
     typename std::conditional<COUNT, TCnt, NoCnt>::type cnt{};
     uint32_t nray = 0;
-    for (uint64_t r = (uint64_t)blockIdx.x * 256 + threadIdx.x; r < n; r += (uint64_t)gridDim.x * 256) {
-        v3f o = mk3(O[r * 3], O[r * 3 + 1], O[r * 3 + 2]);
-        v3f d = mk3(D[r * 3], D[r * 3 + 1], D[r * 3 + 2]);
-        float tstart = maxd > 0.f ? maxd : FLT_MAX;
-        float t;
-        uint32_t slot;
-        nray++;
-        uint32_t h = trace_closest<true, STK>(sc, o, d, 1e-6f, RT_MISS, tstart, t, slot, stk, 256, err, cnt);
-        if (h == RT_MISS) { U[r] = RT_MISS; continue; }
-        U[r] = sc.uuid[h];
-        T[r] = t;
-        const float4 *rr = sc.rec + (size_t)slot * 4;
-        float4 r0 = rr[0], r1 = rr[1], r2 = rr[2], r3 = rr[3];
-        bool patch = (__float_as_uint(r0.w) & RT_PATCH_BIT) != 0;
-        // CD/src/CollisionDetection_RayTracing.cpp:1572-1585: normal of cross(v1-v0, v2-v0), facing the origin
-        v3f nn;
-        bool ok;
-        if (patch) {
-            nn = mk3(r3.x, r3.y, r3.z);
-            v3f raw = cross3(mk3(r1.x, r1.y, r1.z), add3(mk3(r1.x, r1.y, r1.z), mk3(r2.x, r2.y, r2.z)));
-            ok = rsqrt_cr(raw.x * raw.x + raw.y * raw.y + raw.z * raw.z) > 1e-8f;
-        } else {
-            nn = cross3(mk3(r1.x, r1.y, r1.z), mk3(r2.x, r2.y, r2.z));
-            float mag = rsqrt_cr(nn.x * nn.x + nn.y * nn.y + nn.z * nn.z);
-            ok = mag > 1e-8f;
-            if (ok) nn = mul3(nn, rdiv(1.0f, mag));
+    unsigned long long q_next = 0, q_end = 0, my_r = 0;
+    bool drained = false, busy = false, trav = false;
+    LaneRay L;
+    L.o = L.d = mk3(0.f, 0.f, 0.f);
+    lane_start(L, nleaf);
+    uint32_t slot = RT_MISS;
+    while (true) {
+        // finished lanes write their result
+        if (busy && !trav) {
+            const uint32_t h = L.best;
+            if (h == RT_MISS) {
+                U[my_r] = RT_MISS;
+            } else {
+                U[my_r] = uuid[h];
+                T[my_r] = L.best_t;
+                const float4 *rr = recs + (size_t)slot * 4;
+                const float4 r1 = rr[1], r2 = rr[2], r3 = rr[3];
+                const bool patch = (__float_as_uint(rr[0].w) & RT_PATCH_BIT) != 0;
+                // CD/src/CollisionDetection_RayTracing.cpp:1572-1585: normal of cross(v1-v0, v2-v0), facing
+                // the origin
+                v3f nn;
+                bool ok;
+                if (patch) {
+                    nn = mk3(r3.x, r3.y, r3.z);
+                    v3f raw = cross3(mk3(r1.x, r1.y, r1.z), add3(mk3(r1.x, r1.y, r1.z), mk3(r2.x, r2.y, r2.z)));
+                    ok = rsqrt_cr(raw.x * raw.x + raw.y * raw.y + raw.z * raw.z) > 1e-8f;
+                } else {
+                    nn = cross3(mk3(r1.x, r1.y, r1.z), mk3(r2.x, r2.y, r2.z));
+                    float mag = rsqrt_cr(nn.x * nn.x + nn.y * nn.y + nn.z * nn.z);
+                    ok = mag > 1e-8f;
+                    if (ok) nn = mul3(nn, rdiv(1.0f, mag));
+                }
+                if (ok) {
+                    v3f ip = add3(L.o, mul3(L.d, L.best_t));
+                    v3f to = sub3(L.o, ip);
+                    if (nn.x * to.x + nn.y * to.y + nn.z * to.z < 0) nn = mul3(nn, -1.0f);
+                } else {
+                    nn = nrm3(mk3(-L.d.x, -L.d.y, -L.d.z));
+                }
+                N[my_r * 3] = nn.x;
+                N[my_r * 3 + 1] = nn.y;
+                N[my_r * 3 + 2] = nn.z;
+            }
+            busy = false;
         }
-        if (ok) {
-            v3f ip = add3(o, mul3(d, t));
-            v3f to = sub3(o, ip);
-            if (nn.x * to.x + nn.y * to.y + nn.z * to.z < 0) nn = mul3(nn, -1.0f);
-        } else {
-            nn = nrm3(mk3(-d.x, -d.y, -d.z));
+        // idle lanes take the next rays of the wave's chunk
+        while (true) {
+            const uint64_t idle = __ballot(!busy);
+            if (idle == 0) break;
+            if (q_next == q_end) {
+                if (drained) break;
+                const unsigned long long f = wave_chunk64(counter);
+                if (f >= n) { drained = true; break; }
+                q_next = f;
+                q_end = f + 64 < n ? f + 64 : n;
+            }
+            const uint32_t rank =
+                __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+            const uint32_t left = (uint32_t)(q_end - q_next);
+            const uint32_t take = min((uint32_t)__popcll(idle), left);
+            if (!busy && rank < take) {
+                const unsigned long long r = q_next + rank;
+                L.o = mk3(O[r * 3], O[r * 3 + 1], O[r * 3 + 2]);
+                L.d = mk3(D[r * 3], D[r * 3 + 1], D[r * 3 + 2]);
+                lane_start(L, nleaf);
+                L.best_t = tstart;
+                slot = RT_MISS;
+                trav = nleaf > 1;
+                if (nleaf == 1) { cnt.prims++; test_leaf<true>(recs, 0, L.o, L.d, 1e-6f, RT_MISS, L.best_t, L.best, slot); }
+                my_r = r;
+                busy = true;
+                nray++;
+            }
+            q_next += take;
         }
-        N[r * 3] = nn.x;
-        N[r * 3 + 1] = nn.y;
-        N[r * 3 + 2] = nn.z;
+        if (__ballot(busy) == 0) break;  // queue empty and nothing in flight
+        const uint32_t floor_lanes = drained ? 1u : RT_REFILL_BELOW;
+        while (true) {
+            while (true) {
+                const bool inner = trav && !(L.ni & RT_LEAF_BIT);
+                const uint64_t mi = __ballot(inner);
+                if (mi == 0 || (uint32_t)__popcll(__ballot(trav && !inner)) >= RT_LEAF_BATCH) break;
+                if (inner && lane_node<STK, LSTK>(nodes, nleaf, L, stk, ovf, stride, err, cnt)) trav = false;
+            }
+            while (true) {
+                const bool leaf = trav && (L.ni & RT_LEAF_BIT);
+                if (__ballot(leaf) == 0) break;
+                if (leaf && lane_leaf_cd<STK, LSTK>(recs, L, slot, stk, ovf, stride, cnt)) trav = false;
+            }
+            if ((uint32_t)__popcll(__ballot(trav)) < floor_lanes) break;
+        }
     }
     add_stats(stats, cnt, nray);
 }
+
+__global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
 
 __global__ void k_zero_u32(uint32_t *p) { *p = 0u; }
 
@@ -1672,16 +1758,19 @@ int launch_gather(const GatherJob *jobs, uint32_t njobs, uint32_t total, uint32_
 }
 
 int launch_cast(const DevScene *sc, uint32_t depth, uint64_t n, const float *o, const float *d, float maxd, float *t,
-                float *nrm, uint32_t *uuid, int *err, unsigned long long *stats, hipStream_t st) {
+                float *nrm, uint32_t *uuid, unsigned long long *counter, int *err, unsigned long long *stats,
+                int ncu, hipStream_t st) {
     if (n == 0) return 0;
-    uint64_t need = (n + 255) / 256;
-    uint32_t g = (uint32_t)(need < 65536ull * 16 ? need : 65536ull * 16);
-    if (stats)
-        PICK_STK(depth, (k_cast<32, true><<<g, 256, 0, st>>>(sc, n, o, d, maxd, t, nrm, uuid, err, stats)),
-                 (k_cast<64, true><<<g, 256, 0, st>>>(sc, n, o, d, maxd, t, nrm, uuid, err, stats)));
-    else
-        PICK_STK(depth, (k_cast<32, false><<<g, 256, 0, st>>>(sc, n, o, d, maxd, t, nrm, uuid, err, stats)),
-                 (k_cast<64, false><<<g, 256, 0, st>>>(sc, n, o, d, maxd, t, nrm, uuid, err, stats)));
+    if (depth + 1 > RT_STACK) {
+        set_error("BVH depth " + std::to_string(depth) + " exceeds the traversal stack (" + std::to_string(RT_STACK) + ")");
+        return -1;
+    }
+    k_zero_u64<<<1, 1, 0, st>>>(counter);
+    const uint64_t chunks = (n + 63) / 64;
+    uint64_t g = std::min<uint64_t>((uint64_t)8 * ncu, (chunks + 3) / 4);
+    if (g < 1) g = 1;
+    if (stats) k_cast<RT_STACK, true><<<(uint32_t)g, 256, 0, st>>>(sc, n, o, d, maxd, t, nrm, uuid, counter, err, stats);
+    else k_cast<RT_STACK, false><<<(uint32_t)g, 256, 0, st>>>(sc, n, o, d, maxd, t, nrm, uuid, counter, err, stats);
     LCHECK();
     return 0;
 }
