@@ -321,6 +321,8 @@ def bench_config2(args):
 
 def bench_castrays(args):
     import ctypes as C
+    if args.calib_copy_mib:
+        calib_copy(args.calib_copy_mib)
     s = scenes.orchard(3, tiles=5, jitter=0.05)
     ctx = native.Context(0)
     g = ctx.scene(s.transforms, s.types, s.twosided)
@@ -361,8 +363,12 @@ def bench_castrays(args):
             "config": {"workload": f"config5 castRaysSoA: {n} rays (uniform in scene AABB, uniform directions) over "
                                    f"the 5x tiled orchard ({s.P} prims)"},
             "roofline": {"bound": "hbm", "kernel": "k_cast", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                         "bytes_per_ray": round(brays, 1)}}
+                         "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                         "traffic": pmc_traffic("k_cast", profile="pmc_traffic_config5_castrays.json"),
+                         "bytes_per_ray": round(brays, 1),
+                         "nodes_per_ray": round(counts["nodes"] / counts["rays"], 2),
+                         "prims_per_ray": round(counts["prims"] / counts["rays"], 2),
+                         "avg_launch_ms": round(kms / args.steps, 3)}}
 
 
 def bench_serve(args, ctx=None):
