@@ -132,8 +132,11 @@ void backend_release(BackendState &be) {
     if (be.mat) (void)hipFree(be.mat);
     if (be.flux) (void)hipFree(be.flux);
     if (be.rad) (void)hipFree(be.rad);
-    be.mat = be.flux = be.rad = nullptr;
-    be.mat_cap = be.flux_cap = be.rad_cap = 0;
+    if (be.cam) (void)hipFree(be.cam);
+    if (be.lab) (void)hipFree(be.lab);
+    be.mat = be.flux = be.rad = be.cam = be.lab = nullptr;
+    be.mat_cap = be.flux_cap = be.rad_cap = be.cam_cap = be.lab_cap = 0;
+    be.cam_pix = be.lab_pix = 0;
     be.ws.release();
 }
 
@@ -498,6 +501,248 @@ int rt_get_backend_info(rt_ctx *ctx, uint32_t *out) {
     out[1] = be.Bg;
     out[2] = be.Bl;
     out[3] = be.S;
+    return 0;
+}
+
+// ---- cameras ----------------------------------------------------------------------------------
+
+int rt_update_sky_model(rt_ctx *ctx, uint32_t nbands, const float *sky_radiance_params,
+                        const float *camera_sky_radiance, const float *sun_direction, const float *solar_disk_radiance,
+                        float solar_disk_cos_angle, const float *camera_diffuse_flux,
+                        const uint32_t *band_emission_flag) {
+    NN(ctx, "ctx");
+    NN(sun_direction, "sun_direction");
+    if (nbands) {
+        NN(sky_radiance_params, "sky_radiance_params");
+        NN(camera_sky_radiance, "camera_sky_radiance");
+        NN(solar_disk_radiance, "solar_disk_radiance");
+    }
+    if (nbands > 256) { set_error("rt_update_sky_model: more than 256 bands"); return -1; }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    BackendState &be = ctx->be;
+    be.sky_B = nbands;
+    be.sky_params.assign(sky_radiance_params, sky_radiance_params + (size_t)nbands * 4);
+    be.sky_rad.assign(camera_sky_radiance, camera_sky_radiance + nbands);
+    be.solar_rad.assign(solar_disk_radiance, solar_disk_radiance + nbands);
+    be.cam_dflux.assign(nbands, 0.0f);
+    be.emis_flag.assign(nbands, 0u);
+    if (camera_diffuse_flux) be.cam_dflux.assign(camera_diffuse_flux, camera_diffuse_flux + nbands);
+    if (band_emission_flag) be.emis_flag.assign(band_emission_flag, band_emission_flag + nbands);
+    for (int k = 0; k < 3; k++) be.sun_dir[k] = sun_direction[k];
+    be.solar_cos = solar_disk_cos_angle;
+    be.has_sky = true;
+    return 0;
+}
+
+namespace {
+
+// Checks a camera launch and fills the kernel arguments that do not depend on the image buffers.
+int camera_setup(rt_ctx *ctx, const rt_camera_params *p, CameraLaunch &c, const char *who) {
+    BackendState &be = ctx->be;
+    if (need_scene(be, who)) return -1;
+    if (p->specular_reflection_enabled) {
+        set_error(std::string(who) + ": specular reflection is not supported by this backend");
+        return -1;
+    }
+    const uint64_t rx = p->resolution[0], ry = p->resolution[1], fx = p->resolution_full[0], fy = p->resolution_full[1];
+    if (fx == 0 || fy == 0 || fx * fy > (1ull << 31)) {
+        set_error(std::string(who) + ": full resolution must be 1..2^31 pixels");
+        return -1;
+    }
+    if ((uint64_t)p->pixel_offset[0] + rx > fx || (uint64_t)p->pixel_offset[1] + ry > fy) {
+        set_error(std::string(who) + ": tile at offset (" + std::to_string(p->pixel_offset[0]) + ", " +
+                  std::to_string(p->pixel_offset[1]) + ") of " + std::to_string(rx) + "x" + std::to_string(ry) +
+                  " exceeds the full resolution " + std::to_string(fx) + "x" + std::to_string(fy));
+        return -1;
+    }
+    if (!(p->viewplane_length > 0.0f) || !(p->fov_aspect > 0.0f)) {
+        set_error(std::string(who) + ": viewplane_length and fov_aspect must be positive");
+        return -1;
+    }
+    c = CameraLaunch{};
+    c.sc = be.scene.d;
+    for (int k = 0; k < 3; k++) c.pos[k] = p->position[k];
+    // camera_raygen.comp:258-259: theta = elevation - pi/2, phi = pi/2 - azimuth; sin/cos evaluated in
+    // double from the float angles and rounded.
+    const double th = (double)(-0.5f * RT_PI_F + p->direction[0]);
+    const double ph = (double)(0.5f * RT_PI_F - p->direction[1]);
+    c.st = (float)std::sin(th); c.ct = (float)std::cos(th);
+    c.sp = (float)std::sin(ph); c.cp = (float)std::cos(ph);
+    c.focal = p->focal_length;
+    c.lens = p->lens_diameter;
+    c.aspect = p->fov_aspect;
+    c.viewplane = p->viewplane_length;
+    c.res_x = p->resolution[0]; c.res_y = p->resolution[1];
+    c.full_x = p->resolution_full[0]; c.full_y = p->resolution_full[1];
+    c.off_x = p->pixel_offset[0]; c.off_y = p->pixel_offset[1];
+    c.spp = p->antialiasing_samples ? p->antialiasing_samples : 1u;
+    c.seed = p->random_seed;
+    return 0;
+}
+
+}  // namespace
+
+int rt_launch_camera_rays(rt_ctx *ctx, const rt_camera_params *p) {
+    NN(ctx, "ctx");
+    NN(p, "params");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    BackendState &be = ctx->be;
+    const char *who = "rt_launch_camera_rays";
+    CameraLaunch c;
+    if (camera_setup(ctx, p, c, who)) return -1;
+    if (be.Bl == 0 || !be.rad) {
+        set_error(std::string(who) + ": radiation buffers not sized (call rt_zero_radiation_buffers first)");
+        return -1;
+    }
+    const uint32_t B = be.Bl, S = be.S;
+    if (be.has_sky && be.sky_B != B) {
+        set_error(std::string(who) + ": sky model has " + std::to_string(be.sky_B) + " bands, launch has " +
+                  std::to_string(B));
+        return -1;
+    }
+    if (S && !be.has_fluxes) { set_error(std::string(who) + ": no source fluxes (call rt_upload_source_fluxes)"); return -1; }
+    CKR(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->sx.stream;
+    const uint64_t npix = (uint64_t)c.full_x * c.full_y;
+    // camera_radiation at full resolution, zeroed by the first tile (VulkanComputeBackend.cpp:1469-1486)
+    if (grow(be.cam, be.cam_cap, npix * B * sizeof(float))) return -1;
+    const bool first = p->pixel_offset[0] == 0 && p->pixel_offset[1] == 0;
+    if (first || be.cam_pix != npix || be.cam_B != B) {
+        CKR(hipMemsetAsync(be.cam, 0, npix * B * sizeof(float), st));
+        be.cam_pix = npix;
+        be.cam_B = B;
+    }
+    // per-launch tables: sky model [B] and source shapes [S]
+    size_t bytes = 9 * 256 * (size_t)B + 32 * 256 + (size_t)S * 64 + 16 * 256;
+    if (be.ws.reserve(bytes)) return -1;
+    Carve cv{(char *)be.ws.base};
+    float *d_sky = cv.take<float>((size_t)B * 4);
+    float *d_srad = cv.take<float>(B), *d_solar = cv.take<float>(B), *d_dflux = cv.take<float>(B);
+    uint32_t *d_emis = cv.take<uint32_t>(B);
+    uint32_t *d_type = cv.take<uint32_t>(S ? S : 1);
+    float *d_width = cv.take<float>(2 * (size_t)(S ? S : 1)), *d_rot = cv.take<float>(9 * (size_t)(S ? S : 1));
+    int *err = cv.take<int>(64);
+    std::vector<float> hsky(4 * (size_t)B, 0.f), hsrad(B, 0.f), hsolar(B, 0.f), hdflux(B, 0.f);
+    std::vector<uint32_t> hemis(B, 0u), htype(S);
+    std::vector<float> hwidth(2 * (size_t)S), hrot(9 * (size_t)S);
+    if (be.has_sky) {
+        hsky = be.sky_params; hsrad = be.sky_rad; hsolar = be.solar_rad; hdflux = be.cam_dflux; hemis = be.emis_flag;
+    }
+    for (uint32_t s = 0; s < S; s++) {
+        htype[s] = be.shapes[s].type;
+        hwidth[2 * s] = be.shapes[s].width[0]; hwidth[2 * s + 1] = be.shapes[s].width[1];
+        std::memcpy(&hrot[9 * (size_t)s], be.shapes[s].rot, 9 * sizeof(float));
+    }
+    CKR(hipMemcpyAsync(d_sky, hsky.data(), hsky.size() * 4, hipMemcpyHostToDevice, st));
+    CKR(hipMemcpyAsync(d_srad, hsrad.data(), B * 4, hipMemcpyHostToDevice, st));
+    CKR(hipMemcpyAsync(d_solar, hsolar.data(), B * 4, hipMemcpyHostToDevice, st));
+    CKR(hipMemcpyAsync(d_dflux, hdflux.data(), B * 4, hipMemcpyHostToDevice, st));
+    CKR(hipMemcpyAsync(d_emis, hemis.data(), B * 4, hipMemcpyHostToDevice, st));
+    if (S) {
+        CKR(hipMemcpyAsync(d_type, htype.data(), S * 4, hipMemcpyHostToDevice, st));
+        CKR(hipMemcpyAsync(d_width, hwidth.data(), hwidth.size() * 4, hipMemcpyHostToDevice, st));
+        CKR(hipMemcpyAsync(d_rot, hrot.data(), hrot.size() * 4, hipMemcpyHostToDevice, st));
+    }
+    CKR(hipMemsetAsync(err, 0, sizeof(int), st));
+    c.B = B;
+    c.S = S;
+    c.out_top = be.rbuf(3);
+    c.out_bot = be.rbuf(4);
+    c.flux = (const float *)be.flux;
+    c.src_pos = nullptr;
+    c.src_type = d_type;
+    c.src_width = d_width;
+    c.src_rot = d_rot;
+    c.sky_params = d_sky;
+    c.sky_rad = d_srad;
+    c.solar_rad = d_solar;
+    c.cam_dflux = d_dflux;
+    c.emis = d_emis;
+    const float *sun = be.has_sky ? be.sun_dir : nullptr;
+    for (int k = 0; k < 3; k++) c.sun[k] = sun ? sun[k] : (k == 2 ? 1.f : 0.f);
+    c.solar_cos = be.has_sky ? be.solar_cos : 2.0f;
+    c.pixels = (float *)be.cam;
+    // source positions travel with the shapes table
+    float *d_pos = cv.take<float>(3 * (size_t)(S ? S : 1));
+    if (cv.off > be.ws.cap) { set_error(std::string(who) + ": workspace carve overflow"); return -1; }
+    if (S) CKR(hipMemcpyAsync(d_pos, be.src_dir.data(), (size_t)S * 12, hipMemcpyHostToDevice, st));
+    c.src_pos = d_pos;
+    CKR(hipStreamSynchronize(st));
+    if (launch_camera(c, be.scene.h.depth, false, err, st)) return -1;
+    return check_err(ctx, err);
+}
+
+int rt_zero_camera_pixel_buffers(rt_ctx *ctx, uint32_t width, uint32_t height) {
+    NN(ctx, "ctx");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    BackendState &be = ctx->be;
+    const uint64_t n = (uint64_t)width * height;
+    if (n == 0) return 0;
+    if (n > (1ull << 31)) { set_error("rt_zero_camera_pixel_buffers: more than 2^31 pixels"); return -1; }
+    CKR(hipSetDevice(ctx->device));
+    if (grow(be.lab, be.lab_cap, n * 8)) return -1;
+    CKR(hipMemsetAsync(be.lab, 0, n * 8, ctx->sx.stream));
+    CKR(hipStreamSynchronize(ctx->sx.stream));
+    be.lab_pix = n;
+    return 0;
+}
+
+int rt_launch_pixel_label_rays(rt_ctx *ctx, const rt_camera_params *p) {
+    NN(ctx, "ctx");
+    NN(p, "params");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    BackendState &be = ctx->be;
+    const char *who = "rt_launch_pixel_label_rays";
+    CameraLaunch c;
+    if (camera_setup(ctx, p, c, who)) return -1;
+    const uint64_t npix = (uint64_t)c.full_x * c.full_y;
+    if (be.lab_pix != npix) {
+        set_error(std::string(who) + ": label buffers hold " + std::to_string(be.lab_pix) + " pixels, the image has " +
+                  std::to_string(npix) + " (call rt_zero_camera_pixel_buffers with the full resolution)");
+        return -1;
+    }
+    CKR(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->sx.stream;
+    if (be.ws.reserve(256)) return -1;
+    int *err = (int *)be.ws.base;
+    CKR(hipMemsetAsync(err, 0, sizeof(int), st));
+    c.labels = (uint32_t *)be.lab;
+    c.depths = (float *)((uint32_t *)be.lab + npix);
+    if (launch_camera(c, be.scene.h.depth, true, err, st)) return -1;
+    return check_err(ctx, err);
+}
+
+int rt_get_camera_results(rt_ctx *ctx, float *pixel_data, uint32_t *pixel_labels, float *pixel_depths, uint32_t width,
+                          uint32_t height) {
+    NN(ctx, "ctx");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    BackendState &be = ctx->be;
+    const uint64_t n = (uint64_t)width * height;
+    if (n == 0) return 0;
+    CKR(hipSetDevice(ctx->device));
+    CKR(hipStreamSynchronize(ctx->sx.stream));
+    // camera_radiation is band-major on the device; the caller receives [pixel][band]
+    // (VulkanComputeBackend.cpp:1891-1911).  Buffers of another size read as zeros.
+    if (pixel_data) {
+        const uint32_t B = be.cam_B;
+        if (be.cam_pix == n && B && B == be.Bl) {
+            std::vector<float> bm(n * B);
+            CKR(hipMemcpy(bm.data(), be.cam, n * B * sizeof(float), hipMemcpyDeviceToHost));
+            for (uint32_t b = 0; b < B; b++)
+                for (uint64_t q = 0; q < n; q++) pixel_data[q * B + b] = bm[(size_t)b * n + q];
+        } else {
+            std::memset(pixel_data, 0, n * be.Bl * sizeof(float));
+        }
+    }
+    const bool lab = be.lab_pix == n;
+    if (pixel_labels) {
+        if (lab) CKR(hipMemcpy(pixel_labels, be.lab, n * 4, hipMemcpyDeviceToHost));
+        else std::memset(pixel_labels, 0, n * 4);
+    }
+    if (pixel_depths) {
+        if (lab) CKR(hipMemcpy(pixel_depths, (uint32_t *)be.lab + n, n * 4, hipMemcpyDeviceToHost));
+        else std::memset(pixel_depths, 0, n * 4);
+    }
     return 0;
 }
 

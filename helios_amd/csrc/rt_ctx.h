@@ -35,6 +35,20 @@ struct BackendState {
     void *rad = nullptr;         // in, st, sb, ot, ob, each [P][Bl]
     size_t rad_cap = 0;
     rt::Workspace ws;            // per-launch scratch (strength sums, hit words, band tables, jobs)
+    // camera state (updateSkyModel, launchCameraRays, launchPixelLabelRays, getCameraResults)
+    uint32_t sky_B = 0;
+    std::vector<float> sky_params, sky_rad, solar_rad, cam_dflux;  // [B][4], [B], [B], [B]
+    std::vector<uint32_t> emis_flag;                                 // [B]
+    float sun_dir[3] = {0.f, 0.f, 1.f};
+    float solar_cos = 0.f;
+    bool has_sky = false;
+    void *cam = nullptr;         // camera_radiation [B_launch][full pixels] (band-major)
+    size_t cam_cap = 0;
+    uint64_t cam_pix = 0;        // full pixels the radiation image holds (0: none yet)
+    uint32_t cam_B = 0;          // launch bands of that image
+    void *lab = nullptr;         // pixel labels [pixels] then depths [pixels]
+    size_t lab_cap = 0;
+    uint64_t lab_pix = 0;        // pixels the label/depth buffers hold
     float *rho() const { return (float *)mat; }
     float *tau() const { return (float *)mat + (size_t)P * Bg; }
     float *rbuf(int k) const { return (float *)rad + (size_t)k * P * Bl; }  // 0 in 1 st 2 sb 3 ot 4 ob

@@ -261,6 +261,81 @@ RT_HD bool source_ray(uint32_t type, const float *pos, const float *width, const
     return true;
 }
 
+// ---- cameras (camera_raygen.comp:204-310, pixel_label_raygen.comp:80-115) ----
+// camera_rotate_point (transforms.glsl:66-89): Rz(phi) * Ry(theta) * p
+RT_HD v3f camera_rotate(v3f q, float st, float ct, float sp, float cp) {
+    return mk3((cp * ct) * q.x - sp * q.y + (cp * st) * q.z,
+               (sp * ct) * q.x + cp * q.y + (sp * st) * q.z,
+               -st * q.x + ct * q.z);
+}
+
+// Camera ray through pixel (ii, jj) of the full image at sub-pixel offset (rx, ry); (lx, ly) is the lens
+// point in units of the lens radius (0, 0 for a pinhole or the pixel-label rays).
+RT_HD void camera_ray(float px, float py, float pz, float st, float ct, float sp, float cp, float focal, float lens,
+                      float aspect, float viewplane, uint32_t ii, uint32_t jj, uint32_t full_x, uint32_t full_y,
+                      float rx, float ry, float lx, float ly, v3f &o, v3f &d) {
+    const float mult = rdiv(1.0f, aspect);
+    const float spy = -0.5f + rdiv((float)ii + rx, (float)full_x);
+    const float spz = (0.5f - rdiv((float)jj + ry, (float)full_y)) * mult;
+    const v3f p = mk3(focal, rdiv(spy, viewplane) * focal, rdiv(spz, viewplane) * focal);
+    const v3f org = mk3(0.0f, (0.5f * lx) * lens, (0.5f * ly) * lens);
+    const v3f dir = sub3(p, org);
+    o = add3(camera_rotate(org, st, ct, sp, cp), mk3(px, py, pz));
+    d = nrm3(camera_rotate(dir, st, ct, sp, cp));
+}
+
+// Concentric square-to-disk map of two uniform numbers (direct_raygen.comp:227-266): a point of the unit disk.
+RT_HD void unit_disk(float Rx, float Ry, float &dx, float &dy) {
+    float x = -1.0f + 2.0f * Rx, y = -1.0f + 2.0f * Ry;
+    float r, k;
+    if (x > -y) {
+        if (x > y) { r = x; k = rdiv(y, x); }
+        else { r = y; k = 2.0f - rdiv(x, y); }
+    } else {
+        if (x < y) { r = -x; k = 4.0f + rdiv(y, x); }
+        else { r = -y; k = (y != 0.0f) ? 6.0f - rdiv(x, y) : 0.0f; }
+    }
+    float sk, ck;
+    sincos_2pi(k * 0.125f, sk, ck);
+    dx = r * ck;
+    dy = r * sk;
+}
+
+// sphere_source_visible (camera_raygen.comp:131-143)
+RT_HD bool cam_sphere_visible(v3f o, v3f d, v3f c, float radius, float tmax) {
+    const v3f oc = sub3(o, c);
+    const float b = dot3(oc, d);
+    const float cc = dot3(oc, oc) - radius * radius;
+    const float disc = b * b - cc;
+    if (!(disc >= 0.0f)) return false;
+    const float ts = -b - rsqrt_cr(disc);
+    return ts > 0.0f && ts < tmax;
+}
+
+// rectangle_source_visible / disk_source_visible (camera_raygen.comp:146-203); the source normal is the
+// third row of the rotation matrix (m[6..8]) as those shaders read it.  disk: width[0] = radius.
+RT_HD bool cam_planar_visible(bool disk, v3f o, v3f d, v3f c, const float *width, const float *m, float tmax,
+                              float &cos_angle) {
+    const v3f nrm = mk3(m[6], m[7], m[8]);
+    const float denom = dot3(d, nrm);
+    if (denom >= -1e-6f) return false;
+    const v3f oc = sub3(c, o);
+    const float t = rdiv(dot3(oc, nrm), denom);
+    if (t <= 0.0f || t >= tmax) return false;
+    const v3f h = add3(o, mul3(d, t));
+    const v3f off = sub3(h, c);
+    bool in;
+    if (disk) {
+        in = dot3(off, off) <= width[0] * width[0];
+    } else {
+        const float lx = m[0] * off.x + m[3] * off.y + m[6] * off.z;
+        const float ly = m[1] * off.x + m[4] * off.y + m[7] * off.z;
+        in = fabsf(lx) <= width[0] * 0.5f && fabsf(ly) <= width[1] * 0.5f;
+    }
+    if (in) cos_angle = -denom;
+    return in;
+}
+
 // Diffuse-pass sample point and local hemisphere direction (diffuse_raygen.comp:375-482)
 RT_HD void diffuse_sample(uint32_t prim, uint32_t lx, uint32_t ly, uint32_t n, uint32_t seed, bool patch, float &spx,
                           float &spy, v3f &dl) {

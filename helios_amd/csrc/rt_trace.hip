@@ -1547,6 +1547,144 @@ __global__ __launch_bounds__(256) void k_cast(const DevScene *__restrict__ scp, 
     add_stats(stats, cnt, nray);
 }
 
+// ------------------------------------------------------------------------------------------------
+// cameras (camera_raygen.comp, pixel_label_raygen.comp)
+// ------------------------------------------------------------------------------------------------
+// Closest hit of a camera ray with up to 10 periodic wraps (camera_raygen.comp:266-285); a ray still
+// crossing faces after the 10th trace counts as a miss.  depth accumulates the distance of the wraps.
+template <int STK>
+__device__ __forceinline__ uint32_t camera_trace(const DevScene &sc, v3f &o, v3f d, float &t, float &depth,
+                                                 uint32_t *stk, int *err) {
+    NoCnt cnt{};
+    uint32_t hit = RT_MISS;
+    depth = 0.0f;
+    for (int wrap = 0; wrap < 10; ++wrap) {
+        uint32_t slot;
+        hit = trace_closest<false, STK>(sc, o, d, 1e-5f, RT_MISS, 1e30f, t, slot, stk, 256, err, cnt);
+        if (sc.nbbox == 0) break;
+        const float bt = nearest_face(sc, o, d, t);
+        if (!(bt > 0.f)) break;
+        depth += bt;
+        o = wrap_at(sc, o, d, bt);
+        hit = RT_MISS;
+    }
+    return hit;
+}
+
+// Prague sky pattern (sky_distribution.glsl:60-90), or 1 (isotropic) when its normalisation is not positive.
+__device__ __forceinline__ float cam_sky_weight(v3f d, const float *sun, const float *prm) {
+    if (!(prm[3] > 0.0f)) return 1.0f;
+    const float c = fminf(1.0f, fmaxf(-1.0f, d.x * sun[0] + d.y * sun[1] + d.z * sun[2]));
+    const float gamma = acosf(c) * (180.0f / RT_PI_F);
+    const float cos_theta = fmaxf(0.0f, d.z);
+    const float circ = 1.0f + prm[0] * expf(-gamma / prm[1]);
+    const float horizon = 1.0f + (prm[2] - 1.0f) * (1.0f - cos_theta);
+    return circ * horizon * prm[3] * RT_PI_F;
+}
+
+// Radiance of the sources a camera ray meets before tmax, for band b, summed in source order
+// (camera_raygen.comp:349-377 on a hit, 431-455 on a miss, where collimated and sun-sphere sources add the
+// solar disk's radiance when the ray lies within it).
+__device__ __forceinline__ float cam_source_radiance(const CameraLaunch &c, v3f o, v3f d, uint32_t b, float tmax,
+                                                     bool miss) {
+    float rad = 0.0f;
+    for (uint32_t s = 0; s < c.S; s++) {
+        const float flux = c.flux[(size_t)s * c.B + b];
+        if (flux <= 0.0f) continue;
+        const uint32_t ty = c.src_type[s];
+        const v3f sp = mk3(c.src_pos[3 * s], c.src_pos[3 * s + 1], c.src_pos[3 * s + 2]);
+        const float *w = c.src_width + 2 * s;
+        float cosang;
+        if (ty == 0u || ty == 2u) {
+            if (miss && d.x * c.sun[0] + d.y * c.sun[1] + d.z * c.sun[2] >= c.solar_cos && c.solar_rad[b] > 0.0f)
+                rad += c.solar_rad[b];
+        } else if (ty == 1u) {
+            const float r = w[0] * 0.5f;
+            if (cam_sphere_visible(o, d, sp, r, tmax)) rad += rdiv(rdiv(flux, 4.0f * RT_PI_F * r * r), RT_PI_F);
+        } else if (ty == 3u) {
+            if (cam_planar_visible(false, o, d, sp, w, c.src_rot + 9 * s, tmax, cosang))
+                rad += rdiv(rdiv(flux, w[0] * w[1]) * cosang, RT_PI_F);
+        } else if (ty == 4u) {
+            const float r = w[0];
+            if (cam_planar_visible(true, o, d, sp, w, c.src_rot + 9 * s, tmax, cosang))
+                rad += rdiv(rdiv(flux, RT_PI_F * r * r) * cosang, RT_PI_F);
+        }
+    }
+    return rad;
+}
+
+// Camera pixels: one thread per pixel of the tile traces its antialiasing samples in order and adds
+// their contributions to camera_radiation[b][pixel] (a fixed order, so a render is reproducible; the
+// reference adds the samples with float atomics).  No specular term (this backend rejects it).
+template <int STK>
+__global__ __launch_bounds__(256) void k_camera(CameraLaunch c, int *err) {
+    __shared__ uint32_t stk_all[STK * 256];
+    uint32_t *stk = stk_all + threadIdx.x;
+    const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+    if (q >= c.res_x * c.res_y) return;
+    const DevScene &sc = *c.sc;
+    const uint32_t px = q % c.res_x, py = q / c.res_x;
+    const uint32_t ii = c.off_x + px, jj = c.off_y + py;
+    const size_t pix = (size_t)jj * c.full_x + ii, npix = (size_t)c.full_x * c.full_y;
+    const float strength = rdiv(1.0f, (float)c.spp);
+    const uint32_t dimx = c.res_x * c.res_y, indx = c.res_x * py + px;
+    for (uint32_t smp = 0; smp < c.spp; smp++) {
+        uint32_t st = tea16(indx + dimx * smp, c.seed);
+        const float Rx = rnd24(st), Ry = rnd24(st);
+        float lx = 0.f, ly = 0.f;
+        if (c.lens > 0.0f) unit_disk(rnd24(st), rnd24(st), lx, ly);
+        v3f o, d;
+        camera_ray(c.pos[0], c.pos[1], c.pos[2], c.st, c.ct, c.sp, c.cp, c.focal, c.lens, c.aspect, c.viewplane, ii,
+                   jj, c.full_x, c.full_y, Rx, Ry, lx, ly, o, d);
+        float t, depth;
+        const uint32_t h = camera_trace<STK>(sc, o, d, t, depth, stk, err);
+        if (h != RT_MISS) {
+            const float4 n4 = sc.nrm[h];
+            const bool face = n4.x * d.x + n4.y * d.y + n4.z * d.z < 0.0f;  // true: top face hit
+            const bool top = face || sc.type[h] == 4u;
+            for (uint32_t b = 0; b < c.B; b++) {
+                const float srad = cam_source_radiance(c, o, d, b, t, false);
+                float surf = (top ? c.out_top : c.out_bot)[(size_t)h * c.B + b] * strength;
+                if (srad > 0.0f) surf += srad * strength;
+                c.pixels[(size_t)b * npix + pix] += rdiv(surf, RT_PI_F);
+            }
+        } else {
+            for (uint32_t b = 0; b < c.B; b++) {
+                float rad = cam_source_radiance(c, o, d, b, 1e30f, true);
+                if (rad <= 0.0f && c.sky_rad[b] > 0.0f) rad = c.sky_rad[b] * cam_sky_weight(d, c.sun, c.sky_params + 4 * b);
+                if (c.emis[b] != 0u && c.cam_dflux[b] > 0.0f) rad += rdiv(c.cam_dflux[b], RT_PI_F);
+                if (rad > 0.0f) c.pixels[(size_t)b * npix + pix] += rad * strength;
+            }
+        }
+    }
+}
+
+// Pixel labels (pixel_label_raygen.comp:80-155): the pixel-centre pinhole ray's hit UUID + 1 (0 = sky) and
+// its depth along the view direction (-1 on a miss).
+template <int STK>
+__global__ __launch_bounds__(256) void k_pixel_label(CameraLaunch c, int *err) {
+    __shared__ uint32_t stk_all[STK * 256];
+    uint32_t *stk = stk_all + threadIdx.x;
+    const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+    if (q >= c.res_x * c.res_y) return;
+    const DevScene &sc = *c.sc;
+    const uint32_t px = q % c.res_x, py = q / c.res_x;
+    const uint32_t ii = c.off_x + px, jj = c.off_y + py;
+    const size_t pix = (size_t)jj * c.full_x + ii;
+    v3f o, d;
+    camera_ray(c.pos[0], c.pos[1], c.pos[2], c.st, c.ct, c.sp, c.cp, c.focal, 0.0f, c.aspect, c.viewplane, ii, jj,
+               c.full_x, c.full_y, 0.5f, 0.5f, 0.0f, 0.0f, o, d);
+    float t, depth;
+    const uint32_t h = camera_trace<STK>(sc, o, d, t, depth, stk, err);
+    if (h != RT_MISS) {
+        c.labels[pix] = sc.uuid[h] + 1u;
+        const v3f fwd = camera_rotate(mk3(1.0f, 0.0f, 0.0f), c.st, c.ct, c.sp, c.cp);
+        c.depths[pix] = fabsf(dot3(fwd, d)) * (depth + t);
+    } else {
+        c.depths[pix] = -1.0f;
+    }
+}
+
 __global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
 
 __global__ void k_zero_u32(uint32_t *p) { *p = 0u; }
@@ -1753,6 +1891,18 @@ int launch_gather(const GatherJob *jobs, uint32_t njobs, uint32_t total, uint32_
         set_error("more than 256 channels in one gather");
         return -1;
     }
+    LCHECK();
+    return 0;
+}
+
+int launch_camera(const CameraLaunch &c, uint32_t depth, bool labels, int *err, hipStream_t st) {
+    const uint64_t n = (uint64_t)c.res_x * c.res_y;
+    if (n == 0) return 0;
+    const uint32_t g = (uint32_t)((n + 255) / 256);
+    if (labels)
+        PICK_STK(depth, (k_pixel_label<32><<<g, 256, 0, st>>>(c, err)), (k_pixel_label<64><<<g, 256, 0, st>>>(c, err)));
+    else
+        PICK_STK(depth, (k_camera<32><<<g, 256, 0, st>>>(c, err)), (k_camera<64><<<g, 256, 0, st>>>(c, err)));
     LCHECK();
     return 0;
 }

@@ -81,6 +81,35 @@ struct DirectJob {
     float rot[9];          // rotation matrix (rows) of a rectangle or disk source
 };
 
+// One camera launch (R/shaders/camera_raygen.comp, pixel_label_raygen.comp): camera pose and lens, the
+// tile of pixels, and what a hit or a miss reads.  rot = (sin, cos) of theta = elevation - pi/2 and of
+// phi = pi/2 - azimuth, evaluated in double on the host and rounded.
+struct CameraLaunch {
+    const DevScene *sc;
+    float pos[3];
+    float st, ct, sp, cp;
+    float focal, lens, aspect, viewplane;
+    uint32_t res_x, res_y, full_x, full_y, off_x, off_y;
+    uint32_t spp, seed;
+    uint32_t B, S;
+    const float *out_top, *out_bot;    // [P][B] radiation_out of the last passes
+    const float *flux;                 // [S][B] source fluxes
+    const float *src_pos;              // [S][3]
+    const uint32_t *src_type;          // [S]
+    const float *src_width;            // [S][2]
+    const float *src_rot;              // [S][9] rotation matrix rows
+    const float *sky_params;           // [B][4] Prague (circ strength, circ width, horizon, norm)
+    const float *sky_rad;              // [B] camera sky radiance
+    const float *solar_rad;            // [B] solar disk radiance
+    const float *cam_dflux;            // [B] sky flux seen on a miss when the band emits
+    const uint32_t *emis;              // [B] band emission flags
+    float sun[3];
+    float solar_cos;
+    float *pixels;                     // [B][full_x * full_y]
+    uint32_t *labels;                  // [full_x * full_y]
+    float *depths;                     // [full_x * full_y]
+};
+
 // one diffuse trace pass: writes hit words [P][2][n*n]
 struct DiffuseJob {
     const DevScene *sc;

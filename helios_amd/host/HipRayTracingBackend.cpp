@@ -109,8 +109,23 @@ namespace helios {
         }
     }
 
-    void HipRayTracingBackend::updateSkyModel(const std::vector<helios::vec4> &, const std::vector<float> &, const helios::vec3 &, const std::vector<float> &, float, const std::vector<float> &, const std::vector<uint32_t> &) {
-        // sky radiance feeds camera rendering only, which this backend does not provide
+    void HipRayTracingBackend::updateSkyModel(const std::vector<helios::vec4> &sky_radiance_params, const std::vector<float> &camera_sky_radiance, const helios::vec3 &sun_direction, const std::vector<float> &solar_disk_radiance,
+                                              float solar_disk_cos_angle, const std::vector<float> &camera_diffuse_flux, const std::vector<uint32_t> &band_emission_flag) {
+        const size_t B = sky_radiance_params.size();
+        if (camera_sky_radiance.size() != B || solar_disk_radiance.size() != B) {
+            helios_runtime_error("ERROR (HipRayTracingBackend::updateSkyModel): per-band vectors differ in length.");
+        }
+        std::vector<float> prm(4 * B);
+        for (size_t b = 0; b < B; b++) {
+            prm[4 * b] = sky_radiance_params[b].x;
+            prm[4 * b + 1] = sky_radiance_params[b].y;
+            prm[4 * b + 2] = sky_radiance_params[b].z;
+            prm[4 * b + 3] = sky_radiance_params[b].w;
+        }
+        const float sun[3] = {sun_direction.x, sun_direction.y, sun_direction.z};
+        check(rt_update_sky_model(ctx_, static_cast<uint32_t>(B), prm.data(), camera_sky_radiance.data(), sun, solar_disk_radiance.data(), solar_disk_cos_angle,
+                                  camera_diffuse_flux.size() == B ? camera_diffuse_flux.data() : nullptr, band_emission_flag.size() == B ? band_emission_flag.data() : nullptr),
+              "updateSkyModel");
     }
 
     void HipRayTracingBackend::bandTable(const RayTracingLaunchParams &params, const char *who) {
@@ -192,12 +207,47 @@ namespace helios {
         check(rt_launch_diffuse_rays(ctx_, &p), "launchDiffuseRays");
     }
 
-    void HipRayTracingBackend::launchCameraRays(const RayTracingLaunchParams &) {
-        unsupported("launchCameraRays", "camera rendering");
+    rt_camera_params HipRayTracingBackend::cameraParams(const RayTracingLaunchParams &params) {
+        rt_camera_params c{};
+        c.position[0] = params.camera_position.x;
+        c.position[1] = params.camera_position.y;
+        c.position[2] = params.camera_position.z;
+        c.direction[0] = params.camera_direction.x;
+        c.direction[1] = params.camera_direction.y;
+        c.focal_length = params.camera_focal_length;
+        c.lens_diameter = params.camera_lens_diameter;
+        c.fov_aspect = params.camera_fov_aspect;
+        c.viewplane_length = params.camera_viewplane_length;
+        c.resolution[0] = static_cast<uint32_t>(params.camera_resolution.x);
+        c.resolution[1] = static_cast<uint32_t>(params.camera_resolution.y);
+        c.resolution_full[0] = static_cast<uint32_t>(params.camera_resolution_full.x);
+        c.resolution_full[1] = static_cast<uint32_t>(params.camera_resolution_full.y);
+        c.pixel_offset[0] = static_cast<uint32_t>(params.camera_pixel_offset.x);
+        c.pixel_offset[1] = static_cast<uint32_t>(params.camera_pixel_offset.y);
+        c.antialiasing_samples = params.antialiasing_samples;
+        c.random_seed = params.random_seed;
+        c.specular_reflection_enabled = params.specular_reflection_enabled;
+        return c;
     }
 
-    void HipRayTracingBackend::launchPixelLabelRays(const RayTracingLaunchParams &) {
-        unsupported("launchPixelLabelRays", "camera pixel labelling");
+    void HipRayTracingBackend::launchCameraRays(const RayTracingLaunchParams &params) {
+        if (primitive_count_ == 0) {
+            return;
+        }
+        if (params.specular_reflection_enabled != 0) {
+            unsupported("launchCameraRays", "specular reflection");
+        }
+        rt_camera_params c = cameraParams(params);
+        check(rt_launch_camera_rays(ctx_, &c), "launchCameraRays");
+    }
+
+    void HipRayTracingBackend::launchPixelLabelRays(const RayTracingLaunchParams &params) {
+        if (primitive_count_ == 0) {
+            return;
+        }
+        rt_camera_params c = cameraParams(params);
+        c.specular_reflection_enabled = 0;
+        check(rt_launch_pixel_label_rays(ctx_, &c), "launchPixelLabelRays");
     }
 
     void HipRayTracingBackend::getRadiationResults(RayTracingResults &results) {
@@ -219,8 +269,18 @@ namespace helios {
         results.num_bands = launch_band_count_;
     }
 
-    void HipRayTracingBackend::getCameraResults(std::vector<float> &, std::vector<uint> &, std::vector<float> &, uint, const helios::int2 &) {
-        unsupported("getCameraResults", "camera rendering");
+    void HipRayTracingBackend::getCameraResults(std::vector<float> &pixel_data, std::vector<uint> &pixel_labels, std::vector<float> &pixel_depths, uint, const helios::int2 &resolution) {
+        const size_t n = size_t(resolution.x) * size_t(resolution.y);
+        if (n == 0) {
+            return;
+        }
+        pixel_data.assign(n * launch_band_count_, 0.f);
+        pixel_labels.assign(n, 0u);
+        pixel_depths.assign(n, 0.f);
+        if (primitive_count_ == 0) {
+            return;
+        }
+        check(rt_get_camera_results(ctx_, pixel_data.data(), pixel_labels.data(), pixel_depths.data(), static_cast<uint32_t>(resolution.x), static_cast<uint32_t>(resolution.y)), "getCameraResults");
     }
 
     void HipRayTracingBackend::zeroRadiationBuffers(size_t launch_band_count) {
@@ -240,8 +300,8 @@ namespace helios {
         check(rt_zero_scatter_buffers(ctx_), "zeroScatterBuffers");
     }
 
-    void HipRayTracingBackend::zeroCameraPixelBuffers(const helios::int2 &) {
-        unsupported("zeroCameraPixelBuffers", "camera rendering");
+    void HipRayTracingBackend::zeroCameraPixelBuffers(const helios::int2 &resolution) {
+        check(rt_zero_camera_pixel_buffers(ctx_, static_cast<uint32_t>(resolution.x), static_cast<uint32_t>(resolution.y)), "zeroCameraPixelBuffers");
     }
 
     void HipRayTracingBackend::copyScatterToRadiation() {
@@ -259,7 +319,7 @@ namespace helios {
     }
 
     void HipRayTracingBackend::zeroCameraScatterBuffers(size_t) {
-        // no camera buffers exist on this backend
+        // camera scatter buffers feed specular camera terms only, which this backend rejects
     }
 
     void HipRayTracingBackend::uploadSourceFluxes(const std::vector<float> &fluxes) {
@@ -270,7 +330,7 @@ namespace helios {
     }
 
     void HipRayTracingBackend::uploadSourceFluxesCam(const std::vector<float> &) {
-        // camera-weighted fluxes feed camera rendering only
+        // camera-weighted fluxes feed specular camera terms only, which this backend rejects
     }
 
     void HipRayTracingBackend::queryGPUMemory() const {

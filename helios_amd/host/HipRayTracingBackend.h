@@ -3,9 +3,9 @@
 // This file builds inside a Helios source tree (it includes plugins/radiation/include/RayTracingBackend.h);
 // it is not part of this repository's own build.  Each virtual maps to one or two C calls and every
 // failure is rethrown through helios_runtime_error, as the reference backends do.  Radiation launches
-// (direct, diffuse) and their buffer management are supported; camera and pixel-label launches,
-// non-collimated sources, anisotropic skies and specular reflection raise an error naming the
-// unsupported feature.
+// (direct, diffuse) for all five source types, camera and pixel-label launches and their buffer
+// management are supported; anisotropic diffuse distributions and specular reflection raise an error
+// naming the unsupported feature.
 #ifndef HIP_RAYTRACING_BACKEND_H
 #define HIP_RAYTRACING_BACKEND_H
 
@@ -59,6 +59,7 @@ namespace helios {
         // band_launch_flag [B_global] -> launch -> global map and per-slot active flags
         void bandTable(const RayTracingLaunchParams &params, const char *who);
         rt_launch_params cParams(const RayTracingLaunchParams &params) const;
+        static rt_camera_params cameraParams(const RayTracingLaunchParams &params);
 
         int device_;
         rt_ctx *ctx_ = nullptr;

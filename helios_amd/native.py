@@ -76,6 +76,14 @@ class rt_launch_params(C.Structure):
                 ("diffuse_flux", f32p), ("radiation_out_top", f32p), ("radiation_out_bottom", f32p)]
 
 
+class rt_camera_params(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("direction", C.c_float * 2), ("focal_length", C.c_float),
+                ("lens_diameter", C.c_float), ("fov_aspect", C.c_float), ("viewplane_length", C.c_float),
+                ("resolution", C.c_uint32 * 2), ("resolution_full", C.c_uint32 * 2), ("pixel_offset", C.c_uint32 * 2),
+                ("antialiasing_samples", C.c_uint32), ("random_seed", C.c_uint32),
+                ("specular_reflection_enabled", C.c_uint32)]
+
+
 class rt_server_cfg(C.Structure):
     _fields_ = [("slots", C.c_uint32), ("max_batch_suns", C.c_uint32), ("memory_budget", C.c_uint64)]
 
@@ -136,6 +144,11 @@ SIGNATURES = {
     "rt_launch_diffuse_rays": (C.c_int, [C.c_void_p, C.POINTER(rt_launch_params)]),
     "rt_get_radiation_results": (C.c_int, [C.c_void_p, f32p, f32p, f32p, f32p, f32p]),
     "rt_get_backend_info": (C.c_int, [C.c_void_p, u32p]),
+    "rt_update_sky_model": (C.c_int, [C.c_void_p, C.c_uint32, f32p, f32p, f32p, f32p, C.c_float, f32p, u32p]),
+    "rt_launch_camera_rays": (C.c_int, [C.c_void_p, C.POINTER(rt_camera_params)]),
+    "rt_zero_camera_pixel_buffers": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
+    "rt_launch_pixel_label_rays": (C.c_int, [C.c_void_p, C.POINTER(rt_camera_params)]),
+    "rt_get_camera_results": (C.c_int, [C.c_void_p, f32p, u32p, f32p, C.c_uint32, C.c_uint32]),
     "rt_server_create": (C.c_int, [C.c_void_p, C.POINTER(rt_server_cfg), C.POINTER(C.c_void_p)]),
     "rt_server_destroy": (C.c_int, [C.c_void_p]),
     "rt_server_add_model": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(rt_band_request), u32p]),
@@ -473,6 +486,67 @@ class Backend:
         o = np.zeros(4, np.uint32)
         _check(self.lib.rt_get_backend_info(self.ctx._h, _pu(o)))
         return {"nprims": int(o[0]), "nbands_global": int(o[1]), "nbands_launch": int(o[2]), "nsources": int(o[3])}
+
+    # ---- cameras (RayTracingBackend.h:129-210) ----
+    def update_sky_model(self, sky_params, camera_sky_radiance, sun_direction, solar_disk_radiance,
+                         solar_disk_cos_angle: float, camera_diffuse_flux=None, band_emission_flag=None):
+        """Per launch band: sky_params [B][4] (Prague circumsolar strength, width in degrees, horizon
+        brightness, normalisation; normalisation <= 0 = isotropic), camera_sky_radiance [B],
+        solar_disk_radiance [B], camera_diffuse_flux [B], band_emission_flag [B]."""
+        prm = _f32(sky_params).reshape(-1, 4)
+        B = prm.shape[0]
+        srad = _f32(camera_sky_radiance).reshape(B)
+        sun = _f32(sun_direction).reshape(3)
+        solar = _f32(solar_disk_radiance).reshape(B)
+        df = None if camera_diffuse_flux is None else _f32(camera_diffuse_flux).reshape(B)
+        em = None if band_emission_flag is None else _u32(band_emission_flag).reshape(B)
+        _check(self.lib.rt_update_sky_model(self.ctx._h, B, _pf(prm), _pf(srad), _pf(sun), _pf(solar),
+                                            float(solar_disk_cos_angle), _pf(df), _pu(em)))
+
+    @staticmethod
+    def camera_params(camera: "Camera", tile=None, seed: int = 0, specular: bool = False) -> rt_camera_params:
+        """Launch parameters of one tile (x0, y0, w, h) of a camera (the whole image when tile is None)."""
+        fx, fy = camera.resolution
+        x0, y0, w, h = (0, 0, fx, fy) if tile is None else tile
+        return rt_camera_params((C.c_float * 3)(*camera.position), (C.c_float * 2)(*camera.direction),
+                                camera.focal_length, camera.lens_diameter, camera.fov_aspect, camera.viewplane_length,
+                                (C.c_uint32 * 2)(w, h), (C.c_uint32 * 2)(fx, fy), (C.c_uint32 * 2)(x0, y0),
+                                camera.antialiasing_samples, int(seed), int(bool(specular)))
+
+    def launch_camera_rays(self, camera: "Camera", seed: int = 0, tile=None, specular: bool = False):
+        p = self.camera_params(camera, tile, seed, specular)
+        _check(self.lib.rt_launch_camera_rays(self.ctx._h, C.byref(p)))
+
+    def zero_camera_pixel_buffers(self, resolution):
+        _check(self.lib.rt_zero_camera_pixel_buffers(self.ctx._h, int(resolution[0]), int(resolution[1])))
+
+    def launch_pixel_label_rays(self, camera: "Camera", tile=None):
+        p = self.camera_params(camera, tile)
+        _check(self.lib.rt_launch_pixel_label_rays(self.ctx._h, C.byref(p)))
+
+    def get_camera_results(self, resolution) -> dict:
+        """pixel_data [H*W][B_launch] (pixel = row * W + column), pixel_labels [H*W], pixel_depths [H*W]."""
+        W, H = int(resolution[0]), int(resolution[1])
+        out = {"pixel_data": np.zeros((W * H, self.B_launch), np.float32),
+               "pixel_labels": np.zeros(W * H, np.uint32), "pixel_depths": np.zeros(W * H, np.float32)}
+        _check(self.lib.rt_get_camera_results(self.ctx._h, _pf(out["pixel_data"]), _pu(out["pixel_labels"]),
+                                              _pf(out["pixel_depths"]), W, H))
+        return out
+
+
+@dataclass
+class Camera:
+    """A radiation camera's launch fields (RayTracingTypes.h:221-235): position, direction (elevation,
+    azimuth in radians), focal length (working distance), lens diameter (0 = pinhole), HFOV/VFOV aspect,
+    viewplane length, resolution (width, height) and antialiasing samples per pixel."""
+    position: tuple
+    direction: tuple
+    resolution: tuple
+    focal_length: float = 1.0
+    lens_diameter: float = 0.0
+    fov_aspect: float = 1.0
+    viewplane_length: float = 1.0
+    antialiasing_samples: int = 1
 
 
 class Server:

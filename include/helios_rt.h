@@ -210,6 +210,45 @@ int rt_get_radiation_results(rt_ctx *ctx, float *radiation_in, float *radiation_
 /* Backend state: out[4] = primitives of the built scene, B_global, B_launch, sources. */
 int rt_get_backend_info(rt_ctx *ctx, uint32_t *out);
 
+/* Cameras: RayTracingLaunchParams' camera fields (RayTracingTypes.h:221-235). */
+typedef struct {
+    float position[3];
+    float direction[2];              /* elevation, azimuth (radians)                              */
+    float focal_length;              /* working distance                                          */
+    float lens_diameter;             /* 0 = pinhole                                               */
+    float fov_aspect;                /* HFOV / VFOV                                               */
+    float viewplane_length;          /* from the effective HFOV and zoom                          */
+    uint32_t resolution[2];          /* this tile                                                 */
+    uint32_t resolution_full[2];     /* whole image                                               */
+    uint32_t pixel_offset[2];        /* tile origin in the whole image                            */
+    uint32_t antialiasing_samples;   /* 0 is taken as 1                                           */
+    uint32_t random_seed;
+    uint32_t specular_reflection_enabled; /* must be 0: specular camera terms are not supported   */
+} rt_camera_params;
+/* updateSkyModel (RayTracingBackend.h:129): per launch band, sky_radiance_params [B][4] (Prague
+ * circumsolar strength, circumsolar width in degrees, horizon brightness, normalisation; a
+ * normalisation <= 0 means an isotropic sky), camera_sky_radiance [B], solar_disk_radiance [B],
+ * camera_diffuse_flux [B] and band_emission_flag [B] (both NULL = 0); sun_direction[3] unit. */
+int rt_update_sky_model(rt_ctx *ctx, uint32_t nbands, const float *sky_radiance_params,
+                        const float *camera_sky_radiance, const float *sun_direction, const float *solar_disk_radiance,
+                        float solar_disk_cos_angle, const float *camera_diffuse_flux,
+                        const uint32_t *band_emission_flag);
+/* launchCameraRays (RayTracingBackend.h:153; R/shaders/camera_raygen.comp): adds one tile's pixels
+ * to camera_radiation ([B_launch][full pixels]), which the tile at offset (0, 0) zeroes first.  Hit
+ * pixels read radiation_out_top/bottom (the last passes' emitted and scattered radiation), misses
+ * the sky model; sources in view add their radiance.  Each pixel sums its samples in order. */
+int rt_launch_camera_rays(rt_ctx *ctx, const rt_camera_params *params);
+/* zeroCameraPixelBuffers (RayTracingBackend.h:210): sizes and zeroes labels and depths. */
+int rt_zero_camera_pixel_buffers(rt_ctx *ctx, uint32_t width, uint32_t height);
+/* launchPixelLabelRays (RayTracingBackend.h:160; R/shaders/pixel_label_raygen.comp): the
+ * pixel-centre ray's hit UUID + 1 (0 = sky) and its depth along the view axis (-1 = sky). */
+int rt_launch_pixel_label_rays(rt_ctx *ctx, const rt_camera_params *params);
+/* getCameraResults (RayTracingBackend.h:182): pixel_data [pixels][B_launch], labels [pixels],
+ * depths [pixels] for a width x height image; buffers that hold another image read as zeros;
+ * NULL skips an output. */
+int rt_get_camera_results(rt_ctx *ctx, float *pixel_data, uint32_t *pixel_labels, float *pixel_depths, uint32_t width,
+                          uint32_t height);
+
 /* ---------------------------------------------------------------------------------------------
  * Request server (no reference counterpart: the reference solves one runBand per call).  Models
  * (a resident scene + bands, ray counts, depths, optical properties kept in HBM, diffuse seeds) are

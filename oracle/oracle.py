@@ -24,6 +24,20 @@ class orc_band_params(C.Structure):
                 ("schedule", C.c_uint32), ("source_type", u32p), ("source_width", f32p), ("source_rotation", f32p)]
 
 
+class orc_camera(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("direction", C.c_float * 2), ("focal_length", C.c_float),
+                ("lens_diameter", C.c_float), ("fov_aspect", C.c_float), ("viewplane_length", C.c_float),
+                ("resolution", C.c_uint32 * 2), ("resolution_full", C.c_uint32 * 2), ("pixel_offset", C.c_uint32 * 2),
+                ("antialiasing_samples", C.c_uint32), ("random_seed", C.c_uint32)]
+
+
+class orc_camera_light(C.Structure):
+    _fields_ = [("nbands", C.c_uint32), ("nsources", C.c_uint32), ("out_top", f32p), ("out_bottom", f32p),
+                ("source_flux", f32p), ("source_pos", f32p), ("source_type", u32p), ("source_width", f32p),
+                ("source_rotation", f32p), ("sky_params", f32p), ("sky_radiance", f32p), ("solar_radiance", f32p),
+                ("camera_diffuse_flux", f32p), ("emission", u32p), ("sun", C.c_float * 3), ("solar_cos", C.c_float)]
+
+
 _lib = None
 
 
@@ -66,6 +80,8 @@ def lib():
                                           f64p]
         L.orc_rotation_matrix.argtypes = [f32p, f32p]
         L.orc_areas.argtypes = [C.c_void_p, f64p]
+        L.orc_camera_pixels.argtypes = [C.c_void_p, C.POINTER(orc_camera), C.POINTER(orc_camera_light), f64p]
+        L.orc_pixel_labels.argtypes = [C.c_void_p, C.POINTER(orc_camera), u32p, f32p]
         _lib = L
     return _lib
 
@@ -189,6 +205,49 @@ class OracleScene:
                                   None if W is None else W.ctypes.data_as(f32p),
                                   None if R is None else R.ctypes.data_as(f32p), out.ctypes.data_as(f64p))
         return out
+
+    @staticmethod
+    def _camera(cam, tile, seed):
+        fx, fy = cam.resolution
+        x0, y0, w, h = (0, 0, fx, fy) if tile is None else tile
+        return orc_camera((C.c_float * 3)(*cam.position), (C.c_float * 2)(*cam.direction), cam.focal_length,
+                          cam.lens_diameter, cam.fov_aspect, cam.viewplane_length, (C.c_uint32 * 2)(w, h),
+                          (C.c_uint32 * 2)(fx, fy), (C.c_uint32 * 2)(x0, y0), cam.antialiasing_samples, int(seed))
+
+    def camera_pixels(self, cam, out_top, out_bottom, source_flux, source_pos, sky, seed=0, tile=None,
+                      source_types=None, source_widths=None, source_rotations=None, pixels=None):
+        """Camera radiation [B][H*W] (FP64 sums of the FP32 samples), added to `pixels` when given.
+        sky: dict(params [B][4], radiance [B], sun [3], solar [B], solar_cos, diffuse_flux [B], emission [B])."""
+        ot, ob = _f(out_top).reshape(self.P, -1), _f(out_bottom).reshape(self.P, -1)
+        B = ot.shape[1]
+        X = _f(source_pos).reshape(-1, 3)
+        S = X.shape[0]
+        F = _f(source_flux).reshape(S, B) if S else np.zeros((1, B), np.float32)
+        ty = None if source_types is None else _u(source_types).reshape(S)
+        W = None if source_widths is None else _f(source_widths).reshape(S, 2)
+        R = None if source_rotations is None else _f(source_rotations).reshape(S, 3)
+        prm = _f(sky["params"]).reshape(B, 4)
+        srad, solar = _f(sky["radiance"]).reshape(B), _f(sky["solar"]).reshape(B)
+        df = _f(sky.get("diffuse_flux", np.zeros(B))).reshape(B)
+        em = _u(sky.get("emission", np.zeros(B))).reshape(B)
+        pt = lambda a: None if a is None else a.ctypes.data_as(f32p)
+        L = orc_camera_light(B, S, pt(ot), pt(ob), pt(F), pt(X), None if ty is None else ty.ctypes.data_as(u32p),
+                             pt(W), pt(R), pt(prm), pt(srad), pt(solar), pt(df), em.ctypes.data_as(u32p),
+                             (C.c_float * 3)(*[float(v) for v in sky["sun"]]), float(sky["solar_cos"]))
+        fx, fy = cam.resolution
+        out = np.zeros((B, fx * fy), np.float64) if pixels is None else pixels
+        c = self._camera(cam, tile, seed)
+        lib().orc_camera_pixels(self.h, C.byref(c), C.byref(L), out.ctypes.data_as(f64p))
+        return out
+
+    def pixel_labels(self, cam, tile=None, labels=None, depths=None):
+        """(labels [H*W] uint32, depths [H*W] float32) of the pixel-centre rays."""
+        fx, fy = cam.resolution
+        lab = np.zeros(fx * fy, np.uint32) if labels is None else labels
+        dep = np.zeros(fx * fy, np.float32) if depths is None else depths
+        c = self._camera(cam, tile, 0)
+        lib().orc_pixel_labels(self.h, C.byref(c), lab.ctypes.data_as(u32p), dep.ctypes.data_as(f32p))
+        return lab, dep
 
     def cast_rays(self, O, D, max_dist=-1.0, brute=False):
         O = _f(O).reshape(-1, 3)

@@ -964,6 +964,220 @@ void orc_cast_rays(const orc_scene *s, uint64_t n, const float *origins, const f
 /* ------------------------------------------------------------------------------------------------ */
 /* runBand pass schedule (R/src/RadiationModel.cpp:3470-4672)                                        */
 /* ------------------------------------------------------------------------------------------------ */
+/* cameras                                                                                           */
+/* ------------------------------------------------------------------------------------------------ */
+
+/* camera_rotate_point (R/shaders/common/transforms.glsl): Rz(phi) * Ry(theta) * q */
+static v3 cam_rotate(v3 q, float st, float ct, float sp, float cp) {
+    return mk((cp * ct) * q.x - sp * q.y + (cp * st) * q.z, (sp * ct) * q.x + cp * q.y + (sp * st) * q.z,
+              -st * q.x + ct * q.z);
+}
+
+/* theta = elevation - pi/2, phi = pi/2 - azimuth (camera_raygen.comp:258-259); sines and cosines are
+ * taken in double from the FP32 angles and rounded. */
+static void cam_angles(const orc_camera *c, float *st, float *ct, float *sp, float *cp) {
+    const float pi = 3.14159265359f;
+    double th = (double)(-0.5f * pi + c->direction[0]), ph = (double)(0.5f * pi - c->direction[1]);
+    *st = (float)sin(th); *ct = (float)cos(th); *sp = (float)sin(ph); *cp = (float)cos(ph);
+}
+
+/* Ray through pixel (ii, jj) at sub-pixel offset (rx, ry) and lens point (lx, ly) in lens radii
+ * (camera_raygen.comp:228-261, pixel_label_raygen.comp:93-115). */
+static void cam_ray(const orc_camera *c, float st, float ct, float sp, float cp, uint32_t ii, uint32_t jj, float rx,
+                    float ry, float lx, float ly, v3 *o, v3 *d) {
+    float mult = 1.0f / c->fov_aspect;
+    float spy = -0.5f + ((float)ii + rx) / (float)c->resolution_full[0];
+    float spz = (0.5f - ((float)jj + ry) / (float)c->resolution_full[1]) * mult;
+    v3 p = mk(c->focal_length, spy / c->viewplane_length * c->focal_length, spz / c->viewplane_length * c->focal_length);
+    v3 org = mk(0.0f, (0.5f * lx) * c->lens_diameter, (0.5f * ly) * c->lens_diameter);
+    v3 dir = sub(p, org);
+    *o = add(cam_rotate(org, st, ct, sp, cp), ld3(c->position));
+    *d = nrm(cam_rotate(dir, st, ct, sp, cp));
+}
+
+/* Concentric square-to-disk map (Shirley & Chiu 1997; camera_raygen.comp:96-128) */
+static void cam_disk(float Rx, float Ry, float *dx, float *dy) {
+    float x = -1.0f + 2.0f * Rx, y = -1.0f + 2.0f * Ry, r, k;
+    if (x > -y) {
+        if (x > y) { r = x; k = y / x; }
+        else { r = y; k = 2.0f - x / y; }
+    } else {
+        if (x < y) { r = -x; k = 4.0f + y / x; }
+        else { r = -y; k = (y != 0.0f) ? 6.0f - x / y : 0.0f; }
+    }
+    float sk, ck;
+    orc_sincos_2pi(k * 0.125f, &sk, &ck);
+    *dx = r * ck;
+    *dy = r * sk;
+}
+
+/* Closest hit with up to 10 periodic wraps (camera_raygen.comp:266-285); *depth sums the distances to
+ * the faces crossed. */
+static uint32_t cam_trace(const orc_scene *s, v3 *o, v3 d, float *t, float *depth) {
+    uint32_t hit = MISS;
+    *depth = 0.0f;
+    for (int wrap = 0; wrap < 10; ++wrap) {
+        hit = closest(s, *o, d, 1e-5f, MISS, 0, 0, 1e30f, t);
+        if (!s->nbbox) break;
+        float bt = *t;
+        int cf = -1;
+        for (uint32_t f = 0; f < s->nbbox; f++) {
+            float tf = isect_bbox_face(s, f, *o, d);
+            if (tf > 0.0f && tf < bt) { bt = tf; cf = (int)f; }
+        }
+        if (cf < 0) break;
+        v3 w;
+        periodic_wrap(s, *o, d, *t, &w);
+        *depth += bt;
+        *o = w;
+        hit = MISS;
+    }
+    return hit;
+}
+
+/* sphere_source_visible (camera_raygen.comp:131-143) */
+static int cam_sphere_vis(v3 o, v3 d, v3 c, float radius, float tmax) {
+    v3 oc = sub(o, c);
+    float b = dot(oc, d);
+    float cc = dot(oc, oc) - radius * radius;
+    float disc = b * b - cc;
+    if (!(disc >= 0.0f)) return 0;
+    float ts = -b - sqrtf(disc);
+    return ts > 0.0f && ts < tmax;
+}
+
+/* rectangle_source_visible / disk_source_visible (camera_raygen.comp:146-203): normal = third row of
+ * the rotation matrix; disk width[0] = radius. */
+static int cam_planar_vis(int disk, v3 o, v3 d, v3 c, const float *w, const float *m, float tmax, float *cosang) {
+    v3 n = mk(m[6], m[7], m[8]);
+    float denom = dot(d, n);
+    if (denom >= -1e-6f) return 0;
+    float t = dot(sub(c, o), n) / denom;
+    if (t <= 0.0f || t >= tmax) return 0;
+    v3 off = sub(add(o, scl(d, t)), c);
+    int in;
+    if (disk) in = dot(off, off) <= w[0] * w[0];
+    else {
+        float lx = m[0] * off.x + m[3] * off.y + m[6] * off.z;
+        float ly = m[1] * off.x + m[4] * off.y + m[7] * off.z;
+        in = fabsf(lx) <= w[0] * 0.5f && fabsf(ly) <= w[1] * 0.5f;
+    }
+    if (in) *cosang = -denom;
+    return in;
+}
+
+/* Radiance of the sources met before tmax for band b, in source order (camera_raygen.comp:331-365 on a
+ * hit; 414-446 on a miss, where collimated and sun-sphere sources add the solar disk). */
+static float cam_sources(const orc_camera_light *L, const float *mats, v3 o, v3 d, uint32_t b, float tmax, int miss) {
+    const float pi = 3.14159265359f;
+    float rad = 0.0f;
+    for (uint32_t s = 0; s < L->nsources; s++) {
+        float flux = L->source_flux[(size_t)s * L->nbands + b];
+        if (flux <= 0.0f) continue;
+        uint32_t ty = L->source_type ? L->source_type[s] : 0u;
+        v3 sp = ld3(L->source_pos + 3 * s);
+        const float *w = L->source_width ? L->source_width + 2 * s : NULL;
+        float ca;
+        if (ty == 0u || ty == 2u) {
+            if (miss && d.x * L->sun[0] + d.y * L->sun[1] + d.z * L->sun[2] >= L->solar_cos && L->solar_radiance[b] > 0.0f)
+                rad += L->solar_radiance[b];
+        } else if (ty == 1u) {
+            float r = w[0] * 0.5f;
+            if (cam_sphere_vis(o, d, sp, r, tmax)) rad += (flux / (4.0f * pi * r * r)) / pi;
+        } else if (ty == 3u) {
+            if (cam_planar_vis(0, o, d, sp, w, mats + 9 * s, tmax, &ca)) rad += (flux / (w[0] * w[1])) * ca / pi;
+        } else if (ty == 4u) {
+            float r = w[0];
+            if (cam_planar_vis(1, o, d, sp, w, mats + 9 * s, tmax, &ca)) rad += (flux / (pi * r * r)) * ca / pi;
+        }
+    }
+    return rad;
+}
+
+/* Prague sky pattern (R/shaders/common/sky_distribution.glsl), 1 when the normalisation is not positive */
+static float cam_sky(v3 d, const float *sun, const float *prm) {
+    const float pi = 3.14159265359f;
+    if (!(prm[3] > 0.0f)) return 1.0f;
+    float c = fminf(1.0f, fmaxf(-1.0f, d.x * sun[0] + d.y * sun[1] + d.z * sun[2]));
+    float gamma = acosf(c) * (180.0f / pi);
+    float cos_theta = fmaxf(0.0f, d.z);
+    float circ = 1.0f + prm[0] * expf(-gamma / prm[1]);
+    float horizon = 1.0f + (prm[2] - 1.0f) * (1.0f - cos_theta);
+    return circ * horizon * prm[3] * pi;
+}
+
+void orc_camera_pixels(const orc_scene *s, const orc_camera *c, const orc_camera_light *L, double *pixels) {
+    const float pi = 3.14159265359f;
+    float st, ct, sp, cp;
+    cam_angles(c, &st, &ct, &sp, &cp);
+    const uint32_t B = L->nbands, spp = c->antialiasing_samples ? c->antialiasing_samples : 1u;
+    const uint32_t rx = c->resolution[0], ry = c->resolution[1];
+    const size_t npix = (size_t)c->resolution_full[0] * c->resolution_full[1];
+    float *mats = (float *)calloc(9 * (size_t)(L->nsources + 1), sizeof(float));
+    for (uint32_t k = 0; k < L->nsources; k++)
+        if (L->source_rotation) orc_rotation_matrix(L->source_rotation + 3 * k, mats + 9 * k);
+    const float strength = 1.0f / (float)spp;
+    const uint32_t dimx = rx * ry;
+    for (uint32_t py = 0; py < ry; py++)
+        for (uint32_t px = 0; px < rx; px++) {
+            const uint32_t ii = c->pixel_offset[0] + px, jj = c->pixel_offset[1] + py;
+            const size_t pix = (size_t)jj * c->resolution_full[0] + ii;
+            const uint32_t indx = rx * py + px;
+            for (uint32_t smp = 0; smp < spp; smp++) {
+                uint32_t state = orc_tea16(indx + dimx * smp, c->random_seed);
+                float Rx = orc_rnd(&state), Ry = orc_rnd(&state), lx = 0.0f, ly = 0.0f;
+                if (c->lens_diameter > 0.0f) {
+                    float a = orc_rnd(&state), bq = orc_rnd(&state);
+                    cam_disk(a, bq, &lx, &ly);
+                }
+                v3 o, d;
+                cam_ray(c, st, ct, sp, cp, ii, jj, Rx, Ry, lx, ly, &o, &d);
+                float t, depth;
+                uint32_t h = cam_trace(s, &o, d, &t, &depth);
+                if (h != MISS) {
+                    int top = dot(s->normal[h], d) < 0.0f || s->type[h] == 4u;
+                    for (uint32_t b = 0; b < B; b++) {
+                        float srad = cam_sources(L, mats, o, d, b, t, 0);
+                        float surf = (top ? L->out_top : L->out_bottom)[(size_t)h * B + b] * strength;
+                        if (srad > 0.0f) surf += srad * strength;
+                        pixels[(size_t)b * npix + pix] += (double)(surf / pi);
+                    }
+                } else {
+                    for (uint32_t b = 0; b < B; b++) {
+                        float rad = cam_sources(L, mats, o, d, b, 1e30f, 1);
+                        if (rad <= 0.0f && L->sky_radiance[b] > 0.0f) rad = L->sky_radiance[b] * cam_sky(d, L->sun, L->sky_params + 4 * b);
+                        float df = L->camera_diffuse_flux ? L->camera_diffuse_flux[b] : 0.0f;
+                        if (L->emission && L->emission[b] && df > 0.0f) rad += df / pi;
+                        if (rad > 0.0f) pixels[(size_t)b * npix + pix] += (double)(rad * strength);
+                    }
+                }
+            }
+        }
+    free(mats);
+}
+
+void orc_pixel_labels(const orc_scene *s, const orc_camera *c, uint32_t *labels, float *depths) {
+    float st, ct, sp, cp;
+    cam_angles(c, &st, &ct, &sp, &cp);
+    const v3 fwd = cam_rotate(mk(1.0f, 0.0f, 0.0f), st, ct, sp, cp);
+    for (uint32_t py = 0; py < c->resolution[1]; py++)
+        for (uint32_t px = 0; px < c->resolution[0]; px++) {
+            const uint32_t ii = c->pixel_offset[0] + px, jj = c->pixel_offset[1] + py;
+            const size_t pix = (size_t)jj * c->resolution_full[0] + ii;
+            v3 o, d;
+            cam_ray(c, st, ct, sp, cp, ii, jj, 0.5f, 0.5f, 0.0f, 0.0f, &o, &d);
+            float t, depth;
+            uint32_t h = cam_trace(s, &o, d, &t, &depth);
+            if (h != MISS) {
+                labels[pix] = s->uuid[h] + 1u;
+                depths[pix] = fabsf(dot(fwd, d)) * (depth + t);
+            } else {
+                depths[pix] = -1.0f;
+            }
+        }
+}
+
+/* ------------------------------------------------------------------------------------------------ */
 
 /* One diffuse pass, faces 1 then 0 (RadiationModel.cpp:4204-4209, diffuse_raygen.comp:330-656). */
 static void diffuse_pass(const orc_scene *s, const orc_band_params *p, uint32_t n, uint32_t seed,

@@ -90,6 +90,38 @@ void orc_direct_sums_range(const orc_scene *s, uint32_t d, uint32_t seed, uint32
 void orc_cast_rays(const orc_scene *s, uint64_t n, const float *origins, const float *dirs, float max_dist,
                    float *out_t, float *out_normal, uint32_t *out_uuid, int brute);
 
+/* ---- cameras (R/shaders/camera_raygen.comp, pixel_label_raygen.comp) ---- */
+typedef struct {
+    float position[3];
+    float direction[2];        /* elevation, azimuth (radians)                          */
+    float focal_length, lens_diameter, fov_aspect, viewplane_length;
+    uint32_t resolution[2];    /* tile                                                  */
+    uint32_t resolution_full[2];
+    uint32_t pixel_offset[2];
+    uint32_t antialiasing_samples;
+    uint32_t random_seed;
+} orc_camera;
+typedef struct {
+    uint32_t nbands, nsources;
+    const float *out_top, *out_bottom;  /* [P][B] radiation_out of the last passes         */
+    const float *source_flux;           /* [S][B]                                          */
+    const float *source_pos;            /* [S][3]                                          */
+    const uint32_t *source_type;        /* [S]                                             */
+    const float *source_width;          /* [S][2]                                          */
+    const float *source_rotation;       /* [S][3] Euler angles                             */
+    const float *sky_params;            /* [B][4]                                          */
+    const float *sky_radiance;          /* [B]                                             */
+    const float *solar_radiance;        /* [B]                                             */
+    const float *camera_diffuse_flux;   /* [B]                                             */
+    const uint32_t *emission;           /* [B]                                             */
+    float sun[3];
+    float solar_cos;
+} orc_camera_light;
+/* Adds the tile's samples to pixels [B][full_x*full_y] (FP64 accumulation of FP32 samples). */
+void orc_camera_pixels(const orc_scene *s, const orc_camera *c, const orc_camera_light *L, double *pixels);
+/* Pixel labels (UUID + 1, 0 = sky) and depths (-1 = sky) of the tile, [full_x*full_y] each. */
+void orc_pixel_labels(const orc_scene *s, const orc_camera *c, uint32_t *labels, float *depths);
+
 /* ---- runBand ---- */
 typedef struct {
     uint32_t nbands;           /* B (launched bands, alphabetical order)               */
