@@ -13,7 +13,8 @@ primitives) through the request server and through the plain loop and reports bo
 (--no-serve skips it).
 
 Other workloads: --workload config2 (200k-triangle periodic canopy, one 2-band solve per step),
---workload castrays (CollisionDetection::castRaysSoA of 1e8 rays over the 5x tiled orchard) and
+--workload castrays (CollisionDetection::castRaysSoA of 1e8 rays over the 5x tiled orchard),
+--workload camera (a 1024x1024 x 64-sample camera image of the same 5x orchard) and
 --workload serve (the serving leg alone).
 """
 from __future__ import annotations
@@ -371,6 +372,103 @@ def bench_castrays(args):
                          "avg_launch_ms": round(kms / args.steps, 3)}}
 
 
+def orchard_camera(s, res, spp):
+    """A camera south of the orchard (at 30 % of its depth beyond the first row), 8 m up, looking north
+    and 25 degrees down with a 60-degree horizontal field of view."""
+    T = s.transforms.reshape(-1, 16)
+    x0, x1, y0, y1 = float(T[:, 3].min()), float(T[:, 3].max()), float(T[:, 7].min()), float(T[:, 7].max())
+    return native.Camera(((x0 + x1) / 2, y0 - 0.3 * (y1 - y0), 8.0), (math.radians(115.0), 0.0), (res, res),
+                         viewplane_length=0.5 / math.tan(math.radians(30.0)), antialiasing_samples=spp)
+
+
+def bench_camera(args):
+    """Config-5 camera: a 1024 x 1024 image at 64 samples per pixel of the 5x tiled orchard, 3 bands,
+    one collimated sun with its solar disk, a Prague sky."""
+    if args.calib_copy_mib:
+        calib_copy(args.calib_copy_mib)
+    s = scenes.orchard(3, tiles=5, jitter=0.05)
+    ctx = native.Context(0)
+    be = native.Backend(ctx)
+    B = 3
+    be.update_geometry(s.transforms, s.types, s.twosided)
+    be.build_acceleration_structure()
+    be.update_materials(np.full((s.P, B), 0.1, np.float32), np.full((s.P, B), 0.05, np.float32))
+    sun = native.sun_direction(40.0, 150.0)
+    be.update_sources([sun])
+    be.zero_radiation_buffers(B)
+    rng = np.random.default_rng(7)
+    # radiation_out of the last scattering pass: synthetic values (the render's work does not depend on them)
+    be.upload_radiation_out(rng.uniform(0, 300, (s.P, B)).astype(np.float32),
+                            rng.uniform(0, 100, (s.P, B)).astype(np.float32))
+    be.upload_source_fluxes(np.array([[400.0, 300.0, 500.0]], np.float32))
+    be.update_sky_model(np.tile(np.array([2.0, 30.0, 1.5, 0.25], np.float32), (B, 1)),
+                        np.array([20.0, 15.0, 30.0], np.float32), sun, np.array([4e3, 3e3, 5e3], np.float32),
+                        math.cos(math.radians(0.27)), np.zeros(B, np.float32), np.zeros(B, np.uint32))
+    cam = orchard_camera(s, args.cam_res, args.cam_spp)
+    nrays = args.cam_res * args.cam_res * args.cam_spp
+    ctx.enable_timing(True)
+    for _ in range(args.warmup):
+        be.launch_camera_rays(cam, seed=1)
+    ctx.set_counting(True)
+    be.launch_camera_rays(cam, seed=1)
+    counts = ctx.counters()["camera"]
+    ctx.set_counting(False)
+    device_sync()
+    t0 = time.perf_counter()
+    kms = 0.0
+    for k in range(args.steps):
+        be.launch_camera_rays(cam, seed=2 + k)
+        kms += ctx.timing()["direct_ms"]
+    device_sync()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    # pixel labels of the same view
+    be.zero_camera_pixel_buffers(cam.resolution)
+    t1 = time.perf_counter()
+    be.launch_pixel_label_rays(cam)
+    lab_ms = (time.perf_counter() - t1) * 1e3
+    R = be.get_camera_results(cam.resolution)
+    hit = float((R["pixel_labels"] > 0).mean())
+    brays = bytes_per_ray(counts)
+    ach = brays * nrays / (kms / args.steps / 1e3) / 1e9
+    res = {"metric": "camera Mrays/s", "value": round(nrays / (ms / 1e3) / 1e6, 1), "unit": "Mrays/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 2), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"config5 camera: {args.cam_res}x{args.cam_res} pixels x {args.cam_spp} samples, "
+                                  f"{B} bands, over the 5x tiled orchard ({s.P} prims); one step = one image",
+                      "pixels_hit": round(hit, 3), "pixel_label_ms": round(lab_ms, 2)},
+           "roofline": {"bound": "hbm", "kernel": "k_camera", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                        "traffic": pmc_traffic("k_camera", profile="pmc_traffic_config5_camera.json"),
+                        "bytes_per_ray": round(brays, 1), "nodes_per_ray": round(counts["nodes"] / counts["rays"], 2),
+                        "prims_per_ray": round(counts["prims"] / counts["rays"], 2),
+                        "avg_launch_ms": round(kms / args.steps, 3)}}
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline_camera(s, cam, args.cpu_seconds)
+    return res
+
+
+def cpu_baseline_camera(s, cam, seconds):
+    """The oracle's camera restatement (one thread) on rows of the same image until about `seconds`."""
+    from oracle.oracle import OracleScene
+    o = OracleScene.from_arrays(s)
+    P, B = s.P, 3
+    top = np.full((P, B), 100.0, np.float32)
+    sky = dict(params=np.tile(np.array([2.0, 30.0, 1.5, 0.25], np.float32), (B, 1)),
+               radiance=np.full(B, 20.0, np.float32), sun=(0.0, 0.0, 1.0), solar=np.zeros(B, np.float32), solar_cos=2.0)
+    fx, fy = cam.resolution
+    rows, t0 = 0, time.perf_counter()
+    out = np.zeros((B, fx * fy))
+    while time.perf_counter() - t0 < seconds and rows < fy:
+        o.camera_pixels(cam, top, top, np.zeros((0, B)), np.zeros((0, 3)), sky, seed=1, tile=(0, rows, fx, 8),
+                        pixels=out)
+        rows += 8
+    dt = time.perf_counter() - t0
+    n = fx * rows * cam.antialiasing_samples
+    return {"value": round(n / dt / 1e6, 4), "unit": "Mrays/s", "cores": 1, "kind": "port",
+            "sample": f"{rows} image rows x {fx} pixels x {cam.antialiasing_samples} samples ({n} rays) of the same "
+                      f"view, oracle camera restatement, {dt:.1f} s"}
+
+
 def bench_serve(args, ctx=None):
     """Config 4: a Poisson stream of single-sun requests over seeded canopies (P log-uniform 10k-2M,
     1-4 bands), served by the request server (resident models, memory-footprint admission, continuous
@@ -455,7 +553,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="diurnal", choices=["diurnal", "config2", "castrays", "serve"])
+    ap.add_argument("--workload", default="diurnal", choices=["diurnal", "config2", "castrays", "camera", "serve"])
+    ap.add_argument("--cam-res", type=int, default=1024)
+    ap.add_argument("--cam-spp", type=int, default=64)
     ap.add_argument("--suns", type=int, default=64)
     ap.add_argument("--plain-suns", type=int, default=4)
     ap.add_argument("--rays", type=int, default=100_000_000)
@@ -487,6 +587,8 @@ def main():
                "serving": res}
     elif args.workload == "config2":
         res = bench_config2(args)
+    elif args.workload == "camera":
+        res = bench_camera(args)
     else:
         res = bench_castrays(args)
     print(json.dumps(res))

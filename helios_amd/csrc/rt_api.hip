@@ -279,8 +279,8 @@ int rt_get_launch_cfg(rt_ctx *ctx, int kernel, rt_launch_cfg *cfg) {
 int rt_set_counting(rt_ctx *ctx, int on) {
     NN(ctx, "ctx");
     CKR(hipSetDevice(ctx->device));
-    if (!ctx->sx.stats) CKR(hipMalloc(&ctx->sx.stats, 9 * sizeof(unsigned long long)));
-    CKR(hipMemset(ctx->sx.stats, 0, 9 * sizeof(unsigned long long)));
+    if (!ctx->sx.stats) CKR(hipMalloc(&ctx->sx.stats, 12 * sizeof(unsigned long long)));
+    CKR(hipMemset(ctx->sx.stats, 0, 12 * sizeof(unsigned long long)));
     ctx->sx.counting = on != 0;
     return 0;
 }
@@ -289,11 +289,11 @@ int rt_get_counters(rt_ctx *ctx, uint64_t *out) {
     NN(ctx, "ctx");
     NN(out, "out");
     CKR(hipSetDevice(ctx->device));
-    if (!ctx->sx.stats) { for (int k = 0; k < 9; k++) out[k] = 0; return 0; }
+    if (!ctx->sx.stats) { for (int k = 0; k < 12; k++) out[k] = 0; return 0; }
     CKR(hipStreamSynchronize(ctx->sx.stream));
-    unsigned long long h[9];
+    unsigned long long h[12];
     CKR(hipMemcpy(h, ctx->sx.stats, sizeof(h), hipMemcpyDeviceToHost));
-    for (int k = 0; k < 9; k++) out[k] = h[k];
+    for (int k = 0; k < 12; k++) out[k] = h[k];
     return 0;
 }
 

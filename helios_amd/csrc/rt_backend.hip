@@ -668,7 +668,22 @@ int rt_launch_camera_rays(rt_ctx *ctx, const rt_camera_params *p) {
     if (S) CKR(hipMemcpyAsync(d_pos, be.src_dir.data(), (size_t)S * 12, hipMemcpyHostToDevice, st));
     c.src_pos = d_pos;
     CKR(hipStreamSynchronize(st));
-    if (launch_camera(c, be.scene.h.depth, false, err, st)) return -1;
+    hipEvent_t ea = nullptr, eb = nullptr;
+    if (ctx->sx.timing) { CKR(hipEventCreate(&ea)); CKR(hipEventCreate(&eb)); CKR(hipEventRecord(ea, st)); }
+    if (launch_camera(c, be.scene.h.depth, false, err, ctx->sx.stat_ptr(3), st)) return -1;
+    if (eb) {
+        CKR(hipEventRecord(eb, st));
+        CKR(hipEventSynchronize(eb));
+        float ms = 0.f;
+        CKR(hipEventElapsedTime(&ms, ea, eb));
+        (void)hipEventDestroy(ea);
+        (void)hipEventDestroy(eb);
+        // camera kernel time reported in the first slot, as for castRays
+        ctx->sx.timing_info = rt_timing{};
+        ctx->sx.timing_info.direct_ms = ms;
+        ctx->sx.timing_info.direct_rays = (uint64_t)c.res_x * c.res_y * c.spp;
+        ctx->sx.timing_info.direct_launches = 1;
+    }
     return check_err(ctx, err);
 }
 
@@ -708,7 +723,7 @@ int rt_launch_pixel_label_rays(rt_ctx *ctx, const rt_camera_params *p) {
     CKR(hipMemsetAsync(err, 0, sizeof(int), st));
     c.labels = (uint32_t *)be.lab;
     c.depths = (float *)((uint32_t *)be.lab + npix);
-    if (launch_camera(c, be.scene.h.depth, true, err, st)) return -1;
+    if (launch_camera(c, be.scene.h.depth, true, err, nullptr, st)) return -1;
     return check_err(ctx, err);
 }
 

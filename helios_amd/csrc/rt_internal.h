@@ -52,7 +52,8 @@ int launch_diffuse(const DiffuseJob *job_dev, uint32_t total_items, uint32_t dep
 int launch_gather(const GatherJob *jobs_dev, uint32_t njobs, uint32_t total_items, uint32_t cmax, uint32_t *counter,
                   const LaunchCfg &cfg, int ncu, hipStream_t st);
 // Camera pixels (labels = false) or pixel labels/depths (labels = true) of one tile.
-int launch_camera(const CameraLaunch &c, uint32_t depth, bool labels, int *err, hipStream_t st);
+int launch_camera(const CameraLaunch &c, uint32_t depth, bool labels, int *err, unsigned long long *stats,
+                  hipStream_t st);
 // counter: device scratch for the work queue (zeroed by the launch)
 int launch_cast(const DevScene *sc, uint32_t depth, uint64_t n, const float *o, const float *d, float maxd, float *t,
                 float *nrm, uint32_t *uuid, unsigned long long *counter, int *err, unsigned long long *stats,

@@ -54,9 +54,10 @@ struct SolveCtx {
     std::vector<EvRec> ev;
     rt_timing timing_info{};
     std::vector<double> energy;
-    // traversal counters: [0..2] direct, [3..5] diffuse, [6..8] castRays (node fetches, prim tests, rays)
+    // traversal counters: [0..2] direct, [3..5] diffuse, [6..8] castRays, [9..11] camera
+    // (node fetches, prim tests, rays)
     bool counting = false;
-    unsigned long long *stats = nullptr;  // device, 9 counters
+    unsigned long long *stats = nullptr;  // device, 12 counters
     bool use_grid = true;                 // occlusion grids for direct passes of non-periodic scenes
     bool last_direct_grid = false;        // the last solve's direct pass used them
     uint32_t last_grid_entries = 0;       // their list entries per direction per primitive, x100

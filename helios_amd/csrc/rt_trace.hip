@@ -1552,15 +1552,72 @@ __global__ __launch_bounds__(256) void k_cast(const DevScene *__restrict__ scp, 
 // ------------------------------------------------------------------------------------------------
 // Closest hit of a camera ray with up to 10 periodic wraps (camera_raygen.comp:266-285); a ray still
 // crossing faces after the 10th trace counts as a miss.  depth accumulates the distance of the wraps.
-template <int STK>
+// trace_closest() with radiation predicates and a hybrid stack: entries below LSTK in LDS (stride
+// 256), deeper ones in the private array ovf.
+template <int LSTK, int STK, typename C>
+__device__ __forceinline__ uint32_t trace_closest_hyb(const DevScene &sc, v3f o, v3f d, float tmin, float &t_out,
+                                                      lds_u32 *stk, uint32_t *ovf, int *err, C &cnt) {
+    float best_t = 1e30f;
+    uint32_t best = RT_MISS, best_slot = RT_MISS;
+    const float4 *nodes = sc.node, *recs = sc.rec;
+    const uint32_t nleaf = sc.nleaf;
+    if (nleaf <= 1) {
+        if (nleaf == 1) { cnt.prims++; test_leaf<false>(recs, 0, o, d, tmin, RT_MISS, best_t, best, best_slot); }
+        t_out = best_t;
+        return best;
+    }
+    RayPre rp = ray_pre(o, d);
+    uint32_t ni = 0;
+    int sp = 0;
+    uint32_t budget = nleaf + 8;
+    while (true) {
+        if (budget-- == 0 || ni >= nleaf - 1) {
+            atomicOr(err, 2);
+            break;
+        }
+        cnt.nodes++;
+        NodeTest nt = node_test_p(nodes, ni, rp, best_t);
+        bool l0 = (nt.c0 & RT_LEAF_BIT) != 0, l1 = (nt.c1 & RT_LEAF_BIT) != 0;
+        if (nt.h0 && l0) { cnt.prims++; test_leaf<false>(recs, nt.c0 & ~RT_LEAF_BIT, o, d, tmin, RT_MISS, best_t, best, best_slot); }
+        if (nt.h1 && l1) { cnt.prims++; test_leaf<false>(recs, nt.c1 & ~RT_LEAF_BIT, o, d, tmin, RT_MISS, best_t, best, best_slot); }
+        bool g0 = nt.h0 && !l0, g1 = nt.h1 && !l1;
+        if (g0 && g1) {
+            bool first0 = nt.tn0 <= nt.tn1;
+            if (sp >= STK) {
+                atomicOr(err, 1);
+                break;
+            }
+            const uint32_t far = first0 ? nt.c1 : nt.c0;
+            if (sp < LSTK) stk[(uint32_t)sp * 256u] = far;
+            else ovf[sp - LSTK] = far;
+            sp++;
+            ni = first0 ? nt.c0 : nt.c1;
+        } else if (g0) {
+            ni = nt.c0;
+        } else if (g1) {
+            ni = nt.c1;
+        } else {
+            if (sp == 0) break;
+            sp--;
+            ni = sp < LSTK ? stk[(uint32_t)sp * 256u] : ovf[sp - LSTK];
+        }
+    }
+    t_out = best_t;
+    return best;
+}
+
+#ifndef RT_CAMERA_LDS_STACK
+#define RT_CAMERA_LDS_STACK 16
+#endif
+
+template <int STK, typename CT>
 __device__ __forceinline__ uint32_t camera_trace(const DevScene &sc, v3f &o, v3f d, float &t, float &depth,
-                                                 uint32_t *stk, int *err) {
-    NoCnt cnt{};
+                                                 lds_u32 *stk, uint32_t *ovf, int *err, CT &cnt) {
+    constexpr int LSTK = STK < RT_CAMERA_LDS_STACK ? STK : RT_CAMERA_LDS_STACK;
     uint32_t hit = RT_MISS;
     depth = 0.0f;
     for (int wrap = 0; wrap < 10; ++wrap) {
-        uint32_t slot;
-        hit = trace_closest<false, STK>(sc, o, d, 1e-5f, RT_MISS, 1e30f, t, slot, stk, 256, err, cnt);
+        hit = trace_closest_hyb<LSTK, STK>(sc, o, d, 1e-5f, t, stk, ovf, err, cnt);
         if (sc.nbbox == 0) break;
         const float bt = nearest_face(sc, o, d, t);
         if (!(bt > 0.f)) break;
@@ -1569,6 +1626,16 @@ __device__ __forceinline__ uint32_t camera_trace(const DevScene &sc, v3f &o, v3f
         hit = RT_MISS;
     }
     return hit;
+}
+
+// Pixel of this thread: each wave takes an 8 x 8 block of the tile (blocks row by row), so the rays of
+// a wave leave through neighbouring pixels and traverse much the same nodes.
+__device__ __forceinline__ bool camera_pixel(const CameraLaunch &c, uint32_t &px, uint32_t &py) {
+    const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t blk = q >> 6, lane = q & 63u, bx = (c.res_x + 7u) >> 3;
+    px = (blk % bx) * 8u + (lane & 7u);
+    py = (blk / bx) * 8u + (lane >> 3);
+    return px < c.res_x && py < c.res_y;
 }
 
 // Prague sky pattern (sky_distribution.glsl:60-90), or 1 (isotropic) when its normalisation is not positive.
@@ -1616,18 +1683,20 @@ __device__ __forceinline__ float cam_source_radiance(const CameraLaunch &c, v3f 
 // Camera pixels: one thread per pixel of the tile traces its antialiasing samples in order and adds
 // their contributions to camera_radiation[b][pixel] (a fixed order, so a render is reproducible; the
 // reference adds the samples with float atomics).  No specular term (this backend rejects it).
-template <int STK>
-__global__ __launch_bounds__(256) void k_camera(CameraLaunch c, int *err) {
-    __shared__ uint32_t stk_all[STK * 256];
-    uint32_t *stk = stk_all + threadIdx.x;
-    const uint32_t q = blockIdx.x * 256u + threadIdx.x;
-    if (q >= c.res_x * c.res_y) return;
+template <int STK, bool COUNT>
+__global__ __launch_bounds__(256) void k_camera(CameraLaunch c, int *err, unsigned long long *stats) {
+    constexpr int LSTK = STK < RT_CAMERA_LDS_STACK ? STK : RT_CAMERA_LDS_STACK;
+    __shared__ uint32_t stk_all[LSTK * 256];
+    uint32_t ovf[STK > LSTK ? RT_STACK - LSTK : 1];
+    lds_u32 *stk = (lds_u32 *)stk_all + threadIdx.x;
+    uint32_t px, py;
+    if (!camera_pixel(c, px, py)) return;
     const DevScene &sc = *c.sc;
-    const uint32_t px = q % c.res_x, py = q / c.res_x;
     const uint32_t ii = c.off_x + px, jj = c.off_y + py;
     const size_t pix = (size_t)jj * c.full_x + ii, npix = (size_t)c.full_x * c.full_y;
     const float strength = rdiv(1.0f, (float)c.spp);
     const uint32_t dimx = c.res_x * c.res_y, indx = c.res_x * py + px;
+    typename std::conditional<COUNT, TCnt, NoCnt>::type cnt{};
     for (uint32_t smp = 0; smp < c.spp; smp++) {
         uint32_t st = tea16(indx + dimx * smp, c.seed);
         const float Rx = rnd24(st), Ry = rnd24(st);
@@ -1637,7 +1706,7 @@ __global__ __launch_bounds__(256) void k_camera(CameraLaunch c, int *err) {
         camera_ray(c.pos[0], c.pos[1], c.pos[2], c.st, c.ct, c.sp, c.cp, c.focal, c.lens, c.aspect, c.viewplane, ii,
                    jj, c.full_x, c.full_y, Rx, Ry, lx, ly, o, d);
         float t, depth;
-        const uint32_t h = camera_trace<STK>(sc, o, d, t, depth, stk, err);
+        const uint32_t h = camera_trace<STK>(sc, o, d, t, depth, stk, ovf, err, cnt);
         if (h != RT_MISS) {
             const float4 n4 = sc.nrm[h];
             const bool face = n4.x * d.x + n4.y * d.y + n4.z * d.z < 0.0f;  // true: top face hit
@@ -1657,25 +1726,28 @@ __global__ __launch_bounds__(256) void k_camera(CameraLaunch c, int *err) {
             }
         }
     }
+    add_stats(stats, cnt, c.spp);
 }
 
 // Pixel labels (pixel_label_raygen.comp:80-155): the pixel-centre pinhole ray's hit UUID + 1 (0 = sky) and
 // its depth along the view direction (-1 on a miss).
 template <int STK>
 __global__ __launch_bounds__(256) void k_pixel_label(CameraLaunch c, int *err) {
-    __shared__ uint32_t stk_all[STK * 256];
-    uint32_t *stk = stk_all + threadIdx.x;
-    const uint32_t q = blockIdx.x * 256u + threadIdx.x;
-    if (q >= c.res_x * c.res_y) return;
+    constexpr int LSTK = STK < RT_CAMERA_LDS_STACK ? STK : RT_CAMERA_LDS_STACK;
+    __shared__ uint32_t stk_all[LSTK * 256];
+    uint32_t ovf[STK > LSTK ? RT_STACK - LSTK : 1];
+    lds_u32 *stk = (lds_u32 *)stk_all + threadIdx.x;
+    uint32_t px, py;
+    if (!camera_pixel(c, px, py)) return;
     const DevScene &sc = *c.sc;
-    const uint32_t px = q % c.res_x, py = q / c.res_x;
     const uint32_t ii = c.off_x + px, jj = c.off_y + py;
     const size_t pix = (size_t)jj * c.full_x + ii;
     v3f o, d;
     camera_ray(c.pos[0], c.pos[1], c.pos[2], c.st, c.ct, c.sp, c.cp, c.focal, 0.0f, c.aspect, c.viewplane, ii, jj,
                c.full_x, c.full_y, 0.5f, 0.5f, 0.0f, 0.0f, o, d);
     float t, depth;
-    const uint32_t h = camera_trace<STK>(sc, o, d, t, depth, stk, err);
+    NoCnt cnt{};
+    const uint32_t h = camera_trace<STK>(sc, o, d, t, depth, stk, ovf, err, cnt);
     if (h != RT_MISS) {
         c.labels[pix] = sc.uuid[h] + 1u;
         const v3f fwd = camera_rotate(mk3(1.0f, 0.0f, 0.0f), c.st, c.ct, c.sp, c.cp);
@@ -1895,14 +1967,20 @@ int launch_gather(const GatherJob *jobs, uint32_t njobs, uint32_t total, uint32_
     return 0;
 }
 
-int launch_camera(const CameraLaunch &c, uint32_t depth, bool labels, int *err, hipStream_t st) {
-    const uint64_t n = (uint64_t)c.res_x * c.res_y;
-    if (n == 0) return 0;
-    const uint32_t g = (uint32_t)((n + 255) / 256);
+int launch_camera(const CameraLaunch &c, uint32_t depth, bool labels, int *err, unsigned long long *stats,
+                  hipStream_t st) {
+    if ((uint64_t)c.res_x * c.res_y == 0) return 0;
+    // one wave per 8 x 8 pixel block, four waves per workgroup
+    const uint64_t blocks = (uint64_t)((c.res_x + 7) / 8) * ((c.res_y + 7) / 8);
+    const uint32_t g = (uint32_t)((blocks + 3) / 4);
     if (labels)
         PICK_STK(depth, (k_pixel_label<32><<<g, 256, 0, st>>>(c, err)), (k_pixel_label<64><<<g, 256, 0, st>>>(c, err)));
+    else if (stats)
+        PICK_STK(depth, (k_camera<32, true><<<g, 256, 0, st>>>(c, err, stats)),
+                 (k_camera<64, true><<<g, 256, 0, st>>>(c, err, stats)));
     else
-        PICK_STK(depth, (k_camera<32><<<g, 256, 0, st>>>(c, err)), (k_camera<64><<<g, 256, 0, st>>>(c, err)));
+        PICK_STK(depth, (k_camera<32, false><<<g, 256, 0, st>>>(c, err, nullptr)),
+                 (k_camera<64, false><<<g, 256, 0, st>>>(c, err, nullptr)));
     LCHECK();
     return 0;
 }

@@ -289,12 +289,12 @@ class Context:
         _check(self.lib.rt_set_counting(self._h, 1 if on else 0))
 
     def counters(self) -> dict:
-        """BVH node fetches, primitive tests and rays of the direct, diffuse and castRays kernels."""
-        out = (C.c_uint64 * 9)()
+        """BVH node fetches, primitive tests and rays of the direct, diffuse, castRays and camera kernels."""
+        out = (C.c_uint64 * 12)()
         _check(self.lib.rt_get_counters(self._h, out))
         v = list(out)
         return {k: {"nodes": v[3 * i], "prims": v[3 * i + 1], "rays": v[3 * i + 2]}
-                for i, k in enumerate(("direct", "diffuse", "cast"))}
+                for i, k in enumerate(("direct", "diffuse", "cast", "camera"))}
 
 
 class Scene:

@@ -16,7 +16,7 @@ import re
 import sqlite3
 import sys
 
-RAY_KERNELS = ("k_direct", "k_diffuse", "k_gather", "k_cast")
+RAY_KERNELS = ("k_direct", "k_diffuse", "k_gather", "k_cast", "k_camera", "k_pixel_label")
 
 
 def short(name: str) -> str:

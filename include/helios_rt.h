@@ -329,7 +329,8 @@ typedef struct {
 int rt_autotune(rt_ctx *ctx, const rt_scene *scene, const rt_band_request *req, uint32_t nsuns, rt_tune_result *out);
 
 /* ---------------------------------------------------------------------------------------------
- * Timing of the last solve's kernels (HIP events on the context stream), milliseconds.
+ * Timing of the last solve's kernels (HIP events on the context stream), milliseconds.  After a
+ * castRays or camera launch, direct_ms / direct_rays hold that kernel's time and rays.
  * ------------------------------------------------------------------------------------------- */
 typedef struct {
     double direct_ms;   /* direct trace kernels            */
@@ -348,9 +349,9 @@ int rt_enable_timing(rt_ctx *ctx, int on);
  * not periodic and they fit, 0 = always traverse the BVH.  Results are identical either way; grids cost
  * memory (about 100 B per primitive per direction) and are faster. */
 int rt_set_direct_grids(rt_ctx *ctx, int on);
-/* Traversal work counters (zeroed by rt_set_counting): out[9] = direct (BVH node fetches, primitive
- * tests, rays), diffuse/scatter (same three), castRays (same three).  They define the counted
- * algorithmic bytes per ray used for the roofline. */
+/* Traversal work counters (zeroed by rt_set_counting): out[12] = direct (BVH node fetches, primitive
+ * tests, rays), diffuse/scatter (same three), castRays (same three), camera (same three).  They
+ * define the counted algorithmic bytes per ray used for the roofline. */
 int rt_set_counting(rt_ctx *ctx, int on);
 int rt_get_counters(rt_ctx *ctx, uint64_t *out);
 int rt_get_timing(rt_ctx *ctx, rt_timing *out);
