@@ -106,6 +106,16 @@ int rt_scene_destroy(rt_ctx *ctx, rt_scene *scene) {
     return 0;
 }
 
+int rt_scene_set_masks(rt_ctx *ctx, rt_scene *scene, uint32_t nmasks, const uint8_t *texels, const uint32_t *sizes,
+                       const int32_t *mask_ids, const float *uv, const int32_t *uv_ids) {
+    NN(ctx, "ctx");
+    NN(scene, "scene");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    CKR(hipSetDevice(ctx->device));
+    CKR(hipStreamSynchronize(ctx->sx.stream));
+    return scene_set_masks(scene->s, nmasks, texels, sizes, mask_ids, uv, uv_ids, ctx->sx.stream);
+}
+
 int rt_scene_get_info(const rt_scene *scene, rt_scene_info *info) {
     NN(scene, "scene");
     NN(info, "info");

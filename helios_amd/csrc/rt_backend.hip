@@ -160,6 +160,28 @@ int rt_update_geometry(rt_ctx *ctx, uint32_t nprims, const float *transforms, co
     be.gpx = periodic_x ? 1 : 0;
     be.gpy = periodic_y ? 1 : 0;
     be.geometry_staged = true;
+    // new geometry carries no texture masks until rt_update_texture_masks stages them
+    be.nmasks = 0;
+    be.mtex.clear(); be.msize.clear(); be.mid.clear(); be.muv.clear(); be.muvid.clear();
+    return 0;
+}
+
+int rt_update_texture_masks(rt_ctx *ctx, uint32_t nprims, uint32_t nmasks, const uint8_t *texels,
+                            const uint32_t *sizes, const int32_t *mask_ids, const float *uv, const int32_t *uv_ids) {
+    NN(ctx, "ctx");
+    if (nmasks) { NN(texels, "texels"); NN(sizes, "sizes"); NN(mask_ids, "mask_ids"); }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    BackendState &be = ctx->be;
+    be.nmasks = nmasks;
+    size_t ntex = 0;
+    for (uint32_t m = 0; m < nmasks; m++) ntex += (size_t)sizes[2 * m] * sizes[2 * m + 1];
+    be.mtex.assign(texels, texels + (nmasks ? ntex : 0));
+    be.msize.assign(sizes, sizes + 2 * (size_t)nmasks);
+    be.mid.assign(mask_ids, mask_ids + (nmasks ? nprims : 0));
+    be.muv.clear();
+    be.muvid.clear();
+    if (nmasks && uv) be.muv.assign(uv, uv + (size_t)nprims * 8);
+    if (nmasks && uv_ids) be.muvid.assign(uv_ids, uv_ids + nprims);
     return 0;
 }
 
@@ -176,6 +198,17 @@ int rt_build_acceleration_structure(rt_ctx *ctx) {
                     ctx->sx.stream))
         return -1;
     be.has_scene = true;
+    if (be.nmasks) {
+        if (be.mid.size() != be.gP) {
+            set_error("rt_build_acceleration_structure: texture masks were staged for " + std::to_string(be.mid.size()) +
+                      " primitives, the geometry has " + std::to_string(be.gP));
+            return -1;
+        }
+        if (scene_set_masks(be.scene, be.nmasks, be.mtex.data(), be.msize.data(), be.mid.data(),
+                            be.muv.empty() ? nullptr : be.muv.data(), be.muvid.empty() ? nullptr : be.muvid.data(),
+                            ctx->sx.stream))
+            return -1;
+    }
     if (be.P != be.gP) {
         // per-primitive state of the previous geometry no longer applies
         be.has_materials = false;

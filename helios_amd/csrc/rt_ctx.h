@@ -18,6 +18,12 @@ struct BackendState {
     std::vector<uint32_t> gtype, guuid, gtwos;
     int gpx = 0, gpy = 0;
     bool geometry_staged = false;
+    // texture masks staged by rt_update_texture_masks, applied by the next build
+    uint32_t nmasks = 0;
+    std::vector<uint8_t> mtex;
+    std::vector<uint32_t> msize;
+    std::vector<int32_t> mid, muvid;
+    std::vector<float> muv;
 
     rt::Scene scene;
     bool has_scene = false;

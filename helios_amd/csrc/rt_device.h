@@ -101,15 +101,20 @@ RT_HD void sincos_2pi(float r, float &s, float &c) {
 //   r2 = b|e2 .xyz, |b|^2
 //   r3 = patch plane normal .xyz, 0
 #define RT_PATCH_BIT 0x80000000u
+// A primitive with a texture mask carries RT_MASK_BIT in its tag and, in r3.w, the float4 index (from the
+// record base) of its mask descriptor; the index bits of a tag are RT_ID_BITS.
+#define RT_MASK_BIT 0x40000000u
+#define RT_ID_BITS 0x3FFFFFFFu
 
 // Radiation predicates (intersections.glsl) and CD predicates share the arithmetic; only the
 // acceptance thresholds differ.  Written as predicates (every quantity computed, the tests combined at the
 // end) rather than early returns: same floating-point operations and decisions, but no divergent control
 // flow, which on a wave costs exec-mask bookkeeping per test.  t_out is written in every case and is
 // meaningful only when the function returns true.
+// ua, ub: the hit's d.a and d.b (patch) or barycentric u, v (triangle), for texture masks.
 template <bool CD>
-RT_HD bool isect_rec(const float4 r0, const float4 r1, const float4 r2, const float4 r3, bool patch, v3f o, v3f d,
-                     float &t_out) {
+RT_HD bool isect_rec_uv(const float4 r0, const float4 r1, const float4 r2, const float4 r3, bool patch, v3f o, v3f d,
+                        float &t_out, float &ua, float &ub) {
     v3f v0 = mk3(r0.x, r0.y, r0.z);
     v3f e1 = mk3(r1.x, r1.y, r1.z);
     v3f e2 = mk3(r2.x, r2.y, r2.z);
@@ -126,6 +131,8 @@ RT_HD bool isect_rec(const float4 r0, const float4 r1, const float4 r2, const fl
         if (CD) in = (da >= -1e-8f && da <= r1.w + 1e-8f && db >= -1e-8f && db <= r2.w + 1e-8f);
         else in = (da > 0.0f && da < r1.w && db > 0.0f && db < r2.w);
         t_out = t;
+        ua = da;
+        ub = db;
         return ok && in;
     } else {
         const float EPS = CD ? 1e-5f : 1e-8f;
@@ -142,8 +149,71 @@ RT_HD bool isect_rec(const float4 r0, const float4 r1, const float4 r2, const fl
         float t = f * dot3(e2, q);
         ok = ok && (CD ? (t > EPS) : (t > 1e-8f && t < 1e8f));
         t_out = t;
+        ua = u;
+        ub = v;
         return ok;
     }
+}
+
+template <bool CD>
+RT_HD bool isect_rec(const float4 r0, const float4 r1, const float4 r2, const float4 r3, bool patch, v3f o, v3f d,
+                     float &t_out) {
+    float ua, ub;
+    return isect_rec_uv<CD>(r0, r1, r2, r3, patch, o, d, t_out, ua, ub);
+}
+
+// ---- texture masks (R/shaders/common/texture_mask.glsl:20-66) ----
+// Descriptor of a masked primitive, three float4 at the record base + its index:
+//   q0 = bits(texel byte offset from the record base, width, height, mode), mode 0 = patch with default
+//        UVs, 1 = patch with custom UVs, 2 = triangle
+//   q1 = uv0.xy, uv1.xy;  q2 = uv2.xy, uv3.xy
+// Texels are one byte each, row-major from the top row; 0 = transparent.
+RT_HD bool mask_texel(const float4 *base, float4 q0, int tx, int ty) {
+    const uint32_t off = __float_as_uint(q0.x);
+    const int sx = (int)__float_as_uint(q0.y), sy = (int)__float_as_uint(q0.z);
+    tx = tx < 0 ? 0 : (tx > sx - 1 ? sx - 1 : tx);
+    ty = ty < 0 ? 0 : (ty > sy - 1 ? sy - 1 : ty);
+    const unsigned char *tex = (const unsigned char *)base + off;
+    return tex[(size_t)ty * (size_t)sx + (size_t)tx] != 0;
+}
+
+// float -> int as GLSL int(): truncation, saturated (clamped to the texture right after)
+RT_HD int mask_int(float x) { return x >= 2147483520.0f ? 2147483520 : (x > -2147483520.0f ? (int)x : -2147483520); }
+
+// check_texture_mask for a patch at parametric (u, v) in [0,1]^2
+RT_HD bool mask_patch_uv(const float4 *base, float4 q0, float4 q1, float4 q2, float u, float v) {
+    const float fx = (float)((int)__float_as_uint(q0.y) - 1), fy = (float)((int)__float_as_uint(q0.z) - 1);
+    int tx, ty;
+    if (__float_as_uint(q0.w) == 0u) {
+        tx = mask_int(floorf(fx * u));
+        ty = mask_int(floorf(fy * (1.0f - v)));
+    } else {
+        const float du = q1.z - q1.x;   // uv1.x - uv0.x
+        const float dv = q2.y - q1.w;   // uv2.y - uv1.y
+        tx = mask_int(floorf(fx * (q1.x + u * du)));
+        ty = mask_int(floorf(fy * ((1.0f - q1.y) - v * dv)));
+    }
+    return mask_texel(base, q0, tx, ty);
+}
+
+// check_texture_mask for a triangle at barycentric (u, v): interpolated UVs, flipped v, rounded
+RT_HD bool mask_tri_uv(const float4 *base, float4 q0, float4 q1, float4 q2, float u, float v) {
+    const float fx = (float)((int)__float_as_uint(q0.y) - 1), fy = (float)((int)__float_as_uint(q0.z) - 1);
+    const float tu = (q1.x + u * (q1.z - q1.x)) + v * (q2.x - q1.x);
+    const float tv = 1.0f - ((q1.y + u * (q1.w - q1.y)) + v * (q2.y - q1.y));
+    return mask_texel(base, q0, mask_int(roundf(fx * fabsf(tu))), mask_int(roundf(fy * fabsf(tv))));
+}
+
+// Mask test of an accepted intersection: patch (ua, ub) = (d.a, d.b) -> u = d.a / |a|^2 with |a| the
+// rounded length (intersections.glsl:64-66); triangle (ua, ub) = barycentric.
+RT_HD bool mask_hit_opaque(const float4 *base, uint32_t didx, bool patch, float ua, float ub, float aa, float bb) {
+    const float4 *q = base + didx;
+    const float4 q0 = q[0], q1 = q[1], q2 = q[2];
+    if (patch) {
+        const float am = rsqrt_cr(aa), bm = rsqrt_cr(bb);
+        return mask_patch_uv(base, q0, q1, q2, rdiv(ua, am * am), rdiv(ub, bm * bm));
+    }
+    return mask_tri_uv(base, q0, q1, q2, ua, ub);
 }
 
 // Direct-pass sample point in primitive-local coordinates (direct_raygen.comp:360-428)
@@ -336,6 +406,39 @@ RT_HD bool cam_planar_visible(bool disk, v3f o, v3f d, v3f c, const float *width
     return in;
 }
 
+// Origin rejection against a texture mask (direct_raygen.comp:431-473, diffuse_raygen.comp:411-455),
+// continuing the ray's RNG stream after its two sample draws.  Patch: while the sample's texel is
+// transparent, up to 10 redraws of a uniform point of the whole patch (one subdivision); triangle: up to 50
+// draws of a point of the canonical triangle, looked up with floor() and no abs().  Returns false when the
+// ray is dropped: a triangle whose 50 draws all fell on transparent texels, or (diffuse rays only) a patch
+// whose last draw did; direct patch rays keep their last draw.
+RT_HD bool origin_mask(const float4 *base, uint32_t didx, bool patch, bool diffuse, uint32_t &st, float &spx,
+                       float &spy) {
+    const float4 *q = base + didx;
+    const float4 q0 = q[0], q1 = q[1], q2 = q[2];
+    if (patch) {
+        bool ok = mask_patch_uv(base, q0, q1, q2, spx + 0.5f, spy + 0.5f);
+        for (int k = 0; k < 10 && !ok; k++) {
+            spx = -0.5f + rnd24(st);
+            spy = -0.5f + rnd24(st);
+            ok = mask_patch_uv(base, q0, q1, q2, spx + 0.5f, spy + 0.5f);
+        }
+        return ok || !diffuse;
+    }
+    const float fx = (float)((int)__float_as_uint(q0.y) - 1), fy = (float)((int)__float_as_uint(q0.z) - 1);
+    bool masked = true;
+    for (int a = 0; a < 50 && masked; a++) {
+        float r1 = rnd24(st), r2 = rnd24(st);
+        if (r1 + r2 > 1.0f) { r1 = 1.0f - r1; r2 = 1.0f - r2; }
+        spx = r2;
+        spy = r1 + r2;
+        const float tu = (q1.x + r1 * (q1.z - q1.x)) + r2 * (q2.x - q1.x);
+        const float tv = 1.0f - ((q1.y + r1 * (q1.w - q1.y)) + r2 * (q2.y - q1.y));
+        masked = !mask_texel(base, q0, mask_int(floorf(fx * tu)), mask_int(floorf(fy * tv)));
+    }
+    return !masked;
+}
+
 // Diffuse-pass sample point and local hemisphere direction (diffuse_raygen.comp:375-482)
 RT_HD void diffuse_sample(uint32_t prim, uint32_t lx, uint32_t ly, uint32_t n, uint32_t seed, bool patch, float &spx,
                           float &spy, v3f &dl) {
@@ -351,6 +454,26 @@ RT_HD void diffuse_sample(uint32_t prim, uint32_t lx, uint32_t ly, uint32_t n, u
     float sp, cp;
     sincos_2pi(Rp, sp, cp);
     dl = mk3(sth * cp, sth * sp, cth);
+}
+
+// diffuse_sample() for a primitive that may carry a texture mask (descriptor didx, RT_MISS = none): the
+// origin rejection draws come between the origin and direction draws.  Returns false when the ray is dropped.
+RT_HD bool diffuse_sample_m(uint32_t prim, uint32_t lx, uint32_t ly, uint32_t n, uint32_t seed, bool patch,
+                            const float4 *base, uint32_t didx, float &spx, float &spy, v3f &dl) {
+    uint32_t st = tea16(n * ly + lx + n * n * prim, seed);
+    float Rx = rnd24(st), Ry = rnd24(st);
+    if (patch) { spx = -0.5f + Rx; spy = -0.5f + Ry; }
+    else { if (Rx < Ry) { spx = Rx; spy = Ry; } else { spx = Ry; spy = Rx; } }
+    if (didx != RT_MISS && !origin_mask(base, didx, patch, true, st, spx, spy)) return false;
+    float fn = (float)n;
+    float Rt = rdiv((float)lx + rnd24(st), fn);
+    float Rp = rdiv((float)ly + rnd24(st), fn);
+    float sth = rsqrt_cr(Rt);
+    float cth = rsqrt_cr(1.0f - Rt);
+    float sp, cp;
+    sincos_2pi(Rp, sp, cp);
+    dl = mk3(sth * cp, sth * sp, cth);
+    return true;
 }
 
 // rotate_to_normal with the per-primitive frame (ce, se, ca, sa) = cos/sin of elevation and azimuth

@@ -25,6 +25,12 @@ struct Scene {
 int scene_build(Scene &S, uint32_t P, const float *T, const uint32_t *type, const uint32_t *uuid,
                 const uint32_t *twos, int px, int py, hipStream_t st);
 void scene_free(Scene &S);
+// Texture masks (RayTracingGeometry mask_data / mask_sizes / mask_IDs / uv_data / uv_IDs,
+// R/include/RayTracingTypes.h:72-77): nmasks masks of sizes[m] = (width, height) texels, one byte per texel
+// (0 = transparent) concatenated row-major in texels; mask_id[P] (-1 = none); uv[P][4][2] (null = zeros)
+// and uv_id[P] (null or < 0: default patch UVs).  Replaces the masks of the scene.
+int scene_set_masks(Scene &S, uint32_t nmasks, const uint8_t *texels, const uint32_t *sizes, const int32_t *mask_id,
+                    const float *uv, const int32_t *uv_id, hipStream_t st);
 
 // Launch configuration chosen per kernel (autotuned or default).
 struct LaunchCfg {

@@ -725,6 +725,108 @@ int scene_build(Scene &S, uint32_t P, const float *T, const uint32_t *type, cons
     return 0;
 }
 
+static inline float bits_f(uint32_t u) {
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+__global__ void k_mask_tags(uint32_t nleaf, float4 *__restrict__ rec, const uint32_t *__restrict__ mdesc) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nleaf) return;
+    float4 *r = rec + (size_t)k * 4;
+    uint32_t tag = __float_as_uint(r[0].w) & ~RT_MASK_BIT;
+    const uint32_t m = mdesc ? mdesc[tag & RT_ID_BITS] : RT_MISS;
+    float w = 0.f;
+    if (m != RT_MISS) {
+        tag |= RT_MASK_BIT;
+        w = __uint_as_float(m);
+    }
+    r[0].w = __uint_as_float(tag);
+    r[3].w = w;
+}
+
+int scene_set_masks(Scene &S, uint32_t nmasks, const uint8_t *texels, const uint32_t *sizes, const int32_t *mask_id,
+                    const float *uv, const int32_t *uv_id, hipStream_t st) {
+    const uint32_t P = S.h.P, nleaf = S.h.nleaf;
+    if (!S.d) { set_error("texture masks: no scene"); return -1; }
+    if (P > RT_ID_BITS) { set_error("texture masks: more than 2^30 primitives"); return -1; }
+    if (nmasks && (!texels || !sizes)) { set_error("texture masks: texels and sizes are required"); return -1; }
+    if (nmasks && !mask_id) { set_error("texture masks: mask_id is required"); return -1; }
+    std::vector<uint64_t> toff(nmasks + 1, 0);
+    for (uint32_t m = 0; m < nmasks; m++) {
+        const uint64_t w = sizes[2 * m], h = sizes[2 * m + 1];
+        if (w == 0 || h == 0 || w > 65536 || h > 65536) {
+            set_error("texture masks: mask " + std::to_string(m) + " has size " + std::to_string(w) + "x" +
+                      std::to_string(h) + " (1..65536 texels per side)");
+            return -1;
+        }
+        toff[m + 1] = toff[m] + w * h;
+    }
+    std::vector<uint32_t> type(P);
+    if (P) CK(hipMemcpy(type.data(), S.h.type, sizeof(uint32_t) * P, hipMemcpyDeviceToHost));
+    std::vector<uint32_t> mdesc(P, RT_MISS);
+    std::vector<float4> desc;
+    std::vector<uint32_t> dmask;  // mask of each descriptor
+    const uint64_t rec_f4 = (uint64_t)nleaf * 4;
+    for (uint32_t i = 0; i < P && nmasks; i++) {
+        const int32_t mid = mask_id[i];
+        if (mid < 0) continue;
+        if ((uint32_t)mid >= nmasks) {
+            set_error("texture masks: primitive " + std::to_string(i) + " names mask " + std::to_string(mid) + " of " +
+                      std::to_string(nmasks));
+            return -1;
+        }
+        const uint32_t ty = type[i];
+        if (!(ty == 0u || ty == 1u || ty == 3u)) continue;  // not traced
+        // triangles always interpolate their vertex UVs; patches use them when uv_id >= 0 (UVs read as
+        // zeros when uv is NULL, as the reference's zero-filled per-primitive UV buffer)
+        const uint32_t mode = ty == 1u ? 2u : ((uv_id && uv_id[i] >= 0) ? 1u : 0u);
+        mdesc[i] = (uint32_t)(rec_f4 + desc.size());  // float4 index of this descriptor
+        const float *u = uv ? uv + (size_t)i * 8 : nullptr;
+        desc.push_back(make_float4(0.f, bits_f(sizes[2 * mid]), bits_f(sizes[2 * mid + 1]),
+                                   bits_f(mode)));
+        desc.push_back(u ? make_float4(u[0], u[1], u[2], u[3]) : make_float4(0.f, 0.f, 0.f, 0.f));
+        desc.push_back(u ? make_float4(u[4], u[5], u[6], u[7]) : make_float4(0.f, 0.f, 0.f, 0.f));
+        dmask.push_back((uint32_t)mid);
+    }
+    const uint64_t tex_base = (rec_f4 + desc.size()) * sizeof(float4);
+    const uint64_t total = tex_base + toff[nmasks];
+    if (total >= (1ull << 32)) {
+        set_error("texture masks: records, descriptors and texels exceed 4 GiB");
+        return -1;
+    }
+    for (size_t j = 0; j < dmask.size(); j++) desc[3 * j].x = bits_f((uint32_t)(tex_base + toff[dmask[j]]));
+    if (nleaf == 0) return 0;
+    float4 *R = nullptr;
+    uint32_t *dm = nullptr;
+    CK(hipMalloc(&R, (total + 15) & ~(uint64_t)15));
+    CK(hipMemcpyAsync(R, S.h.rec, rec_f4 * sizeof(float4), hipMemcpyDeviceToDevice, st));
+    if (!desc.empty()) {
+        CK(hipMemcpyAsync(R + rec_f4, desc.data(), desc.size() * sizeof(float4), hipMemcpyHostToDevice, st));
+        CK(hipMemcpyAsync((char *)R + tex_base, texels, toff[nmasks], hipMemcpyHostToDevice, st));
+        CK(hipMalloc(&dm, sizeof(uint32_t) * P));
+        CK(hipMemcpyAsync(dm, mdesc.data(), sizeof(uint32_t) * P, hipMemcpyHostToDevice, st));
+    }
+    k_mask_tags<<<blocks(nleaf, 256), 256, 0, st>>>(nleaf, R, dm);
+    CK(hipGetLastError());
+    CK(hipStreamSynchronize(st));
+    // replace the record buffer and the descriptor map
+    auto drop = [&S](const void *p) {
+        for (size_t k = 0; k < S.own.size(); k++)
+            if (S.own[k] == p) { (void)hipFree(S.own[k]); S.own.erase(S.own.begin() + k); return; }
+    };
+    drop(S.h.rec);
+    if (S.h.mdesc) drop(S.h.mdesc);
+    S.own.push_back(R);
+    if (dm) S.own.push_back(dm);
+    S.h.rec = R;
+    S.h.mdesc = dm;
+    CK(hipMemcpyAsync(S.d, &S.h, sizeof(DevScene), hipMemcpyHostToDevice, st));
+    CK(hipStreamSynchronize(st));
+    return 0;
+}
+
 void scene_free(Scene &S) {
     for (void *p : S.own) (void)hipFree(p);
     S.own.clear();

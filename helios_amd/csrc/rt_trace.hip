@@ -111,24 +111,22 @@ __device__ __forceinline__ bool leaf_hit_p(const float4 *recs, uint32_t k, v3f o
     const float4 *r = recs + (size_t)k * 4;
     float4 r0 = ldc(r);
     uint32_t tag = __float_as_uint(r0.w);
-    id = tag & ~RT_PATCH_BIT;
+    id = tag & RT_ID_BITS;
     bool patch = (tag & RT_PATCH_BIT) != 0;
+    const bool masked = !CD && (tag & RT_MASK_BIT) != 0;
     float4 r1 = ldc(r + 1), r2 = ldc(r + 2);
-    float4 r3 = patch ? ldc(r + 3) : make_float4(0.f, 0.f, 0.f, 0.f);
-    return isect_rec<CD>(r0, r1, r2, r3, patch, o, d, t) && id != skip;
+    float4 r3 = (patch || masked) ? ldc(r + 3) : make_float4(0.f, 0.f, 0.f, 0.f);
+    float ua, ub;
+    bool hit = isect_rec_uv<CD>(r0, r1, r2, r3, patch, o, d, t, ua, ub) && id != skip;
+    // texture mask (bvh_traversal.glsl:143-148): a transparent texel lets the ray pass
+    if (masked && hit) hit = mask_hit_opaque(recs, __float_as_uint(r3.w), patch, ua, ub, r1.w, r2.w);
+    return hit;
 }
 
 template <bool CD>
 __device__ __forceinline__ bool leaf_hit(const DevScene &sc, uint32_t k, v3f o, v3f d, uint32_t skip, uint32_t &id,
                                          float &t) {
-    const float4 *r = sc.rec + (size_t)k * 4;
-    float4 r0 = ldc(r);
-    uint32_t tag = __float_as_uint(r0.w);
-    id = tag & ~RT_PATCH_BIT;
-    bool patch = (tag & RT_PATCH_BIT) != 0;
-    float4 r1 = ldc(r + 1), r2 = ldc(r + 2);
-    float4 r3 = patch ? ldc(r + 3) : make_float4(0.f, 0.f, 0.f, 0.f);
-    return isect_rec<CD>(r0, r1, r2, r3, patch, o, d, t) && id != skip;
+    return leaf_hit_p<CD>(sc.rec, k, o, d, skip, id, t);
 }
 
 template <bool CD>
@@ -675,11 +673,15 @@ __device__ __forceinline__ int grid_pair(const uint32_t *items, const float *key
     const float4 a0 = ldc(q0), a1 = ldc(q0 + 1), a2 = ldc(q0 + 2), a3 = ldc(q0 + 3);
     const float4 b0 = ldc(q1), b1 = ldc(q1 + 1), b2 = ldc(q1 + 2), b3 = ldc(q1 + 3);
     const uint32_t ta = __float_as_uint(a0.w), tb = __float_as_uint(b0.w);
-    float t0, t1;
-    const bool h0 = v0 && isect_rec<false>(a0, a1, a2, a3, (ta & RT_PATCH_BIT) != 0, o, d, t0) &&
-                    (ta & ~RT_PATCH_BIT) != skip && t0 > 1e-4f && t0 <= 1e30f;
-    const bool h1 = v1 && isect_rec<false>(b0, b1, b2, b3, (tb & RT_PATCH_BIT) != 0, o, d, t1) &&
-                    (tb & ~RT_PATCH_BIT) != skip && t1 > 1e-4f && t1 <= 1e30f;
+    float t0, t1, ua0, ub0, ua1, ub1;
+    bool h0 = v0 && isect_rec_uv<false>(a0, a1, a2, a3, (ta & RT_PATCH_BIT) != 0, o, d, t0, ua0, ub0) &&
+              (ta & RT_ID_BITS) != skip && t0 > 1e-4f && t0 <= 1e30f;
+    bool h1 = v1 && isect_rec_uv<false>(b0, b1, b2, b3, (tb & RT_PATCH_BIT) != 0, o, d, t1, ua1, ub1) &&
+              (tb & RT_ID_BITS) != skip && t1 > 1e-4f && t1 <= 1e30f;
+    if (h0 && (ta & RT_MASK_BIT))
+        h0 = mask_hit_opaque(recs, __float_as_uint(a3.w), (ta & RT_PATCH_BIT) != 0, ua0, ub0, a1.w, a2.w);
+    if (h1 && (tb & RT_MASK_BIT))
+        h1 = mask_hit_opaque(recs, __float_as_uint(b3.w), (tb & RT_PATCH_BIT) != 0, ua1, ub1, b1.w, b2.w);
     cnt.prims += (v0 ? 1u : 0u) + ((v1 && !h0) ? 1u : 0u);  // the tests a one-at-a-time scan makes
     if (h0 || h1) return 1;
     return (sorted && (!(key0 >= depth) || (two && !(key1 >= depth)))) ? 2 : 0;
@@ -694,6 +696,15 @@ __device__ __forceinline__ int grid_pair(const uint32_t *items, const float *key
 #ifndef RT_DIRECT_REFILL
 #define RT_DIRECT_REFILL 24
 #endif
+// Origin of direct ray r of primitive i (stratified sample, then the texture-mask rejection when the
+// primitive has a mask: mdi = its descriptor, RT_MISS = none); false when the ray is dropped.
+__device__ __forceinline__ bool direct_origin(const float4 *recs, uint32_t mdi, uint32_t i, uint32_t r, uint32_t d,
+                                              uint32_t seed, bool patch, float &spx, float &spy) {
+    uint32_t st;
+    direct_sample_st(i, r % d, r / d, d, seed, patch, spx, spy, st);
+    return mdi == RT_MISS || origin_mask(recs, mdi, patch, false, st, spx, spy);
+}
+
 template <int WPB, int STK, bool COUNT>
 __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict__ jobs, uint32_t njobs, uint32_t total,
                                                      uint32_t *counter, uint32_t chunk, int *err,
@@ -721,6 +732,8 @@ __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict
             bool top = true;
             const SunGrid *grid = ldu(&J.grid);
             const float4 *recs = ldu(&sc.rec);
+            const uint32_t *mdesc = ldu(&sc.mdesc);
+            const uint32_t mdi = mdesc ? ldu(mdesc + i) : RT_MISS;
             if (traced_type(ty)) {
                 float4 n4 = ldu(ldu(&sc.nrm) + i);
                 v3f n = mk3(n4.x, n4.y, n4.z);
@@ -756,7 +769,7 @@ __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict
                         bool act = r < Nrays;
                         if (act) {
                             float spx, spy;
-                            direct_sample(i, r % d, r / d, d, seed, patch, spx, spy);
+                            act = direct_origin(recs, mdi, i, r, d, seed, patch, spx, spy);
                             o = xform3(tm, spx, spy, 0.0f);
                             grid_cell(G, o, pc, pe);
                             depth = o.x * gw0 + o.y * gw1 + o.z * gw2;
@@ -788,13 +801,12 @@ __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict
                             if (!act && rank < take) {
                                 r = qnext + rank;
                                 float spx, spy;
-                                direct_sample(i, r % d, r / d, d, seed, patch, spx, spy);
+                                act = direct_origin(recs, mdi, i, r, d, seed, patch, spx, spy);
                                 o = xform3(tm, spx, spy, 0.0f);
                                 grid_cell(G, o, pc, pe);
                                 depth = o.x * gw0 + o.y * gw1 + o.z * gw2;
                                 cnt.nodes++;
                                 nray++;
-                                act = true;
                             }
                             qnext += take;
                         }
@@ -805,7 +817,7 @@ __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict
                             v3f o = mk3(0.f, 0.f, 0.f);
                             if (need) {
                                 float spx, spy;
-                                direct_sample(i, r % d, r / d, d, seed, patch, spx, spy);
+                                need = direct_origin(recs, mdi, i, r, d, seed, patch, spx, spy);
                                 o = xform3(tm, spx, spy, 0.0f);
                                 nray++;
                             }
@@ -889,6 +901,8 @@ __global__ __launch_bounds__(256) void k_direct_src(const DirectJob *__restrict_
                 float spx, spy;
                 uint32_t st;
                 direct_sample_st(i, r % d, r / d, d, J.seed, patch, spx, spy, st);
+                const uint32_t mdi = sc.mdesc ? sc.mdesc[i] : RT_MISS;
+                if (mdi != RT_MISS && !origin_mask(sc.rec, mdi, patch, false, st, spx, spy)) continue;
                 const v3f o = xform3(tm, spx, spy, 0.0f);
                 for (uint32_t k = 0; k < J.skip_draws; k++) (void)rnd24(st);
                 nray++;
@@ -1130,6 +1144,7 @@ __global__ __launch_bounds__(WPB * 64) void k_diffuse(const DiffuseJob *__restri
     const bool use4 = RT_DIFFUSE_BVH4 && !COUNT && nodes4 != nullptr && 3u * ldu(&sc.depth4) + 1u <= (uint32_t)STK;
     const float *T = ldu(&sc.T);
     const uint32_t *type = ldu(&sc.type);
+    const uint32_t *mdesc = ldu(&sc.mdesc);
     const uint32_t nleaf = ldu(&sc.nleaf), nbbox = ldu(&sc.nbbox);
     const uint32_t n = ldu(&job->n), nn = n * n, seed = ldu(&job->seed);
     const uint32_t faces = ldu(&job->faces) ? ldu(&job->faces) : 3u;
@@ -1140,7 +1155,7 @@ __global__ __launch_bounds__(WPB * 64) void k_diffuse(const DiffuseJob *__restri
     // (scalar) whenever rays are generated, so they hold no registers across the traversal
     uint32_t it = 0, it_end = 0;
     bool drained = false;
-    uint32_t op_i = 0, op_next = 0, op_cnt = 0, op_base = 0;
+    uint32_t op_i = 0, op_next = 0, op_cnt = 0, op_base = 0, op_m = RT_MISS;
     bool patch = false, flip = false;
     // lane state: busy = holds a ray, trav = its traversal has not ended
     bool busy = false, trav = false;
@@ -1199,6 +1214,7 @@ __global__ __launch_bounds__(WPB * 64) void k_diffuse(const DiffuseJob *__restri
                 if (!s0) for (uint32_t r = lane; r < nn; r += 64) w[r] = RT_NOT_LAUNCHED;
                 if (!s1) for (uint32_t r = nn + lane; r < 2 * nn; r += 64) w[r] = RT_NOT_LAUNCHED;
                 op_i = i;
+                op_m = mdesc ? ldu(mdesc + i) : RT_MISS;
                 op_next = 0;
                 op_cnt = ((s0 ? 1u : 0u) + (s1 ? 1u : 0u)) * nn;
                 op_base = s0 ? 0u : nn;
@@ -1215,22 +1231,27 @@ __global__ __launch_bounds__(WPB * 64) void k_diffuse(const DiffuseJob *__restri
                 const uint32_t rr = r - slot * nn;
                 float spx, spy;
                 v3f dl;
-                diffuse_sample(op_i, rr % n, rr / n, n, seed, patch, spx, spy, dl);
-                float tm[12];
-                const float *Tp = T + (size_t)op_i * 16;
+                if (diffuse_sample_m(op_i, rr % n, rr / n, n, seed, patch, recs, op_m, spx, spy, dl)) {
+                    float tm[12];
+                    const float *Tp = T + (size_t)op_i * 16;
 #pragma unroll
-                for (int q = 0; q < 12; q++) tm[q] = ldu(Tp + q);
-                const float4 fr = ldu(frame + op_i);
-                L.o = xform3(tm, spx, spy, 0.0f);
-                v3f dir = frame_rotate(dl, fr.x, fr.y, fr.z, fr.w);
-                if (slot == 1 && flip) dir = mk3(-dir.x, -dir.y, -dir.z);
-                L.d = dir;
-                my_i = op_i;
-                my_r = r;
-                wraps = 0;
-                nray++;
-                busy = true;
-                trav = lane_begin(L, recs, nleaf, 1e-5f, my_i, cnt);
+                    for (int q = 0; q < 12; q++) tm[q] = ldu(Tp + q);
+                    const float4 fr = ldu(frame + op_i);
+                    L.o = xform3(tm, spx, spy, 0.0f);
+                    v3f dir = frame_rotate(dl, fr.x, fr.y, fr.z, fr.w);
+                    if (slot == 1 && flip) dir = mk3(-dir.x, -dir.y, -dir.z);
+                    L.d = dir;
+                    my_i = op_i;
+                    my_r = r;
+                    wraps = 0;
+                    nray++;
+                    busy = true;
+                    trav = lane_begin(L, recs, nleaf, 1e-5f, my_i, cnt);
+                } else {
+                    // origin on transparent texels: the ray is dropped (adds nothing to the gather); the lane
+                    // stays idle and takes another ray
+                    hits[(size_t)op_i * 2 * nn + r] = RT_NOT_LAUNCHED;
+                }
             }
             op_next += take;
         }
@@ -1777,11 +1798,15 @@ __global__ void k_debug_rays(const DevScene *__restrict__ scp, uint32_t n, uint3
     float4 fr = sc.frame[i];
     float spx, spy;
     v3f dl;
-    diffuse_sample(i, rr % n, rr / n, n, seed, patch, spx, spy, dl);
+    float *w = out + g * 6;
+    const uint32_t mdi = sc.mdesc ? sc.mdesc[i] : RT_MISS;
+    if (!diffuse_sample_m(i, rr % n, rr / n, n, seed, patch, sc.rec, mdi, spx, spy, dl)) {
+        for (int k = 0; k < 6; k++) w[k] = __uint_as_float(0x7FC00000u);  // dropped ray: NaN
+        return;
+    }
     v3f o = xform3(sc.T + (size_t)i * 16, spx, spy, 0.0f);
     v3f dir = frame_rotate(dl, fr.x, fr.y, fr.z, fr.w);
     if (slot == 1 && tw == 1) dir = mk3(-dir.x, -dir.y, -dir.z);
-    float *w = out + g * 6;
     w[0] = o.x; w[1] = o.y; w[2] = o.z; w[3] = dir.x; w[4] = dir.y; w[5] = dir.z;
 }
 

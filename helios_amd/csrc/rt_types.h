@@ -42,6 +42,8 @@ struct DevScene {
     float dom[4];          // xmin, xmax, ymin, ymax of the periodic faces
     BBoxFace bbox[4];
     float smax;            // largest |coordinate| of any vertex (scale of FP32 rounding in the scene)
+    const uint32_t *mdesc; // [P] mask descriptor index (float4 units from rec) per primitive, RT_MISS = none;
+                           // null when no primitive has a texture mask
 };
 
 // Occlusion grid of one collimated-source direction: every traced primitive's padded projection onto the

@@ -111,6 +111,10 @@ SIGNATURES = {
                                   C.POINTER(C.c_void_p)]),
     "rt_scene_destroy": (C.c_int, [C.c_void_p, C.c_void_p]),
     "rt_scene_get_info": (C.c_int, [C.c_void_p, C.POINTER(rt_scene_info)]),
+    "rt_scene_set_masks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint8), u32p,
+                                     C.POINTER(C.c_int32), f32p, C.POINTER(C.c_int32)]),
+    "rt_update_texture_masks": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint8), u32p,
+                                          C.POINTER(C.c_int32), f32p, C.POINTER(C.c_int32)]),
     "rt_cast_rays_soa": (C.c_int, [C.c_void_p, C.c_void_p, f32p, f32p, C.c_size_t, C.c_float, f32p, f32p, u32p]),
     "rt_cast_rays_soa_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_float,
                                           C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -330,6 +334,12 @@ class Scene:
                 "domain": list(i.domain), "device_bytes": i.device_bytes,
                 "builder": "ploc" if i.builder == 1 else "lbvh", "depth4": i.depth4}
 
+    def set_masks(self, masks, mask_ids, uv=None, uv_ids=None):
+        """Texture masks: masks = list of 2-D arrays [height][width] (nonzero = opaque, row 0 = top),
+        mask_ids [P] (-1 = none), uv [P][4][2] vertex UVs, uv_ids [P] (< 0: parametric patch UVs)."""
+        args = _mask_args(masks, mask_ids, uv, uv_ids, self.P)
+        _check(self.ctx.lib.rt_scene_set_masks(self.ctx._h, self._h, *args[:-1]))
+
     # ---- CollisionDetection::castRaysSoA ----
     def cast_rays(self, origins, dirs, max_dist: float = -1.0):
         O = _f32(origins).reshape(-1, 3)
@@ -385,6 +395,19 @@ class Scene:
         return out
 
 
+def _mask_args(masks, mask_ids, uv, uv_ids, P):
+    tex = np.concatenate([np.ascontiguousarray(m, np.uint8).reshape(-1) != 0 for m in masks]).astype(np.uint8) \
+        if len(masks) else np.zeros(1, np.uint8)
+    sizes = _u32([[np.shape(m)[1], np.shape(m)[0]] for m in masks] if len(masks) else np.zeros((1, 2)))
+    mid = np.ascontiguousarray(mask_ids, np.int32).reshape(P)
+    U = None if uv is None else _f32(uv).reshape(P, 8)
+    uid = None if uv_ids is None else np.ascontiguousarray(uv_ids, np.int32).reshape(P)
+    p8 = tex.ctypes.data_as(C.POINTER(C.c_uint8))
+    pi = lambda a: None if a is None else a.ctypes.data_as(C.POINTER(C.c_int32))
+    keep = (tex, sizes, mid, U, uid)
+    return (len(masks), p8, _pu(sizes), pi(mid), _pf(U), pi(uid), keep)
+
+
 class Backend:
     """The call-by-call backend interface of one context (helios::RayTracingBackend,
     plugins/radiation/include/RayTracingBackend.h:43-312); method names follow the reference's."""
@@ -401,6 +424,11 @@ class Backend:
         _check(self.lib.rt_update_geometry(self.ctx._h, T.shape[0], _pf(T), _pu(_u32(types)),
                                            _pu(None if uuids is None else _u32(uuids)), _pu(_u32(twosided)),
                                            int(periodic[0]), int(periodic[1])))
+
+    def update_texture_masks(self, masks, mask_ids, uv=None, uv_ids=None):
+        """Texture masks of the staged geometry (as Scene.set_masks), applied by the next build."""
+        args = _mask_args(masks, mask_ids, uv, uv_ids, self._geom_P)
+        _check(self.lib.rt_update_texture_masks(self.ctx._h, self._geom_P, *args[:-1]))
 
     def build_acceleration_structure(self):
         _check(self.lib.rt_build_acceleration_structure(self.ctx._h))

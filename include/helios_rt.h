@@ -69,6 +69,17 @@ typedef struct {
 } rt_scene_info;
 int rt_scene_get_info(const rt_scene *scene, rt_scene_info *info);
 
+/* Texture masks (RayTracingGeometry mask_data / mask_sizes / mask_IDs / uv_data / uv_IDs,
+ * R/include/RayTracingTypes.h:72-77; R/shaders/common/texture_mask.glsl:20-66).  nmasks masks of
+ * sizes[m] = (width, height) texels, one byte per texel (0 = transparent), concatenated row-major from
+ * the top row in `texels`; mask_ids[P] (-1 = none); uv[P][4][2] per-primitive vertex UVs (NULL = zeros)
+ * and uv_ids[P] (NULL or < 0: a patch uses its parametric UVs).  A transparent texel at a hit lets the
+ * ray pass; direct and diffuse ray origins are redrawn off transparent texels (direct_raygen.comp:431-473,
+ * diffuse_raygen.comp:411-455).  Replaces the scene's masks; nmasks = 0 removes them.  castRays ignores
+ * masks, as CollisionDetection does. */
+int rt_scene_set_masks(rt_ctx *ctx, rt_scene *scene, uint32_t nmasks, const uint8_t *texels, const uint32_t *sizes,
+                       const int32_t *mask_ids, const float *uv, const int32_t *uv_ids);
+
 /* ---------------------------------------------------------------------------------------------
  * CollisionDetection::castRaysSoA (CD/include/CollisionDetection.h:395,
  * CD/src/CollisionDetection_RayTracing.cpp:551-680): closest hit per ray with the CD predicates.
@@ -157,6 +168,11 @@ int rt_energy_residual_fp64(rt_ctx *ctx, double *out, uint32_t nchannels);
  * arrays, same meaning as rt_scene_create; uuids NULL = 0..P-1. */
 int rt_update_geometry(rt_ctx *ctx, uint32_t nprims, const float *transforms, const uint32_t *types,
                        const uint32_t *uuids, const uint32_t *twosided, int periodic_x, int periodic_y);
+/* The texture-mask fields of updateGeometry's RayTracingGeometry (as rt_scene_set_masks, with the
+ * reference's compact uv_data expanded to [P][4][2]); staged, applied by the next build.  Call after
+ * rt_update_geometry, which clears staged masks. */
+int rt_update_texture_masks(rt_ctx *ctx, uint32_t nprims, uint32_t nmasks, const uint8_t *texels,
+                            const uint32_t *sizes, const int32_t *mask_ids, const float *uv, const int32_t *uv_ids);
 /* buildAccelerationStructure (RayTracingBackend.h:81): GPU LBVH of the staged geometry. */
 int rt_build_acceleration_structure(rt_ctx *ctx);
 /* updateMaterials (RayTracingBackend.h:90; R/include/RayTracingTypes.h:157-184): rho, tau are

@@ -80,6 +80,9 @@ def lib():
                                           f64p]
         L.orc_rotation_matrix.argtypes = [f32p, f32p]
         L.orc_areas.argtypes = [C.c_void_p, f64p]
+        L.orc_scene_set_masks.restype = C.c_int
+        L.orc_scene_set_masks.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint8), u32p, C.POINTER(C.c_int32),
+                                          f32p, C.POINTER(C.c_int32)]
         L.orc_camera_pixels.argtypes = [C.c_void_p, C.POINTER(orc_camera), C.POINTER(orc_camera_light), f64p]
         L.orc_pixel_labels.argtypes = [C.c_void_p, C.POINTER(orc_camera), u32p, f32p]
         _lib = L
@@ -205,6 +208,23 @@ class OracleScene:
                                   None if W is None else W.ctypes.data_as(f32p),
                                   None if R is None else R.ctypes.data_as(f32p), out.ctypes.data_as(f64p))
         return out
+
+    def set_masks(self, masks, mask_ids, uv=None, uv_ids=None):
+        """Texture masks: list of 2-D arrays [height][width] (nonzero = opaque, row 0 = top), mask_ids [P]
+        (-1 = none), uv [P][4][2], uv_ids [P] (< 0: parametric patch UVs)."""
+        tex = np.concatenate([np.asarray(m).reshape(-1) != 0 for m in masks]).astype(np.uint8) if len(masks) \
+            else np.zeros(1, np.uint8)
+        sizes = _u([[np.shape(m)[1], np.shape(m)[0]] for m in masks] if len(masks) else np.zeros((1, 2)))
+        mid = np.ascontiguousarray(mask_ids, np.int32).reshape(self.P)
+        U = None if uv is None else _f(uv).reshape(self.P, 8)
+        uid = None if uv_ids is None else np.ascontiguousarray(uv_ids, np.int32).reshape(self.P)
+        pi = lambda a: None if a is None else a.ctypes.data_as(C.POINTER(C.c_int32))
+        self._masks = (tex, sizes, mid, U, uid)
+        rc = lib().orc_scene_set_masks(self.h, len(masks), tex.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                       sizes.ctypes.data_as(u32p), pi(mid), None if U is None else U.ctypes.data_as(f32p),
+                                       pi(uid))
+        if rc != 0:
+            raise ValueError("mask id out of range")
 
     @staticmethod
     def _camera(cam, tile, seed):

@@ -124,6 +124,14 @@ struct orc_scene {
     onode *nodes;
     uint32_t nnodes, cap;
     uint32_t *pidx;
+    /* texture masks (R/shaders/common/texture_mask.glsl) */
+    uint32_t nmasks;
+    uint8_t *tex;      /* texels, one byte each, masks concatenated */
+    uint32_t *msize;   /* [M][2] width, height */
+    uint64_t *moff;    /* [M] first texel */
+    int32_t *mid;      /* [P] mask of each primitive, -1 = none (NULL: no masks) */
+    float *muv;        /* [P][8] vertex UVs */
+    int32_t *muvid;    /* [P] custom-UV flag of a patch (>= 0) */
 };
 
 static int is_patch(uint32_t t) { return t == 0 || t == 3; }
@@ -396,6 +404,7 @@ void orc_scene_destroy(orc_scene *s) {
     if (!s) return;
     free(s->T); free(s->type); free(s->twos); free(s->uuid); free(s->verts); free(s->normal); free(s->frame);
     free(s->pn); free(s->aa); free(s->bb); free(s->blo); free(s->bhi); free(s->nodes); free(s->pidx);
+    free(s->tex); free(s->msize); free(s->moff); free(s->mid); free(s->muv); free(s->muvid);
     free(s);
 }
 
@@ -431,7 +440,7 @@ void orc_areas(const orc_scene *s, double *out) {
 /* ------------------------------------------------------------------------------------------------ */
 
 /* R/shaders/common/intersections.glsl:26-70 */
-static int isect_patch_rad(const orc_scene *s, uint32_t i, v3 o, v3 d, float *t_out) {
+static int isect_patch_rad_uv(const orc_scene *s, uint32_t i, v3 o, v3 d, float *t_out, float *ua, float *ub) {
     v3 n = s->pn[i];
     float denom = dot(d, n);
     if (fabsf(denom) < 1e-8f) return 0;
@@ -442,12 +451,12 @@ static int isect_patch_rad(const orc_scene *s, uint32_t i, v3 o, v3 d, float *t_
     v3 dd = sub(p, anchor);
     v3 a = sub(s->verts[i * 4 + 1], anchor), b = sub(s->verts[i * 4 + 3], anchor);
     float da = dot(dd, a), db = dot(dd, b);
-    if (da > 0.0f && da < s->aa[i] && db > 0.0f && db < s->bb[i]) { *t_out = t; return 1; }
+    if (da > 0.0f && da < s->aa[i] && db > 0.0f && db < s->bb[i]) { *t_out = t; *ua = da; *ub = db; return 1; }
     return 0;
 }
 
 /* R/shaders/common/intersections.glsl:73-112 (Moller-Trumbore) */
-static int isect_tri_rad(const orc_scene *s, uint32_t i, v3 o, v3 d, float *t_out) {
+static int isect_tri_rad_uv(const orc_scene *s, uint32_t i, v3 o, v3 d, float *t_out, float *ua, float *ub) {
     v3 v0 = s->verts[(size_t)i * 4];
     v3 e1 = sub(s->verts[i * 4 + 1], v0), e2 = sub(s->verts[i * 4 + 2], v0);
     v3 h = cross(d, e2);
@@ -461,7 +470,120 @@ static int isect_tri_rad(const orc_scene *s, uint32_t i, v3 o, v3 d, float *t_ou
     float v = f * dot(d, q);
     if (v < 0.0f || u + v > 1.0f) return 0;
     float t = f * dot(e2, q);
-    if (t > 1e-8f && t < 1e8f) { *t_out = t; return 1; }
+    if (t > 1e-8f && t < 1e8f) { *t_out = t; *ua = u; *ub = v; return 1; }
+    return 0;
+}
+
+/* ---- texture masks (R/shaders/common/texture_mask.glsl:20-66) ---- */
+static int tex_int(float x) { return x >= 2147483520.0f ? 2147483520 : (x > -2147483520.0f ? (int)x : -2147483520); }
+
+static int mask_texel_orc(const orc_scene *s, int m, int tx, int ty) {
+    int sx = (int)s->msize[2 * m], sy = (int)s->msize[2 * m + 1];
+    if (tx < 0) tx = 0;
+    if (tx > sx - 1) tx = sx - 1;
+    if (ty < 0) ty = 0;
+    if (ty > sy - 1) ty = sy - 1;
+    return s->tex[s->moff[m] + (uint64_t)ty * (uint64_t)sx + (uint64_t)tx] != 0;
+}
+
+/* check_texture_mask: patch at parametric (u, v), triangle at barycentric (u, v) */
+static int mask_opaque_orc(const orc_scene *s, uint32_t i, float u, float v) {
+    int m = s->mid[i];
+    float fx = (float)((int)s->msize[2 * m] - 1), fy = (float)((int)s->msize[2 * m + 1] - 1);
+    const float *uv = s->muv + (size_t)i * 8;
+    int tx, ty;
+    if (s->type[i] == 1) {
+        float tu = (uv[0] + u * (uv[2] - uv[0])) + v * (uv[4] - uv[0]);
+        float tv = 1.0f - ((uv[1] + u * (uv[3] - uv[1])) + v * (uv[5] - uv[1]));
+        tx = tex_int(roundf(fx * fabsf(tu)));
+        ty = tex_int(roundf(fy * fabsf(tv)));
+    } else if (s->muvid[i] < 0) {
+        tx = tex_int(floorf(fx * u));
+        ty = tex_int(floorf(fy * (1.0f - v)));
+    } else {
+        float du = uv[2] - uv[0], dv = uv[5] - uv[3];
+        tx = tex_int(floorf(fx * (uv[0] + u * du)));
+        ty = tex_int(floorf(fy * ((1.0f - uv[1]) - v * dv)));
+    }
+    return mask_texel_orc(s, m, tx, ty);
+}
+
+static int masked_prim(const orc_scene *s, uint32_t i) { return s->mid && s->mid[i] >= 0; }
+
+/* radiation predicates with the mask applied at the hit (bvh_traversal.glsl:143-148) */
+static int isect_patch_rad(const orc_scene *s, uint32_t i, v3 o, v3 d, float *t_out) {
+    float ua, ub;
+    if (!isect_patch_rad_uv(s, i, o, d, t_out, &ua, &ub)) return 0;
+    if (!masked_prim(s, i)) return 1;
+    /* u = d.a / |a|^2 with |a| the rounded length (intersections.glsl:64-66) */
+    float am = sqrtf(s->aa[i]), bm = sqrtf(s->bb[i]);
+    return mask_opaque_orc(s, i, ua / (am * am), ub / (bm * bm));
+}
+
+static int isect_tri_rad(const orc_scene *s, uint32_t i, v3 o, v3 d, float *t_out) {
+    float ua, ub;
+    if (!isect_tri_rad_uv(s, i, o, d, t_out, &ua, &ub)) return 0;
+    if (!masked_prim(s, i)) return 1;
+    return mask_opaque_orc(s, i, ua, ub);
+}
+
+/* Origin rejection (direct_raygen.comp:431-473, diffuse_raygen.comp:411-455): patch, up to 10 redraws of
+ * a uniform point of the whole patch; triangle, up to 50 draws of a canonical-triangle point looked up
+ * with floor() and no abs().  Returns 0 when the ray is dropped (triangle: every draw transparent;
+ * diffuse patch: the last draw transparent). */
+static int origin_mask_orc(const orc_scene *s, uint32_t i, int diffuse, uint32_t *st, float *spx, float *spy) {
+    if (!masked_prim(s, i)) return 1;
+    if (is_patch(s->type[i])) {
+        int ok = mask_opaque_orc(s, i, *spx + 0.5f, *spy + 0.5f);
+        for (int k = 0; k < 10 && !ok; k++) {
+            *spx = -0.5f + orc_rnd(st);
+            *spy = -0.5f + orc_rnd(st);
+            ok = mask_opaque_orc(s, i, *spx + 0.5f, *spy + 0.5f);
+        }
+        return ok || !diffuse;
+    }
+    int m = s->mid[i];
+    float fx = (float)((int)s->msize[2 * m] - 1), fy = (float)((int)s->msize[2 * m + 1] - 1);
+    const float *uv = s->muv + (size_t)i * 8;
+    int masked = 1;
+    for (int a = 0; a < 50 && masked; a++) {
+        float r1 = orc_rnd(st), r2 = orc_rnd(st);
+        if (r1 + r2 > 1.0f) { r1 = 1.0f - r1; r2 = 1.0f - r2; }
+        *spx = r2;
+        *spy = r1 + r2;
+        float tu = (uv[0] + r1 * (uv[2] - uv[0])) + r2 * (uv[4] - uv[0]);
+        float tv = 1.0f - ((uv[1] + r1 * (uv[3] - uv[1])) + r2 * (uv[5] - uv[1]));
+        masked = !mask_texel_orc(s, m, tex_int(floorf(fx * tu)), tex_int(floorf(fy * tv)));
+    }
+    return !masked;
+}
+
+int orc_scene_set_masks(orc_scene *s, uint32_t nmasks, const uint8_t *texels, const uint32_t *sizes,
+                        const int32_t *mask_ids, const float *uv, const int32_t *uv_ids) {
+    free(s->tex); free(s->msize); free(s->moff); free(s->mid); free(s->muv); free(s->muvid);
+    s->tex = NULL; s->msize = NULL; s->moff = NULL; s->mid = NULL; s->muv = NULL; s->muvid = NULL;
+    s->nmasks = nmasks;
+    if (!nmasks) return 0;
+    s->msize = (uint32_t *)malloc(sizeof(uint32_t) * 2 * nmasks);
+    s->moff = (uint64_t *)malloc(sizeof(uint64_t) * nmasks);
+    uint64_t tot = 0;
+    for (uint32_t m = 0; m < nmasks; m++) {
+        s->msize[2 * m] = sizes[2 * m];
+        s->msize[2 * m + 1] = sizes[2 * m + 1];
+        s->moff[m] = tot;
+        tot += (uint64_t)sizes[2 * m] * sizes[2 * m + 1];
+    }
+    s->tex = (uint8_t *)malloc(tot ? tot : 1);
+    memcpy(s->tex, texels, tot);
+    s->mid = (int32_t *)malloc(sizeof(int32_t) * (s->P ? s->P : 1));
+    s->muv = (float *)calloc((size_t)(s->P ? s->P : 1) * 8, sizeof(float));
+    s->muvid = (int32_t *)malloc(sizeof(int32_t) * (s->P ? s->P : 1));
+    for (uint32_t i = 0; i < s->P; i++) {
+        s->mid[i] = mask_ids[i];
+        if (mask_ids[i] >= (int32_t)nmasks) return -1;
+        s->muvid[i] = uv_ids ? uv_ids[i] : -1;
+        if (uv) memcpy(s->muv + (size_t)i * 8, uv + (size_t)i * 8, 8 * sizeof(float));
+    }
     return 0;
 }
 
@@ -647,7 +769,10 @@ uint32_t orc_trace_periodic(const orc_scene *s, const float o[3], const float d[
 /* ------------------------------------------------------------------------------------------------ */
 
 /* direct_raygen.comp:360-480 for patch/tile (stratified, subdivisions (1,1)) and triangle samples */
-static v3 direct_origin(const orc_scene *s, uint32_t i, uint32_t lx, uint32_t ly, uint32_t d, uint32_t seed) {
+/* *st receives the RNG state after the origin's draws (including mask redraws); returns 0 when the ray
+ * is dropped by its texture mask */
+static int direct_origin(const orc_scene *s, uint32_t i, uint32_t lx, uint32_t ly, uint32_t d, uint32_t seed, v3 *o,
+                         uint32_t *st_out) {
     uint32_t Nrays = d * d;
     uint32_t ray_index = d * ly + lx;
     uint32_t st = orc_tea16(ray_index + Nrays * i, seed);
@@ -660,17 +785,21 @@ static v3 direct_origin(const orc_scene *s, uint32_t i, uint32_t lx, uint32_t ly
     } else {
         if (Rx < Ry) { spx = Rx; spy = Ry; } else { spx = Ry; spy = Rx; }
     }
-    return xform(s->T + (size_t)i * 16, spx, spy, 0.0f);
+    int ok = origin_mask_orc(s, i, 0, &st, &spx, &spy);
+    *o = xform(s->T + (size_t)i * 16, spx, spy, 0.0f);
+    if (st_out) *st_out = st;
+    return ok;
 }
 
 void orc_direct_ray(const orc_scene *s, uint32_t prim, uint32_t lx, uint32_t ly, uint32_t d, uint32_t seed, float origin[3]) {
-    v3 o = direct_origin(s, prim, lx, ly, d, seed);
+    v3 o;
+    if (!direct_origin(s, prim, lx, ly, d, seed, &o, NULL)) o = mk(NAN, NAN, NAN);
     origin[0] = o.x; origin[1] = o.y; origin[2] = o.z;
 }
 
 /* diffuse_raygen.comp:375-482 */
-static void diffuse_ray(const orc_scene *s, uint32_t i, uint32_t lx, uint32_t ly, uint32_t n, uint32_t face,
-                        uint32_t seed, v3 *o, v3 *dir) {
+static int diffuse_ray(const orc_scene *s, uint32_t i, uint32_t lx, uint32_t ly, uint32_t n, uint32_t face,
+                       uint32_t seed, v3 *o, v3 *dir) {
     uint32_t dimx = n * n;
     uint32_t ray_index = n * ly + lx;
     uint32_t st = orc_tea16(ray_index + dimx * i, seed);
@@ -679,6 +808,7 @@ static void diffuse_ray(const orc_scene *s, uint32_t i, uint32_t lx, uint32_t ly
     float Rx = orc_rnd(&st), Ry = orc_rnd(&st);
     if (is_patch(ty)) { spx = -0.5f + Rx; spy = -0.5f + Ry; }
     else { if (Rx < Ry) { spx = Rx; spy = Ry; } else { spx = Ry; spy = Rx; } }
+    if (!origin_mask_orc(s, i, 1, &st, &spx, &spy)) return 0;
     *o = xform(s->T + (size_t)i * 16, spx, spy, 0.0f);
     float Rt = ((float)lx + orc_rnd(&st)) / (float)n;
     float Rp = ((float)ly + orc_rnd(&st)) / (float)n;
@@ -696,12 +826,13 @@ static void diffuse_ray(const orc_scene *s, uint32_t i, uint32_t lx, uint32_t ly
     v3 r = nrm(dw);
     if (face == 0 && s->twos[i] == 1) r = mk(-r.x, -r.y, -r.z);
     *dir = r;
+    return 1;
 }
 
 void orc_diffuse_ray(const orc_scene *s, uint32_t prim, uint32_t lx, uint32_t ly, uint32_t n, uint32_t face,
                      uint32_t seed, float origin[3], float dir[3]) {
     v3 o, d;
-    diffuse_ray(s, prim, lx, ly, n, face, seed, &o, &d);
+    if (!diffuse_ray(s, prim, lx, ly, n, face, seed, &o, &d)) { o = mk(NAN, NAN, NAN); d = o; }
     origin[0] = o.x; origin[1] = o.y; origin[2] = o.z;
     dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
 }
@@ -720,7 +851,7 @@ void orc_diffuse_hits_range(const orc_scene *s, uint32_t n, uint32_t seed, uint3
                 if (!launched) { w[r] = 0xFFFFFFFEu; continue; }
                 uint32_t lx = r % n, ly = r / n;
                 v3 o, d;
-                diffuse_ray(s, i, lx, ly, n, face, seed, &o, &d);
+                if (!diffuse_ray(s, i, lx, ly, n, face, seed, &o, &d)) { w[r] = 0xFFFFFFFEu; continue; }
                 int ex;
                 uint32_t h = trace_loop(s, o, d, 1e-5f, i, &ex);
                 if (h == MISS) { w[r] = MISS; continue; }
@@ -749,7 +880,8 @@ void orc_direct_sums_range(const orc_scene *s, uint32_t d, uint32_t seed, uint32
         v3 n = s->normal[i];
         for (uint32_t r = 0; r < Nrays; r++) {
             uint32_t lx = r % d, ly = r / d;
-            v3 o = direct_origin(s, i, lx, ly, d, seed);
+            v3 o;
+            if (!direct_origin(s, i, lx, ly, d, seed, &o, NULL)) continue;
             for (uint32_t src = 0; src < nsrc; src++) {
                 v3 dir = nrm(ld3(src_dir + src * 3));
                 float strength = fabsf(dot(n, dir)) / (float)Nrays;
@@ -884,16 +1016,11 @@ void orc_direct_sums_src_range(const orc_scene *s, uint32_t d, uint32_t seed, ui
         uint32_t tw = s->twos[i];
         for (uint32_t r = 0; r < Nrays; r++) {
             uint32_t lx = r % d, ly = r / d;
-            /* origin sample (two draws), then each source in order draws from the same state */
-            uint32_t st = orc_tea16(d * ly + lx + Nrays * i, seed);
-            float Rx = orc_rnd(&st), Ry = orc_rnd(&st), spx, spy;
-            if (is_patch(s->type[i])) {
-                spx = (-0.5f + (float)lx / (float)d) + Rx / (float)d;
-                spy = (-0.5f + (float)ly / (float)d) + Ry / (float)d;
-            } else {
-                if (Rx < Ry) { spx = Rx; spy = Ry; } else { spx = Ry; spy = Rx; }
-            }
-            v3 o = xform(s->T + (size_t)i * 16, spx, spy, 0.0f);
+            /* origin sample (two draws, then any mask redraws), then each source in order draws from the
+             * same state */
+            uint32_t st;
+            v3 o;
+            if (!direct_origin(s, i, lx, ly, d, seed, &o, &st)) continue;
             for (uint32_t src = 0; src < nsrc; src++) {
                 uint32_t ty = types ? types[src] : 0;
                 v3 dir;
@@ -1194,7 +1321,7 @@ static void diffuse_pass(const orc_scene *s, const orc_band_params *p, uint32_t 
             if (face == 0 && s->twos[i] == 0) continue;
             for (uint32_t r = 0; r < nn; r++) {
                 v3 o, d;
-                diffuse_ray(s, i, r % n, r / n, n, face, seed, &o, &d);
+                if (!diffuse_ray(s, i, r % n, r / n, n, face, seed, &o, &d)) continue;
                 int ex;
                 uint32_t h = trace_loop(s, o, d, 1e-5f, i, &ex);
                 int top = 0;

@@ -86,6 +86,11 @@ void orc_diffuse_hits_range(const orc_scene *s, uint32_t n, uint32_t seed, uint3
 void orc_direct_sums_range(const orc_scene *s, uint32_t d, uint32_t seed, uint32_t nsrc, const float *src_dir,
                            uint32_t p0, uint32_t p1, double *out);
 
+/* Texture masks (one byte per texel, 0 = transparent; mask_ids [P], -1 = none; uv [P][8] or NULL;
+ * uv_ids [P] or NULL).  Returns -1 on a mask id out of range. */
+int orc_scene_set_masks(orc_scene *s, uint32_t nmasks, const uint8_t *texels, const uint32_t *sizes,
+                        const int32_t *mask_ids, const float *uv, const int32_t *uv_ids);
+
 /* CollisionDetection::castRaysSoA semantics. */
 void orc_cast_rays(const orc_scene *s, uint64_t n, const float *origins, const float *dirs, float max_dist,
                    float *out_t, float *out_normal, uint32_t *out_uuid, int brute);
