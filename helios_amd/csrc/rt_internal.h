@@ -42,24 +42,25 @@ struct LaunchCfg {
 
 // kernel launchers (rt_trace.hip)
 // stats (may be null): [0] BVH node fetches, [1] primitive tests, [2] rays traced
+// masked: the scene carries texture masks (the kernels without mask tests run otherwise)
 int launch_direct(const DirectJob *jobs_dev, uint32_t njobs, uint32_t total_items, uint32_t depth, uint32_t *counter,
-                  int *err, unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st);
+                  int *err, unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st, bool masked);
 // Non-collimated sources (k_direct_src): items [item_lo, item_lo + nitems) of jobs with ascending item_begin.
 int launch_direct_src(const DirectJob *jobs_dev, uint32_t njobs, uint32_t item_lo, uint32_t nitems, uint32_t depth,
-                      uint32_t *counter, int *err, unsigned long long *stats, int ncu, hipStream_t st);
+                      uint32_t *counter, int *err, unsigned long long *stats, int ncu, hipStream_t st, bool masked);
 // Any mix of source types: hjobs is the host copy of jobs_dev, collimated jobs first (their items precede
 // the others'); collimated jobs go to k_direct, the rest to k_direct_src.
 int launch_direct_jobs(const DirectJob *hjobs, const DirectJob *jobs_dev, uint32_t njobs, uint32_t total_items,
                        uint32_t depth, uint32_t *counter, int *err, unsigned long long *stats, const LaunchCfg &cfg,
-                       int ncu, hipStream_t st);
+                       int ncu, hipStream_t st, bool masked);
 // one diffuse job (device pointer) per launch: items [0, total_items) of that job
 int launch_diffuse(const DiffuseJob *job_dev, uint32_t total_items, uint32_t depth, uint32_t *counter, int *err,
-                   unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st);
+                   unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st, bool masked);
 int launch_gather(const GatherJob *jobs_dev, uint32_t njobs, uint32_t total_items, uint32_t cmax, uint32_t *counter,
                   const LaunchCfg &cfg, int ncu, hipStream_t st);
 // Camera pixels (labels = false) or pixel labels/depths (labels = true) of one tile.
 int launch_camera(const CameraLaunch &c, uint32_t depth, bool labels, int *err, unsigned long long *stats,
-                  hipStream_t st);
+                  hipStream_t st, bool masked);
 // counter: device scratch for the work queue (zeroed by the launch)
 int launch_cast(const DevScene *sc, uint32_t depth, uint64_t n, const float *o, const float *d, float maxd, float *t,
                 float *nrm, uint32_t *uuid, unsigned long long *counter, int *err, unsigned long long *stats,

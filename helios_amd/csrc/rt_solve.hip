@@ -476,7 +476,7 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
         if (ev_pair(sx, a, b)) return -1;
         if (a) CK(hipEventRecord(a, st));
         if (launch_direct_jobs(hd.data(), djobs, (uint32_t)hd.size(), items, S.h.depth, counter, err, sx.stat_ptr(0),
-                               sx.cfg[0], sx.ncu, st))
+                               sx.cfg[0], sx.ncu, st, S.h.mdesc != nullptr))
             return -1;
         if (b) { CK(hipEventRecord(b, st)); tick(sx, 0, a, b); }
         sx.timing_info.direct_rays += (uint64_t)items * d * d;
@@ -493,7 +493,9 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
         hipEvent_t a, b;
         if (ev_pair(sx, a, b)) return -1;
         if (a) CK(hipEventRecord(a, st));
-        if (launch_diffuse(fjobs + g, P, S.h.depth, counter, err, sx.stat_ptr(1), sx.cfg[1], sx.ncu, st)) return -1;
+        if (launch_diffuse(fjobs + g, P, S.h.depth, counter, err, sx.stat_ptr(1), sx.cfg[1], sx.ncu, st,
+                           S.h.mdesc != nullptr))
+            return -1;
         if (b) { CK(hipEventRecord(b, st)); tick(sx, 1, a, b); }
         sx.timing_info.diffuse_rays += (uint64_t)P * 2 * nn;
         sx.timing_info.diffuse_launches++;
@@ -572,7 +574,8 @@ int debug_diffuse_hits(SolveCtx &sx, const Scene &S, uint32_t n, uint32_t seed, 
     fj.sc = S.d; fj.n = n; fj.seed = seed; fj.hits = hits; fj.item_begin = 0; fj.P = P;
     CK(hipMemsetAsync(err, 0, 4, st));
     CK(hipMemcpyAsync(fjob, &fj, sizeof(fj), hipMemcpyHostToDevice, st));
-    if (launch_diffuse(fjob, P, S.h.depth, counter, err, sx.stat_ptr(1), sx.cfg[1], sx.ncu, st)) return -1;
+    if (launch_diffuse(fjob, P, S.h.depth, counter, err, sx.stat_ptr(1), sx.cfg[1], sx.ncu, st, S.h.mdesc != nullptr))
+        return -1;
     CK(hipMemcpyAsync(out, hits, (size_t)P * 2 * nn * 4, hipMemcpyDeviceToHost, st));
     CK(hipStreamSynchronize(st));
     int herr = 0;
@@ -619,7 +622,7 @@ int debug_direct_sums(SolveCtx &sx, const Scene &S, uint32_t d, uint32_t seed, u
     CK(hipMemsetAsync(err, 0, 4, st));
     CK(hipMemcpyAsync(jobs, hj.data(), nsrc * sizeof(DirectJob), hipMemcpyHostToDevice, st));
     if (launch_direct_jobs(hj.data(), jobs, nsrc, nsrc * P, S.h.depth, counter, err, sx.stat_ptr(0), sx.cfg[0], sx.ncu,
-                           st))
+                           st, S.h.mdesc != nullptr))
         return -1;
     std::vector<float> tmp((size_t)nsrc * P * 2);
     CK(hipMemcpyAsync(tmp.data(), sums, tmp.size() * 4, hipMemcpyDeviceToHost, st));

@@ -42,18 +42,28 @@ namespace rt {
 __device__ __forceinline__ float min3f(float a, float b, float c) { return fminf(fminf(a, b), c); }
 __device__ __forceinline__ float max3f(float a, float b, float c) { return fmaxf(fmaxf(a, b), c); }
 
-// traversal work counters (node fetches, primitive tests) of one lane
-struct TCnt {
+// traversal work counters (node fetches, primitive tests) of one lane.  The counter type also carries
+// whether the traversal honours texture masks (mask = false compiles the mask tests away for scenes
+// without masks, keeping the leaf code and its register use as small as before masks existed).
+template <bool MK>
+struct TCntT {
     uint32_t nodes, prims;
+    static constexpr bool mask = MK;
 };
 // the same interface with every update compiled away (timed kernels)
 struct NopCounter {
     __device__ __forceinline__ NopCounter operator++(int) { return *this; }
     __device__ __forceinline__ NopCounter &operator+=(uint32_t) { return *this; }
 };
-struct NoCnt {
+template <bool MK>
+struct NoCntT {
     NopCounter nodes, prims;
+    static constexpr bool mask = MK;
 };
+using TCnt = TCntT<true>;
+using NoCnt = NoCntT<true>;
+template <bool COUNT, bool MK>
+using Cnt = typename std::conditional<COUNT, TCntT<MK>, NoCntT<MK>>::type;
 
 struct RayPre {
     float ix, iy, iz, ox, oy, oz;
@@ -105,7 +115,7 @@ __device__ __forceinline__ T ldu(const T *p) {
 }
 
 // Intersect the primitive in leaf slot k; returns its index (RT_MISS if skipped) and the hit.
-template <bool CD>
+template <bool CD, bool MK = true>
 __device__ __forceinline__ bool leaf_hit_p(const float4 *recs, uint32_t k, v3f o, v3f d, uint32_t skip, uint32_t &id,
                                            float &t) {
     const float4 *r = recs + (size_t)k * 4;
@@ -113,7 +123,7 @@ __device__ __forceinline__ bool leaf_hit_p(const float4 *recs, uint32_t k, v3f o
     uint32_t tag = __float_as_uint(r0.w);
     id = tag & RT_ID_BITS;
     bool patch = (tag & RT_PATCH_BIT) != 0;
-    const bool masked = !CD && (tag & RT_MASK_BIT) != 0;
+    const bool masked = MK && !CD && (tag & RT_MASK_BIT) != 0;
     float4 r1 = ldc(r + 1), r2 = ldc(r + 2);
     float4 r3 = (patch || masked) ? ldc(r + 3) : make_float4(0.f, 0.f, 0.f, 0.f);
     float ua, ub;
@@ -123,18 +133,18 @@ __device__ __forceinline__ bool leaf_hit_p(const float4 *recs, uint32_t k, v3f o
     return hit;
 }
 
-template <bool CD>
+template <bool CD, bool MK = true>
 __device__ __forceinline__ bool leaf_hit(const DevScene &sc, uint32_t k, v3f o, v3f d, uint32_t skip, uint32_t &id,
                                          float &t) {
-    return leaf_hit_p<CD>(sc.rec, k, o, d, skip, id, t);
+    return leaf_hit_p<CD, MK>(sc.rec, k, o, d, skip, id, t);
 }
 
-template <bool CD>
+template <bool CD, bool MK = true>
 __device__ __forceinline__ void test_leaf(const float4 *recs, uint32_t k, v3f o, v3f d, float tmin, uint32_t skip,
                                           float &best_t, uint32_t &best, uint32_t &best_slot) {
     uint32_t id;
     float t;
-    if (leaf_hit_p<CD>(recs, k, o, d, skip, id, t) && t > tmin) {
+    if (leaf_hit_p<CD, MK>(recs, k, o, d, skip, id, t) && t > tmin) {
         if (t < best_t || (t == best_t && best != RT_MISS && id < best)) {
             best_t = t;
             best = id;
@@ -192,7 +202,7 @@ __device__ uint32_t trace_closest(const DevScene &sc, v3f o, v3f d, float tmin, 
     const float4 *nodes = sc.node, *recs = sc.rec;
     const uint32_t nleaf = sc.nleaf;
     if (nleaf <= 1) {
-        if (nleaf == 1) { cnt.prims++; test_leaf<CD>(recs, 0, o, d, tmin, skip, best_t, best, best_slot); }
+        if (nleaf == 1) { cnt.prims++; test_leaf<CD, C::mask>(recs, 0, o, d, tmin, skip, best_t, best, best_slot); }
         t_out = best_t;
         slot_out = best_slot;
         return best;
@@ -210,8 +220,8 @@ __device__ uint32_t trace_closest(const DevScene &sc, v3f o, v3f d, float tmin, 
         cnt.nodes++;
         NodeTest nt = node_test_p(nodes, ni, rp, best_t);
         bool l0 = (nt.c0 & RT_LEAF_BIT) != 0, l1 = (nt.c1 & RT_LEAF_BIT) != 0;
-        if (nt.h0 && l0) { cnt.prims++; test_leaf<CD>(recs, nt.c0 & ~RT_LEAF_BIT, o, d, tmin, skip, best_t, best, best_slot); }
-        if (nt.h1 && l1) { cnt.prims++; test_leaf<CD>(recs, nt.c1 & ~RT_LEAF_BIT, o, d, tmin, skip, best_t, best, best_slot); }
+        if (nt.h0 && l0) { cnt.prims++; test_leaf<CD, C::mask>(recs, nt.c0 & ~RT_LEAF_BIT, o, d, tmin, skip, best_t, best, best_slot); }
+        if (nt.h1 && l1) { cnt.prims++; test_leaf<CD, C::mask>(recs, nt.c1 & ~RT_LEAF_BIT, o, d, tmin, skip, best_t, best, best_slot); }
         bool g0 = nt.h0 && !l0, g1 = nt.h1 && !l1;
         if (g0 && g1) {
             bool first0 = nt.tn0 <= nt.tn1;
@@ -248,7 +258,7 @@ __device__ bool trace_any(const DevScene &sc, v3f o, v3f d, float tmin, uint32_t
     if (nleaf <= 1) {
         if (nleaf == 1) {
             cnt.prims++;
-            return leaf_hit_p<false>(recs, 0, o, d, skip, id, t) && t > tmin && t <= tmax;
+            return leaf_hit_p<false, C::mask>(recs, 0, o, d, skip, id, t) && t > tmin && t <= tmax;
         }
         return false;
     }
@@ -266,11 +276,11 @@ __device__ bool trace_any(const DevScene &sc, v3f o, v3f d, float tmin, uint32_t
         bool l0 = (nt.c0 & RT_LEAF_BIT) != 0, l1 = (nt.c1 & RT_LEAF_BIT) != 0;
         if (nt.h0 && l0) {
             cnt.prims++;
-            if (leaf_hit_p<false>(recs, nt.c0 & ~RT_LEAF_BIT, o, d, skip, id, t) && t > tmin && t <= tmax) return true;
+            if (leaf_hit_p<false, C::mask>(recs, nt.c0 & ~RT_LEAF_BIT, o, d, skip, id, t) && t > tmin && t <= tmax) return true;
         }
         if (nt.h1 && l1) {
             cnt.prims++;
-            if (leaf_hit_p<false>(recs, nt.c1 & ~RT_LEAF_BIT, o, d, skip, id, t) && t > tmin && t <= tmax) return true;
+            if (leaf_hit_p<false, C::mask>(recs, nt.c1 & ~RT_LEAF_BIT, o, d, skip, id, t) && t > tmin && t <= tmax) return true;
         }
         bool g0 = nt.h0 && !l0, g1 = nt.h1 && !l1;
         if (g0 && g1) {
@@ -308,7 +318,7 @@ __device__ bool packet_any(const DevScene &sc, v3f o, v3f d, float tmin, uint32_
     if (sc.nleaf <= 1) {
         if (sc.nleaf == 1 && act) {
             cnt.prims++;
-            found = leaf_hit<false>(sc, 0, o, d, skip, id, t) && t > tmin && t <= tmax;
+            found = leaf_hit<false, C::mask>(sc, 0, o, d, skip, id, t) && t > tmin && t <= tmax;
         }
         return found;
     }
@@ -333,14 +343,14 @@ __device__ bool packet_any(const DevScene &sc, v3f o, v3f d, float tmin, uint32_
         bool h0 = act && nt.h0, h1 = act && nt.h1;
         if (l0 && h0) {
             cnt.prims++;
-            if (leaf_hit_p<false>(recs, nt.c0 & ~RT_LEAF_BIT, o, d, skip, id, t) && t > tmin && t <= tmax) {
+            if (leaf_hit_p<false, C::mask>(recs, nt.c0 & ~RT_LEAF_BIT, o, d, skip, id, t) && t > tmin && t <= tmax) {
                 found = true;
                 act = false;
             }
         }
         if (l1 && h1 && act) {
             cnt.prims++;
-            if (leaf_hit_p<false>(recs, nt.c1 & ~RT_LEAF_BIT, o, d, skip, id, t) && t > tmin && t <= tmax) {
+            if (leaf_hit_p<false, C::mask>(recs, nt.c1 & ~RT_LEAF_BIT, o, d, skip, id, t) && t > tmin && t <= tmax) {
                 found = true;
                 act = false;
             }
@@ -408,7 +418,7 @@ __device__ bool packet_any4(const DevScene &sc, v3f o, v3f d, float tmin, uint32
     if (sc.nleaf <= 1) {
         if (sc.nleaf == 1 && act) {
             cnt.prims++;
-            found = leaf_hit<false>(sc, 0, o, d, skip, id, t) && t > tmin && t <= tmax;
+            found = leaf_hit<false, C::mask>(sc, 0, o, d, skip, id, t) && t > tmin && t <= tmax;
         }
         return found;
     }
@@ -447,7 +457,7 @@ __device__ bool packet_any4(const DevScene &sc, v3f o, v3f d, float tmin, uint32
                 const bool want = h[k] && act;
                 if (__ballot(want) != 0) {
                     cnt.prims += want ? 1u : 0u;
-                    const bool hit = leaf_hit_p<false>(recs, lnk[k] & ~RT_LEAF_BIT, o, d, skip, id, t) && t > tmin &&
+                    const bool hit = leaf_hit_p<false, C::mask>(recs, lnk[k] & ~RT_LEAF_BIT, o, d, skip, id, t) && t > tmin &&
                                      t <= tmax;
                     const bool take = want && hit;
                     found = found || take;
@@ -626,8 +636,10 @@ __device__ __forceinline__ void wave_reduce(ksum &k) {
 
 __device__ __forceinline__ bool traced_type(uint32_t t) { return t == 0 || t == 1 || t == 3; }
 
-__device__ __forceinline__ void add_stats(unsigned long long *, const NoCnt &, uint32_t) {}
-__device__ __forceinline__ void add_stats(unsigned long long *stats, const TCnt &c, uint32_t rays) {
+template <bool MK>
+__device__ __forceinline__ void add_stats(unsigned long long *, const NoCntT<MK> &, uint32_t) {}
+template <bool MK>
+__device__ __forceinline__ void add_stats(unsigned long long *stats, const TCntT<MK> &c, uint32_t rays) {
     if (stats) {
         atomicAdd(&stats[0], (unsigned long long)c.nodes);
         atomicAdd(&stats[1], (unsigned long long)c.prims);
@@ -678,9 +690,9 @@ __device__ __forceinline__ int grid_pair(const uint32_t *items, const float *key
               (ta & RT_ID_BITS) != skip && t0 > 1e-4f && t0 <= 1e30f;
     bool h1 = v1 && isect_rec_uv<false>(b0, b1, b2, b3, (tb & RT_PATCH_BIT) != 0, o, d, t1, ua1, ub1) &&
               (tb & RT_ID_BITS) != skip && t1 > 1e-4f && t1 <= 1e30f;
-    if (h0 && (ta & RT_MASK_BIT))
+    if (C::mask && h0 && (ta & RT_MASK_BIT))
         h0 = mask_hit_opaque(recs, __float_as_uint(a3.w), (ta & RT_PATCH_BIT) != 0, ua0, ub0, a1.w, a2.w);
-    if (h1 && (tb & RT_MASK_BIT))
+    if (C::mask && h1 && (tb & RT_MASK_BIT))
         h1 = mask_hit_opaque(recs, __float_as_uint(b3.w), (tb & RT_PATCH_BIT) != 0, ua1, ub1, b1.w, b2.w);
     cnt.prims += (v0 ? 1u : 0u) + ((v1 && !h0) ? 1u : 0u);  // the tests a one-at-a-time scan makes
     if (h0 || h1) return 1;
@@ -705,7 +717,7 @@ __device__ __forceinline__ bool direct_origin(const float4 *recs, uint32_t mdi, 
     return mdi == RT_MISS || origin_mask(recs, mdi, patch, false, st, spx, spy);
 }
 
-template <int WPB, int STK, bool COUNT>
+template <int WPB, int STK, bool COUNT, bool MK>
 __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict__ jobs, uint32_t njobs, uint32_t total,
                                                      uint32_t *counter, uint32_t chunk, int *err,
                                                      unsigned long long *stats) {
@@ -714,7 +726,7 @@ __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict
     uint32_t *wstk = COUNT ? wstk_all + threadIdx.x : wstk_all + (threadIdx.x >> 6) * STK;
     uint32_t *wstk4 = COUNT ? wstk4_all : wstk4_all + (threadIdx.x >> 6) * RT_WSTK4;
     const uint32_t lane = threadIdx.x & 63;
-    typename std::conditional<COUNT, TCnt, NoCnt>::type cnt{};
+    Cnt<COUNT, MK> cnt{};
     uint32_t nray = 0;
     for (uint32_t first = wave_item(counter, chunk); first < total; first = wave_item(counter, chunk)) {
         const uint32_t end = min(first + chunk, total);
@@ -733,7 +745,7 @@ __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict
             const SunGrid *grid = ldu(&J.grid);
             const float4 *recs = ldu(&sc.rec);
             const uint32_t *mdesc = ldu(&sc.mdesc);
-            const uint32_t mdi = mdesc ? ldu(mdesc + i) : RT_MISS;
+            const uint32_t mdi = (MK && mdesc) ? ldu(mdesc + i) : RT_MISS;
             if (traced_type(ty)) {
                 float4 n4 = ldu(ldu(&sc.nrm) + i);
                 v3f n = mk3(n4.x, n4.y, n4.z);
@@ -865,14 +877,14 @@ __device__ __forceinline__ bool source_occluded(const DevScene &sc, v3f o, v3f d
 // Direct rays towards sphere, sun-sphere, rectangle and disk sources (direct_raygen.comp:482-561): one
 // wave per (job, primitive), d*d stratified origins as k_direct, per-ray direction, strength and lit face,
 // per-lane closest-hit traversal.  Items [item_lo, item_lo + nitems) of jobs whose item_begin ascend.
-template <int STK, bool COUNT>
+template <int STK, bool COUNT, bool MK>
 __global__ __launch_bounds__(256) void k_direct_src(const DirectJob *__restrict__ jobs, uint32_t njobs, uint32_t item_lo,
                                                     uint32_t nitems, uint32_t *counter, int *err,
                                                     unsigned long long *stats) {
     __shared__ uint32_t stk_all[STK * 256];
     uint32_t *stk = stk_all + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63;
-    typename std::conditional<COUNT, TCnt, NoCnt>::type cnt{};
+    Cnt<COUNT, MK> cnt{};
     uint32_t nray = 0;
     for (uint32_t first = wave_item(counter, 1u); first < nitems; first = wave_item(counter, 1u)) {
         const uint32_t item = item_lo + first;
@@ -901,8 +913,8 @@ __global__ __launch_bounds__(256) void k_direct_src(const DirectJob *__restrict_
                 float spx, spy;
                 uint32_t st;
                 direct_sample_st(i, r % d, r / d, d, J.seed, patch, spx, spy, st);
-                const uint32_t mdi = sc.mdesc ? sc.mdesc[i] : RT_MISS;
-                if (mdi != RT_MISS && !origin_mask(sc.rec, mdi, patch, false, st, spx, spy)) continue;
+                const uint32_t mdi = (MK && sc.mdesc) ? sc.mdesc[i] : RT_MISS;
+                if (MK && mdi != RT_MISS && !origin_mask(sc.rec, mdi, patch, false, st, spx, spy)) continue;
                 const v3f o = xform3(tm, spx, spy, 0.0f);
                 for (uint32_t k = 0; k < J.skip_draws; k++) (void)rnd24(st);
                 nray++;
@@ -954,7 +966,7 @@ __device__ __forceinline__ bool lane_begin(LaneRay &L, const float4 *recs, uint3
     lane_start(L, nleaf);
     if (nleaf > 1) return true;
     uint32_t slot_unused;
-    if (nleaf == 1) { cnt.prims++; test_leaf<false>(recs, 0, L.o, L.d, tmin, skip, L.best_t, L.best, slot_unused); }
+    if (nleaf == 1) { cnt.prims++; test_leaf<false, C::mask>(recs, 0, L.o, L.d, tmin, skip, L.best_t, L.best, slot_unused); }
     return false;
 }
 
@@ -1092,7 +1104,7 @@ __device__ __forceinline__ bool lane_leaf(const float4 *recs, LaneRay &L, float 
                                           const lds_u32 *stk, const uint32_t *ovf, uint32_t stride, C &cnt) {
     cnt.prims++;
     uint32_t slot_unused;
-    test_leaf<false>(recs, L.ni & ~RT_LEAF_BIT, L.o, L.d, tmin, skip, L.best_t, L.best, slot_unused);
+    test_leaf<false, C::mask>(recs, L.ni & ~RT_LEAF_BIT, L.o, L.d, tmin, skip, L.best_t, L.best, slot_unused);
     return lane_pop<STK, LSTK>(L, stk, ovf, stride);
 }
 
@@ -1120,7 +1132,7 @@ __device__ __forceinline__ bool lane_leaf(const float4 *recs, LaneRay &L, float 
 #ifndef RT_DIFFUSE_LDS_STACK
 #define RT_DIFFUSE_LDS_STACK 16
 #endif
-template <int WPB, int STK, bool COUNT>
+template <int WPB, int STK, bool COUNT, bool MK>
 __global__ __launch_bounds__(WPB * 64) void k_diffuse(const DiffuseJob *__restrict__ job, uint32_t total,
                                                       uint32_t *counter, uint32_t chunk, int *err,
                                                       unsigned long long *stats) {
@@ -1132,7 +1144,7 @@ __global__ __launch_bounds__(WPB * 64) void k_diffuse(const DiffuseJob *__restri
     lds_u32 *stk = (lds_u32 *)stk_all + threadIdx.x;
     const uint32_t stride = WPB * 64;
     const uint32_t lane = threadIdx.x & 63;
-    typename std::conditional<COUNT, TCnt, NoCnt>::type cnt{};
+    Cnt<COUNT, MK> cnt{};
     uint32_t nray = 0;
     // job and scene (wave-uniform, scalar loads)
     const DevScene &sc = *ldu(&job->sc);
@@ -1214,7 +1226,7 @@ __global__ __launch_bounds__(WPB * 64) void k_diffuse(const DiffuseJob *__restri
                 if (!s0) for (uint32_t r = lane; r < nn; r += 64) w[r] = RT_NOT_LAUNCHED;
                 if (!s1) for (uint32_t r = nn + lane; r < 2 * nn; r += 64) w[r] = RT_NOT_LAUNCHED;
                 op_i = i;
-                op_m = mdesc ? ldu(mdesc + i) : RT_MISS;
+                op_m = (MK && mdesc) ? ldu(mdesc + i) : RT_MISS;
                 op_next = 0;
                 op_cnt = ((s0 ? 1u : 0u) + (s1 ? 1u : 0u)) * nn;
                 op_base = s0 ? 0u : nn;
@@ -1471,7 +1483,7 @@ __global__ __launch_bounds__(256) void k_cast(const DevScene *__restrict__ scp, 
     const uint32_t *uuid = ldu(&sc.uuid);
     const uint32_t nleaf = ldu(&sc.nleaf);
     const float tstart = maxd > 0.f ? maxd : FLT_MAX;
-    typename std::conditional<COUNT, TCnt, NoCnt>::type cnt{};
+    Cnt<COUNT, true> cnt{};
     uint32_t nray = 0;
     unsigned long long q_next = 0, q_end = 0, my_r = 0;
     bool drained = false, busy = false, trav = false;
@@ -1583,7 +1595,7 @@ __device__ __forceinline__ uint32_t trace_closest_hyb(const DevScene &sc, v3f o,
     const float4 *nodes = sc.node, *recs = sc.rec;
     const uint32_t nleaf = sc.nleaf;
     if (nleaf <= 1) {
-        if (nleaf == 1) { cnt.prims++; test_leaf<false>(recs, 0, o, d, tmin, RT_MISS, best_t, best, best_slot); }
+        if (nleaf == 1) { cnt.prims++; test_leaf<false, C::mask>(recs, 0, o, d, tmin, RT_MISS, best_t, best, best_slot); }
         t_out = best_t;
         return best;
     }
@@ -1599,8 +1611,8 @@ __device__ __forceinline__ uint32_t trace_closest_hyb(const DevScene &sc, v3f o,
         cnt.nodes++;
         NodeTest nt = node_test_p(nodes, ni, rp, best_t);
         bool l0 = (nt.c0 & RT_LEAF_BIT) != 0, l1 = (nt.c1 & RT_LEAF_BIT) != 0;
-        if (nt.h0 && l0) { cnt.prims++; test_leaf<false>(recs, nt.c0 & ~RT_LEAF_BIT, o, d, tmin, RT_MISS, best_t, best, best_slot); }
-        if (nt.h1 && l1) { cnt.prims++; test_leaf<false>(recs, nt.c1 & ~RT_LEAF_BIT, o, d, tmin, RT_MISS, best_t, best, best_slot); }
+        if (nt.h0 && l0) { cnt.prims++; test_leaf<false, C::mask>(recs, nt.c0 & ~RT_LEAF_BIT, o, d, tmin, RT_MISS, best_t, best, best_slot); }
+        if (nt.h1 && l1) { cnt.prims++; test_leaf<false, C::mask>(recs, nt.c1 & ~RT_LEAF_BIT, o, d, tmin, RT_MISS, best_t, best, best_slot); }
         bool g0 = nt.h0 && !l0, g1 = nt.h1 && !l1;
         if (g0 && g1) {
             bool first0 = nt.tn0 <= nt.tn1;
@@ -1704,7 +1716,7 @@ __device__ __forceinline__ float cam_source_radiance(const CameraLaunch &c, v3f 
 // Camera pixels: one thread per pixel of the tile traces its antialiasing samples in order and adds
 // their contributions to camera_radiation[b][pixel] (a fixed order, so a render is reproducible; the
 // reference adds the samples with float atomics).  No specular term (this backend rejects it).
-template <int STK, bool COUNT>
+template <int STK, bool COUNT, bool MK>
 __global__ __launch_bounds__(256) void k_camera(CameraLaunch c, int *err, unsigned long long *stats) {
     constexpr int LSTK = STK < RT_CAMERA_LDS_STACK ? STK : RT_CAMERA_LDS_STACK;
     __shared__ uint32_t stk_all[LSTK * 256];
@@ -1717,7 +1729,7 @@ __global__ __launch_bounds__(256) void k_camera(CameraLaunch c, int *err, unsign
     const size_t pix = (size_t)jj * c.full_x + ii, npix = (size_t)c.full_x * c.full_y;
     const float strength = rdiv(1.0f, (float)c.spp);
     const uint32_t dimx = c.res_x * c.res_y, indx = c.res_x * py + px;
-    typename std::conditional<COUNT, TCnt, NoCnt>::type cnt{};
+    Cnt<COUNT, MK> cnt{};
     for (uint32_t smp = 0; smp < c.spp; smp++) {
         uint32_t st = tea16(indx + dimx * smp, c.seed);
         const float Rx = rnd24(st), Ry = rnd24(st);
@@ -1752,7 +1764,7 @@ __global__ __launch_bounds__(256) void k_camera(CameraLaunch c, int *err, unsign
 
 // Pixel labels (pixel_label_raygen.comp:80-155): the pixel-centre pinhole ray's hit UUID + 1 (0 = sky) and
 // its depth along the view direction (-1 on a miss).
-template <int STK>
+template <int STK, bool MK>
 __global__ __launch_bounds__(256) void k_pixel_label(CameraLaunch c, int *err) {
     constexpr int LSTK = STK < RT_CAMERA_LDS_STACK ? STK : RT_CAMERA_LDS_STACK;
     __shared__ uint32_t stk_all[LSTK * 256];
@@ -1767,7 +1779,7 @@ __global__ __launch_bounds__(256) void k_pixel_label(CameraLaunch c, int *err) {
     camera_ray(c.pos[0], c.pos[1], c.pos[2], c.st, c.ct, c.sp, c.cp, c.focal, 0.0f, c.aspect, c.viewplane, ii, jj,
                c.full_x, c.full_y, 0.5f, 0.5f, 0.0f, 0.0f, o, d);
     float t, depth;
-    NoCnt cnt{};
+    NoCntT<MK> cnt{};
     const uint32_t h = camera_trace<STK>(sc, o, d, t, depth, stk, ovf, err, cnt);
     if (h != RT_MISS) {
         c.labels[pix] = sc.uuid[h] + 1u;
@@ -1879,8 +1891,9 @@ static inline uint32_t grid_for(const LaunchCfg &cfg, int ncu, uint32_t total, u
         if ((depth) + 1 <= 32) { CALL32; } else { CALL64; } \
     } while (0)
 
-int launch_direct(const DirectJob *jobs, uint32_t njobs, uint32_t total, uint32_t depth, uint32_t *counter, int *err,
-                  unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st) {
+template <bool MK>
+static int launch_direct_t(const DirectJob *jobs, uint32_t njobs, uint32_t total, uint32_t depth, uint32_t *counter,
+                           int *err, unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st) {
     if (total == 0) return 0;
     const uint32_t ch = std::max(cfg.chunk, 1u);
     k_zero_u32<<<1, 1, 0, st>>>(counter);
@@ -1888,42 +1901,56 @@ int launch_direct(const DirectJob *jobs, uint32_t njobs, uint32_t total, uint32_
     uint32_t g = grid_for(cfg, ncu, total, wpb);
     if (stats) {
         // counting variant: per-lane stacks, 4 waves per block
-        PICK_STK(depth, (k_direct<4, 32, true><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)),
-                 (k_direct<4, 64, true><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)));
+        PICK_STK(depth, (k_direct<4, 32, true, true><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)),
+                 (k_direct<4, 64, true, true><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)));
         LCHECK();
         return 0;
     }
     switch (wpb) {
-        case 1: PICK_STK(depth, (k_direct<1, 32, false><<<g, 64, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)),
-                         (k_direct<1, 64, false><<<g, 64, 0, st>>>(jobs, njobs, total, counter, ch, err, stats))); break;
-        case 2: PICK_STK(depth, (k_direct<2, 32, false><<<g, 128, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)),
-                         (k_direct<2, 64, false><<<g, 128, 0, st>>>(jobs, njobs, total, counter, ch, err, stats))); break;
-        default: PICK_STK(depth, (k_direct<4, 32, false><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)),
-                          (k_direct<4, 64, false><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch, err, stats))); break;
+        case 1: PICK_STK(depth, (k_direct<1, 32, false, MK><<<g, 64, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)),
+                         (k_direct<1, 64, false, MK><<<g, 64, 0, st>>>(jobs, njobs, total, counter, ch, err, stats))); break;
+        case 2: PICK_STK(depth, (k_direct<2, 32, false, MK><<<g, 128, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)),
+                         (k_direct<2, 64, false, MK><<<g, 128, 0, st>>>(jobs, njobs, total, counter, ch, err, stats))); break;
+        default: PICK_STK(depth, (k_direct<4, 32, false, MK><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)),
+                          (k_direct<4, 64, false, MK><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch, err, stats))); break;
     }
     LCHECK();
     return 0;
 }
 
-int launch_direct_src(const DirectJob *jobs, uint32_t njobs, uint32_t item_lo, uint32_t nitems, uint32_t depth,
-                      uint32_t *counter, int *err, unsigned long long *stats, int ncu, hipStream_t st) {
+int launch_direct(const DirectJob *jobs, uint32_t njobs, uint32_t total, uint32_t depth, uint32_t *counter, int *err,
+                  unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st, bool masked) {
+    return masked ? launch_direct_t<true>(jobs, njobs, total, depth, counter, err, stats, cfg, ncu, st)
+                  : launch_direct_t<false>(jobs, njobs, total, depth, counter, err, stats, cfg, ncu, st);
+}
+
+template <bool MK>
+static int launch_direct_src_t(const DirectJob *jobs, uint32_t njobs, uint32_t item_lo, uint32_t nitems,
+                               uint32_t depth, uint32_t *counter, int *err, unsigned long long *stats, int ncu,
+                               hipStream_t st) {
     if (nitems == 0) return 0;
     k_zero_u32<<<1, 1, 0, st>>>(counter);
     uint64_t g64 = std::min<uint64_t>((uint64_t)4 * ncu, ((uint64_t)nitems + 3) / 4);
     uint32_t g = (uint32_t)std::max<uint64_t>(g64, 1);
     if (stats)
-        PICK_STK(depth, (k_direct_src<32, true><<<g, 256, 0, st>>>(jobs, njobs, item_lo, nitems, counter, err, stats)),
-                 (k_direct_src<64, true><<<g, 256, 0, st>>>(jobs, njobs, item_lo, nitems, counter, err, stats)));
+        PICK_STK(depth, (k_direct_src<32, true, true><<<g, 256, 0, st>>>(jobs, njobs, item_lo, nitems, counter, err, stats)),
+                 (k_direct_src<64, true, true><<<g, 256, 0, st>>>(jobs, njobs, item_lo, nitems, counter, err, stats)));
     else
-        PICK_STK(depth, (k_direct_src<32, false><<<g, 256, 0, st>>>(jobs, njobs, item_lo, nitems, counter, err, stats)),
-                 (k_direct_src<64, false><<<g, 256, 0, st>>>(jobs, njobs, item_lo, nitems, counter, err, stats)));
+        PICK_STK(depth, (k_direct_src<32, false, MK><<<g, 256, 0, st>>>(jobs, njobs, item_lo, nitems, counter, err, stats)),
+                 (k_direct_src<64, false, MK><<<g, 256, 0, st>>>(jobs, njobs, item_lo, nitems, counter, err, stats)));
     LCHECK();
     return 0;
 }
 
+int launch_direct_src(const DirectJob *jobs, uint32_t njobs, uint32_t item_lo, uint32_t nitems, uint32_t depth,
+                      uint32_t *counter, int *err, unsigned long long *stats, int ncu, hipStream_t st, bool masked) {
+    return masked ? launch_direct_src_t<true>(jobs, njobs, item_lo, nitems, depth, counter, err, stats, ncu, st)
+                  : launch_direct_src_t<false>(jobs, njobs, item_lo, nitems, depth, counter, err, stats, ncu, st);
+}
+
 int launch_direct_jobs(const DirectJob *hjobs, const DirectJob *jobs, uint32_t njobs, uint32_t total, uint32_t depth,
                        uint32_t *counter, int *err, unsigned long long *stats, const LaunchCfg &cfg, int ncu,
-                       hipStream_t st) {
+                       hipStream_t st, bool masked) {
     uint32_t ncol = 0;
     while (ncol < njobs && hjobs[ncol].stype == 0) ncol++;
     for (uint32_t k = ncol; k < njobs; k++)
@@ -1931,19 +1958,20 @@ int launch_direct_jobs(const DirectJob *hjobs, const DirectJob *jobs, uint32_t n
             set_error("internal: collimated direct jobs must precede the other source types");
             return -1;
         }
-    if (ncol == njobs) return launch_direct(jobs, njobs, total, depth, counter, err, stats, cfg, ncu, st);
-    if (ncol == 0) return launch_direct_src(jobs, njobs, 0, total, depth, counter, err, stats, ncu, st);
+    if (ncol == njobs) return launch_direct(jobs, njobs, total, depth, counter, err, stats, cfg, ncu, st, masked);
+    if (ncol == 0) return launch_direct_src(jobs, njobs, 0, total, depth, counter, err, stats, ncu, st, masked);
     const uint32_t split = hjobs[ncol].item_begin;
     if (split > total) {
         set_error("internal: direct job items out of order");
         return -1;
     }
-    if (launch_direct(jobs, ncol, split, depth, counter, err, stats, cfg, ncu, st)) return -1;
-    return launch_direct_src(jobs + ncol, njobs - ncol, split, total - split, depth, counter, err, stats, ncu, st);
+    if (launch_direct(jobs, ncol, split, depth, counter, err, stats, cfg, ncu, st, masked)) return -1;
+    return launch_direct_src(jobs + ncol, njobs - ncol, split, total - split, depth, counter, err, stats, ncu, st,
+                             masked);
 }
 
 int launch_diffuse(const DiffuseJob *job, uint32_t total, uint32_t depth, uint32_t *counter, int *err,
-                   unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st) {
+                   unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st, bool masked) {
     if (total == 0) return 0;
     // the stack's first RT_DIFFUSE_LDS_STACK entries live in LDS and the rest in private memory, so its
     // full depth (RT_STACK) costs no occupancy: one stack size serves the binary and the 4-wide tree
@@ -1957,14 +1985,22 @@ int launch_diffuse(const DiffuseJob *job, uint32_t total, uint32_t depth, uint32
     uint32_t g = grid_for(cfg, ncu, total, wpb);
     if (stats) {
         // counting variant: 4 waves per block
-        k_diffuse<4, RT_STACK, true><<<g, 256, 0, st>>>(job, total, counter, ch, err, stats);
+        k_diffuse<4, RT_STACK, true, true><<<g, 256, 0, st>>>(job, total, counter, ch, err, stats);
         LCHECK();
         return 0;
     }
-    switch (wpb) {
-        case 1: k_diffuse<1, RT_STACK, false><<<g, 64, 0, st>>>(job, total, counter, ch, err, stats); break;
-        case 2: k_diffuse<2, RT_STACK, false><<<g, 128, 0, st>>>(job, total, counter, ch, err, stats); break;
-        default: k_diffuse<4, RT_STACK, false><<<g, 256, 0, st>>>(job, total, counter, ch, err, stats); break;
+    if (masked) {
+        switch (wpb) {
+            case 1: k_diffuse<1, RT_STACK, false, true><<<g, 64, 0, st>>>(job, total, counter, ch, err, stats); break;
+            case 2: k_diffuse<2, RT_STACK, false, true><<<g, 128, 0, st>>>(job, total, counter, ch, err, stats); break;
+            default: k_diffuse<4, RT_STACK, false, true><<<g, 256, 0, st>>>(job, total, counter, ch, err, stats); break;
+        }
+    } else {
+        switch (wpb) {
+            case 1: k_diffuse<1, RT_STACK, false, false><<<g, 64, 0, st>>>(job, total, counter, ch, err, stats); break;
+            case 2: k_diffuse<2, RT_STACK, false, false><<<g, 128, 0, st>>>(job, total, counter, ch, err, stats); break;
+            default: k_diffuse<4, RT_STACK, false, false><<<g, 256, 0, st>>>(job, total, counter, ch, err, stats); break;
+        }
     }
     LCHECK();
     return 0;
@@ -1993,19 +2029,26 @@ int launch_gather(const GatherJob *jobs, uint32_t njobs, uint32_t total, uint32_
 }
 
 int launch_camera(const CameraLaunch &c, uint32_t depth, bool labels, int *err, unsigned long long *stats,
-                  hipStream_t st) {
+                  hipStream_t st, bool masked) {
     if ((uint64_t)c.res_x * c.res_y == 0) return 0;
     // one wave per 8 x 8 pixel block, four waves per workgroup
     const uint64_t blocks = (uint64_t)((c.res_x + 7) / 8) * ((c.res_y + 7) / 8);
     const uint32_t g = (uint32_t)((blocks + 3) / 4);
-    if (labels)
-        PICK_STK(depth, (k_pixel_label<32><<<g, 256, 0, st>>>(c, err)), (k_pixel_label<64><<<g, 256, 0, st>>>(c, err)));
+    if (labels && masked)
+        PICK_STK(depth, (k_pixel_label<32, true><<<g, 256, 0, st>>>(c, err)),
+                 (k_pixel_label<64, true><<<g, 256, 0, st>>>(c, err)));
+    else if (labels)
+        PICK_STK(depth, (k_pixel_label<32, false><<<g, 256, 0, st>>>(c, err)),
+                 (k_pixel_label<64, false><<<g, 256, 0, st>>>(c, err)));
     else if (stats)
-        PICK_STK(depth, (k_camera<32, true><<<g, 256, 0, st>>>(c, err, stats)),
-                 (k_camera<64, true><<<g, 256, 0, st>>>(c, err, stats)));
+        PICK_STK(depth, (k_camera<32, true, true><<<g, 256, 0, st>>>(c, err, stats)),
+                 (k_camera<64, true, true><<<g, 256, 0, st>>>(c, err, stats)));
+    else if (masked)
+        PICK_STK(depth, (k_camera<32, false, true><<<g, 256, 0, st>>>(c, err, nullptr)),
+                 (k_camera<64, false, true><<<g, 256, 0, st>>>(c, err, nullptr)));
     else
-        PICK_STK(depth, (k_camera<32, false><<<g, 256, 0, st>>>(c, err, nullptr)),
-                 (k_camera<64, false><<<g, 256, 0, st>>>(c, err, nullptr)));
+        PICK_STK(depth, (k_camera<32, false, false><<<g, 256, 0, st>>>(c, err, nullptr)),
+                 (k_camera<64, false, false><<<g, 256, 0, st>>>(c, err, nullptr)));
     LCHECK();
     return 0;
 }

@@ -52,15 +52,50 @@ namespace helios {
         for (size_t i = 0; i < P; i++) {
             twosided[i] = static_cast<uint32_t>(static_cast<unsigned char>(geometry.twosided_flags.at(i)));
         }
-        for (size_t i = 0; i < geometry.mask_IDs.size(); i++) {
-            if (geometry.mask_IDs[i] >= 0) {
-                unsupported("updateGeometry", "texture-masked (alpha) primitives");
-            }
-        }
         check(rt_update_geometry(ctx_, static_cast<uint32_t>(P), geometry.transform_matrices.data(), types.data(), uuids.empty() ? nullptr : uuids.data(), twosided.data(),
                                  geometry.periodic_flag.x != 0.f ? 1 : 0, geometry.periodic_flag.y != 0.f ? 1 : 0),
               "updateGeometry");
         primitive_count_ = P;
+        bool any_mask = false;
+        for (int id: geometry.mask_IDs) {
+            any_mask = any_mask || id >= 0;
+        }
+        if (!any_mask) {
+            return;
+        }
+        // texture masks: bool texels -> bytes, compact uv_data (4 entries per primitive with uv_IDs >= 0, in
+        // primitive order) -> [P][4][2] (VulkanComputeBackend.cpp:439-470)
+        const size_t M = geometry.mask_sizes.size();
+        std::vector<uint32_t> sizes(2 * M);
+        size_t ntex = 0;
+        for (size_t m = 0; m < M; m++) {
+            sizes[2 * m] = static_cast<uint32_t>(geometry.mask_sizes[m].x);
+            sizes[2 * m + 1] = static_cast<uint32_t>(geometry.mask_sizes[m].y);
+            ntex += size_t(geometry.mask_sizes[m].x) * size_t(geometry.mask_sizes[m].y);
+        }
+        if (geometry.mask_data.size() < ntex) {
+            helios_runtime_error("ERROR (HipRayTracingBackend::updateGeometry): mask_data holds " + std::to_string(geometry.mask_data.size()) + " texels, the masks need " + std::to_string(ntex) + ".");
+        }
+        std::vector<uint8_t> texels(ntex);
+        for (size_t k = 0; k < ntex; k++) {
+            texels[k] = geometry.mask_data[k] ? 1 : 0;
+        }
+        std::vector<int32_t> mask_ids(P, -1), uv_ids(P, -1);
+        std::vector<float> uv(8 * P, 0.f);
+        size_t next = 0;
+        for (size_t p = 0; p < P; p++) {
+            if (p < geometry.mask_IDs.size()) {
+                mask_ids[p] = geometry.mask_IDs[p];
+            }
+            if (p < geometry.uv_IDs.size() && geometry.uv_IDs[p] >= 0) {
+                uv_ids[p] = geometry.uv_IDs[p];
+                for (int v = 0; v < 4 && next < geometry.uv_data.size(); v++, next++) {
+                    uv[8 * p + 2 * v] = geometry.uv_data[next].x;
+                    uv[8 * p + 2 * v + 1] = geometry.uv_data[next].y;
+                }
+            }
+        }
+        check(rt_update_texture_masks(ctx_, static_cast<uint32_t>(P), static_cast<uint32_t>(M), texels.data(), sizes.data(), mask_ids.data(), uv.data(), uv_ids.data()), "updateGeometry");
     }
 
     void HipRayTracingBackend::buildAccelerationStructure() {
