@@ -244,7 +244,8 @@ int rt_run_band_batch(rt_ctx *ctx, const rt_scene *scene, const rt_band_request 
     NN(suns->direct_seed, "suns.direct_seed");
     std::lock_guard<std::mutex> lk(ctx->mu);
     CKR(hipSetDevice(ctx->device));
-    SunSet s{suns->nsuns, suns->source_dir, suns->source_flux, suns->diffuse_flux, suns->direct_seed};
+    SunSet s{suns->nsuns, suns->source_dir, suns->source_flux, suns->diffuse_flux, suns->direct_seed,
+             suns->diffuse_peak_dir};
     return run_solve(ctx->sx, scene->s, *base, s, R_out, out_on_device);
 }
 

@@ -454,7 +454,40 @@ int rt_launch_diffuse_rays(rt_ctx *ctx, const rt_launch_params *p) {
     if (p->launch_count == 0 || p->rays_per_primitive == 0) return 0;
     const uint32_t n = (uint32_t)std::ceil(std::sqrt((double)p->rays_per_primitive));
     const size_t nn = (size_t)n * n;
-    size_t bytes = (size_t)P * 2 * nn * 4 + 2 * PB * 4 + Bl * 9 + sizeof(DiffuseJob) + sizeof(GatherJob) + 16 * 256;
+    // anisotropic sky (diffuse_raygen.comp:613-625): power law from the launch, Prague from the sky model
+    std::vector<float> hsky;
+    {
+        bool any = false;
+        for (uint32_t b = 0; b < Bl; b++) {
+            // the sky weight only scales sky flux (diffuse_raygen.comp:616): bands without any need none
+            if (!(p->diffuse_flux && p->diffuse_flux[b] > 0.f)) continue;
+            const bool pl = p->diffuse_extinction && p->diffuse_extinction[b] > 0.f;
+            const bool pr = be.has_sky && be.sky_B == Bl && be.sky_params[4 * b + 3] > 0.f;
+            if (pl && !p->diffuse_peak_dir) {
+                set_error("rt_launch_diffuse_rays: a power-law sky needs diffuse_peak_dir");
+                return -1;
+            }
+            if (pl && !p->diffuse_dist_norm) {
+                set_error("rt_launch_diffuse_rays: a power-law sky needs diffuse_dist_norm");
+                return -1;
+            }
+            any = any || pl || pr;
+        }
+        if (any) {
+            hsky.assign((size_t)Bl * 9, 0.f);
+            for (uint32_t b = 0; b < Bl; b++) {
+                float *q = hsky.data() + (size_t)b * 9;
+                q[0] = p->diffuse_extinction ? p->diffuse_extinction[b] : 0.f;
+                q[1] = p->diffuse_dist_norm ? p->diffuse_dist_norm[b] : 0.f;
+                // peak: the launch's, else the sky model's sun direction
+                for (int a = 0; a < 3; a++) q[2 + a] = p->diffuse_peak_dir ? p->diffuse_peak_dir[3 * b + a] : be.sun_dir[a];
+                if (be.has_sky && be.sky_B == Bl)
+                    for (int a = 0; a < 4; a++) q[5 + a] = be.sky_params[4 * b + a];
+            }
+        }
+    }
+    size_t bytes = (size_t)P * 2 * nn * 4 + 2 * PB * 4 + Bl * 9 + sizeof(DiffuseJob) + sizeof(GatherJob) +
+                   hsky.size() * 4 + 17 * 256;
     if (be.ws.reserve(bytes)) return -1;
     Carve cv{(char *)be.ws.base};
     uint32_t *hits = cv.take<uint32_t>((size_t)P * 2 * nn);
@@ -464,6 +497,7 @@ int rt_launch_diffuse_rays(rt_ctx *ctx, const rt_launch_params *p) {
     uint8_t *dact = cv.take<uint8_t>(Bl);
     DiffuseJob *fjob = cv.take<DiffuseJob>(1);
     GatherJob *gjob = cv.take<GatherJob>(1);
+    float *dsky = hsky.empty() ? nullptr : cv.take<float>(hsky.size());
     uint32_t *counter = cv.take<uint32_t>(16);
     int *err = reinterpret_cast<int *>(counter + 8);
     std::vector<float> hf(Bl, 0.f);
@@ -494,7 +528,10 @@ int rt_launch_diffuse_rays(rt_ctx *ctx, const rt_launch_params *p) {
     gj.sb = be.rbuf(2);
     gj.item_begin = range_begin(p->launch_offset);
     gj.P = P;
+    gj.sky = dsky;
+    gj.seed = p->random_seed;
     CKR(hipMemsetAsync(err, 0, sizeof(int), st));
+    if (dsky) CKR(hipMemcpyAsync(dsky, hsky.data(), hsky.size() * 4, hipMemcpyHostToDevice, st));
     CKR(hipMemcpyAsync(dmap, map.data(), Bl * 4, hipMemcpyHostToDevice, st));
     CKR(hipMemcpyAsync(dfl, hf.data(), Bl * 4, hipMemcpyHostToDevice, st));
     CKR(hipMemcpyAsync(dact, active.data(), Bl, hipMemcpyHostToDevice, st));

@@ -439,6 +439,31 @@ RT_HD bool origin_mask(const float4 *base, uint32_t didx, bool patch, bool diffu
     return !masked;
 }
 
+// ---- sky radiance distributions (R/shaders/common/sky_distribution.glsl:40-116) ----
+// Prague pattern (Perez-type circumsolar and horizon brightening) times its normalisation and pi
+RT_HD float sky_prague(v3f d, float px, float py, float pz, const float *prm) {
+    const float c = fminf(1.0f, fmaxf(-1.0f, d.x * px + d.y * py + d.z * pz));
+    const float gamma = acosf(c) * (180.0f / RT_PI_F);
+    const float cos_theta = fmaxf(0.0f, d.z);
+    const float circ = 1.0f + prm[0] * expf(-gamma / prm[1]);
+    const float horizon = 1.0f + (prm[2] - 1.0f) * (1.0f - cos_theta);
+    return circ * horizon * prm[3] * RT_PI_F;
+}
+
+// evaluate_sky_distribution: p = {power-law K, power-law normalisation, peak direction xyz, Prague
+// circumsolar strength, width (degrees), horizon brightness, normalisation}.  Power law when K > 0
+// (angular distance from the peak clamped to 1 degree), else Prague when its normalisation is positive,
+// else isotropic (1).
+RT_HD float sky_weight(v3f d, const float *p) {
+    if (p[0] > 0.0f) {
+        const float c = fminf(1.0f, fmaxf(-1.0f, p[2] * d.x + p[3] * d.y + p[4] * d.z));
+        const float psi = fmaxf(acosf(c), RT_PI_F / 180.0f);
+        return powf(psi, -p[0]) * p[1];
+    }
+    if (p[8] > 0.0f) return sky_prague(d, p[2], p[3], p[4], p + 5);
+    return 1.0f;
+}
+
 // Diffuse-pass sample point and local hemisphere direction (diffuse_raygen.comp:375-482)
 RT_HD void diffuse_sample(uint32_t prim, uint32_t lx, uint32_t ly, uint32_t n, uint32_t seed, bool patch, float &spx,
                           float &spy, v3f &dl) {

@@ -38,6 +38,7 @@ struct Model {
     rt_band_request q{};  // host copy; pointers aim at the vectors below (rho/tau/eps/temp are null)
     std::vector<uint32_t> depth, emission, dseed, stype;
     std::vector<float> swidth, srot;
+    std::vector<float> kext, kpeak, knorm, kprague;  // anisotropic sky of the model's bands
     void *dmem = nullptr;  // resident rho, tau[, eps][, temp]
     DevMaterials dm{};
     size_t resident_bytes = 0;
@@ -341,6 +342,19 @@ int rt_server_add_model(rt_server *srv, const rt_scene *scene, const rt_band_req
                 m->q.source_rotation = m->srot.data();
             }
         }
+    }
+    // anisotropic sky: copied with the model (every request of the model shares its peak directions)
+    {
+        const uint32_t Bq = q->nbands;
+        auto keep = [](std::vector<float> &v, const float *src, size_t n) -> const float * {
+            if (!src) return nullptr;
+            v.assign(src, src + n);
+            return v.data();
+        };
+        m->q.diffuse_extinction = keep(m->kext, q->diffuse_extinction, Bq);
+        m->q.diffuse_peak_dir = keep(m->kpeak, q->diffuse_peak_dir, (size_t)3 * Bq);
+        m->q.diffuse_dist_norm = keep(m->knorm, q->diffuse_dist_norm, Bq);
+        m->q.sky_params = keep(m->kprague, q->sky_params, (size_t)4 * Bq);
     }
     m->q.rho = m->q.tau = m->q.eps = m->q.temperature = nullptr;
     // resident optical properties

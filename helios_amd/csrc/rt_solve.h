@@ -86,6 +86,7 @@ struct SunSet {
     const float *flux;    // [K][S][B]
     const float *dflux;   // [K][B]
     const uint32_t *seed; // [K]
+    const float *peak = nullptr;  // [K][B][3] anisotropic-sky peak directions, null = the request's
 };
 
 // Optical properties already resident in device memory ([P][B] rho, tau, eps or null, [P] temp or null).

@@ -266,6 +266,40 @@ static int ev_pair(SolveCtx &sx, hipEvent_t &a, hipEvent_t &b) {
     return 0;
 }
 
+// Per-channel sky_weight parameters ([C][9]: K, norm, peak xyz, Prague 4) of a request whose diffuse sky
+// is anisotropic in some band; empty when every band's sky is isotropic.
+static int sky_table(const rt_band_request &q, const SunSet &suns, uint32_t K, std::vector<float> &out) {
+    const uint32_t B = q.nbands;
+    out.clear();
+    bool any = false;
+    for (uint32_t b = 0; b < B; b++) {
+        const bool pl = q.diffuse_extinction && q.diffuse_extinction[b] > 0.f;
+        const bool pr = q.sky_params && q.sky_params[4 * b + 3] > 0.f;
+        if ((pl || pr) && !q.diffuse_peak_dir && !suns.peak) {
+            set_error("run_band: an anisotropic sky (diffuse_extinction or sky_params) needs diffuse_peak_dir");
+            return -1;
+        }
+        if (pl && !q.diffuse_dist_norm) {
+            set_error("run_band: a power-law sky (diffuse_extinction > 0) needs diffuse_dist_norm");
+            return -1;
+        }
+        any = any || pl || pr;
+    }
+    if (!any) return 0;
+    out.assign((size_t)K * B * 9, 0.f);
+    for (uint32_t k = 0; k < K; k++)
+        for (uint32_t b = 0; b < B; b++) {
+            float *p = out.data() + ((size_t)k * B + b) * 9;
+            p[0] = q.diffuse_extinction ? q.diffuse_extinction[b] : 0.f;
+            p[1] = q.diffuse_dist_norm ? q.diffuse_dist_norm[b] : 0.f;
+            const float *pk = suns.peak ? suns.peak + ((size_t)k * B + b) * 3 : (q.diffuse_peak_dir ? q.diffuse_peak_dir + 3 * b : nullptr);
+            if (pk) { p[2] = pk[0]; p[3] = pk[1]; p[4] = pk[2]; }
+            if (q.sky_params)
+                for (int a = 0; a < 4; a++) p[5 + a] = q.sky_params[4 * b + a];
+        }
+    return 0;
+}
+
 size_t solve_workspace_bytes(uint32_t P, const rt_band_request &q, uint32_t K, bool resident_materials) {
     const uint32_t B = q.nbands, Sn = q.nsources, C = K * B;
     uint32_t maxdepth = 0;
@@ -279,7 +313,7 @@ size_t solve_workspace_bytes(uint32_t P, const rt_band_request &q, uint32_t K, b
     size_t bytes = 5 * PC * 4 + (need_hits ? (size_t)P * 2 * nn * 4 : 0) + (size_t)K * Sn * P * 2 * 4 + (size_t)P * 8 +
                    (size_t)K * Sn * B * 4 + (size_t)C * 4 * 2 + (size_t)nm * C + (size_t)B * 4 + (size_t)C * 8 * 3 +
                    (size_t)K * Sn * sizeof(DirectJob) + (sizeof(DiffuseJob) + sizeof(GatherJob)) * (1 + maxdepth) +
-                   64 * 64;
+                   (size_t)C * 9 * 4 + 64 * 64;
     if (!resident_materials) bytes += (size_t)P * B * 4 * 3 + (size_t)P * 4;
     return bytes + 32 * 256;  // carve alignment
 }
@@ -350,6 +384,9 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
     }
     std::vector<float> dfl((size_t)C, 0.f);
     for (uint32_t c = 0; c < C; c++) dfl[c] = suns.dflux[(size_t)(c / B) * B + c % B];
+    // anisotropic sky of each channel (power law and/or Prague), when any band has one
+    std::vector<float> hsky;
+    if (sky_table(q, suns, K, hsky)) return -1;
 
     // ---- workspace ----
     size_t PC = (size_t)P * C;
@@ -374,6 +411,7 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
     DirectJob *djobs = cv.take<DirectJob>((size_t)K * Sn + 1);
     DiffuseJob *fjobs = cv.take<DiffuseJob>(1 + maxdepth);
     GatherJob *gjobs = cv.take<GatherJob>(1 + maxdepth);
+    float *dsky = hsky.empty() ? nullptr : cv.take<float>(hsky.size());
     uint32_t *counter = cv.take<uint32_t>(16);
     int *err = reinterpret_cast<int *>(counter + 8);
 
@@ -393,6 +431,7 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
     if (Sn) CK(hipMemcpyAsync(flux, suns.flux, (size_t)K * Sn * B * 4, hipMemcpyHostToDevice, st));
     CK(hipMemcpyAsync(dflux, dfl.data(), (size_t)C * 4, hipMemcpyHostToDevice, st));
     CK(hipMemsetAsync(zflux, 0, (size_t)C * 4, st));
+    if (dsky) CK(hipMemcpyAsync(dsky, hsky.data(), hsky.size() * 4, hipMemcpyHostToDevice, st));
     CK(hipMemcpyAsync(masks, mk.data(), mk.size(), hipMemcpyHostToDevice, st));
     CK(hipMemcpyAsync(depth, q.band_depth, (size_t)B * 4, hipMemcpyHostToDevice, st));
 
@@ -462,6 +501,8 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
         J.rho = rho; J.tau = tau; J.in = in; J.st = stb; J.sb = sbb;
         J.item_begin = 0; J.P = P;
         J.order = S.h.order;
+        J.sky = g == 0 ? dsky : nullptr;  // the sky is seen by the primary diffuse pass only
+        J.seed = q.diffuse_seed[g];
     }
     CK(hipMemcpyAsync(gjobs, hg.data(), hg.size() * sizeof(GatherJob), hipMemcpyHostToDevice, st));
     // the host vectors above must outlive the async copies

@@ -1296,6 +1296,21 @@ __global__ __launch_bounds__(WPB * 64) void k_diffuse(const DiffuseJob *__restri
 // ------------------------------------------------------------------------------------------------
 // gather: lanes over rays (few channels)
 // ------------------------------------------------------------------------------------------------
+// Direction of diffuse ray r (face slot r / nn) of primitive i, regenerated from the pass seed exactly as
+// the trace kernel generated it: the gather weighs a sky hit by the sky distribution in that direction.
+__device__ __forceinline__ v3f diffuse_dir(const DevScene &sc, uint32_t i, uint32_t r, uint32_t n, uint32_t seed) {
+    const uint32_t nn = n * n, slot = r >= nn ? 1u : 0u, rr = r - slot * nn;
+    const uint32_t ty = sc.type[i];
+    const uint32_t mdi = sc.mdesc ? sc.mdesc[i] : RT_MISS;
+    float spx, spy;
+    v3f dl;
+    (void)diffuse_sample_m(i, rr % n, rr / n, n, seed, ty == 0 || ty == 3, sc.rec, mdi, spx, spy, dl);
+    const float4 fr = sc.frame[i];
+    v3f dir = frame_rotate(dl, fr.x, fr.y, fr.z, fr.w);
+    if (slot == 1 && __float_as_uint(sc.nrm[i].w) == 1u) dir = mk3(-dir.x, -dir.y, -dir.z);
+    return dir;
+}
+
 template <int CM>
 __global__ __launch_bounds__(256) void k_gather_rays(const GatherJob *__restrict__ jobs, uint32_t njobs, uint32_t total,
                                                      uint32_t *counter, uint32_t chunk) {
@@ -1320,7 +1335,16 @@ __global__ __launch_bounds__(256) void k_gather_rays(const GatherJob *__restrict
                 uint32_t word = w[r];
                 if (word == RT_NOT_LAUNCHED) continue;
                 uint32_t slot = r >= nn ? 1u : 0u;
-                if (word == RT_MISS) {
+                if (word == RT_MISS && J.sky) {
+                    // anisotropic sky: fd(direction) x diffuse flux (diffuse_raygen.comp:613-625)
+                    const v3f dir = diffuse_dir(*J.sc, i, r, J.n, J.seed);
+#pragma unroll
+                    for (int c = 0; c < CM; c++)
+                        if ((uint32_t)c < C) {
+                            float v = sky_weight(dir, J.sky + 9 * c) * J.dflux[c];
+                            if (slot) acc[1][c].add(v); else acc[0][c].add(v);
+                        }
+                } else if (word == RT_MISS) {
 #pragma unroll
                     for (int c = 0; c < CM; c++)
                         if ((uint32_t)c < C) {
@@ -1393,9 +1417,14 @@ __global__ __launch_bounds__(256) void k_gather_chan(const GatherJob *__restrict
                 acc[1][k].init();
             }
             const float *dfl = J.dflux, *otop = J.out_top, *obot = J.out_bot;
+            const float *sky = J.sky;
             for (uint32_t base = 0; base < 2 * nn; base += 64) {
                 uint32_t mine = base + lane < 2 * nn ? w[base + lane] : RT_NOT_LAUNCHED;
                 uint32_t cnt = min(64u, 2 * nn - base);
+                // anisotropic sky: each lane regenerates the direction of its own ray if it missed; the
+                // directions are broadcast with the words
+                v3f mdir = mk3(0.f, 0.f, 1.f);
+                if (sky && mine == RT_MISS) mdir = diffuse_dir(*J.sc, i, base + lane, J.n, J.seed);
                 // RT_GATHER_UNROLL hit words per step: every row (the hit primitive's radiation_out C-vector,
                 // or the sky fluxes for a miss) is loaded before any is summed, so that many rows are in
                 // flight per wave; the sums take them in ray order.  A face slot that was not launched adds
@@ -1413,6 +1442,14 @@ __global__ __launch_bounds__(256) void k_gather_chan(const GatherJob *__restrict
                         for (int k = 0; k < CB; k++) {
                             uint32_t c = lane + 64 * k;
                             v[u][k] = (!skip && c < C) ? src[c] : 0.f;
+                        }
+                        if (sky && word == RT_MISS) {
+                            const v3f dq = mk3(__shfl(mdir.x, q), __shfl(mdir.y, q), __shfl(mdir.z, q));
+#pragma unroll
+                            for (int k = 0; k < CB; k++) {
+                                uint32_t c = lane + 64 * k;
+                                if (c < C) v[u][k] = sky_weight(dq, sky + 9 * c) * v[u][k];
+                            }
                         }
                     }
 #pragma unroll
@@ -1674,12 +1711,7 @@ __device__ __forceinline__ bool camera_pixel(const CameraLaunch &c, uint32_t &px
 // Prague sky pattern (sky_distribution.glsl:60-90), or 1 (isotropic) when its normalisation is not positive.
 __device__ __forceinline__ float cam_sky_weight(v3f d, const float *sun, const float *prm) {
     if (!(prm[3] > 0.0f)) return 1.0f;
-    const float c = fminf(1.0f, fmaxf(-1.0f, d.x * sun[0] + d.y * sun[1] + d.z * sun[2]));
-    const float gamma = acosf(c) * (180.0f / RT_PI_F);
-    const float cos_theta = fmaxf(0.0f, d.z);
-    const float circ = 1.0f + prm[0] * expf(-gamma / prm[1]);
-    const float horizon = 1.0f + (prm[2] - 1.0f) * (1.0f - cos_theta);
-    return circ * horizon * prm[3] * RT_PI_F;
+    return sky_prague(d, sun[0], sun[1], sun[2], prm);
 }
 
 // Radiance of the sources a camera ray meets before tmax, for band b, summed in source order

@@ -143,4 +143,6 @@ struct GatherJob {
     uint32_t item_begin;
     uint32_t P;
     const uint32_t *order; // item -> primitive map (null: item k is primitive k)
+    const float *sky;      // [C][9] anisotropic sky of each channel (sky_weight parameters), null = isotropic
+    uint32_t seed;         // seed of the diffuse pass whose hit words these are (miss directions regenerated)
 };

@@ -135,13 +135,9 @@ namespace helios {
               "updateSources");
     }
 
-    void HipRayTracingBackend::updateDiffuseRadiation(const std::vector<float> &flux, const std::vector<float> &extinction, const std::vector<helios::vec3> &peak_dir, const std::vector<float> &dist_norm,
-                                                      const std::vector<float> &sky_energy) {
-        for (float K: extinction) {
-            if (K > 0.f) {
-                unsupported("updateDiffuseRadiation", "anisotropic (power-law) diffuse distribution");
-            }
-        }
+    void HipRayTracingBackend::updateDiffuseRadiation(const std::vector<float> &, const std::vector<float> &, const std::vector<helios::vec3> &, const std::vector<float> &, const std::vector<float> &) {
+        // the diffuse distribution travels with each launch (RayTracingLaunchParams), as in the Vulkan
+        // backend (VulkanComputeBackend.cpp:752-757)
     }
 
     void HipRayTracingBackend::updateSkyModel(const std::vector<helios::vec4> &sky_radiance_params, const std::vector<float> &camera_sky_radiance, const helios::vec3 &sun_direction, const std::vector<float> &solar_disk_radiance,
@@ -227,18 +223,26 @@ namespace helios {
         if (primitive_count_ == 0) {
             return;
         }
-        for (float K: params.diffuse_extinction) {
-            if (K > 0.f) {
-                unsupported("launchDiffuseRays", "anisotropic (power-law) diffuse distribution");
-            }
-        }
-        for (const auto &q: params.sky_radiance_params) {
-            if (q.w > 0.f) {
-                unsupported("launchDiffuseRays", "Prague sky radiance distribution");
-            }
-        }
         bandTable(params, "launchDiffuseRays");
         rt_launch_params p = cParams(params);
+        // anisotropic sky: power law from the launch, Prague from updateSkyModel (diffuse_raygen.comp:613-625)
+        const size_t Bl = params.num_bands_launch;
+        std::vector<float> peak;
+        if (params.diffuse_extinction.size() >= Bl) {
+            p.diffuse_extinction = params.diffuse_extinction.data();
+        }
+        if (params.diffuse_dist_norm.size() >= Bl) {
+            p.diffuse_dist_norm = params.diffuse_dist_norm.data();
+        }
+        if (params.diffuse_peak_dir.size() >= Bl) {
+            peak.resize(3 * Bl);
+            for (size_t b = 0; b < Bl; b++) {
+                peak[3 * b] = params.diffuse_peak_dir[b].x;
+                peak[3 * b + 1] = params.diffuse_peak_dir[b].y;
+                peak[3 * b + 2] = params.diffuse_peak_dir[b].z;
+            }
+            p.diffuse_peak_dir = peak.data();
+        }
         check(rt_launch_diffuse_rays(ctx_, &p), "launchDiffuseRays");
     }
 

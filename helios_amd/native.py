@@ -38,12 +38,13 @@ class rt_band_request(C.Structure):
         ("rho", f32p), ("tau", f32p), ("eps", f32p), ("temperature", f32p),
         ("direct_seed", C.c_uint32), ("diffuse_seed", u32p), ("schedule", C.c_uint32),
         ("source_type", u32p), ("source_width", f32p), ("source_rotation", f32p),
+        ("diffuse_extinction", f32p), ("diffuse_peak_dir", f32p), ("diffuse_dist_norm", f32p), ("sky_params", f32p),
     ]
 
 
 class rt_sun_batch(C.Structure):
     _fields_ = [("nsuns", C.c_uint32), ("source_dir", f32p), ("source_flux", f32p),
-                ("diffuse_flux", f32p), ("direct_seed", u32p)]
+                ("diffuse_flux", f32p), ("direct_seed", u32p), ("diffuse_peak_dir", f32p)]
 
 
 class rt_scene_info(C.Structure):
@@ -73,7 +74,8 @@ class rt_launch_params(C.Structure):
     _fields_ = [("launch_offset", C.c_uint32), ("launch_count", C.c_uint32), ("rays_per_primitive", C.c_uint32),
                 ("random_seed", C.c_uint32), ("num_bands_global", C.c_uint32), ("num_bands_launch", C.c_uint32),
                 ("band_map", u32p), ("band_active", C.POINTER(C.c_uint8)), ("launch_face", C.c_uint32),
-                ("diffuse_flux", f32p), ("radiation_out_top", f32p), ("radiation_out_bottom", f32p)]
+                ("diffuse_flux", f32p), ("radiation_out_top", f32p), ("radiation_out_bottom", f32p),
+                ("diffuse_extinction", f32p), ("diffuse_peak_dir", f32p), ("diffuse_dist_norm", f32p)]
 
 
 class rt_camera_params(C.Structure):
@@ -474,7 +476,7 @@ class Backend:
         _check(self.lib.rt_upload_source_fluxes(self.ctx._h, _pf(f), f.size))
 
     def _params(self, rays, seed, nbands_global, band_map=None, band_active=None, launch_face=1,
-                diffuse_flux=None, radiation_out=None, offset=0, count=None):
+                diffuse_flux=None, radiation_out=None, offset=0, count=None, sky=None):
         keep = []
         bm = None if band_map is None else _u32(band_map)
         ba = None if band_active is None else np.ascontiguousarray(band_active, dtype=np.uint8)
@@ -482,11 +484,12 @@ class Backend:
         ot = ob = None
         if radiation_out is not None:
             ot, ob = _f32(radiation_out[0]).reshape(-1), _f32(radiation_out[1]).reshape(-1)
-        keep += [bm, ba, df, ot, ob]
+        sk = [None, None, None] if sky is None else [None if a is None else _f32(a).reshape(-1) for a in sky]
+        keep += [bm, ba, df, ot, ob] + sk
         p = rt_launch_params(int(offset), int(self.P - offset if count is None else count), int(rays), int(seed),
                              int(nbands_global), int(self.B_launch), _pu(bm),
                              None if ba is None else ba.ctypes.data_as(C.POINTER(C.c_uint8)), int(launch_face),
-                             _pf(df), _pf(ot), _pf(ob))
+                             _pf(df), _pf(ot), _pf(ob), _pf(sk[0]), _pf(sk[1]), _pf(sk[2]))
         return p, keep
 
     def launch_direct_rays(self, rays_per_primitive: int, seed: int, nbands_global: int, band_map=None,
@@ -496,9 +499,13 @@ class Backend:
 
     def launch_diffuse_rays(self, rays_per_primitive: int, seed: int, nbands_global: int, launch_face: int,
                             diffuse_flux=None, band_map=None, band_active=None, radiation_out=None,
-                            offset: int = 0, count: Optional[int] = None):
+                            offset: int = 0, count: Optional[int] = None, sky_extinction=None, sky_peak_dir=None,
+                            sky_norm=None):
+        """sky_*: per launch band power-law K, peak direction [B][3] and normalisation (the Prague
+        parameters come from update_sky_model)."""
+        sky = None if sky_extinction is None and sky_peak_dir is None else (sky_extinction, sky_peak_dir, sky_norm)
         p, keep = self._params(rays_per_primitive, seed, nbands_global, band_map, band_active, launch_face,
-                               diffuse_flux, radiation_out, offset, count)
+                               diffuse_flux, radiation_out, offset, count, sky)
         _check(self.lib.rt_launch_diffuse_rays(self.ctx._h, C.byref(p)))
 
     def get_radiation_results(self) -> dict:
@@ -643,10 +650,11 @@ class BandRequest:
     def __init__(self, P: int, bands: Sequence[BandSpec], rho, tau, direct_rays: int, diffuse_rays: int,
                  source_dirs=None, source_flux=None, direct_seed: int = 1, diffuse_seeds=None, eps=None,
                  temperature=None, schedule: int = 0, source_types=None, source_widths=None,
-                 source_rotations=None):
+                 source_rotations=None, sky_extinction=None, sky_peak_dir=None, sky_norm=None, sky_prague=None):
         """source_dirs [S][3]: direction towards a collimated source, else the source position;
         source_types [S] (SOURCE_*; None = all collimated), source_widths [S][2] and source_rotations [S][3]
-        as RayTracingSource::width / ::rotation."""
+        as RayTracingSource::width / ::rotation.  Anisotropic sky per band: sky_extinction [B] (power-law K),
+        sky_peak_dir [B][3], sky_norm [B] (power-law normalisation), sky_prague [B][4]."""
         B = len(bands)
         self.P, self.B = P, B
         self.bands = list(bands)
@@ -670,13 +678,17 @@ class BandRequest:
         self.stype = None if source_types is None else _u32(source_types).reshape(S)
         self.swidth = None if source_widths is None else _f32(source_widths).reshape(S, 2)
         self.srot = None if source_rotations is None else _f32(source_rotations).reshape(S, 3)
+        self.sky_ext = None if sky_extinction is None else _f32(sky_extinction).reshape(B)
+        self.sky_peak = None if sky_peak_dir is None else _f32(sky_peak_dir).reshape(B, 3)
+        self.sky_norm = None if sky_norm is None else _f32(sky_norm).reshape(B)
+        self.sky_prague = None if sky_prague is None else _f32(sky_prague).reshape(B, 4)
 
     def c_struct(self) -> rt_band_request:
         return rt_band_request(
             self.B, self.sdir.shape[0], self.direct_rays, self.diffuse_rays, _pu(self.depth), _pu(self.emission),
             _pf(self.sdir), _pf(self.sflux), _pf(self.dflux), _pf(self.rho), _pf(self.tau), _pf(self.eps),
             _pf(self.temp), self.direct_seed, _pu(self.dseeds), self.schedule, _pu(self.stype), _pf(self.swidth),
-            _pf(self.srot))
+            _pf(self.srot), _pf(self.sky_ext), _pf(self.sky_peak), _pf(self.sky_norm), _pf(self.sky_prague))
 
 
 def run_band(scene: Scene, req: BandRequest) -> np.ndarray:
@@ -688,15 +700,17 @@ def run_band(scene: Scene, req: BandRequest) -> np.ndarray:
 
 
 def run_band_batch(scene: Scene, req: BandRequest, sun_dirs, sun_flux, sky_flux, direct_seeds,
-                   out_device_ptr: Optional[int] = None) -> Optional[np.ndarray]:
-    """K suns sharing bands/materials/diffuse seeds -> radiation_flux [K][P][B]."""
+                   out_device_ptr: Optional[int] = None, sky_peak_dirs=None) -> Optional[np.ndarray]:
+    """K suns sharing bands/materials/diffuse seeds -> radiation_flux [K][P][B]; sky_peak_dirs [K][B][3]
+    (anisotropic sky peak of each sun, None = the request's)."""
     D = _f32(sun_dirs)
     K = D.shape[0]
     D = D.reshape(K, -1, 3)
     F = _f32(sun_flux).reshape(K, D.shape[1], req.B)
     Q = _f32(sky_flux).reshape(K, req.B)
     seeds = _u32(direct_seeds).reshape(K)
-    sb = rt_sun_batch(K, _pf(D), _pf(F), _pf(Q), _pu(seeds))
+    PK = None if sky_peak_dirs is None else _f32(sky_peak_dirs).reshape(K, req.B, 3)
+    sb = rt_sun_batch(K, _pf(D), _pf(F), _pf(Q), _pu(seeds), _pf(PK))
     cs = req.c_struct()
     if out_device_ptr is not None:
         _check(scene.ctx.lib.rt_run_band_batch(scene.ctx._h, scene._h, C.byref(cs), C.byref(sb),

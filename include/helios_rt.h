@@ -131,6 +131,14 @@ typedef struct {
     const uint32_t *source_type; /* [S] RT_SOURCE_*, or NULL                                        */
     const float *source_width;   /* [S][2], required when a source is not collimated               */
     const float *source_rotation;/* [S][3], required when a source is a rectangle or disk          */
+    /* Sky radiance distribution of the diffuse flux (R/shaders/common/sky_distribution.glsl:40-116,
+     * diffuse_raygen.comp:613-625), per band, all NULL = isotropic: power law psi^-K * norm about the
+     * peak direction when K > 0, else the Prague pattern when sky_params[b][3] (its normalisation) > 0. */
+    const float *diffuse_extinction; /* [B] power-law exponent K                               */
+    const float *diffuse_peak_dir;   /* [B][3] unit direction of the peak (the sun)            */
+    const float *diffuse_dist_norm;  /* [B] power-law normalisation                            */
+    const float *sky_params;         /* [B][4] Prague (circumsolar strength, width in degrees, */
+                                     /* horizon brightness, normalisation)                     */
 } rt_band_request;
 
 /* R_out: [P][B] radiation_flux_<band> (host memory). */
@@ -146,6 +154,7 @@ typedef struct {
     const float *source_flux;    /* [K][S][B]                              */
     const float *diffuse_flux;   /* [K][B]                                 */
     const uint32_t *direct_seed; /* [K]                                    */
+    const float *diffuse_peak_dir; /* [K][B][3] sky peak of each sun, NULL = base's */
 } rt_sun_batch;
 /* R_out: [K][P][B]; if out_on_device != 0, R_out is a device pointer. */
 int rt_run_band_batch(rt_ctx *ctx, const rt_scene *scene, const rt_band_request *base, const rt_sun_batch *suns,
@@ -214,6 +223,12 @@ typedef struct {
     const float *diffuse_flux;     /* [B_launch] sky flux of a diffuse launch, NULL = 0       */
     const float *radiation_out_top;    /* [P][B_launch] uploaded before a diffuse launch, or NULL */
     const float *radiation_out_bottom; /* [P][B_launch], NULL iff radiation_out_top is NULL      */
+    /* anisotropic sky of a diffuse launch (RayTracingLaunchParams diffuse_extinction / _peak_dir /
+     * _dist_norm), [B_launch] each, NULL = 0; the Prague parameters come from rt_update_sky_model, whose
+     * sun direction is the peak when diffuse_peak_dir is NULL.  Bands without sky flux ignore them. */
+    const float *diffuse_extinction;
+    const float *diffuse_peak_dir;     /* [B_launch][3] */
+    const float *diffuse_dist_norm;
 } rt_launch_params;
 /* launchDirectRays (RayTracingBackend.h:139; R/shaders/direct_raygen.comp): every source, then
  * radiation_in / scatter_top / scatter_bottom += the deposits. */

@@ -21,7 +21,9 @@ class orc_band_params(C.Structure):
                 ("diffuse_rays", C.c_uint32), ("band_depth", u32p), ("emission", u32p), ("source_dir", f32p),
                 ("source_flux", f32p), ("diffuse_flux", f32p), ("rho", f32p), ("tau", f32p), ("eps", f32p),
                 ("temperature", f32p), ("direct_seed", C.c_uint32), ("diffuse_seed", u32p),
-                ("schedule", C.c_uint32), ("source_type", u32p), ("source_width", f32p), ("source_rotation", f32p)]
+                ("schedule", C.c_uint32), ("source_type", u32p), ("source_width", f32p), ("source_rotation", f32p),
+                ("diffuse_extinction", f32p), ("diffuse_peak_dir", f32p), ("diffuse_dist_norm", f32p),
+                ("sky_params", f32p)]
 
 
 class orc_camera(C.Structure):
@@ -289,7 +291,11 @@ class OracleScene:
                     temp=None if req.temp is None else _f(req.temp), dseeds=_u(req.dseeds),
                     stype=None if getattr(req, "stype", None) is None else _u(req.stype),
                     swidth=None if getattr(req, "swidth", None) is None else _f(req.swidth),
-                    srot=None if getattr(req, "srot", None) is None else _f(req.srot))
+                    srot=None if getattr(req, "srot", None) is None else _f(req.srot),
+                    kext=None if getattr(req, "sky_ext", None) is None else _f(req.sky_ext),
+                    kpeak=None if getattr(req, "sky_peak", None) is None else _f(req.sky_peak),
+                    knorm=None if getattr(req, "sky_norm", None) is None else _f(req.sky_norm),
+                    kprague=None if getattr(req, "sky_prague", None) is None else _f(req.sky_prague))
         p = orc_band_params(
             B, keep["sdir"].reshape(-1, 3).shape[0], req.direct_rays, req.diffuse_rays,
             keep["depth"].ctypes.data_as(u32p), keep["emission"].ctypes.data_as(u32p),
@@ -300,7 +306,8 @@ class OracleScene:
             req.direct_seed, keep["dseeds"].ctypes.data_as(u32p), req.schedule,
             None if keep["stype"] is None else keep["stype"].ctypes.data_as(u32p),
             None if keep["swidth"] is None else keep["swidth"].ctypes.data_as(f32p),
-            None if keep["srot"] is None else keep["srot"].ctypes.data_as(f32p))
+            None if keep["srot"] is None else keep["srot"].ctypes.data_as(f32p),
+            *[None if keep[k] is None else keep[k].ctypes.data_as(f32p) for k in ("kext", "kpeak", "knorm", "kprague")])
         R = np.zeros((P, B), np.float64)
         dg = np.zeros((3, B), np.float64)
         rc = lib().orc_run_band(self.h, C.byref(p), R.ctypes.data_as(f64p), dg.ctypes.data_as(f64p))

@@ -1307,6 +1307,30 @@ void orc_pixel_labels(const orc_scene *s, const orc_camera *c, uint32_t *labels,
 /* ------------------------------------------------------------------------------------------------ */
 
 /* One diffuse pass, faces 1 then 0 (RadiationModel.cpp:4204-4209, diffuse_raygen.comp:330-656). */
+/* evaluate_sky_distribution (R/shaders/common/sky_distribution.glsl:40-116) for band b: power law
+ * psi^-K * norm about the peak direction (psi clamped to 1 degree) when K > 0, else the Prague pattern
+ * (times its normalisation and pi) when that normalisation is positive, else 1 */
+static float sky_weight_orc(const orc_band_params *p, uint32_t b, v3 d) {
+    const float pi = 3.14159265359f;
+    float K = p->diffuse_extinction ? p->diffuse_extinction[b] : 0.0f;
+    const float *pk = p->diffuse_peak_dir ? p->diffuse_peak_dir + 3 * b : NULL;
+    if (K > 0.0f) {
+        float c = fminf(1.0f, fmaxf(-1.0f, pk[0] * d.x + pk[1] * d.y + pk[2] * d.z));
+        float psi = fmaxf(acosf(c), pi / 180.0f);
+        return powf(psi, -K) * p->diffuse_dist_norm[b];
+    }
+    const float *prm = p->sky_params ? p->sky_params + 4 * b : NULL;
+    if (prm && prm[3] > 0.0f) {
+        float c = fminf(1.0f, fmaxf(-1.0f, d.x * pk[0] + d.y * pk[1] + d.z * pk[2]));
+        float gamma = acosf(c) * (180.0f / pi);
+        float cos_theta = fmaxf(0.0f, d.z);
+        float circ = 1.0f + prm[0] * expf(-gamma / prm[1]);
+        float horizon = 1.0f + (prm[2] - 1.0f) * (1.0f - cos_theta);
+        return circ * horizon * prm[3] * pi;
+    }
+    return 1.0f;
+}
+
 static void diffuse_pass(const orc_scene *s, const orc_band_params *p, uint32_t n, uint32_t seed,
                          const double *out_top, const double *out_bot, const float *dflux, const uint8_t *active,
                          double *in, double *st, double *sb) {
@@ -1333,7 +1357,9 @@ static void diffuse_pass(const orc_scene *s, const orc_band_params *p, uint32_t 
                         double inc = top ? out_top[(size_t)h * B + b] : out_bot[(size_t)h * B + b];
                         E = inc * (double)strength;
                     } else if (dflux[b] > 0.f) {
-                        E = 1.0 * (double)dflux[b] * (double)strength; /* isotropic sky: fd = 1 */
+                        /* fd = 1 for an isotropic sky, else the distribution in the ray's direction */
+                        float fd = sky_weight_orc(p, b, d);
+                        E = (double)(fd * dflux[b]) * (double)strength;
                     }
                     if (E == 0.0) continue;
                     double rho = p->rho[(size_t)i * B + b], tau = p->tau[(size_t)i * B + b];

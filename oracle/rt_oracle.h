@@ -148,6 +148,10 @@ typedef struct {
     const uint32_t *source_type;   /* [S] or NULL (all collimated)                       */
     const float *source_width;     /* [S][2]                                             */
     const float *source_rotation;  /* [S][3]                                             */
+    const float *diffuse_extinction; /* [B] power-law K, NULL = isotropic                   */
+    const float *diffuse_peak_dir;   /* [B][3]                                             */
+    const float *diffuse_dist_norm;  /* [B]                                                */
+    const float *sky_params;         /* [B][4] Prague                                      */
 } orc_band_params;
 
 /* R[P][B] = radiation_flux_<band>; diag (may be NULL) receives [B] FP64 energy totals:

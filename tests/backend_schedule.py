@@ -68,7 +68,9 @@ def run_band_via_backend(be, req, twosided, band_map=None, nbands_global=None):
                 flux_bottom[two, b] += out[two]
         for face in (1, 0):
             be.launch_diffuse_rays(req.diffuse_rays, int(req.dseeds[0]), Bg, face, diffuse_flux=dflux,
-                                   band_map=band_map, radiation_out=(flux_top, flux_bottom))
+                                   band_map=band_map, radiation_out=(flux_top, flux_bottom),
+                                   sky_extinction=getattr(req, "sky_ext", None),
+                                   sky_peak_dir=getattr(req, "sky_peak", None), sky_norm=getattr(req, "sky_norm", None))
 
     if scattering and (emission or diffuse) and not rundirect:
         be.copy_scatter_to_radiation()
