@@ -205,6 +205,75 @@ int rt_cast_rays_soa(rt_ctx *ctx, const rt_scene *scene, const float *o, const f
     return rc;
 }
 
+int rt_cast_rays_soa_packets(rt_ctx *ctx, const rt_scene *scene, const float *o, const float *d, size_t n,
+                             size_t packet_size, float max_dist, float *t, float *nrm, uint32_t *uuid) {
+    // Results equal the per-ray path (CollisionDetection.h:404-408); each wave already takes 64
+    // consecutive rays, so the sub-rays of a pulse traverse together whatever packet_size is.
+    (void)packet_size;
+    return rt_cast_rays_soa(ctx, scene, o, d, n, max_dist, t, nrm, uuid);
+}
+
+namespace {
+// LiDARcloud::castRaysUnified's per-ray conversion (plugins/lidar/src/LiDAR.cpp:585-597)
+__global__ void k_lidar_convert(size_t n, const float *__restrict__ dirs, const float *__restrict__ nrm,
+                                const uint32_t *__restrict__ uuid, float miss_t, float *__restrict__ hit_t,
+                                float *__restrict__ fnorm, int32_t *__restrict__ id) {
+    const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const uint32_t u = uuid[r];
+    if (u != RT_MISS_UUID) {
+        id[r] = (int32_t)u;
+        fnorm[r] = dirs[3 * r] * nrm[3 * r] + dirs[3 * r + 1] * nrm[3 * r + 1] + dirs[3 * r + 2] * nrm[3 * r + 2];
+    } else {
+        hit_t[r] = miss_t;
+        id[r] = -1;
+        fnorm[r] = 1e6f;
+    }
+}
+}  // namespace
+
+int rt_lidar_cast_rays(rt_ctx *ctx, const rt_scene *scene, const float *o, const float *d, size_t n,
+                       size_t packet_size, float miss_t, float *hit_t, float *hit_fnorm, int32_t *hit_id) {
+    NN(ctx, "ctx");
+    NN(scene, "scene");
+    (void)packet_size;
+    if (n == 0) return 0;
+    NN(o, "origins"); NN(d, "dirs"); NN(hit_t, "hit_t"); NN(hit_fnorm, "hit_fnorm"); NN(hit_id, "hit_id");
+    CKR(hipSetDevice(ctx->device));
+    float *dbuf = nullptr;
+    const size_t per = 3 + 3 + 1 + 3 + 1 + 1 + 1;  // o, d, t, normal, uuid, fnorm, id (4 bytes each)
+    CKR(hipMalloc(&dbuf, n * per * 4));
+    float *dO = dbuf, *dD = dbuf + n * 3, *dT = dbuf + n * 6, *dN = dbuf + n * 7, *dF = dbuf + n * 11;
+    uint32_t *dU = reinterpret_cast<uint32_t *>(dbuf + n * 10);
+    int32_t *dI = reinterpret_cast<int32_t *>(dbuf + n * 12);
+    int rc = 0;
+    do {
+        if (hipMemcpy(dO, o, n * 12, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(dD, d, n * 12, hipMemcpyHostToDevice) != hipSuccess) {
+            set_error("LiDAR cast upload failed");
+            rc = -1;
+            break;
+        }
+        // castRaysSoA with the miss distance as the maximum distance (LiDAR.cpp:577-579)
+        if (rt_cast_rays_soa_device(ctx, scene, dO, dD, n, miss_t, dT, dN, dU)) { rc = -1; break; }
+        k_lidar_convert<<<(unsigned)((n + 255) / 256), 256, 0, ctx->sx.stream>>>(n, dD, dN, dU, miss_t, dT, dF, dI);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->sx.stream) != hipSuccess) {
+            set_error("LiDAR conversion kernel failed");
+            rc = -1;
+            break;
+        }
+        if (hipMemcpy(hit_t, dT, n * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(hit_fnorm, dF, n * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(hit_id, dI, n * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+            set_error("LiDAR cast download failed");
+            rc = -1;
+            break;
+        }
+    } while (0);
+    (void)hipFree(dbuf);
+    return rc;
+}
+
 static int check_request(const rt_scene *scene, const rt_band_request *q) {
     NN(scene, "scene");
     NN(q, "request");

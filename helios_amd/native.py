@@ -113,6 +113,10 @@ SIGNATURES = {
                                   C.POINTER(C.c_void_p)]),
     "rt_scene_destroy": (C.c_int, [C.c_void_p, C.c_void_p]),
     "rt_scene_get_info": (C.c_int, [C.c_void_p, C.POINTER(rt_scene_info)]),
+    "rt_cast_rays_soa_packets": (C.c_int, [C.c_void_p, C.c_void_p, f32p, f32p, C.c_size_t, C.c_size_t, C.c_float, f32p,
+                                           f32p, u32p]),
+    "rt_lidar_cast_rays": (C.c_int, [C.c_void_p, C.c_void_p, f32p, f32p, C.c_size_t, C.c_size_t, C.c_float, f32p, f32p,
+                                     C.POINTER(C.c_int32)]),
     "rt_scene_set_masks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint8), u32p,
                                      C.POINTER(C.c_int32), f32p, C.POINTER(C.c_int32)]),
     "rt_update_texture_masks": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint8), u32p,
@@ -353,6 +357,30 @@ class Scene:
         _check(self.ctx.lib.rt_cast_rays_soa(self.ctx._h, self._h, _pf(O), _pf(D), n, max_dist, _pf(t), _pf(nrm),
                                              _pu(uu)))
         return t, nrm, uu
+
+    def cast_rays_packets(self, origins, dirs, packet_size: int, max_dist: float = -1.0):
+        """castRaysSoA_packets: pulses of packet_size contiguous rays (results equal cast_rays)."""
+        O = _f32(origins).reshape(-1, 3)
+        D = _f32(dirs).reshape(-1, 3)
+        n = O.shape[0]
+        t = np.full(n, np.nan, np.float32)
+        nrm = np.full((n, 3), np.nan, np.float32)
+        uu = np.zeros(n, np.uint32)
+        _check(self.ctx.lib.rt_cast_rays_soa_packets(self.ctx._h, self._h, _pf(O), _pf(D), n, int(packet_size), max_dist,
+                                                     _pf(t), _pf(nrm), _pu(uu)))
+        return t, nrm, uu
+
+    def lidar_cast_rays(self, origins, dirs, packet_size: int, miss_t: float):
+        """LiDARcloud::castRaysUnified: (hit_t, hit_fnorm, hit_id) per ray."""
+        O = _f32(origins).reshape(-1, 3)
+        D = _f32(dirs).reshape(-1, 3)
+        n = O.shape[0]
+        t = np.zeros(n, np.float32)
+        f = np.zeros(n, np.float32)
+        ids = np.zeros(n, np.int32)
+        _check(self.ctx.lib.rt_lidar_cast_rays(self.ctx._h, self._h, _pf(O), _pf(D), n, int(packet_size), float(miss_t),
+                                               _pf(t), _pf(f), ids.ctypes.data_as(C.POINTER(C.c_int32))))
+        return t, f, ids
 
     def cast_rays_device(self, d_origins: int, d_dirs: int, n: int, max_dist: float, d_t: int, d_n: int, d_u: int):
         _check(self.ctx.lib.rt_cast_rays_soa_device(self.ctx._h, self._h, C.c_void_p(d_origins), C.c_void_p(d_dirs),

@@ -88,6 +88,15 @@ int rt_scene_set_masks(rt_ctx *ctx, rt_scene *scene, uint32_t nmasks, const uint
  * ------------------------------------------------------------------------------------------- */
 int rt_cast_rays_soa(rt_ctx *ctx, const rt_scene *scene, const float *origins, const float *dirs, size_t n,
                      float max_dist, float *out_t, float *out_normal, uint32_t *out_uuid);
+/* CollisionDetection::castRaysSoA_packets (CD/include/CollisionDetection.h:395-422): rays grouped in
+ * pulses of packet_size contiguous sub-rays; results identical to rt_cast_rays_soa. */
+int rt_cast_rays_soa_packets(rt_ctx *ctx, const rt_scene *scene, const float *origins, const float *dirs, size_t n,
+                             size_t packet_size, float max_dist, float *out_t, float *out_normal, uint32_t *out_uuid);
+/* LiDARcloud::castRaysUnified (plugins/lidar/src/LiDAR.cpp:558-598), the LiDAR synthetic scan's ray cast:
+ * castRaysSoA with max distance miss_t, then per ray hit_id = UUID and hit_fnorm = dot(dir, normal) on a
+ * hit; hit_t = miss_t, hit_id = -1, hit_fnorm = 1e6 on a miss.  The conversion runs on the device. */
+int rt_lidar_cast_rays(rt_ctx *ctx, const rt_scene *scene, const float *origins, const float *dirs, size_t n,
+                       size_t packet_size, float miss_t, float *hit_t, float *hit_fnorm, int32_t *hit_id);
 /* Same, with every array already in device memory; runs on the context stream and waits for it. */
 int rt_cast_rays_soa_device(rt_ctx *ctx, const rt_scene *scene, const float *d_origins, const float *d_dirs,
                             size_t n, float max_dist, float *d_out_t, float *d_out_normal, uint32_t *d_out_uuid);
