@@ -1794,6 +1794,112 @@ __global__ __launch_bounds__(256) void k_camera(CameraLaunch c, int *err, unsign
     add_stats(stats, cnt, c.spp);
 }
 
+// Camera pixels with lane refill: a lane keeps its pixel and runs its samples in order, but a lane whose
+// ray has ended shades it and starts its next sample while the others still traverse (traversal in node
+// and leaf phases as the diffuse kernel).  Same rays, same per-pixel summation order as k_camera.
+// Selectable at build time; measured slower on the orchard image (1,094 vs 1,460 Mrays/s): lanes drift to
+// different samples and the wave loses the coherence of neighbouring pixels' rays, so it is off.
+#ifndef RT_CAMERA_REFILL
+#define RT_CAMERA_REFILL 0
+#endif
+template <int STK, bool COUNT, bool MK>
+__global__ __launch_bounds__(256) void k_camera_r(CameraLaunch c, int *err, unsigned long long *stats) {
+    constexpr int LSTK = STK < RT_CAMERA_LDS_STACK ? STK : RT_CAMERA_LDS_STACK;
+    __shared__ uint32_t stk_all[LSTK * 256];
+    uint32_t ovf[STK > LSTK ? RT_STACK - LSTK : 1];
+    lds_u32 *stk = (lds_u32 *)stk_all + threadIdx.x;
+    const uint32_t stride = 256;
+    Cnt<COUNT, MK> cnt{};
+    uint32_t nray = 0;
+    const DevScene &sc = *c.sc;
+    const float4 *nodes = sc.node, *recs = sc.rec;
+    const uint32_t nleaf = sc.nleaf, nbbox = sc.nbbox;
+    uint32_t px, py;
+    const bool valid = camera_pixel(c, px, py);
+    const uint32_t ii = c.off_x + px, jj = c.off_y + py;
+    const size_t pix = (size_t)jj * c.full_x + ii, npix = (size_t)c.full_x * c.full_y;
+    const float strength = rdiv(1.0f, (float)c.spp);
+    const uint32_t dimx = c.res_x * c.res_y, indx = c.res_x * py + px;
+    LaneRay L;
+    uint32_t smp = 0, wraps = 0;
+    bool busy = false, trav = false;
+    auto start = [&]() {
+        uint32_t st = tea16(indx + dimx * smp, c.seed);
+        const float Rx = rnd24(st), Ry = rnd24(st);
+        float lx = 0.f, ly = 0.f;
+        if (c.lens > 0.0f) unit_disk(rnd24(st), rnd24(st), lx, ly);
+        camera_ray(c.pos[0], c.pos[1], c.pos[2], c.st, c.ct, c.sp, c.cp, c.focal, c.lens, c.aspect, c.viewplane, ii,
+                   jj, c.full_x, c.full_y, Rx, Ry, lx, ly, L.o, L.d);
+        wraps = 0;
+        busy = true;
+        nray++;
+        trav = lane_begin(L, recs, nleaf, 1e-5f, RT_MISS, cnt);
+    };
+    if (valid && c.spp > 0) start();
+    while (true) {
+        if (busy && !trav) {
+            // periodic wrap, up to 10 traces (camera_raygen.comp:266-285)
+            bool done = true;
+            if (nbbox) {
+                const float bt = nearest_face(sc, L.o, L.d, L.best_t);
+                if (bt > 0.f) {
+                    L.o = wrap_at(sc, L.o, L.d, bt);
+                    if (wraps < 9) {
+                        wraps++;
+                        trav = lane_begin(L, recs, nleaf, 1e-5f, RT_MISS, cnt);
+                        done = false;
+                    } else {
+                        L.best = RT_MISS;
+                    }
+                }
+            }
+            if (done) {
+                const uint32_t h = L.best;
+                const v3f o = L.o, d = L.d;
+                if (h != RT_MISS) {
+                    const float4 n4 = sc.nrm[h];
+                    const bool face = n4.x * d.x + n4.y * d.y + n4.z * d.z < 0.0f;
+                    const bool top = face || sc.type[h] == 4u;
+                    for (uint32_t b = 0; b < c.B; b++) {
+                        const float srad = cam_source_radiance(c, o, d, b, L.best_t, false);
+                        float surf = (top ? c.out_top : c.out_bot)[(size_t)h * c.B + b] * strength;
+                        if (srad > 0.0f) surf += srad * strength;
+                        c.pixels[(size_t)b * npix + pix] += rdiv(surf, RT_PI_F);
+                    }
+                } else {
+                    for (uint32_t b = 0; b < c.B; b++) {
+                        float rad = cam_source_radiance(c, o, d, b, 1e30f, true);
+                        if (rad <= 0.0f && c.sky_rad[b] > 0.0f) rad = c.sky_rad[b] * cam_sky_weight(d, c.sun, c.sky_params + 4 * b);
+                        if (c.emis[b] != 0u && c.cam_dflux[b] > 0.0f) rad += rdiv(c.cam_dflux[b], RT_PI_F);
+                        if (rad > 0.0f) c.pixels[(size_t)b * npix + pix] += rad * strength;
+                    }
+                }
+                smp++;
+                if (smp < c.spp) start();
+                else busy = false;
+            }
+        }
+        if (__ballot(busy) == 0) break;
+        // traverse until fewer than RT_REFILL_BELOW lanes still do while some lane has samples left to start
+        const uint32_t floor_lanes = __ballot(busy && smp + 1 < c.spp) ? RT_REFILL_BELOW : 1u;
+        while (true) {
+            while (true) {
+                const bool inner = trav && !(L.ni & RT_LEAF_BIT);
+                const uint64_t mi = __ballot(inner);
+                if (mi == 0 || (uint32_t)__popcll(__ballot(trav && !inner)) >= RT_LEAF_BATCH) break;
+                if (inner && lane_node<STK, LSTK>(nodes, nleaf, L, stk, ovf, stride, err, cnt)) trav = false;
+            }
+            while (true) {
+                const bool leaf = trav && (L.ni & RT_LEAF_BIT);
+                if (__ballot(leaf) == 0) break;
+                if (leaf && lane_leaf<STK, LSTK>(recs, L, 1e-5f, RT_MISS, stk, ovf, stride, cnt)) trav = false;
+            }
+            if ((uint32_t)__popcll(__ballot(trav)) < floor_lanes) break;
+        }
+    }
+    add_stats(stats, cnt, nray);
+}
+
 // Pixel labels (pixel_label_raygen.comp:80-155): the pixel-centre pinhole ray's hit UUID + 1 (0 = sky) and
 // its depth along the view direction (-1 on a miss).
 template <int STK, bool MK>
@@ -2072,15 +2178,21 @@ int launch_camera(const CameraLaunch &c, uint32_t depth, bool labels, int *err, 
     else if (labels)
         PICK_STK(depth, (k_pixel_label<32, false><<<g, 256, 0, st>>>(c, err)),
                  (k_pixel_label<64, false><<<g, 256, 0, st>>>(c, err)));
+#if RT_CAMERA_REFILL
+#define K_CAM k_camera_r
+#else
+#define K_CAM k_camera
+#endif
     else if (stats)
-        PICK_STK(depth, (k_camera<32, true, true><<<g, 256, 0, st>>>(c, err, stats)),
-                 (k_camera<64, true, true><<<g, 256, 0, st>>>(c, err, stats)));
+        PICK_STK(depth, (K_CAM<32, true, true><<<g, 256, 0, st>>>(c, err, stats)),
+                 (K_CAM<64, true, true><<<g, 256, 0, st>>>(c, err, stats)));
     else if (masked)
-        PICK_STK(depth, (k_camera<32, false, true><<<g, 256, 0, st>>>(c, err, nullptr)),
-                 (k_camera<64, false, true><<<g, 256, 0, st>>>(c, err, nullptr)));
+        PICK_STK(depth, (K_CAM<32, false, true><<<g, 256, 0, st>>>(c, err, nullptr)),
+                 (K_CAM<64, false, true><<<g, 256, 0, st>>>(c, err, nullptr)));
     else
-        PICK_STK(depth, (k_camera<32, false, false><<<g, 256, 0, st>>>(c, err, nullptr)),
-                 (k_camera<64, false, false><<<g, 256, 0, st>>>(c, err, nullptr)));
+        PICK_STK(depth, (K_CAM<32, false, false><<<g, 256, 0, st>>>(c, err, nullptr)),
+                 (K_CAM<64, false, false><<<g, 256, 0, st>>>(c, err, nullptr)));
+#undef K_CAM
     LCHECK();
     return 0;
 }
