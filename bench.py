@@ -369,7 +369,24 @@ def bench_castrays(args):
                          "bytes_per_ray": round(brays, 1),
                          "nodes_per_ray": round(counts["nodes"] / counts["rays"], 2),
                          "prims_per_ray": round(counts["prims"] / counts["rays"], 2),
-                         "avg_launch_ms": round(kms / args.steps, 3)}}
+                         "avg_launch_ms": round(kms / args.steps, 3)},
+            **({} if args.no_cpu_baseline else {"cpu_baseline": cpu_baseline_cast(s, O, D, args.cpu_seconds)})}
+
+
+def cpu_baseline_cast(s, O, D, seconds):
+    """The oracle's castRaysSoA restatement (OpenMP, CD predicates) on leading chunks of the same ray
+    stream until about `seconds`."""
+    from oracle.oracle import OracleScene
+    o = OracleScene.from_arrays(s)
+    n, chunk, t0 = 0, 1 << 20, time.perf_counter()
+    while time.perf_counter() - t0 < seconds and n < O.shape[0]:
+        m = min(chunk, O.shape[0] - n)
+        o.cast_rays(O[n:n + m], D[n:n + m])
+        n += m
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(n / dt / 1e6, 3), "unit": "Mrays/s", "cores": cores, "kind": "port",
+            "sample": f"first {n} rays of the same stream, oracle castRaysSoA restatement (OpenMP), {dt:.1f} s"}
 
 
 def orchard_camera(s, res, spp):
