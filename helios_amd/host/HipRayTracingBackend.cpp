@@ -299,11 +299,14 @@ namespace helios {
         results.radiation_out_bottom.resize(n);
         results.scatter_buff_top.resize(n);
         results.scatter_buff_bottom.resize(n);
-        results.scatter_buff_top_cam.clear();
-        results.scatter_buff_bottom_cam.clear();
         results.radiation_specular.clear();
         check(rt_get_radiation_results(ctx_, results.radiation_in.data(), results.radiation_out_top.data(), results.radiation_out_bottom.data(), results.scatter_buff_top.data(), results.scatter_buff_bottom.data()),
               "getRadiationResults");
+        // camera scatter = the scatter buffers (camera spectral response taken as uniform), which
+        // RadiationModel accumulates and uploads as radiation_out before camera launches
+        // (VulkanComputeBackend.cpp:1871-1876, RadiationModel.cpp:3992-4012, 4359-4366)
+        results.scatter_buff_top_cam = results.scatter_buff_top;
+        results.scatter_buff_bottom_cam = results.scatter_buff_bottom;
         results.num_primitives = primitive_count_;
         results.num_bands = launch_band_count_;
     }
