@@ -588,8 +588,8 @@ def main():
                     help="run a plain copy of this many MiB first (PMC byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is described in "
-              f"DESIGN.md and is not part of this build", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the solver shards by primitives or "
+              f"suns across GPUs only on paper, and no multi-GPU path exists to measure", file=sys.stderr)
         sys.exit(2)
     if args.workload == "diurnal":
         res = bench_diurnal(args)

@@ -2,7 +2,7 @@
 
 Bars: hit words and castRays IDs bit-exact (ties resolve to the lower primitive index on both
 sides); FP32 compensated per-primitive sums within 1e-5 relative of the FP64 oracle; radiation_flux
-within 1e-4 relative (north-star parity bound)."""
+within 1e-4 relative (the parity bound of the flux)."""
 import numpy as np
 import pytest
 
