@@ -129,6 +129,31 @@ void RadiationModel::setDiffuseRadiationFlux(const std::string &label, float flu
     bands_[label].diffuseFlux = flux;
 }
 
+void RadiationModel::setDiffuseRadiationExtinctionCoeff(const std::string &label, float K, const vec3 &peak_dir) {
+    band(label, "setDiffuseRadiationExtinctionCoeff");
+    const double m = std::sqrt((double)peak_dir.x * peak_dir.x + (double)peak_dir.y * peak_dir.y + (double)peak_dir.z * peak_dir.z);
+    if (!(m > 0.0)) throw std::runtime_error("ERROR (RadiationModel::setDiffuseRadiationExtinctionCoeff): peak direction is zero.");
+    const float d[3] = {(float)(peak_dir.x / m), (float)(peak_dir.y / m), (float)(peak_dir.z / m)};
+    // midpoint rule over 100 x 100 (theta, phi) cells of the upper hemisphere of
+    // fd(psi) cos(theta) sin(theta) dtheta dphi / pi, with psi clamped below at one degree
+    const int N = 100;
+    const float pi = 3.14159265358979f;
+    float norm = 0.f;
+    for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++) {
+            const float th = 0.5f * pi / (float)N * (0.5f + (float)i);
+            const float ph = 2.f * pi / (float)N * (0.5f + (float)j);
+            const float nx = std::sin(th) * std::cos(ph), ny = std::sin(th) * std::sin(ph), nz = std::cos(th);
+            const float c = std::fmin(1.f, std::fmax(-1.f, nx * d[0] + ny * d[1] + nz * d[2]));
+            const float psi = std::fmax(std::acos(c), pi / 180.f);
+            norm += std::pow(psi, -K) * std::cos(th) * std::sin(th) * pi / (float)(N * N);
+        }
+    RadiationBand &b = bands_[label];
+    b.diffuseExtinction = K;
+    for (int a = 0; a < 3; a++) b.diffusePeakDir[a] = d[a];
+    b.diffuseDistNorm = 1.f / norm;
+}
+
 float RadiationModel::getDiffuseFlux(const std::string &label) const {
     // RadiationModel.cpp:1312-1343 without spectra: a flux that was never set is 0
     const RadiationBand &b = band(label, "getDiffuseFlux");
@@ -286,6 +311,16 @@ void RadiationModel::runBand(const std::vector<std::string> &labels) {
         for (size_t b = 0; b < B; b++) sflux[s * B + b] = getSourceFlux((uint32_t)s, launched[b]);
     }
     for (size_t b = 0; b < B; b++) dflux[b] = getDiffuseFlux(launched[b]);
+    // power-law skies of the launched bands (all bands isotropic: the request carries none)
+    std::vector<float> kext(B, 0.f), kpeak(3 * B, 0.f), knorm(B, 0.f);
+    bool any_sky = false;
+    for (size_t b = 0; b < B; b++) {
+        const RadiationBand &rb = bands_.at(launched[b]);
+        kext[b] = rb.diffuseExtinction;
+        knorm[b] = rb.diffuseDistNorm;
+        for (int a = 0; a < 3; a++) kpeak[3 * b + a] = rb.diffusePeakDir[a];
+        any_sky = any_sky || rb.diffuseExtinction > 0.f;
+    }
 
     std::vector<uint32_t> dseeds(maxdepth + 1);
     uint32_t dseed;
@@ -319,6 +354,11 @@ void RadiationModel::runBand(const std::vector<std::string> &labels) {
     q.direct_seed = dseed;
     q.diffuse_seed = dseeds.data();
     q.schedule = schedule_;
+    if (any_sky) {
+        q.diffuse_extinction = kext.data();
+        q.diffuse_peak_dir = kpeak.data();
+        q.diffuse_dist_norm = knorm.data();
+    }
     std::vector<float> R(P * B);
     check(rt_run_band(ctx_, scene_, &q, R.data()), "runBand");
     for (size_t b = 0; b < B; b++) {

@@ -32,6 +32,9 @@ struct RadiationBand {
     float diffuseFlux = -1.f;  // < 0: not set
     uint32_t scatteringDepth = 0;
     bool emissionFlag = true;
+    float diffuseExtinction = 0.f;           // power-law sky exponent K (0 = isotropic)
+    float diffusePeakDir[3] = {0.f, 0.f, 1.f};
+    float diffuseDistNorm = 1.f;
 };
 
 // Applies RadiationModel.cpp:6252-6311 to one primitive's properties in one band (clamp rho/tau to
@@ -69,6 +72,9 @@ public:
     void setDirectRayCount(const std::string &label, size_t N);
     void setDiffuseRayCount(const std::string &label, size_t N);
     void setDiffuseRadiationFlux(const std::string &label, float flux);
+    // Power-law diffuse sky psi^-K about peak_dir (RadiationModel.cpp:185-217); the normalisation makes
+    // the cosine-weighted hemisphere integral of the distribution one.
+    void setDiffuseRadiationExtinctionCoeff(const std::string &label, float K, const vec3 &peak_dir);
     float getDiffuseFlux(const std::string &label) const;
     void setScatteringDepth(const std::string &label, uint32_t depth);
 

@@ -124,6 +124,28 @@ TEST("RadiationModel Simple Direct (selfTest.cpp:518-545)", true) {
     CHECK(std::fabs(flux - 1000.f) / 1000.f <= 0.01f);
 }
 
+TEST("RadiationModel power-law sky: a lone black patch absorbs the sky flux (normalised distribution)", true) {
+    // the normalisation makes the cosine-weighted hemisphere integral of psi^-K one, so a horizontal black
+    // patch with nothing around absorbs the diffuse flux whatever K and the peak; a wrong normalisation or
+    // unit slip moves it far beyond the Monte Carlo error of 2500 rays
+    for (float K : {0.5f, 1.2f}) {
+        RadiationModel radiation;
+        radiation.disableMessages();
+        radiation.setGeometry(patch_transform({0, 0, 0}, 1, 1, 0, 0), {0}, {0}, {0});
+        radiation.setPrimitiveData("reflectivity_SW", {0.f});
+        radiation.addRadiationBand("SW");
+        radiation.disableEmission("SW");
+        radiation.setDiffuseRadiationFlux("SW", 100.f);
+        radiation.setDiffuseRadiationExtinctionCoeff("SW", K, vec3{0.3f, 0.2f, 0.9f});
+        radiation.setDiffuseRayCount("SW", 2500);
+        radiation.setScatteringDepth("SW", 0);
+        radiation.updateGeometry();
+        radiation.runBand("SW");
+        float flux = radiation.getPrimitiveData("radiation_flux_SW")[0];
+        CHECK(std::fabs(flux - 100.f) <= 4.f);
+    }
+}
+
 TEST("RadiationModel 90 Degree Common-Edge Squares (selfTest.cpp:585-668)", true) {
     const float error_threshold = 0.005f;
     const int Nensemble = 500;
