@@ -143,3 +143,48 @@ def test_mask_errors(ctx):
         g.set_masks([np.ones((2, 2))], np.full(s.P, 3))
     with pytest.raises(native.RTError, match="size"):
         g.set_masks([np.ones((0, 2))], np.zeros(s.P))
+
+
+def test_direct_sums_all_source_types_with_masks(ctx):
+    """Masked scene lit by collimated, sphere, sun-sphere, rectangle and disk sources: origin redraws
+    continue the RNG stream before the per-source draws; occlusion honours transparent texels."""
+    from tests.test_gpu_sources import POS, ROTS, TYPES, WIDTHS
+    s = scenes.random_canopy(12, 5000)
+    g, o, _ = _both(ctx, s, seed=6)
+    sg = g.direct_sums_src(6, 41, POS, TYPES, WIDTHS, ROTS)
+    so = o.direct_sums_src(6, 41, POS, TYPES, WIDTHS, ROTS)
+    for k in range(len(TYPES)):
+        scale = max(np.abs(so[:, k]).max(), 1e-30)
+        assert np.abs(so[:, k]).max() > 0.0, f"source {k} lights nothing"
+        assert np.max(np.abs(sg[:, k] - so[:, k])) <= 1e-5 * scale, f"source type {TYPES[k]}"
+
+
+def test_camera_pixels_with_masks(ctx):
+    """Camera radiance of a masked canopy: hits behind transparent texels show the primitives behind."""
+    s = scenes.random_canopy(13, 4000)
+    masks, mid, uv, uvid = _masks(s, 8)
+    B = 2
+    be = native.Backend(ctx)
+    be.update_geometry(s.transforms, s.types, s.twosided)
+    be.update_texture_masks(masks, mid, uv, uvid)
+    be.build_acceleration_structure()
+    be.update_materials(np.full((s.P, B), 0.1), np.full((s.P, B), 0.05))
+    be.update_sources([[0.3, 0.2, 0.9]])
+    be.zero_radiation_buffers(B)
+    rng = np.random.default_rng(2)
+    top = rng.uniform(0, 300, (s.P, B)).astype(np.float32)
+    bot = rng.uniform(0, 80, (s.P, B)).astype(np.float32)
+    be.upload_radiation_out(top, bot)
+    be.upload_source_fluxes([[500.0, 700.0]])
+    sky = dict(params=np.tile(np.asarray([0, 1, 1, 0], np.float32), (B, 1)), radiance=np.array([10.0, 20.0], np.float32),
+               sun=(0.0, 0.0, 1.0), solar=np.zeros(B, np.float32), solar_cos=2.0, diffuse_flux=np.zeros(B, np.float32),
+               emission=np.zeros(B, np.uint32))
+    be.update_sky_model(sky["params"], sky["radiance"], sky["sun"], sky["solar"], sky["solar_cos"])
+    cam = Camera((0.0, 0.0, 4.0), (math.pi, 0.0), (48, 32), viewplane_length=2.0, fov_aspect=1.5,
+                 antialiasing_samples=2)
+    be.launch_camera_rays(cam, seed=9)
+    R = be.get_camera_results(cam.resolution)
+    o = OracleScene.from_arrays(s)
+    o.set_masks(masks, mid, uv, uvid)
+    ref = o.camera_pixels(cam, top, bot, [[500.0, 700.0]], [[0.3, 0.2, 0.9]], sky, seed=9).T
+    assert rel_err(R["pixel_data"], ref, floor=1e-3).max() <= 1e-5
