@@ -14,7 +14,8 @@ primitives) through the request server and through the plain loop and reports bo
 
 Other workloads: --workload config2 (200k-triangle periodic canopy, one 2-band solve per step),
 --workload castrays (CollisionDetection::castRaysSoA of 1e8 rays over the 5x tiled orchard),
---workload camera (a 1024x1024 x 64-sample camera image of the same 5x orchard) and
+--workload camera (a 1024x1024 x 64-sample camera image of the same 5x orchard),
+--workload homogeneous (the reference's published radiation benchmark, timed as it times runBand) and
 --workload serve (the serving leg alone).
 """
 from __future__ import annotations
@@ -320,6 +321,84 @@ def bench_config2(args):
                          "bytes_per_ray": round(brays, 1)}}
 
 
+# published runBand wall times of the homogeneous-canopy benchmark (seconds: direct, diffuse)
+PUBLISHED_HOMOGENEOUS = {"NVIDIA RTX 6000 Ada": (1.054, 1.910), "NVIDIA L40S": (0.950, 1.821),
+                         "NVIDIA A100 80GB PCIe": (1.727, 14.186)}
+
+
+def bench_homogeneous(args):
+    """The reference's published radiation benchmark (benchmarks/radiation_homogeneous_canopy): LAI 1 over a
+    20 x 20 m periodic domain of 5 cm disk-masked tile leaves (2 x 2 sub-patches) + a 100 x 100 ground tile,
+    1024 direct rays (band "direct", sun at 36 degrees zenith) and 1024 diffuse rays (band "diffuse"),
+    scattering depth 0, one band per runBand.  One step = runBand("direct") + runBand("diffuse"), each a
+    full rt_run_band call with host inputs and outputs, as the harness times runBand."""
+    t0 = time.perf_counter()
+    s, masks, mid, uv, uvid = scenes.homogeneous_canopy(1)
+    gen_s = time.perf_counter() - t0
+    ctx = native.Context(0)
+    # the geometry update (BVH build + mask upload) timed on its second run: the first also loads the
+    # kernels' code objects
+    for _ in range(2):
+        t0 = time.perf_counter()
+        g = ctx.scene(s.transforms, s.types, s.twosided, periodic=s.periodic)
+        g.set_masks(masks, mid, uv, uvid)
+        device_sync()
+        geom_s = time.perf_counter() - t0
+    P = s.P
+    zero = np.zeros((P, 1), np.float32)
+    th = 0.2 * math.pi
+    sun = native.sun_direction(math.degrees(th), 0.0)
+    direct = native.BandRequest(P, [native.BandSpec("direct", 0, False, 0.0)], rho=zero, tau=zero, direct_rays=1024,
+                                diffuse_rays=1000, source_dirs=[sun], source_flux=[[1.0 / math.cos(th)]],
+                                direct_seed=3, diffuse_seeds=[5])
+    diffuse = native.BandRequest(P, [native.BandSpec("diffuse", 0, False, 1.0)], rho=zero, tau=zero,
+                                 direct_rays=100, diffuse_rays=1024, source_dirs=[sun], source_flux=[[0.0]],
+                                 direct_seed=7, diffuse_seeds=[9])
+    ctx.enable_timing(True)
+    for _ in range(args.warmup):
+        native.run_band(g, direct)
+        native.run_band(g, diffuse)
+    ctx.set_counting(True)
+    native.run_band(g, diffuse)
+    counts = ctx.counters()
+    ctx.set_counting(False)
+    td, tf, kms = [], [], 0.0
+    device_sync()
+    t_all = time.perf_counter()
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        Rd = native.run_band(g, direct)
+        td.append(time.perf_counter() - t)
+        t = time.perf_counter()
+        Rf = native.run_band(g, diffuse)
+        tf.append(time.perf_counter() - t)
+        kms += ctx.timing()["diffuse_ms"]
+    device_sync()
+    ms = (time.perf_counter() - t_all) * 1e3 / args.steps
+    d_s, f_s = float(np.mean(td)), float(np.mean(tf))
+    nl = int((s.twosided == 1).sum())
+    direct_rays = P * 1024
+    diffuse_rays = (2 * nl + (P - nl)) * 1024
+    brays = bytes_per_ray(counts["diffuse"])
+    ach = brays * counts["diffuse"]["rays"] / (kms / args.steps / 1e3) / 1e9
+    ref = PUBLISHED_HOMOGENEOUS["NVIDIA RTX 6000 Ada"]
+    return {"metric": "runBand seconds (direct + diffuse), homogeneous canopy benchmark", "value": round(d_s + f_s, 4),
+            "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 2),
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": round((d_s + f_s) / sum(ref), 4),
+            "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"radiation_homogeneous_canopy: {P} prims ({nl} masked leaf sub-patches, 800x800 "
+                                   f"analytic disk mask), periodic 20x20 m, 1024 direct + 1024 diffuse rays, depth 0",
+                       "direct_s": round(d_s, 4), "diffuse_s": round(f_s, 4), "geometry_update_s": round(geom_s, 3),
+                       "scene_generation_s": round(gen_s, 2),
+                       "direct_grays_per_s": round(direct_rays / d_s / 1e9, 3),
+                       "diffuse_grays_per_s": round(diffuse_rays / f_s / 1e9, 3),
+                       "published_s": {k: {"direct": v[0], "diffuse": v[1]} for k, v in PUBLISHED_HOMOGENEOUS.items()},
+                       "absorbed_mean_direct": round(float(Rd.mean()), 5), "absorbed_mean_diffuse": round(float(Rf.mean()), 5)},
+            "roofline": {"bound": "hbm", "kernel": "k_diffuse", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                         "bytes_per_ray": round(brays, 1), "avg_launch_ms": round(kms / args.steps, 3)}}
+
+
 def bench_castrays(args):
     import ctypes as C
     if args.calib_copy_mib:
@@ -570,7 +649,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="diurnal", choices=["diurnal", "config2", "castrays", "camera", "serve"])
+    ap.add_argument("--workload", default="diurnal",
+                    choices=["diurnal", "config2", "castrays", "camera", "homogeneous", "serve"])
     ap.add_argument("--cam-res", type=int, default=1024)
     ap.add_argument("--cam-spp", type=int, default=64)
     ap.add_argument("--suns", type=int, default=64)
@@ -606,6 +686,8 @@ def main():
         res = bench_config2(args)
     elif args.workload == "camera":
         res = bench_camera(args)
+    elif args.workload == "homogeneous":
+        res = bench_homogeneous(args)
     else:
         res = bench_castrays(args)
     print(json.dumps(res))

@@ -115,3 +115,69 @@ def sincos_2pi(r: float):
     s, c = C.c_float(), C.c_float()
     lib.sg_sincos_2pi(r, C.byref(s), C.byref(c))
     return s.value, c.value
+
+
+def homogeneous_canopy(seed: int = 1, D: float = 20.0, lai: float = 1.0, h: float = 3.0, w_leaf: float = 0.05,
+                       mask_res: int = 800):
+    """The radiation homogeneous-canopy benchmark scene (benchmarks/radiation_homogeneous_canopy/main.cpp):
+    square leaves of side w_leaf made of 2 x 2 sub-patches, each textured with its quarter of a disk mask
+    (the harness uses an 800 x 800 disk.png; here an analytic disk of mask_res texels), enough leaves for
+    the leaf area index over a D x D periodic domain (leaf area = the mask's solid area), positions
+    uniform in x, y and in z over [w_leaf / 2, w_leaf / 2 + h], spherical leaf angles (elevation
+    acos(1 - u), azimuth 2 pi u'), rotated about y by -elevation then about z by the azimuth; a one-sided
+    ground tile of 100 x 100 patches.  Returns (scene, masks, mask_ids, uv, uv_ids)."""
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:mask_res, 0:mask_res]
+    r = (mask_res - 1) / 2.0
+    disk = ((xx - r) ** 2 + (yy - r) ** 2 <= (mask_res / 2.0) ** 2).astype(np.uint8)
+    solid = float(disk.mean())
+    n_leaves = int(round(lai * D * D / (w_leaf * w_leaf * solid)))
+    u = rng.uniform(size=(n_leaves, 5))
+    pos = np.stack([(-0.5 + u[:, 0]) * D, (-0.5 + u[:, 1]) * D, 0.5 * w_leaf + u[:, 2] * h], 1)
+    elev = np.arccos(1.0 - u[:, 3])
+    az = 2.0 * np.pi * u[:, 4]
+    # R = Rz(az) Ry(-elev)
+    ce, se, ca, sa = np.cos(-elev), np.sin(-elev), np.cos(az), np.sin(az)
+    R = np.zeros((n_leaves, 3, 3))
+    R[:, 0, 0], R[:, 0, 1], R[:, 0, 2] = ca * ce, -sa, ca * se
+    R[:, 1, 0], R[:, 1, 1], R[:, 1, 2] = sa * ce, ca, sa * se
+    R[:, 2, 0], R[:, 2, 1], R[:, 2, 2] = -se, 0.0, ce
+    q = w_leaf / 2.0
+    Ts, uvs = [], []
+    for jj in range(2):
+        for ii in range(2):
+            c_l = np.array([(-0.5 + (ii + 0.5) / 2) * w_leaf, (-0.5 + (jj + 0.5) / 2) * w_leaf, 0.0])
+            ex = R @ np.array([q, 0.0, 0.0])
+            ey = R @ np.array([0.0, q, 0.0])
+            ez = np.cross(ex, ey)
+            ez /= np.linalg.norm(ez, axis=1, keepdims=True)
+            c = pos + R @ c_l
+            T = np.zeros((n_leaves, 16))
+            T[:, 0], T[:, 4], T[:, 8] = ex[:, 0], ex[:, 1], ex[:, 2]
+            T[:, 1], T[:, 5], T[:, 9] = ey[:, 0], ey[:, 1], ey[:, 2]
+            T[:, 2], T[:, 6], T[:, 10] = ez[:, 0], ez[:, 1], ez[:, 2]
+            T[:, 3], T[:, 7], T[:, 11] = c[:, 0], c[:, 1], c[:, 2]
+            T[:, 15] = 1.0
+            Ts.append(T)
+            u0, u1, v0, v1 = ii / 2, (ii + 1) / 2, jj / 2, (jj + 1) / 2
+            uvs.append(np.tile(np.array([[u0, v0], [u1, v0], [u1, v1], [u0, v1]]), (n_leaves, 1, 1)))
+    leaf_T = np.stack(Ts, 1).reshape(-1, 16)   # leaf k's sub-patches are consecutive
+    leaf_uv = np.stack(uvs, 1).reshape(-1, 4, 2)
+    ng = 100
+    g = (np.arange(ng) + 0.5) / ng * D - D / 2
+    gx, gy = np.meshgrid(g, g, indexing="xy")
+    GT = np.zeros((ng * ng, 16))
+    GT[:, 0] = D / ng
+    GT[:, 5] = D / ng
+    GT[:, 10] = 1.0
+    GT[:, 3], GT[:, 7] = gx.reshape(-1), gy.reshape(-1)
+    GT[:, 15] = 1.0
+    T = np.concatenate([leaf_T, GT]).astype(np.float32)
+    P = T.shape[0]
+    nl = leaf_T.shape[0]
+    types = np.zeros(P, np.uint32)
+    twosided = np.r_[np.ones(nl, np.uint32), np.zeros(ng * ng, np.uint32)]
+    mask_ids = np.r_[np.zeros(nl, np.int32), np.full(ng * ng, -1, np.int32)]
+    uv = np.concatenate([leaf_uv, np.zeros((ng * ng, 4, 2))]).astype(np.float32)
+    uv_ids = np.r_[np.arange(nl, dtype=np.int32), np.full(ng * ng, -1, np.int32)]
+    return SceneArrays(T, types, twosided, (True, True)), [disk], mask_ids, uv, uv_ids
