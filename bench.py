@@ -354,6 +354,7 @@ def bench_homogeneous(args):
     diffuse = native.BandRequest(P, [native.BandSpec("diffuse", 0, False, 1.0)], rho=zero, tau=zero,
                                  direct_rays=100, diffuse_rays=1024, source_dirs=[sun], source_flux=[[0.0]],
                                  direct_seed=7, diffuse_seeds=[9])
+    # default launch shapes (the autotuner picks the same ones for this scene)
     ctx.enable_timing(True)
     for _ in range(args.warmup):
         native.run_band(g, direct)

@@ -42,9 +42,8 @@ double time_cfg(hipStream_t st, Timer &tm, const LaunchCfg &cfg, F launch) {
 }
 
 template <typename F>
-int search(hipStream_t st, Timer &tm, const std::vector<LaunchCfg> &cands, F launch, LaunchCfg &best, double &best_ms,
-           double &default_ms, uint32_t &count) {
-    const LaunchCfg def{};
+int search(hipStream_t st, Timer &tm, const std::vector<LaunchCfg> &cands, F launch, const LaunchCfg &def,
+           LaunchCfg &best, double &best_ms, double &default_ms, uint32_t &count) {
     default_ms = time_cfg(st, tm, def, launch);
     if (default_ms < 0) return -1;
     best = def;
@@ -198,7 +197,7 @@ extern "C" int rt_autotune(rt_ctx *ctx, const rt_scene *scene, const rt_band_req
                 j.sdir[0] = dv[0]; j.sdir[1] = dv[1]; j.sdir[2] = dv[2];
                 j.grid = gs.dev;
                 if (hipMemcpy(dj, &j, sizeof(j), hipMemcpyHostToDevice) != hipSuccess) { rc = -1; break; }
-                const LaunchCfg def{};
+                const LaunchCfg def = default_launch_cfg(0);
                 double ms = time_cfg(st, tm, def, [&](const LaunchCfg &c) {
                     return launch_direct(dj, 1, P, depth, counter, err, nullptr, c, ncu, st, S.h.mdesc != nullptr);
                 });
@@ -224,18 +223,18 @@ extern "C" int rt_autotune(rt_ctx *ctx, const rt_scene *scene, const rt_band_req
                     [&](const LaunchCfg &c) {
                         return launch_direct(dj, 1, sample, depth, counter, err, nullptr, c, ncu, st, S.h.mdesc != nullptr);
                     },
-                    best[0], best_ms[0], def_ms[0], count);
+                    default_launch_cfg(0), best[0], best_ms[0], def_ms[0], count);
     best[0].grid_density = density;
     if (rc == 0)
         rc = search(st, tm, trace_c,
                     [&](const LaunchCfg &c) {
                         return launch_diffuse(fj, sample, depth, counter, err, nullptr, c, ncu, st, S.h.mdesc != nullptr);
                     },
-                    best[1], best_ms[1], def_ms[1], count);
+                    default_launch_cfg(1), best[1], best_ms[1], def_ms[1], count);
     if (rc == 0)
         rc = search(st, tm, gather_c,
                     [&](const LaunchCfg &c) { return launch_gather(gj, 1, sample, C, counter, c, ncu, st); },
-                    best[2], best_ms[2], def_ms[2], count);
+                    default_launch_cfg(2), best[2], best_ms[2], def_ms[2], count);
     int herr = 0;
     if (rc == 0 && (hipMemcpy(&herr, err, 4, hipMemcpyDeviceToHost) != hipSuccess || herr != 0)) {
         set_error("rt_autotune: traversal error code " + std::to_string(herr));

@@ -39,6 +39,15 @@ struct LaunchCfg {
     uint32_t chunk = 1;            // work items taken per work-queue fetch
     uint32_t grid_density = 2;     // direct pass: occlusion-grid cells per primitive
 };
+// Default shape of each kernel class (RT_KERNEL_*): the autotuner's pick on both the 1 M-primitive orchard
+// and the 825 k-primitive masked canopy (two-wave workgroups, 16 per CU, for the traversal kernels;
+// 8 four-wave workgroups per CU for the gather).
+inline LaunchCfg default_launch_cfg(int kernel) {
+    LaunchCfg c;
+    if (kernel == 2) { c.waves_per_block = 4; c.blocks_per_cu = 8; }
+    else { c.waves_per_block = 2; c.blocks_per_cu = 16; }
+    return c;
+}
 
 // kernel launchers (rt_trace.hip)
 // stats (may be null): [0] BVH node fetches, [1] primitive tests, [2] rays traced

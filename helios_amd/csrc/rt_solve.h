@@ -48,7 +48,7 @@ int build_sun_grids(GridSet &gs, const Scene &S, const float *dirs, uint32_t n, 
 struct SolveCtx {
     hipStream_t stream = nullptr;
     int ncu = 256;
-    LaunchCfg cfg[3];
+    LaunchCfg cfg[3] = {default_launch_cfg(0), default_launch_cfg(1), default_launch_cfg(2)};
     Workspace ws;
     bool timing = false;
     std::vector<EvRec> ev;
