@@ -24,6 +24,7 @@ import argparse
 import json
 import math
 import os
+import re
 import sys
 import time
 
@@ -59,6 +60,18 @@ def pct(a, q):
     return float(np.percentile(np.asarray(a, np.float64), q)) if len(a) else None
 
 
+# position of the COUNT template argument: k_cast<STK, COUNT>, k_camera / k_direct_src<STK, COUNT, MASK>,
+# k_direct / k_diffuse<WAVES, STK, COUNT, MASK>
+COUNT_ARG = {"k_cast": 1, "k_camera": 1, "k_direct_src": 1, "k_direct": 2, "k_diffuse": 2}
+
+
+def counting_variant(name):
+    m = re.search(r"(k_\w+)<([^>]*)>", name)
+    if not m or m.group(1) not in COUNT_ARG:
+        return False
+    return [a.strip() for a in m.group(2).split(",")][COUNT_ARG[m.group(1)]] == "true"
+
+
 def pmc_traffic(kernel, profile="pmc_traffic_config3_diurnal.json"):
     """HBM bytes per dispatch of `kernel` from the committed rocprofv3 PMC summary (FETCH_SIZE +
     WRITE_SIZE of the batched dispatch, calibrated against a copy of known size), or None."""
@@ -70,7 +83,7 @@ def pmc_traffic(kernel, profile="pmc_traffic_config3_diurnal.json"):
     # the timed (non-counting) instances of the kernel; the largest dispatch is the batched series'
     best = None
     for name, cs in d.get("kernels", {}).items():
-        if kernel in name and "true>" not in name and "FETCH_SIZE" in cs and "WRITE_SIZE" in cs:
+        if kernel in name and not counting_variant(name) and "FETCH_SIZE" in cs and "WRITE_SIZE" in cs:
             b = cs["FETCH_SIZE"]["max_bytes"] + cs["WRITE_SIZE"]["max_bytes"]
             best = b if best is None else max(best, b)
     return best
@@ -332,6 +345,8 @@ def bench_homogeneous(args):
     1024 direct rays (band "direct", sun at 36 degrees zenith) and 1024 diffuse rays (band "diffuse"),
     scattering depth 0, one band per runBand.  One step = runBand("direct") + runBand("diffuse"), each a
     full rt_run_band call with host inputs and outputs, as the harness times runBand."""
+    if args.calib_copy_mib:
+        calib_copy(args.calib_copy_mib)
     t0 = time.perf_counter()
     s, masks, mid, uv, uvid = scenes.homogeneous_canopy(1)
     gen_s = time.perf_counter() - t0
@@ -396,7 +411,8 @@ def bench_homogeneous(args):
                        "published_s": {k: {"direct": v[0], "diffuse": v[1]} for k, v in PUBLISHED_HOMOGENEOUS.items()},
                        "absorbed_mean_direct": round(float(Rd.mean()), 5), "absorbed_mean_diffuse": round(float(Rf.mean()), 5)},
             "roofline": {"bound": "hbm", "kernel": "k_diffuse", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                         "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                         "traffic": pmc_traffic("k_diffuse", profile="pmc_traffic_homogeneous_canopy.json"),
                          "bytes_per_ray": round(brays, 1), "avg_launch_ms": round(kms / args.steps, 3)}}
 
 
