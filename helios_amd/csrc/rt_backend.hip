@@ -146,6 +146,7 @@ int rt_update_geometry(rt_ctx *ctx, uint32_t nprims, const float *transforms, co
                        const uint32_t *uuids, const uint32_t *twosided, int periodic_x, int periodic_y) {
     NN(ctx, "ctx");
     if (nprims) { NN(transforms, "transforms"); NN(types, "types"); NN(twosided, "twosided"); }
+    if (validate_geometry(nprims, types, "updateGeometry")) return -1;
     std::lock_guard<std::mutex> lk(ctx->mu);
     BackendState &be = ctx->be;
     be.gP = nprims;

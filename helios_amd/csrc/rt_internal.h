@@ -22,6 +22,18 @@ struct Scene {
     int builder = 0;  // 1 = PLOC, 0 = Karras LBVH
 };
 
+// Largest primitive count of a scene: leaf tags and hit words carry a 30-bit primitive index.
+constexpr uint64_t RT_MAX_PRIMS = (uint64_t)RT_ID_BITS + 1u;
+// Largest work-item count of one persistent launch: the 32-bit work counter overshoots the item count by at
+// most one chunk per wave (<= 256 items x 65,536 waves) before every wave has seen the end.
+constexpr uint64_t RT_MAX_WORK_ITEMS = 0xFFFFFFFFull - (1ull << 25);
+// Checks a primitive set before it is staged or built: at most RT_MAX_PRIMS primitives, every type one this
+// backend traces (0 patch, 1 triangle, 3 tile); disks (2), voxels (4) and anything else are rejected with an
+// error naming the type, so no primitive silently drops out of the scene.  `who` prefixes the message.
+int validate_geometry(uint64_t P, const uint32_t *types, const char *who);
+// Rejects a launch of more than RT_MAX_WORK_ITEMS work items (e.g. suns x sources x primitives of a direct pass).
+int check_work_items(uint64_t items, const char *what);
+
 int scene_build(Scene &S, uint32_t P, const float *T, const uint32_t *type, const uint32_t *uuid,
                 const uint32_t *twos, int px, int py, hipStream_t st);
 void scene_free(Scene &S);

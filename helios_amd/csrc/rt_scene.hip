@@ -577,8 +577,34 @@ static int build_bvh4(uint32_t n, const uint32_t *child, const float *lbox, cons
     return 0;
 }
 
+int validate_geometry(uint64_t P, const uint32_t *types, const char *who) {
+    if (P > RT_MAX_PRIMS) {
+        set_error(std::string(who) + ": " + std::to_string(P) + " primitives (at most 2^30)");
+        return -1;
+    }
+    for (uint64_t i = 0; i < P; i++) {
+        const uint32_t t = types[i];
+        if (t == 0u || t == 1u || t == 3u) continue;
+        const char *what = t == 2u ? "a disk (type 2): disks are not traced by this backend (Helios disks are "
+                                     "triangle fans; pass their triangles)"
+                           : t == 4u ? "a voxel (type 4): voxels are not traced by this backend"
+                                     : "an unknown type (0 patch, 1 triangle, 3 tile)";
+        set_error(std::string(who) + ": primitive " + std::to_string(i) + " is " + what);
+        return -1;
+    }
+    return 0;
+}
+
+int check_work_items(uint64_t items, const char *what) {
+    if (items <= RT_MAX_WORK_ITEMS) return 0;
+    set_error(std::string(what) + ": " + std::to_string(items) + " work items in one launch (at most " +
+              std::to_string(RT_MAX_WORK_ITEMS) + "); split the batch");
+    return -1;
+}
+
 int scene_build(Scene &S, uint32_t P, const float *T, const uint32_t *type, const uint32_t *uuid,
                 const uint32_t *twos, int px, int py, hipStream_t st) {
+    if (validate_geometry(P, type, "scene")) return -1;
     scene_free(S);
     memset(&S.h, 0, sizeof(S.h));
     S.h.P = P;

@@ -135,7 +135,10 @@ void rt_server::worker(uint32_t w) {
         }
         Request *head = queue.front();
         Model &m = *models[head->model];
-        const uint32_t kmax = std::max(1u, std::min(max_batch, 256u / m.B));
+        uint32_t kmax = std::max(1u, std::min(max_batch, 256u / m.B));
+        // the batch's direct pass must stay within one launch's work-item limit (a lone sun that exceeds it
+        // fails in run_solve with that message)
+        if (m.Sn) kmax = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(kmax, RT_MAX_WORK_ITEMS / ((uint64_t)m.Sn * m.P)));
         std::vector<Request *> batch;
         for (Request *r : queue)
             if (r->model == head->model && batch.size() < kmax) batch.push_back(r);
@@ -489,6 +492,9 @@ int rt_solve_footprint(uint32_t nprims, const rt_band_request *req, uint32_t nsu
     NN(bytes, "bytes");
     NN(req->band_depth, "band_depth");
     if (nsuns == 0 || req->nbands == 0) { set_error("rt_solve_footprint: no suns or no bands"); return -1; }
+    if (check_work_items((uint64_t)nsuns * req->nsources * nprims,
+                         "rt_solve_footprint: direct pass (suns x sources x primitives)"))
+        return -1;
     *bytes = Workspace::capacity_for(solve_workspace_bytes(nprims, *req, nsuns, true)) +
              Workspace::capacity_for(GridSet::budget(nprims, nsuns * req->nsources)) + (3u << 20) +
              (uint64_t)nsuns * nprims * req->nbands * sizeof(float);

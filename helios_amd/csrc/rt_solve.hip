@@ -354,6 +354,9 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
         if (emission || diffuse[k]) any_primary = true;
     }
     bool need_hits = (any_primary || scattering) && n > 0;
+    // one direct launch covers every (sun, source, primitive) work item with a 32-bit work counter
+    if (any_direct && check_work_items((uint64_t)K * Sn * P, "run_band: direct pass (suns x sources x primitives)"))
+        return -1;
 
     // ---- masks [nm][C] ----
     enum { M_RD, M_AFTER, M_PRIM, M_EMIT, M_POST, M_NONE, M_BASE };
@@ -631,6 +634,7 @@ int debug_direct_sums(SolveCtx &sx, const Scene &S, uint32_t d, uint32_t seed, u
     const uint32_t P = S.h.P;
     std::vector<SourceShape> shapes;
     if (source_shapes(nsrc, types, widths, rotations, shapes, "rt_debug_direct_sums")) return -1;
+    if (check_work_items((uint64_t)nsrc * P, "rt_debug_direct_sums (sources x primitives)")) return -1;
     size_t bytes = (size_t)nsrc * P * 2 * 4 + nsrc * sizeof(DirectJob) + 4096;
     if (sx.ws.reserve(bytes)) return -1;
     Carve cv{(char *)sx.ws.base};

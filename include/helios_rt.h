@@ -10,8 +10,12 @@
  *     index" that seeds the per-ray RNG (R/shaders/direct_raygen.comp:360-362);
  *   - transforms are 4x4 row-major; canonical patch (-0.5..0.5)^2, canonical triangle
  *     (0,0,0),(0,1,0),(1,1,0) (R/src/backends/VulkanComputeBackend.cpp:328-329);
- *   - types: 0 patch, 1 triangle, 3 tile (traced as patch); others are carried but not traced
- *     (the Vulkan backend traces patch/tile/triangle only, R/shaders/common/bvh_traversal.glsl:311-325);
+ *   - types: 0 patch, 1 triangle, 3 tile (traced as patch); scene creation and rt_update_geometry reject
+ *     any other type with an error naming it, so no primitive silently drops out of a scene (the Vulkan
+ *     backend traces patch/tile/triangle only, R/shaders/common/bvh_traversal.glsl:311-325; the OptiX 8
+ *     backend rejects disks and voxels, R/src/optix8/OptiX8DeviceCode.cu:492-504);
+ *   - at most 2^30 primitives per scene; one direct pass covers at most 2^32 - 2^25 (sun, source,
+ *     primitive) work items, and larger requests fail with an error saying so;
  *   - twosided: raw Helios flag 0..3 (R/src/RadiationModel.cpp:5941);
  *   - radiation buffers are [primitive][band] (R/include/BufferIndexing.h:65-262).
  */
@@ -334,7 +338,8 @@ int rt_server_submit(rt_server *srv, uint32_t model_id, const rt_solve_sun *sun,
 int rt_server_wait(rt_server *srv, uint64_t ticket, double *latency_ms);
 int rt_server_get_stats(rt_server *srv, rt_server_stats *out);
 /* Device bytes (workspace + output) a server slot needs for a batch of nsuns suns of this request
- * over nprims primitives -- the figure admission compares with the budget. */
+ * over nprims primitives -- the figure admission compares with the budget.  Host-only (no device call);
+ * fails, as the solve would, when nsuns x sources x nprims exceeds one launch's work-item limit. */
 int rt_solve_footprint(uint32_t nprims, const rt_band_request *req, uint32_t nsuns, uint64_t *bytes);
 int rt_server_reset_stats(rt_server *srv);
 

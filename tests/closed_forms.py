@@ -82,3 +82,48 @@ def beer_diffuse(lai, N=50):
         th = (i + 0.5) * dth
         tot += 2.0 * (1.0 - math.exp(-0.5 * lai / math.cos(th))) * math.cos(th) * math.sin(th) * dth
     return tot
+
+
+def perpendicular_plates_F(W=1.0, H=1.0):
+    """View factor from a 1 x W plate to a perpendicular 1 x H plate sharing the edge of length 1 (both
+    lengths in units of the common edge; Howell's catalogue, case C-14).  0.20004 for two unit squares."""
+    A, B = W * W, H * H
+    Cc = A + B
+    t = W * math.atan(1.0 / W) + H * math.atan(1.0 / H) - math.sqrt(Cc) * math.atan(1.0 / math.sqrt(Cc))
+    lg = 0.25 * math.log(((1 + A) * (1 + B) / (1 + Cc)) * (A * (1 + Cc) / ((1 + A) * Cc)) ** A
+                         * (B * (1 + Cc) / ((1 + B) * Cc)) ** B)
+    return (t + lg) / (math.pi * W)
+
+
+def common_edge_squares():
+    """Two one-sided unit squares at 90 degrees sharing an edge (selfTest.cpp:585-668): patch 0 horizontal at
+    z = 0 facing +z, patch 1 in the plane x = 0.5 facing -x (towards the space above patch 0)."""
+    T = np.stack([patch_T((0, 0, 0), (1, 0, 0), (0, 1, 0)),
+                  patch_T((0.5, 0, 0.5), (0, 0, 1), (0, 1, 0))])
+    return T, np.zeros(2, np.uint32), np.zeros(2, np.uint32), perpendicular_plates_F()
+
+
+def common_edge_squares_sw(Q, Qd, rho0, F, schedule):
+    """Absorbed shortwave flux of the two squares above, sun at the zenith (flux Q, patch 1 edge-on), isotropic
+    sky Qd, patch 0 reflectivity rho0, patch 1 black, scattering depth 1, emission off.
+
+    Patch 0 absorbs (1 - rho0) Q of the sun and (1 - rho0)(1 - F) Qd of the sky under both schedules.  Patch 1
+    sees the sky through 1 - F and the reflected sun rho0 Q of patch 0 through F.  The reference schedule
+    (RadiationModel.cpp:4188-4189, 4286-4289: no copy at s = 0 when emission is off) gathers that reflected sun
+    in the primary pass and again in the first scattering pass, and drops what patch 0 reflects of the sky; the
+    conserving schedule gathers the reflected sun once and propagates rho0 (1 - F) Qd, of which patch 1 sees F."""
+    R0 = (1 - rho0) * Q + (1 - rho0) * (1 - F) * Qd
+    if schedule == 0:
+        R1 = 2 * F * rho0 * Q + (1 - F) * Qd
+    else:
+        R1 = F * rho0 * Q + (1 - F) * Qd + F * rho0 * (1 - F) * Qd
+    return R0, R1
+
+
+def gray_parallel_rectangles_lw(eps, T, F):
+    """Absorbed longwave flux of each of two facing gray one-sided rectangles at equal temperature T with
+    emissivity eps and reflectivity 1 - eps (selfTest.cpp:731-807; the reference's closed form reduces to this
+    for T0 = T1): each receives F J of the other's radiosity J = eps sigma T^4 / (1 - (1 - eps) F) and absorbs
+    the fraction eps of it."""
+    J = eps * SIGMA * T ** 4 / (1.0 - (1.0 - eps) * F)
+    return eps * F * J

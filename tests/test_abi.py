@@ -43,3 +43,23 @@ def test_no_silent_cpu_fallback():
         pass
     with pytest.raises(native.RTError, match="no HIP device|gfx950"):
         native.Context(0)
+
+
+def _footprint(P, nsuns, nsources):
+    req = native.BandRequest(1, [native.BandSpec("PAR", 1, False, 100.0)], rho=[[0.1]], tau=[[0.05]],
+                             direct_rays=4, diffuse_rays=4, source_dirs=[[0.0, 0.0, 1.0]] * nsources,
+                             source_flux=[[1000.0]] * nsources)
+    return native.solve_footprint(P, req, nsuns)
+
+
+def test_direct_work_items_are_bounded():
+    # suns x sources x primitives of one direct launch is counted in 32 bits: a batch that would wrap it is
+    # rejected (host-only check, the same one run_band makes before any launch), one that fits is sized
+    limit = 0xFFFFFFFF - (1 << 25)
+    P = 1 << 26
+    assert 63 * P <= limit < 64 * P
+    assert _footprint(P, 63, 1) > 0
+    with pytest.raises(native.RTError, match="work items"):
+        _footprint(P, 64, 1)
+    with pytest.raises(native.RTError, match="work items"):
+        _footprint(P, 16, 4)

@@ -10,16 +10,23 @@ from tests.helpers import small_canopy, two_band_request
 
 pytestmark = pytest.mark.gpu
 
-DEFAULT = (4, 4, 1)
 
 
-def _restore(ctx):
+@pytest.fixture(scope="module")
+def initial(ctx):
+    """The context's launch shapes and grid density as this module found them (the library's defaults, or
+    whatever an earlier module left); every test restores them."""
+    return [ctx.get_launch_cfg(k) for k in range(3)], ctx.grid_density()
+
+
+def _restore(ctx, initial):
+    shapes, density = initial
     for k in range(3):
-        ctx.set_launch_cfg(k, *DEFAULT)
+        ctx.set_launch_cfg(k, *shapes[k], grid_density=density if k == 0 else 0)
 
 
 @pytest.mark.parametrize("shape", [(1, 2, 1), (2, 8, 4), (4, 16, 16), (1, 1, 3)])
-def test_results_independent_of_launch_shape(ctx, shape):
+def test_results_independent_of_launch_shape(ctx, initial, shape):
     s = small_canopy(5, 6000)
     g = ctx.scene(s.transforms, s.types, s.twosided)
     req = two_band_request(s, depth=2, nrays=36)
@@ -30,11 +37,11 @@ def test_results_independent_of_launch_shape(ctx, shape):
             assert ctx.get_launch_cfg(k) == shape
         R1 = native.run_band(g, req)
     finally:
-        _restore(ctx)
+        _restore(ctx, initial)
     assert np.array_equal(R0, R1)
 
 
-def test_autotune_installs_fastest_shapes(ctx):
+def test_autotune_installs_fastest_shapes(ctx, initial):
     s = small_canopy(5, 20000)
     g = ctx.scene(s.transforms, s.types, s.twosided)
     req = two_band_request(s, depth=2, nrays=36)
@@ -54,15 +61,16 @@ def test_autotune_installs_fastest_shapes(ctx):
                                       source_dirs=[[0.0, 0.0, 0.0]], source_flux=[[0.0, 0.0]])
         assert ctx.autotune(g, zero_dir, nsuns=8)["candidates"] == 67
     finally:
-        _restore(ctx)
+        _restore(ctx, initial)
     assert np.array_equal(R0, R1)
 
 
-def test_launch_cfg_validation(ctx):
+def test_launch_cfg_validation(ctx, initial):
     with pytest.raises(native.RTError, match="waves_per_block"):
         ctx.set_launch_cfg(0, 3, 4, 1)
     with pytest.raises(native.RTError, match="blocks_per_cu"):
         ctx.set_launch_cfg(1, 4, 0, 1)
     with pytest.raises(native.RTError, match="chunk"):
         ctx.set_launch_cfg(2, 4, 4, 1000)
-    assert ctx.get_launch_cfg(0) == DEFAULT
+    # rejected shapes leave the installed ones untouched
+    assert [ctx.get_launch_cfg(k) for k in range(3)] == initial[0]

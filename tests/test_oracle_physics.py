@@ -88,7 +88,7 @@ def test_beer_law_periodic_canopy_direct_and_diffuse():
         assert abs(ground_int - exact) <= 0.01, (ground_int, exact)
 
 
-def test_schedules_agree_without_direct_and_conserve_energy():
+def test_schedules_agree_without_direct():
     # emission-only enclosure: the reference schedule already propagates every pass, so both
     # schedules must give identical results (RadiationModel.cpp:4225-4234, 4286-4288)
     T, ty, tw = cf.closed_box(m=3)
