@@ -1,6 +1,7 @@
 // rt_api.hip -- extern "C" entry points declared in include/helios_rt.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -318,12 +319,46 @@ int rt_run_band_batch(rt_ctx *ctx, const rt_scene *scene, const rt_band_request 
     return run_solve(ctx->sx, scene->s, *base, s, R_out, out_on_device);
 }
 
+int rt_set_energy_accounting(rt_ctx *ctx, int on) {
+    NN(ctx, "ctx");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->sx.energy_acct = on != 0;
+    return 0;
+}
+
+static int energy_ready(rt_ctx *ctx, uint32_t nchannels) {
+    if (!ctx->sx.energy_valid) {
+        set_error("no energy bookkeeping: call rt_set_energy_accounting(ctx, 1) before the solve");
+        return -1;
+    }
+    if (nchannels != ctx->sx.energy_C) {
+        set_error("channel count " + std::to_string(nchannels) + " does not match the last solve's " +
+                  std::to_string(ctx->sx.energy_C));
+        return -1;
+    }
+    return 0;
+}
+
 int rt_energy_residual_fp64(rt_ctx *ctx, double *out, uint32_t nchannels) {
     NN(ctx, "ctx");
     NN(out, "out");
-    size_t C = ctx->sx.energy.size() / 3;
-    if (nchannels != C) { set_error("channel count does not match the last solve"); return -1; }
-    for (size_t k = 0; k < 3 * C; k++) out[k] = ctx->sx.energy[k];
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (energy_ready(ctx, nchannels)) return -1;
+    std::copy(ctx->sx.energy.begin(), ctx->sx.energy.end(), out);
+    return 0;
+}
+
+int rt_energy_passes_fp64(rt_ctx *ctx, double *out, uint32_t nchannels, uint32_t max_passes, uint32_t *npasses) {
+    NN(ctx, "ctx");
+    NN(npasses, "npasses");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (energy_ready(ctx, nchannels)) return -1;
+    const uint32_t np = ctx->sx.energy_npass;
+    *npasses = np;
+    if (out) {
+        const size_t n = (size_t)std::min(np, max_passes) * RT_ENERGY_PASS_ROWS * nchannels;
+        std::copy(ctx->sx.energy_passes.begin(), ctx->sx.energy_passes.begin() + n, out);
+    }
     return 0;
 }
 

@@ -77,8 +77,9 @@ int launch_direct_jobs(const DirectJob *hjobs, const DirectJob *jobs_dev, uint32
 // one diffuse job (device pointer) per launch: items [0, total_items) of that job
 int launch_diffuse(const DiffuseJob *job_dev, uint32_t total_items, uint32_t depth, uint32_t *counter, int *err,
                    unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st, bool masked);
+// acct: the jobs carry energy-bookkeeping outputs (GatherJob.acct / .miss)
 int launch_gather(const GatherJob *jobs_dev, uint32_t njobs, uint32_t total_items, uint32_t cmax, uint32_t *counter,
-                  const LaunchCfg &cfg, int ncu, hipStream_t st);
+                  const LaunchCfg &cfg, int ncu, hipStream_t st, bool acct = false);
 // Camera pixels (labels = false) or pixel labels/depths (labels = true) of one tile.
 int launch_camera(const CameraLaunch &c, uint32_t depth, bool labels, int *err, unsigned long long *stats,
                   hipStream_t st, bool masked);

@@ -53,7 +53,14 @@ struct SolveCtx {
     bool timing = false;
     std::vector<EvRec> ev;
     rt_timing timing_info{};
-    std::vector<double> energy;
+    // FP64 energy bookkeeping (rt_set_energy_accounting): off by default; when on, each solve records the
+    // per-channel totals (RT_ENERGY_ROWS rows of C) and per-pass rows (RT_ENERGY_PASS_ROWS rows of C per
+    // diffuse/scattering pass) of its channels
+    bool energy_acct = false;
+    bool energy_valid = false;
+    uint32_t energy_C = 0, energy_npass = 0;
+    std::vector<double> energy;         // [RT_ENERGY_ROWS][C]
+    std::vector<double> energy_passes;  // [npass][RT_ENERGY_PASS_ROWS][C]
     // traversal counters: [0..2] direct, [3..5] diffuse, [6..8] castRays, [9..11] camera
     // (node fetches, prim tests, rays)
     bool counting = false;
@@ -95,7 +102,9 @@ struct DevMaterials {
 };
 
 // Device bytes a solve of K suns needs in its workspace.
-size_t solve_workspace_bytes(uint32_t P, const rt_band_request &q, uint32_t K, bool resident_materials);
+// acct: with the energy bookkeeping's buffers
+size_t solve_workspace_bytes(uint32_t P, const rt_band_request &q, uint32_t K, bool resident_materials,
+                             bool acct = false);
 
 int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunSet &suns, float *R_out,
               int out_on_device, const DevMaterials *dm = nullptr);

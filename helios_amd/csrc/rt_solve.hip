@@ -102,32 +102,113 @@ __global__ void k_final(uint32_t P, uint32_t C, uint32_t B, const float *__restr
     R[((size_t)k * P + i) * B + b] = in[q] + st[q] + sb[q];
 }
 
-// FP64 per-channel totals weighted by primitive area (absorbed, direct incident)
-__global__ void k_energy(uint32_t P, uint32_t C, uint32_t B, uint32_t S, const float4 *__restrict__ rec_unused,
-                         const double *__restrict__ area, const float *__restrict__ R, const float *__restrict__ dsum,
-                         const float *__restrict__ flux, double *out) {
+// Emitted flux density of primitive i in band b (RadiationModel.cpp:4041-4137): sigma eps T^4, FP64.
+__device__ __forceinline__ double emitted(uint32_t i, uint32_t B, uint32_t b, const float *eps, const float *temp,
+                                          const uint32_t *depth) {
+    float e = eps ? eps[(size_t)i * B + b] : 1.0f;
+    if (depth[b] == 0 && e != 1.f) e = 1.f;
+    float T = temp ? temp[i] : 300.f;
+    if (T < 0.f) T = 300.f;
+    return 5.6703744E-8 * (double)e * ((double)T * (double)T * (double)T * (double)T);
+}
+
+// FP64 per-channel totals weighted by primitive area, one block per channel (rows of out, [row][C]):
+// 0 source (direct incident), 3 absorbed (sum of radiation_flux), 5 leftover scatter (deposited as absorbed by
+// the final combine, part of row 3), 2 emitted (sigma eps T^4 from each radiating face: the top, and the bottom
+// of a two-sided primitive) when emit_mask is given.  Rows 1 and 4 (sky, escaped) come from the passes.
+__global__ void k_energy(uint32_t P, uint32_t C, uint32_t B, uint32_t S, const double *__restrict__ area,
+                         const float *__restrict__ R, const float *__restrict__ dsum, const float *__restrict__ flux,
+                         const float *__restrict__ st, const float *__restrict__ sb, const float4 *__restrict__ nrm,
+                         const uint8_t *__restrict__ emit_mask, const float *__restrict__ eps,
+                         const float *__restrict__ temp, const uint32_t *__restrict__ depth, double *out) {
     uint32_t c = blockIdx.x;
     uint32_t k = c / B, b = c % B;
-    double a0 = 0.0, a1 = 0.0;
+    const bool em = emit_mask && emit_mask[c];
+    double a_abs = 0.0, a_src = 0.0, a_left = 0.0, a_emit = 0.0;
     for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) {
         double A = area[i];
-        a0 += A * (double)R[((size_t)k * P + i) * B + b];
+        a_abs += A * (double)R[((size_t)k * P + i) * B + b];
+        a_left += A * ((double)st[(size_t)i * C + c] + (double)sb[(size_t)i * C + c]);
         if (dsum) {
             for (uint32_t s = 0; s < S; s++) {
                 const float *ds = dsum + (((size_t)k * S + s) * P + i) * 2;
-                a1 += A * (double)flux[((size_t)k * S + s) * B + b] * ((double)ds[0] + (double)ds[1]);
+                a_src += A * (double)flux[((size_t)k * S + s) * B + b] * ((double)ds[0] + (double)ds[1]);
             }
         }
+        if (em && A > 0.0) {
+            const double faces = __float_as_uint(nrm[i].w) != 0u ? 2.0 : 1.0;
+            a_emit += A * faces * emitted(i, B, b, eps, temp, depth);
+        }
     }
-    __shared__ double r0[256], r1[256];
-    r0[threadIdx.x] = a0;
-    r1[threadIdx.x] = a1;
+    __shared__ double r[4][256];
+    r[0][threadIdx.x] = a_src;
+    r[1][threadIdx.x] = a_emit;
+    r[2][threadIdx.x] = a_abs;
+    r[3][threadIdx.x] = a_left;
     __syncthreads();
     for (int off = 128; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off) { r0[threadIdx.x] += r0[threadIdx.x + off]; r1[threadIdx.x] += r1[threadIdx.x + off]; }
+        if ((int)threadIdx.x < off)
+            for (int q = 0; q < 4; q++) r[q][threadIdx.x] += r[q][threadIdx.x + off];
         __syncthreads();
     }
-    if (threadIdx.x == 0) { out[c] = r0[0]; out[C + c] = r1[0]; }
+    if (threadIdx.x == 0) {
+        out[0 * C + c] = r[0][0];
+        out[2 * C + c] = r[1][0];
+        out[3 * C + c] = r[2][0];
+        out[5 * C + c] = r[3][0];
+    }
+}
+
+// Energy bookkeeping of one diffuse/scattering pass, FP64 partial sums of one block over its range of
+// primitives, thread per channel (partial[block][row][C]):
+//   0 radiated: A (out_top + out_bottom) of the faces that launch rays (the top of every traced primitive,
+//     the bottom of a two-sided one) -- what the gather hands on from each primitive;
+//   1 arrived:  A x (incident - sky) -- flux gathered from surfaces;
+//   2 sky:      A x sky incident;
+//   3 escaped:  A (out_top m_top + out_bottom m_bottom) / n^2, with m the face's rays that reached the sky:
+//     the radiated flux times the primitive's own Monte Carlo estimate of its view factor to the sky.
+// A channel the pass does not deposit (a retired band) moves no energy and adds nothing.  Radiated =
+// arrived + escaped up to Monte Carlo error when the pass hands on exactly radiation_out.
+__global__ void k_energy_pass(uint32_t P, uint32_t C, uint32_t nn, const double *__restrict__ area,
+                              const float4 *__restrict__ nrm, const uint32_t *__restrict__ type,
+                              const uint8_t *__restrict__ active, const float *__restrict__ acct,
+                              const uint32_t *__restrict__ miss, const float *__restrict__ ot,
+                              const float *__restrict__ ob, double *partial) {
+    const uint32_t per = (P + gridDim.x - 1) / gridDim.x;
+    const uint32_t i0 = min(P, blockIdx.x * per), i1 = min(P, i0 + per);
+    const double inv = 1.0 / (double)nn;
+    for (uint32_t c = threadIdx.x; c < C; c += blockDim.x) {
+        double rad = 0.0, arr = 0.0, sky = 0.0, esc = 0.0;
+        if (active[c]) {
+            for (uint32_t i = i0; i < i1; i++) {
+                const uint32_t ty = type[i];
+                if (!(ty == 0u || ty == 1u || ty == 3u)) continue;
+                const double A = area[i];
+                const size_t q = (size_t)i * C + c;
+                const double t = ot[q];
+                const double bt = __float_as_uint(nrm[i].w) != 0u ? (double)ob[q] : 0.0;
+                const double in_tot = acct[2 * q], in_sky = acct[2 * q + 1];
+                rad += A * (t + bt);
+                arr += A * (in_tot - in_sky);
+                sky += A * in_sky;
+                esc += A * (t * (double)miss[2 * (size_t)i] + bt * (double)miss[2 * (size_t)i + 1]) * inv;
+            }
+        }
+        double *pp = partial + (size_t)blockIdx.x * 4 * C;
+        pp[0 * C + c] = rad;
+        pp[1 * C + c] = arr;
+        pp[2 * C + c] = sky;
+        pp[3 * C + c] = esc;
+    }
+}
+
+// Sums the partial rows of k_energy_pass over its blocks in block order (deterministic): out[row][C].
+__global__ void k_energy_sum(uint32_t nblocks, uint32_t C, const double *__restrict__ partial, double *out) {
+    for (uint32_t q = threadIdx.x; q < 4 * C; q += blockDim.x) {
+        double a = 0.0;
+        for (uint32_t g = 0; g < nblocks; g++) a += partial[(size_t)g * 4 * C + q];
+        out[q] = a;
+    }
 }
 
 // primitive areas in FP64 from the transforms (|T e_x x T e_y| for patches, half that for triangles)
@@ -300,7 +381,9 @@ static int sky_table(const rt_band_request &q, const SunSet &suns, uint32_t K, s
     return 0;
 }
 
-size_t solve_workspace_bytes(uint32_t P, const rt_band_request &q, uint32_t K, bool resident_materials) {
+static constexpr uint32_t ENERGY_BLOCKS = 1024;  // blocks of k_energy_pass
+
+size_t solve_workspace_bytes(uint32_t P, const rt_band_request &q, uint32_t K, bool resident_materials, bool acct) {
     const uint32_t B = q.nbands, Sn = q.nsources, C = K * B;
     uint32_t maxdepth = 0;
     for (uint32_t b = 0; b < B; b++) maxdepth = std::max(maxdepth, q.band_depth[b]);
@@ -315,7 +398,11 @@ size_t solve_workspace_bytes(uint32_t P, const rt_band_request &q, uint32_t K, b
                    (size_t)K * Sn * sizeof(DirectJob) + (sizeof(DiffuseJob) + sizeof(GatherJob)) * (1 + maxdepth) +
                    (size_t)C * 9 * 4 + 64 * 64;
     if (!resident_materials) bytes += (size_t)P * B * 4 * 3 + (size_t)P * 4;
-    return bytes + 32 * 256;  // carve alignment
+    // energy bookkeeping: [P][C][2] pass densities, [P][2] sky-bound rays, partials, per-pass rows, totals
+    if (acct)
+        bytes += PC * 8 + (size_t)P * 8 + (size_t)ENERGY_BLOCKS * 4 * C * 8 + (size_t)(1 + maxdepth) * 4 * C * 8 +
+                 (size_t)C * RT_ENERGY_ROWS * 8;
+    return bytes + 40 * 256;  // carve alignment
 }
 
 size_t Workspace::capacity_for(size_t bytes) { return bytes + bytes / 8 + (1u << 20); }
@@ -393,7 +480,7 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
 
     // ---- workspace ----
     size_t PC = (size_t)P * C;
-    size_t bytes = solve_workspace_bytes(P, q, K, dm != nullptr);
+    size_t bytes = solve_workspace_bytes(P, q, K, dm != nullptr, sx.energy_acct);
     if (sx.ws.reserve(bytes)) return -1;
     Carve cv{(char *)sx.ws.base};
     float *in = cv.take<float>(PC), *stb = cv.take<float>(PC), *sbb = cv.take<float>(PC);
@@ -410,7 +497,12 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
     uint8_t *masks = cv.take<uint8_t>(mk.size());
     uint32_t *depth = cv.take<uint32_t>(B);
     double *area = cv.take<double>(P);
-    double *energy = cv.take<double>((size_t)C * 3);
+    const bool acct = sx.energy_acct;
+    double *energy = cv.take<double>((size_t)C * RT_ENERGY_ROWS);
+    float *eacct = acct ? cv.take<float>(PC * 2) : nullptr;
+    uint32_t *emiss = acct ? cv.take<uint32_t>((size_t)P * 2) : nullptr;
+    double *epart = acct ? cv.take<double>((size_t)ENERGY_BLOCKS * 4 * C) : nullptr;
+    double *epass = acct ? cv.take<double>((size_t)(1 + maxdepth) * 4 * C) : nullptr;
     DirectJob *djobs = cv.take<DirectJob>((size_t)K * Sn + 1);
     DiffuseJob *fjobs = cv.take<DiffuseJob>(1 + maxdepth);
     GatherJob *gjobs = cv.take<GatherJob>(1 + maxdepth);
@@ -506,6 +598,8 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
         J.order = S.h.order;
         J.sky = g == 0 ? dsky : nullptr;  // the sky is seen by the primary diffuse pass only
         J.seed = q.diffuse_seed[g];
+        J.acct = eacct;
+        J.miss = emiss;
     }
     CK(hipMemcpyAsync(gjobs, hg.data(), hg.size() * sizeof(GatherJob), hipMemcpyHostToDevice, st));
     // the host vectors above must outlive the async copies
@@ -533,6 +627,7 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
         k_after_direct<<<nb(PC), 256, 0, st>>>(P, C, S.h.nrm, masks + (size_t)M_AFTER * C, stb, sbb, ot, ob);
         LK();
     }
+    uint32_t npass = 0;  // passes with energy rows
     auto diffuse_pass = [&](uint32_t g) -> int {
         hipEvent_t a, b;
         if (ev_pair(sx, a, b)) return -1;
@@ -546,8 +641,17 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
         hipEvent_t c2, d2;
         if (ev_pair(sx, c2, d2)) return -1;
         if (c2) CK(hipEventRecord(c2, st));
-        if (launch_gather(gjobs + g, 1, P, C, counter, sx.cfg[2], sx.ncu, st)) return -1;
+        if (launch_gather(gjobs + g, 1, P, C, counter, sx.cfg[2], sx.ncu, st, acct)) return -1;
         if (d2) { CK(hipEventRecord(d2, st)); tick(sx, 2, c2, d2); }
+        if (acct) {
+            const uint8_t *act = hg[g].active;
+            k_energy_pass<<<ENERGY_BLOCKS, 256, 0, st>>>(P, C, nn, area, S.h.nrm, S.h.type, act, eacct, emiss, ot, ob,
+                                                         epart);
+            LK();
+            k_energy_sum<<<1, 256, 0, st>>>(ENERGY_BLOCKS, C, epart, epass + (size_t)npass * 4 * C);
+            LK();
+            npass++;
+        }
         return 0;
     };
     // ---- EMISSION + PRIMARY DIFFUSE (RadiationModel.cpp:4040-4222) ----
@@ -584,12 +688,40 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
     }
     k_final<<<nb(PC), 256, 0, st>>>(P, C, B, in, stb, sbb, R);
     LK();
-    k_energy<<<C, 256, 0, st>>>(P, C, B, Sn, nullptr, area, R, any_direct ? dsum : nullptr, flux, energy);
-    LK();
+    sx.energy_valid = false;
+    if (acct) {
+        CK(hipMemsetAsync(energy, 0, (size_t)C * RT_ENERGY_ROWS * 8, st));
+        const bool emits = any_primary && n > 0 && emission;
+        k_energy<<<C, 256, 0, st>>>(P, C, B, Sn, area, R, any_direct ? dsum : nullptr, flux, stb, sbb, S.h.nrm,
+                                    emits ? masks + (size_t)M_EMIT * C : nullptr, eps, temp, depth, energy);
+        LK();
+    }
     if (!out_on_device) CK(hipMemcpyAsync(R_out, Rd, PC * 4, hipMemcpyDeviceToHost, st));
-    sx.energy.assign((size_t)C * 3, 0.0);
-    CK(hipMemcpyAsync(sx.energy.data(), energy, (size_t)C * 2 * 8, hipMemcpyDeviceToHost, st));
+    if (acct) {
+        sx.energy.assign((size_t)C * RT_ENERGY_ROWS, 0.0);
+        sx.energy_passes.assign((size_t)npass * 4 * C, 0.0);
+        CK(hipMemcpyAsync(sx.energy.data(), energy, (size_t)C * RT_ENERGY_ROWS * 8, hipMemcpyDeviceToHost, st));
+        if (npass)
+            CK(hipMemcpyAsync(sx.energy_passes.data(), epass, (size_t)npass * 4 * C * 8, hipMemcpyDeviceToHost, st));
+    }
     CK(hipStreamSynchronize(st));
+    if (acct) {
+        // sky (row 1) and escaped (row 4) from the passes; residual (row 6) = in - out
+        for (uint32_t c = 0; c < C; c++) {
+            double sky = 0.0, esc = 0.0;
+            for (uint32_t g = 0; g < npass; g++) {
+                sky += sx.energy_passes[((size_t)g * 4 + 2) * C + c];
+                esc += sx.energy_passes[((size_t)g * 4 + 3) * C + c];
+            }
+            double *e = sx.energy.data();
+            e[1 * C + c] = sky;
+            e[4 * C + c] = esc;
+            e[6 * C + c] = (e[0 * C + c] + sky + e[2 * C + c]) - (e[3 * C + c] + esc);
+        }
+        sx.energy_C = C;
+        sx.energy_npass = npass;
+        sx.energy_valid = true;
+    }
     int herr = 0;
     CK(hipMemcpy(&herr, err, sizeof(int), hipMemcpyDeviceToHost));
     if (herr) { set_error("traversal error code " + std::to_string(herr) + " (1 = stack overflow, 2 = malformed tree)"); return -1; }

@@ -1311,7 +1311,9 @@ __device__ __forceinline__ v3f diffuse_dir(const DevScene &sc, uint32_t i, uint3
     return dir;
 }
 
-template <int CM>
+// ACCT: also record the pass's incident and sky flux densities and the sky-bound ray counts (GatherJob.acct,
+// .miss) for the energy bookkeeping; a separate instance, so the plain gather carries no extra registers.
+template <int CM, bool ACCT>
 __global__ __launch_bounds__(256) void k_gather_rays(const GatherJob *__restrict__ jobs, uint32_t njobs, uint32_t total,
                                                      uint32_t *counter, uint32_t chunk) {
     const uint32_t lane = threadIdx.x & 63;
@@ -1327,14 +1329,19 @@ __global__ __launch_bounds__(256) void k_gather_rays(const GatherJob *__restrict
             uint32_t nn = J.n * J.n, C = J.C;
             const uint32_t *w = J.hits + (size_t)i * 2 * nn;
             ksum acc[2][CM];
+            ksum asky[ACCT ? CM : 1];
+            uint32_t mt = 0, mb = 0;
 #pragma unroll
             for (int f = 0; f < 2; f++)
 #pragma unroll
                 for (int c = 0; c < CM; c++) acc[f][c].init();
+#pragma unroll
+            for (int c = 0; c < (ACCT ? CM : 1); c++) asky[c].init();
             for (uint32_t r = lane; r < 2 * nn; r += 64) {
                 uint32_t word = w[r];
                 if (word == RT_NOT_LAUNCHED) continue;
                 uint32_t slot = r >= nn ? 1u : 0u;
+                if (ACCT && word == RT_MISS) { if (slot) mb++; else mt++; }
                 if (word == RT_MISS && J.sky) {
                     // anisotropic sky: fd(direction) x diffuse flux (diffuse_raygen.comp:613-625)
                     const v3f dir = diffuse_dir(*J.sc, i, r, J.n, J.seed);
@@ -1343,6 +1350,7 @@ __global__ __launch_bounds__(256) void k_gather_rays(const GatherJob *__restrict
                         if ((uint32_t)c < C) {
                             float v = sky_weight(dir, J.sky + 9 * c) * J.dflux[c];
                             if (slot) acc[1][c].add(v); else acc[0][c].add(v);
+                            if (ACCT) asky[ACCT ? c : 0].add(v);
                         }
                 } else if (word == RT_MISS) {
 #pragma unroll
@@ -1350,6 +1358,7 @@ __global__ __launch_bounds__(256) void k_gather_rays(const GatherJob *__restrict
                         if ((uint32_t)c < C) {
                             float v = J.dflux[c];
                             if (slot) acc[1][c].add(v); else acc[0][c].add(v);
+                            if (ACCT) asky[ACCT ? c : 0].add(v);
                         }
                 } else {
                     uint32_t h = word & ~RT_HIT_BOTTOM;
@@ -1367,22 +1376,36 @@ __global__ __launch_bounds__(256) void k_gather_rays(const GatherJob *__restrict
             for (int c = 0; c < CM; c++) {
                 wave_reduce(acc[0][c]);
                 wave_reduce(acc[1][c]);
+                if (ACCT) wave_reduce(asky[ACCT ? c : 0]);
+            }
+            if (ACCT) {
+                for (int off = 32; off > 0; off >>= 1) { mt += __shfl_xor(mt, off); mb += __shfl_xor(mb, off); }
+                if (lane == 0) { J.miss[(size_t)i * 2] = mt; J.miss[(size_t)i * 2 + 1] = mb; }
             }
             if (lane < C && lane < CM) {
-                float I1 = 0.f, I0 = 0.f;
+                float I1 = 0.f, I0 = 0.f, Is = 0.f;
 #pragma unroll
                 for (int c = 0; c < CM; c++)
-                    if ((uint32_t)c == lane) { I1 = acc[0][c].value(); I0 = acc[1][c].value(); }
+                    if ((uint32_t)c == lane) {
+                        I1 = acc[0][c].value(); I0 = acc[1][c].value();
+                        if (ACCT) Is = asky[ACCT ? c : 0].value();
+                    }
                 uint32_t c = lane;
-                if (J.active[c]) {
-                    I1 = I1 * strength;
-                    I0 = I0 * strength;
+                const bool act = J.active[c] != 0;
+                I1 = I1 * strength;
+                I0 = I0 * strength;
+                if (act) {
                     uint32_t b = c % J.B;
                     float rho = J.rho[(size_t)i * J.B + b], tau = J.tau[(size_t)i * J.B + b];
                     size_t q = (size_t)i * C + c;
                     J.in[q] += (I1 + I0) * (1.0f - rho - tau);
                     J.st[q] += I1 * rho + I0 * tau;
                     J.sb[q] += I1 * tau + I0 * rho;
+                }
+                if (ACCT) {
+                    const size_t q = ((size_t)i * C + c) * 2;
+                    J.acct[q] = act ? I1 + I0 : 0.f;
+                    J.acct[q + 1] = act ? Is * strength : 0.f;
                 }
             }
         }
@@ -1395,7 +1418,7 @@ __global__ __launch_bounds__(256) void k_gather_rays(const GatherJob *__restrict
 #ifndef RT_GATHER_UNROLL
 #define RT_GATHER_UNROLL 4
 #endif
-template <int CB>
+template <int CB, bool ACCT>
 __global__ __launch_bounds__(256) void k_gather_chan(const GatherJob *__restrict__ jobs, uint32_t njobs, uint32_t total,
                                                      uint32_t *counter, uint32_t chunk) {
     const uint32_t lane = threadIdx.x & 63;
@@ -1411,16 +1434,24 @@ __global__ __launch_bounds__(256) void k_gather_chan(const GatherJob *__restrict
             uint32_t nn = J.n * J.n, C = J.C;
             const uint32_t *w = J.hits + (size_t)i * 2 * nn;
             ksum acc[2][CB];
+            ksum asky[ACCT ? CB : 1];
+            uint32_t mt = 0, mb = 0;
 #pragma unroll
             for (int k = 0; k < CB; k++) {
                 acc[0][k].init();
                 acc[1][k].init();
             }
+#pragma unroll
+            for (int k = 0; k < (ACCT ? CB : 1); k++) asky[k].init();
             const float *dfl = J.dflux, *otop = J.out_top, *obot = J.out_bot;
             const float *sky = J.sky;
             for (uint32_t base = 0; base < 2 * nn; base += 64) {
                 uint32_t mine = base + lane < 2 * nn ? w[base + lane] : RT_NOT_LAUNCHED;
                 uint32_t cnt = min(64u, 2 * nn - base);
+                if (ACCT) {
+                    mt += (uint32_t)__popcll(__ballot(mine == RT_MISS && base + lane < nn));
+                    mb += (uint32_t)__popcll(__ballot(mine == RT_MISS && base + lane >= nn));
+                }
                 // anisotropic sky: each lane regenerates the direction of its own ray if it missed; the
                 // directions are broadcast with the words
                 v3f mdir = mk3(0.f, 0.f, 1.f);
@@ -1459,13 +1490,27 @@ __global__ __launch_bounds__(256) void k_gather_chan(const GatherJob *__restrict
                         for (int k = 0; k < CB; k++) {
                             if (slot) acc[1][k].add(v[u][k]); else acc[0][k].add(v[u][k]);
                         }
+                        if (ACCT) {
+                            const uint32_t q = q0 + u;
+                            const bool is_miss = q < cnt && __builtin_amdgcn_readlane(mine, q) == RT_MISS;
+#pragma unroll
+                            for (int k = 0; k < (ACCT ? CB : 1); k++)
+                                if (is_miss) asky[k].add(v[u][k]);
+                        }
                     }
                 }
             }
             float strength = rdiv(1.0f, (float)nn);
+            if (ACCT && lane == 0) { J.miss[(size_t)i * 2] = mt; J.miss[(size_t)i * 2 + 1] = mb; }
 #pragma unroll
             for (int k = 0; k < CB; k++) {
                 uint32_t c = lane + 64 * k;
+                if (ACCT && c < C) {
+                    const size_t q = ((size_t)i * C + c) * 2;
+                    const bool act = J.active[c] != 0;
+                    J.acct[q] = act ? (acc[0][k].value() * strength + acc[1][k].value() * strength) : 0.f;
+                    J.acct[q + 1] = act ? asky[ACCT ? k : 0].value() * strength : 0.f;
+                }
                 if (c >= C || !J.active[c]) continue;
                 float I1 = acc[0][k].value() * strength, I0 = acc[1][k].value() * strength;
                 uint32_t b = c % J.B;
@@ -2144,26 +2189,33 @@ int launch_diffuse(const DiffuseJob *job, uint32_t total, uint32_t depth, uint32
     return 0;
 }
 
-int launch_gather(const GatherJob *jobs, uint32_t njobs, uint32_t total, uint32_t cmax, uint32_t *counter,
-                  const LaunchCfg &cfg, int ncu, hipStream_t st) {
-    if (total == 0) return 0;
-    const uint32_t ch = std::max(cfg.chunk, 1u);
-    k_zero_u32<<<1, 1, 0, st>>>(counter);
-    uint32_t g = grid_for(cfg, ncu, total, 4);
-    if (cmax <= 1) k_gather_rays<1><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch);
-    else if (cmax <= 2) k_gather_rays<2><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch);
-    else if (cmax <= 4) k_gather_rays<4><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch);
-    else if (cmax <= 8) k_gather_rays<8><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch);
-    else if (cmax <= 64) k_gather_chan<1><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch);
-    else if (cmax <= 128) k_gather_chan<2><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch);
-    else if (cmax <= 192) k_gather_chan<3><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch);
-    else if (cmax <= 256) k_gather_chan<4><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch);
+template <bool ACCT>
+static int launch_gather_t(const GatherJob *jobs, uint32_t njobs, uint32_t total, uint32_t cmax, uint32_t *counter,
+                           uint32_t g, uint32_t ch, hipStream_t st) {
+    if (cmax <= 1) k_gather_rays<1, ACCT><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch);
+    else if (cmax <= 2) k_gather_rays<2, ACCT><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch);
+    else if (cmax <= 4) k_gather_rays<4, ACCT><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch);
+    else if (cmax <= 8) k_gather_rays<8, ACCT><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch);
+    else if (cmax <= 64) k_gather_chan<1, ACCT><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch);
+    else if (cmax <= 128) k_gather_chan<2, ACCT><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch);
+    else if (cmax <= 192) k_gather_chan<3, ACCT><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch);
+    else if (cmax <= 256) k_gather_chan<4, ACCT><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch);
     else {
         set_error("more than 256 channels in one gather");
         return -1;
     }
     LCHECK();
     return 0;
+}
+
+int launch_gather(const GatherJob *jobs, uint32_t njobs, uint32_t total, uint32_t cmax, uint32_t *counter,
+                  const LaunchCfg &cfg, int ncu, hipStream_t st, bool acct) {
+    if (total == 0) return 0;
+    const uint32_t ch = std::max(cfg.chunk, 1u);
+    k_zero_u32<<<1, 1, 0, st>>>(counter);
+    uint32_t g = grid_for(cfg, ncu, total, 4);
+    return acct ? launch_gather_t<true>(jobs, njobs, total, cmax, counter, g, ch, st)
+                : launch_gather_t<false>(jobs, njobs, total, cmax, counter, g, ch, st);
 }
 
 int launch_camera(const CameraLaunch &c, uint32_t depth, bool labels, int *err, unsigned long long *stats,

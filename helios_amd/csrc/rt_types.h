@@ -145,4 +145,9 @@ struct GatherJob {
     const uint32_t *order; // item -> primitive map (null: item k is primitive k)
     const float *sky;      // [C][9] anisotropic sky of each channel (sky_weight parameters), null = isotropic
     uint32_t seed;         // seed of the diffuse pass whose hit words these are (miss directions regenerated)
+    // energy bookkeeping (null = off): per primitive and channel the incident flux density of this pass and
+    // its sky part ([P][C][2]; 0 for a channel the pass does not deposit), and per primitive the rays of each
+    // face that reached the sky ([P][2]: top launch, bottom launch)
+    float *acct;
+    uint32_t *miss;
 };

@@ -128,6 +128,8 @@ SIGNATURES = {
     "rt_run_band_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(rt_band_request), C.POINTER(rt_sun_batch),
                                     C.c_void_p, C.c_int]),
     "rt_energy_residual_fp64": (C.c_int, [C.c_void_p, f64p, C.c_uint32]),
+    "rt_set_energy_accounting": (C.c_int, [C.c_void_p, C.c_int]),
+    "rt_energy_passes_fp64": (C.c_int, [C.c_void_p, f64p, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32)]),
     "rt_set_launch_cfg": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(rt_launch_cfg)]),
     "rt_get_launch_cfg": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(rt_launch_cfg)]),
     "rt_enable_timing": (C.c_int, [C.c_void_p, C.c_int]),
@@ -759,10 +761,29 @@ def solve_footprint(P: int, req: "BandRequest", nsuns: int) -> int:
     return out.value
 
 
-def energy_fp64(ctx: Context, nchannels: int) -> np.ndarray:
-    out = np.zeros((3, nchannels), np.float64)
+ENERGY_ROWS = ("source", "sky", "emitted", "absorbed", "escaped", "leftover", "residual")
+ENERGY_PASS_ROWS = ("radiated", "arrived", "sky", "escaped")
+
+
+def set_energy_accounting(ctx: Context, on: bool = True):
+    """FP64 energy bookkeeping of later solves on ctx (rt_set_energy_accounting)."""
+    _check(ctx.lib.rt_set_energy_accounting(ctx._h, 1 if on else 0))
+
+
+def energy_fp64(ctx: Context, nchannels: int) -> dict:
+    """Totals of the last solve (it must have run with the bookkeeping on): {row name: [C] float64}, rows as
+    ENERGY_ROWS, plus "passes": [npass][4][C] (ENERGY_PASS_ROWS per diffuse/scattering pass)."""
+    out = np.zeros((len(ENERGY_ROWS), nchannels), np.float64)
     _check(ctx.lib.rt_energy_residual_fp64(ctx._h, out.ctypes.data_as(f64p), nchannels))
-    return out
+    npass = C.c_uint32()
+    _check(ctx.lib.rt_energy_passes_fp64(ctx._h, None, nchannels, 0, C.byref(npass)))
+    passes = np.zeros((npass.value, len(ENERGY_PASS_ROWS), nchannels), np.float64)
+    if npass.value:
+        _check(ctx.lib.rt_energy_passes_fp64(ctx._h, passes.ctypes.data_as(f64p), nchannels, npass.value,
+                                             C.byref(npass)))
+    res = {k: out[i] for i, k in enumerate(ENERGY_ROWS)}
+    res["passes"] = passes
+    return res
 
 
 def sun_direction(zenith_deg: float, azimuth_deg: float) -> np.ndarray:

@@ -101,7 +101,7 @@ int rt_cast_rays_soa_packets(rt_ctx *ctx, const rt_scene *scene, const float *or
  * hit; hit_t = miss_t, hit_id = -1, hit_fnorm = 1e6 on a miss.  The conversion runs on the device. */
 int rt_lidar_cast_rays(rt_ctx *ctx, const rt_scene *scene, const float *origins, const float *dirs, size_t n,
                        size_t packet_size, float miss_t, float *hit_t, float *hit_fnorm, int32_t *hit_id);
-/* Same, with every array already in device memory; runs on the context stream and waits for it. */
+/* rt_cast_rays_soa with every array already in device memory; runs on the context stream and waits for it. */
 int rt_cast_rays_soa_device(rt_ctx *ctx, const rt_scene *scene, const float *d_origins, const float *d_dirs,
                             size_t n, float max_dist, float *d_out_t, float *d_out_normal, uint32_t *d_out_uuid);
 
@@ -173,10 +173,32 @@ typedef struct {
 int rt_run_band_batch(rt_ctx *ctx, const rt_scene *scene, const rt_band_request *base, const rt_sun_batch *suns,
                       float *R_out, int out_on_device);
 
-/* FP64 energy bookkeeping of the last rt_run_band / rt_run_band_batch on this context, per channel
- * (channel = sun*B + band): [0][C] sum(area*R) absorbed, [1][C] sum(area*direct incident),
- * [2][C] reserved, written as 0. */
+/* FP64 energy bookkeeping of the scattering iterations (no reference counterpart: the reference checks
+ * conservation only in its closed-form tests).  Off by default; when on, every rt_run_band /
+ * rt_run_band_batch on the context also records, per channel (channel = sun*B + band), area-weighted
+ * FP64 totals in W (flux density x m^2):
+ *   row 0 source   direct incident flux of the sources;
+ *   row 1 sky      sky flux arriving at primitives in the primary diffuse pass;
+ *   row 2 emitted  sigma eps T^4 from every radiating face (the top, and the bottom of a two-sided one);
+ *   row 3 absorbed sum of radiation_flux (includes row 5);
+ *   row 4 escaped  flux handed on by a pass that reaches the sky, summed over the passes (each primitive's
+ *                  radiated flux times its own rays' Monte Carlo estimate of its view factor to the sky);
+ *   row 5 leftover scatter still undelivered after the last pass (the reference deposits it as absorbed);
+ *   row 6 residual (source + sky + emitted) - (absorbed + escaped).
+ * The residual is zero up to Monte Carlo error when the schedule hands every scattered flux on exactly once
+ * (RT_SCHEDULE_CONSERVING); under RT_SCHEDULE_REFERENCE with direct rays, diffuse/emission off and
+ * scattering it measures the flux the reference gathers twice and drops (RadiationModel.cpp:4286-4289).
+ * Costs one extra [P][C] x 8-byte buffer and a reduction per pass. */
+#define RT_ENERGY_ROWS 7
+#define RT_ENERGY_PASS_ROWS 4
+int rt_set_energy_accounting(rt_ctx *ctx, int on);
+/* out: [RT_ENERGY_ROWS][nchannels] of the last solve (fails unless it ran with the bookkeeping on). */
 int rt_energy_residual_fp64(rt_ctx *ctx, double *out, uint32_t nchannels);
+/* Per diffuse/scattering pass g in run order (the primary pass first): out[g][row][nchannels] with rows
+ * 0 radiated (flux handed on by the launching faces), 1 arrived (gathered from surfaces), 2 sky (arrived
+ * from the sky), 3 escaped; radiated = arrived + escaped up to Monte Carlo error.  Writes at most
+ * max_passes passes (out may be NULL) and the number of passes to *npasses. */
+int rt_energy_passes_fp64(rt_ctx *ctx, double *out, uint32_t nchannels, uint32_t max_passes, uint32_t *npasses);
 
 /* ---------------------------------------------------------------------------------------------
  * Call-by-call backend: helios::RayTracingBackend (R/include/RayTracingBackend.h:43-312) as C
