@@ -19,6 +19,41 @@ using namespace rt;
 
 void backend_release(BackendState &be);
 
+// ------------------------------------------------------------------------------------------------
+// Scratch of the allocation-free LiDAR cast (rt_lidar_cast_rays_noalloc): one array per device, in the
+// library's own device data, holding the work counter and the traversal error flag in its first 256 bytes,
+// then the castRays normals [n][3] and UUIDs [n] of up to RT_NOALLOC_MAX_RAYS rays.  Host code never uses
+// the array's name as an address: rt_create looks up each device's copy with hipGetSymbolAddress once and
+// the kernels get that address.
+// ------------------------------------------------------------------------------------------------
+namespace {
+constexpr size_t NOALLOC_HEAD = 256;
+constexpr size_t NOALLOC_BYTES = NOALLOC_HEAD + (size_t)RT_NOALLOC_MAX_RAYS * 16;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;                       // guards the table below
+void *g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {}; // device address of each device's scratch
+std::mutex g_noalloc_run[NOALLOC_MAX_DEVICES]; // one call at a time per device in this process
+
+// Looks up this device's scratch (the device must be current).  Called by rt_create, once per device.
+int noalloc_scratch_init(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return 0;  // such a device gets no allocation-free path
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return 0;
+    void *p = nullptr;
+    CKR(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) { set_error("hipGetSymbolAddress returned no address for the scratch array"); return -1; }
+    g_noalloc_ptr[device] = p;
+    return 0;
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+}  // namespace
+
 extern "C" {
 
 const char *rt_last_error(void) { return g_err.c_str(); }
@@ -40,6 +75,7 @@ int rt_create(int device, rt_ctx **out) {
         set_error("device " + std::to_string(device) + " is " + arch + "; this library is built for gfx950 only");
         return -1;
     }
+    if (noalloc_scratch_init(device)) return -1;
     rt_ctx *c = new rt_ctx();
     c->device = device;
     c->name = std::string("hip-gfx950 (") + prop.name + ")";
@@ -273,6 +309,52 @@ int rt_lidar_cast_rays(rt_ctx *ctx, const rt_scene *scene, const float *o, const
     } while (0);
     (void)hipFree(dbuf);
     return rc;
+}
+
+int rt_lidar_cast_rays_noalloc(rt_ctx *ctx, const rt_scene *scene, const float *d_origins, const float *d_dirs,
+                               size_t n, float miss_t, float *d_hit_t, float *d_hit_fnorm, int32_t *d_hit_id) {
+    NN(ctx, "ctx");
+    NN(scene, "scene");
+    if (n == 0) return 0;
+    NN(d_origins, "origins"); NN(d_dirs, "dirs"); NN(d_hit_t, "hit_t"); NN(d_hit_fnorm, "hit_fnorm");
+    NN(d_hit_id, "hit_id");
+    if (n > RT_NOALLOC_MAX_RAYS) {
+        set_error("rt_lidar_cast_rays_noalloc: " + std::to_string(n) + " rays in one call (at most " +
+                  std::to_string(RT_NOALLOC_MAX_RAYS) + "); cast larger scans in slices");
+        return -1;
+    }
+    const int dev = ctx->device;
+    if (dev >= NOALLOC_MAX_DEVICES) { set_error("rt_lidar_cast_rays_noalloc: device index above 63"); return -1; }
+    void *base;
+    {
+        std::lock_guard<std::mutex> lk(g_noalloc_mu);
+        base = g_noalloc_ptr[dev];
+    }
+    if (!base) { set_error("rt_lidar_cast_rays_noalloc: no scratch for this device (rt_create did not set it up)"); return -1; }
+    std::lock_guard<std::mutex> run(g_noalloc_run[dev]);
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    CKR(hipSetDevice(dev));
+    hipStream_t st = ctx->sx.stream;
+    int *err = reinterpret_cast<int *>(base);
+    unsigned long long *counter = reinterpret_cast<unsigned long long *>(static_cast<char *>(base) + 8);
+    float *nrm = reinterpret_cast<float *>(static_cast<char *>(base) + NOALLOC_HEAD);
+    uint32_t *uuid = reinterpret_cast<uint32_t *>(nrm + (size_t)RT_NOALLOC_MAX_RAYS * 3);
+    // the error flag keeps the last call's value: clear it with a kernel (no memset node if captured)
+    k_fill_u32<<<1, 64, 0, st>>>(reinterpret_cast<uint32_t *>(err), 1, 0u);
+    CKR(hipGetLastError());
+    // castRaysSoA with the miss distance as the maximum distance (LiDAR.cpp:577-579); hits write hit_t
+    // directly, misses leave it for the conversion
+    if (launch_cast(scene->s.d, scene->s.h.depth, n, d_origins, d_dirs, miss_t, d_hit_t, nrm, uuid, counter, err,
+                    nullptr, ctx->sx.ncu, st))
+        return -1;
+    k_lidar_convert<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, d_dirs, nrm, uuid, miss_t, d_hit_t, d_hit_fnorm,
+                                                                 d_hit_id);
+    CKR(hipGetLastError());
+    int herr = 0;
+    CKR(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, st));
+    CKR(hipStreamSynchronize(st));
+    if (herr) { set_error("traversal error code " + std::to_string(herr) + " (1 = stack overflow, 2 = malformed tree)"); return -1; }
+    return 0;
 }
 
 static int check_request(const rt_scene *scene, const rt_band_request *q) {

@@ -27,6 +27,7 @@ class RTError(RuntimeError):
 
 # radiation source types (RayTracingTypes.h:189-196)
 SOURCE_COLLIMATED, SOURCE_SPHERE, SOURCE_SUN_SPHERE, SOURCE_RECTANGLE, SOURCE_DISK = 0, 1, 2, 3, 4
+NOALLOC_MAX_RAYS = 1 << 20  # RT_NOALLOC_MAX_RAYS
 
 
 class rt_band_request(C.Structure):
@@ -122,6 +123,8 @@ SIGNATURES = {
     "rt_update_texture_masks": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint8), u32p,
                                           C.POINTER(C.c_int32), f32p, C.POINTER(C.c_int32)]),
     "rt_cast_rays_soa": (C.c_int, [C.c_void_p, C.c_void_p, f32p, f32p, C.c_size_t, C.c_float, f32p, f32p, u32p]),
+    "rt_lidar_cast_rays_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_float,
+                                             C.c_void_p, C.c_void_p, C.c_void_p]),
     "rt_cast_rays_soa_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_float,
                                           C.c_void_p, C.c_void_p, C.c_void_p]),
     "rt_run_band": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(rt_band_request), f32p]),
@@ -383,6 +386,13 @@ class Scene:
         _check(self.ctx.lib.rt_lidar_cast_rays(self.ctx._h, self._h, _pf(O), _pf(D), n, int(packet_size), float(miss_t),
                                                _pf(t), _pf(f), ids.ctypes.data_as(C.POINTER(C.c_int32))))
         return t, f, ids
+
+    def lidar_cast_rays_noalloc(self, d_origins: int, d_dirs: int, n: int, miss_t: float, d_hit_t: int,
+                                d_hit_fnorm: int, d_hit_id: int):
+        """rt_lidar_cast_rays with device arrays and no allocation (at most NOALLOC_MAX_RAYS rays per call)."""
+        _check(self.ctx.lib.rt_lidar_cast_rays_noalloc(self.ctx._h, self._h, C.c_void_p(d_origins), C.c_void_p(d_dirs),
+                                                       n, float(miss_t), C.c_void_p(d_hit_t), C.c_void_p(d_hit_fnorm),
+                                                       C.c_void_p(d_hit_id)))
 
     def cast_rays_device(self, d_origins: int, d_dirs: int, n: int, max_dist: float, d_t: int, d_n: int, d_u: int):
         _check(self.ctx.lib.rt_cast_rays_soa_device(self.ctx._h, self._h, C.c_void_p(d_origins), C.c_void_p(d_dirs),

@@ -104,6 +104,19 @@ int rt_lidar_cast_rays(rt_ctx *ctx, const rt_scene *scene, const float *origins,
 /* rt_cast_rays_soa with every array already in device memory; runs on the context stream and waits for it. */
 int rt_cast_rays_soa_device(rt_ctx *ctx, const rt_scene *scene, const float *d_origins, const float *d_dirs,
                             size_t n, float max_dist, float *d_out_t, float *d_out_normal, uint32_t *d_out_uuid);
+/* Allocation-free LiDAR cast.  No counterpart in the reference: for callers that cannot allocate device
+ * memory (HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  rt_lidar_cast_rays
+ * with every array in device memory (origins/dirs [n][3]; hit_t, hit_fnorm, hit_id [n]) and no
+ * allocation of any kind: the castRays normals and UUIDs that the conversion reads, the work counter and
+ * the error flag live in a fixed scratch array in the library's device data (16 bytes per ray plus 256,
+ * i.e. 16 MiB + 256 B per device, held until the process ends).  At most RT_NOALLOC_MAX_RAYS rays per
+ * call (larger n fails with an error; cast a longer scan in slices).  The scratch is shared by every
+ * call on one device and keeps the last call's contents; the library serialises calls per device in
+ * this process (concurrent calls wait), and a caller must not read the scratch.  Runs on the context
+ * stream and waits for it.  Results equal rt_lidar_cast_rays bit for bit. */
+#define RT_NOALLOC_MAX_RAYS (1u << 20)
+int rt_lidar_cast_rays_noalloc(rt_ctx *ctx, const rt_scene *scene, const float *d_origins, const float *d_dirs,
+                               size_t n, float miss_t, float *d_hit_t, float *d_hit_fnorm, int32_t *d_hit_id);
 
 /* ---------------------------------------------------------------------------------------------
  * Whole runBand on the device.  Replaces the pass schedule of RadiationModel::runBand

@@ -36,3 +36,50 @@ def rel_err(a, b, floor=1e-3):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     return np.abs(a - b) / np.maximum(np.abs(b), floor)
+
+
+class DeviceArray:
+    """A device copy of a numpy array made with the HIP runtime the library itself links (libamdhip64.so.7),
+    for tests of entry points that take device pointers.  Test plumbing only."""
+
+    _hip = None
+
+    @classmethod
+    def _rt(cls):
+        if cls._hip is None:
+            import ctypes as C
+            native.load_library()  # the library has loaded the runtime already; this returns the same one
+            hip = C.CDLL("libamdhip64.so.7")
+            hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            hip.hipFree.argtypes = [C.c_void_p]
+            hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            cls._hip = hip
+        return cls._hip
+
+    def __init__(self, a):
+        import ctypes as C
+        self.a = np.ascontiguousarray(a)
+        self.ptr = C.c_void_p()
+        hip = self._rt()
+        assert hip.hipMalloc(C.byref(self.ptr), max(self.a.nbytes, 4)) == 0
+        assert hip.hipMemcpy(self.ptr, self.a.ctypes.data, self.a.nbytes, 1) == 0  # host to device
+
+    @property
+    def addr(self) -> int:
+        return int(self.ptr.value)
+
+    def get(self) -> np.ndarray:
+        out = np.empty_like(self.a)
+        assert self._rt().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, 2) == 0  # device to host
+        return out
+
+    def free(self):
+        if self.ptr.value:
+            self._rt().hipFree(self.ptr)
+            self.ptr.value = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
