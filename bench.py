@@ -72,21 +72,48 @@ def counting_variant(name):
     return [a.strip() for a in m.group(2).split(",")][COUNT_ARG[m.group(1)]] == "true"
 
 
-def pmc_traffic(kernel, profile="pmc_traffic_config3_diurnal.json"):
-    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 PMC summary (FETCH_SIZE +
-    WRITE_SIZE of the batched dispatch, calibrated against a copy of known size), or None."""
+def pmc_profile(kernel, profile="pmc_traffic_config3_diurnal.json"):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 PMC summary: FETCH_SIZE + WRITE_SIZE of
+    its largest timed (non-counting) dispatch, each scaled by what the counter read for a copy of known size,
+    and that kernel's L2 hit rate when the profile has one.  None without a profile of the kernel."""
     path = os.path.join(ROOT, "profiles", profile)
     if not os.path.exists(path):
         return None
     with open(path) as f:
         d = json.load(f)
-    # the timed (non-counting) instances of the kernel; the largest dispatch is the batched series'
     best = None
     for name, cs in d.get("kernels", {}).items():
         if kernel in name and not counting_variant(name) and "FETCH_SIZE" in cs and "WRITE_SIZE" in cs:
             b = cs["FETCH_SIZE"]["max_bytes"] + cs["WRITE_SIZE"]["max_bytes"]
-            best = b if best is None else max(best, b)
+            if best is None or b > best["bytes"]:
+                best = {"bytes": b, "fetch": cs["FETCH_SIZE"]["max_bytes"], "write": cs["WRITE_SIZE"]["max_bytes"],
+                        "l2_hit_rate": cs.get("l2_hit_rate"), "profiled_kernel": name}
     return best
+
+
+def pmc_traffic(kernel, profile="pmc_traffic_config3_diurnal.json"):
+    p = pmc_profile(kernel, profile)
+    return None if p is None else p["bytes"]
+
+
+def roofline(kernel, avg_launch_ms, canonical_bytes, profile, **extra):
+    """Roofline of the dominant kernel.  achieved = the kernel's HBM bytes per launch (rocprofv3 FETCH_SIZE +
+    WRITE_SIZE, calibrated, from the committed profile) / its HIP-event launch time, so frac is a fraction of the
+    8 TB/s HBM peak.  The canonical per-ray byte model (64 B per BVH node visit, 48 B per primitive test, 32 B of
+    I/O, counted by the kernels) is reported under "canonical": most of those bytes come from L2 and the
+    Infinity Cache, so its rate is a measure of work, not a bandwidth, and may exceed the peak."""
+    prof = pmc_profile(kernel, profile)
+    sec = avg_launch_ms / 1e3
+    ach = None if prof is None else prof["bytes"] / sec / 1e9
+    canon_gbs = canonical_bytes / sec / 1e9
+    return {"bound": "hbm", "kernel": kernel, "achieved": None if ach is None else round(ach, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": None if ach is None else round(ach / HBM_PEAK_GBS, 4),
+            "traffic": None if prof is None else round(prof["bytes"]), "avg_launch_ms": round(avg_launch_ms, 3),
+            "traffic_fetch": None if prof is None else round(prof["fetch"]),
+            "traffic_write": None if prof is None else round(prof["write"]),
+            "l2_hit_rate": None if prof is None else prof["l2_hit_rate"],
+            "canonical": {"bytes_per_launch": round(canonical_bytes), "gbs": round(canon_gbs, 1),
+                          "exceeds_hbm_peak": canon_gbs > HBM_PEAK_GBS, **extra}}
 
 
 # occlusion-grid direct pass: 8 B cell range per lookup ("nodes" counts lookups), 4 B list entry + 48 B
@@ -102,55 +129,98 @@ def bytes_per_ray(c, grid=False):
     return B_NODE * c["nodes"] / c["rays"] + B_PRIM * c["prims"] / c["rays"] + B_IO
 
 
-def cpu_baseline_diurnal(s, base, dirs, seconds):
-    """Time the CPU oracle (the restatement of the reference CPU path) on bounded samples of
-    primitives for one direct and one diffuse pass; extrapolate to plain solves/s by ray count."""
-    import ctypes as C
+FRACS = (0.1, 0.3, 0.5, 0.7, 0.9)  # starts of the sampled primitive ranges, as fractions of P
 
-    from oracle import oracle as O
-    o = O.OracleScene.from_arrays(s)
-    L = O.lib()
-    u32p, f64p, f32p = C.POINTER(C.c_uint32), C.POINTER(C.c_double), C.POINTER(C.c_float)
-    L.orc_diffuse_hits_range.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, u32p]
-    L.orc_direct_sums_range.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, f32p, C.c_uint32, C.c_uint32,
-                                        f64p]
+
+def oracle_samples(o, s, base, sun_dir, direct_seed, width):
+    """The CPU oracle (the restatement of the reference CPU path) on primitive ranges [f P, f P + width) for the
+    fractions FRACS: one direct pass towards sun_dir with direct_seed and one diffuse pass (both faces) with
+    the request's primary diffuse seed -- the bench's own seeds, so the results double as a parity sample.
+    Returns the per-range outputs and the oracle's time and rays for each pass."""
     rp = W.rays_per_solve(s, base)
     d, n = rp["d"], rp["n"]
-    sd = np.ascontiguousarray(dirs[0], np.float32)
+    sd = np.ascontiguousarray(sun_dir, np.float32).reshape(3)
     P = s.P
-    fracs = (0.1, 0.3, 0.5, 0.7, 0.9)
+    out = {"ranges": [], "hits": [], "sums": [], "t_dif": 0.0, "r_dif": 0, "t_dir": 0.0, "r_dir": 0}
+    for f in FRACS:
+        p0 = int(f * P)
+        p1 = min(P, p0 + width)
+        t = time.perf_counter()
+        hits = o.diffuse_hits_range(n, int(base.dseeds[0]), p0, p1)
+        out["t_dif"] += time.perf_counter() - t
+        out["r_dif"] += int(np.sum(hits != 0xFFFFFFFE))
+        t = time.perf_counter()
+        sums = o.direct_sums_range(d, int(direct_seed), sd, p0, p1)
+        out["t_dir"] += time.perf_counter() - t
+        out["r_dir"] += (p1 - p0) * d * d
+        out["ranges"].append((p0, p1))
+        out["hits"].append(hits)
+        out["sums"].append(sums)
+    return out
 
-    def run(width):
-        tdif = tdir = 0.0
-        rdif = rdir = 0
-        for f in fracs:
-            p0 = int(f * P)
-            p1 = min(P, p0 + width)
-            out = np.zeros((p1 - p0, 2, n * n), np.uint32)
-            t = time.perf_counter()
-            L.orc_diffuse_hits_range(o.h, n, 77, p0, p1, out.ctypes.data_as(u32p))
-            tdif += time.perf_counter() - t
-            rdif += int(np.sum(out != 0xFFFFFFFE))
-            outd = np.zeros((p1 - p0, 1, 2))
-            t = time.perf_counter()
-            L.orc_direct_sums_range(o.h, d, 5, 1, sd.ctypes.data_as(f32p), p0, p1, outd.ctypes.data_as(f64p))
-            tdir += time.perf_counter() - t
-            rdir += (p1 - p0) * d * d
-        return tdif, rdif, tdir, rdir
 
-    tdif, rdif, tdir, rdir = run(200)
-    per = (tdif + tdir) / 200.0
-    width = max(200, int(seconds / max(per, 1e-9)))
-    tdif, rdif, tdir, rdir = run(width)
-    dif_rate, dir_rate = rdif / tdif, rdir / tdir
+def cpu_baseline_diurnal(o, s, base, sun_dir, direct_seed, seconds):
+    """Time the oracle on bounded samples of primitives (about `seconds` of CPU work); extrapolate to plain
+    solves/s by ray count.  Returns the baseline and the samples (reused as the parity sample)."""
+    rp = W.rays_per_solve(s, base)
+    d, n = rp["d"], rp["n"]
+    probe = oracle_samples(o, s, base, sun_dir, direct_seed, 200)
+    per = (probe["t_dif"] + probe["t_dir"]) / 200.0
+    width = max(200, int(seconds / max(per, 1e-9) / len(FRACS)))
+    smp = oracle_samples(o, s, base, sun_dir, direct_seed, width)
+    dif_rate, dir_rate = smp["r_dif"] / smp["t_dif"], smp["r_dir"] / smp["t_dir"]
     t_solve = rp["direct"] / dir_rate + rp["diffuse"] / dif_rate
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    nprim = sum(p1 - p0 for p0, p1 in smp["ranges"])
     return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": cores, "kind": "port",
-            "sample": (f"CPU oracle (FP32 rays, FP64 sums, OpenMP) on {5 * width} primitives x (one direct pass d={d} + "
-                       f"one diffuse pass n={n}, both faces) of the config-3 orchard: {rdir + rdif} rays in "
-                       f"{tdif + tdir:.1f} s ({dir_rate / 1e6:.2f} Mrays/s direct, {dif_rate / 1e6:.2f} Mrays/s diffuse); "
-                       f"plain solve = {rp['total']} rays -> {t_solve:.0f} s"),
-            "mrays_per_s": (rdir + rdif) / (tdif + tdir) / 1e6}
+            "sample": (f"CPU oracle (FP32 rays, FP64 sums, OpenMP) on {nprim} primitives x (one direct pass d={d} + "
+                       f"one diffuse pass n={n}, both faces) of the config-3 orchard: {smp['r_dir'] + smp['r_dif']} rays "
+                       f"in {smp['t_dif'] + smp['t_dir']:.1f} s ({dir_rate / 1e6:.2f} Mrays/s direct, "
+                       f"{dif_rate / 1e6:.2f} Mrays/s diffuse); plain solve = {rp['total']} rays -> {t_solve:.0f} s"),
+            "mrays_per_s": (smp["r_dir"] + smp["r_dif"]) / (smp["t_dif"] + smp["t_dir"]) / 1e6}, smp
+
+
+def parity_diurnal(g, base, sun_dir, direct_seed, smp):
+    """The GPU's diffuse hit words and direct strength sums of the full scene, at the bench's seeds, against the
+    oracle's on the sampled primitive ranges: hit words must be equal, sums within 1e-5 (they are fractions of
+    the source flux in [0, 1])."""
+    rp_d = int(round(math.sqrt(smp["hits"][0].shape[2])))
+    hits = g.diffuse_hits(rp_d, int(base.dseeds[0]))
+    d = int(math.ceil(math.sqrt(base.direct_rays)))
+    sums = g.direct_sums(d, int(direct_seed), [sun_dir])
+    mism, rays, worst, nprim = 0, 0, 0.0, 0
+    for (p0, p1), ho, so in zip(smp["ranges"], smp["hits"], smp["sums"]):
+        hg = hits[p0:p1]
+        launched = ho != 0xFFFFFFFE
+        mism += int(np.sum(hg != ho))
+        rays += int(np.sum(launched))
+        worst = max(worst, float(np.max(np.abs(sums[p0:p1].astype(np.float64) - so))))
+        nprim += p1 - p0
+    return {"sampled_prims": nprim, "sampled_diffuse_rays": rays, "hit_word_mismatches": mism,
+            "direct_sampled_rays": nprim * d * d, "direct_sum_max_abs_err": worst,
+            "seeds": {"diffuse": int(base.dseeds[0]), "direct": int(direct_seed)}}
+
+
+def energy_diurnal(ctx, g, base, dirs, flux, sky, seeds, out_ptr):
+    """FP64 energy bookkeeping of the series under both pass schedules (one untimed series each): the largest
+    |residual| / inputs over the channels, and the channel sums."""
+    K, B = dirs.shape[0], base.B
+    res = {}
+    native.set_energy_accounting(ctx, True)
+    try:
+        for name, sched in (("reference", 0), ("conserving", 1)):
+            base.schedule = sched
+            native.run_band_batch(g, base, dirs, flux, sky, seeds, out_device_ptr=out_ptr)
+            e = native.energy_fp64(ctx, K * B)
+            inputs = e["source"] + e["sky"] + e["emitted"]
+            rel = np.abs(e["residual"]) / np.maximum(inputs, 1e-30)
+            res[name] = {"max_rel_residual": float(rel.max()), "mean_rel_residual": float(rel.mean()),
+                         "inputs_W": float(inputs.sum()), "absorbed_W": float(e["absorbed"].sum()),
+                         "escaped_W": float(e["escaped"].sum()), "residual_W": float(e["residual"].sum())}
+    finally:
+        base.schedule = 0
+        native.set_energy_accounting(ctx, False)
+    return res
 
 
 def calib_copy(mib):
@@ -231,17 +301,26 @@ def bench_diurnal(args):
     ms_per_step = wall * 1e3 / args.steps
     solves_s = K / (ms_per_step / 1e3)
 
-    # plain backend: one solve per launch sequence, a few suns of the same series
+    # plain backend: one solve per launch sequence (rt_run_band's path: a batch of one sun), a few suns of the
+    # same series, output left in HBM like the batched leg's
     plain_lat = []
     idx = np.linspace(0, K - 1, args.plain_suns).round().astype(int)
-    native.run_band(g, W.single_sun_request(base, dirs[idx[0]], flux[idx[0]], sky[idx[0]], seeds[idx[0]]))
+    native.run_band_batch(g, base, dirs[idx[0]:idx[0] + 1], flux[idx[0]:idx[0] + 1], sky[idx[0]:idx[0] + 1],
+                          seeds[idx[0]:idx[0] + 1], out_device_ptr=out_dev.value)
     for k in idx:
-        rq = W.single_sun_request(base, dirs[k], flux[k], sky[k], seeds[k])
         t = time.perf_counter()
-        native.run_band(g, rq)
+        native.run_band_batch(g, base, dirs[k:k + 1], flux[k:k + 1], sky[k:k + 1], seeds[k:k + 1],
+                              out_device_ptr=out_dev.value)
         plain_lat.append((time.perf_counter() - t) * 1e3)
         log(f"[bench] plain sun {k}: {plain_lat[-1]:.1f} ms")
     plain_solves_s = 1e3 / float(np.mean(plain_lat))
+    # with the results downloaded to host memory, as runBand hands them to the Context (PCIe-inclusive)
+    t = time.perf_counter()
+    native.run_band_batch(g, base, dirs, flux, sky, seeds)
+    series_host_ms = (time.perf_counter() - t) * 1e3
+    t = time.perf_counter()
+    native.run_band(g, W.single_sun_request(base, dirs[idx[0]], flux[idx[0]], sky[idx[0]], seeds[idx[0]]))
+    plain_host_ms = (time.perf_counter() - t) * 1e3
 
     # roofline of the dominant kernel class (HIP events on the solve stream)
     per_step = {c: kt[f"{c}_ms"] / args.steps for c in ("direct", "diffuse", "gather", "other")}
@@ -250,9 +329,15 @@ def bench_diurnal(args):
     avg_launch_ms = per_step[dom] / max(launches, 1)
     brays = bytes_per_ray(counts[dom], grid=(dom == "direct" and grid))
     rays_per_launch = counts[dom]["rays"] / max(launches, 1)
-    achieved = brays * rays_per_launch / (avg_launch_ms / 1e3) / 1e9
     traced_rays = counts["direct"]["rays"] + counts["diffuse"]["rays"]
-    traffic = pmc_traffic(f"k_{dom}")
+    # the canonical bytes of the whole step against its wall time: above the HBM peak, they cannot all be HBM bytes
+    canon_step = sum(bytes_per_ray(counts[c], grid=(c == "direct" and grid)) * counts[c]["rays"]
+                     for c in ("direct", "diffuse"))
+    rl = roofline(f"k_{dom}", avg_launch_ms, brays * rays_per_launch, "pmc_traffic_config3_diurnal.json",
+                  bytes_per_ray=round(brays, 1), rays_per_launch=int(rays_per_launch),
+                  nodes_per_ray=round(counts[dom]["nodes"] / counts[dom]["rays"], 2),
+                  prims_per_ray=round(counts[dom]["prims"] / counts[dom]["rays"], 2),
+                  step_gbs=round(canon_step / (ms_per_step / 1e3) / 1e9, 1))
     res = {
         "metric": METRIC, "value": round(solves_s, 4), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 2), "higher_is_better": True, "scaling": "weak",
@@ -261,13 +346,14 @@ def bench_diurnal(args):
                                 f"ground patches), {K} suns x {B} bands (LW emission, NIR, PAR), d=n=15, depth 4, "
                                 f"batched (one series per step)"),
                    "solves_per_step": K, "prims": P, "bands": B, "suns": K},
-        "mrays_per_s": round(rp["total"] * solves_s / 1e6, 1),
         "mrays_per_s_traced": round(traced_rays / (ms_per_step / 1e3) / 1e6, 1),
+        "mrays_per_s_reference_equivalent": round(rp["total"] * solves_s / 1e6, 1),
         "plain_solves_per_s": round(plain_solves_s, 4),
         "plain_mrays_per_s": round(rp["total"] * plain_solves_s / 1e6, 1),
         "speedup_batched_vs_plain": round(solves_s / plain_solves_s, 2),
         "latency_ms": {"series_p50": pct(lat, 50), "series_p99": pct(lat, 99),
                        "plain_solve_p50": pct(plain_lat, 50), "plain_solve_p99": pct(plain_lat, 99)},
+        "with_host_download_ms": {"series": round(series_host_ms, 1), "plain_solve": round(plain_host_ms, 1)},
         "kernel_ms_per_step": {("grid_build" if k == "other" else k): round(v, 2) for k, v in per_step.items()},
         "direct_occlusion": "grid" if grid else "bvh4-packet",
         "grid_entries_per_prim": round(grid_entries, 2) if grid else None,
@@ -276,19 +362,21 @@ def bench_diurnal(args):
         "rays_per_solve": rp["total"],
         "scene_build_ms": round(build_s * 1e3, 1),
         "autotune": tune,
-        "roofline": {"bound": "hbm", "kernel": f"k_{dom}", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "algorithmic_bytes_per_launch": round(brays * rays_per_launch),
-                     "bytes_per_ray": round(brays, 1),
-                     "nodes_per_ray": round(counts[dom]["nodes"] / counts[dom]["rays"], 2),
-                     "prims_per_ray": round(counts[dom]["prims"] / counts[dom]["rays"], 2),
-                     "avg_launch_ms": round(avg_launch_ms, 3), "rays_per_launch": int(rays_per_launch),
-                     "traffic_gbs": None if traffic is None else round(traffic / (avg_launch_ms / 1e3) / 1e9, 1)},
+        "roofline": rl,
     }
+    if not args.no_energy:
+        log("[bench] energy bookkeeping (both schedules) ...")
+        res["energy_fp64"] = energy_diurnal(ctx, g, base, dirs, flux, sky, seeds, out_dev.value)
     hip.hipFree(out_dev)
+    from oracle.oracle import OracleScene
+    o = OracleScene.from_arrays(s)
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle) ...")
-        res["cpu_baseline"] = cpu_baseline_diurnal(s, base, dirs, args.cpu_seconds)
+        res["cpu_baseline"], smp = cpu_baseline_diurnal(o, s, base, dirs[0][0], seeds[0], args.cpu_seconds)
+    else:
+        smp = oracle_samples(o, s, base, dirs[0][0], seeds[0], 400)
+    log("[bench] parity sample ...")
+    res["parity"] = parity_diurnal(g, base, dirs[0][0], seeds[0], smp)
     return res
 
 
@@ -322,16 +410,13 @@ def bench_config2(args):
     per = kt["diffuse_ms"] / args.steps / max(kt["diffuse_launches"] / args.steps, 1)
     brays = bytes_per_ray(counts[dom])
     rpl = counts[dom]["rays"] / max(kt["diffuse_launches"] / args.steps, 1)
-    ach = brays * rpl / (per / 1e3) / 1e9
     return {"metric": METRIC, "value": round(1e3 / ms, 4), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": round(ms, 2), "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": "config2: 200k leaf triangles + 400 ground patches, periodic, NIR+PAR, d=n=32, depth 5"},
             "mrays_per_s": round(rp["total"] / (ms / 1e3) / 1e6, 1),
             "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99)},
-            "roofline": {"bound": "hbm", "kernel": "k_diffuse", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                         "bytes_per_ray": round(brays, 1)}}
+            "roofline": roofline("k_diffuse", per, brays * rpl, "pmc_traffic_config2.json", bytes_per_ray=round(brays, 1))}
 
 
 # published runBand wall times of the homogeneous-canopy benchmark (seconds: direct, diffuse)
@@ -396,7 +481,6 @@ def bench_homogeneous(args):
     direct_rays = P * 1024
     diffuse_rays = (2 * nl + (P - nl)) * 1024
     brays = bytes_per_ray(counts["diffuse"])
-    ach = brays * counts["diffuse"]["rays"] / (kms / args.steps / 1e3) / 1e9
     ref = PUBLISHED_HOMOGENEOUS["NVIDIA RTX 6000 Ada"]
     return {"metric": "runBand seconds (direct + diffuse), homogeneous canopy benchmark", "value": round(d_s + f_s, 4),
             "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 2),
@@ -410,10 +494,8 @@ def bench_homogeneous(args):
                        "diffuse_grays_per_s": round(diffuse_rays / f_s / 1e9, 3),
                        "published_s": {k: {"direct": v[0], "diffuse": v[1]} for k, v in PUBLISHED_HOMOGENEOUS.items()},
                        "absorbed_mean_direct": round(float(Rd.mean()), 5), "absorbed_mean_diffuse": round(float(Rf.mean()), 5)},
-            "roofline": {"bound": "hbm", "kernel": "k_diffuse", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                         "traffic": pmc_traffic("k_diffuse", profile="pmc_traffic_homogeneous_canopy.json"),
-                         "bytes_per_ray": round(brays, 1), "avg_launch_ms": round(kms / args.steps, 3)}}
+            "roofline": roofline("k_diffuse", kms / args.steps, brays * counts["diffuse"]["rays"],
+                                 "pmc_traffic_homogeneous_canopy.json", bytes_per_ray=round(brays, 1))}
 
 
 def bench_castrays(args):
@@ -453,19 +535,15 @@ def bench_castrays(args):
     device_sync()
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
     brays = bytes_per_ray(counts)
-    ach = brays * n / (kms / args.steps / 1e3) / 1e9
     return {"metric": "castRays Mrays/s", "value": round(n / (ms / 1e3) / 1e6, 1), "unit": "Mrays/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 2), "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": f"config5 castRaysSoA: {n} rays (uniform in scene AABB, uniform directions) over "
                                    f"the 5x tiled orchard ({s.P} prims)"},
-            "roofline": {"bound": "hbm", "kernel": "k_cast", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                         "traffic": pmc_traffic("k_cast", profile="pmc_traffic_config5_castrays.json"),
-                         "bytes_per_ray": round(brays, 1),
-                         "nodes_per_ray": round(counts["nodes"] / counts["rays"], 2),
-                         "prims_per_ray": round(counts["prims"] / counts["rays"], 2),
-                         "avg_launch_ms": round(kms / args.steps, 3)},
+            "roofline": roofline("k_cast", kms / args.steps, brays * n, "pmc_traffic_config5_castrays.json",
+                                 bytes_per_ray=round(brays, 1), nodes_per_ray=round(counts["nodes"] / counts["rays"], 2),
+                                 prims_per_ray=round(counts["prims"] / counts["rays"], 2),
+                                 result_bytes_per_launch=20 * n),
             **({} if args.no_cpu_baseline else {"cpu_baseline": cpu_baseline_cast(s, O, D, args.cpu_seconds)})}
 
 
@@ -542,19 +620,15 @@ def bench_camera(args):
     R = be.get_camera_results(cam.resolution)
     hit = float((R["pixel_labels"] > 0).mean())
     brays = bytes_per_ray(counts)
-    ach = brays * nrays / (kms / args.steps / 1e3) / 1e9
     res = {"metric": "camera Mrays/s", "value": round(nrays / (ms / 1e3) / 1e6, 1), "unit": "Mrays/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 2), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": f"config5 camera: {args.cam_res}x{args.cam_res} pixels x {args.cam_spp} samples, "
                                   f"{B} bands, over the 5x tiled orchard ({s.P} prims); one step = one image",
                       "pixels_hit": round(hit, 3), "pixel_label_ms": round(lab_ms, 2)},
-           "roofline": {"bound": "hbm", "kernel": "k_camera", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                        "traffic": pmc_traffic("k_camera", profile="pmc_traffic_config5_camera.json"),
-                        "bytes_per_ray": round(brays, 1), "nodes_per_ray": round(counts["nodes"] / counts["rays"], 2),
-                        "prims_per_ray": round(counts["prims"] / counts["rays"], 2),
-                        "avg_launch_ms": round(kms / args.steps, 3)}}
+           "roofline": roofline("k_camera", kms / args.steps, brays * nrays, "pmc_traffic_config5_camera.json",
+                                bytes_per_ray=round(brays, 1), nodes_per_ray=round(counts["nodes"] / counts["rays"], 2),
+                                prims_per_ray=round(counts["prims"] / counts["rays"], 2))}
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline_camera(s, cam, args.cpu_seconds)
     return res
@@ -677,6 +751,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the config-4 serving leg of the default run")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--no-energy", action="store_true", help="skip the energy-bookkeeping series")
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--serve-scenes", type=int, default=6)
     ap.add_argument("--serve-slots", type=int, default=4)

@@ -74,6 +74,7 @@ def summarize(kt_db, pmc_dbs, calib_bytes=1 << 30, brief=False):
                 traffic[sn][c].append(val)
                 if not brief:
                     out.append(f"dispatch {d:6d} {sn:60s} {c:12s} {val:16.1f}")
+    l2_rate = {}
     if l2:
         out.append("\n== L2 (TCC) hit rate of each ray kernel's largest dispatch")
         for k, ds in l2.items():
@@ -81,6 +82,7 @@ def summarize(kt_db, pmc_dbs, calib_bytes=1 << 30, brief=False):
             hit = sum(v for c, v in cs.items() if c.startswith("TCC_HIT"))
             miss = sum(v for c, v in cs.items() if c.startswith("TCC_MISS"))
             if hit + miss > 0:
+                l2_rate[k] = hit / (hit + miss)
                 out.append(f"{k}: dispatch {d} hits {hit:.3e} misses {miss:.3e} hit rate {hit / (hit + miss):.3f}")
     summary = {}
     if scale:
@@ -94,6 +96,8 @@ def summarize(kt_db, pmc_dbs, calib_bytes=1 << 30, brief=False):
             out.append(f"{k}: " + "; ".join(f"{c} max {max(v) / 1e9:.2f} GB min {min(v) / 1e9:.2f} GB"
                                             for c, v in tot.items()))
             summary[k] = {c: {"max_bytes": max(v), "min_bytes": min(v)} for c, v in tot.items()}
+            if k in l2_rate:
+                summary[k]["l2_hit_rate"] = round(l2_rate[k], 4)
     return "\n".join(out), summary, scale
 
 

@@ -75,6 +75,9 @@ def lib():
                                       C.c_uint32, f32p, f32p]
         L.orc_diffuse_hits.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, u32p]
         L.orc_direct_sums.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, f32p, f64p]
+        L.orc_diffuse_hits_range.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, u32p]
+        L.orc_direct_sums_range.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, f32p, C.c_uint32,
+                                            C.c_uint32, f64p]
         L.orc_cast_rays.argtypes = [C.c_void_p, C.c_uint64, f32p, f32p, C.c_float, f32p, f32p, u32p, C.c_int]
         L.orc_run_band.restype = C.c_int
         L.orc_run_band.argtypes = [C.c_void_p, C.POINTER(orc_band_params), f64p, f64p]
@@ -189,6 +192,20 @@ class OracleScene:
     def diffuse_hits(self, n, seed):
         out = np.zeros((self.P, 2, n * n), np.uint32)
         lib().orc_diffuse_hits(self.h, n, seed, out.ctypes.data_as(u32p))
+        return out
+
+    def diffuse_hits_range(self, n, seed, p0, p1):
+        """diffuse_hits() of primitives [p0, p1) only: [p1 - p0][2][n*n]."""
+        out = np.zeros((p1 - p0, 2, n * n), np.uint32)
+        lib().orc_diffuse_hits_range(self.h, n, seed, p0, p1, out.ctypes.data_as(u32p))
+        return out
+
+    def direct_sums_range(self, d, seed, src_dirs, p0, p1):
+        """direct_sums() of primitives [p0, p1) only: [p1 - p0][S][2]."""
+        S = _f(src_dirs).reshape(-1, 3)
+        out = np.zeros((p1 - p0, S.shape[0], 2), np.float64)
+        lib().orc_direct_sums_range(self.h, d, seed, S.shape[0], S.ctypes.data_as(f32p), p0, p1,
+                                    out.ctypes.data_as(f64p))
         return out
 
     def direct_sums(self, d, seed, src_dirs):
