@@ -321,6 +321,58 @@ uint32_t sg_random_canopy(uint32_t seed, uint32_t P, float *T, uint32_t *type, u
     return o.n;
 }
 
+/* Disk of radius rad as the Context builds one (core/src/Context.cpp:2503-2537, one ring): ndiv triangles
+ * (centre, p_t, p_t+1) with p_t = rad (cos, sin)(2 pi t / ndiv) in the disk's plane, oriented by a
+ * spherical leaf rotation and translated to the centre. */
+static void disk_fan(sg_out *o, sg_rng *r, float cx, float cy, float cz, float rad, uint32_t ndiv, uint32_t twos) {
+    float R[9];
+    spherical_leaf(r, R);
+    float c[3] = {cx, cy, cz};
+    for (uint32_t t = 0; t < ndiv; t++) {
+        float s0, c0, s1, c1;
+        sc2pi((float)t / (float)ndiv, &s0, &c0);
+        sc2pi((float)(t + 1) / (float)ndiv, &s1, &c1);
+        float l[2][2] = {{rad * c0, rad * s0}, {rad * c1, rad * s1}};
+        float v[2][3];
+        for (int k = 0; k < 2; k++) {
+            v[k][0] = cx + (R[0] * l[k][0] + R[1] * l[k][1]);
+            v[k][1] = cy + (R[3] * l[k][0] + R[4] * l[k][1]);
+            v[k][2] = cz + (R[6] * l[k][0] + R[7] * l[k][1]);
+        }
+        put_tri(o, c, v[0], v[1], twos);
+    }
+}
+
+/* Mixed canopy of about P primitives (config 4): leaves of area 1e-3 m^2 cycling through square patches,
+ * isosceles triangles and 6-triangle disks, centres uniform over a square of LAI ~2 between 0.05 and 2.05 m,
+ * spherical leaf angles, over a 10 x 10 ground of one-sided patches. */
+uint32_t sg_mixed_canopy(uint32_t seed, uint32_t P, float *T, uint32_t *type, uint32_t *twos) {
+    sg_out o = {T, type, twos, 0};
+    sg_rng r = {0xD15Cull ^ ((uint64_t)seed << 8)};
+    const uint32_t target = P > 100 ? P - 100 : P;
+    float A = 1e-3f;
+    /* three leaves (patch, triangle, disk) take 1 + 1 + 6 primitives */
+    uint32_t nleaf = 3 * (target / 8);
+    if (nleaf == 0) nleaf = 1;
+    float D = sqrtf((float)nleaf * A / 2.0f);
+    float rad = sqrtf(A / 3.14159265f);
+    for (uint32_t i = 0; i < nleaf; i++) {
+        float cx = (-0.5f + uni(&r)) * D, cy = (-0.5f + uni(&r)) * D, cz = 0.05f + 2.0f * uni(&r);
+        if (i % 3 == 0) {
+            float R[9];
+            spherical_leaf(&r, R);
+            float sd = sqrtf(A);
+            put_patch(&o, cx, cy, cz, sd, sd, R, 1);
+        } else if (i % 3 == 1) {
+            leaf_tri(&o, &r, cx, cy, cz, A, 1);
+        } else {
+            disk_fan(&o, &r, cx, cy, cz, rad, 6, 1);
+        }
+    }
+    if (P > 100) ground(&o, -0.5f * D, 0.5f * D, -0.5f * D, 0.5f * D, 10, 10);
+    return o.n;
+}
+
 /* Random rays: origins uniform in the box [lo, hi], directions uniform on the sphere
  * (z = 1 - 2u, phi = 2*pi*u'). */
 void sg_rays(uint32_t seed, uint64_t n, const float lo[3], const float hi[3], float *O, float *D) {

@@ -25,9 +25,10 @@ def _load():
         _lib.sg_config2.argtypes = [u32, u32, f32p, u32p, u32p]
         _lib.sg_orchard.argtypes = [u32, u32, u32, u32, u32, u32, C.c_float, f32p, u32p, u32p]
         _lib.sg_random_canopy.argtypes = [u32, u32, f32p, u32p, u32p]
+        _lib.sg_mixed_canopy.argtypes = [u32, u32, f32p, u32p, u32p]
         _lib.sg_rays.argtypes = [u32, C.c_uint64, f32p, f32p, f32p, f32p]
         _lib.sg_sincos_2pi.argtypes = [C.c_float, f32p, f32p]
-        for n in ("sg_config1", "sg_patch_canopy", "sg_config2", "sg_orchard", "sg_random_canopy"):
+        for n in ("sg_config1", "sg_patch_canopy", "sg_config2", "sg_orchard", "sg_random_canopy", "sg_mixed_canopy"):
             getattr(_lib, n).restype = u32
         _lib.sg_rays.restype = None
     return _lib
@@ -95,6 +96,12 @@ def orchard(seed: int = 3, nx: int = 5, ny: int = 5, leaves_per_tree: int = 38_0
 def random_canopy(seed: int, P: int) -> SceneArrays:
     """Mixed patch/triangle canopy of about P primitives (serving requests)."""
     return _gen(_load().sg_random_canopy, seed, P)
+
+
+def mixed_canopy(seed: int, P: int) -> SceneArrays:
+    """Canopy of about P primitives whose leaves cycle through square patches, triangles and 6-triangle disks
+    (Helios disks are triangle fans), over a one-sided ground (config 4's mixed scenes)."""
+    return _gen(_load().sg_mixed_canopy, seed, P)
 
 
 def rays(seed: int, n: int, lo, hi):

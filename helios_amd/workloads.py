@@ -112,14 +112,15 @@ def rays_per_solve(s, req) -> dict:
 
 def serve_models(seed: int = 4, nscenes: int = 6, pmin: float = 1e4, pmax: float = 2e6, rays: int = 100,
                  depth: int = 2):
-    """Config 4 models: seeded random canopies (mixed patches/triangles) with P log-uniform in
-    [pmin, pmax], each with 1-4 bands of its own optical properties and sky flux, one collimated
-    source, d = n = ceil(sqrt(rays)), scattering depth `depth`.  Returns [(SceneArrays, BandRequest)]."""
+    """Config 4 models: seeded mixed canopies (square patches, triangles and triangle-fan disks) with P
+    log-uniform in [pmin, pmax], each with 1-4 bands of its own optical properties and sky flux, one
+    collimated source, d = n = ceil(sqrt(rays)), scattering depth `depth`.  Returns
+    [(SceneArrays, BandRequest)]."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(nscenes):
         P = int(round(10 ** rng.uniform(math.log10(pmin), math.log10(pmax))))
-        s = scenes.random_canopy(100 + i, P)
+        s = scenes.mixed_canopy(100 + i, P)
         nb = int(rng.integers(1, 5))
         leaf = s.twosided == 1
         rho = np.zeros((s.P, nb), np.float32)

@@ -36,8 +36,30 @@ def test_config2_leaf_areas_and_lai():
 
 def test_deterministic_and_golden():
     got = {"config1": digest(scenes.config1(1)), "config2_small": digest(scenes.config2(2, 5000)),
-           "orchard_small": digest(scenes.orchard(3, 2, 1, 500, 320))}
+           "orchard_small": digest(scenes.orchard(3, 2, 1, 500, 320)),
+           "mixed_small": digest(scenes.mixed_canopy(5, 4000))}
     assert got["config1"] == digest(scenes.config1(1))
     with open(GOLD) as f:
         want = json.load(f)
     assert got == want
+
+
+def test_mixed_canopy_disks_are_triangle_fans():
+    # every third leaf is a 6-triangle disk built as Context::addDisk builds one ring (core/src/Context.cpp:
+    # 2503-2537): triangles (centre, p_t, p_t+1) with the p_t on a circle around the shared centre
+    s = scenes.mixed_canopy(5, 4000)
+    T = s.transforms.reshape(-1, 16).astype(np.float64)
+    # per leaf triple: 1 patch, 1 triangle, 6 disk triangles
+    assert np.array_equal(s.types[:8], [0, 1, 1, 1, 1, 1, 1, 1])
+    v0 = T[2:8][:, [3, 7, 11]]                            # triangle vertex 0 = the disk centre
+    v1 = v0 + T[2:8][:, [1, 5, 9]]                        # vertex 1 = v0 + column 1
+    v2 = v1 + T[2:8][:, [0, 4, 8]]                        # vertex 2 = v1 + column 0
+    assert np.allclose(v0, v0[0], atol=0)
+    r1, r2 = np.linalg.norm(v1 - v0, axis=1), np.linalg.norm(v2 - v0, axis=1)
+    rad = np.sqrt(1e-3 / np.pi)
+    assert np.allclose(r1, rad, rtol=1e-5) and np.allclose(r2, rad, rtol=1e-5)
+    assert np.allclose(v2[:-1], v1[1:], atol=1e-7) and np.allclose(v2[-1], v1[0], atol=1e-7)  # closed fan
+    n = np.cross(v1 - v0, v2 - v0)
+    assert np.allclose(n / np.linalg.norm(n, axis=1, keepdims=True), (n / np.linalg.norm(n, axis=1, keepdims=True))[0],
+                       atol=1e-5)  # coplanar, one orientation
+    assert np.all(s.twosided[:-100] == 1) and np.all(s.twosided[-100:] == 0)
