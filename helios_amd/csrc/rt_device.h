@@ -511,6 +511,42 @@ RT_HD v3f frame_rotate(v3f dl, float ce, float se, float ca, float sa) {
 }
 
 // Neumaier-compensated FP32 accumulator
+// Angular transmittance, reflectance and absorptance of a glass sheet (R/shaders/common/glass_cover.glsl:15-53):
+// Snell refraction, Fresnel reflectance of each polarisation (sin^2(tr - t) / sin^2(tr + t) and
+// tan^2(tr - t) / tan^2(tr + t), the normal-incidence limit below 1e-3 rad), Bouguer attenuation
+// exp(-KL / cos tr) and the closed-form sums over the slab's internal reflections, averaged over the two
+// polarisations.  tra = (tau, rho, alpha), tau + rho + alpha = 1.
+RT_HD void glass_tra(float cos_theta, float n, float KL, float &tau, float &rho, float &alpha) {
+    cos_theta = fmaxf(1e-4f, fminf(1.0f, cos_theta));
+    const float theta = acosf(fmaxf(-1.0f, fminf(1.0f, cos_theta)));
+    const float sin_tr = sinf(theta) / n;
+    const float cos_tr = sqrtf(fmaxf(0.0f, 1.0f - sin_tr * sin_tr));
+    const float theta_r = asinf(fmaxf(-1.0f, fminf(1.0f, sin_tr)));
+    float r_par, r_per;
+    if (theta < 1e-3f) {
+        const float r0 = ((n - 1.0f) / (n + 1.0f)) * ((n - 1.0f) / (n + 1.0f));
+        r_par = r0;
+        r_per = r0;
+    } else {
+        const float sm = sinf(theta_r - theta), sp = sinf(theta_r + theta);
+        const float tm = tanf(theta_r - theta), tp = tanf(theta_r + theta);
+        r_per = (sm * sm) / fmaxf(1e-12f, sp * sp);
+        r_par = (tm * tm) / fmaxf(1e-12f, tp * tp);
+    }
+    const float tau_a = KL > 0.0f ? expf(-KL / fmaxf(1e-4f, cos_tr)) : 1.0f;
+    float t = 0.0f, r_sum = 0.0f;
+    for (int pol = 0; pol < 2; pol++) {
+        const float r = pol == 0 ? r_per : r_par;
+        const float den = fmaxf(1e-6f, 1.0f - (r * tau_a) * (r * tau_a));
+        const float ti = tau_a * (1.0f - r) * (1.0f - r) / den;
+        t += 0.5f * ti;
+        r_sum += 0.5f * (r * (1.0f + tau_a * ti));
+    }
+    tau = fmaxf(0.0f, fminf(1.0f, t));
+    rho = fmaxf(0.0f, fminf(1.0f, r_sum));
+    alpha = fmaxf(0.0f, 1.0f - tau - rho);
+}
+
 struct ksum {
     float s, c;
     RT_HD void init() { s = 0.f; c = 0.f; }

@@ -80,6 +80,14 @@ int launch_diffuse(const DiffuseJob *job_dev, uint32_t total_items, uint32_t dep
 // acct: the jobs carry energy-bookkeeping outputs (GatherJob.acct / .miss)
 int launch_gather(const GatherJob *jobs_dev, uint32_t njobs, uint32_t total_items, uint32_t cmax, uint32_t *counter,
                   const LaunchCfg &cfg, int ncu, hipStream_t st, bool acct = false);
+// Solves with translucent covers (GlassCtx g, device): the direct jobs (items ascending by dj.item_begin),
+// one diffuse/scattering pass traced and gathered together, and the covers' pushed shares added to the
+// radiation buffers (PC = P * C).
+int launch_direct_glass(const GlassDirectJob *jobs_dev, uint32_t njobs, uint32_t total_items, uint32_t depth,
+                        uint32_t *counter, const GlassCtx *g, int *err, int ncu, hipStream_t st, bool masked);
+int launch_diffuse_glass(const DiffuseJob *fj_dev, const GatherJob *gj_dev, uint32_t total_items, uint32_t depth,
+                         uint32_t *counter, const GlassCtx *g, int *err, int ncu, hipStream_t st, bool masked);
+int launch_add_push(size_t PC, double *push, float *in, float *st, float *sb, hipStream_t s);
 // Camera pixels (labels = false) or pixel labels/depths (labels = true) of one tile.
 int launch_camera(const CameraLaunch &c, uint32_t depth, bool labels, int *err, unsigned long long *stats,
                   hipStream_t st, bool masked);

@@ -21,10 +21,13 @@
 
 namespace rt {
 
+// radiation_in / scatter_top / scatter_bottom += source flux x strength sums x optical properties; dsum
+// [K][S][P][2] (strength per lit face), or with covers gsum [K][S][P][B][2] (strength x the covers'
+// transmittance per band) when gsum is given
 __global__ void k_deposit_direct(uint32_t P, uint32_t C, uint32_t B, uint32_t S, const float *__restrict__ dsum,
                                  const float *__restrict__ flux, const float *__restrict__ rho,
                                  const float *__restrict__ tau, const uint8_t *__restrict__ mask, float *in, float *st,
-                                 float *sb, double *inc) {
+                                 float *sb, const float *__restrict__ gsum) {
     size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= (size_t)P * C) return;
     uint32_t i = (uint32_t)(q / C), c = (uint32_t)(q % C);
@@ -33,7 +36,8 @@ __global__ void k_deposit_direct(uint32_t P, uint32_t C, uint32_t B, uint32_t S,
     float r = rho[(size_t)i * B + b], t = tau[(size_t)i * B + b];
     float a_in = 0.f, a_st = 0.f, a_sb = 0.f;
     for (uint32_t s = 0; s < S; s++) {
-        const float *ds = dsum + (((size_t)k * S + s) * P + i) * 2;
+        const float *ds = gsum ? gsum + ((((size_t)k * S + s) * P + i) * B + b) * 2
+                               : dsum + (((size_t)k * S + s) * P + i) * 2;
         float stp = ds[0], sbt = ds[1];
         float F = flux[((size_t)k * S + s) * B + b];
         a_in += F * (stp + sbt) * (1.0f - r - t);
@@ -398,6 +402,10 @@ size_t solve_workspace_bytes(uint32_t P, const rt_band_request &q, uint32_t K, b
                    (size_t)K * Sn * sizeof(DirectJob) + (sizeof(DiffuseJob) + sizeof(GatherJob)) * (1 + maxdepth) +
                    (size_t)C * 9 * 4 + 64 * 64;
     if (!resident_materials) bytes += (size_t)P * B * 4 * 3 + (size_t)P * 4;
+    // translucent covers: flags, glass parameters, pushed shares [P][C][3] FP64, per-band direct sums, jobs
+    if (q.is_glass)
+        bytes += (size_t)P + (size_t)P * B * 8 + PC * 24 + (size_t)K * Sn * P * B * 8 +
+                 (size_t)K * Sn * sizeof(GlassDirectJob) + sizeof(GlassCtx) + 4 * 256;
     // energy bookkeeping: [P][C][2] pass densities, [P][2] sky-bound rays, partials, per-pass rows, totals
     if (acct)
         bytes += PC * 8 + (size_t)P * 8 + (size_t)ENERGY_BLOCKS * 4 * C * 8 + (size_t)(1 + maxdepth) * 4 * C * 8 +
@@ -478,6 +486,35 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
     std::vector<float> hsky;
     if (sky_table(q, suns, K, hsky)) return -1;
 
+    // translucent covers: primitives glass in every band of the request (direct_raygen.comp:629-637)
+    std::vector<uint8_t> hcover;
+    bool glass = false;
+    if (q.is_glass && P) {
+        if (!q.glass_n) { set_error("run_band: is_glass needs glass_n"); return -1; }
+        if (dm) { set_error("run_band: translucent covers are not supported by the request server"); return -1; }
+        hcover.assign(P, 0);
+        for (uint32_t i = 0; i < P; i++) {
+            bool all = true;
+            for (uint32_t b = 0; b < B; b++) {
+                const size_t k = (size_t)i * B + b;
+                if (!q.is_glass[k]) { all = false; continue; }
+                if (!(q.glass_n[k] >= 1.0f) || (q.glass_KL && !(q.glass_KL[k] >= 0.0f))) {
+                    set_error("run_band: primitive " + std::to_string(i) + " band " + std::to_string(b) +
+                              ": glass needs a refractive index >= 1 and an absorption K*L >= 0");
+                    return -1;
+                }
+            }
+            hcover[i] = all ? 1 : 0;
+            glass = glass || all;
+        }
+        if (glass && C > RT_GLASS_MAX_CH) {
+            set_error("run_band: a solve with translucent covers takes at most " + std::to_string(RT_GLASS_MAX_CH) +
+                      " channels (suns x bands), this one has " + std::to_string(C));
+            return -1;
+        }
+    }
+    if (glass) need_hits = false;  // covers trace and gather in one kernel
+
     // ---- workspace ----
     size_t PC = (size_t)P * C;
     size_t bytes = solve_workspace_bytes(P, q, K, dm != nullptr, sx.energy_acct);
@@ -497,8 +534,15 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
     uint8_t *masks = cv.take<uint8_t>(mk.size());
     uint32_t *depth = cv.take<uint32_t>(B);
     double *area = cv.take<double>(P);
-    const bool acct = sx.energy_acct;
+    const bool acct = sx.energy_acct && !glass;  // the bookkeeping follows the hit-table passes only
     double *energy = cv.take<double>((size_t)C * RT_ENERGY_ROWS);
+    uint8_t *gcover = glass ? cv.take<uint8_t>(P) : nullptr;
+    float *ggn = glass ? cv.take<float>((size_t)P * B) : nullptr;
+    float *ggkl = glass ? cv.take<float>((size_t)P * B) : nullptr;
+    double *gpush = glass ? cv.take<double>(PC * 3) : nullptr;
+    float *ggsum = glass ? cv.take<float>((size_t)K * Sn * P * B * 2 + 1) : nullptr;
+    GlassDirectJob *gdjobs = glass ? cv.take<GlassDirectJob>((size_t)K * Sn + 1) : nullptr;
+    GlassCtx *gctx = glass ? cv.take<GlassCtx>(1) : nullptr;
     float *eacct = acct ? cv.take<float>(PC * 2) : nullptr;
     uint32_t *emiss = acct ? cv.take<uint32_t>((size_t)P * 2) : nullptr;
     double *epart = acct ? cv.take<double>((size_t)ENERGY_BLOCKS * 4 * C) : nullptr;
@@ -529,6 +573,17 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
     if (dsky) CK(hipMemcpyAsync(dsky, hsky.data(), hsky.size() * 4, hipMemcpyHostToDevice, st));
     CK(hipMemcpyAsync(masks, mk.data(), mk.size(), hipMemcpyHostToDevice, st));
     CK(hipMemcpyAsync(depth, q.band_depth, (size_t)B * 4, hipMemcpyHostToDevice, st));
+    GlassCtx hg_ctx{};
+    if (glass) {
+        CK(hipMemcpyAsync(gcover, hcover.data(), P, hipMemcpyHostToDevice, st));
+        CK(hipMemcpyAsync(ggn, q.glass_n, (size_t)P * B * 4, hipMemcpyHostToDevice, st));
+        if (q.glass_KL) CK(hipMemcpyAsync(ggkl, q.glass_KL, (size_t)P * B * 4, hipMemcpyHostToDevice, st));
+        else CK(hipMemsetAsync(ggkl, 0, (size_t)P * B * 4, st));
+        CK(hipMemsetAsync(gpush, 0, PC * 3 * 8, st));
+        CK(hipMemsetAsync(ggsum, 0, ((size_t)K * Sn * P * B * 2 + 1) * 4, st));
+        hg_ctx.cover = gcover; hg_ctx.gn = ggn; hg_ctx.gkl = ggkl; hg_ctx.push = gpush; hg_ctx.B = B; hg_ctx.C = C;
+        CK(hipMemcpyAsync(gctx, &hg_ctx, sizeof(GlassCtx), hipMemcpyHostToDevice, st));
+    }
 
     std::vector<DirectJob> hd;
     uint32_t items = 0, ncol = 0;
@@ -559,7 +614,7 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
             }
         // occlusion grids, one per collimated (sun, source) direction, when the scene is not periodic
         sx.last_direct_grid = false;
-        if (sx.use_grid && ncol) {
+        if (sx.use_grid && ncol && !glass) {
             std::vector<float> jd((size_t)ncol * 3);
             for (size_t j = 0; j < ncol; j++)
                 for (int a = 0; a < 3; a++) jd[j * 3 + a] = hd[j].sdir[a];
@@ -578,6 +633,20 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
             }
         }
         CK(hipMemcpyAsync(djobs, hd.data(), hd.size() * sizeof(DirectJob), hipMemcpyHostToDevice, st));
+    }
+    std::vector<GlassDirectJob> hgd;
+    if (glass && any_direct) {
+        for (const DirectJob &j : hd) {
+            GlassDirectJob g{};
+            g.dj = j;
+            // job (sun k, source s): its outputs were laid out at dsum + (k * Sn + s) * P * 2
+            const size_t ks = (size_t)(j.out - dsum) / ((size_t)P * 2);
+            g.flux = flux + ks * B;
+            g.gsum = ggsum + ks * P * B * 2;
+            g.chan0 = (uint32_t)(ks / Sn) * B;
+            hgd.push_back(g);
+        }
+        CK(hipMemcpyAsync(gdjobs, hgd.data(), hgd.size() * sizeof(GlassDirectJob), hipMemcpyHostToDevice, st));
     }
     std::vector<DiffuseJob> hf(1 + maxdepth);
     for (uint32_t g = 0; g < hf.size(); g++) {
@@ -613,15 +682,21 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
         hipEvent_t a, b;
         if (ev_pair(sx, a, b)) return -1;
         if (a) CK(hipEventRecord(a, st));
-        if (launch_direct_jobs(hd.data(), djobs, (uint32_t)hd.size(), items, S.h.depth, counter, err, sx.stat_ptr(0),
-                               sx.cfg[0], sx.ncu, st, S.h.mdesc != nullptr))
+        if (glass) {
+            if (launch_direct_glass(gdjobs, (uint32_t)hgd.size(), items, S.h.depth, counter, gctx, err, sx.ncu, st,
+                                    S.h.mdesc != nullptr))
+                return -1;
+        } else if (launch_direct_jobs(hd.data(), djobs, (uint32_t)hd.size(), items, S.h.depth, counter, err,
+                                      sx.stat_ptr(0), sx.cfg[0], sx.ncu, st, S.h.mdesc != nullptr)) {
             return -1;
+        }
         if (b) { CK(hipEventRecord(b, st)); tick(sx, 0, a, b); }
         sx.timing_info.direct_rays += (uint64_t)items * d * d;
         sx.timing_info.direct_launches++;
         k_deposit_direct<<<nb(PC), 256, 0, st>>>(P, C, B, Sn, dsum, flux, rho, tau, masks + (size_t)M_RD * C, in,
-                                                   stb, sbb, nullptr);
+                                                   stb, sbb, glass ? ggsum : nullptr);
         LK();
+        if (glass && launch_add_push(PC, gpush, in, stb, sbb, st)) return -1;
     }
     if (scattering) {
         k_after_direct<<<nb(PC), 256, 0, st>>>(P, C, S.h.nrm, masks + (size_t)M_AFTER * C, stb, sbb, ot, ob);
@@ -632,6 +707,15 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
         hipEvent_t a, b;
         if (ev_pair(sx, a, b)) return -1;
         if (a) CK(hipEventRecord(a, st));
+        if (glass) {
+            if (launch_diffuse_glass(fjobs + g, gjobs + g, P, S.h.depth, counter, gctx, err, sx.ncu, st,
+                                     S.h.mdesc != nullptr))
+                return -1;
+            if (b) { CK(hipEventRecord(b, st)); tick(sx, 1, a, b); }
+            sx.timing_info.diffuse_rays += (uint64_t)P * 2 * nn;
+            sx.timing_info.diffuse_launches++;
+            return launch_add_push(PC, gpush, in, stb, sbb, st);
+        }
         if (launch_diffuse(fjobs + g, P, S.h.depth, counter, err, sx.stat_ptr(1), sx.cfg[1], sx.ncu, st,
                            S.h.mdesc != nullptr))
             return -1;

@@ -2047,6 +2047,274 @@ int launch_debug_trace(const DevScene *sc, uint32_t n, const float *rays, const 
 }
 
 // ------------------------------------------------------------------------------------------------
+// translucent covers (R/shaders/common/glass_cover.glsl, direct_raygen.comp:587-664,
+// diffuse_raygen.comp:484-560)
+// ------------------------------------------------------------------------------------------------
+// The reference's trace loop with cover pass-through: up to 32 closest-hit traces; a periodic face wraps the
+// ray; a miss -- or, with dist0 >= 0, a hit at or beyond dist0 -- ends it (status 0); a hit on a primitive that
+// is not a cover stops it (status 1, that primitive returned); a cover calls push(cover, top face, band,
+// transmittance so far, rho, alpha) for every band, multiplies ctau[b] by its transmittance at the crossing
+// angle and the ray goes on from just past it.  After 32 traces the last trace's hit is returned (status 2).
+template <int STK, typename C, typename Push>
+__device__ __forceinline__ uint32_t trace_covers(const DevScene &sc, const GlassCtx &G, v3f o, v3f d, float tmin,
+                                                 uint32_t skip, float dist0, float *ctau, uint32_t *stk,
+                                                 uint32_t stride, int *err, C &cnt, int &status, Push &&push) {
+    v3f cur = o;
+    uint32_t h = RT_MISS;
+    for (int step = 0; step < 32; ++step) {
+        float t;
+        uint32_t slot;
+        h = trace_closest<false, STK>(sc, cur, d, tmin, skip, 1e30f, t, slot, stk, stride, err, cnt);
+        if (sc.nbbox) {
+            const float bt = nearest_face(sc, cur, d, t);
+            if (bt > 0.f) { cur = wrap_at(sc, cur, d, bt); continue; }
+        }
+        if (h == RT_MISS) { status = 0; return RT_MISS; }
+        if (dist0 >= 0.f && t >= dist0) { status = 0; return h; }
+        if (!G.cover[h]) { status = 1; return h; }
+        const float4 hn = sc.nrm[h];
+        const v3f nh = mk3(hn.x, hn.y, hn.z);
+        const float c = fabsf(dot3(d, nh));
+        const bool top = dot3(nh, d) < 0.0f;
+        for (uint32_t b = 0; b < G.B; b++) {
+            float tau, rho, alpha;
+            glass_tra(c, G.gn[(size_t)h * G.B + b], G.gkl[(size_t)h * G.B + b], tau, rho, alpha);
+            push(h, top, b, ctau[b], rho, alpha);
+            ctau[b] *= tau;
+        }
+        cur = add3(cur, mul3(d, t + 1e-4f));
+    }
+    status = 2;
+    return h;
+}
+
+// pushed share of a cover: FP64 atomic into GlassCtx.push[cover][channel][slot]
+__device__ __forceinline__ void glass_push(const GlassCtx &G, uint32_t h, uint32_t c, bool top, float inc, float rho,
+                                           float alpha) {
+    if (!(inc > 0.0f)) return;
+    double *q = G.push + ((size_t)h * G.C + c) * 3;
+    atomicAdd(q + 0, (double)(inc * alpha));
+    atomicAdd(q + (top ? 1 : 2), (double)(inc * rho));
+}
+
+// Direct rays of a solve with covers (direct_raygen.comp:413-731 with the glass loop): one wave per (job,
+// primitive); d*d stratified origins, every source type; an unblocked ray adds strength x (transmittance of the
+// covers crossed, per band) to its lit face's sums, and every cover crossed takes its shares of
+// flux x strength x transmittance so far.  gsum[i][b][face] of each job is deposited like the strength sums.
+template <int STK, bool MK>
+__global__ __launch_bounds__(256) void k_direct_glass(const GlassDirectJob *__restrict__ jobs, uint32_t njobs,
+                                                      uint32_t total, uint32_t *counter, const GlassCtx *__restrict__ gp,
+                                                      int *err) {
+    __shared__ uint32_t stk_all[STK * 256];
+    uint32_t *stk = stk_all + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63;
+    const GlassCtx G = *gp;
+    NoCntT<MK> cnt{};
+    for (uint32_t item = wave_item(counter, 1u); item < total; item = wave_item(counter, 1u)) {
+        uint32_t j = 0;
+        {
+            uint32_t lo = 0, hi = njobs;  // last job with dj.item_begin <= item
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) / 2;
+                if (jobs[mid].dj.item_begin <= item) lo = mid; else hi = mid;
+            }
+            j = lo;
+        }
+        const GlassDirectJob &GJ = jobs[j];
+        const DirectJob &J = GJ.dj;
+        const DevScene &sc = *J.sc;
+        uint32_t i = item - J.item_begin;
+        if (J.order) i = J.order[i];
+        const uint32_t ty = sc.type[i];
+        ksum acc[2][RT_GLASS_MAX_CH];
+#pragma unroll
+        for (int f = 0; f < 2; f++)
+#pragma unroll
+            for (int b = 0; b < RT_GLASS_MAX_CH; b++) acc[f][b].init();
+        if (traced_type(ty)) {
+            const float4 n4 = sc.nrm[i];
+            const v3f n = mk3(n4.x, n4.y, n4.z);
+            const uint32_t tw = __float_as_uint(n4.w);
+            const bool patch = ty == 0 || ty == 3;
+            const uint32_t d = J.d, Nrays = d * d;
+            float tm[12];
+            const float *Tp = sc.T + (size_t)i * 16;
+#pragma unroll
+            for (int q = 0; q < 12; q++) tm[q] = Tp[q];
+            const v3f sp3 = mk3(J.sdir[0], J.sdir[1], J.sdir[2]);
+            for (uint32_t base = 0; base < Nrays; base += 64) {
+                const uint32_t r = base + lane;
+                if (r >= Nrays) continue;
+                float spx, spy;
+                uint32_t st;
+                direct_sample_st(i, r % d, r / d, d, J.seed, patch, spx, spy, st);
+                const uint32_t mdi = (MK && sc.mdesc) ? sc.mdesc[i] : RT_MISS;
+                if (MK && mdi != RT_MISS && !origin_mask(sc.rec, mdi, patch, false, st, spx, spy)) continue;
+                const v3f o = xform3(tm, spx, spy, 0.0f);
+                for (uint32_t k = 0; k < J.skip_draws; k++) (void)rnd24(st);
+                v3f dir;
+                float strength;
+                float dist0 = -1.0f;
+                if (J.stype == 0u) {
+                    dir = nrm3(sp3);
+                    strength = rdiv(fabsf(dot3(n, dir)), (float)Nrays);
+                } else {
+                    if (!source_ray(J.stype, J.sdir, J.width, J.rot, o, n, Nrays, st, dir, strength)) continue;
+                    const v3f dv = sub3(sp3, o);
+                    dist0 = rsqrt_cr(dot3(dv, dv));
+                }
+                const bool top = dot3(dir, n) > 0.0f;
+                if (tw == 3 || (!top && tw == 0)) continue;
+                float ctau[RT_GLASS_MAX_CH];
+                for (uint32_t b = 0; b < G.B; b++) ctau[b] = 1.0f;
+                int status;
+                (void)trace_covers<STK>(sc, G, o, dir, 1e-4f, i, dist0, ctau, stk, 256u, err, cnt, status,
+                                        [&](uint32_t h, bool ctop, uint32_t b, float tr, float rho, float alpha) {
+                                            glass_push(G, h, GJ.chan0 + b, ctop, GJ.flux[b] * strength * tr, rho,
+                                                       alpha);
+                                        });
+                if (status == 1) continue;  // an opaque primitive blocks the source
+#pragma unroll
+                for (int b = 0; b < RT_GLASS_MAX_CH; b++)
+                    if ((uint32_t)b < G.B) {
+                        if (top) acc[0][b].add(strength * ctau[b]);
+                        else acc[1][b].add(strength * ctau[b]);
+                    }
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < RT_GLASS_MAX_CH; b++) {
+            wave_reduce(acc[0][b]);
+            wave_reduce(acc[1][b]);
+        }
+        if (lane == 0) {
+            float *o = GJ.gsum + (size_t)i * G.B * 2;
+#pragma unroll
+            for (int b = 0; b < RT_GLASS_MAX_CH; b++)
+                if ((uint32_t)b < G.B) { o[2 * b] = acc[0][b].value(); o[2 * b + 1] = acc[1][b].value(); }
+        }
+    }
+}
+
+// Diffuse / scattering pass of a solve with covers: trace and gather in one kernel (diffuse_raygen.comp:330-656
+// with the glass loop).  One wave per primitive, both launched faces, rays in rounds of 64; a ray passes
+// through covers, which take their shares of sky flux x strength x transmittance so far (only a pass with sky
+// flux pushes anything); a sky ray brings the sky flux times the covers' transmittance, a surface hit the hit
+// face's radiation_out unattenuated.  Each channel's sum per face is deposited as the gather deposits it.
+template <int STK, bool MK>
+__global__ __launch_bounds__(256) void k_diffuse_glass(const DiffuseJob *__restrict__ fj, const GatherJob *__restrict__ gj,
+                                                       uint32_t total, uint32_t *counter,
+                                                       const GlassCtx *__restrict__ gp, int *err) {
+    __shared__ uint32_t stk_all[STK * 256];
+    uint32_t *stk = stk_all + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63;
+    const GlassCtx G = *gp;
+    const DiffuseJob &F = *fj;
+    const GatherJob &J = *gj;
+    const DevScene &sc = *F.sc;
+    const uint32_t n = F.n, nn = n * n, seed = F.seed, C = J.C;
+    const float strength = rdiv(1.0f, (float)nn);
+    NoCntT<MK> cnt{};
+    for (uint32_t item = wave_item(counter, 1u); item < total; item = wave_item(counter, 1u)) {
+        uint32_t i = item - F.item_begin;
+        if (F.order) i = F.order[i];
+        const uint32_t ty = sc.type[i];
+        if (!traced_type(ty)) continue;
+        const float4 n4 = sc.nrm[i];
+        const uint32_t tw = __float_as_uint(n4.w);
+        const bool patch = ty == 0 || ty == 3;
+        const uint32_t mdi = (MK && sc.mdesc) ? sc.mdesc[i] : RT_MISS;
+        const float4 fr = sc.frame[i];
+        float tm[12];
+        const float *Tp = sc.T + (size_t)i * 16;
+#pragma unroll
+        for (int q = 0; q < 12; q++) tm[q] = Tp[q];
+        ksum acc[2][RT_GLASS_MAX_CH];
+#pragma unroll
+        for (int f = 0; f < 2; f++)
+#pragma unroll
+            for (int c = 0; c < RT_GLASS_MAX_CH; c++) acc[f][c].init();
+        const uint32_t nslot = tw != 0 ? 2u : 1u;  // bottom face launched for two-sided primitives only
+        for (uint32_t base = 0; base < nslot * nn; base += 64) {
+            const uint32_t r = base + lane;
+            if (r >= nslot * nn) continue;
+            const uint32_t slot = r >= nn ? 1u : 0u, rr = r - slot * nn;
+            float spx, spy;
+            v3f dl;
+            if (!diffuse_sample_m(i, rr % n, rr / n, n, seed, patch, sc.rec, mdi, spx, spy, dl)) continue;
+            const v3f o = xform3(tm, spx, spy, 0.0f);
+            v3f dir = frame_rotate(dl, fr.x, fr.y, fr.z, fr.w);
+            if (slot == 1 && tw == 1) dir = mk3(-dir.x, -dir.y, -dir.z);
+            float ctau[RT_GLASS_MAX_CH];
+            for (uint32_t b = 0; b < G.B; b++) ctau[b] = 1.0f;
+            int status;
+            const uint32_t h = trace_covers<STK>(
+                sc, G, o, dir, 1e-5f, i, -1.0f, ctau, stk, 256u, err, cnt, status,
+                [&](uint32_t hc, bool ctop, uint32_t b, float tr, float rho, float alpha) {
+                    for (uint32_t c = b; c < C; c += G.B)
+                        if (J.active[c]) glass_push(G, hc, c, ctop, J.dflux[c] * strength * tr, rho, alpha);
+                });
+            if (h != RT_MISS) {
+                const float4 hn = sc.nrm[h];
+                const bool hit_top = dot3(mk3(hn.x, hn.y, hn.z), dir) < 0.0f;
+                const float *src = (hit_top ? J.out_top : J.out_bot) + (size_t)h * C;
+#pragma unroll
+                for (int c = 0; c < RT_GLASS_MAX_CH; c++)
+                    if ((uint32_t)c < C) {
+                        if (slot) acc[1][c].add(src[c]);
+                        else acc[0][c].add(src[c]);
+                    }
+            } else {
+#pragma unroll
+                for (int c = 0; c < RT_GLASS_MAX_CH; c++)
+                    if ((uint32_t)c < C) {
+                        float v = J.dflux[c];
+                        if (J.sky) v = sky_weight(dir, J.sky + 9 * c) * v;
+                        v = v * ctau[(uint32_t)c % G.B];
+                        if (slot) acc[1][c].add(v);
+                        else acc[0][c].add(v);
+                    }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < RT_GLASS_MAX_CH; c++) {
+            wave_reduce(acc[0][c]);
+            wave_reduce(acc[1][c]);
+        }
+        if (lane < C) {
+            float I1 = 0.f, I0 = 0.f;
+#pragma unroll
+            for (int c = 0; c < RT_GLASS_MAX_CH; c++)
+                if ((uint32_t)c == lane) { I1 = acc[0][c].value(); I0 = acc[1][c].value(); }
+            const uint32_t c = lane;
+            if (J.active[c]) {
+                I1 = I1 * strength;
+                I0 = I0 * strength;
+                const uint32_t b = c % J.B;
+                const float rho = J.rho[(size_t)i * J.B + b], tau = J.tau[(size_t)i * J.B + b];
+                const size_t q = (size_t)i * C + c;
+                J.in[q] += (I1 + I0) * (1.0f - rho - tau);
+                J.st[q] += I1 * rho + I0 * tau;
+                J.sb[q] += I1 * tau + I0 * rho;
+            }
+        }
+    }
+}
+
+// radiation_in / scatter_top / scatter_bottom += the covers' pushed shares (rounded once), which are cleared
+__global__ void k_add_push(size_t PC, double *push, float *in, float *st, float *sb) {
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= PC) return;
+    double *p = push + q * 3;
+    if (p[0] != 0.0) in[q] += (float)p[0];
+    if (p[1] != 0.0) st[q] += (float)p[1];
+    if (p[2] != 0.0) sb[q] += (float)p[2];
+    p[0] = 0.0;
+    p[1] = 0.0;
+    p[2] = 0.0;
+}
+
+// ------------------------------------------------------------------------------------------------
 // launchers
 // ------------------------------------------------------------------------------------------------
 static inline uint32_t grid_for(const LaunchCfg &cfg, int ncu, uint32_t total, uint32_t wpb) {
@@ -2245,6 +2513,47 @@ int launch_camera(const CameraLaunch &c, uint32_t depth, bool labels, int *err, 
         PICK_STK(depth, (K_CAM<32, false, false><<<g, 256, 0, st>>>(c, err, nullptr)),
                  (K_CAM<64, false, false><<<g, 256, 0, st>>>(c, err, nullptr)));
 #undef K_CAM
+    LCHECK();
+    return 0;
+}
+
+int launch_direct_glass(const GlassDirectJob *jobs, uint32_t njobs, uint32_t total, uint32_t depth, uint32_t *counter,
+                        const GlassCtx *g, int *err, int ncu, hipStream_t st, bool masked) {
+    if (total == 0) return 0;
+    if (depth + 1 > RT_STACK) {
+        set_error("BVH depth " + std::to_string(depth) + " exceeds the traversal stack (" + std::to_string(RT_STACK) + ")");
+        return -1;
+    }
+    k_zero_u32<<<1, 1, 0, st>>>(counter);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)4 * ncu, (total + 3) / 4);
+    if (masked) PICK_STK(depth, (k_direct_glass<32, true><<<grid, 256, 0, st>>>(jobs, njobs, total, counter, g, err)),
+                         (k_direct_glass<RT_STACK, true><<<grid, 256, 0, st>>>(jobs, njobs, total, counter, g, err)));
+    else PICK_STK(depth, (k_direct_glass<32, false><<<grid, 256, 0, st>>>(jobs, njobs, total, counter, g, err)),
+                  (k_direct_glass<RT_STACK, false><<<grid, 256, 0, st>>>(jobs, njobs, total, counter, g, err)));
+    LCHECK();
+    return 0;
+}
+
+int launch_diffuse_glass(const DiffuseJob *fj, const GatherJob *gj, uint32_t total, uint32_t depth, uint32_t *counter,
+                         const GlassCtx *g, int *err, int ncu, hipStream_t st, bool masked) {
+    if (total == 0) return 0;
+    if (depth + 1 > RT_STACK) {
+        set_error("BVH depth " + std::to_string(depth) + " exceeds the traversal stack (" + std::to_string(RT_STACK) + ")");
+        return -1;
+    }
+    k_zero_u32<<<1, 1, 0, st>>>(counter);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)4 * ncu, (total + 3) / 4);
+    if (masked) PICK_STK(depth, (k_diffuse_glass<32, true><<<grid, 256, 0, st>>>(fj, gj, total, counter, g, err)),
+                         (k_diffuse_glass<RT_STACK, true><<<grid, 256, 0, st>>>(fj, gj, total, counter, g, err)));
+    else PICK_STK(depth, (k_diffuse_glass<32, false><<<grid, 256, 0, st>>>(fj, gj, total, counter, g, err)),
+                  (k_diffuse_glass<RT_STACK, false><<<grid, 256, 0, st>>>(fj, gj, total, counter, g, err)));
+    LCHECK();
+    return 0;
+}
+
+int launch_add_push(size_t PC, double *push, float *in, float *stb, float *sbb, hipStream_t st) {
+    if (PC == 0) return 0;
+    k_add_push<<<(uint32_t)((PC + 255) / 256), 256, 0, st>>>(PC, push, in, stb, sbb);
     LCHECK();
     return 0;
 }

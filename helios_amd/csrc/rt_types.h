@@ -112,6 +112,27 @@ struct CameraLaunch {
     float *depths;                     // [full_x * full_y]
 };
 
+// Translucent covers of one solve (device copy): primitives glass in every band pass rays through, and the
+// kernels push their absorbed / reflected shares into `push` (FP64 atomics, added to the FP32 buffers after
+// the pass, so the sums do not depend on the atomics' order beyond FP64 rounding).
+#define RT_GLASS_MAX_CH 8
+struct GlassCtx {
+    const uint8_t *cover;  // [P] 1 = cover for this solve's bands
+    const float *gn;       // [P][B] refractive index
+    const float *gkl;      // [P][B] absorption product K*L
+    double *push;          // [P][C][3] radiation_in, scatter_top, scatter_bottom
+    uint32_t B, C;
+};
+
+// one direct job of a solve with covers: DirectJob's source and rays, plus its channels' source fluxes and
+// the per-band strength sums it writes
+struct GlassDirectJob {
+    DirectJob dj;
+    const float *flux;     // [B] source flux of this (sun, source) per band
+    float *gsum;           // [P][B][2] strength x cover transmittance, lit from top / bottom
+    uint32_t chan0;        // first channel of this job's sun (sun * B)
+};
+
 // one diffuse trace pass: writes hit words [P][2][n*n]
 struct DiffuseJob {
     const DevScene *sc;

@@ -40,6 +40,7 @@ class rt_band_request(C.Structure):
         ("direct_seed", C.c_uint32), ("diffuse_seed", u32p), ("schedule", C.c_uint32),
         ("source_type", u32p), ("source_width", f32p), ("source_rotation", f32p),
         ("diffuse_extinction", f32p), ("diffuse_peak_dir", f32p), ("diffuse_dist_norm", f32p), ("sky_params", f32p),
+        ("is_glass", C.POINTER(C.c_uint8)), ("glass_n", f32p), ("glass_KL", f32p),
     ]
 
 
@@ -690,11 +691,14 @@ class BandRequest:
     def __init__(self, P: int, bands: Sequence[BandSpec], rho, tau, direct_rays: int, diffuse_rays: int,
                  source_dirs=None, source_flux=None, direct_seed: int = 1, diffuse_seeds=None, eps=None,
                  temperature=None, schedule: int = 0, source_types=None, source_widths=None,
-                 source_rotations=None, sky_extinction=None, sky_peak_dir=None, sky_norm=None, sky_prague=None):
+                 source_rotations=None, sky_extinction=None, sky_peak_dir=None, sky_norm=None, sky_prague=None,
+                 is_glass=None, glass_n=None, glass_KL=None):
         """source_dirs [S][3]: direction towards a collimated source, else the source position;
         source_types [S] (SOURCE_*; None = all collimated), source_widths [S][2] and source_rotations [S][3]
         as RayTracingSource::width / ::rotation.  Anisotropic sky per band: sky_extinction [B] (power-law K),
-        sky_peak_dir [B][3], sky_norm [B] (power-law normalisation), sky_prague [B][4]."""
+        sky_peak_dir [B][3], sky_norm [B] (power-law normalisation), sky_prague [B][4].  Translucent covers:
+        is_glass [P][B] (a primitive glass in every band is a cover), glass_n [P][B] refractive index,
+        glass_KL [P][B] absorption product (None = 0)."""
         B = len(bands)
         self.P, self.B = P, B
         self.bands = list(bands)
@@ -722,13 +726,19 @@ class BandRequest:
         self.sky_peak = None if sky_peak_dir is None else _f32(sky_peak_dir).reshape(B, 3)
         self.sky_norm = None if sky_norm is None else _f32(sky_norm).reshape(B)
         self.sky_prague = None if sky_prague is None else _f32(sky_prague).reshape(B, 4)
+        self.is_glass = None if is_glass is None else np.ascontiguousarray(is_glass, np.uint8).reshape(P, B)
+        self.glass_n = None if glass_n is None else _f32(glass_n).reshape(P, B)
+        self.glass_KL = None if is_glass is None else (np.zeros((P, B), np.float32) if glass_KL is None
+                                                         else _f32(glass_KL).reshape(P, B))
 
     def c_struct(self) -> rt_band_request:
         return rt_band_request(
             self.B, self.sdir.shape[0], self.direct_rays, self.diffuse_rays, _pu(self.depth), _pu(self.emission),
             _pf(self.sdir), _pf(self.sflux), _pf(self.dflux), _pf(self.rho), _pf(self.tau), _pf(self.eps),
             _pf(self.temp), self.direct_seed, _pu(self.dseeds), self.schedule, _pu(self.stype), _pf(self.swidth),
-            _pf(self.srot), _pf(self.sky_ext), _pf(self.sky_peak), _pf(self.sky_norm), _pf(self.sky_prague))
+            _pf(self.srot), _pf(self.sky_ext), _pf(self.sky_peak), _pf(self.sky_norm), _pf(self.sky_prague),
+            None if self.is_glass is None else self.is_glass.ctypes.data_as(C.POINTER(C.c_uint8)), _pf(self.glass_n),
+            _pf(self.glass_KL))
 
 
 def run_band(scene: Scene, req: BandRequest) -> np.ndarray:

@@ -165,7 +165,19 @@ typedef struct {
     const float *diffuse_dist_norm;  /* [B] power-law normalisation                            */
     const float *sky_params;         /* [B][4] Prague (circumsolar strength, width in degrees, */
                                      /* horizon brightness, normalisation)                     */
+    /* Translucent covers (RayTracingMaterial is_glass / glass_n / glass_KL, R/include/RayTracingTypes.h:
+     * 173-178; R/shaders/common/glass_cover.glsl:15-53, direct_raygen.comp:587-664,
+     * diffuse_raygen.comp:484-560), [P][B] each, is_glass NULL = no covers.  A primitive that is glass in
+     * every band of the request is a cover: rays pass through it, multiplying each band's flux by the
+     * sheet's Fresnel + Bouguer transmittance at the crossing angle and pushing the absorbed and reflected
+     * shares into the cover (direct rays: of the source flux; diffuse rays: of the sky flux); a sky ray
+     * is attenuated by the covers it crossed, a surface hit behind a cover is not.  A solve with covers
+     * takes at most RT_GLASS_MAX_CHANNELS channels (suns x bands). */
+    const uint8_t *is_glass;
+    const float *glass_n;            /* refractive index (>= 1)                                */
+    const float *glass_KL;           /* absorption product K*L, NULL = 0                       */
 } rt_band_request;
+#define RT_GLASS_MAX_CHANNELS 8u
 
 /* R_out: [P][B] radiation_flux_<band> (host memory). */
 int rt_run_band(rt_ctx *ctx, const rt_scene *scene, const rt_band_request *req, float *R_out);

@@ -23,7 +23,7 @@ class orc_band_params(C.Structure):
                 ("temperature", f32p), ("direct_seed", C.c_uint32), ("diffuse_seed", u32p),
                 ("schedule", C.c_uint32), ("source_type", u32p), ("source_width", f32p), ("source_rotation", f32p),
                 ("diffuse_extinction", f32p), ("diffuse_peak_dir", f32p), ("diffuse_dist_norm", f32p),
-                ("sky_params", f32p)]
+                ("sky_params", f32p), ("is_glass", C.POINTER(C.c_uint8)), ("glass_n", f32p), ("glass_KL", f32p)]
 
 
 class orc_camera(C.Structure):
@@ -74,6 +74,7 @@ def lib():
         L.orc_diffuse_ray.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                       C.c_uint32, f32p, f32p]
         L.orc_diffuse_hits.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, u32p]
+        L.orc_glass_tra.argtypes = [C.c_float, C.c_float, C.c_float, f32p]
         L.orc_direct_sums.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, f32p, f64p]
         L.orc_diffuse_hits_range.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, u32p]
         L.orc_direct_sums_range.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, f32p, C.c_uint32,
@@ -100,6 +101,13 @@ def _f(a):
 
 def _u(a):
     return np.ascontiguousarray(a, np.uint32)
+
+
+def glass_tra(cos_theta, n, KL=0.0):
+    """(tau, rho, alpha) of a glass sheet at incidence cosine cos_theta (orc_glass_tra)."""
+    out = np.zeros(3, np.float32)
+    lib().orc_glass_tra(float(cos_theta), float(n), float(KL), out.ctypes.data_as(f32p))
+    return tuple(float(x) for x in out)
 
 
 def tea(v0, v1, rounds=16, key=(0xa341316c, 0xc8013ea4, 0xad90777d, 0x7e95761e)):
@@ -312,7 +320,11 @@ class OracleScene:
                     kext=None if getattr(req, "sky_ext", None) is None else _f(req.sky_ext),
                     kpeak=None if getattr(req, "sky_peak", None) is None else _f(req.sky_peak),
                     knorm=None if getattr(req, "sky_norm", None) is None else _f(req.sky_norm),
-                    kprague=None if getattr(req, "sky_prague", None) is None else _f(req.sky_prague))
+                    kprague=None if getattr(req, "sky_prague", None) is None else _f(req.sky_prague),
+                    glass=None if getattr(req, "is_glass", None) is None else
+                    np.ascontiguousarray(req.is_glass, np.uint8),
+                    gn=None if getattr(req, "glass_n", None) is None else _f(req.glass_n),
+                    gkl=None if getattr(req, "glass_KL", None) is None else _f(req.glass_KL))
         p = orc_band_params(
             B, keep["sdir"].reshape(-1, 3).shape[0], req.direct_rays, req.diffuse_rays,
             keep["depth"].ctypes.data_as(u32p), keep["emission"].ctypes.data_as(u32p),
@@ -324,7 +336,9 @@ class OracleScene:
             None if keep["stype"] is None else keep["stype"].ctypes.data_as(u32p),
             None if keep["swidth"] is None else keep["swidth"].ctypes.data_as(f32p),
             None if keep["srot"] is None else keep["srot"].ctypes.data_as(f32p),
-            *[None if keep[k] is None else keep[k].ctypes.data_as(f32p) for k in ("kext", "kpeak", "knorm", "kprague")])
+            *[None if keep[k] is None else keep[k].ctypes.data_as(f32p) for k in ("kext", "kpeak", "knorm", "kprague")],
+            None if keep["glass"] is None else keep["glass"].ctypes.data_as(C.POINTER(C.c_uint8)),
+            *[None if keep[k] is None else keep[k].ctypes.data_as(f32p) for k in ("gn", "gkl")])
         R = np.zeros((P, B), np.float64)
         dg = np.zeros((3, B), np.float64)
         rc = lib().orc_run_band(self.h, C.byref(p), R.ctypes.data_as(f64p), dg.ctypes.data_as(f64p))

@@ -1331,9 +1331,159 @@ static float sky_weight_orc(const orc_band_params *p, uint32_t b, v3 d) {
     return 1.0f;
 }
 
+/* ------------------------------------------------------------------------------------------------ */
+/* translucent covers (R/shaders/common/glass_cover.glsl, direct_raygen.comp:587-664,                 */
+/* diffuse_raygen.comp:484-560)                                                                       */
+/* ------------------------------------------------------------------------------------------------ */
+/* glass_cover.glsl:15-53: Snell refraction, Fresnel reflectance of each polarisation
+ * (sin^2(tr - t) / sin^2(tr + t), tan^2(tr - t) / tan^2(tr + t); the normal-incidence limit below 1e-3 rad),
+ * Bouguer attenuation exp(-KL / cos tr) inside the sheet and the sums over internal reflections of a slab,
+ * averaged over the two polarisations. */
+void orc_glass_tra(float cos_theta, float n, float KL, float out[3]) {
+    cos_theta = fmaxf(1e-4f, fminf(1.0f, cos_theta));
+    float theta = acosf(fmaxf(-1.0f, fminf(1.0f, cos_theta)));
+    float sin_t = sinf(theta);
+    float sin_tr = sin_t / n;
+    float cos_tr = sqrtf(fmaxf(0.0f, 1.0f - sin_tr * sin_tr));
+    float theta_r = asinf(fmaxf(-1.0f, fminf(1.0f, sin_tr)));
+    float r_par, r_per;
+    if (theta < 1e-3f) {
+        float r0 = ((n - 1.0f) / (n + 1.0f)) * ((n - 1.0f) / (n + 1.0f));
+        r_par = r0;
+        r_per = r0;
+    } else {
+        float sm = sinf(theta_r - theta), sp = sinf(theta_r + theta);
+        float tm = tanf(theta_r - theta), tp = tanf(theta_r + theta);
+        r_per = (sm * sm) / fmaxf(1e-12f, sp * sp);
+        r_par = (tm * tm) / fmaxf(1e-12f, tp * tp);
+    }
+    float tau_a = KL > 0.0f ? expf(-KL / fmaxf(1e-4f, cos_tr)) : 1.0f;
+    float tau = 0.0f, rho = 0.0f;
+    for (int pol = 0; pol < 2; pol++) {
+        float r = pol == 0 ? r_per : r_par;
+        float den = fmaxf(1e-6f, 1.0f - (r * tau_a) * (r * tau_a));
+        float ti = tau_a * (1.0f - r) * (1.0f - r) / den;
+        tau += 0.5f * ti;
+        rho += 0.5f * (r * (1.0f + tau_a * ti));
+    }
+    tau = fmaxf(0.0f, fminf(1.0f, tau));
+    rho = fmaxf(0.0f, fminf(1.0f, rho));
+    out[0] = tau;
+    out[1] = rho;
+    out[2] = fmaxf(0.0f, 1.0f - tau - rho);
+}
+
+static void atomic_addd(double *a, double v) {
+#pragma omp atomic
+    *a += v;
+}
+
+/* The reference's trace loop with cover pass-through: up to 32 closest-hit traces, a periodic face wraps
+ * the ray, a miss (or, with dist0 >= 0, a hit at or beyond dist0) reaches the end, a hit on a non-cover
+ * stops it, and a hit on a cover multiplies ctau[b] by the cover's transmittance, pushes
+ * incoming_b = base[b] * ctau[b] (before the crossing) x alpha into the cover's radiation_in and x rho into
+ * its scatter buffer of the face hit (top when the ray meets the cover's normal head-on; nothing when
+ * incoming_b is 0), and continues from just past the cover.  status: 0 reached, 1 stopped by the returned
+ * primitive, 2 the traces ran out (the last trace's hit is returned). */
+static uint32_t trace_covers_orc(const orc_scene *s, const orc_band_params *p, const uint8_t *cover, v3 o, v3 d,
+                                 float tmin, uint32_t skip, float dist0, const float *base, float *ctau, double *in,
+                                 double *st, double *sb, int *status) {
+    uint32_t B = p->nbands;
+    v3 cur = o;
+    uint32_t h = MISS;
+    for (int step = 0; step < 32; ++step) {
+        float t;
+        h = closest(s, cur, d, tmin, skip, 0, 0, 1e30f, &t);
+        if (s->nbbox) {
+            v3 w;
+            if (periodic_wrap(s, cur, d, t, &w)) { cur = w; continue; }
+        }
+        if (h == MISS) { *status = 0; return MISS; }
+        if (dist0 >= 0.0f && t >= dist0) { *status = 0; return h; }
+        if (!cover[h]) { *status = 1; return h; }
+        v3 nh = s->normal[h];
+        float c = fabsf(dot(d, nh));
+        int top = dot(nh, d) < 0.0f;
+        for (uint32_t b = 0; b < B; b++) {
+            float tra[3];
+            orc_glass_tra(c, p->glass_n[(size_t)h * B + b], p->glass_KL[(size_t)h * B + b], tra);
+            float inc = base[b] * ctau[b];
+            if (inc > 0.0f) {
+                size_t k = (size_t)h * B + b;
+                atomic_addd(&in[k], (double)(inc * tra[2]));
+                atomic_addd(top ? &st[k] : &sb[k], (double)(inc * tra[1]));
+            }
+            ctau[b] *= tra[0];
+        }
+        cur = add(cur, scl(d, t + 1e-4f));
+    }
+    *status = 2;
+    return h;
+}
+
+/* Direct pass with covers (direct_raygen.comp:413-731): per primitive, ray and source, the strength of an
+ * unblocked ray times the source flux and the transmittance of the covers crossed, deposited per band on
+ * the lit face; covers take their pushed shares.  Deposits are atomic (covers receive from every primitive). */
+static void direct_glass(const orc_scene *s, const orc_band_params *p, const uint8_t *cover, double *in, double *st,
+                         double *sb, const double *areas, double *diag) {
+    const uint32_t B = p->nbands, S = p->nsources, d = (uint32_t)ceil(sqrt((double)p->direct_rays)), Nrays = d * d;
+    float *mats = (float *)calloc((size_t)S * 9 + 1, sizeof(float));
+    for (uint32_t q = 0; q < S; q++) {
+        uint32_t ty = p->source_type ? p->source_type[q] : 0;
+        if (ty == 3 || ty == 4) orc_rotation_matrix(p->source_rotation + 3 * q, mats + 9 * q);
+    }
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long long ii = 0; ii < (long long)s->P; ii++) {
+        uint32_t i = (uint32_t)ii;
+        if (!traced_type(s->type[i])) continue;
+        v3 n = s->normal[i];
+        uint32_t tw = s->twos[i];
+        float base[256], ctau[256];
+        for (uint32_t r = 0; r < Nrays; r++) {
+            uint32_t stt;
+            v3 o;
+            if (!direct_origin(s, i, r % d, r / d, d, p->direct_seed, &o, &stt)) continue;
+            for (uint32_t src = 0; src < S; src++) {
+                uint32_t ty = p->source_type ? p->source_type[src] : 0;
+                v3 dir;
+                float strength;
+                if (ty == 0) {
+                    dir = nrm(ld3(p->source_dir + src * 3));
+                    strength = fabsf(dot(n, dir)) / (float)Nrays;
+                } else if (!source_sample_orc(ty, ld3(p->source_dir + src * 3), p->source_width + 2 * src,
+                                              mats + 9 * src, o, n, Nrays, &stt, &dir, &strength)) {
+                    continue;
+                }
+                int top = dot(dir, n) > 0.0f;
+                if (tw == 3 || (!top && tw == 0)) continue;
+                float dist0 = -1.0f;
+                if (ty != 0) {
+                    v3 dv = sub(ld3(p->source_dir + src * 3), o);
+                    dist0 = sqrtf(dot(dv, dv));
+                }
+                for (uint32_t b = 0; b < B; b++) { base[b] = p->source_flux[src * B + b] * strength; ctau[b] = 1.0f; }
+                int status;
+                trace_covers_orc(s, p, cover, o, dir, 1e-4f, i, dist0, base, ctau, in, st, sb, &status);
+                if (status == 1) continue;
+                for (uint32_t b = 0; b < B; b++) {
+                    double E = (double)(p->source_flux[src * B + b] * ctau[b]) * (double)strength;
+                    if (E == 0.0) continue;
+                    double rho = p->rho[(size_t)i * B + b], tau = p->tau[(size_t)i * B + b];
+                    size_t k = (size_t)i * B + b;
+                    atomic_addd(&in[k], E * (1.0 - rho - tau));
+                    atomic_addd(top ? &st[k] : &sb[k], E * rho);
+                    atomic_addd(top ? &sb[k] : &st[k], E * tau);
+                    if (diag) atomic_addd(&diag[2 * B + b], E * areas[i]);
+                }
+            }
+        }
+    }
+    free(mats);
+}
+
 static void diffuse_pass(const orc_scene *s, const orc_band_params *p, uint32_t n, uint32_t seed,
                          const double *out_top, const double *out_bot, const float *dflux, const uint8_t *active,
-                         double *in, double *st, double *sb) {
+                         double *in, double *st, double *sb, const uint8_t *cover) {
     uint32_t B = p->nbands, nn = n * n;
     float strength = 1.0f / (float)nn;
 #pragma omp parallel for schedule(dynamic, 16)
@@ -1343,27 +1493,49 @@ static void diffuse_pass(const orc_scene *s, const orc_band_params *p, uint32_t 
         for (int fi = 0; fi < 2; fi++) {
             uint32_t face = fi == 0 ? 1u : 0u;
             if (face == 0 && s->twos[i] == 0) continue;
+            float base[256], ctau[256];
             for (uint32_t r = 0; r < nn; r++) {
                 v3 o, d;
                 if (!diffuse_ray(s, i, r % n, r / n, n, face, seed, &o, &d)) continue;
-                int ex;
-                uint32_t h = trace_loop(s, o, d, 1e-5f, i, &ex);
+                uint32_t h;
+                if (cover) {
+                    /* covers: the sky flux (0 in scattering passes) times the transmittance so far is what
+                     * a crossed cover takes its shares of (diffuse_raygen.comp:536-557) */
+                    for (uint32_t b = 0; b < B; b++) {
+                        base[b] = active[b] ? dflux[b] * strength : 0.0f;
+                        ctau[b] = 1.0f;
+                    }
+                    int status;
+                    h = trace_covers_orc(s, p, cover, o, d, 1e-5f, i, -1.0f, base, ctau, in, st, sb, &status);
+                } else {
+                    int ex;
+                    h = trace_loop(s, o, d, 1e-5f, i, &ex);
+                }
                 int top = 0;
                 if (h != MISS) top = dot(s->normal[h], d) < 0.0f;
                 for (uint32_t b = 0; b < B; b++) {
                     if (!active[b]) continue;
                     double E = 0.0;
                     if (h != MISS) {
+                        /* a surface hit is not attenuated by the covers in front of it (diffuse_raygen.comp:593-598) */
                         double inc = top ? out_top[(size_t)h * B + b] : out_bot[(size_t)h * B + b];
                         E = inc * (double)strength;
                     } else if (dflux[b] > 0.f) {
                         /* fd = 1 for an isotropic sky, else the distribution in the ray's direction */
                         float fd = sky_weight_orc(p, b, d);
-                        E = (double)(fd * dflux[b]) * (double)strength;
+                        float sky = fd * dflux[b];
+                        if (cover) sky = sky * ctau[b];
+                        E = (double)sky * (double)strength;
                     }
                     if (E == 0.0) continue;
                     double rho = p->rho[(size_t)i * B + b], tau = p->tau[(size_t)i * B + b];
                     size_t k = (size_t)i * B + b;
+                    if (cover) {
+                        atomic_addd(&in[k], E * (1.0 - rho - tau));
+                        atomic_addd(face == 1 ? &st[k] : &sb[k], E * rho);
+                        atomic_addd(face == 1 ? &sb[k] : &st[k], E * tau);
+                        continue;
+                    }
                     in[k] += E * (1.0 - rho - tau);
                     if (face == 1) { st[k] += E * rho; sb[k] += E * tau; }
                     else { sb[k] += E * rho; st[k] += E * tau; }
@@ -1388,6 +1560,19 @@ int orc_run_band(const orc_scene *s, const orc_band_params *p, double *R, double
     uint8_t *active = (uint8_t *)calloc(B ? B : 1, 1);
     double *areas = (double *)calloc(P ? P : 1, sizeof(double));
     orc_areas(s, areas);
+    /* covers: glass in every launched band (direct_raygen.comp:629-637) */
+    uint8_t *cover = NULL;
+    if (p->is_glass && B <= 256) {
+        int any = 0;
+        cover = (uint8_t *)calloc(P ? P : 1, 1);
+        for (uint32_t i = 0; i < P; i++) {
+            int all = B > 0;
+            for (uint32_t b = 0; b < B; b++) all = all && p->is_glass[(size_t)i * B + b];
+            cover[i] = (uint8_t)all;
+            any |= all;
+        }
+        if (!any) { free(cover); cover = NULL; }
+    }
     if (diag) for (uint32_t k = 0; k < 3 * B; k++) diag[k] = 0.0;
 
     uint32_t maxdepth = 0;
@@ -1403,7 +1588,9 @@ int orc_run_band(const orc_scene *s, const orc_band_params *p, double *R, double
     uint32_t n = (uint32_t)ceil(sqrt((double)p->diffuse_rays));
 
     /* DIRECT (:3847-3982) */
-    if (S > 0 && rundirect && d > 0) {
+    if (cover && S > 0 && rundirect && d > 0) {
+        direct_glass(s, p, cover, in, st, sb, areas, diag);
+    } else if (S > 0 && rundirect && d > 0) {
         double *sums = (double *)calloc((size_t)P * S * 2, sizeof(double));
         orc_direct_sums_src(s, d, p->direct_seed, S, p->source_dir, p->source_type, p->source_width, p->source_rotation,
                             sums);
@@ -1452,7 +1639,7 @@ int orc_run_band(const orc_scene *s, const orc_band_params *p, double *R, double
         }
         memcpy(ot, ft, PB * sizeof(double)); memcpy(ob, fb, PB * sizeof(double));
         for (uint32_t b = 0; b < B; b++) { dflux[b] = p->diffuse_flux[b]; active[b] = 1; }
-        diffuse_pass(s, p, n, p->diffuse_seed[0], ot, ob, dflux, active, in, st, sb);
+        diffuse_pass(s, p, n, p->diffuse_seed[0], ot, ob, dflux, active, in, st, sb, cover);
         primary = 1;
         if (diag) for (uint32_t b = 0; b < B; b++) diag[1 * B + b] = 0.0;
     }
@@ -1480,12 +1667,13 @@ int orc_run_band(const orc_scene *s, const orc_band_params *p, double *R, double
             if (p->schedule == 1) copy = (sidx > 0) || (primary && rundirect);
             if (copy) { memcpy(ot, st, PB * sizeof(double)); memcpy(ob, sb, PB * sizeof(double)); }
             memset(st, 0, PB * sizeof(double)); memset(sb, 0, PB * sizeof(double));
-            diffuse_pass(s, p, n, p->diffuse_seed[1 + sidx], ot, ob, dflux, active, in, st, sb);
+            diffuse_pass(s, p, n, p->diffuse_seed[1 + sidx], ot, ob, dflux, active, in, st, sb, cover);
         }
     }
     /* leftover scatter deposited as absorbed (:4631-4654) */
     for (size_t q = 0; q < PB; q++) R[q] = in[q] + st[q] + sb[q];
     if (diag) for (uint32_t i = 0; i < P; i++) for (uint32_t b = 0; b < B; b++) diag[b] += R[(size_t)i * B + b] * areas[i];
     free(in); free(st); free(sb); free(ot); free(ob); free(ft); free(fb); free(dflux); free(active); free(areas);
+    free(cover);
     return 0;
 }

@@ -152,7 +152,17 @@ typedef struct {
     const float *diffuse_peak_dir;   /* [B][3]                                             */
     const float *diffuse_dist_norm;  /* [B]                                                */
     const float *sky_params;         /* [B][4] Prague                                      */
+    /* translucent covers (RayTracingMaterial is_glass / glass_n / glass_KL, RayTracingTypes.h:173-178),
+     * [P][B] each, is_glass NULL = none: a primitive that is glass in every band is a cover */
+    const uint8_t *is_glass;
+    const float *glass_n;
+    const float *glass_KL;
 } orc_band_params;
+
+/* Angular transmittance, reflectance and absorptance of a glass sheet (Fresnel, both polarisations,
+ * internal reflections in closed form, Bouguer absorption; R/shaders/common/glass_cover.glsl:15-53):
+ * out[3] = tau, rho, alpha. */
+void orc_glass_tra(float cos_theta, float n, float KL, float out[3]);
 
 /* R[P][B] = radiation_flux_<band>; diag (may be NULL) receives [B] FP64 energy totals:
  * diag[0*B+b] absorbed sum(A*R), diag[1*B+b] primary diffuse incident, diag[2*B+b] direct incident */
