@@ -60,8 +60,8 @@ def pct(a, q):
     return float(np.percentile(np.asarray(a, np.float64), q)) if len(a) else None
 
 
-# position of the COUNT template argument: k_cast<STK, COUNT>, k_camera / k_direct_src<STK, COUNT, MASK>,
-# k_direct / k_diffuse<WAVES, STK, COUNT, MASK>
+# position of the COUNT template argument: k_cast<STK, COUNT, FLAGS>, k_camera / k_direct_src<STK, COUNT, FLAGS>,
+# k_direct / k_diffuse<WAVES, STK, COUNT, FLAGS>
 COUNT_ARG = {"k_cast": 1, "k_camera": 1, "k_direct_src": 1, "k_direct": 2, "k_diffuse": 2}
 
 
@@ -199,6 +199,29 @@ def parity_diurnal(g, base, sun_dir, direct_seed, smp):
     return {"sampled_prims": nprim, "sampled_diffuse_rays": rays, "hit_word_mismatches": mism,
             "direct_sampled_rays": nprim * d * d, "direct_sum_max_abs_err": worst,
             "seeds": {"diffuse": int(base.dseeds[0]), "direct": int(direct_seed)}}
+
+
+def predicates_diurnal(ctx, g, base, dirs, flux, sky, seeds):
+    """One plain solve of the series' first sun with the reference's intersection predicates and again with the
+    watertight ones (rt_set_predicates): per-primitive |R_wt - R_ref| / max(|R_ref|, 1 W/m^2) over primitives
+    and bands (max, 99.9th percentile, counts above 1e-4 and above 0.5%) and each mode's solve time."""
+    req = W.single_sun_request(base, dirs[0], flux[0], sky[0], seeds[0])
+    out, ms = {}, {}
+    try:
+        for mode in ("reference", "watertight"):
+            ctx.set_predicates(mode)
+            native.run_band(g, req)  # warm
+            device_sync()
+            t0 = time.perf_counter()
+            out[mode] = native.run_band(g, req)
+            ms[mode] = (time.perf_counter() - t0) * 1e3
+    finally:
+        ctx.set_predicates("reference")
+    a, b = out["watertight"].astype(np.float64), out["reference"].astype(np.float64)
+    e = (np.abs(a - b) / np.maximum(np.abs(b), 1.0)).ravel()
+    return {"max_rel_diff": float(e.max()), "p99_9_rel_diff": float(np.percentile(e, 99.9)),
+            "n_above_1e-4": int(np.sum(e > 1e-4)), "n_above_0.5pct": int(np.sum(e > 5e-3)), "n": int(e.size),
+            "plain_solve_ms": {k: round(v, 1) for k, v in ms.items()}}
 
 
 def energy_diurnal(ctx, g, base, dirs, flux, sky, seeds, out_ptr):
@@ -367,6 +390,8 @@ def bench_diurnal(args):
     if not args.no_energy:
         log("[bench] energy bookkeeping (both schedules) ...")
         res["energy_fp64"] = energy_diurnal(ctx, g, base, dirs, flux, sky, seeds, out_dev.value)
+    log("[bench] watertight vs reference predicates ...")
+    res["predicates"] = predicates_diurnal(ctx, g, base, dirs, flux, sky, seeds)
     hip.hipFree(out_dev)
     from oracle.oracle import OracleScene
     o = OracleScene.from_arrays(s)

@@ -183,7 +183,7 @@ int rt_cast_rays_soa_device(rt_ctx *ctx, const rt_scene *scene, const float *o, 
     hipEvent_t a = nullptr, b = nullptr;
     if (ctx->sx.timing) { CKR(hipEventCreate(&a)); CKR(hipEventCreate(&b)); CKR(hipEventRecord(a, st)); }
     if (launch_cast(scene->s.d, scene->s.h.depth, n, o, d, max_dist, t, nrm, uuid, counter, err, ctx->sx.stat_ptr(2),
-                    ctx->sx.ncu, st))
+                    ctx->sx.ncu, st, ctx->sx.watertight))
         return -1;
     if (b) CKR(hipEventRecord(b, st));
     int herr = 0;
@@ -345,7 +345,7 @@ int rt_lidar_cast_rays_noalloc(rt_ctx *ctx, const rt_scene *scene, const float *
     // castRaysSoA with the miss distance as the maximum distance (LiDAR.cpp:577-579); hits write hit_t
     // directly, misses leave it for the conversion
     if (launch_cast(scene->s.d, scene->s.h.depth, n, d_origins, d_dirs, miss_t, d_hit_t, nrm, uuid, counter, err,
-                    nullptr, ctx->sx.ncu, st))
+                    nullptr, ctx->sx.ncu, st, ctx->sx.watertight))
         return -1;
     k_lidar_convert<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, d_dirs, nrm, uuid, miss_t, d_hit_t, d_hit_fnorm,
                                                                  d_hit_id);
@@ -399,6 +399,17 @@ int rt_run_band_batch(rt_ctx *ctx, const rt_scene *scene, const rt_band_request 
     SunSet s{suns->nsuns, suns->source_dir, suns->source_flux, suns->diffuse_flux, suns->direct_seed,
              suns->diffuse_peak_dir};
     return run_solve(ctx->sx, scene->s, *base, s, R_out, out_on_device);
+}
+
+int rt_set_predicates(rt_ctx *ctx, int mode) {
+    NN(ctx, "ctx");
+    if (mode != RT_PREDICATES_REFERENCE && mode != RT_PREDICATES_WATERTIGHT) {
+        set_error("rt_set_predicates: mode must be RT_PREDICATES_REFERENCE (0) or RT_PREDICATES_WATERTIGHT (1)");
+        return -1;
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->sx.watertight = mode == RT_PREDICATES_WATERTIGHT;
+    return 0;
 }
 
 int rt_set_energy_accounting(rt_ctx *ctx, int on) {
@@ -569,7 +580,7 @@ int rt_debug_trace(rt_ctx *ctx, const rt_scene *scene, uint32_t n, const float *
         set_error("debug trace upload failed");
         rc = -1;
     }
-    if (!rc) rc = launch_debug_trace(scene->s.d, n, dr, ds, tmin, brute, dh, dt, err, st);
+    if (!rc) rc = launch_debug_trace(scene->s.d, n, dr, ds, tmin, brute, dh, dt, err, st, scene_flags(scene->s, ctx->sx));
     if (!rc && (hipMemcpyAsync(hit, dh, b_s, hipMemcpyDeviceToHost, st) != hipSuccess ||
                 hipMemcpyAsync(t, dt, b_s, hipMemcpyDeviceToHost, st) != hipSuccess ||
                 hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||

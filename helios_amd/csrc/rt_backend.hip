@@ -409,7 +409,7 @@ int rt_launch_direct_rays(rt_ctx *ctx, const rt_launch_params *p) {
     // the modular offset of a sub-range breaks for S > 1: such launches go one source at a time.
     if (p->launch_offset == 0) {
         if (launch_direct_jobs(hj.data(), jobs, S, S * p->launch_count, be.scene.h.depth, counter, err,
-                               ctx->sx.stat_ptr(0), ctx->sx.cfg[0], ctx->sx.ncu, st, be.scene.h.mdesc != nullptr))
+                               ctx->sx.stat_ptr(0), ctx->sx.cfg[0], ctx->sx.ncu, st, scene_flags(be.scene, ctx->sx)))
             return -1;
     } else {
         for (uint32_t s = 0; s < S; s++) {
@@ -418,7 +418,7 @@ int rt_launch_direct_rays(rt_ctx *ctx, const rt_launch_params *p) {
             CKR(hipMemcpyAsync(jobs + s, &j, sizeof(DirectJob), hipMemcpyHostToDevice, st));
             CKR(hipStreamSynchronize(st));
             if (launch_direct_jobs(&j, jobs + s, 1, p->launch_count, be.scene.h.depth, counter, err,
-                                   ctx->sx.stat_ptr(0), ctx->sx.cfg[0], ctx->sx.ncu, st, be.scene.h.mdesc != nullptr))
+                                   ctx->sx.stat_ptr(0), ctx->sx.cfg[0], ctx->sx.ncu, st, scene_flags(be.scene, ctx->sx)))
                 return -1;
         }
     }
@@ -542,7 +542,7 @@ int rt_launch_diffuse_rays(rt_ctx *ctx, const rt_launch_params *p) {
     k_select_bands<<<nblk(PB), 256, 0, st>>>(P, be.Bg, Bl, dmap, be.rho(), be.tau(), rl, tl);
     CKR(hipGetLastError());
     if (launch_diffuse(fjob, p->launch_count, be.scene.h.depth, counter, err, ctx->sx.stat_ptr(1), ctx->sx.cfg[1],
-                       ctx->sx.ncu, st, be.scene.h.mdesc != nullptr))
+                       ctx->sx.ncu, st, scene_flags(be.scene, ctx->sx)))
         return -1;
     if (launch_gather(gjob, 1, p->launch_count, Bl, counter, ctx->sx.cfg[2], ctx->sx.ncu, st)) return -1;
     return check_err(ctx, err);
@@ -741,7 +741,7 @@ int rt_launch_camera_rays(rt_ctx *ctx, const rt_camera_params *p) {
     CKR(hipStreamSynchronize(st));
     hipEvent_t ea = nullptr, eb = nullptr;
     if (ctx->sx.timing) { CKR(hipEventCreate(&ea)); CKR(hipEventCreate(&eb)); CKR(hipEventRecord(ea, st)); }
-    if (launch_camera(c, be.scene.h.depth, false, err, ctx->sx.stat_ptr(3), st, be.scene.h.mdesc != nullptr)) return -1;
+    if (launch_camera(c, be.scene.h.depth, false, err, ctx->sx.stat_ptr(3), st, scene_flags(be.scene, ctx->sx))) return -1;
     if (eb) {
         CKR(hipEventRecord(eb, st));
         CKR(hipEventSynchronize(eb));
@@ -794,7 +794,7 @@ int rt_launch_pixel_label_rays(rt_ctx *ctx, const rt_camera_params *p) {
     CKR(hipMemsetAsync(err, 0, sizeof(int), st));
     c.labels = (uint32_t *)be.lab;
     c.depths = (float *)((uint32_t *)be.lab + npix);
-    if (launch_camera(c, be.scene.h.depth, true, err, nullptr, st, be.scene.h.mdesc != nullptr)) return -1;
+    if (launch_camera(c, be.scene.h.depth, true, err, nullptr, st, scene_flags(be.scene, ctx->sx))) return -1;
     return check_err(ctx, err);
 }
 

@@ -112,12 +112,71 @@ RT_HD void sincos_2pi(float r, float &s, float &c) {
 // flow, which on a wave costs exec-mask bookkeeping per test.  t_out is written in every case and is
 // meaningful only when the function returns true.
 // ua, ub: the hit's d.a and d.b (patch) or barycentric u, v (triangle), for texture masks.
-template <bool CD>
+// Watertight ray / triangle test (Woop, Benthin and Wald, "Watertight Ray/Triangle Intersection", JCGT 2(1),
+// 2013): the vertices are moved into a frame where the ray runs along +z through the origin (the largest
+// direction component becomes z, x and y swapped to keep the winding, then sheared), and the 2-D edge functions
+// U, V, W of the sheared vertices decide containment.  They are computed the same way for an edge shared by two
+// triangles with the same vertices, so a ray crossing that edge hits at least one of them; an edge function
+// of exactly 0 is recomputed in double, and points on an edge count as inside.  Both faces are hit.  Returns
+// the scaled distance T and determinant det (t = T / det) and the barycentrics of a and b.
+RT_HD bool wt_tri(v3f o, v3f d, v3f a, v3f b, v3f c, float &t, float &ub, float &uc) {
+    const float ax_ = fabsf(d.x), ay_ = fabsf(d.y), az_ = fabsf(d.z);
+    int kz = ax_ > ay_ ? (ax_ > az_ ? 0 : 2) : (ay_ > az_ ? 1 : 2);
+    int kx = kz == 2 ? 0 : kz + 1, ky = kx == 2 ? 0 : kx + 1;
+    const float dz = kz == 0 ? d.x : (kz == 1 ? d.y : d.z);
+    if (dz < 0.0f) { const int tmp = kx; kx = ky; ky = tmp; }
+    auto comp = [](v3f v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); };
+    const float dx = comp(d, kx), dy = comp(d, ky);
+    const float Sx = dx / dz, Sy = dy / dz, Sz = 1.0f / dz;
+    const v3f A = sub3(a, o), B = sub3(b, o), C = sub3(c, o);
+    const float Ax = comp(A, kx) - Sx * comp(A, kz), Ay = comp(A, ky) - Sy * comp(A, kz);
+    const float Bx = comp(B, kx) - Sx * comp(B, kz), By = comp(B, ky) - Sy * comp(B, kz);
+    const float Cx = comp(C, kx) - Sx * comp(C, kz), Cy = comp(C, ky) - Sy * comp(C, kz);
+    float U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    const float det = U + V + W;
+    if (det == 0.0f) return false;
+    const float T = U * (Sz * comp(A, kz)) + V * (Sz * comp(B, kz)) + W * (Sz * comp(C, kz));
+    const float inv = 1.0f / det;
+    t = T * inv;
+    ub = V * inv;
+    uc = W * inv;
+    return true;
+}
+
+template <bool CD, bool WT = false>
 RT_HD bool isect_rec_uv(const float4 r0, const float4 r1, const float4 r2, const float4 r3, bool patch, v3f o, v3f d,
                         float &t_out, float &ua, float &ub) {
     v3f v0 = mk3(r0.x, r0.y, r0.z);
     v3f e1 = mk3(r1.x, r1.y, r1.z);
     v3f e2 = mk3(r2.x, r2.y, r2.z);
+    if (WT) {
+        // watertight predicates: a triangle (v0, v0 + e1, v0 + e2) and a patch as the triangles
+        // (v0, v0 + a, v0 + a + b) and (v0, v0 + a + b, v0 + b), closed edges, both faces; the distance window
+        // is the reference's ((1e-8, 1e8) for radiation rays, t > 1e-5 for castRays)
+        float t = 0.f, u = 0.f, v = 0.f;
+        bool hit;
+        if (patch) {
+            const v3f p1 = add3(v0, e1), p2 = add3(p1, e2), p3 = add3(v0, e2);
+            hit = wt_tri(o, d, v0, p1, p2, t, u, v);
+            if (!hit) hit = wt_tri(o, d, v0, p2, p3, t, u, v);
+            const v3f dd = sub3(add3(o, mul3(d, t)), v0);
+            u = dot3(dd, e1);
+            v = dot3(dd, e2);
+        } else {
+            hit = wt_tri(o, d, v0, add3(v0, e1), add3(v0, e2), t, u, v);
+        }
+        hit = hit && (CD ? (t > 1e-5f) : (t > 1e-8f && t < 1e8f));
+        t_out = t;
+        ua = u;
+        ub = v;
+        return hit;
+    }
     if (patch) {
         v3f n = mk3(r3.x, r3.y, r3.z);
         float denom = dot3(d, n);
@@ -155,11 +214,11 @@ RT_HD bool isect_rec_uv(const float4 r0, const float4 r1, const float4 r2, const
     }
 }
 
-template <bool CD>
+template <bool CD, bool WT = false>
 RT_HD bool isect_rec(const float4 r0, const float4 r1, const float4 r2, const float4 r3, bool patch, v3f o, v3f d,
                      float &t_out) {
     float ua, ub;
-    return isect_rec_uv<CD>(r0, r1, r2, r3, patch, o, d, t_out, ua, ub);
+    return isect_rec_uv<CD, WT>(r0, r1, r2, r3, patch, o, d, t_out, ua, ub);
 }
 
 // ---- texture masks (R/shaders/common/texture_mask.glsl:20-66) ----

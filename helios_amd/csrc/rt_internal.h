@@ -63,20 +63,21 @@ inline LaunchCfg default_launch_cfg(int kernel) {
 
 // kernel launchers (rt_trace.hip)
 // stats (may be null): [0] BVH node fetches, [1] primitive tests, [2] rays traced
-// masked: the scene carries texture masks (the kernels without mask tests run otherwise)
+// flags: traversal flags -- 1 = the scene carries texture masks (the kernels without mask tests run
+// otherwise), 2 = watertight intersection predicates (rt_set_predicates); see scene_flags()
 int launch_direct(const DirectJob *jobs_dev, uint32_t njobs, uint32_t total_items, uint32_t depth, uint32_t *counter,
-                  int *err, unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st, bool masked);
+                  int *err, unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st, int flags);
 // Non-collimated sources (k_direct_src): items [item_lo, item_lo + nitems) of jobs with ascending item_begin.
 int launch_direct_src(const DirectJob *jobs_dev, uint32_t njobs, uint32_t item_lo, uint32_t nitems, uint32_t depth,
-                      uint32_t *counter, int *err, unsigned long long *stats, int ncu, hipStream_t st, bool masked);
+                      uint32_t *counter, int *err, unsigned long long *stats, int ncu, hipStream_t st, int flags);
 // Any mix of source types: hjobs is the host copy of jobs_dev, collimated jobs first (their items precede
 // the others'); collimated jobs go to k_direct, the rest to k_direct_src.
 int launch_direct_jobs(const DirectJob *hjobs, const DirectJob *jobs_dev, uint32_t njobs, uint32_t total_items,
                        uint32_t depth, uint32_t *counter, int *err, unsigned long long *stats, const LaunchCfg &cfg,
-                       int ncu, hipStream_t st, bool masked);
+                       int ncu, hipStream_t st, int flags);
 // one diffuse job (device pointer) per launch: items [0, total_items) of that job
 int launch_diffuse(const DiffuseJob *job_dev, uint32_t total_items, uint32_t depth, uint32_t *counter, int *err,
-                   unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st, bool masked);
+                   unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st, int flags);
 // acct: the jobs carry energy-bookkeeping outputs (GatherJob.acct / .miss)
 int launch_gather(const GatherJob *jobs_dev, uint32_t njobs, uint32_t total_items, uint32_t cmax, uint32_t *counter,
                   const LaunchCfg &cfg, int ncu, hipStream_t st, bool acct = false);
@@ -84,19 +85,19 @@ int launch_gather(const GatherJob *jobs_dev, uint32_t njobs, uint32_t total_item
 // one diffuse/scattering pass traced and gathered together, and the covers' pushed shares added to the
 // radiation buffers (PC = P * C).
 int launch_direct_glass(const GlassDirectJob *jobs_dev, uint32_t njobs, uint32_t total_items, uint32_t depth,
-                        uint32_t *counter, const GlassCtx *g, int *err, int ncu, hipStream_t st, bool masked);
+                        uint32_t *counter, const GlassCtx *g, int *err, int ncu, hipStream_t st, int flags);
 int launch_diffuse_glass(const DiffuseJob *fj_dev, const GatherJob *gj_dev, uint32_t total_items, uint32_t depth,
-                         uint32_t *counter, const GlassCtx *g, int *err, int ncu, hipStream_t st, bool masked);
+                         uint32_t *counter, const GlassCtx *g, int *err, int ncu, hipStream_t st, int flags);
 int launch_add_push(size_t PC, double *push, float *in, float *st, float *sb, hipStream_t s);
 // Camera pixels (labels = false) or pixel labels/depths (labels = true) of one tile.
 int launch_camera(const CameraLaunch &c, uint32_t depth, bool labels, int *err, unsigned long long *stats,
-                  hipStream_t st, bool masked);
+                  hipStream_t st, int flags);
 // counter: device scratch for the work queue (zeroed by the launch)
 int launch_cast(const DevScene *sc, uint32_t depth, uint64_t n, const float *o, const float *d, float maxd, float *t,
                 float *nrm, uint32_t *uuid, unsigned long long *counter, int *err, unsigned long long *stats,
-                int ncu, hipStream_t st);
+                int ncu, hipStream_t st, bool watertight = false);
 int launch_debug_rays(const DevScene *sc, uint32_t P, uint32_t n, uint32_t seed, float *out, hipStream_t st);
 int launch_debug_trace(const DevScene *sc, uint32_t n, const float *rays, const uint32_t *skip, float tmin, int brute,
-                       uint32_t *hit, float *t, int *err, hipStream_t st);
+                       uint32_t *hit, float *t, int *err, hipStream_t st, int flags = 0);
 
 }  // namespace rt

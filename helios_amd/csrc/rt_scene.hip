@@ -419,6 +419,58 @@ __global__ void k_c4_write(uint32_t count, uint32_t level_base, uint32_t next_ba
     o[7] = make_float4(__uint_as_float((uint32_t)cnt), 0.f, 0.f, 0.f);
 }
 
+// 8-bit BVH4 nodes (64 B) from the float ones (128 B).  Per node: the union box of its children as the
+// origin o and a power-of-two step s_a = 2^e_a per axis with 255 s_a >= extent; a child bound is stored as
+// floor((lo - o) / s) and ceil((hi - o) / s), so the decoded box o + q s contains the float box exactly
+// (the differences and quotients are exact in double: floats differ by a double-exact amount, s is a power
+// of two).  Layout, as uint words:
+//   q0 = (o.x, o.y, o.z bits, e_x+127 | e_y+127 << 8 | e_z+127 << 16 | children << 24)
+//   q1 = (lo.x, hi.x, lo.y, hi.y), q2 = (lo.z, hi.z, link0, link1), q3 = (link2, link3, 0, 0)
+// where each bound word holds the four children's bytes, child k in bits 8k..8k+7.  Unused slots (k >=
+// children) are never tested.
+__device__ __forceinline__ int q_exponent(double ext) {
+    // smallest e in [-126, 127] with 255 * 2^e >= ext
+    if (!(ext > 0.0)) return -126;
+    int x;
+    (void)frexp(ext / 255.0, &x);  // ext / 255 = m 2^x with m in [0.5, 1): 2^x > ext / 255
+    return x < -126 ? -126 : (x > 127 ? 127 : x);
+}
+
+__global__ void k_q4(uint32_t nn4, const float4 *__restrict__ node4, uint4 *__restrict__ q4, int *__restrict__ bad) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nn4) return;
+    const float4 *o = node4 + (size_t)t * 8;
+    const uint32_t cnt = __float_as_uint(o[7].x);
+    float lo[3][4], hi[3][4];
+    for (int a = 0; a < 3; a++) {
+        const float4 l = o[2 * a], h = o[2 * a + 1];
+        lo[a][0] = l.x; lo[a][1] = l.y; lo[a][2] = l.z; lo[a][3] = l.w;
+        hi[a][0] = h.x; hi[a][1] = h.y; hi[a][2] = h.z; hi[a][3] = h.w;
+    }
+    uint32_t org[3], meta = cnt << 24, wl[3] = {0u, 0u, 0u}, wh[3] = {0u, 0u, 0u};
+    for (int a = 0; a < 3; a++) {
+        float u0 = INFINITY, u1 = -INFINITY;
+        for (uint32_t k = 0; k < cnt; k++) { u0 = fminf(u0, lo[a][k]); u1 = fmaxf(u1, hi[a][k]); }
+        const int e = q_exponent((double)u1 - (double)u0);
+        const double s = ldexp(1.0, e);
+        for (uint32_t k = 0; k < cnt; k++) {
+            const double ql = floor(((double)lo[a][k] - (double)u0) / s);
+            const double qh = ceil(((double)hi[a][k] - (double)u0) / s);
+            if (!(ql >= 0.0 && qh <= 255.0 && ql <= qh)) atomicOr(bad, 1);
+            wl[a] |= (uint32_t)fmin(fmax(ql, 0.0), 255.0) << (8 * k);
+            wh[a] |= (uint32_t)fmin(fmax(qh, 0.0), 255.0) << (8 * k);
+        }
+        org[a] = __float_as_uint(u0);
+        meta |= (uint32_t)(e + 127) << (8 * a);
+    }
+    const float4 lk = o[6];
+    uint4 *q = q4 + (size_t)t * 4;
+    q[0] = make_uint4(org[0], org[1], org[2], meta);
+    q[1] = make_uint4(wl[0], wh[0], wl[1], wh[1]);
+    q[2] = make_uint4(wl[2], wh[2], __float_as_uint(lk.x), __float_as_uint(lk.y));
+    q[3] = make_uint4(__float_as_uint(lk.z), __float_as_uint(lk.w), 0u, 0u);
+}
+
 // node layout: (c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y), (c0.lo.z, c0.hi.z, c1.lo.x, c1.hi.x),
 //              (c1.lo.y, c1.hi.y, c1.lo.z, c1.hi.z), (c0 link, c1 link, 0, 0)
 __global__ void k_emit(int n, const uint32_t *__restrict__ child, const float *__restrict__ lbox,
@@ -670,7 +722,7 @@ int scene_build(Scene &S, uint32_t P, const float *T, const uint32_t *type, cons
     S.h.smax = 0.f;
     for (int a = 0; a < 3; a++) S.h.smax = fmaxf(S.h.smax, fmaxf(fabsf(vmin[a]), fabsf(vmax[a])));
 
-    float4 *rec = nullptr, *node = nullptr, *node4 = nullptr;
+    float4 *rec = nullptr, *node = nullptr, *node4 = nullptr, *qnode4 = nullptr;
     uint32_t depth = 0, nnode4 = 0, depth4 = 0;
     if (nleaf > 0) {
         float *lbox;
@@ -713,7 +765,18 @@ int scene_build(Scene &S, uint32_t P, const float *T, const uint32_t *type, cons
             CK(hipMalloc(&node4, sizeof(float4) * 8 * (size_t)(nleaf - 1)));
             S.own.push_back(node4);
             if (build_bvh4(nleaf, child, lbox, ibox, node4, nnode4, depth4, st)) return -1;
+            CK(hipMalloc(&qnode4, sizeof(uint4) * 4 * (size_t)nnode4 + sizeof(int)));
+            S.own.push_back(qnode4);
+            int *qbad = (int *)((uint4 *)qnode4 + 4 * (size_t)nnode4);
+            CK(hipMemsetAsync(qbad, 0, sizeof(int), st));
+            k_q4<<<blocks(nnode4, 256), 256, 0, st>>>(nnode4, node4, (uint4 *)qnode4, qbad);
+            int hbad = 0;
+            CK(hipMemcpyAsync(&hbad, qbad, sizeof(int), hipMemcpyDeviceToHost, st));
             CK(hipStreamSynchronize(st));
+            if (hbad) {
+                set_error("8-bit BVH4 node quantisation failed (a child box outside its node's grid)");
+                return -1;
+            }
             CK(hipFree(child)); CK(hipFree(ibox));
         } else {
             CK(hipStreamSynchronize(st));
@@ -738,6 +801,7 @@ int scene_build(Scene &S, uint32_t P, const float *T, const uint32_t *type, cons
     S.h.order = vals2;
     // a BVH4 traversal pushes at most 3 entries per level: use it only when that fits the wave stack (128)
     S.h.node4 = (node4 && 3 * depth4 + 1 <= 128) ? node4 : nullptr;
+    S.h.qnode4 = S.h.node4 ? qnode4 : nullptr;
     S.h.nnode4 = nnode4;
     S.h.depth4 = depth4;
     S.h.nleaf = nleaf;
@@ -747,7 +811,7 @@ int scene_build(Scene &S, uint32_t P, const float *T, const uint32_t *type, cons
     CK(hipMemcpyAsync(S.d, &S.h, sizeof(DevScene), hipMemcpyHostToDevice, st));
     CK(hipStreamSynchronize(st));
     S.bytes = (size_t)P * (64 + 4 * 4 + 16 * 2) + (size_t)nleaf * 64 + (nleaf > 1 ? (size_t)(nleaf - 1) * 64 : 0) +
-              (size_t)(nleaf > 1 ? nleaf - 1 : 0) * 128;
+              (size_t)(nleaf > 1 ? nleaf - 1 : 0) * 128 + (size_t)nnode4 * 64;
     return 0;
 }
 

@@ -69,8 +69,13 @@ struct SolveCtx {
     bool last_direct_grid = false;        // the last solve's direct pass used them
     uint32_t last_grid_entries = 0;       // their list entries per direction per primitive, x100
     GridSet grids;
+    bool watertight = false;              // watertight intersection predicates (rt_set_predicates)
     unsigned long long *stat_ptr(int k) { return counting ? stats + 3 * k : nullptr; }
 };
+// traversal flags of launches on scene S from this context (RT_FL_*)
+inline int scene_flags(const Scene &S, const SolveCtx &sx) {
+    return (S.h.mdesc ? RT_FL_MASK : 0) | (sx.watertight ? RT_FL_WT : 0);
+}
 
 // The per-sun part of a solve (K suns share everything else).
 // Shape of one radiation source as the direct kernels take it.

@@ -684,10 +684,10 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
         if (a) CK(hipEventRecord(a, st));
         if (glass) {
             if (launch_direct_glass(gdjobs, (uint32_t)hgd.size(), items, S.h.depth, counter, gctx, err, sx.ncu, st,
-                                    S.h.mdesc != nullptr))
+                                    scene_flags(S, sx)))
                 return -1;
         } else if (launch_direct_jobs(hd.data(), djobs, (uint32_t)hd.size(), items, S.h.depth, counter, err,
-                                      sx.stat_ptr(0), sx.cfg[0], sx.ncu, st, S.h.mdesc != nullptr)) {
+                                      sx.stat_ptr(0), sx.cfg[0], sx.ncu, st, scene_flags(S, sx))) {
             return -1;
         }
         if (b) { CK(hipEventRecord(b, st)); tick(sx, 0, a, b); }
@@ -709,7 +709,7 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
         if (a) CK(hipEventRecord(a, st));
         if (glass) {
             if (launch_diffuse_glass(fjobs + g, gjobs + g, P, S.h.depth, counter, gctx, err, sx.ncu, st,
-                                     S.h.mdesc != nullptr))
+                                     scene_flags(S, sx)))
                 return -1;
             if (b) { CK(hipEventRecord(b, st)); tick(sx, 1, a, b); }
             sx.timing_info.diffuse_rays += (uint64_t)P * 2 * nn;
@@ -717,7 +717,7 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
             return launch_add_push(PC, gpush, in, stb, sbb, st);
         }
         if (launch_diffuse(fjobs + g, P, S.h.depth, counter, err, sx.stat_ptr(1), sx.cfg[1], sx.ncu, st,
-                           S.h.mdesc != nullptr))
+                           scene_flags(S, sx)))
             return -1;
         if (b) { CK(hipEventRecord(b, st)); tick(sx, 1, a, b); }
         sx.timing_info.diffuse_rays += (uint64_t)P * 2 * nn;
@@ -834,7 +834,7 @@ int debug_diffuse_hits(SolveCtx &sx, const Scene &S, uint32_t n, uint32_t seed, 
     fj.sc = S.d; fj.n = n; fj.seed = seed; fj.hits = hits; fj.item_begin = 0; fj.P = P;
     CK(hipMemsetAsync(err, 0, 4, st));
     CK(hipMemcpyAsync(fjob, &fj, sizeof(fj), hipMemcpyHostToDevice, st));
-    if (launch_diffuse(fjob, P, S.h.depth, counter, err, sx.stat_ptr(1), sx.cfg[1], sx.ncu, st, S.h.mdesc != nullptr))
+    if (launch_diffuse(fjob, P, S.h.depth, counter, err, sx.stat_ptr(1), sx.cfg[1], sx.ncu, st, scene_flags(S, sx)))
         return -1;
     CK(hipMemcpyAsync(out, hits, (size_t)P * 2 * nn * 4, hipMemcpyDeviceToHost, st));
     CK(hipStreamSynchronize(st));
@@ -883,7 +883,7 @@ int debug_direct_sums(SolveCtx &sx, const Scene &S, uint32_t d, uint32_t seed, u
     CK(hipMemsetAsync(err, 0, 4, st));
     CK(hipMemcpyAsync(jobs, hj.data(), nsrc * sizeof(DirectJob), hipMemcpyHostToDevice, st));
     if (launch_direct_jobs(hj.data(), jobs, nsrc, nsrc * P, S.h.depth, counter, err, sx.stat_ptr(0), sx.cfg[0], sx.ncu,
-                           st, S.h.mdesc != nullptr))
+                           st, scene_flags(S, sx)))
         return -1;
     std::vector<float> tmp((size_t)nsrc * P * 2);
     CK(hipMemcpyAsync(tmp.data(), sums, tmp.size() * 4, hipMemcpyDeviceToHost, st));

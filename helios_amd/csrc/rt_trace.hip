@@ -33,6 +33,12 @@
 #ifndef RT_DIFFUSE_BVH4
 #define RT_DIFFUSE_BVH4 0
 #endif
+#ifndef RT_DIFFUSE_QBVH4
+#define RT_DIFFUSE_QBVH4 1
+#endif
+#ifndef RT_CAST_QBVH4
+#define RT_CAST_QBVH4 1
+#endif
 #ifndef RT_BVH4_SORT
 #define RT_BVH4_SORT 1
 #endif
@@ -42,27 +48,32 @@ namespace rt {
 __device__ __forceinline__ float min3f(float a, float b, float c) { return fminf(fminf(a, b), c); }
 __device__ __forceinline__ float max3f(float a, float b, float c) { return fmaxf(fmaxf(a, b), c); }
 
-// traversal work counters (node fetches, primitive tests) of one lane.  The counter type also carries
-// whether the traversal honours texture masks (mask = false compiles the mask tests away for scenes
-// without masks, keeping the leaf code and its register use as small as before masks existed).
-template <bool MK>
+// traversal work counters (node fetches, primitive tests) of one lane.  The counter type also carries the
+// traversal's flags MK: bit 0 = honour texture masks (off compiles the mask tests away for scenes without
+// masks, keeping the leaf code and its register use as small as before masks existed), bit 1 = watertight
+// intersection predicates instead of the reference's (rt_set_predicates).
+template <int MK>
 struct TCntT {
     uint32_t nodes, prims;
-    static constexpr bool mask = MK;
+    static constexpr bool mask = (MK & RT_FL_MASK) != 0;
+    static constexpr bool wt = (MK & RT_FL_WT) != 0;
+    static constexpr int flags = MK;
 };
 // the same interface with every update compiled away (timed kernels)
 struct NopCounter {
     __device__ __forceinline__ NopCounter operator++(int) { return *this; }
     __device__ __forceinline__ NopCounter &operator+=(uint32_t) { return *this; }
 };
-template <bool MK>
+template <int MK>
 struct NoCntT {
     NopCounter nodes, prims;
-    static constexpr bool mask = MK;
+    static constexpr bool mask = (MK & RT_FL_MASK) != 0;
+    static constexpr bool wt = (MK & RT_FL_WT) != 0;
+    static constexpr int flags = MK;
 };
-using TCnt = TCntT<true>;
-using NoCnt = NoCntT<true>;
-template <bool COUNT, bool MK>
+using TCnt = TCntT<RT_FL_MASK>;
+using NoCnt = NoCntT<RT_FL_MASK>;
+template <bool COUNT, int MK>
 using Cnt = typename std::conditional<COUNT, TCntT<MK>, NoCntT<MK>>::type;
 
 struct RayPre {
@@ -115,7 +126,7 @@ __device__ __forceinline__ T ldu(const T *p) {
 }
 
 // Intersect the primitive in leaf slot k; returns its index (RT_MISS if skipped) and the hit.
-template <bool CD, bool MK = true>
+template <bool CD, int MK = RT_FL_MASK>
 __device__ __forceinline__ bool leaf_hit_p(const float4 *recs, uint32_t k, v3f o, v3f d, uint32_t skip, uint32_t &id,
                                            float &t) {
     const float4 *r = recs + (size_t)k * 4;
@@ -123,23 +134,23 @@ __device__ __forceinline__ bool leaf_hit_p(const float4 *recs, uint32_t k, v3f o
     uint32_t tag = __float_as_uint(r0.w);
     id = tag & RT_ID_BITS;
     bool patch = (tag & RT_PATCH_BIT) != 0;
-    const bool masked = MK && !CD && (tag & RT_MASK_BIT) != 0;
+    const bool masked = (MK & RT_FL_MASK) && !CD && (tag & RT_MASK_BIT) != 0;
     float4 r1 = ldc(r + 1), r2 = ldc(r + 2);
     float4 r3 = (patch || masked) ? ldc(r + 3) : make_float4(0.f, 0.f, 0.f, 0.f);
     float ua, ub;
-    bool hit = isect_rec_uv<CD>(r0, r1, r2, r3, patch, o, d, t, ua, ub) && id != skip;
+    bool hit = isect_rec_uv<CD, (MK & RT_FL_WT) != 0>(r0, r1, r2, r3, patch, o, d, t, ua, ub) && id != skip;
     // texture mask (bvh_traversal.glsl:143-148): a transparent texel lets the ray pass
     if (masked && hit) hit = mask_hit_opaque(recs, __float_as_uint(r3.w), patch, ua, ub, r1.w, r2.w);
     return hit;
 }
 
-template <bool CD, bool MK = true>
+template <bool CD, int MK = RT_FL_MASK>
 __device__ __forceinline__ bool leaf_hit(const DevScene &sc, uint32_t k, v3f o, v3f d, uint32_t skip, uint32_t &id,
                                          float &t) {
     return leaf_hit_p<CD, MK>(sc.rec, k, o, d, skip, id, t);
 }
 
-template <bool CD, bool MK = true>
+template <bool CD, int MK = RT_FL_MASK>
 __device__ __forceinline__ void test_leaf(const float4 *recs, uint32_t k, v3f o, v3f d, float tmin, uint32_t skip,
                                           float &best_t, uint32_t &best, uint32_t &best_slot) {
     uint32_t id;
@@ -202,7 +213,7 @@ __device__ uint32_t trace_closest(const DevScene &sc, v3f o, v3f d, float tmin, 
     const float4 *nodes = sc.node, *recs = sc.rec;
     const uint32_t nleaf = sc.nleaf;
     if (nleaf <= 1) {
-        if (nleaf == 1) { cnt.prims++; test_leaf<CD, C::mask>(recs, 0, o, d, tmin, skip, best_t, best, best_slot); }
+        if (nleaf == 1) { cnt.prims++; test_leaf<CD, C::flags>(recs, 0, o, d, tmin, skip, best_t, best, best_slot); }
         t_out = best_t;
         slot_out = best_slot;
         return best;
@@ -220,8 +231,8 @@ __device__ uint32_t trace_closest(const DevScene &sc, v3f o, v3f d, float tmin, 
         cnt.nodes++;
         NodeTest nt = node_test_p(nodes, ni, rp, best_t);
         bool l0 = (nt.c0 & RT_LEAF_BIT) != 0, l1 = (nt.c1 & RT_LEAF_BIT) != 0;
-        if (nt.h0 && l0) { cnt.prims++; test_leaf<CD, C::mask>(recs, nt.c0 & ~RT_LEAF_BIT, o, d, tmin, skip, best_t, best, best_slot); }
-        if (nt.h1 && l1) { cnt.prims++; test_leaf<CD, C::mask>(recs, nt.c1 & ~RT_LEAF_BIT, o, d, tmin, skip, best_t, best, best_slot); }
+        if (nt.h0 && l0) { cnt.prims++; test_leaf<CD, C::flags>(recs, nt.c0 & ~RT_LEAF_BIT, o, d, tmin, skip, best_t, best, best_slot); }
+        if (nt.h1 && l1) { cnt.prims++; test_leaf<CD, C::flags>(recs, nt.c1 & ~RT_LEAF_BIT, o, d, tmin, skip, best_t, best, best_slot); }
         bool g0 = nt.h0 && !l0, g1 = nt.h1 && !l1;
         if (g0 && g1) {
             bool first0 = nt.tn0 <= nt.tn1;
@@ -258,7 +269,7 @@ __device__ bool trace_any(const DevScene &sc, v3f o, v3f d, float tmin, uint32_t
     if (nleaf <= 1) {
         if (nleaf == 1) {
             cnt.prims++;
-            return leaf_hit_p<false, C::mask>(recs, 0, o, d, skip, id, t) && t > tmin && t <= tmax;
+            return leaf_hit_p<false, C::flags>(recs, 0, o, d, skip, id, t) && t > tmin && t <= tmax;
         }
         return false;
     }
@@ -276,11 +287,11 @@ __device__ bool trace_any(const DevScene &sc, v3f o, v3f d, float tmin, uint32_t
         bool l0 = (nt.c0 & RT_LEAF_BIT) != 0, l1 = (nt.c1 & RT_LEAF_BIT) != 0;
         if (nt.h0 && l0) {
             cnt.prims++;
-            if (leaf_hit_p<false, C::mask>(recs, nt.c0 & ~RT_LEAF_BIT, o, d, skip, id, t) && t > tmin && t <= tmax) return true;
+            if (leaf_hit_p<false, C::flags>(recs, nt.c0 & ~RT_LEAF_BIT, o, d, skip, id, t) && t > tmin && t <= tmax) return true;
         }
         if (nt.h1 && l1) {
             cnt.prims++;
-            if (leaf_hit_p<false, C::mask>(recs, nt.c1 & ~RT_LEAF_BIT, o, d, skip, id, t) && t > tmin && t <= tmax) return true;
+            if (leaf_hit_p<false, C::flags>(recs, nt.c1 & ~RT_LEAF_BIT, o, d, skip, id, t) && t > tmin && t <= tmax) return true;
         }
         bool g0 = nt.h0 && !l0, g1 = nt.h1 && !l1;
         if (g0 && g1) {
@@ -318,7 +329,7 @@ __device__ bool packet_any(const DevScene &sc, v3f o, v3f d, float tmin, uint32_
     if (sc.nleaf <= 1) {
         if (sc.nleaf == 1 && act) {
             cnt.prims++;
-            found = leaf_hit<false, C::mask>(sc, 0, o, d, skip, id, t) && t > tmin && t <= tmax;
+            found = leaf_hit<false, C::flags>(sc, 0, o, d, skip, id, t) && t > tmin && t <= tmax;
         }
         return found;
     }
@@ -343,14 +354,14 @@ __device__ bool packet_any(const DevScene &sc, v3f o, v3f d, float tmin, uint32_
         bool h0 = act && nt.h0, h1 = act && nt.h1;
         if (l0 && h0) {
             cnt.prims++;
-            if (leaf_hit_p<false, C::mask>(recs, nt.c0 & ~RT_LEAF_BIT, o, d, skip, id, t) && t > tmin && t <= tmax) {
+            if (leaf_hit_p<false, C::flags>(recs, nt.c0 & ~RT_LEAF_BIT, o, d, skip, id, t) && t > tmin && t <= tmax) {
                 found = true;
                 act = false;
             }
         }
         if (l1 && h1 && act) {
             cnt.prims++;
-            if (leaf_hit_p<false, C::mask>(recs, nt.c1 & ~RT_LEAF_BIT, o, d, skip, id, t) && t > tmin && t <= tmax) {
+            if (leaf_hit_p<false, C::flags>(recs, nt.c1 & ~RT_LEAF_BIT, o, d, skip, id, t) && t > tmin && t <= tmax) {
                 found = true;
                 act = false;
             }
@@ -418,7 +429,7 @@ __device__ bool packet_any4(const DevScene &sc, v3f o, v3f d, float tmin, uint32
     if (sc.nleaf <= 1) {
         if (sc.nleaf == 1 && act) {
             cnt.prims++;
-            found = leaf_hit<false, C::mask>(sc, 0, o, d, skip, id, t) && t > tmin && t <= tmax;
+            found = leaf_hit<false, C::flags>(sc, 0, o, d, skip, id, t) && t > tmin && t <= tmax;
         }
         return found;
     }
@@ -457,7 +468,7 @@ __device__ bool packet_any4(const DevScene &sc, v3f o, v3f d, float tmin, uint32
                 const bool want = h[k] && act;
                 if (__ballot(want) != 0) {
                     cnt.prims += want ? 1u : 0u;
-                    const bool hit = leaf_hit_p<false, C::mask>(recs, lnk[k] & ~RT_LEAF_BIT, o, d, skip, id, t) && t > tmin &&
+                    const bool hit = leaf_hit_p<false, C::flags>(recs, lnk[k] & ~RT_LEAF_BIT, o, d, skip, id, t) && t > tmin &&
                                      t <= tmax;
                     const bool take = want && hit;
                     found = found || take;
@@ -636,9 +647,9 @@ __device__ __forceinline__ void wave_reduce(ksum &k) {
 
 __device__ __forceinline__ bool traced_type(uint32_t t) { return t == 0 || t == 1 || t == 3; }
 
-template <bool MK>
+template <int MK>
 __device__ __forceinline__ void add_stats(unsigned long long *, const NoCntT<MK> &, uint32_t) {}
-template <bool MK>
+template <int MK>
 __device__ __forceinline__ void add_stats(unsigned long long *stats, const TCntT<MK> &c, uint32_t rays) {
     if (stats) {
         atomicAdd(&stats[0], (unsigned long long)c.nodes);
@@ -686,9 +697,9 @@ __device__ __forceinline__ int grid_pair(const uint32_t *items, const float *key
     const float4 b0 = ldc(q1), b1 = ldc(q1 + 1), b2 = ldc(q1 + 2), b3 = ldc(q1 + 3);
     const uint32_t ta = __float_as_uint(a0.w), tb = __float_as_uint(b0.w);
     float t0, t1, ua0, ub0, ua1, ub1;
-    bool h0 = v0 && isect_rec_uv<false>(a0, a1, a2, a3, (ta & RT_PATCH_BIT) != 0, o, d, t0, ua0, ub0) &&
+    bool h0 = v0 && isect_rec_uv<false, C::wt>(a0, a1, a2, a3, (ta & RT_PATCH_BIT) != 0, o, d, t0, ua0, ub0) &&
               (ta & RT_ID_BITS) != skip && t0 > 1e-4f && t0 <= 1e30f;
-    bool h1 = v1 && isect_rec_uv<false>(b0, b1, b2, b3, (tb & RT_PATCH_BIT) != 0, o, d, t1, ua1, ub1) &&
+    bool h1 = v1 && isect_rec_uv<false, C::wt>(b0, b1, b2, b3, (tb & RT_PATCH_BIT) != 0, o, d, t1, ua1, ub1) &&
               (tb & RT_ID_BITS) != skip && t1 > 1e-4f && t1 <= 1e30f;
     if (C::mask && h0 && (ta & RT_MASK_BIT))
         h0 = mask_hit_opaque(recs, __float_as_uint(a3.w), (ta & RT_PATCH_BIT) != 0, ua0, ub0, a1.w, a2.w);
@@ -717,7 +728,7 @@ __device__ __forceinline__ bool direct_origin(const float4 *recs, uint32_t mdi, 
     return mdi == RT_MISS || origin_mask(recs, mdi, patch, false, st, spx, spy);
 }
 
-template <int WPB, int STK, bool COUNT, bool MK>
+template <int WPB, int STK, bool COUNT, int MK>
 __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict__ jobs, uint32_t njobs, uint32_t total,
                                                      uint32_t *counter, uint32_t chunk, int *err,
                                                      unsigned long long *stats) {
@@ -877,7 +888,7 @@ __device__ __forceinline__ bool source_occluded(const DevScene &sc, v3f o, v3f d
 // Direct rays towards sphere, sun-sphere, rectangle and disk sources (direct_raygen.comp:482-561): one
 // wave per (job, primitive), d*d stratified origins as k_direct, per-ray direction, strength and lit face,
 // per-lane closest-hit traversal.  Items [item_lo, item_lo + nitems) of jobs whose item_begin ascend.
-template <int STK, bool COUNT, bool MK>
+template <int STK, bool COUNT, int MK>
 __global__ __launch_bounds__(256) void k_direct_src(const DirectJob *__restrict__ jobs, uint32_t njobs, uint32_t item_lo,
                                                     uint32_t nitems, uint32_t *counter, int *err,
                                                     unsigned long long *stats) {
@@ -966,7 +977,7 @@ __device__ __forceinline__ bool lane_begin(LaneRay &L, const float4 *recs, uint3
     lane_start(L, nleaf);
     if (nleaf > 1) return true;
     uint32_t slot_unused;
-    if (nleaf == 1) { cnt.prims++; test_leaf<false, C::mask>(recs, 0, L.o, L.d, tmin, skip, L.best_t, L.best, slot_unused); }
+    if (nleaf == 1) { cnt.prims++; test_leaf<false, C::flags>(recs, 0, L.o, L.d, tmin, skip, L.best_t, L.best, slot_unused); }
     return false;
 }
 
@@ -1022,34 +1033,12 @@ __device__ __forceinline__ bool lane_node(const float4 *nodes, uint32_t nleaf, L
     return false;
 }
 
-// Visit the BVH4 node L.ni: four child boxes against [0, best_t]; the hit children (inner nodes or leaves)
-// in ascending entry distance -- the nearest becomes the next entry, the others are pushed farthest first.
-// Unused slots are +inf point boxes that never pass the test.  Returns true when the traversal has ended.
-template <int STK, int LSTK, typename C>
-__device__ __forceinline__ bool lane_node4(const float4 *nodes4, uint32_t nn4, LaneRay &L, lds_u32 *stk, uint32_t *ovf,
-                                           uint32_t stride, int *err, C &cnt) {
-    if (L.budget-- == 0 || L.ni >= nn4) {
-        atomicOr(err, 2);
-        return true;
-    }
-    cnt.nodes++;
-    const float4 *nd = nodes4 + (size_t)L.ni * 8;
-    const float4 lx = ldc(nd), hx = ldc(nd + 1), ly = ldc(nd + 2), hy = ldc(nd + 3);
-    const float4 lz = ldc(nd + 4), hz = ldc(nd + 5), lk = ldc(nd + 6);
-    float tn[4], tf[4];
-    slab4(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, L.rp, tn[0], tf[0]);
-    slab4(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, L.rp, tn[1], tf[1]);
-    slab4(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, L.rp, tn[2], tf[2]);
-    slab4(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, L.rp, tn[3], tf[3]);
-    float key[4];
-    uint32_t lnk[4] = {__float_as_uint(lk.x), __float_as_uint(lk.y), __float_as_uint(lk.z), __float_as_uint(lk.w)};
-    uint32_t nhit = 0;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const bool h = fmaxf(tn[k], 0.f) <= fminf(tf[k], L.best_t);
-        key[k] = h ? tn[k] : INFINITY;
-        nhit += h ? 1u : 0u;
-    }
+// The hit children of a BVH4 visit (key = entry distance, +inf = missed): the nearest becomes the next
+// entry and the others are pushed (farthest first with RT_BVH4_SORT, else in slot order); the stack empty
+// after no hit ends the traversal (returns true), as does a stack overflow (error bit 1).
+template <int STK, int LSTK>
+__device__ __forceinline__ bool node4_order(LaneRay &L, float *key, uint32_t *lnk, uint32_t nhit, lds_u32 *stk,
+                                            uint32_t *ovf, uint32_t stride, int *err) {
     if (nhit == 0) return lane_pop<STK, LSTK>(L, stk, ovf, stride);
 #if RT_BVH4_SORT
     // ascending sort of the four (key, link) pairs; missed children (key +inf) end up last
@@ -1098,13 +1087,88 @@ __device__ __forceinline__ bool lane_node4(const float4 *nodes4, uint32_t nn4, L
 #endif
 }
 
-// Test the leaf entry L.ni (closest accepted hit, ties to the lower primitive index), then pop.// Test the leaf entry L.ni (closest accepted hit, ties to the lower primitive index), then pop.
+// Visit the BVH4 node L.ni: four child boxes against [0, best_t]; the hit children (inner nodes or leaves)
+// in ascending entry distance -- the nearest becomes the next entry, the others are pushed farthest first.
+// Unused slots are +inf point boxes that never pass the test.  Returns true when the traversal has ended.
+template <int STK, int LSTK, typename C>
+__device__ __forceinline__ bool lane_node4(const float4 *nodes4, uint32_t nn4, LaneRay &L, lds_u32 *stk, uint32_t *ovf,
+                                           uint32_t stride, int *err, C &cnt) {
+    if (L.budget-- == 0 || L.ni >= nn4) {
+        atomicOr(err, 2);
+        return true;
+    }
+    cnt.nodes++;
+    const float4 *nd = nodes4 + (size_t)L.ni * 8;
+    const float4 lx = ldc(nd), hx = ldc(nd + 1), ly = ldc(nd + 2), hy = ldc(nd + 3);
+    const float4 lz = ldc(nd + 4), hz = ldc(nd + 5), lk = ldc(nd + 6);
+    float tn[4], tf[4];
+    slab4(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, L.rp, tn[0], tf[0]);
+    slab4(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, L.rp, tn[1], tf[1]);
+    slab4(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, L.rp, tn[2], tf[2]);
+    slab4(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, L.rp, tn[3], tf[3]);
+    float key[4];
+    uint32_t lnk[4] = {__float_as_uint(lk.x), __float_as_uint(lk.y), __float_as_uint(lk.z), __float_as_uint(lk.w)};
+    uint32_t nhit = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const bool h = fmaxf(tn[k], 0.f) <= fminf(tf[k], L.best_t);
+        key[k] = h ? tn[k] : INFINITY;
+        nhit += h ? 1u : 0u;
+    }
+    return node4_order<STK, LSTK>(L, key, lnk, nhit, stk, ovf, stride, err);
+}
+
+// Visit the 8-bit BVH4 node L.ni (64 B: see k_q4 in rt_scene.hip).  A child bound q decodes in t-space as
+// (o + q 2^e - ray origin) / dir = q (2^e / dir) + (o - ray origin) / dir: one convert and one fma per bound
+// after a multiply and an fma per axis.  The decoded boxes contain the float boxes, and the t rounding is a
+// few ulps of the scene scale, far inside the leaf boxes' padding, so no ray a float box passes is culled
+// and the hits are those of the float trees.
+__device__ __forceinline__ float qbyte(uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xFFu); }
+
+template <int STK, int LSTK, typename C>
+__device__ __forceinline__ bool lane_node4q(const float4 *q4, uint32_t nn4, LaneRay &L, lds_u32 *stk, uint32_t *ovf,
+                                            uint32_t stride, int *err, C &cnt) {
+    if (L.budget-- == 0 || L.ni >= nn4) {
+        atomicOr(err, 2);
+        return true;
+    }
+    cnt.nodes++;
+    const float4 *nd = q4 + (size_t)L.ni * 4;
+    const float4 f0 = ldc(nd), f1 = ldc(nd + 1), f2 = ldc(nd + 2), f3 = ldc(nd + 3);
+    const uint32_t meta = __float_as_uint(f0.w);
+    const uint32_t nch = meta >> 24;
+    // 2^e as a float from the biased exponent byte (e + 127 in [1, 254])
+    const float sx = __uint_as_float((meta & 0xFFu) << 23), sy = __uint_as_float(((meta >> 8) & 0xFFu) << 23),
+                sz = __uint_as_float(((meta >> 16) & 0xFFu) << 23);
+    const float Ax = sx * L.rp.ix, Ay = sy * L.rp.iy, Az = sz * L.rp.iz;
+    const float Bx = fmaf(f0.x, L.rp.ix, -L.rp.ox), By = fmaf(f0.y, L.rp.iy, -L.rp.oy),
+                Bz = fmaf(f0.z, L.rp.iz, -L.rp.oz);
+    const uint32_t wlx = __float_as_uint(f1.x), whx = __float_as_uint(f1.y), wly = __float_as_uint(f1.z),
+                   why = __float_as_uint(f1.w), wlz = __float_as_uint(f2.x), whz = __float_as_uint(f2.y);
+    float key[4];
+    uint32_t lnk[4] = {__float_as_uint(f2.z), __float_as_uint(f2.w), __float_as_uint(f3.x), __float_as_uint(f3.y)};
+    uint32_t nhit = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const float lx = fmaf(qbyte(wlx, k), Ax, Bx), hx = fmaf(qbyte(whx, k), Ax, Bx);
+        const float ly = fmaf(qbyte(wly, k), Ay, By), hy = fmaf(qbyte(why, k), Ay, By);
+        const float lz = fmaf(qbyte(wlz, k), Az, Bz), hz = fmaf(qbyte(whz, k), Az, Bz);
+        const float tn = max3f(fminf(lx, hx), fminf(ly, hy), fminf(lz, hz));
+        const float tf = min3f(fmaxf(lx, hx), fmaxf(ly, hy), fmaxf(lz, hz));
+        const bool h = (uint32_t)k < nch && fmaxf(tn, 0.f) <= fminf(tf, L.best_t);
+        key[k] = h ? tn : INFINITY;
+        nhit += h ? 1u : 0u;
+    }
+    return node4_order<STK, LSTK>(L, key, lnk, nhit, stk, ovf, stride, err);
+}
+
+// Test the leaf entry L.ni (closest accepted hit, ties to the lower primitive index), then pop.
 template <int STK, int LSTK, typename C>
 __device__ __forceinline__ bool lane_leaf(const float4 *recs, LaneRay &L, float tmin, uint32_t skip,
                                           const lds_u32 *stk, const uint32_t *ovf, uint32_t stride, C &cnt) {
     cnt.prims++;
     uint32_t slot_unused;
-    test_leaf<false, C::mask>(recs, L.ni & ~RT_LEAF_BIT, L.o, L.d, tmin, skip, L.best_t, L.best, slot_unused);
+    test_leaf<false, C::flags>(recs, L.ni & ~RT_LEAF_BIT, L.o, L.d, tmin, skip, L.best_t, L.best, slot_unused);
     return lane_pop<STK, LSTK>(L, stk, ovf, stride);
 }
 
@@ -1132,7 +1196,7 @@ __device__ __forceinline__ bool lane_leaf(const float4 *recs, LaneRay &L, float 
 #ifndef RT_DIFFUSE_LDS_STACK
 #define RT_DIFFUSE_LDS_STACK 16
 #endif
-template <int WPB, int STK, bool COUNT, bool MK>
+template <int WPB, int STK, bool COUNT, int MK>
 __global__ __launch_bounds__(WPB * 64) void k_diffuse(const DiffuseJob *__restrict__ job, uint32_t total,
                                                       uint32_t *counter, uint32_t chunk, int *err,
                                                       unsigned long long *stats) {
@@ -1154,6 +1218,9 @@ __global__ __launch_bounds__(WPB * 64) void k_diffuse(const DiffuseJob *__restri
     const float4 *nodes4 = ldu(&sc.node4);
     const uint32_t nn4 = ldu(&sc.nnode4);
     const bool use4 = RT_DIFFUSE_BVH4 && !COUNT && nodes4 != nullptr && 3u * ldu(&sc.depth4) + 1u <= (uint32_t)STK;
+    // the 8-bit 4-wide tree (64 B per node): half the node fetches of the binary tree per ray
+    const float4 *q4 = ldu(&sc.qnode4);
+    const bool useq = RT_DIFFUSE_QBVH4 && !COUNT && q4 != nullptr && 3u * ldu(&sc.depth4) + 1u <= (uint32_t)STK;
     const float *T = ldu(&sc.T);
     const uint32_t *type = ldu(&sc.type);
     const uint32_t *mdesc = ldu(&sc.mdesc);
@@ -1278,8 +1345,9 @@ __global__ __launch_bounds__(WPB * 64) void k_diffuse(const DiffuseJob *__restri
                 const bool inner = trav && !(L.ni & RT_LEAF_BIT);
                 const uint64_t mi = __ballot(inner);
                 if (mi == 0 || (uint32_t)__popcll(__ballot(trav && !inner)) >= RT_LEAF_BATCH) break;
-                if (inner && (use4 ? lane_node4<STK, LSTK>(nodes4, nn4, L, stk, ovf, stride, err, cnt)
-                                   : lane_node<STK, LSTK>(nodes, nleaf, L, stk, ovf, stride, err, cnt)))
+                if (inner && (useq   ? lane_node4q<STK, LSTK>(q4, nn4, L, stk, ovf, stride, err, cnt)
+                              : use4 ? lane_node4<STK, LSTK>(nodes4, nn4, L, stk, ovf, stride, err, cnt)
+                                     : lane_node<STK, LSTK>(nodes, nleaf, L, stk, ovf, stride, err, cnt)))
                     trav = false;
             }
             while (true) {
@@ -1533,7 +1601,7 @@ template <int STK, int LSTK, typename C>
 __device__ __forceinline__ bool lane_leaf_cd(const float4 *recs, LaneRay &L, uint32_t &slot, const lds_u32 *stk,
                                              const uint32_t *ovf, uint32_t stride, C &cnt) {
     cnt.prims++;
-    test_leaf<true>(recs, L.ni & ~RT_LEAF_BIT, L.o, L.d, 1e-6f, RT_MISS, L.best_t, L.best, slot);
+    test_leaf<true, C::flags>(recs, L.ni & ~RT_LEAF_BIT, L.o, L.d, 1e-6f, RT_MISS, L.best_t, L.best, slot);
     return lane_pop<STK, LSTK>(L, stk, ovf, stride);
 }
 
@@ -1550,7 +1618,7 @@ __device__ __forceinline__ unsigned long long wave_chunk64(unsigned long long *c
 // the CD predicates.  Persistent waves take 64-ray chunks from a counter and keep 64 rays in flight: a lane
 // whose trace ends writes its result and takes the chunk's next ray; traversal runs in node and leaf phases
 // as k_diffuse's.  Each ray is traced exactly as on its own.
-template <int STK, bool COUNT>
+template <int STK, bool COUNT, int MK>
 __global__ __launch_bounds__(256) void k_cast(const DevScene *__restrict__ scp, uint64_t n, const float *__restrict__ O,
                                               const float *__restrict__ D, float maxd, float *__restrict__ T,
                                               float *__restrict__ N, uint32_t *__restrict__ U,
@@ -1565,7 +1633,12 @@ __global__ __launch_bounds__(256) void k_cast(const DevScene *__restrict__ scp, 
     const uint32_t *uuid = ldu(&sc.uuid);
     const uint32_t nleaf = ldu(&sc.nleaf);
     const float tstart = maxd > 0.f ? maxd : FLT_MAX;
-    Cnt<COUNT, true> cnt{};
+    // the 8-bit 4-wide tree when present and its worst-case stack fits; the counting variant walks the
+    // binary tree, whose node visits are the canonical work measure
+    const float4 *q4 = ldu(&sc.qnode4);
+    const uint32_t nn4 = ldu(&sc.nnode4);
+    const bool useq = RT_CAST_QBVH4 && !COUNT && q4 != nullptr && 3u * ldu(&sc.depth4) + 1u <= (uint32_t)STK;
+    Cnt<COUNT, MK> cnt{};
     uint32_t nray = 0;
     unsigned long long q_next = 0, q_end = 0, my_r = 0;
     bool drained = false, busy = false, trav = false;
@@ -1635,7 +1708,7 @@ __global__ __launch_bounds__(256) void k_cast(const DevScene *__restrict__ scp, 
                 L.best_t = tstart;
                 slot = RT_MISS;
                 trav = nleaf > 1;
-                if (nleaf == 1) { cnt.prims++; test_leaf<true>(recs, 0, L.o, L.d, 1e-6f, RT_MISS, L.best_t, L.best, slot); }
+                if (nleaf == 1) { cnt.prims++; test_leaf<true, MK>(recs, 0, L.o, L.d, 1e-6f, RT_MISS, L.best_t, L.best, slot); }
                 my_r = r;
                 busy = true;
                 nray++;
@@ -1649,7 +1722,9 @@ __global__ __launch_bounds__(256) void k_cast(const DevScene *__restrict__ scp, 
                 const bool inner = trav && !(L.ni & RT_LEAF_BIT);
                 const uint64_t mi = __ballot(inner);
                 if (mi == 0 || (uint32_t)__popcll(__ballot(trav && !inner)) >= RT_LEAF_BATCH) break;
-                if (inner && lane_node<STK, LSTK>(nodes, nleaf, L, stk, ovf, stride, err, cnt)) trav = false;
+                if (inner && (useq ? lane_node4q<STK, LSTK>(q4, nn4, L, stk, ovf, stride, err, cnt)
+                                   : lane_node<STK, LSTK>(nodes, nleaf, L, stk, ovf, stride, err, cnt)))
+                    trav = false;
             }
             while (true) {
                 const bool leaf = trav && (L.ni & RT_LEAF_BIT);
@@ -1677,7 +1752,7 @@ __device__ __forceinline__ uint32_t trace_closest_hyb(const DevScene &sc, v3f o,
     const float4 *nodes = sc.node, *recs = sc.rec;
     const uint32_t nleaf = sc.nleaf;
     if (nleaf <= 1) {
-        if (nleaf == 1) { cnt.prims++; test_leaf<false, C::mask>(recs, 0, o, d, tmin, RT_MISS, best_t, best, best_slot); }
+        if (nleaf == 1) { cnt.prims++; test_leaf<false, C::flags>(recs, 0, o, d, tmin, RT_MISS, best_t, best, best_slot); }
         t_out = best_t;
         return best;
     }
@@ -1693,8 +1768,8 @@ __device__ __forceinline__ uint32_t trace_closest_hyb(const DevScene &sc, v3f o,
         cnt.nodes++;
         NodeTest nt = node_test_p(nodes, ni, rp, best_t);
         bool l0 = (nt.c0 & RT_LEAF_BIT) != 0, l1 = (nt.c1 & RT_LEAF_BIT) != 0;
-        if (nt.h0 && l0) { cnt.prims++; test_leaf<false, C::mask>(recs, nt.c0 & ~RT_LEAF_BIT, o, d, tmin, RT_MISS, best_t, best, best_slot); }
-        if (nt.h1 && l1) { cnt.prims++; test_leaf<false, C::mask>(recs, nt.c1 & ~RT_LEAF_BIT, o, d, tmin, RT_MISS, best_t, best, best_slot); }
+        if (nt.h0 && l0) { cnt.prims++; test_leaf<false, C::flags>(recs, nt.c0 & ~RT_LEAF_BIT, o, d, tmin, RT_MISS, best_t, best, best_slot); }
+        if (nt.h1 && l1) { cnt.prims++; test_leaf<false, C::flags>(recs, nt.c1 & ~RT_LEAF_BIT, o, d, tmin, RT_MISS, best_t, best, best_slot); }
         bool g0 = nt.h0 && !l0, g1 = nt.h1 && !l1;
         if (g0 && g1) {
             bool first0 = nt.tn0 <= nt.tn1;
@@ -1793,7 +1868,7 @@ __device__ __forceinline__ float cam_source_radiance(const CameraLaunch &c, v3f 
 // Camera pixels: one thread per pixel of the tile traces its antialiasing samples in order and adds
 // their contributions to camera_radiation[b][pixel] (a fixed order, so a render is reproducible; the
 // reference adds the samples with float atomics).  No specular term (this backend rejects it).
-template <int STK, bool COUNT, bool MK>
+template <int STK, bool COUNT, int MK>
 __global__ __launch_bounds__(256) void k_camera(CameraLaunch c, int *err, unsigned long long *stats) {
     constexpr int LSTK = STK < RT_CAMERA_LDS_STACK ? STK : RT_CAMERA_LDS_STACK;
     __shared__ uint32_t stk_all[LSTK * 256];
@@ -1847,7 +1922,7 @@ __global__ __launch_bounds__(256) void k_camera(CameraLaunch c, int *err, unsign
 #ifndef RT_CAMERA_REFILL
 #define RT_CAMERA_REFILL 0
 #endif
-template <int STK, bool COUNT, bool MK>
+template <int STK, bool COUNT, int MK>
 __global__ __launch_bounds__(256) void k_camera_r(CameraLaunch c, int *err, unsigned long long *stats) {
     constexpr int LSTK = STK < RT_CAMERA_LDS_STACK ? STK : RT_CAMERA_LDS_STACK;
     __shared__ uint32_t stk_all[LSTK * 256];
@@ -1947,7 +2022,7 @@ __global__ __launch_bounds__(256) void k_camera_r(CameraLaunch c, int *err, unsi
 
 // Pixel labels (pixel_label_raygen.comp:80-155): the pixel-centre pinhole ray's hit UUID + 1 (0 = sky) and
 // its depth along the view direction (-1 on a miss).
-template <int STK, bool MK>
+template <int STK, int MK>
 __global__ __launch_bounds__(256) void k_pixel_label(CameraLaunch c, int *err) {
     constexpr int LSTK = STK < RT_CAMERA_LDS_STACK ? STK : RT_CAMERA_LDS_STACK;
     __shared__ uint32_t stk_all[LSTK * 256];
@@ -1977,6 +2052,9 @@ __global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
 
 __global__ void k_zero_u32(uint32_t *p) { *p = 0u; }
 
+template <typename F>
+static void with_flags(int fl, F &&f);
+
 // ------------------------------------------------------------------------------------------------
 // parity hooks: generated diffuse rays, and radiation-predicate traces of arbitrary rays
 // ------------------------------------------------------------------------------------------------
@@ -2005,12 +2083,13 @@ __global__ void k_debug_rays(const DevScene *__restrict__ scp, uint32_t n, uint3
     w[0] = o.x; w[1] = o.y; w[2] = o.z; w[3] = dir.x; w[4] = dir.y; w[5] = dir.z;
 }
 
+template <int MK>
 __global__ __launch_bounds__(256) void k_debug_trace(const DevScene *__restrict__ scp, uint32_t n,
                                                      const float *__restrict__ rays, const uint32_t *__restrict__ skip,
                                                      float tmin, int brute, uint32_t *__restrict__ hit,
                                                      float *__restrict__ tout, int *err) {
     __shared__ uint32_t stk_all[RT_STACK * 256];
-    TCnt cnt{0, 0};
+    TCntT<MK> cnt{0, 0};
     uint32_t *stk = stk_all + threadIdx.x;
     const DevScene &sc = *scp;
     uint32_t r = blockIdx.x * 256 + threadIdx.x;
@@ -2020,7 +2099,7 @@ __global__ __launch_bounds__(256) void k_debug_trace(const DevScene *__restrict_
     float t = 1e30f;
     uint32_t h = RT_MISS, slot;
     if (brute) {
-        for (uint32_t k = 0; k < sc.nleaf; k++) test_leaf<false>(sc.rec, k, o, d, tmin, skip[r], t, h, slot);
+        for (uint32_t k = 0; k < sc.nleaf; k++) test_leaf<false, MK>(sc.rec, k, o, d, tmin, skip[r], t, h, slot);
     } else {
         h = trace_closest<false, RT_STACK>(sc, o, d, tmin, skip[r], 1e30f, t, slot, stk, 256, err, cnt);
     }
@@ -2038,9 +2117,11 @@ int launch_debug_rays(const DevScene *sc, uint32_t P, uint32_t n, uint32_t seed,
 }
 
 int launch_debug_trace(const DevScene *sc, uint32_t n, const float *rays, const uint32_t *skip, float tmin, int brute,
-                       uint32_t *hit, float *t, int *err, hipStream_t st) {
+                       uint32_t *hit, float *t, int *err, hipStream_t st, int flags) {
     if (n == 0) return 0;
-    k_debug_trace<<<(n + 255) / 256, 256, 0, st>>>(sc, n, rays, skip, tmin, brute, hit, t, err);
+    with_flags(flags | RT_FL_MASK, [&](auto fl) {
+        k_debug_trace<decltype(fl)::value><<<(n + 255) / 256, 256, 0, st>>>(sc, n, rays, skip, tmin, brute, hit, t, err);
+    });
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) { set_error(std::string("kernel launch failed: ") + hipGetErrorString(e)); return -1; }
     return 0;
@@ -2101,7 +2182,7 @@ __device__ __forceinline__ void glass_push(const GlassCtx &G, uint32_t h, uint32
 // primitive); d*d stratified origins, every source type; an unblocked ray adds strength x (transmittance of the
 // covers crossed, per band) to its lit face's sums, and every cover crossed takes its shares of
 // flux x strength x transmittance so far.  gsum[i][b][face] of each job is deposited like the strength sums.
-template <int STK, bool MK>
+template <int STK, int MK>
 __global__ __launch_bounds__(256) void k_direct_glass(const GlassDirectJob *__restrict__ jobs, uint32_t njobs,
                                                       uint32_t total, uint32_t *counter, const GlassCtx *__restrict__ gp,
                                                       int *err) {
@@ -2201,7 +2282,7 @@ __global__ __launch_bounds__(256) void k_direct_glass(const GlassDirectJob *__re
 // through covers, which take their shares of sky flux x strength x transmittance so far (only a pass with sky
 // flux pushes anything); a sky ray brings the sky flux times the covers' transmittance, a surface hit the hit
 // face's radiation_out unattenuated.  Each channel's sum per face is deposited as the gather deposits it.
-template <int STK, bool MK>
+template <int STK, int MK>
 __global__ __launch_bounds__(256) void k_diffuse_glass(const DiffuseJob *__restrict__ fj, const GatherJob *__restrict__ gj,
                                                        uint32_t total, uint32_t *counter,
                                                        const GlassCtx *__restrict__ gp, int *err) {
@@ -2342,7 +2423,20 @@ static inline uint32_t grid_for(const LaunchCfg &cfg, int ncu, uint32_t total, u
         if ((depth) + 1 <= 32) { CALL32; } else { CALL64; } \
     } while (0)
 
-template <bool MK>
+// Runs f(std::integral_constant<int, FL>) for the traversal flags fl (RT_FL_*): one kernel instance per flag set.
+template <typename F>
+static void with_flags(int fl, F &&f) {
+    switch (fl & (RT_FL_MASK | RT_FL_WT)) {
+        case 0: f(std::integral_constant<int, 0>{}); break;
+        case RT_FL_MASK: f(std::integral_constant<int, RT_FL_MASK>{}); break;
+        case RT_FL_WT: f(std::integral_constant<int, RT_FL_WT>{}); break;
+        default: f(std::integral_constant<int, RT_FL_MASK | RT_FL_WT>{}); break;
+    }
+}
+// the counting variants always compile the mask tests in (a scene without masks skips them at run time)
+static inline int counting_flags(int fl) { return (fl & RT_FL_WT) | RT_FL_MASK; }
+
+template <int MK>
 static int launch_direct_t(const DirectJob *jobs, uint32_t njobs, uint32_t total, uint32_t depth, uint32_t *counter,
                            int *err, unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st) {
     if (total == 0) return 0;
@@ -2352,8 +2446,8 @@ static int launch_direct_t(const DirectJob *jobs, uint32_t njobs, uint32_t total
     uint32_t g = grid_for(cfg, ncu, total, wpb);
     if (stats) {
         // counting variant: per-lane stacks, 4 waves per block
-        PICK_STK(depth, (k_direct<4, 32, true, true><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)),
-                 (k_direct<4, 64, true, true><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)));
+        PICK_STK(depth, (k_direct<4, 32, true, MK><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)),
+                 (k_direct<4, 64, true, MK><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)));
         LCHECK();
         return 0;
     }
@@ -2370,12 +2464,15 @@ static int launch_direct_t(const DirectJob *jobs, uint32_t njobs, uint32_t total
 }
 
 int launch_direct(const DirectJob *jobs, uint32_t njobs, uint32_t total, uint32_t depth, uint32_t *counter, int *err,
-                  unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st, bool masked) {
-    return masked ? launch_direct_t<true>(jobs, njobs, total, depth, counter, err, stats, cfg, ncu, st)
-                  : launch_direct_t<false>(jobs, njobs, total, depth, counter, err, stats, cfg, ncu, st);
+                  unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st, int flags) {
+    int rc = 0;
+    with_flags(stats ? counting_flags(flags) : flags, [&](auto fl) {
+        rc = launch_direct_t<decltype(fl)::value>(jobs, njobs, total, depth, counter, err, stats, cfg, ncu, st);
+    });
+    return rc;
 }
 
-template <bool MK>
+template <int MK>
 static int launch_direct_src_t(const DirectJob *jobs, uint32_t njobs, uint32_t item_lo, uint32_t nitems,
                                uint32_t depth, uint32_t *counter, int *err, unsigned long long *stats, int ncu,
                                hipStream_t st) {
@@ -2384,8 +2481,8 @@ static int launch_direct_src_t(const DirectJob *jobs, uint32_t njobs, uint32_t i
     uint64_t g64 = std::min<uint64_t>((uint64_t)4 * ncu, ((uint64_t)nitems + 3) / 4);
     uint32_t g = (uint32_t)std::max<uint64_t>(g64, 1);
     if (stats)
-        PICK_STK(depth, (k_direct_src<32, true, true><<<g, 256, 0, st>>>(jobs, njobs, item_lo, nitems, counter, err, stats)),
-                 (k_direct_src<64, true, true><<<g, 256, 0, st>>>(jobs, njobs, item_lo, nitems, counter, err, stats)));
+        PICK_STK(depth, (k_direct_src<32, true, MK><<<g, 256, 0, st>>>(jobs, njobs, item_lo, nitems, counter, err, stats)),
+                 (k_direct_src<64, true, MK><<<g, 256, 0, st>>>(jobs, njobs, item_lo, nitems, counter, err, stats)));
     else
         PICK_STK(depth, (k_direct_src<32, false, MK><<<g, 256, 0, st>>>(jobs, njobs, item_lo, nitems, counter, err, stats)),
                  (k_direct_src<64, false, MK><<<g, 256, 0, st>>>(jobs, njobs, item_lo, nitems, counter, err, stats)));
@@ -2394,14 +2491,17 @@ static int launch_direct_src_t(const DirectJob *jobs, uint32_t njobs, uint32_t i
 }
 
 int launch_direct_src(const DirectJob *jobs, uint32_t njobs, uint32_t item_lo, uint32_t nitems, uint32_t depth,
-                      uint32_t *counter, int *err, unsigned long long *stats, int ncu, hipStream_t st, bool masked) {
-    return masked ? launch_direct_src_t<true>(jobs, njobs, item_lo, nitems, depth, counter, err, stats, ncu, st)
-                  : launch_direct_src_t<false>(jobs, njobs, item_lo, nitems, depth, counter, err, stats, ncu, st);
+                      uint32_t *counter, int *err, unsigned long long *stats, int ncu, hipStream_t st, int flags) {
+    int rc = 0;
+    with_flags(stats ? counting_flags(flags) : flags, [&](auto fl) {
+        rc = launch_direct_src_t<decltype(fl)::value>(jobs, njobs, item_lo, nitems, depth, counter, err, stats, ncu, st);
+    });
+    return rc;
 }
 
 int launch_direct_jobs(const DirectJob *hjobs, const DirectJob *jobs, uint32_t njobs, uint32_t total, uint32_t depth,
                        uint32_t *counter, int *err, unsigned long long *stats, const LaunchCfg &cfg, int ncu,
-                       hipStream_t st, bool masked) {
+                       hipStream_t st, int flags) {
     uint32_t ncol = 0;
     while (ncol < njobs && hjobs[ncol].stype == 0) ncol++;
     for (uint32_t k = ncol; k < njobs; k++)
@@ -2409,20 +2509,20 @@ int launch_direct_jobs(const DirectJob *hjobs, const DirectJob *jobs, uint32_t n
             set_error("internal: collimated direct jobs must precede the other source types");
             return -1;
         }
-    if (ncol == njobs) return launch_direct(jobs, njobs, total, depth, counter, err, stats, cfg, ncu, st, masked);
-    if (ncol == 0) return launch_direct_src(jobs, njobs, 0, total, depth, counter, err, stats, ncu, st, masked);
+    if (ncol == njobs) return launch_direct(jobs, njobs, total, depth, counter, err, stats, cfg, ncu, st, flags);
+    if (ncol == 0) return launch_direct_src(jobs, njobs, 0, total, depth, counter, err, stats, ncu, st, flags);
     const uint32_t split = hjobs[ncol].item_begin;
     if (split > total) {
         set_error("internal: direct job items out of order");
         return -1;
     }
-    if (launch_direct(jobs, ncol, split, depth, counter, err, stats, cfg, ncu, st, masked)) return -1;
+    if (launch_direct(jobs, ncol, split, depth, counter, err, stats, cfg, ncu, st, flags)) return -1;
     return launch_direct_src(jobs + ncol, njobs - ncol, split, total - split, depth, counter, err, stats, ncu, st,
-                             masked);
+                             flags);
 }
 
 int launch_diffuse(const DiffuseJob *job, uint32_t total, uint32_t depth, uint32_t *counter, int *err,
-                   unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st, bool masked) {
+                   unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st, int flags) {
     if (total == 0) return 0;
     // the stack's first RT_DIFFUSE_LDS_STACK entries live in LDS and the rest in private memory, so its
     // full depth (RT_STACK) costs no occupancy: one stack size serves the binary and the 4-wide tree
@@ -2436,23 +2536,20 @@ int launch_diffuse(const DiffuseJob *job, uint32_t total, uint32_t depth, uint32
     uint32_t g = grid_for(cfg, ncu, total, wpb);
     if (stats) {
         // counting variant: 4 waves per block
-        k_diffuse<4, RT_STACK, true, true><<<g, 256, 0, st>>>(job, total, counter, ch, err, stats);
+        with_flags(counting_flags(flags), [&](auto fl) {
+            k_diffuse<4, RT_STACK, true, decltype(fl)::value><<<g, 256, 0, st>>>(job, total, counter, ch, err, stats);
+        });
         LCHECK();
         return 0;
     }
-    if (masked) {
+    with_flags(flags, [&](auto fl) {
+        constexpr int FL = decltype(fl)::value;
         switch (wpb) {
-            case 1: k_diffuse<1, RT_STACK, false, true><<<g, 64, 0, st>>>(job, total, counter, ch, err, stats); break;
-            case 2: k_diffuse<2, RT_STACK, false, true><<<g, 128, 0, st>>>(job, total, counter, ch, err, stats); break;
-            default: k_diffuse<4, RT_STACK, false, true><<<g, 256, 0, st>>>(job, total, counter, ch, err, stats); break;
+            case 1: k_diffuse<1, RT_STACK, false, FL><<<g, 64, 0, st>>>(job, total, counter, ch, err, stats); break;
+            case 2: k_diffuse<2, RT_STACK, false, FL><<<g, 128, 0, st>>>(job, total, counter, ch, err, stats); break;
+            default: k_diffuse<4, RT_STACK, false, FL><<<g, 256, 0, st>>>(job, total, counter, ch, err, stats); break;
         }
-    } else {
-        switch (wpb) {
-            case 1: k_diffuse<1, RT_STACK, false, false><<<g, 64, 0, st>>>(job, total, counter, ch, err, stats); break;
-            case 2: k_diffuse<2, RT_STACK, false, false><<<g, 128, 0, st>>>(job, total, counter, ch, err, stats); break;
-            default: k_diffuse<4, RT_STACK, false, false><<<g, 256, 0, st>>>(job, total, counter, ch, err, stats); break;
-        }
-    }
+    });
     LCHECK();
     return 0;
 }
@@ -2487,38 +2584,41 @@ int launch_gather(const GatherJob *jobs, uint32_t njobs, uint32_t total, uint32_
 }
 
 int launch_camera(const CameraLaunch &c, uint32_t depth, bool labels, int *err, unsigned long long *stats,
-                  hipStream_t st, bool masked) {
+                  hipStream_t st, int flags) {
     if ((uint64_t)c.res_x * c.res_y == 0) return 0;
     // one wave per 8 x 8 pixel block, four waves per workgroup
     const uint64_t blocks = (uint64_t)((c.res_x + 7) / 8) * ((c.res_y + 7) / 8);
     const uint32_t g = (uint32_t)((blocks + 3) / 4);
-    if (labels && masked)
-        PICK_STK(depth, (k_pixel_label<32, true><<<g, 256, 0, st>>>(c, err)),
-                 (k_pixel_label<64, true><<<g, 256, 0, st>>>(c, err)));
-    else if (labels)
-        PICK_STK(depth, (k_pixel_label<32, false><<<g, 256, 0, st>>>(c, err)),
-                 (k_pixel_label<64, false><<<g, 256, 0, st>>>(c, err)));
 #if RT_CAMERA_REFILL
 #define K_CAM k_camera_r
 #else
 #define K_CAM k_camera
 #endif
+    if (labels)
+        with_flags(flags, [&](auto fl) {
+            constexpr int FL = decltype(fl)::value;
+            PICK_STK(depth, (k_pixel_label<32, FL><<<g, 256, 0, st>>>(c, err)),
+                     (k_pixel_label<64, FL><<<g, 256, 0, st>>>(c, err)));
+        });
     else if (stats)
-        PICK_STK(depth, (K_CAM<32, true, true><<<g, 256, 0, st>>>(c, err, stats)),
-                 (K_CAM<64, true, true><<<g, 256, 0, st>>>(c, err, stats)));
-    else if (masked)
-        PICK_STK(depth, (K_CAM<32, false, true><<<g, 256, 0, st>>>(c, err, nullptr)),
-                 (K_CAM<64, false, true><<<g, 256, 0, st>>>(c, err, nullptr)));
+        with_flags(counting_flags(flags), [&](auto fl) {
+            constexpr int FL = decltype(fl)::value;
+            PICK_STK(depth, (K_CAM<32, true, FL><<<g, 256, 0, st>>>(c, err, stats)),
+                     (K_CAM<64, true, FL><<<g, 256, 0, st>>>(c, err, stats)));
+        });
     else
-        PICK_STK(depth, (K_CAM<32, false, false><<<g, 256, 0, st>>>(c, err, nullptr)),
-                 (K_CAM<64, false, false><<<g, 256, 0, st>>>(c, err, nullptr)));
+        with_flags(flags, [&](auto fl) {
+            constexpr int FL = decltype(fl)::value;
+            PICK_STK(depth, (K_CAM<32, false, FL><<<g, 256, 0, st>>>(c, err, nullptr)),
+                     (K_CAM<64, false, FL><<<g, 256, 0, st>>>(c, err, nullptr)));
+        });
 #undef K_CAM
     LCHECK();
     return 0;
 }
 
 int launch_direct_glass(const GlassDirectJob *jobs, uint32_t njobs, uint32_t total, uint32_t depth, uint32_t *counter,
-                        const GlassCtx *g, int *err, int ncu, hipStream_t st, bool masked) {
+                        const GlassCtx *g, int *err, int ncu, hipStream_t st, int flags) {
     if (total == 0) return 0;
     if (depth + 1 > RT_STACK) {
         set_error("BVH depth " + std::to_string(depth) + " exceeds the traversal stack (" + std::to_string(RT_STACK) + ")");
@@ -2526,16 +2626,17 @@ int launch_direct_glass(const GlassDirectJob *jobs, uint32_t njobs, uint32_t tot
     }
     k_zero_u32<<<1, 1, 0, st>>>(counter);
     const uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)4 * ncu, (total + 3) / 4);
-    if (masked) PICK_STK(depth, (k_direct_glass<32, true><<<grid, 256, 0, st>>>(jobs, njobs, total, counter, g, err)),
-                         (k_direct_glass<RT_STACK, true><<<grid, 256, 0, st>>>(jobs, njobs, total, counter, g, err)));
-    else PICK_STK(depth, (k_direct_glass<32, false><<<grid, 256, 0, st>>>(jobs, njobs, total, counter, g, err)),
-                  (k_direct_glass<RT_STACK, false><<<grid, 256, 0, st>>>(jobs, njobs, total, counter, g, err)));
+    with_flags(flags, [&](auto fl) {
+        constexpr int FL = decltype(fl)::value;
+        PICK_STK(depth, (k_direct_glass<32, FL><<<grid, 256, 0, st>>>(jobs, njobs, total, counter, g, err)),
+                 (k_direct_glass<RT_STACK, FL><<<grid, 256, 0, st>>>(jobs, njobs, total, counter, g, err)));
+    });
     LCHECK();
     return 0;
 }
 
 int launch_diffuse_glass(const DiffuseJob *fj, const GatherJob *gj, uint32_t total, uint32_t depth, uint32_t *counter,
-                         const GlassCtx *g, int *err, int ncu, hipStream_t st, bool masked) {
+                         const GlassCtx *g, int *err, int ncu, hipStream_t st, int flags) {
     if (total == 0) return 0;
     if (depth + 1 > RT_STACK) {
         set_error("BVH depth " + std::to_string(depth) + " exceeds the traversal stack (" + std::to_string(RT_STACK) + ")");
@@ -2543,10 +2644,11 @@ int launch_diffuse_glass(const DiffuseJob *fj, const GatherJob *gj, uint32_t tot
     }
     k_zero_u32<<<1, 1, 0, st>>>(counter);
     const uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)4 * ncu, (total + 3) / 4);
-    if (masked) PICK_STK(depth, (k_diffuse_glass<32, true><<<grid, 256, 0, st>>>(fj, gj, total, counter, g, err)),
-                         (k_diffuse_glass<RT_STACK, true><<<grid, 256, 0, st>>>(fj, gj, total, counter, g, err)));
-    else PICK_STK(depth, (k_diffuse_glass<32, false><<<grid, 256, 0, st>>>(fj, gj, total, counter, g, err)),
-                  (k_diffuse_glass<RT_STACK, false><<<grid, 256, 0, st>>>(fj, gj, total, counter, g, err)));
+    with_flags(flags, [&](auto fl) {
+        constexpr int FL = decltype(fl)::value;
+        PICK_STK(depth, (k_diffuse_glass<32, FL><<<grid, 256, 0, st>>>(fj, gj, total, counter, g, err)),
+                 (k_diffuse_glass<RT_STACK, FL><<<grid, 256, 0, st>>>(fj, gj, total, counter, g, err)));
+    });
     LCHECK();
     return 0;
 }
@@ -2560,7 +2662,7 @@ int launch_add_push(size_t PC, double *push, float *in, float *stb, float *sbb, 
 
 int launch_cast(const DevScene *sc, uint32_t depth, uint64_t n, const float *o, const float *d, float maxd, float *t,
                 float *nrm, uint32_t *uuid, unsigned long long *counter, int *err, unsigned long long *stats,
-                int ncu, hipStream_t st) {
+                int ncu, hipStream_t st, bool watertight) {
     if (n == 0) return 0;
     if (depth + 1 > RT_STACK) {
         set_error("BVH depth " + std::to_string(depth) + " exceeds the traversal stack (" + std::to_string(RT_STACK) + ")");
@@ -2570,8 +2672,14 @@ int launch_cast(const DevScene *sc, uint32_t depth, uint64_t n, const float *o, 
     const uint64_t chunks = (n + 63) / 64;
     uint64_t g = std::min<uint64_t>((uint64_t)8 * ncu, (chunks + 3) / 4);
     if (g < 1) g = 1;
-    if (stats) k_cast<RT_STACK, true><<<(uint32_t)g, 256, 0, st>>>(sc, n, o, d, maxd, t, nrm, uuid, counter, err, stats);
-    else k_cast<RT_STACK, false><<<(uint32_t)g, 256, 0, st>>>(sc, n, o, d, maxd, t, nrm, uuid, counter, err, stats);
+    // castRays ignores texture masks (as CollisionDetection does); only the predicate flag applies
+    if (watertight) {
+        if (stats) k_cast<RT_STACK, true, RT_FL_WT><<<(uint32_t)g, 256, 0, st>>>(sc, n, o, d, maxd, t, nrm, uuid, counter, err, stats);
+        else k_cast<RT_STACK, false, RT_FL_WT><<<(uint32_t)g, 256, 0, st>>>(sc, n, o, d, maxd, t, nrm, uuid, counter, err, stats);
+    } else {
+        if (stats) k_cast<RT_STACK, true, 0><<<(uint32_t)g, 256, 0, st>>>(sc, n, o, d, maxd, t, nrm, uuid, counter, err, stats);
+        else k_cast<RT_STACK, false, 0><<<(uint32_t)g, 256, 0, st>>>(sc, n, o, d, maxd, t, nrm, uuid, counter, err, stats);
+    }
     LCHECK();
     return 0;
 }

@@ -11,6 +11,9 @@
 #endif
 
 #define RT_LEAF_BIT 0x80000000u
+// traversal flags of a launch: honour texture masks, watertight intersection predicates
+#define RT_FL_MASK 1
+#define RT_FL_WT 2
 #define RT_NOT_LAUNCHED 0xFFFFFFFEu
 #define RT_HIT_BOTTOM 0x80000000u
 
@@ -44,6 +47,7 @@ struct DevScene {
     float smax;            // largest |coordinate| of any vertex (scale of FP32 rounding in the scene)
     const uint32_t *mdesc; // [P] mask descriptor index (float4 units from rec) per primitive, RT_MISS = none;
                            // null when no primitive has a texture mask
+    const float4 *qnode4;  // [nnode4][4] the BVH4 nodes with 8-bit child boxes (64 B per node), null = none
 };
 
 // Occlusion grid of one collimated-source direction: every traced primitive's padded projection onto the

@@ -133,6 +133,7 @@ SIGNATURES = {
                                     C.c_void_p, C.c_int]),
     "rt_energy_residual_fp64": (C.c_int, [C.c_void_p, f64p, C.c_uint32]),
     "rt_set_energy_accounting": (C.c_int, [C.c_void_p, C.c_int]),
+    "rt_set_predicates": (C.c_int, [C.c_void_p, C.c_int]),
     "rt_energy_passes_fp64": (C.c_int, [C.c_void_p, f64p, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32)]),
     "rt_set_launch_cfg": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(rt_launch_cfg)]),
     "rt_get_launch_cfg": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(rt_launch_cfg)]),
@@ -287,6 +288,10 @@ class Context:
                               "chunk": r.cfg[k].chunk, "ms": round(r.ms[k], 3),
                               "default_ms": round(r.default_ms[k], 3)} for k in range(3)},
                 "grid_density": r.cfg[0].grid_density}
+
+    def set_predicates(self, mode: str):
+        """Intersection predicates of later traces: "reference" (parity mode, default) or "watertight"."""
+        _check(self.lib.rt_set_predicates(self._h, {"reference": 0, "watertight": 1}[mode]))
 
     def enable_timing(self, on: bool = True):
         _check(self.lib.rt_enable_timing(self._h, 1 if on else 0))

@@ -437,6 +437,16 @@ typedef struct {
     uint32_t grid_entries;  /* list entries per direction per traced primitive (x100) of those grids */
 } rt_timing;
 int rt_enable_timing(rt_ctx *ctx, int on);
+/* Intersection predicates of every later trace on the context (radiation passes, cameras, castRays, debug traces).
+ * RT_PREDICATES_REFERENCE (default): the reference's FP32 tests, operation for operation
+ * (R/shaders/common/intersections.glsl:26-112; castRays: CD/src/CollisionDetection_RayTracing.cpp:1793-1884),
+ * the parity mode.  RT_PREDICATES_WATERTIGHT: the watertight ray/triangle test of Woop, Benthin and Wald (2013),
+ * patches as two triangles, closed edges, so no ray passes between primitives that share an edge's vertices
+ * (the reference's open patch interval lets a ray through a shared patch edge); the distance window is the
+ * reference's.  Hit decisions differ only for rays that graze an edge. */
+#define RT_PREDICATES_REFERENCE 0
+#define RT_PREDICATES_WATERTIGHT 1
+int rt_set_predicates(rt_ctx *ctx, int mode);
 /* Occlusion test of collimated direct rays: 1 (default) = per-direction occlusion grids when the scene is
  * not periodic and they fit, 0 = always traverse the BVH.  Results are identical either way; grids cost
  * memory (about 100 B per primitive per direction) and are faster. */

@@ -75,6 +75,7 @@ def lib():
                                       C.c_uint32, f32p, f32p]
         L.orc_diffuse_hits.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, u32p]
         L.orc_glass_tra.argtypes = [C.c_float, C.c_float, C.c_float, f32p]
+        L.orc_set_predicates.argtypes = [C.c_int]
         L.orc_direct_sums.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, f32p, f64p]
         L.orc_diffuse_hits_range.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, u32p]
         L.orc_direct_sums_range.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, f32p, C.c_uint32,
@@ -101,6 +102,20 @@ def _f(a):
 
 def _u(a):
     return np.ascontiguousarray(a, np.uint32)
+
+
+class predicates:
+    """Context manager: the oracle's intersection predicates inside the block ("reference" or "watertight")."""
+
+    def __init__(self, mode: str):
+        self.mode = {"reference": 0, "watertight": 1}[mode]
+
+    def __enter__(self):
+        lib().orc_set_predicates(self.mode)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_predicates(0)
 
 
 def glass_tra(cos_theta, n, KL=0.0):

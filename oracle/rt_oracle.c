@@ -439,8 +439,70 @@ void orc_areas(const orc_scene *s, double *out) {
 /* intersection predicates                                                                           */
 /* ------------------------------------------------------------------------------------------------ */
 
+/* ---- watertight predicates (orc_set_predicates) ---- */
+static int g_watertight = 0;
+void orc_set_predicates(int mode) { g_watertight = mode != 0; }
+
+static float comp3(v3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+/* Woop, Benthin and Wald, "Watertight Ray/Triangle Intersection", JCGT 2(1), 2013: shear the vertices into
+ * the ray's frame (largest direction component as z, x and y swapped to keep the winding), decide with the 2-D
+ * edge functions (recomputed in double when one is exactly 0; on an edge counts as inside), both faces. */
+static int wt_tri_orc(v3 o, v3 d, v3 a, v3 b, v3 c, float *t, float *ub, float *uc) {
+    float ax_ = fabsf(d.x), ay_ = fabsf(d.y), az_ = fabsf(d.z);
+    int kz = ax_ > ay_ ? (ax_ > az_ ? 0 : 2) : (ay_ > az_ ? 1 : 2);
+    int kx = kz == 2 ? 0 : kz + 1, ky = kx == 2 ? 0 : kx + 1;
+    float dz = comp3(d, kz);
+    if (dz < 0.0f) { int tmp = kx; kx = ky; ky = tmp; }
+    float Sx = comp3(d, kx) / dz, Sy = comp3(d, ky) / dz, Sz = 1.0f / dz;
+    v3 A = sub(a, o), B = sub(b, o), C = sub(c, o);
+    float Ax = comp3(A, kx) - Sx * comp3(A, kz), Ay = comp3(A, ky) - Sy * comp3(A, kz);
+    float Bx = comp3(B, kx) - Sx * comp3(B, kz), By = comp3(B, ky) - Sy * comp3(B, kz);
+    float Cx = comp3(C, kx) - Sx * comp3(C, kz), Cy = comp3(C, ky) - Sy * comp3(C, kz);
+    float U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return 0;
+    float det = U + V + W;
+    if (det == 0.0f) return 0;
+    float T = U * (Sz * comp3(A, kz)) + V * (Sz * comp3(B, kz)) + W * (Sz * comp3(C, kz));
+    float inv = 1.0f / det;
+    *t = T * inv;
+    *ub = V * inv;
+    *uc = W * inv;
+    return 1;
+}
+
+/* the record's vertices as the GPU rebuilds them: v0, v0 + e1, v0 + e2 (triangle); v0, v0 + a, v0 + a + b,
+ * v0 + b (patch, two triangles); the distance window of the mode (castRays: t > 1e-5; radiation: (1e-8, 1e8)) */
+static int isect_wt_orc(const orc_scene *s, uint32_t i, v3 o, v3 d, int cd, float *t_out, float *ua, float *ub) {
+    v3 v0 = s->verts[(size_t)i * 4];
+    v3 e1 = sub(s->verts[i * 4 + 1], v0);
+    int patch = is_patch(s->type[i]);
+    v3 e2 = sub(s->verts[i * 4 + (patch ? 3 : 2)], v0);
+    float t = 0.f, u = 0.f, v = 0.f;
+    int hit;
+    if (patch) {
+        v3 p1 = add(v0, e1), p2 = add(p1, e2), p3 = add(v0, e2);
+        hit = wt_tri_orc(o, d, v0, p1, p2, &t, &u, &v);
+        if (!hit) hit = wt_tri_orc(o, d, v0, p2, p3, &t, &u, &v);
+        v3 dd = sub(add(o, scl(d, t)), v0);
+        u = dot(dd, e1);
+        v = dot(dd, e2);
+    } else {
+        hit = wt_tri_orc(o, d, v0, add(v0, e1), add(v0, e2), &t, &u, &v);
+    }
+    hit = hit && (cd ? (t > 1e-5f) : (t > 1e-8f && t < 1e8f));
+    if (hit) { *t_out = t; *ua = u; *ub = v; }
+    return hit;
+}
+
 /* R/shaders/common/intersections.glsl:26-70 */
 static int isect_patch_rad_uv(const orc_scene *s, uint32_t i, v3 o, v3 d, float *t_out, float *ua, float *ub) {
+    if (g_watertight) return isect_wt_orc(s, i, o, d, 0, t_out, ua, ub);
     v3 n = s->pn[i];
     float denom = dot(d, n);
     if (fabsf(denom) < 1e-8f) return 0;
@@ -457,6 +519,7 @@ static int isect_patch_rad_uv(const orc_scene *s, uint32_t i, v3 o, v3 d, float 
 
 /* R/shaders/common/intersections.glsl:73-112 (Moller-Trumbore) */
 static int isect_tri_rad_uv(const orc_scene *s, uint32_t i, v3 o, v3 d, float *t_out, float *ua, float *ub) {
+    if (g_watertight) return isect_wt_orc(s, i, o, d, 0, t_out, ua, ub);
     v3 v0 = s->verts[(size_t)i * 4];
     v3 e1 = sub(s->verts[i * 4 + 1], v0), e2 = sub(s->verts[i * 4 + 2], v0);
     v3 h = cross(d, e2);
@@ -589,6 +652,7 @@ int orc_scene_set_masks(orc_scene *s, uint32_t nmasks, const uint8_t *texels, co
 
 /* CD/src/CollisionDetection_RayTracing.cpp:1793-1847 */
 static int isect_tri_cd(const orc_scene *s, uint32_t i, v3 o, v3 d, float *t_out) {
+    if (g_watertight) { float ua, ub; return isect_wt_orc(s, i, o, d, 1, t_out, &ua, &ub); }
     const float EPS = 1e-5f;
     v3 v0 = s->verts[(size_t)i * 4];
     v3 e1 = sub(s->verts[i * 4 + 1], v0), e2 = sub(s->verts[i * 4 + 2], v0);
@@ -609,6 +673,7 @@ static int isect_tri_cd(const orc_scene *s, uint32_t i, v3 o, v3 d, float *t_out
 
 /* CD/src/CollisionDetection_RayTracing.cpp:1849-1884 */
 static int isect_patch_cd(const orc_scene *s, uint32_t i, v3 o, v3 d, float *t_out) {
+    if (g_watertight) { float ua, ub; return isect_wt_orc(s, i, o, d, 1, t_out, &ua, &ub); }
     const float EPS = 1e-8f;
     v3 v0 = s->verts[(size_t)i * 4];
     v3 n = nrm(cross(sub(s->verts[i * 4 + 1], v0), sub(s->verts[i * 4 + 2], v0)));

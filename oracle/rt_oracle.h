@@ -168,6 +168,10 @@ void orc_glass_tra(float cos_theta, float n, float KL, float out[3]);
  * diag[0*B+b] absorbed sum(A*R), diag[1*B+b] primary diffuse incident, diag[2*B+b] direct incident */
 int orc_run_band(const orc_scene *s, const orc_band_params *p, double *R, double *diag);
 
+/* Intersection predicates of every later call: 0 = the reference's (default), 1 = watertight (Woop, Benthin
+ * and Wald 2013, as rt_set_predicates).  Process-wide. */
+void orc_set_predicates(int mode);
+
 /* primitive areas [P] (patch |a x b|, triangle |e1 x e2|/2) in FP64 */
 void orc_areas(const orc_scene *s, double *out);
 

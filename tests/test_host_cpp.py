@@ -41,6 +41,28 @@ def test_adapter_compiles_against_reference_interface():
     assert r.returncode == 0, r.stderr
 
 
+@pytest.mark.skipif(not os.path.exists(os.path.join(REF, "plugins/radiation/src/RayTracingTypes.cpp")),
+                    reason="Helios source tree not present")
+def test_adapter_runs_against_recording_abi(tmp_path):
+    """The adapter runs through the reference's RayTracingBackend interface on the CPU, over a recording test
+    double of the C ABI (tests/cpp/adapter_fake_abi.cpp): geometry/mask/material/source conversion, the band
+    table with retired bands, launch parameters, results and errors (tests/cpp/test_adapter.cpp).  The binary
+    is built in a temporary directory from the reference's RayTracingTypes.cpp where it lies and never leaves
+    this machine."""
+    exe = str(tmp_path / "test_adapter")
+    cpp = os.path.join(ROOT, "tests", "cpp")
+    inc = ["-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROOT, "helios_amd", "host"), "-I", cpp,
+           "-I", os.path.join(REF, "core/include"), "-I", os.path.join(REF, "core/lib/pugixml"),
+           "-I", os.path.join(REF, "plugins/radiation/include")]
+    srcs = [os.path.join(ROOT, "helios_amd", "host", "HipRayTracingBackend.cpp"), os.path.join(cpp, "adapter_fake_abi.cpp"),
+            os.path.join(cpp, "test_adapter.cpp"), os.path.join(REF, "plugins/radiation/src/RayTracingTypes.cpp")]
+    r = subprocess.run(["g++", "-std=c++17", "-O1", *inc, *srcs, "-o", exe], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    print(r.stdout)
+    assert r.returncode == 0 and "all checks passed" in r.stdout, r.stdout + r.stderr
+
+
 @pytest.mark.gpu
 def test_cpp_host_cases_gpu():
     r = subprocess.run([_binary()], capture_output=True, text=True, timeout=300)
