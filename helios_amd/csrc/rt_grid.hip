@@ -127,8 +127,7 @@ __global__ void k_grid_rank(const SunGrid *__restrict__ grids, const uint32_t *_
                             const unsigned long long *__restrict__ item_base,
                             const unsigned long long *__restrict__ item_count, const uint32_t *__restrict__ start,
                             const uint32_t *__restrict__ t_items, const float *__restrict__ t_keys,
-                            const uint32_t *__restrict__ t_cell, uint32_t *__restrict__ items,
-                            float *__restrict__ keys) {
+                            const uint32_t *__restrict__ t_cell, uint2 *__restrict__ ents) {
     const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= item_count[blockIdx.y]) return;
     const unsigned long long ib = item_base[blockIdx.y];
@@ -147,12 +146,10 @@ __global__ void k_grid_rank(const SunGrid *__restrict__ grids, const uint32_t *_
     for (uint32_t y = b; y < e; y++) {
         const float ky = t_keys[ib + y];
         const uint32_t iy = t_items[ib + y];
-        rank += (ky > kx || (ky == kx && (iy < ix || (iy == ix && y < (uint32_t)x)))) ? 1u : 0u;
+        rank += (ky > kx || (ky == kx && ((iy & RT_GRID_SLOT) < (ix & RT_GRID_SLOT) ||
+                                          (iy == ix && y < (uint32_t)x)))) ? 1u : 0u;
     }
-    if (b + rank < e) {
-        items[ib + b + rank] = ix;
-        keys[ib + b + rank] = kx;
-    }
+    if (b + rank < e) ents[ib + b + rank] = make_uint2(ix, __float_as_uint(kx));
 }
 
 // start[] is one exclusive scan over the cells of all directions in uint32 arithmetic, so it holds the
@@ -174,6 +171,8 @@ __global__ void k_grid_fill(uint32_t nleaf, const float4 *__restrict__ rec, floa
     const size_t base = cell_base[blockIdx.y];
     const unsigned long long ib = item_base[blockIdx.y], ni = item_count[blockIdx.y];
     const uint32_t sbase = (uint32_t)ib;
+    const uint32_t tag = __float_as_uint(rec[(size_t)k * 4].w);
+    const uint32_t item = k | ((tag & (RT_PATCH_BIT | RT_MASK_BIT)) ? RT_GRID_REC3 : 0u);
     for (int y = y0; y <= y1; y++)
         for (int x = x0; x <= x1; x++) {
             const size_t c = base + (size_t)y * G.nx + x;
@@ -183,7 +182,7 @@ __global__ void k_grid_fill(uint32_t nleaf, const float4 *__restrict__ rec, floa
             if (left == 0u) continue;
             const unsigned long long at = (unsigned long long)(uint32_t)(start[c] - sbase) + left - 1u;
             if (at < ni) {
-                items[ib + at] = k;
+                items[ib + at] = item;
                 keys[ib + at] = key;
                 cells[ib + at] = (uint32_t)(c - base);
             }
@@ -231,8 +230,7 @@ int build_sun_grids(GridSet &gs, const Scene &S, const float *dirs, uint32_t n, 
         G.sbase = 0;
         G.sorted = 0;
         G.start = nullptr;
-        G.items = nullptr;
-        G.keys = nullptr;
+        G.ents = nullptr;
     }
     // 1. projected extents
     if (gs.small.reserve(n * sizeof(SunGrid) + n * 16 + n * 4 + n * 16 + 2048)) return -1;
@@ -300,9 +298,8 @@ int build_sun_grids(GridSet &gs, const Scene &S, const float *dirs, uint32_t n, 
     }
     if (gs.cap && cell_bytes + total * 20 + 1024 > gs.cap) return 1;  // lists (8 B) + sort scratch (12 B)
     GK(hipcub::DeviceScan::ExclusiveSum(tmp, scan_bytes, counts, starts, (int)(total_cells + 1), st));
-    if (gs.items.reserve((size_t)total * 4 + 256) || gs.keys.reserve((size_t)total * 4 + 256)) return -1;
-    uint32_t *items = (uint32_t *)gs.items.base;
-    float *keys = (float *)gs.keys.base;
+    if (gs.items.reserve((size_t)total * 8 + 256)) return -1;
+    uint2 *ents = (uint2 *)gs.items.base;
     // the fill writes unsorted lists (entry, key, cell) to scratch; k_grid_rank moves each entry to its
     // place.  Slots nothing is written to read as "no primitive" with a NaN key.
     const size_t tb = ((size_t)total * 4 + 255) & ~(size_t)255;
@@ -311,8 +308,7 @@ int build_sun_grids(GridSet &gs, const Scene &S, const float *dirs, uint32_t n, 
     float *t_keys = (float *)((char *)gs.sortbuf.base + tb);
     uint32_t *t_cell = (uint32_t *)((char *)gs.sortbuf.base + 2 * tb);
     GK(hipMemsetAsync(gs.sortbuf.base, 0xFF, 3 * tb, st));
-    GK(hipMemsetAsync(items, 0xFF, (size_t)total * 4, st));
-    GK(hipMemsetAsync(keys, 0xFF, (size_t)total * 4, st));
+    GK(hipMemsetAsync(ents, 0xFF, (size_t)total * 8, st));
     GK(hipMemcpyAsync(dibase, ibase.data(), (size_t)n * 8, hipMemcpyHostToDevice, st));
     k_grid_fill<<<grid, 256, 0, st>>>(nleaf, S.h.rec, S.h.smax, dgrids, dbase, dibase, dentries, starts, counts,
                                       t_items, t_keys, t_cell);
@@ -323,15 +319,14 @@ int build_sun_grids(GridSet &gs, const Scene &S, const float *dirs, uint32_t n, 
         if (maxent) {
             k_grid_rank<<<dim3((uint32_t)((maxent + 255) / 256), n), 256, 0, st>>>(dgrids, dbase, dibase, dentries,
                                                                                    starts, t_items, t_keys, t_cell,
-                                                                                   items, keys);
+                                                                                   ents);
             GK(hipGetLastError());
         }
     }
     const bool sorted = true;
     for (uint32_t j = 0; j < n; j++) {
         hg[j].start = starts + base[j];
-        hg[j].items = items + ibase[j];
-        hg[j].keys = keys + ibase[j];
+        hg[j].ents = ents + ibase[j];
         hg[j].sbase = (uint32_t)ibase[j];
         hg[j].sorted = sorted ? 1u : 0u;
     }

@@ -675,26 +675,32 @@ __device__ __forceinline__ void grid_cell(const SunGrid &G, v3f o, uint32_t &b, 
 
 // Candidates p and p + 1 (when p + 1 < e) of a cell list: is either accepted as an occluder?  Both
 // candidates' loads are issued before either test (the list entry -> record chain is latency-bound); a slot
-// that is out of the list or unfilled loads record 0 and is ignored.  A candidate whose depth key (an upper
-// bound of the depth of its points along the source direction, padded by 1e-5 of the scene's largest
-// coordinate) is below the ray origin's depth lies wholly behind the origin: no FP32 hit of it can have
-// t > 1e-4 (the FP32 errors of t, of the key and of the depth are a few ulps of that coordinate), so it is
-// skipped without loading its record.
+// that is out of the list or unfilled loads record 0 and is ignored.  An entry is one 8-byte load (slot and
+// key), and a record's fourth float4 (the patch plane normal, the mask descriptor) is loaded only for the
+// entries flagged as needing it, so a triangle candidate costs four vector loads instead of six.  A
+// candidate whose depth key (an upper bound of the depth of its points along the source direction, padded
+// by 1e-5 of the scene's largest coordinate) is below the ray origin's depth lies wholly behind the origin:
+// no FP32 hit of it can have t > 1e-4 (the FP32 errors of t, of the key and of the depth are a few ulps of
+// that coordinate), so it is skipped without loading its record.
 // Returns 1 when a candidate occludes the ray, 2 when the list is known to hold no further occluder (a
 // sorted list whose next key is below the origin's depth), else 0.
 template <typename C>
-__device__ __forceinline__ int grid_pair(const uint32_t *items, const float *keys, bool sorted, float depth,
-                                         const float4 *recs, uint32_t nleaf, uint32_t p, uint32_t e, v3f o, v3f d,
-                                         uint32_t skip, C &cnt) {
+__device__ __forceinline__ int grid_pair(const uint2 *ents, bool sorted, float depth, const float4 *recs,
+                                         uint32_t nleaf, uint32_t p, uint32_t e, v3f o, v3f d, uint32_t skip,
+                                         C &cnt) {
     const bool two = p + 1 < e;
-    const uint32_t k0 = items[p];
-    const uint32_t k1 = two ? items[p + 1] : RT_MISS;
-    const float key0 = keys[p];
-    const float key1 = two ? keys[p + 1] : -INFINITY;
-    const bool v0 = k0 < nleaf && !(key0 < depth), v1 = k1 < nleaf && !(key1 < depth);
+    const uint2 e0 = ents[p];
+    const uint2 e1 = two ? ents[p + 1] : make_uint2(RT_MISS, __float_as_uint(-INFINITY));
+    const float key0 = __uint_as_float(e0.y), key1 = __uint_as_float(e1.y);
+    const uint32_t k0 = e0.x & RT_GRID_SLOT, k1 = e1.x & RT_GRID_SLOT;
+    const bool v0 = e0.x != RT_MISS && k0 < nleaf && !(key0 < depth);
+    const bool v1 = e1.x != RT_MISS && k1 < nleaf && !(key1 < depth);
     const float4 *q0 = recs + (size_t)(v0 ? k0 : 0u) * 4, *q1 = recs + (size_t)(v1 ? k1 : 0u) * 4;
-    const float4 a0 = ldc(q0), a1 = ldc(q0 + 1), a2 = ldc(q0 + 2), a3 = ldc(q0 + 3);
-    const float4 b0 = ldc(q1), b1 = ldc(q1 + 1), b2 = ldc(q1 + 2), b3 = ldc(q1 + 3);
+    const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 a0 = ldc(q0), a1 = ldc(q0 + 1), a2 = ldc(q0 + 2);
+    const float4 a3 = (v0 && (e0.x & RT_GRID_REC3)) ? ldc(q0 + 3) : zero;
+    const float4 b0 = ldc(q1), b1 = ldc(q1 + 1), b2 = ldc(q1 + 2);
+    const float4 b3 = (v1 && (e1.x & RT_GRID_REC3)) ? ldc(q1 + 3) : zero;
     const uint32_t ta = __float_as_uint(a0.w), tb = __float_as_uint(b0.w);
     float t0, t1, ua0, ub0, ua1, ub1;
     bool h0 = v0 && isect_rec_uv<false, C::wt>(a0, a1, a2, a3, (ta & RT_PATCH_BIT) != 0, o, d, t0, ua0, ub0) &&
@@ -781,8 +787,7 @@ __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict
                         // scan (every ray is tested exactly as on its own; only the ray -> lane assignment
                         // and the summation grouping differ).
                         const SunGrid &G = *grid;
-                        const uint32_t *items = G.items;
-                        const float *keys = G.keys;
+                        const uint2 *ents = G.ents;
                         const bool sorted = G.sorted != 0;
                         const uint32_t nleaf = ldu(&sc.nleaf);
                         const float gw0 = G.w[0], gw1 = G.w[1], gw2 = G.w[2];
@@ -805,7 +810,7 @@ __global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict
                             while (true) {
                                 if (act) {
                                     const int res = pc >= pe ? 2
-                                                             : grid_pair(items, keys, sorted, depth, recs, nleaf, pc, pe,
+                                                             : grid_pair(ents, sorted, depth, recs, nleaf, pc, pe,
                                                                          o, dir, i, cnt);
                                     if (res == 2) acc.add(strength);  // lit
                                     if (res != 0) act = false;
@@ -1188,7 +1193,7 @@ __device__ __forceinline__ bool lane_leaf(const float4 *recs, LaneRay &L, float 
 // The traversal stack keeps its first RT_DIFFUSE_LDS_STACK entries in LDS and deeper ones in private
 // memory: 4 KB of LDS per wave leaves the register file (about 70 VGPRs) as the occupancy limit.
 #ifndef RT_REFILL_BELOW
-#define RT_REFILL_BELOW 40
+#define RT_REFILL_BELOW 48
 #endif
 #ifndef RT_LEAF_BATCH
 #define RT_LEAF_BATCH 16

@@ -52,8 +52,11 @@ struct DevScene {
 
 // Occlusion grid of one collimated-source direction: every traced primitive's padded projection onto the
 // plane perpendicular to the direction, binned into a uniform grid (CSR: start[cell] .. start[cell + 1]
-// index items[], which hold leaf slots).  A shadow ray can only hit primitives binned in the cell of its
-// origin's projection.
+// index ents[]).  A shadow ray can only hit primitives binned in the cell of its origin's projection.
+// A list entry is (leaf slot | RT_GRID_REC3 when the test needs the record's fourth float4 -- patches and
+// masked primitives --, bits of the depth key): one 8-byte load per candidate.  0xFFFFFFFF = no primitive.
+#define RT_GRID_REC3 0x40000000u
+#define RT_GRID_SLOT 0x3FFFFFFFu
 struct SunGrid {
     float u[3], v[3];      // orthonormal basis of the projection plane
     float w[3];            // unit direction towards the source (depth axis of the keys)
@@ -63,10 +66,10 @@ struct SunGrid {
     uint32_t sorted;       // 1: each cell's list is in descending key order (a scan may stop at the first
                            // key below the ray origin's depth)
     uint32_t sbase;        // start[] values count from sbase (modulo 2^32): cell c lists
-                           // items[start[c] - sbase, start[c + 1] - sbase)
+                           // ents[start[c] - sbase, start[c + 1] - sbase)
     const uint32_t *start; // [nx*ny + 1]
-    const uint32_t *items; // this direction's list
-    const float *keys;     // [entry] the primitive's largest depth along w (max over its vertices of v.w)
+    const uint2 *ents;     // this direction's list: (slot | flag, key); the key is the primitive's largest
+                           // depth along w (max over its vertices of v.w), padded
 };
 
 // one direct trace job: one scene, one collimated source

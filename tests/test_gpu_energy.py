@@ -68,6 +68,25 @@ def test_enclosure_conserves_energy(acct):
     assert e["passes"].shape[0] == 6
 
 
+def test_one_sided_back_face_takes_nothing(acct):
+    # an emitting one-sided rectangle under a one-sided rectangle whose back faces it: the upper one launches
+    # no bottom-face rays (diffuse_raygen.comp:365-370), so what the lower one radiates at it is absorbed by
+    # nothing and does not escape -- the residual is exactly that share, F12 of the emission
+    T0, _, _, F12 = cf.parallel_rectangles()
+    T = np.stack([T0[0], cf.patch_T((0, 0, 0.5), (1.0, 0, 0), (0, 2.0, 0))])  # both face +z
+    g = acct.scene(T, [0, 0], [0, 0])
+    req = native.BandRequest(2, [native.BandSpec("LW", 0, True, 0.0)], rho=np.zeros((2, 1)), tau=np.zeros((2, 1)),
+                             direct_rays=0, diffuse_rays=250000, eps=np.ones((2, 1)), temperature=[300.0, 0.0],
+                             diffuse_seeds=[41])
+    R = native.run_band(g, req)
+    e = native.energy_fp64(acct, 1)
+    E = cf.SIGMA * 300.0 ** 4 * 2.0
+    assert abs(e["emitted"][0] - E) <= 1e-6 * E
+    assert R[1, 0] == 0.0
+    assert abs(e["escaped"][0] - (1.0 - F12) * E) <= 3e-3 * E
+    assert abs(e["residual"][0] - F12 * E) <= 3e-3 * E, (e["residual"][0] / E, F12)
+
+
 def test_open_canopy_residual_both_schedules(acct):
     s = scenes.config1(3)
     g = acct.scene(s.transforms, s.types, s.twosided)
