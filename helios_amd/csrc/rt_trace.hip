@@ -39,6 +39,13 @@
 #ifndef RT_CAST_QBVH4
 #define RT_CAST_QBVH4 1
 #endif
+#ifndef RT_Q4_EAGER
+#define RT_Q4_EAGER 1
+#endif
+// BVH4 pushes without branches into a spare LDS row per lane (the per-lane stacks then hold LSTK + 1 rows)
+#ifndef RT_PUSH_SPARE
+#define RT_PUSH_SPARE 1
+#endif
 #ifndef RT_BVH4_SORT
 #define RT_BVH4_SORT 1
 #endif
@@ -1056,6 +1063,21 @@ __device__ __forceinline__ bool node4_order(LaneRay &L, float *key, uint32_t *ln
         atomicOr(err, 1);
         return true;
     }
+#if RT_PUSH_SPARE
+    if (LSTK >= STK || (uint32_t)L.sp + 3u <= (uint32_t)LSTK) {
+        // every slot a push may use is in LDS: three stores without branches, those beyond the pushed
+        // children aimed at the lane's spare row (index LSTK, which nothing reads); child k < nhit goes to
+        // slot sp + nhit - 1 - k, so the farthest lies deepest and child 1 on top
+#pragma unroll
+        for (int k = 3; k >= 1; k--) {
+            const uint32_t at = (uint32_t)k < nhit ? (uint32_t)L.sp + nhit - 1u - (uint32_t)k : (uint32_t)LSTK;
+            stk[at * stride] = lnk[k];
+        }
+        L.sp += (int)nhit - 1;
+        L.ni = lnk[0];
+        return false;
+    }
+#endif
 #pragma unroll
     for (int k = 3; k >= 1; k--) {
         if ((uint32_t)k < nhit) {
@@ -1148,18 +1170,27 @@ __device__ __forceinline__ bool lane_node4q(const float4 *q4, uint32_t nn4, Lane
     const float Ax = sx * L.rp.ix, Ay = sy * L.rp.iy, Az = sz * L.rp.iz;
     const float Bx = fmaf(f0.x, L.rp.ix, -L.rp.ox), By = fmaf(f0.y, L.rp.iy, -L.rp.oy),
                 Bz = fmaf(f0.z, L.rp.iz, -L.rp.oz);
+#if RT_Q4_EAGER
+    // keep the link words' load beside the box words' (the compiler would otherwise sink it into the
+    // branch that pushes, a second dependent round trip per step)
+    asm volatile("" ::"v"(f2.z), "v"(f2.w), "v"(f3.x), "v"(f3.y));
+#endif
     const uint32_t wlx = __float_as_uint(f1.x), whx = __float_as_uint(f1.y), wly = __float_as_uint(f1.z),
                    why = __float_as_uint(f1.w), wlz = __float_as_uint(f2.x), whz = __float_as_uint(f2.y);
     float key[4];
     uint32_t lnk[4] = {__float_as_uint(f2.z), __float_as_uint(f2.w), __float_as_uint(f3.x), __float_as_uint(f3.y)};
     uint32_t nhit = 0;
+    // per axis, the bound word nearer the origin: the lower bounds when the direction is positive (A > 0:
+    // t rises with q), else the upper ones.  t(near) = min(t(lo), t(hi)) exactly, so the entry and exit
+    // distances equal the min/max form's with one select per axis instead of a min and a max per child.
+    const bool sx_pos = L.rp.ix >= 0.f, sy_pos = L.rp.iy >= 0.f, sz_pos = L.rp.iz >= 0.f;
+    const uint32_t nwx = sx_pos ? wlx : whx, fwx = sx_pos ? whx : wlx;
+    const uint32_t nwy = sy_pos ? wly : why, fwy = sy_pos ? why : wly;
+    const uint32_t nwz = sz_pos ? wlz : whz, fwz = sz_pos ? whz : wlz;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        const float lx = fmaf(qbyte(wlx, k), Ax, Bx), hx = fmaf(qbyte(whx, k), Ax, Bx);
-        const float ly = fmaf(qbyte(wly, k), Ay, By), hy = fmaf(qbyte(why, k), Ay, By);
-        const float lz = fmaf(qbyte(wlz, k), Az, Bz), hz = fmaf(qbyte(whz, k), Az, Bz);
-        const float tn = max3f(fminf(lx, hx), fminf(ly, hy), fminf(lz, hz));
-        const float tf = min3f(fmaxf(lx, hx), fmaxf(ly, hy), fmaxf(lz, hz));
+        const float tn = max3f(fmaf(qbyte(nwx, k), Ax, Bx), fmaf(qbyte(nwy, k), Ay, By), fmaf(qbyte(nwz, k), Az, Bz));
+        const float tf = min3f(fmaf(qbyte(fwx, k), Ax, Bx), fmaf(qbyte(fwy, k), Ay, By), fmaf(qbyte(fwz, k), Az, Bz));
         const bool h = (uint32_t)k < nch && fmaxf(tn, 0.f) <= fminf(tf, L.best_t);
         key[k] = h ? tn : INFINITY;
         nhit += h ? 1u : 0u;
@@ -1201,12 +1232,23 @@ __device__ __forceinline__ bool lane_leaf(const float4 *recs, LaneRay &L, float 
 #ifndef RT_DIFFUSE_LDS_STACK
 #define RT_DIFFUSE_LDS_STACK 16
 #endif
+// Waves per SIMD the compiler must fit the timed diffuse kernel into (the default launch shape, 16 two-wave
+// workgroups per CU, runs 8 per SIMD): on the orchard 8 waves with a few spills beat 5 waves without (622
+// against 726 ms per diurnal series).  The masked variant, whose leaf test is larger, is held to 5 waves;
+// the counting variants keep their registers.
+#ifndef RT_DIFFUSE_WAVES_PER_EU
+#define RT_DIFFUSE_WAVES_PER_EU 8
+#endif
+#ifndef RT_DIFFUSE_WAVES_PER_EU_MASKED
+#define RT_DIFFUSE_WAVES_PER_EU_MASKED 5
+#endif
 template <int WPB, int STK, bool COUNT, int MK>
-__global__ __launch_bounds__(WPB * 64) void k_diffuse(const DiffuseJob *__restrict__ job, uint32_t total,
+__global__ __launch_bounds__(WPB * 64, COUNT ? 1 : ((MK & RT_FL_MASK) ? RT_DIFFUSE_WAVES_PER_EU_MASKED : RT_DIFFUSE_WAVES_PER_EU))
+void k_diffuse(const DiffuseJob *__restrict__ job, uint32_t total,
                                                       uint32_t *counter, uint32_t chunk, int *err,
                                                       unsigned long long *stats) {
     constexpr int LSTK = STK < RT_DIFFUSE_LDS_STACK ? STK : RT_DIFFUSE_LDS_STACK;
-    __shared__ uint32_t stk_all[LSTK * WPB * 64];
+    __shared__ uint32_t stk_all[(LSTK + 1) * WPB * 64];  // + the spare row of the branch-free BVH4 pushes
     // overflow entries: sized for the deepest stack (RT_STACK) so the array stays in private memory
     // instead of being promoted to registers, which would cost the occupancy the small LDS part buys
     uint32_t ovf[STK > LSTK ? RT_STACK - LSTK : 1];
@@ -1624,12 +1666,15 @@ __device__ __forceinline__ unsigned long long wave_chunk64(unsigned long long *c
 // whose trace ends writes its result and takes the chunk's next ray; traversal runs in node and leaf phases
 // as k_diffuse's.  Each ray is traced exactly as on its own.
 template <int STK, bool COUNT, int MK>
-__global__ __launch_bounds__(256) void k_cast(const DevScene *__restrict__ scp, uint64_t n, const float *__restrict__ O,
+#ifndef RT_CAST_WAVES_PER_EU
+#define RT_CAST_WAVES_PER_EU 1
+#endif
+__global__ __launch_bounds__(256, COUNT ? 1 : RT_CAST_WAVES_PER_EU) void k_cast(const DevScene *__restrict__ scp, uint64_t n, const float *__restrict__ O,
                                               const float *__restrict__ D, float maxd, float *__restrict__ T,
                                               float *__restrict__ N, uint32_t *__restrict__ U,
                                               unsigned long long *counter, int *err, unsigned long long *stats) {
     constexpr int LSTK = STK < RT_DIFFUSE_LDS_STACK ? STK : RT_DIFFUSE_LDS_STACK;
-    __shared__ uint32_t stk_all[LSTK * 256];
+    __shared__ uint32_t stk_all[(LSTK + 1) * 256];  // + the spare row of the branch-free BVH4 pushes
     uint32_t ovf[STK > LSTK ? RT_STACK - LSTK : 1];
     lds_u32 *stk = (lds_u32 *)stk_all + threadIdx.x;
     const uint32_t stride = 256;
