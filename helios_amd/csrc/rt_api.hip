@@ -244,8 +244,11 @@ int rt_cast_rays_soa(rt_ctx *ctx, const rt_scene *scene, const float *o, const f
 
 int rt_cast_rays_soa_packets(rt_ctx *ctx, const rt_scene *scene, const float *o, const float *d, size_t n,
                              size_t packet_size, float max_dist, float *t, float *nrm, uint32_t *uuid) {
-    // Results equal the per-ray path (CollisionDetection.h:404-408); each wave already takes 64
-    // consecutive rays, so the sub-rays of a pulse traverse together whatever packet_size is.
+    // Results equal the per-ray path (CollisionDetection.h:404-408).  The reference's packet traversal is
+    // its CPU path; with a GPU-resident scene it defers to the per-ray GPU kernel, whose massively parallel
+    // traversal the packet sharing does not map onto (CD/src/CollisionDetection_RayTracing.cpp:714-720).
+    // The same holds here: a wave takes 64 consecutive rays, so the sub-rays of a pulse traverse together
+    // (the nodes they share come from the same cache lines) whatever packet_size is.
     (void)packet_size;
     return rt_cast_rays_soa(ctx, scene, o, d, n, max_dist, t, nrm, uuid);
 }

@@ -1533,8 +1533,13 @@ __global__ __launch_bounds__(256) void k_gather_rays(const GatherJob *__restrict
 #ifndef RT_GATHER_UNROLL
 #define RT_GATHER_UNROLL 4
 #endif
+// Waves per SIMD of the channel-parallel gather: 8 (64 VGPRs, a few spills) took the diurnal series'
+// gathers from 180 to 166 ms against the 5 the compiler's register use allowed.
+#ifndef RT_GATHER_WAVES_PER_EU
+#define RT_GATHER_WAVES_PER_EU 8
+#endif
 template <int CB, bool ACCT>
-__global__ __launch_bounds__(256) void k_gather_chan(const GatherJob *__restrict__ jobs, uint32_t njobs, uint32_t total,
+__global__ __launch_bounds__(256, RT_GATHER_WAVES_PER_EU) void k_gather_chan(const GatherJob *__restrict__ jobs, uint32_t njobs, uint32_t total,
                                                      uint32_t *counter, uint32_t chunk) {
     const uint32_t lane = threadIdx.x & 63;
     for (uint32_t first = wave_item(counter, chunk); first < total; first = wave_item(counter, chunk)) {

@@ -93,7 +93,8 @@ int rt_scene_set_masks(rt_ctx *ctx, rt_scene *scene, uint32_t nmasks, const uint
 int rt_cast_rays_soa(rt_ctx *ctx, const rt_scene *scene, const float *origins, const float *dirs, size_t n,
                      float max_dist, float *out_t, float *out_normal, uint32_t *out_uuid);
 /* CollisionDetection::castRaysSoA_packets (CD/include/CollisionDetection.h:395-422): rays grouped in
- * pulses of packet_size contiguous sub-rays; results identical to rt_cast_rays_soa. */
+ * pulses of packet_size contiguous sub-rays; results identical to rt_cast_rays_soa, and the same kernel,
+ * as the reference's own GPU-resident path (CD/src/CollisionDetection_RayTracing.cpp:714-720). */
 int rt_cast_rays_soa_packets(rt_ctx *ctx, const rt_scene *scene, const float *origins, const float *dirs, size_t n,
                              size_t packet_size, float max_dist, float *out_t, float *out_normal, uint32_t *out_uuid);
 /* LiDARcloud::castRaysUnified (plugins/lidar/src/LiDAR.cpp:558-598), the LiDAR synthetic scan's ray cast:
