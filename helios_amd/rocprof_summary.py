@@ -5,8 +5,9 @@ usage: python -m helios_amd.rocprof_summary KERNEL_TRACE_DB [PMC_DB ...] [--json
 
 With --json, also writes HBM bytes per dispatch of each ray kernel: FETCH_SIZE and WRITE_SIZE
 (KiB) scaled by what the same counters read for the calibration copy of known size.  A PMC run with
-TCC_HIT_sum / TCC_MISS_sum adds each ray kernel's L2 hit rate (its largest dispatch).  --brief leaves
-out the per-dispatch counter listing.
+TCC_HIT_sum / TCC_MISS_sum adds each ray kernel's L2 hit rate (its largest dispatch); other counters (SQ_*
+and the like) are listed for each ray kernel's largest dispatch.  --brief leaves out the per-dispatch
+listing of FETCH_SIZE / WRITE_SIZE.
 """
 from __future__ import annotations
 
@@ -57,6 +58,7 @@ def summarize(kt_db, pmc_dbs, calib_bytes=1 << 30, brief=False):
     traffic = collections.defaultdict(lambda: collections.defaultdict(list))
     scale = {}
     l2 = collections.defaultdict(dict)  # kernel -> dispatch -> {counter: value}
+    other = collections.defaultdict(dict)  # kernel -> dispatch -> {counter: value} (SQ_* and the like)
     for db in pmc_dbs:
         rows = counter_rows(db)
         if not brief:
@@ -66,6 +68,11 @@ def summarize(kt_db, pmc_dbs, calib_bytes=1 << 30, brief=False):
             if c.startswith("TCC_HIT") or c.startswith("TCC_MISS"):
                 if any(t in sn for t in RAY_KERNELS):
                     l2[sn].setdefault(d, {})[c] = val
+                continue
+            if c not in ("FETCH_SIZE", "WRITE_SIZE"):
+                if any(t in sn for t in RAY_KERNELS):
+                    cs = other[sn].setdefault(d, {})
+                    cs[c] = cs.get(c, 0.0) + val
                 continue
             if "copyBuffer" in sn and val >= calib_bytes / 1024 / 4:
                 scale[c] = calib_bytes / 1024.0 / val
@@ -84,6 +91,11 @@ def summarize(kt_db, pmc_dbs, calib_bytes=1 << 30, brief=False):
             if hit + miss > 0:
                 l2_rate[k] = hit / (hit + miss)
                 out.append(f"{k}: dispatch {d} hits {hit:.3e} misses {miss:.3e} hit rate {hit / (hit + miss):.3f}")
+    if other:
+        out.append("\n== other counters of each ray kernel's largest dispatch (raw values)")
+        for k, ds in other.items():
+            d, cs = max(ds.items(), key=lambda kv: sum(kv[1].values()))
+            out.append(f"{k}: dispatch {d}: " + ", ".join(f"{c} {v:.4g}" for c, v in sorted(cs.items())))
     summary = {}
     if scale:
         out.append("\n== HBM bytes per dispatch (counter KiB x 1024 x calibration factor)")
