@@ -100,11 +100,54 @@ RT_HD void sincos_2pi(float r, float &s, float &c) {
 //   r1 = a|e1 .xyz, |a|^2
 //   r2 = b|e2 .xyz, |b|^2
 //   r3 = patch plane normal .xyz, 0
+// A voxel (type 4) carries RT_VOXEL_BIT: r0 = its box's lower corner, r1 = the upper corner.
 #define RT_PATCH_BIT 0x80000000u
 // A primitive with a texture mask carries RT_MASK_BIT in its tag and, in r3.w, the float4 index (from the
 // record base) of its mask descriptor; the index bits of a tag are RT_ID_BITS.
 #define RT_MASK_BIT 0x40000000u
-#define RT_ID_BITS 0x3FFFFFFFu
+#define RT_VOXEL_BIT 0x20000000u
+#define RT_ID_BITS 0x1FFFFFFFu
+
+// castRays' voxel test (CD/src/CollisionDetection_RayTracing.cpp:1625-1680): slab method over the voxel's
+// axis-aligned box [lo, hi] with the reference's arithmetic -- a direction component below 1e-8 in magnitude
+// is parallel (a miss when the origin lies outside that slab), else t1, t2 = (lo - o) / d, (hi - o) / d;
+// the entry and exit distances are the largest t1 and the smallest t2; the hit distance is the entry if
+// it is at least 1e-6, else the exit (an origin inside the box); a hit needs exit >= 0 and distance >= 1e-6
+// (the closest-hit test takes a voxel at exactly 1e-6, a surface only beyond it).  The radiation passes do
+// not intersect voxels (R/shaders/common/intersections.glsl:128-133).
+RT_HD bool voxel_hit_cd(const float4 lo, const float4 hi, v3f o, v3f d, float &t_out) {
+    const float dc[3] = {d.x, d.y, d.z}, oc[3] = {o.x, o.y, o.z};
+    const float lc[3] = {lo.x, lo.y, lo.z}, hc[3] = {hi.x, hi.y, hi.z};
+    float tn = -3.402823466e+38f, tf = 3.402823466e+38f;
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        if (fabsf(dc[a]) < 1e-8f) {
+            ok = ok && !(oc[a] < lc[a] || oc[a] > hc[a]);
+        } else {
+            float t1 = rdiv(lc[a] - oc[a], dc[a]), t2 = rdiv(hc[a] - oc[a], dc[a]);
+            if (t1 > t2) { const float x = t1; t1 = t2; t2 = x; }
+            tn = fmaxf(tn, t1);
+            tf = fminf(tf, t2);
+            ok = ok && !(tn > tf);
+        }
+    }
+    const float dist = tn >= 1e-6f ? tn : tf;
+    t_out = dist;
+    return ok && tf >= 0.0f && dist >= 1e-6f;
+}
+
+// Outward normal of the voxel face a castRays hit lies on (CollisionDetection_RayTracing.cpp:1685-1711):
+// the axis along which the hit point is relatively farthest from the box centre.
+RT_HD v3f voxel_normal_cd(const float4 lo, const float4 hi, v3f p) {
+    const v3f c = mk3((lo.x + hi.x) * 0.5f, (lo.y + hi.y) * 0.5f, (lo.z + hi.z) * 0.5f);
+    const v3f e = mk3((hi.x - lo.x) * 0.5f, (hi.y - lo.y) * 0.5f, (hi.z - lo.z) * 0.5f);
+    const v3f l = sub3(p, c);
+    const float rx = rdiv(fabsf(l.x), e.x), ry = rdiv(fabsf(l.y), e.y), rz = rdiv(fabsf(l.z), e.z);
+    if (rx >= ry && rx >= rz) return mk3(l.x > 0.f ? 1.0f : -1.0f, 0.f, 0.f);
+    if (ry >= rz) return mk3(0.f, l.y > 0.f ? 1.0f : -1.0f, 0.f);
+    return mk3(0.f, 0.f, l.z > 0.f ? 1.0f : -1.0f);
+}
 
 // Radiation predicates (intersections.glsl) and CD predicates share the arithmetic; only the
 // acceptance thresholds differ.  Written as predicates (every quantity computed, the tests combined at the

@@ -80,12 +80,18 @@ __device__ __forceinline__ int cell_of(float q, float g0, float inv_cs, uint32_t
     return c;
 }
 
+// voxels occlude no radiation ray (the radiation traversal tests patches, tiles and triangles only), so
+// the grids leave them out
+__device__ __forceinline__ bool leaf_is_voxel(const float4 *__restrict__ rec, uint32_t k) {
+    return (__float_as_uint(rec[(size_t)k * 4].w) & RT_VOXEL_BIT) != 0;
+}
+
 __global__ void k_grid_bounds(uint32_t nleaf, const float4 *__restrict__ rec, float smax,
                               const SunGrid *__restrict__ grids, uint32_t *__restrict__ bounds /*[n][4]*/) {
     const SunGrid G = grids[blockIdx.y];
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     float u0 = INFINITY, u1 = -INFINITY, v0 = INFINITY, v1 = -INFINITY, key;
-    if (k < nleaf) proj_rect(rec, k, G, smax, u0, u1, v0, v1, key);
+    if (k < nleaf && !leaf_is_voxel(rec, k)) proj_rect(rec, k, G, smax, u0, u1, v0, v1, key);
     // block reduction, then one atomic per block
     __shared__ float s[4][256];
     s[0][threadIdx.x] = u0; s[1][threadIdx.x] = -u1; s[2][threadIdx.x] = v0; s[3][threadIdx.x] = -v1;
@@ -109,7 +115,7 @@ __global__ void k_grid_count(uint32_t nleaf, const float4 *__restrict__ rec, flo
                              uint32_t *__restrict__ counts, unsigned long long *__restrict__ entries /*[n]*/) {
     const SunGrid G = grids[blockIdx.y];
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nleaf) return;
+    if (k >= nleaf || leaf_is_voxel(rec, k)) return;
     float u0, u1, v0, v1, key;
     proj_rect(rec, k, G, smax, u0, u1, v0, v1, key);
     const int x0 = cell_of(u0, G.g0u, G.inv_cs, G.nx), x1 = cell_of(u1, G.g0u, G.inv_cs, G.nx);
@@ -163,7 +169,7 @@ __global__ void k_grid_fill(uint32_t nleaf, const float4 *__restrict__ rec, floa
                             uint32_t *__restrict__ cells) {
     const SunGrid G = grids[blockIdx.y];
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nleaf) return;
+    if (k >= nleaf || leaf_is_voxel(rec, k)) return;
     float u0, u1, v0, v1, key;
     proj_rect(rec, k, G, smax, u0, u1, v0, v1, key);
     const int x0 = cell_of(u0, G.g0u, G.inv_cs, G.nx), x1 = cell_of(u1, G.g0u, G.inv_cs, G.nx);

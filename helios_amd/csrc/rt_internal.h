@@ -28,7 +28,7 @@ constexpr uint64_t RT_MAX_PRIMS = (uint64_t)RT_ID_BITS + 1u;
 // most one chunk per wave (<= 256 items x 65,536 waves) before every wave has seen the end.
 constexpr uint64_t RT_MAX_WORK_ITEMS = 0xFFFFFFFFull - (1ull << 25);
 // Checks a primitive set before it is staged or built: at most RT_MAX_PRIMS primitives, every type one this
-// backend traces (0 patch, 1 triangle, 3 tile); disks (2), voxels (4) and anything else are rejected with an
+// backend takes (0 patch, 1 triangle, 3 tile, 4 voxel); disks (2) and anything else are rejected with an
 // error naming the type, so no primitive silently drops out of the scene.  `who` prefixes the message.
 int validate_geometry(uint64_t P, const uint32_t *types, const char *who);
 // Rejects a launch of more than RT_MAX_WORK_ITEMS work items (e.g. suns x sources x primitives of a direct pass).

@@ -36,7 +36,8 @@ __host__ __device__ __forceinline__ float o2f(uint32_t u) {
 }
 
 __device__ __forceinline__ bool is_patch_t(uint32_t t) { return t == 0 || t == 3; }
-__device__ __forceinline__ bool traced_t(uint32_t t) { return t == 0 || t == 1 || t == 3; }
+// primitives that are leaves of the tree: the traced surfaces and voxels (castRays only)
+__device__ __forceinline__ bool traced_t(uint32_t t) { return t == 0 || t == 1 || t == 3 || t == 4; }
 
 // bounds layout (uint, order-preserving): [0..2] centroid min, [3..5] centroid max,
 // [6..8] vertex min, [9..11] vertex max, [12] traced count
@@ -96,16 +97,36 @@ __global__ void k_prep(uint32_t P, const float *__restrict__ T, const uint32_t *
     float aa = patch ? dot3(e1, e1) : 0.f, bb = patch ? dot3(e2, e2) : 0.f;
     uint32_t tag = i | (patch ? RT_PATCH_BIT : 0u);
     float4 *rr = rec_u + (size_t)i * 4;
-    rr[0] = make_float4(v[0].x, v[0].y, v[0].z, __uint_as_float(tag));
-    rr[1] = make_float4(e1.x, e1.y, e1.z, aa);
-    rr[2] = make_float4(e2.x, e2.y, e2.z, bb);
-    rr[3] = make_float4(pn.x, pn.y, pn.z, 0.f);
     // raw AABB (inflated later) and bounds
     int nv = patch ? 4 : 3;
     float lo[3] = {v[0].x, v[0].y, v[0].z}, hi[3] = {v[0].x, v[0].y, v[0].z};
     for (int k = 1; k < nv; k++) {
         float c[3] = {v[k].x, v[k].y, v[k].z};
         for (int a = 0; a < 3; a++) { lo[a] = fminf(lo[a], c[a]); hi[a] = fmaxf(hi[a], c[a]); }
+    }
+    if (ty == 4) {
+        // a voxel: the box of its eight transformed unit-cube corners, as castRays takes it from the
+        // Context's vertices (core/src/Context_primitive.cpp:917-929, in that corner order) with a running
+        // min/max that keeps the earlier of equal values (CD/src/CollisionDetection_RayTracing.cpp:1627-1637)
+        for (int k = 0; k < 8; k++) {
+            const float cx = ((k & 3) == 1 || (k & 3) == 2) ? 0.5f : -0.5f, cy = (k & 2) ? 0.5f : -0.5f;
+            const v3f c = xform3(t, cx, cy, (k & 4) ? 0.5f : -0.5f);
+            const float cc[3] = {c.x, c.y, c.z};
+            for (int a = 0; a < 3; a++) {
+                lo[a] = (k == 0 || cc[a] < lo[a]) ? cc[a] : lo[a];
+                hi[a] = (k == 0 || cc[a] > hi[a]) ? cc[a] : hi[a];
+            }
+        }
+        tag = i | RT_VOXEL_BIT;
+        rr[0] = make_float4(lo[0], lo[1], lo[2], __uint_as_float(tag));
+        rr[1] = make_float4(hi[0], hi[1], hi[2], 0.f);
+        rr[2] = make_float4(0.f, 0.f, 0.f, 0.f);
+        rr[3] = make_float4(0.f, 0.f, 0.f, 0.f);
+    } else {
+        rr[0] = make_float4(v[0].x, v[0].y, v[0].z, __uint_as_float(tag));
+        rr[1] = make_float4(e1.x, e1.y, e1.z, aa);
+        rr[2] = make_float4(e2.x, e2.y, e2.z, bb);
+        rr[3] = make_float4(pn.x, pn.y, pn.z, 0.f);
     }
     float *bx = box + (size_t)i * 6;
     for (int a = 0; a < 3; a++) { bx[a] = lo[a]; bx[3 + a] = hi[a]; }
@@ -631,16 +652,15 @@ static int build_bvh4(uint32_t n, const uint32_t *child, const float *lbox, cons
 
 int validate_geometry(uint64_t P, const uint32_t *types, const char *who) {
     if (P > RT_MAX_PRIMS) {
-        set_error(std::string(who) + ": " + std::to_string(P) + " primitives (at most 2^30)");
+        set_error(std::string(who) + ": " + std::to_string(P) + " primitives (at most 2^29)");
         return -1;
     }
     for (uint64_t i = 0; i < P; i++) {
         const uint32_t t = types[i];
-        if (t == 0u || t == 1u || t == 3u) continue;
+        if (t == 0u || t == 1u || t == 3u || t == 4u) continue;
         const char *what = t == 2u ? "a disk (type 2): disks are not traced by this backend (Helios disks are "
                                      "triangle fans; pass their triangles)"
-                           : t == 4u ? "a voxel (type 4): voxels are not traced by this backend"
-                                     : "an unknown type (0 patch, 1 triangle, 3 tile)";
+                                   : "an unknown type (0 patch, 1 triangle, 3 tile, 4 voxel)";
         set_error(std::string(who) + ": primitive " + std::to_string(i) + " is " + what);
         return -1;
     }
@@ -840,7 +860,7 @@ int scene_set_masks(Scene &S, uint32_t nmasks, const uint8_t *texels, const uint
                     const float *uv, const int32_t *uv_id, hipStream_t st) {
     const uint32_t P = S.h.P, nleaf = S.h.nleaf;
     if (!S.d) { set_error("texture masks: no scene"); return -1; }
-    if (P > RT_ID_BITS) { set_error("texture masks: more than 2^30 primitives"); return -1; }
+    if (P > RT_ID_BITS) { set_error("texture masks: more than 2^29 primitives"); return -1; }
     if (nmasks && (!texels || !sizes)) { set_error("texture masks: texels and sizes are required"); return -1; }
     if (nmasks && !mask_id) { set_error("texture masks: mask_id is required"); return -1; }
     std::vector<uint64_t> toff(nmasks + 1, 0);

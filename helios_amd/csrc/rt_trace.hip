@@ -143,9 +143,13 @@ __device__ __forceinline__ bool leaf_hit_p(const float4 *recs, uint32_t k, v3f o
     bool patch = (tag & RT_PATCH_BIT) != 0;
     const bool masked = (MK & RT_FL_MASK) && !CD && (tag & RT_MASK_BIT) != 0;
     float4 r1 = ldc(r + 1), r2 = ldc(r + 2);
+    // a voxel is a box that castRays hits (CollisionDetection_RayTracing.cpp:1625-1711) and the radiation
+    // traversal never does (bvh_traversal.glsl:118-140 tests patches, tiles and triangles only)
+    if (CD && (tag & RT_VOXEL_BIT)) return voxel_hit_cd(r0, r1, o, d, t) && id != skip;
     float4 r3 = (patch || masked) ? ldc(r + 3) : make_float4(0.f, 0.f, 0.f, 0.f);
     float ua, ub;
-    bool hit = isect_rec_uv<CD, (MK & RT_FL_WT) != 0>(r0, r1, r2, r3, patch, o, d, t, ua, ub) && id != skip;
+    bool hit = isect_rec_uv<CD, (MK & RT_FL_WT) != 0>(r0, r1, r2, r3, patch, o, d, t, ua, ub) && id != skip &&
+               (CD || !(tag & RT_VOXEL_BIT));
     // texture mask (bvh_traversal.glsl:143-148): a transparent texel lets the ray pass
     if (masked && hit) hit = mask_hit_opaque(recs, __float_as_uint(r3.w), patch, ua, ub, r1.w, r2.w);
     return hit;
@@ -162,7 +166,10 @@ __device__ __forceinline__ void test_leaf(const float4 *recs, uint32_t k, v3f o,
                                           float &best_t, uint32_t &best, uint32_t &best_slot) {
     uint32_t id;
     float t;
-    if (leaf_hit_p<CD, MK>(recs, k, o, d, skip, id, t) && t > tmin) {
+    // castRays takes a patch or triangle beyond 1e-6 and a voxel at 1e-6 or beyond
+    // (CD/src/CollisionDetection_RayTracing.cpp:1562, 1595, 1681)
+    if (leaf_hit_p<CD, MK>(recs, k, o, d, skip, id, t) &&
+        (t > tmin || (CD && t == tmin && (__float_as_uint(ldc(recs + (size_t)k * 4).w) & RT_VOXEL_BIT)))) {
         if (t < best_t || (t == best_t && best != RT_MISS && id < best)) {
             best_t = t;
             best = id;
@@ -1711,13 +1718,17 @@ __global__ __launch_bounds__(256, COUNT ? 1 : RT_CAST_WAVES_PER_EU) void k_cast(
                 U[my_r] = uuid[h];
                 T[my_r] = L.best_t;
                 const float4 *rr = recs + (size_t)slot * 4;
-                const float4 r1 = rr[1], r2 = rr[2], r3 = rr[3];
-                const bool patch = (__float_as_uint(rr[0].w) & RT_PATCH_BIT) != 0;
+                const float4 r0 = rr[0], r1 = rr[1], r2 = rr[2], r3 = rr[3];
+                const uint32_t tag = __float_as_uint(r0.w);
+                const bool patch = (tag & RT_PATCH_BIT) != 0;
                 // CD/src/CollisionDetection_RayTracing.cpp:1572-1585: normal of cross(v1-v0, v2-v0), facing
-                // the origin
+                // the origin; a voxel's is the outward normal of the face hit (1685-1711), not turned
                 v3f nn;
                 bool ok;
-                if (patch) {
+                if (tag & RT_VOXEL_BIT) {
+                    nn = voxel_normal_cd(r0, r1, add3(L.o, mul3(L.d, L.best_t)));
+                    ok = false;
+                } else if (patch) {
                     nn = mk3(r3.x, r3.y, r3.z);
                     v3f raw = cross3(mk3(r1.x, r1.y, r1.z), add3(mk3(r1.x, r1.y, r1.z), mk3(r2.x, r2.y, r2.z)));
                     ok = rsqrt_cr(raw.x * raw.x + raw.y * raw.y + raw.z * raw.z) > 1e-8f;
@@ -1731,7 +1742,7 @@ __global__ __launch_bounds__(256, COUNT ? 1 : RT_CAST_WAVES_PER_EU) void k_cast(
                     v3f ip = add3(L.o, mul3(L.d, L.best_t));
                     v3f to = sub3(L.o, ip);
                     if (nn.x * to.x + nn.y * to.y + nn.z * to.z < 0) nn = mul3(nn, -1.0f);
-                } else {
+                } else if (!(tag & RT_VOXEL_BIT)) {
                     nn = nrm3(mk3(-L.d.x, -L.d.y, -L.d.z));
                 }
                 N[my_r * 3] = nn.x;

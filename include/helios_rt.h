@@ -10,11 +10,17 @@
  *     index" that seeds the per-ray RNG (R/shaders/direct_raygen.comp:360-362);
  *   - transforms are 4x4 row-major; canonical patch (-0.5..0.5)^2, canonical triangle
  *     (0,0,0),(0,1,0),(1,1,0) (R/src/backends/VulkanComputeBackend.cpp:328-329);
- *   - types: 0 patch, 1 triangle, 3 tile (traced as patch); scene creation and rt_update_geometry reject
- *     any other type with an error naming it, so no primitive silently drops out of a scene (the Vulkan
- *     backend traces patch/tile/triangle only, R/shaders/common/bvh_traversal.glsl:311-325; the OptiX 8
- *     backend rejects disks and voxels, R/src/optix8/OptiX8DeviceCode.cu:492-504);
- *   - at most 2^30 primitives per scene; one direct pass covers at most 2^32 - 2^25 (sun, source,
+ *   - types: 0 patch, 1 triangle, 3 tile (traced as patch), 4 voxel; scene creation and
+ *     rt_update_geometry reject any other type (disks, 2, included) with an error naming it, so no primitive
+ *     silently drops out of a scene (the Vulkan backend traces patch/tile/triangle only,
+ *     R/shaders/common/bvh_traversal.glsl:311-325; the OptiX 8 backend rejects disks and voxels,
+ *     R/src/optix8/OptiX8DeviceCode.cu:492-504);
+ *   - a voxel is the axis-aligned box of its eight transformed unit-cube corners: castRays hits it as
+ *     CollisionDetection does (CD/src/CollisionDetection_RayTracing.cpp:1625-1711); the radiation passes,
+ *     cameras and pixel labels neither hit it nor launch rays from it (the Vulkan traversal never hits
+ *     one, bvh_traversal.glsl:118-140, and its ray generation has no sample point for one,
+ *     R/shaders/direct_raygen.comp:417-426), so its radiation_flux is 0;
+ *   - at most 2^29 primitives per scene; one direct pass covers at most 2^32 - 2^25 (sun, source,
  *     primitive) work items, and larger requests fail with an error saying so;
  *   - twosided: raw Helios flag 0..3 (R/src/RadiationModel.cpp:5941);
  *   - radiation buffers are [primitive][band] (R/include/BufferIndexing.h:65-262).

@@ -151,6 +151,20 @@ static void prep_prim(orc_scene *s, uint32_t i) {
         v[1] = xform(t, 0.f, 1.f, 0.f);
         v[2] = xform(t, 1.f, 1.f, 0.f);
         v[3] = v[2];
+    } else if (ty == 4) {
+        /* a voxel's box: running min/max over the Context's eight corners in their order
+         * (core/src/Context_primitive.cpp:917-929), as castRays forms it
+         * (CD/src/CollisionDetection_RayTracing.cpp:1627-1637); v[0] = lower, v[1] = upper corner */
+        static const float Y[8][3] = {{-0.5f, -0.5f, -0.5f}, {0.5f, -0.5f, -0.5f}, {0.5f, 0.5f, -0.5f},
+                                      {-0.5f, 0.5f, -0.5f}, {-0.5f, -0.5f, 0.5f}, {0.5f, -0.5f, 0.5f},
+                                      {0.5f, 0.5f, 0.5f}, {-0.5f, 0.5f, 0.5f}};
+        v3 lo = xform(t, Y[0][0], Y[0][1], Y[0][2]), hi = lo;
+        for (int k = 1; k < 8; k++) {
+            v3 c = xform(t, Y[k][0], Y[k][1], Y[k][2]);
+            if (c.x < lo.x) lo.x = c.x; if (c.y < lo.y) lo.y = c.y; if (c.z < lo.z) lo.z = c.z;
+            if (c.x > hi.x) hi.x = c.x; if (c.y > hi.y) hi.y = c.y; if (c.z > hi.z) hi.z = c.z;
+        }
+        v[0] = lo; v[1] = hi; v[2] = v[3] = lo;
     } else {
         v[0] = v[1] = v[2] = v[3] = xform(t, 0.f, 0.f, 0.f);
     }
@@ -191,7 +205,7 @@ static void prep_prim(orc_scene *s, uint32_t i) {
     /* conservative AABB: exact vertex bounds widened by 2^-18 of the extent plus 2^-20 of the
      * magnitude, so a slab test can never reject a ray that a predicate (incl. the 1e-5 barycentric
      * slack of the CD triangle test) would accept. */
-    int nv = is_patch(ty) ? 4 : 3;
+    int nv = is_patch(ty) ? 4 : (ty == 4 ? 2 : 3);
     float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
     for (int k = 0; k < nv; k++) {
         float c[3] = {v[k].x, v[k].y, v[k].z};
@@ -331,7 +345,8 @@ static void build_periodic(orc_scene *s) {
     if (!s->px && !s->py) return;
     float xb[2] = {FLT_MAX, -FLT_MAX}, yb[2] = {FLT_MAX, -FLT_MAX}, zb[2] = {FLT_MAX, -FLT_MAX};
     for (uint32_t i = 0; i < s->P; i++) {
-        int nv = is_patch(s->type[i]) ? 4 : 3;
+        /* Context::getDomainBoundingBox: every vertex, a voxel's eight corners (its box) included */
+        int nv = is_patch(s->type[i]) ? 4 : (s->type[i] == 4 ? 2 : 3);
         for (int k = 0; k < nv; k++) {
             v3 v = s->verts[(size_t)i * 4 + k];
             if (v.x < xb[0]) xb[0] = v.x; if (v.x > xb[1]) xb[1] = v.x;
@@ -392,7 +407,7 @@ orc_scene *orc_scene_create(uint32_t P, const float *transforms, const uint32_t 
     s->pidx = (uint32_t *)malloc(sizeof(uint32_t) * (P ? P : 1));
     uint32_t np = 0;
     for (uint32_t i = 0; i < P; i++)
-        if (s->type[i] <= 1 || s->type[i] == 3) s->pidx[np++] = i;
+        if (s->type[i] <= 1 || s->type[i] == 3 || s->type[i] == 4) s->pidx[np++] = i;
     if (np) {
         uint32_t root = new_node(s);
         build_rec(s, root, 0, np, 0);
@@ -695,6 +710,51 @@ static int isect_patch_cd(const orc_scene *s, uint32_t i, v3 o, v3 d, float *t_o
     return 0;
 }
 
+/* castRays' voxel test, CD/src/CollisionDetection_RayTracing.cpp:1640-1682: slab method over the box in
+ * FP32 with IEEE division; a component below 1e-8 in magnitude is parallel; the distance is the entry if it
+ * is at least 1e-6, else the exit (an origin inside the box) */
+static int isect_voxel_cd(const orc_scene *s, uint32_t i, v3 o, v3 d, float *t_out) {
+    const v3 lo = s->verts[(size_t)i * 4], hi = s->verts[(size_t)i * 4 + 1];
+    const float dc[3] = {d.x, d.y, d.z}, oc[3] = {o.x, o.y, o.z};
+    const float lc[3] = {lo.x, lo.y, lo.z}, hc[3] = {hi.x, hi.y, hi.z};
+    float tn = -FLT_MAX, tf = FLT_MAX;
+    for (int a = 0; a < 3; a++) {
+        if (fabsf(dc[a]) < 1e-8f) {
+            if (oc[a] < lc[a] || oc[a] > hc[a]) return 0;
+        } else {
+            float t1 = (lc[a] - oc[a]) / dc[a], t2 = (hc[a] - oc[a]) / dc[a];
+            if (t1 > t2) { float x = t1; t1 = t2; t2 = x; }
+            if (t1 > tn) tn = t1;
+            if (t2 < tf) tf = t2;
+            if (tn > tf) return 0;
+        }
+    }
+    if (!(tf >= 0.0f)) return 0;
+    float dist = tn >= 1e-6f ? tn : tf;
+    if (!(dist >= 1e-6f)) return 0;
+    *t_out = dist;
+    return 1;
+}
+
+/* outward normal of the voxel face hit, CD/src/CollisionDetection_RayTracing.cpp:1689-1711 */
+static v3 voxel_normal_cd(const orc_scene *s, uint32_t i, v3 p) {
+    const v3 lo = s->verts[(size_t)i * 4], hi = s->verts[(size_t)i * 4 + 1];
+    v3 c = scl(add(lo, hi), 0.5f), e = scl(sub(hi, lo), 0.5f), l = sub(p, c);
+    float rx = fabsf(l.x) / e.x, ry = fabsf(l.y) / e.y, rz = fabsf(l.z) / e.z;
+    if (rx >= ry && rx >= rz) return mk(l.x > 0 ? 1.0f : -1.0f, 0.f, 0.f);
+    if (ry >= rz) return mk(0.f, l.y > 0 ? 1.0f : -1.0f, 0.f);
+    return mk(0.f, 0.f, l.z > 0 ? 1.0f : -1.0f);
+}
+
+/* One primitive's predicate: castRays' in cd_mode (voxels included), else the radiation traversal's,
+ * which tests patches, tiles and triangles only (R/shaders/common/bvh_traversal.glsl:118-140). */
+static int isect_prim(const orc_scene *s, uint32_t i, v3 o, v3 d, int cd_mode, float *t) {
+    const uint32_t ty = s->type[i];
+    if (ty == 4) return cd_mode ? isect_voxel_cd(s, i, o, d, t) : 0;
+    if (cd_mode) return is_patch(ty) ? isect_patch_cd(s, i, o, d, t) : isect_tri_cd(s, i, o, d, t);
+    return is_patch(ty) ? isect_patch_rad(s, i, o, d, t) : isect_tri_rad(s, i, o, d, t);
+}
+
 /* conservative slab test in FP64 (boxes are inflated; culling decisions never change results) */
 static int slab(const float *lo, const float *hi, const double o[3], const double inv[3], double tmax) {
     double tn = -1e300, tf = 1e300;
@@ -725,12 +785,9 @@ static uint32_t closest(const orc_scene *s, v3 o, v3 d, float tmin, uint32_t ski
     if (brute) {
         for (uint32_t i = 0; i < s->P; i++) {
             uint32_t ty = s->type[i];
-            if (i == skip || !(ty <= 1 || ty == 3)) continue;
+            if (i == skip || !(ty <= 1 || ty == 3 || ty == 4)) continue;
             float t;
-            int h;
-            if (cd_mode) h = is_patch(ty) ? isect_patch_cd(s, i, o, d, &t) : isect_tri_cd(s, i, o, d, &t);
-            else h = is_patch(ty) ? isect_patch_rad(s, i, o, d, &t) : isect_tri_rad(s, i, o, d, &t);
-            if (!h || !(t > tmin)) continue;
+            if (!isect_prim(s, i, o, d, cd_mode, &t) || !(t > tmin || (cd_mode && ty == 4 && t == tmin))) continue;
             if (t < bt || (t == bt && best != MISS && i < best)) { bt = t; best = i; }
         }
         *t_out = bt;
@@ -744,12 +801,12 @@ static uint32_t closest(const orc_scene *s, v3 o, v3 d, float tmin, uint32_t ski
             for (uint32_t k = 0; k < nd->count; k++) {
                 uint32_t i = s->pidx[nd->start + k];
                 if (i == skip) continue;
-                uint32_t ty = s->type[i];
                 float t;
-                int h;
-                if (cd_mode) h = is_patch(ty) ? isect_patch_cd(s, i, o, d, &t) : isect_tri_cd(s, i, o, d, &t);
-                else h = is_patch(ty) ? isect_patch_rad(s, i, o, d, &t) : isect_tri_rad(s, i, o, d, &t);
-                if (!h || !(t > tmin)) continue;
+                /* castRays takes a patch or triangle beyond 1e-6 and a voxel at 1e-6 or beyond
+                 * (CD/src/CollisionDetection_RayTracing.cpp:1562, 1595, 1681) */
+                if (!isect_prim(s, i, o, d, cd_mode, &t) ||
+                    !(t > tmin || (cd_mode && s->type[i] == 4 && t == tmin)))
+                    continue;
                 if (t < bt || (t == bt && best != MISS && i < best)) { bt = t; best = i; }
             }
         } else {
@@ -1134,6 +1191,11 @@ void orc_cast_rays(const orc_scene *s, uint64_t n, const float *origins, const f
         if (h == MISS) { out_uuid[r] = MISS; continue; }
         out_uuid[r] = s->uuid[h];
         out_t[r] = t;
+        if (s->type[h] == 4) {
+            v3 nn = voxel_normal_cd(s, h, add(o, scl(d, t)));
+            out_normal[r * 3] = nn.x; out_normal[r * 3 + 1] = nn.y; out_normal[r * 3 + 2] = nn.z;
+            continue;
+        }
         const v3 *v = s->verts + (size_t)h * 4;
         v3 nn = cross(sub(v[1], v[0]), sub(v[2], v[0]));
         float mag = sqrtf(nn.x * nn.x + nn.y * nn.y + nn.z * nn.z);

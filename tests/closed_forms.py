@@ -20,6 +20,26 @@ def patch_T(center, ex, ey):
     return T
 
 
+def voxel_T(center, size, rotation=0.0):
+    """Row-major transform of a voxel as Context::addVoxel forms it (core/src/Context_primitive.cpp:256-269):
+    scale by size, rotate by ``rotation`` about z, translate to center; the unit cube is (-0.5..0.5)^3."""
+    c, s = np.cos(rotation), np.sin(rotation)
+    T = np.zeros(16, np.float32)
+    T[0], T[1], T[4], T[5] = c * size[0], -s * size[1], s * size[0], c * size[1]
+    T[10] = size[2]
+    T[3], T[7], T[11] = center
+    T[15] = 1.0
+    return T
+
+
+def voxel_box(T):
+    """FP64 axis-aligned box of a voxel's eight transformed corners."""
+    M = np.asarray(T, np.float64).reshape(4, 4)
+    C = np.array([[x, y, z, 1.0] for x in (-0.5, 0.5) for y in (-0.5, 0.5) for z in (-0.5, 0.5)])
+    W = C @ M.T
+    return W[:, :3].min(0), W[:, :3].max(0)
+
+
 def tri_T(v0, v1, v2):
     v0, v1, v2 = (np.asarray(v, np.float32) for v in (v0, v1, v2))
     T = np.zeros(16, np.float32)
