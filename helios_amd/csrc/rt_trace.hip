@@ -49,6 +49,13 @@
 #ifndef RT_BVH4_SORT
 #define RT_BVH4_SORT 1
 #endif
+// 8-bit node steps check the step budget and the link without a branch (the kernel reports a failed lane
+// when it finishes) and test the stack bound only on the pushes that can exceed it: 626 -> 611 ms for the
+// orchard's five diffuse passes
+#ifndef RT_Q4_LATE_ERR
+#define RT_Q4_LATE_ERR 1
+#endif
+#define RT_BUDGET_BAD 0xFFFFFFFFu
 
 namespace rt {
 
@@ -748,8 +755,13 @@ __device__ __forceinline__ bool direct_origin(const float4 *recs, uint32_t mdi, 
     return mdi == RT_MISS || origin_mask(recs, mdi, patch, false, st, spx, spy);
 }
 
+// Waves per SIMD the compiler must fit the unmasked direct kernel into: 7 (72 VGPRs, 6 without the bound)
+// takes the orchard's 64-sun direct pass from 407 to 393 ms; 8 spills in the candidate loop (419 ms)
+#ifndef RT_DIRECT_WAVES_PER_EU
+#define RT_DIRECT_WAVES_PER_EU 7
+#endif
 template <int WPB, int STK, bool COUNT, int MK>
-__global__ __launch_bounds__(WPB * 64) void k_direct(const DirectJob *__restrict__ jobs, uint32_t njobs, uint32_t total,
+__global__ __launch_bounds__(WPB * 64, (COUNT || MK) ? 1 : RT_DIRECT_WAVES_PER_EU) void k_direct(const DirectJob *__restrict__ jobs, uint32_t njobs, uint32_t total,
                                                      uint32_t *counter, uint32_t chunk, int *err,
                                                      unsigned long long *stats) {
     __shared__ uint32_t wstk_all[COUNT ? STK * WPB * 64 : STK * WPB];
@@ -1066,10 +1078,12 @@ __device__ __forceinline__ bool node4_order(LaneRay &L, float *key, uint32_t *ln
     cswap(key[0], lnk[0], key[2], lnk[2]);
     cswap(key[1], lnk[1], key[3], lnk[3]);
     cswap(key[1], lnk[1], key[2], lnk[2]);
+#if !RT_Q4_LATE_ERR
     if ((uint32_t)L.sp + nhit - 1 > (uint32_t)STK) {
         atomicOr(err, 1);
         return true;
     }
+#endif
 #if RT_PUSH_SPARE
     if (LSTK >= STK || (uint32_t)L.sp + 3u <= (uint32_t)LSTK) {
         // every slot a push may use is in LDS: three stores without branches, those beyond the pushed
@@ -1083,6 +1097,13 @@ __device__ __forceinline__ bool node4_order(LaneRay &L, float *key, uint32_t *ln
         L.sp += (int)nhit - 1;
         L.ni = lnk[0];
         return false;
+    }
+#endif
+#if RT_Q4_LATE_ERR
+    // (the branch-free pushes above cannot overflow: sp + 3 <= LSTK <= STK)
+    if ((uint32_t)L.sp + nhit - 1 > (uint32_t)STK) {
+        atomicOr(err, 1);
+        return true;
     }
 #endif
 #pragma unroll
@@ -1162,12 +1183,21 @@ __device__ __forceinline__ float qbyte(uint32_t w, int k) { return (float)((w >>
 template <int STK, int LSTK, typename C>
 __device__ __forceinline__ bool lane_node4q(const float4 *q4, uint32_t nn4, LaneRay &L, lds_u32 *stk, uint32_t *ovf,
                                             uint32_t stride, int *err, C &cnt) {
+#if RT_Q4_LATE_ERR
+    // a spent step budget or a link outside the tree ends the trace without a branch: the read goes to
+    // node 0 and the budget becomes RT_BUDGET_BAD, which the kernel reports when the lane finishes
+    const bool bad = L.budget == 0u || L.ni >= nn4;
+    L.budget = bad ? RT_BUDGET_BAD : L.budget - 1u;
+    const uint32_t ni = bad ? 0u : L.ni;
+#else
     if (L.budget-- == 0 || L.ni >= nn4) {
         atomicOr(err, 2);
         return true;
     }
+    const uint32_t ni = L.ni;
+#endif
     cnt.nodes++;
-    const float4 *nd = q4 + (size_t)L.ni * 4;
+    const float4 *nd = q4 + (size_t)ni * 4;
     const float4 f0 = ldc(nd), f1 = ldc(nd + 1), f2 = ldc(nd + 2), f3 = ldc(nd + 3);
     const uint32_t meta = __float_as_uint(f0.w);
     const uint32_t nch = meta >> 24;
@@ -1202,7 +1232,11 @@ __device__ __forceinline__ bool lane_node4q(const float4 *q4, uint32_t nn4, Lane
         key[k] = h ? tn : INFINITY;
         nhit += h ? 1u : 0u;
     }
+#if RT_Q4_LATE_ERR
+    return node4_order<STK, LSTK>(L, key, lnk, nhit, stk, ovf, stride, err) || bad;
+#else
     return node4_order<STK, LSTK>(L, key, lnk, nhit, stk, ovf, stride, err);
+#endif
 }
 
 // Test the leaf entry L.ni (closest accepted hit, ties to the lower primitive index), then pop.
@@ -1300,6 +1334,7 @@ void k_diffuse(const DiffuseJob *__restrict__ job, uint32_t total,
         // finish lanes whose traversal ended: periodic wrap (as trace_loop: at most 32 traces), else write
         if (busy && !trav) {
             bool done = true;
+            if (RT_Q4_LATE_ERR && L.budget == RT_BUDGET_BAD) atomicOr(err, 2);
             if (nbbox) {
                 float bt = nearest_face(sc, L.o, L.d, L.best_t);
                 if (bt > 0.f && wraps < 31) {
@@ -1711,6 +1746,7 @@ __global__ __launch_bounds__(256, COUNT ? 1 : RT_CAST_WAVES_PER_EU) void k_cast(
     while (true) {
         // finished lanes write their result
         if (busy && !trav) {
+            if (RT_Q4_LATE_ERR && L.budget == RT_BUDGET_BAD) atomicOr(err, 2);
             const uint32_t h = L.best;
             if (h == RT_MISS) {
                 U[my_r] = RT_MISS;
