@@ -56,6 +56,12 @@
 #define RT_Q4_LATE_ERR 1
 #endif
 #define RT_BUDGET_BAD 0xFFFFFFFFu
+// Diffuse rays take the 8-bit node step with the pop read issued beside the node load and no branch on the
+// hit-child count (611 -> 600 ms for the orchard's five passes); castRays keep the branching step, which is
+// faster for their long rays (3,112 against 2,700 Mrays/s)
+#ifndef RT_Q4_FLAT_DIFFUSE
+#define RT_Q4_FLAT_DIFFUSE 1
+#endif
 
 namespace rt {
 
@@ -1085,7 +1091,8 @@ __device__ __forceinline__ bool node4_order(LaneRay &L, float *key, uint32_t *ln
     }
 #endif
 #if RT_PUSH_SPARE
-    if (LSTK >= STK || (uint32_t)L.sp + 3u <= (uint32_t)LSTK) {
+    // (with the stack bound tested after this path, the path needs sp + 3 <= LSTK even when LSTK >= STK)
+    if ((!RT_Q4_LATE_ERR && LSTK >= STK) || (uint32_t)L.sp + 3u <= (uint32_t)LSTK) {
         // every slot a push may use is in LDS: three stores without branches, those beyond the pushed
         // children aimed at the lane's spare row (index LSTK, which nothing reads); child k < nhit goes to
         // slot sp + nhit - 1 - k, so the farthest lies deepest and child 1 on top
@@ -1180,7 +1187,7 @@ __device__ __forceinline__ bool lane_node4(const float4 *nodes4, uint32_t nn4, L
 // and the hits are those of the float trees.
 __device__ __forceinline__ float qbyte(uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xFFu); }
 
-template <int STK, int LSTK, typename C>
+template <int STK, int LSTK, bool FLAT, typename C>
 __device__ __forceinline__ bool lane_node4q(const float4 *q4, uint32_t nn4, LaneRay &L, lds_u32 *stk, uint32_t *ovf,
                                             uint32_t stride, int *err, C &cnt) {
 #if RT_Q4_LATE_ERR
@@ -1195,10 +1202,16 @@ __device__ __forceinline__ bool lane_node4q(const float4 *q4, uint32_t nn4, Lane
         return true;
     }
     const uint32_t ni = L.ni;
+    constexpr bool bad = false;
 #endif
     cnt.nodes++;
     const float4 *nd = q4 + (size_t)ni * 4;
     const float4 f0 = ldc(nd), f1 = ldc(nd + 1), f2 = ldc(nd + 2), f3 = ldc(nd + 3);
+    // FLAT: the entry a pop would take is read now, so its latency hides under the node's (a lane whose
+    // stack top is not in LDS reads its spare row and takes the general path below)
+    const uint32_t sp0 = (uint32_t)L.sp;
+    uint32_t popv = 0u;
+    if (FLAT) popv = stk[((sp0 != 0u && sp0 <= (uint32_t)LSTK) ? sp0 - 1u : (uint32_t)LSTK) * stride];
     const uint32_t meta = __float_as_uint(f0.w);
     const uint32_t nch = meta >> 24;
     // 2^e as a float from the biased exponent byte (e + 127 in [1, 254])
@@ -1231,6 +1244,24 @@ __device__ __forceinline__ bool lane_node4q(const float4 *q4, uint32_t nn4, Lane
         const bool h = (uint32_t)k < nch && fmaxf(tn, 0.f) <= fminf(tf, L.best_t);
         key[k] = h ? tn : INFINITY;
         nhit += h ? 1u : 0u;
+    }
+    if (FLAT && sp0 + 3u <= (uint32_t)LSTK) {
+        // no branch on the number of hit children: the sorted children are pushed into the rows above the
+        // stack top (those beyond the hit children into the spare row), and the next entry is the nearest
+        // hit child or, with none, the popped entry; the pushes write rows >= sp0, never the one popped
+        cswap(key[0], lnk[0], key[1], lnk[1]);
+        cswap(key[2], lnk[2], key[3], lnk[3]);
+        cswap(key[0], lnk[0], key[2], lnk[2]);
+        cswap(key[1], lnk[1], key[3], lnk[3]);
+        cswap(key[1], lnk[1], key[2], lnk[2]);
+#pragma unroll
+        for (int k = 3; k >= 1; k--) {
+            const uint32_t at = (uint32_t)k < nhit ? sp0 + nhit - 1u - (uint32_t)k : (uint32_t)LSTK;
+            stk[at * stride] = lnk[k];
+        }
+        L.ni = nhit != 0u ? lnk[0] : popv;
+        L.sp = (int)(nhit != 0u ? sp0 + nhit - 1u : (sp0 != 0u ? sp0 - 1u : 0u));
+        return (nhit == 0u && sp0 == 0u) || bad;
     }
 #if RT_Q4_LATE_ERR
     return node4_order<STK, LSTK>(L, key, lnk, nhit, stk, ovf, stride, err) || bad;
@@ -1434,7 +1465,7 @@ void k_diffuse(const DiffuseJob *__restrict__ job, uint32_t total,
                 const bool inner = trav && !(L.ni & RT_LEAF_BIT);
                 const uint64_t mi = __ballot(inner);
                 if (mi == 0 || (uint32_t)__popcll(__ballot(trav && !inner)) >= RT_LEAF_BATCH) break;
-                if (inner && (useq   ? lane_node4q<STK, LSTK>(q4, nn4, L, stk, ovf, stride, err, cnt)
+                if (inner && (useq   ? lane_node4q<STK, LSTK, RT_Q4_FLAT_DIFFUSE != 0>(q4, nn4, L, stk, ovf, stride, err, cnt)
                               : use4 ? lane_node4<STK, LSTK>(nodes4, nn4, L, stk, ovf, stride, err, cnt)
                                      : lane_node<STK, LSTK>(nodes, nleaf, L, stk, ovf, stride, err, cnt)))
                     trav = false;
@@ -1824,7 +1855,7 @@ __global__ __launch_bounds__(256, COUNT ? 1 : RT_CAST_WAVES_PER_EU) void k_cast(
                 const bool inner = trav && !(L.ni & RT_LEAF_BIT);
                 const uint64_t mi = __ballot(inner);
                 if (mi == 0 || (uint32_t)__popcll(__ballot(trav && !inner)) >= RT_LEAF_BATCH) break;
-                if (inner && (useq ? lane_node4q<STK, LSTK>(q4, nn4, L, stk, ovf, stride, err, cnt)
+                if (inner && (useq ? lane_node4q<STK, LSTK, false>(q4, nn4, L, stk, ovf, stride, err, cnt)
                                    : lane_node<STK, LSTK>(nodes, nleaf, L, stk, ovf, stride, err, cnt)))
                     trav = false;
             }
