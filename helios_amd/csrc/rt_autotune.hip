@@ -199,7 +199,8 @@ extern "C" int rt_autotune(rt_ctx *ctx, const rt_scene *scene, const rt_band_req
                 if (hipMemcpy(dj, &j, sizeof(j), hipMemcpyHostToDevice) != hipSuccess) { rc = -1; break; }
                 const LaunchCfg def = default_launch_cfg(0);
                 double ms = time_cfg(st, tm, def, [&](const LaunchCfg &c) {
-                    return launch_direct(dj, 1, P, depth, counter, err, nullptr, c, ncu, st, scene_flags(S, ctx->sx));
+                    return launch_direct(dj, 1, P, depth, counter, err, nullptr, c, ncu, st,
+                                         scene_flags(S, ctx->sx) | RT_FL_GRIDS);
                 });
                 if (ms < 0) { rc = -1; break; }
                 cost += ms + build_ms;
@@ -221,7 +222,8 @@ extern "C" int rt_autotune(rt_ctx *ctx, const rt_scene *scene, const rt_band_req
     if (rc == 0)
         rc = search(st, tm, trace_c,
                     [&](const LaunchCfg &c) {
-                        return launch_direct(dj, 1, sample, depth, counter, err, nullptr, c, ncu, st, scene_flags(S, ctx->sx));
+                        return launch_direct(dj, 1, sample, depth, counter, err, nullptr, c, ncu, st,
+                                             scene_flags(S, ctx->sx) | (use_grid ? RT_FL_GRIDS : 0));
                     },
                     default_launch_cfg(0), best[0], best_ms[0], def_ms[0], count);
     best[0].grid_density = density;

@@ -664,6 +664,14 @@ __device__ __forceinline__ uint32_t find_job(const J *jobs, uint32_t njobs, uint
     return lo;
 }
 
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+    for (int off = 32; off >= 1; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off, 64);
+    return v;
+}
+
+// Sum of k rays of strength s, each added once: k s is exact in double (k < 2^24), then rounded to float.
+__device__ __forceinline__ float lit_sum(uint32_t k, float s) { return (float)((double)k * (double)s); }
+
 __device__ __forceinline__ void wave_reduce(ksum &k) {
     for (int off = 32; off >= 1; off >>= 1) {
         float os = __shfl_xor(k.s, off, 64);
@@ -809,8 +817,9 @@ __global__ __launch_bounds__(WPB * 64, (COUNT || MK) ? 1 : RT_DIRECT_WAVES_PER_E
                     const float *Tp = ldu(&sc.T) + (size_t)i * 16;
 #pragma unroll
                     for (int q = 0; q < 12; q++) tm[q] = ldu(Tp + q);
-                    ksum acc;
-                    acc.init();
+                    // every ray of the primitive carries the same strength: the lit rays are counted and the
+                    // sum is count x strength, rounded once (exact up to that rounding)
+                    uint32_t lit = 0;
                     const uint32_t seed = ldu(&J.seed);
                     if (grid) {
                         // Occlusion grid: each lane scans its ray's cell list two candidates per step.  A lane
@@ -844,7 +853,7 @@ __global__ __launch_bounds__(WPB * 64, (COUNT || MK) ? 1 : RT_DIRECT_WAVES_PER_E
                                     const int res = pc >= pe ? 2
                                                              : grid_pair(ents, sorted, depth, recs, nleaf, pc, pe,
                                                                          o, dir, i, cnt);
-                                    if (res == 2) acc.add(strength);  // lit
+                                    if (res == 2) lit++;
                                     if (res != 0) act = false;
                                     else pc += 2;
                                 }
@@ -884,11 +893,10 @@ __global__ __launch_bounds__(WPB * 64, (COUNT || MK) ? 1 : RT_DIRECT_WAVES_PER_E
                             bool occl;
                             if (COUNT) occl = need && occluded<STK>(sc, o, dir, 1e-4f, i, wstk, WPB * 64, err, cnt);
                             else occl = packet_occluded<STK>(sc, o, dir, 1e-4f, i, need, wstk, wstk4, err, cnt);
-                            if (need && !occl) acc.add(strength);
+                            if (need && !occl) lit++;
                         }
                     }
-                    wave_reduce(acc);
-                    sum_out = acc.value();
+                    sum_out = lit_sum(wave_sum_u32(lit), strength);
                 }
             }
             if (lane == 0) {
@@ -899,6 +907,155 @@ __global__ __launch_bounds__(WPB * 64, (COUNT || MK) ? 1 : RT_DIRECT_WAVES_PER_E
         }
     }
     add_stats(stats, cnt, nray);
+}
+
+// Collimated direct rays when every job of the launch has an occlusion grid.  The lanes of a wave are refilled
+// across the primitives of a job, as the diffuse kernel refills them: a lane whose ray ends takes the next
+// ray of the open primitive, or of the next primitive of the wave's work queue, so no lane idles until the
+// longest ray of one primitive ends.  Each primitive with rays in flight holds a slot of the wave's ring in
+// LDS that counts its lit rays; the primitive's sums are written once its last ray has ended.  A job change
+// (another direction, grid and output) waits until the rays in flight have ended.  Every ray is tested
+// exactly as in k_direct, and the sums are the same count x strength.
+#define RT_DQ_SLOTS 8
+template <int WPB, int MK>
+__global__ __launch_bounds__(WPB * 64, MK ? 1 : RT_DIRECT_WAVES_PER_EU) void k_direct_q(const DirectJob *__restrict__ jobs,
+                                                                                        uint32_t njobs, uint32_t total,
+                                                                                        uint32_t *counter, uint32_t chunk) {
+    __shared__ uint32_t ring_all[3 * RT_DQ_SLOTS * WPB];  // per wave: lit count, primitive, strength of each slot
+    uint32_t *lit = ring_all + (threadIdx.x >> 6) * 3 * RT_DQ_SLOTS;
+    uint32_t *sprim = lit + RT_DQ_SLOTS;
+    float *sstr = (float *)(lit + 2 * RT_DQ_SLOTS);
+    const uint32_t lane = threadIdx.x & 63;
+    Cnt<false, MK> cnt{};
+    // work queue (wave-uniform)
+    uint32_t it = 0, it_end = 0;
+    bool drained = false;
+    // job (wave-uniform): loaded when the wave first takes one of its items
+    uint32_t jcur = RT_MISS, jbegin = 0, jend = 0, d = 0, Nrays = 0, seed = 0, nleaf = 0;
+    const DevScene *scp = nullptr;
+    const SunGrid *grid = nullptr;
+    const uint32_t *order = nullptr;
+    float *out = nullptr;
+    v3f dir = mk3(0.f, 0.f, 0.f);
+    // open primitive (wave-uniform) and the slots in use (bit mask; a slot's bit 8 + k: its sums are lit from
+    // the top)
+    uint32_t op_i = 0, op_next = 0, op_cnt = 0, op_slot = 0, op_mdi = RT_MISS, slots = 0;
+    bool op_patch = false;
+    // lane: its ray (origin, depth along the source direction, cell list position), primitive and slot
+    bool act = false;
+    v3f o = mk3(0.f, 0.f, 0.f);
+    float depth = 0.f;
+    uint32_t pc = 0, pe = 0, my_i = 0, my_slot = 0;
+    while (true) {
+        bool blocked = false;
+        while (true) {
+            const uint64_t idle = __ballot(!act);
+            if (idle == 0) break;
+            if (op_next == op_cnt) {
+                // the open primitive has no rays left: open the next item of the queue
+                if (it == it_end) {
+                    if (drained) break;
+                    it = wave_item(counter, chunk);
+                    if (it >= total) { drained = true; it = it_end = 0; break; }
+                    it_end = min(it + chunk, total);
+                }
+                if (jcur == RT_MISS || it < jbegin || it >= jend) {
+                    if (__ballot(act) != 0 || (slots & 0xFFu) != 0) { blocked = true; break; }
+                    jcur = find_job(jobs, njobs, it);
+                    const DirectJob &J = jobs[jcur];
+                    jbegin = ldu(&J.item_begin);
+                    jend = jcur + 1 < njobs ? ldu(&jobs[jcur + 1].item_begin) : total;
+                    scp = ldu(&J.sc);
+                    grid = ldu(&J.grid);
+                    order = ldu(&J.order);
+                    out = ldu(&J.out);
+                    d = ldu(&J.d);
+                    Nrays = d * d;
+                    seed = ldu(&J.seed);
+                    dir = nrm3(mk3(ldu(&J.sdir[0]), ldu(&J.sdir[1]), ldu(&J.sdir[2])));
+                    nleaf = ldu(&scp->nleaf);
+                }
+                if ((slots & 0xFFu) == 0xFFu) { blocked = true; break; }
+                uint32_t i = it - jbegin;
+                if (order) i = ldu(order + i);
+                i = __builtin_amdgcn_readfirstlane(i);
+                it++;
+                const uint32_t ty = ldu(ldu(&scp->type) + i);
+                const float4 n4 = ldu(ldu(&scp->nrm) + i);
+                const v3f n = mk3(n4.x, n4.y, n4.z);
+                const uint32_t tw = __float_as_uint(n4.w);
+                const bool top = dot3(dir, n) > 0.0f;
+                if (!traced_type(ty) || tw == 3 || (!top && tw == 0)) {
+                    if (lane == 0) { out[(size_t)i * 2] = 0.f; out[(size_t)i * 2 + 1] = 0.f; }
+                    continue;
+                }
+                const uint32_t sl = (uint32_t)__builtin_ctz(~slots & 0xFFu);
+                slots |= (1u << sl) | (top ? (0x100u << sl) : 0u);
+                lit[sl] = 0u;
+                sprim[sl] = i;
+                sstr[sl] = rdiv(fabsf(dot3(n, dir)), (float)Nrays);
+                const uint32_t *mdesc = ldu(&scp->mdesc);
+                op_mdi = (MK && mdesc) ? ldu(mdesc + i) : RT_MISS;
+                op_i = i;
+                op_next = 0;
+                op_cnt = Nrays;
+                op_slot = sl;
+                op_patch = ty == 0 || ty == 3;
+                continue;
+            }
+            const uint32_t rank =
+                __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+            const uint32_t take = min((uint32_t)__popcll(idle), op_cnt - op_next);
+            if (!act && rank < take) {
+                const uint32_t r = op_next + rank;
+                const float4 *recs = ldu(&scp->rec);
+                float spx, spy;
+                act = direct_origin(recs, op_mdi, op_i, r, d, seed, op_patch, spx, spy);
+                float tm[12];
+                const float *Tp = ldu(&scp->T) + (size_t)op_i * 16;
+#pragma unroll
+                for (int q = 0; q < 12; q++) tm[q] = ldu(Tp + q);
+                o = xform3(tm, spx, spy, 0.0f);
+                grid_cell(*grid, o, pc, pe);
+                depth = o.x * grid->w[0] + o.y * grid->w[1] + o.z * grid->w[2];
+                my_i = op_i;
+                my_slot = op_slot;
+            }
+            op_next += take;
+        }
+        // write the sums of the primitives none of whose rays remain (the open one once all its rays are out)
+        for (uint32_t pend = slots & 0xFFu; pend != 0u; pend &= pend - 1u) {
+            const uint32_t sl = (uint32_t)__builtin_ctz(pend);
+            if (sl == op_slot && op_next < op_cnt) continue;
+            if (__ballot(act && my_slot == sl) != 0) continue;
+            const uint32_t i = sprim[sl];
+            const float sum = lit_sum(lit[sl], sstr[sl]);
+            const bool top = (slots >> (8 + sl)) & 1u;
+            if (lane == 0) {
+                out[(size_t)i * 2] = top ? sum : 0.f;
+                out[(size_t)i * 2 + 1] = top ? 0.f : sum;
+            }
+            slots &= ~((1u << sl) | (0x100u << sl));
+        }
+        if (__ballot(act) == 0) {
+            if (drained && op_next == op_cnt && (slots & 0xFFu) == 0u) break;
+            continue;
+        }
+        // trace until fewer than RT_DIRECT_REFILL lanes scan (all of them when no ray can be handed out)
+        const uint32_t floor_lanes = (blocked || (drained && op_next == op_cnt)) ? 1u : RT_DIRECT_REFILL;
+        const uint2 *ents = grid->ents;
+        const bool sorted = grid->sorted != 0;
+        const float4 *recs = ldu(&scp->rec);
+        while (true) {
+            if (act) {
+                const int res = pc >= pe ? 2 : grid_pair(ents, sorted, depth, recs, nleaf, pc, pe, o, dir, my_i, cnt);
+                if (res == 2) atomicAdd(&lit[my_slot], 1u);
+                if (res != 0) act = false;
+                else pc += 2;
+            }
+            if ((uint32_t)__popcll(__ballot(act)) < floor_lanes) break;
+        }
+    }
 }
 
 // Occlusion of a ray towards a non-collimated source, literally as the reference loop
@@ -2569,14 +2726,27 @@ static void with_flags(int fl, F &&f) {
 // the counting variants always compile the mask tests in (a scene without masks skips them at run time)
 static inline int counting_flags(int fl) { return (fl & RT_FL_WT) | RT_FL_MASK; }
 
+#ifndef RT_DIRECT_QUEUE
+#define RT_DIRECT_QUEUE 1
+#endif
 template <int MK>
 static int launch_direct_t(const DirectJob *jobs, uint32_t njobs, uint32_t total, uint32_t depth, uint32_t *counter,
-                           int *err, unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st) {
+                           int *err, unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st,
+                           bool grids) {
     if (total == 0) return 0;
     const uint32_t ch = std::max(cfg.chunk, 1u);
     k_zero_u32<<<1, 1, 0, st>>>(counter);
     uint32_t wpb = cfg.waves_per_block;
     uint32_t g = grid_for(cfg, ncu, total, wpb);
+    if (RT_DIRECT_QUEUE && grids && !stats) {
+        switch (wpb) {
+            case 1: k_direct_q<1, MK><<<g, 64, 0, st>>>(jobs, njobs, total, counter, ch); break;
+            case 2: k_direct_q<2, MK><<<g, 128, 0, st>>>(jobs, njobs, total, counter, ch); break;
+            default: k_direct_q<4, MK><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch); break;
+        }
+        LCHECK();
+        return 0;
+    }
     if (stats) {
         // counting variant: per-lane stacks, 4 waves per block
         PICK_STK(depth, (k_direct<4, 32, true, MK><<<g, 256, 0, st>>>(jobs, njobs, total, counter, ch, err, stats)),
@@ -2600,7 +2770,8 @@ int launch_direct(const DirectJob *jobs, uint32_t njobs, uint32_t total, uint32_
                   unsigned long long *stats, const LaunchCfg &cfg, int ncu, hipStream_t st, int flags) {
     int rc = 0;
     with_flags(stats ? counting_flags(flags) : flags, [&](auto fl) {
-        rc = launch_direct_t<decltype(fl)::value>(jobs, njobs, total, depth, counter, err, stats, cfg, ncu, st);
+        rc = launch_direct_t<decltype(fl)::value>(jobs, njobs, total, depth, counter, err, stats, cfg, ncu, st,
+                                                  (flags & RT_FL_GRIDS) != 0);
     });
     return rc;
 }
@@ -2642,14 +2813,18 @@ int launch_direct_jobs(const DirectJob *hjobs, const DirectJob *jobs, uint32_t n
             set_error("internal: collimated direct jobs must precede the other source types");
             return -1;
         }
-    if (ncol == njobs) return launch_direct(jobs, njobs, total, depth, counter, err, stats, cfg, ncu, st, flags);
+    // the collimated jobs' launch runs the queue kernel when each of them has an occlusion grid
+    bool grids = ncol > 0;
+    for (uint32_t k = 0; k < ncol; k++) grids = grids && hjobs[k].grid != nullptr;
+    const int cflags = grids ? (flags | RT_FL_GRIDS) : (flags & ~RT_FL_GRIDS);
+    if (ncol == njobs) return launch_direct(jobs, njobs, total, depth, counter, err, stats, cfg, ncu, st, cflags);
     if (ncol == 0) return launch_direct_src(jobs, njobs, 0, total, depth, counter, err, stats, ncu, st, flags);
     const uint32_t split = hjobs[ncol].item_begin;
     if (split > total) {
         set_error("internal: direct job items out of order");
         return -1;
     }
-    if (launch_direct(jobs, ncol, split, depth, counter, err, stats, cfg, ncu, st, flags)) return -1;
+    if (launch_direct(jobs, ncol, split, depth, counter, err, stats, cfg, ncu, st, cflags)) return -1;
     return launch_direct_src(jobs + ncol, njobs - ncol, split, total - split, depth, counter, err, stats, ncu, st,
                              flags);
 }

@@ -14,6 +14,8 @@
 // traversal flags of a launch: honour texture masks, watertight intersection predicates
 #define RT_FL_MASK 1
 #define RT_FL_WT 2
+// host-side launch flag (not a kernel template argument): every job of a direct launch has an occlusion grid
+#define RT_FL_GRIDS 4
 #define RT_NOT_LAUNCHED 0xFFFFFFFEu
 #define RT_HIT_BOTTOM 0x80000000u
 
