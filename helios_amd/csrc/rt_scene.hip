@@ -276,7 +276,9 @@ __global__ void k_refit(int n, const uint32_t *__restrict__ child, const uint32_
 // nodes are numbered from the top (the last merge, the root, is node 0), and ranks come from the prefix
 // sum, so the tree is the same on every run.
 // ------------------------------------------------------------------------------------------------
+#ifndef PLOC_R
 #define PLOC_R 16
+#endif
 #define PLOC_BLOCK 256
 
 __device__ __forceinline__ float half_area(const float a[6]) {

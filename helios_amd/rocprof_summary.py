@@ -21,7 +21,7 @@ RAY_KERNELS = ("k_direct", "k_diffuse", "k_gather", "k_cast", "k_camera", "k_pix
 
 
 def short(name: str) -> str:
-    return re.sub(r"\(.*", "", name)[:90]
+    return re.sub(r"\(.*", "", name.replace("(anonymous namespace)::", ""))[:90]
 
 
 def kernel_rows(db):
