@@ -72,10 +72,21 @@ def counting_variant(name):
     return [a.strip() for a in m.group(2).split(",")][COUNT_ARG[m.group(1)]] == "true"
 
 
-def pmc_profile(kernel, profile="pmc_traffic_config3_diurnal.json"):
+def kernel_flags(name):
+    """The traversal-flags template argument (the last one) of a ray kernel's name, as an int, else None."""
+    m = re.search(r"(k_\w+)<([^>]*)>", name)
+    if not m:
+        return None
+    last = m.group(2).split(",")[-1].strip()
+    return int(last) if last.isdigit() else None
+
+
+def pmc_profile(kernel, profile="pmc_traffic_config3_diurnal.json", flags=0):
     """HBM bytes per dispatch of `kernel` from the committed rocprofv3 PMC summary: FETCH_SIZE + WRITE_SIZE of
-    its largest timed (non-counting) dispatch, each scaled by what the counter read for a copy of known size,
-    and that kernel's L2 hit rate when the profile has one.  None without a profile of the kernel."""
+    its largest timed (non-counting) dispatch with the traversal flags of the timed launch (0: no masks,
+    reference predicates; other variants in the same profile, such as the watertight solves of the predicate
+    comparison, are different kernels), each scaled by what the counter read for a copy of known size, and
+    that kernel's L2 hit rate when the profile has one.  None without a profile of the kernel."""
     path = os.path.join(ROOT, "profiles", profile)
     if not os.path.exists(path):
         return None
@@ -83,7 +94,8 @@ def pmc_profile(kernel, profile="pmc_traffic_config3_diurnal.json"):
         d = json.load(f)
     best = None
     for name, cs in d.get("kernels", {}).items():
-        if kernel in name and not counting_variant(name) and "FETCH_SIZE" in cs and "WRITE_SIZE" in cs:
+        if (re.search(re.escape(kernel) + r"<", name) and not counting_variant(name)
+                and (flags is None or kernel_flags(name) == flags) and "FETCH_SIZE" in cs and "WRITE_SIZE" in cs):
             b = cs["FETCH_SIZE"]["max_bytes"] + cs["WRITE_SIZE"]["max_bytes"]
             if best is None or b > best["bytes"]:
                 best = {"bytes": b, "fetch": cs["FETCH_SIZE"]["max_bytes"], "write": cs["WRITE_SIZE"]["max_bytes"],
@@ -91,18 +103,18 @@ def pmc_profile(kernel, profile="pmc_traffic_config3_diurnal.json"):
     return best
 
 
-def pmc_traffic(kernel, profile="pmc_traffic_config3_diurnal.json"):
-    p = pmc_profile(kernel, profile)
+def pmc_traffic(kernel, profile="pmc_traffic_config3_diurnal.json", flags=0):
+    p = pmc_profile(kernel, profile, flags)
     return None if p is None else p["bytes"]
 
 
-def roofline(kernel, avg_launch_ms, canonical_bytes, profile, **extra):
+def roofline(kernel, avg_launch_ms, canonical_bytes, profile, flags=0, **extra):
     """Roofline of the dominant kernel.  achieved = the kernel's HBM bytes per launch (rocprofv3 FETCH_SIZE +
     WRITE_SIZE, calibrated, from the committed profile) / its HIP-event launch time, so frac is a fraction of the
     8 TB/s HBM peak.  The canonical per-ray byte model (64 B per BVH node visit, 48 B per primitive test, 32 B of
     I/O, counted by the kernels) is reported under "canonical": most of those bytes come from L2 and the
     Infinity Cache, so its rate is a measure of work, not a bandwidth, and may exceed the peak."""
-    prof = pmc_profile(kernel, profile)
+    prof = pmc_profile(kernel, profile, flags)
     sec = avg_launch_ms / 1e3
     ach = None if prof is None else prof["bytes"] / sec / 1e9
     canon_gbs = canonical_bytes / sec / 1e9
@@ -356,7 +368,10 @@ def bench_diurnal(args):
     # the canonical bytes of the whole step against its wall time: above the HBM peak, they cannot all be HBM bytes
     canon_step = sum(bytes_per_ray(counts[c], grid=(c == "direct" and grid)) * counts[c]["rays"]
                      for c in ("direct", "diffuse"))
-    rl = roofline(f"k_{dom}", avg_launch_ms, brays * rays_per_launch, "pmc_traffic_config3_diurnal.json",
+    # the timed series' kernels: no texture masks, reference predicates (flags 0); direct rays with occlusion
+    # grids run k_direct_q
+    dom_kernel = "k_direct_q" if (dom == "direct" and grid) else f"k_{dom}"
+    rl = roofline(dom_kernel, avg_launch_ms, brays * rays_per_launch, "pmc_traffic_config3_diurnal.json",
                   bytes_per_ray=round(brays, 1), rays_per_launch=int(rays_per_launch),
                   nodes_per_ray=round(counts[dom]["nodes"] / counts[dom]["rays"], 2),
                   prims_per_ray=round(counts[dom]["prims"] / counts[dom]["rays"], 2),
@@ -520,7 +535,8 @@ def bench_homogeneous(args):
                        "published_s": {k: {"direct": v[0], "diffuse": v[1]} for k, v in PUBLISHED_HOMOGENEOUS.items()},
                        "absorbed_mean_direct": round(float(Rd.mean()), 5), "absorbed_mean_diffuse": round(float(Rf.mean()), 5)},
             "roofline": roofline("k_diffuse", kms / args.steps, brays * counts["diffuse"]["rays"],
-                                 "pmc_traffic_homogeneous_canopy.json", bytes_per_ray=round(brays, 1))}
+                                 "pmc_traffic_homogeneous_canopy.json", flags=1,  # texture-masked leaves
+                                 bytes_per_ray=round(brays, 1))}
 
 
 def bench_castrays(args):
