@@ -421,6 +421,8 @@ def bench_diurnal(args):
 
 
 def bench_config2(args):
+    if args.calib_copy_mib:
+        calib_copy(args.calib_copy_mib)
     s = scenes.config2(2)
     ctx = native.Context(0)
     g = ctx.scene(s.transforms, s.types, s.twosided, periodic=s.periodic)
