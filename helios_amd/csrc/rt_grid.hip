@@ -132,26 +132,28 @@ __global__ void k_grid_count(uint32_t nleaf, const float4 *__restrict__ rec, flo
 __global__ void k_grid_rank(const SunGrid *__restrict__ grids, const uint32_t *__restrict__ cell_base,
                             const unsigned long long *__restrict__ item_base,
                             const unsigned long long *__restrict__ item_count, const uint32_t *__restrict__ start,
-                            const uint32_t *__restrict__ t_items, const float *__restrict__ t_keys,
-                            const uint32_t *__restrict__ t_cell, uint2 *__restrict__ ents) {
+                            const uint2 *__restrict__ tmp, const uint32_t *__restrict__ tcell,
+                            uint2 *__restrict__ ents) {
     const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= item_count[blockIdx.y]) return;
     const unsigned long long ib = item_base[blockIdx.y];
     const uint32_t sb = (uint32_t)ib;
     const size_t e0 = ib + x;
-    const uint32_t cl = t_cell[e0];
+    const uint2 me = tmp[e0];  // (entry, key)
+    const uint32_t cl = tcell[e0];
     // a slot the fill left empty (none when the fill matches the count, as it does) has no cell: it is not
     // placed, and its NaN key never counts as preceding anything; the final list keeps its trailing slots
     // as "no primitive" with a NaN key, which ends a scan
     if (cl >= grids[blockIdx.y].nx * grids[blockIdx.y].ny) return;
     const size_t cc = cell_base[blockIdx.y] + cl;
     const uint32_t b = start[cc] - sb, e = start[cc + 1] - sb;
-    const float kx = t_keys[e0];
-    const uint32_t ix = t_items[e0];
+    const float kx = __uint_as_float(me.y);
+    const uint32_t ix = me.x;
     uint32_t rank = 0;
     for (uint32_t y = b; y < e; y++) {
-        const float ky = t_keys[ib + y];
-        const uint32_t iy = t_items[ib + y];
+        const uint2 oy = tmp[ib + y];
+        const float ky = __uint_as_float(oy.y);
+        const uint32_t iy = oy.x;
         rank += (ky > kx || (ky == kx && ((iy & RT_GRID_SLOT) < (ix & RT_GRID_SLOT) ||
                                           (iy == ix && y < (uint32_t)x)))) ? 1u : 0u;
     }
@@ -165,8 +167,7 @@ __global__ void k_grid_fill(uint32_t nleaf, const float4 *__restrict__ rec, floa
                             const SunGrid *__restrict__ grids, const uint32_t *__restrict__ cell_base,
                             const unsigned long long *__restrict__ item_base,
                             const unsigned long long *__restrict__ item_count, const uint32_t *__restrict__ start,
-                            uint32_t *__restrict__ counts, uint32_t *__restrict__ items, float *__restrict__ keys,
-                            uint32_t *__restrict__ cells) {
+                            uint32_t *__restrict__ counts, uint2 *__restrict__ tmp, uint32_t *__restrict__ tcell) {
     const SunGrid G = grids[blockIdx.y];
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nleaf || leaf_is_voxel(rec, k)) return;
@@ -188,9 +189,10 @@ __global__ void k_grid_fill(uint32_t nleaf, const float4 *__restrict__ rec, floa
             if (left == 0u) continue;
             const unsigned long long at = (unsigned long long)(uint32_t)(start[c] - sbase) + left - 1u;
             if (at < ni) {
-                items[ib + at] = item;
-                keys[ib + at] = key;
-                cells[ib + at] = (uint32_t)(c - base);
+                // the entry and its key in one 8-byte store (the list slots are scattered: each store is a
+                // partial-line write), the cell in a second
+                tmp[ib + at] = make_uint2(item, __float_as_uint(key));
+                tcell[ib + at] = (uint32_t)(c - base);
             }
         }
 }
@@ -306,26 +308,24 @@ int build_sun_grids(GridSet &gs, const Scene &S, const float *dirs, uint32_t n, 
     GK(hipcub::DeviceScan::ExclusiveSum(tmp, scan_bytes, counts, starts, (int)(total_cells + 1), st));
     if (gs.items.reserve((size_t)total * 8 + 256)) return -1;
     uint2 *ents = (uint2 *)gs.items.base;
-    // the fill writes unsorted lists (entry, key, cell) to scratch; k_grid_rank moves each entry to its
-    // place.  Slots nothing is written to read as "no primitive" with a NaN key.
+    // the fill writes unsorted lists of (entry, key, cell) to scratch; k_grid_rank moves each entry to its
+    // place.  Slots nothing is written to read as "no primitive" with a NaN key and no cell.
     const size_t tb = ((size_t)total * 4 + 255) & ~(size_t)255;
     if (gs.sortbuf.reserve(3 * tb + 256)) return -1;
-    uint32_t *t_items = (uint32_t *)gs.sortbuf.base;
-    float *t_keys = (float *)((char *)gs.sortbuf.base + tb);
-    uint32_t *t_cell = (uint32_t *)((char *)gs.sortbuf.base + 2 * tb);
+    uint2 *unsorted = (uint2 *)gs.sortbuf.base;
+    uint32_t *ucell = (uint32_t *)((char *)gs.sortbuf.base + 2 * tb);
     GK(hipMemsetAsync(gs.sortbuf.base, 0xFF, 3 * tb, st));
     GK(hipMemsetAsync(ents, 0xFF, (size_t)total * 8, st));
     GK(hipMemcpyAsync(dibase, ibase.data(), (size_t)n * 8, hipMemcpyHostToDevice, st));
     k_grid_fill<<<grid, 256, 0, st>>>(nleaf, S.h.rec, S.h.smax, dgrids, dbase, dibase, dentries, starts, counts,
-                                      t_items, t_keys, t_cell);
+                                      unsorted, ucell);
     GK(hipGetLastError());
     {
         unsigned long long maxent = 0;
         for (uint32_t j = 0; j < n; j++) maxent = std::max(maxent, ent[j]);
         if (maxent) {
             k_grid_rank<<<dim3((uint32_t)((maxent + 255) / 256), n), 256, 0, st>>>(dgrids, dbase, dibase, dentries,
-                                                                                   starts, t_items, t_keys, t_cell,
-                                                                                   ents);
+                                                                                   starts, unsorted, ucell, ents);
             GK(hipGetLastError());
         }
     }
