@@ -110,9 +110,13 @@ __global__ void k_grid_bounds(uint32_t nleaf, const float4 *__restrict__ rec, fl
     }
 }
 
-__global__ void k_grid_count(uint32_t nleaf, const float4 *__restrict__ rec, float smax,
-                             const SunGrid *__restrict__ grids, const uint32_t *__restrict__ cell_base,
-                             uint32_t *__restrict__ counts, unsigned long long *__restrict__ entries /*[n]*/) {
+// Cell counts from a 2D difference array: each primitive adds +1 at (x0, y0), -1 at (x1 + 1, y0) and
+// (x0, y1 + 1), +1 at (x1 + 1, y1 + 1) (corners outside the grid dropped), four atomics instead of one per
+// covered cell; a prefix sum along the rows and then along the columns turns the differences into the
+// number of rectangles covering each cell (uint32 arithmetic, exact modulo 2^32 and so exact).
+__global__ void k_grid_count_diff(uint32_t nleaf, const float4 *__restrict__ rec, float smax,
+                                  const SunGrid *__restrict__ grids, const uint32_t *__restrict__ cell_base,
+                                  uint32_t *__restrict__ diff, unsigned long long *__restrict__ entries /*[n]*/) {
     const SunGrid G = grids[blockIdx.y];
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nleaf || leaf_is_voxel(rec, k)) return;
@@ -121,9 +125,50 @@ __global__ void k_grid_count(uint32_t nleaf, const float4 *__restrict__ rec, flo
     const int x0 = cell_of(u0, G.g0u, G.inv_cs, G.nx), x1 = cell_of(u1, G.g0u, G.inv_cs, G.nx);
     const int y0 = cell_of(v0, G.g0v, G.inv_cs, G.ny), y1 = cell_of(v1, G.g0v, G.inv_cs, G.ny);
     atomicAdd(&entries[blockIdx.y], (unsigned long long)(x1 - x0 + 1) * (unsigned long long)(y1 - y0 + 1));
-    uint32_t *c = counts + cell_base[blockIdx.y];
-    for (int y = y0; y <= y1; y++)
-        for (int x = x0; x <= x1; x++) atomicAdd(&c[(size_t)y * G.nx + x], 1u);
+    uint32_t *d = diff + cell_base[blockIdx.y];
+    const bool xr = (uint32_t)(x1 + 1) < G.nx, yr = (uint32_t)(y1 + 1) < G.ny;
+    atomicAdd(&d[(size_t)y0 * G.nx + x0], 1u);
+    if (xr) atomicAdd(&d[(size_t)y0 * G.nx + x1 + 1], 0xFFFFFFFFu);
+    if (yr) atomicAdd(&d[(size_t)(y1 + 1) * G.nx + x0], 0xFFFFFFFFu);
+    if (xr && yr) atomicAdd(&d[(size_t)(y1 + 1) * G.nx + x1 + 1], 1u);
+}
+
+// inclusive prefix sums along each row (one wave per row: 64 cells per step with a running carry)
+__global__ void k_grid_scan_rows(const SunGrid *__restrict__ grids, const uint32_t *__restrict__ cell_base,
+                                 uint32_t *__restrict__ diff) {
+    const SunGrid &G = grids[blockIdx.y];
+    const uint32_t nx = G.nx, ny = G.ny;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (row >= ny) return;
+    uint32_t *d = diff + cell_base[blockIdx.y] + (size_t)row * nx;
+    uint32_t carry = 0;
+    for (uint32_t x0 = 0; x0 < nx; x0 += 64) {
+        const uint32_t x = x0 + lane;
+        uint32_t v = x < nx ? d[x] : 0u;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t t = (uint32_t)__shfl_up((int)v, off, 64);
+            if (lane >= (uint32_t)off) v += t;
+        }
+        if (x < nx) d[x] = v + carry;
+        carry += (uint32_t)__shfl((int)v, 63, 64);
+    }
+}
+
+// inclusive prefix sums down each column (one thread per column; neighbouring threads read neighbouring
+// cells of a row)
+__global__ void k_grid_scan_cols(const SunGrid *__restrict__ grids, const uint32_t *__restrict__ cell_base,
+                                 uint32_t *__restrict__ diff) {
+    const SunGrid &G = grids[blockIdx.y];
+    const uint32_t nx = G.nx, ny = G.ny;
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nx) return;
+    uint32_t *d = diff + cell_base[blockIdx.y] + x;
+    uint32_t run = 0;
+    for (uint32_t y = 0; y < ny; y++) {
+        run += d[(size_t)y * nx];
+        d[(size_t)y * nx] = run;
+    }
 }
 
 // Each cell's list in descending key order (ties: ascending leaf slot), so the list order is deterministic
@@ -293,8 +338,16 @@ int build_sun_grids(GridSet &gs, const Scene &S, const float *dirs, uint32_t n, 
     GK(hipMemcpyAsync(dgrids, hg.data(), n * sizeof(SunGrid), hipMemcpyHostToDevice, st));
     GK(hipMemsetAsync(counts, 0, (total_cells + 1) * 4, st));
     GK(hipMemsetAsync(dentries, 0, (size_t)n * 8, st));
-    k_grid_count<<<grid, 256, 0, st>>>(nleaf, S.h.rec, S.h.smax, dgrids, dbase, counts, dentries);
-    GK(hipGetLastError());
+    {
+        uint32_t maxnx = 1, maxny = 1;
+        for (uint32_t j = 0; j < n; j++) { maxnx = std::max(maxnx, hg[j].nx); maxny = std::max(maxny, hg[j].ny); }
+        k_grid_count_diff<<<grid, 256, 0, st>>>(nleaf, S.h.rec, S.h.smax, dgrids, dbase, counts, dentries);
+        GK(hipGetLastError());
+        k_grid_scan_rows<<<dim3((maxny + 3) / 4, n), 256, 0, st>>>(dgrids, dbase, counts);
+        GK(hipGetLastError());
+        k_grid_scan_cols<<<dim3((maxnx + 255) / 256, n), 256, 0, st>>>(dgrids, dbase, counts);
+        GK(hipGetLastError());
+    }
     std::vector<unsigned long long> ent(n), ibase(n);
     GK(hipMemcpyAsync(ent.data(), dentries, (size_t)n * 8, hipMemcpyDeviceToHost, st));
     GK(hipStreamSynchronize(st));
