@@ -208,29 +208,75 @@ __global__ void k_grid_rank(const SunGrid *__restrict__ grids, const uint32_t *_
 // start[] is one exclusive scan over the cells of all directions in uint32 arithmetic, so it holds the
 // true offsets modulo 2^32; a direction's own list starts at the 64-bit offset item_base[j] and its
 // offsets count from sbase = item_base[j] mod 2^32 (each direction has fewer than 2^32 entries).
-__global__ void k_grid_fill(uint32_t nleaf, const float4 *__restrict__ rec, float smax,
-                            const SunGrid *__restrict__ grids, const uint32_t *__restrict__ cell_base,
-                            const unsigned long long *__restrict__ item_base,
-                            const unsigned long long *__restrict__ item_count, const uint32_t *__restrict__ start,
-                            uint32_t *__restrict__ counts, uint2 *__restrict__ tmp, uint32_t *__restrict__ tcell) {
+// The fill takes one list slot per entry from its cell's count (atomicSub, counting down from the cell's
+// total).  A workgroup's 256 leaves are consecutive in leaf order, so their projected rectangles overlap:
+// when the union of the workgroup's rectangles spans at most RT_FILL_LDS cells, the workgroup first counts
+// its entries per cell in LDS, takes each cell's block of slots with one global atomic, and hands the slots
+// out from LDS; otherwise every entry takes its slot with a global atomic.
+#ifndef RT_FILL_LDS
+#define RT_FILL_LDS 8192
+#endif
+__global__ __launch_bounds__(256) void k_grid_fill(uint32_t nleaf, const float4 *__restrict__ rec, float smax,
+                                                   const SunGrid *__restrict__ grids,
+                                                   const uint32_t *__restrict__ cell_base,
+                                                   const unsigned long long *__restrict__ item_base,
+                                                   const unsigned long long *__restrict__ item_count,
+                                                   const uint32_t *__restrict__ start, uint32_t *__restrict__ counts,
+                                                   uint2 *__restrict__ tmp, uint32_t *__restrict__ tcell) {
+    __shared__ uint32_t lc[RT_FILL_LDS];
+    __shared__ int bb[4];  // union of the workgroup's cell ranges: x0, -x1, y0, -y1 (minima)
     const SunGrid G = grids[blockIdx.y];
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nleaf || leaf_is_voxel(rec, k)) return;
-    float u0, u1, v0, v1, key;
-    proj_rect(rec, k, G, smax, u0, u1, v0, v1, key);
-    const int x0 = cell_of(u0, G.g0u, G.inv_cs, G.nx), x1 = cell_of(u1, G.g0u, G.inv_cs, G.nx);
-    const int y0 = cell_of(v0, G.g0v, G.inv_cs, G.ny), y1 = cell_of(v1, G.g0v, G.inv_cs, G.ny);
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = k < nleaf && !leaf_is_voxel(rec, k);
+    int x0 = 0, x1 = -1, y0 = 0, y1 = -1;
+    float key = 0.f;
+    if (valid) {
+        float u0, u1, v0, v1;
+        proj_rect(rec, k, G, smax, u0, u1, v0, v1, key);
+        x0 = cell_of(u0, G.g0u, G.inv_cs, G.nx); x1 = cell_of(u1, G.g0u, G.inv_cs, G.nx);
+        y0 = cell_of(v0, G.g0v, G.inv_cs, G.ny); y1 = cell_of(v1, G.g0v, G.inv_cs, G.ny);
+    }
+    if (threadIdx.x < 4) bb[threadIdx.x] = 0x7FFFFFFF;
+    __syncthreads();
+    if (valid) {
+        atomicMin(&bb[0], x0); atomicMin(&bb[1], -x1);
+        atomicMin(&bb[2], y0); atomicMin(&bb[3], -y1);
+    }
+    __syncthreads();
+    const int bx0 = bb[0], bx1 = -bb[1], by0 = bb[2], by1 = -bb[3];
     const size_t base = cell_base[blockIdx.y];
     const unsigned long long ib = item_base[blockIdx.y], ni = item_count[blockIdx.y];
     const uint32_t sbase = (uint32_t)ib;
-    const uint32_t tag = __float_as_uint(rec[(size_t)k * 4].w);
-    const uint32_t item = k | ((tag & (RT_PATCH_BIT | RT_MASK_BIT)) ? RT_GRID_REC3 : 0u);
+    uint32_t item = 0;
+    if (valid) {
+        const uint32_t tag = __float_as_uint(rec[(size_t)k * 4].w);
+        item = k | ((tag & (RT_PATCH_BIT | RT_MASK_BIT)) ? RT_GRID_REC3 : 0u);
+    }
+    const bool any = bx0 <= bx1;  // the workgroup has a traced leaf
+    const uint32_t bw = any ? (uint32_t)(bx1 - bx0 + 1) : 0u, bh = any ? (uint32_t)(by1 - by0 + 1) : 0u;
+    const bool dense = any && (uint64_t)bw * bh <= (uint64_t)RT_FILL_LDS;
+    // the ranges equal the counting kernel's (same code, same FP32 operations); the guards below only keep
+    // a mismatch from writing outside the direction's list
+    if (dense) {
+        const uint32_t area = bw * bh;
+        for (uint32_t q = threadIdx.x; q < area; q += blockDim.x) lc[q] = 0u;
+        __syncthreads();
+        for (int y = y0; y <= y1; y++)
+            for (int x = x0; x <= x1; x++) atomicAdd(&lc[(uint32_t)(y - by0) * bw + (uint32_t)(x - bx0)], 1u);
+        __syncthreads();
+        for (uint32_t q = threadIdx.x; q < area; q += blockDim.x) {
+            const uint32_t cnt = lc[q];
+            if (cnt == 0u) continue;
+            const size_t c = base + (size_t)(by0 + (int)(q / bw)) * G.nx + (uint32_t)bx0 + q % bw;
+            lc[q] = atomicSub(&counts[c], cnt);  // the slots left + 1 - cnt .. left are this workgroup's
+        }
+        __syncthreads();
+    }
     for (int y = y0; y <= y1; y++)
         for (int x = x0; x <= x1; x++) {
             const size_t c = base + (size_t)y * G.nx + x;
-            // the ranges equal the counting kernel's (same code, same FP32 operations); the guards only
-            // keep a mismatch from writing outside the direction's list
-            const uint32_t left = atomicSub(&counts[c], 1u);
+            const uint32_t left = dense ? atomicSub(&lc[(uint32_t)(y - by0) * bw + (uint32_t)(x - bx0)], 1u)
+                                        : atomicSub(&counts[c], 1u);
             if (left == 0u) continue;
             const unsigned long long at = (unsigned long long)(uint32_t)(start[c] - sbase) + left - 1u;
             if (at < ni) {
