@@ -174,17 +174,31 @@ __global__ void k_grid_scan_cols(const SunGrid *__restrict__ grids, const uint32
 // Each cell's list in descending key order (ties: ascending leaf slot), so the list order is deterministic
 // and a scan can stop at the first key below the ray origin's depth.  One thread per entry counts the entries of its cell that precede it and
 // moves the entry to that rank: the work is the sum of squared list lengths, spread over all entries.
-__global__ void k_grid_rank(const SunGrid *__restrict__ grids, const uint32_t *__restrict__ cell_base,
-                            const unsigned long long *__restrict__ item_base,
-                            const unsigned long long *__restrict__ item_count, const uint32_t *__restrict__ start,
-                            const uint2 *__restrict__ tmp, const uint32_t *__restrict__ tcell,
-                            uint2 *__restrict__ ents) {
-    const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= item_count[blockIdx.y]) return;
+#define RT_RANK_MARGIN 256
+__global__ __launch_bounds__(256) void k_grid_rank(const SunGrid *__restrict__ grids,
+                                                   const uint32_t *__restrict__ cell_base,
+                                                   const unsigned long long *__restrict__ item_base,
+                                                   const unsigned long long *__restrict__ item_count,
+                                                   const uint32_t *__restrict__ start,
+                                                   const uint2 *__restrict__ tmp, const uint32_t *__restrict__ tcell,
+                                                   uint2 *__restrict__ ents) {
+    // The unsorted lists lie cell after cell in the scratch array, so the lists of a workgroup's entries
+    // mostly fall in a window of its 256 entries and RT_RANK_MARGIN on either side: that window is read
+    // into LDS once, and an entry whose cell's list fits it scans the list there instead of in memory.
+    __shared__ uint2 win[256 + 2 * RT_RANK_MARGIN];
+    const unsigned long long ni = item_count[blockIdx.y];
     const unsigned long long ib = item_base[blockIdx.y];
+    const long long w0 = (long long)blockIdx.x * 256 - RT_RANK_MARGIN;
+    for (uint32_t t = threadIdx.x; t < 256 + 2 * RT_RANK_MARGIN; t += 256) {
+        const long long y = w0 + (long long)t;
+        win[t] = (y >= 0 && (unsigned long long)y < ni) ? tmp[ib + (unsigned long long)y] : make_uint2(RT_MISS, 0u);
+    }
+    __syncthreads();
+    const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= ni) return;
     const uint32_t sb = (uint32_t)ib;
     const size_t e0 = ib + x;
-    const uint2 me = tmp[e0];  // (entry, key)
+    const uint2 me = win[(long long)x - w0];  // (entry, key)
     const uint32_t cl = tcell[e0];
     // a slot the fill left empty (none when the fill matches the count, as it does) has no cell: it is not
     // placed, and its NaN key never counts as preceding anything; the final list keeps its trailing slots
@@ -195,8 +209,9 @@ __global__ void k_grid_rank(const SunGrid *__restrict__ grids, const uint32_t *_
     const float kx = __uint_as_float(me.y);
     const uint32_t ix = me.x;
     uint32_t rank = 0;
+    const bool inwin = (long long)b >= w0 && (long long)e <= w0 + 256 + 2 * RT_RANK_MARGIN;
     for (uint32_t y = b; y < e; y++) {
-        const uint2 oy = tmp[ib + y];
+        const uint2 oy = inwin ? win[(long long)y - w0] : tmp[ib + y];
         const float ky = __uint_as_float(oy.y);
         const uint32_t iy = oy.x;
         rank += (ky > kx || (ky == kx && ((iy & RT_GRID_SLOT) < (ix & RT_GRID_SLOT) ||
