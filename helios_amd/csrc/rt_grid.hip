@@ -110,27 +110,75 @@ __global__ void k_grid_bounds(uint32_t nleaf, const float4 *__restrict__ rec, fl
     }
 }
 
+#ifndef RT_FILL_LDS
+#define RT_FILL_LDS 8192  // cells of a workgroup's union rectangle that the count and fill kernels keep in LDS
+#endif
+
 // Cell counts from a 2D difference array: each primitive adds +1 at (x0, y0), -1 at (x1 + 1, y0) and
 // (x0, y1 + 1), +1 at (x1 + 1, y1 + 1) (corners outside the grid dropped), four atomics instead of one per
 // covered cell; a prefix sum along the rows and then along the columns turns the differences into the
 // number of rectangles covering each cell (uint32 arithmetic, exact modulo 2^32 and so exact).
-__global__ void k_grid_count_diff(uint32_t nleaf, const float4 *__restrict__ rec, float smax,
-                                  const SunGrid *__restrict__ grids, const uint32_t *__restrict__ cell_base,
-                                  uint32_t *__restrict__ diff, unsigned long long *__restrict__ entries /*[n]*/) {
+__global__ __launch_bounds__(256) void k_grid_count_diff(uint32_t nleaf, const float4 *__restrict__ rec, float smax,
+                                                         const SunGrid *__restrict__ grids,
+                                                         const uint32_t *__restrict__ cell_base,
+                                                         uint32_t *__restrict__ diff,
+                                                         unsigned long long *__restrict__ entries /*[n]*/) {
+    // as in k_grid_fill, a workgroup whose rectangles' union spans at most RT_FILL_LDS cells adds its
+    // corners in LDS first and then each nonzero cell once to the global array
+    __shared__ uint32_t lc[RT_FILL_LDS];
+    __shared__ int bb[4];
+    __shared__ unsigned long long nent;
     const SunGrid G = grids[blockIdx.y];
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nleaf || leaf_is_voxel(rec, k)) return;
-    float u0, u1, v0, v1, key;
-    proj_rect(rec, k, G, smax, u0, u1, v0, v1, key);
-    const int x0 = cell_of(u0, G.g0u, G.inv_cs, G.nx), x1 = cell_of(u1, G.g0u, G.inv_cs, G.nx);
-    const int y0 = cell_of(v0, G.g0v, G.inv_cs, G.ny), y1 = cell_of(v1, G.g0v, G.inv_cs, G.ny);
-    atomicAdd(&entries[blockIdx.y], (unsigned long long)(x1 - x0 + 1) * (unsigned long long)(y1 - y0 + 1));
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = k < nleaf && !leaf_is_voxel(rec, k);
+    int x0 = 0, x1 = -1, y0 = 0, y1 = -1;
+    if (valid) {
+        float u0, u1, v0, v1, key;
+        proj_rect(rec, k, G, smax, u0, u1, v0, v1, key);
+        x0 = cell_of(u0, G.g0u, G.inv_cs, G.nx); x1 = cell_of(u1, G.g0u, G.inv_cs, G.nx);
+        y0 = cell_of(v0, G.g0v, G.inv_cs, G.ny); y1 = cell_of(v1, G.g0v, G.inv_cs, G.ny);
+    }
+    if (threadIdx.x < 4) bb[threadIdx.x] = 0x7FFFFFFF;
+    if (threadIdx.x == 0) nent = 0ull;
+    __syncthreads();
+    if (valid) {
+        atomicMin(&bb[0], x0); atomicMin(&bb[1], -x1);
+        atomicMin(&bb[2], y0); atomicMin(&bb[3], -y1);
+        atomicAdd(&nent, (unsigned long long)(x1 - x0 + 1) * (unsigned long long)(y1 - y0 + 1));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && nent) atomicAdd(&entries[blockIdx.y], nent);
+    // bounds of the corner cells: the rectangles' cells plus one row and column past them
+    const int bx0 = bb[0], bx1 = -bb[1] + 1, by0 = bb[2], by1 = -bb[3] + 1;
+    const bool any = bx0 <= bx1 - 1;
+    const uint32_t bw = any ? (uint32_t)(bx1 - bx0 + 1) : 0u, bh = any ? (uint32_t)(by1 - by0 + 1) : 0u;
+    const bool dense = any && (uint64_t)bw * bh <= (uint64_t)RT_FILL_LDS;
     uint32_t *d = diff + cell_base[blockIdx.y];
     const bool xr = (uint32_t)(x1 + 1) < G.nx, yr = (uint32_t)(y1 + 1) < G.ny;
-    atomicAdd(&d[(size_t)y0 * G.nx + x0], 1u);
-    if (xr) atomicAdd(&d[(size_t)y0 * G.nx + x1 + 1], 0xFFFFFFFFu);
-    if (yr) atomicAdd(&d[(size_t)(y1 + 1) * G.nx + x0], 0xFFFFFFFFu);
-    if (xr && yr) atomicAdd(&d[(size_t)(y1 + 1) * G.nx + x1 + 1], 1u);
+    if (dense) {
+        const uint32_t area = bw * bh;
+        for (uint32_t q = threadIdx.x; q < area; q += blockDim.x) lc[q] = 0u;
+        __syncthreads();
+        if (valid) {
+            const uint32_t r0 = (uint32_t)(y0 - by0) * bw, r1 = (uint32_t)(y1 + 1 - by0) * bw;
+            const uint32_t c0 = (uint32_t)(x0 - bx0), c1 = (uint32_t)(x1 + 1 - bx0);
+            atomicAdd(&lc[r0 + c0], 1u);
+            if (xr) atomicAdd(&lc[r0 + c1], 0xFFFFFFFFu);
+            if (yr) atomicAdd(&lc[r1 + c0], 0xFFFFFFFFu);
+            if (xr && yr) atomicAdd(&lc[r1 + c1], 1u);
+        }
+        __syncthreads();
+        for (uint32_t q = threadIdx.x; q < area; q += blockDim.x) {
+            const uint32_t v = lc[q];
+            if (v == 0u) continue;
+            atomicAdd(&d[(size_t)(by0 + (int)(q / bw)) * G.nx + (uint32_t)bx0 + q % bw], v);
+        }
+    } else if (valid) {
+        atomicAdd(&d[(size_t)y0 * G.nx + x0], 1u);
+        if (xr) atomicAdd(&d[(size_t)y0 * G.nx + x1 + 1], 0xFFFFFFFFu);
+        if (yr) atomicAdd(&d[(size_t)(y1 + 1) * G.nx + x0], 0xFFFFFFFFu);
+        if (xr && yr) atomicAdd(&d[(size_t)(y1 + 1) * G.nx + x1 + 1], 1u);
+    }
 }
 
 // inclusive prefix sums along each row (one wave per row: 64 cells per step with a running carry)
@@ -228,9 +276,6 @@ __global__ __launch_bounds__(256) void k_grid_rank(const SunGrid *__restrict__ g
 // when the union of the workgroup's rectangles spans at most RT_FILL_LDS cells, the workgroup first counts
 // its entries per cell in LDS, takes each cell's block of slots with one global atomic, and hands the slots
 // out from LDS; otherwise every entry takes its slot with a global atomic.
-#ifndef RT_FILL_LDS
-#define RT_FILL_LDS 8192
-#endif
 __global__ __launch_bounds__(256) void k_grid_fill(uint32_t nleaf, const float4 *__restrict__ rec, float smax,
                                                    const SunGrid *__restrict__ grids,
                                                    const uint32_t *__restrict__ cell_base,
