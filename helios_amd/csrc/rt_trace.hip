@@ -1464,6 +1464,15 @@ __device__ __forceinline__ bool lane_leaf(const float4 *recs, LaneRay &L, float 
 #define RT_LEAF_DEFER 1
 #endif
 #define RT_DEFER_END 0xFFFFFFFFu
+// With parked leaves, the leaf phase tests one leaf per waiting lane and returns to the node phase (lanes
+// that reach further leaves park them there), and starts once 8 lanes wait: 589 -> 554 ms for the
+// orchard's five passes (batches of 2-24 lanes swept; the full leaf phase with 16 lanes was the 589)
+#ifndef RT_LEAF_ONE_ROUND
+#define RT_LEAF_ONE_ROUND 1
+#endif
+#ifndef RT_LEAF_BATCH_PARKED
+#define RT_LEAF_BATCH_PARKED 8
+#endif
 #ifndef RT_DIFFUSE_LDS_STACK
 #define RT_DIFFUSE_LDS_STACK 16
 #endif
@@ -1627,8 +1636,9 @@ void k_diffuse(const DiffuseJob *__restrict__ job, uint32_t total,
         if (RT_LEAF_DEFER && !COUNT) {
         // A lane that reaches a leaf parks it (pend) and goes on with its next entry, so it keeps taking node
         // steps; it waits only when it reaches a second leaf or runs out of entries (L.ni = RT_DEFER_END).  The
-        // leaf phase tests the parked leaves and the leaves lanes wait at, many lanes at a time.  (The counting
-        // variant keeps the plain order, whose node visits are the canonical work measure.)
+        // leaf phase tests the parked leaves and the leaves lanes wait at, many lanes at a time, one leaf per
+        // lane per phase.  (The counting variant keeps the plain order, whose node visits are the canonical
+        // work measure.)
         while (true) {
             while (true) {
                 if (trav && pend == 0u && (L.ni & RT_LEAF_BIT) && L.ni != RT_DEFER_END) {
@@ -1637,7 +1647,7 @@ void k_diffuse(const DiffuseJob *__restrict__ job, uint32_t total,
                 }
                 const bool inner = trav && !(L.ni & RT_LEAF_BIT);
                 const uint64_t mi = __ballot(inner);
-                if (mi == 0 || (uint32_t)__popcll(__ballot(trav && !inner)) >= RT_LEAF_BATCH) break;
+                if (mi == 0 || (uint32_t)__popcll(__ballot(trav && !inner)) >= RT_LEAF_BATCH_PARKED) break;
                 if (inner && (useq   ? lane_node4q<STK, LSTK, RT_Q4_FLAT_DIFFUSE != 0>(q4, nn4, L, stk, ovf, stride, err, cnt)
                               : use4 ? lane_node4<STK, LSTK>(nodes4, nn4, L, stk, ovf, stride, err, cnt)
                                      : lane_node<STK, LSTK>(nodes, nleaf, L, stk, ovf, stride, err, cnt))) {
@@ -1645,7 +1655,7 @@ void k_diffuse(const DiffuseJob *__restrict__ job, uint32_t total,
                     else trav = false;
                 }
             }
-            while (true) {
+            for (int round = 0; !RT_LEAF_ONE_ROUND || round < 1; round++) {
                 const bool atleaf = trav && (L.ni & RT_LEAF_BIT) && L.ni != RT_DEFER_END;
                 const bool work = trav && (pend != 0u || atleaf);
                 if (__ballot(work) == 0) break;
