@@ -1478,16 +1478,22 @@ __device__ __forceinline__ bool lane_leaf(const float4 *recs, LaneRay &L, float 
 #endif
 // Waves per SIMD the compiler must fit the timed diffuse kernel into (the default launch shape, 16 two-wave
 // workgroups per CU, runs 8 per SIMD): on the orchard 8 waves with a few spills beat 5 waves without (622
-// against 726 ms per diurnal series).  The masked variant, whose leaf test is larger, is held to 5 waves;
-// the counting variants keep their registers.
+// against 726 ms per diurnal series).  With texture masks, 8 as well (the homogeneous-canopy benchmark's
+// diffuse runBand 0.67 s at 5 waves, 0.56 s at 8); with watertight predicates, 6 (one plain orchard solve
+// 804 ms at 8, 732 ms at 6: the larger leaf test spills more); the counting variants keep their registers.
 #ifndef RT_DIFFUSE_WAVES_PER_EU
 #define RT_DIFFUSE_WAVES_PER_EU 8
 #endif
+#ifndef RT_DIFFUSE_WAVES_PER_EU_WT
+#define RT_DIFFUSE_WAVES_PER_EU_WT 6
+#endif
 #ifndef RT_DIFFUSE_WAVES_PER_EU_MASKED
-#define RT_DIFFUSE_WAVES_PER_EU_MASKED 5
+#define RT_DIFFUSE_WAVES_PER_EU_MASKED 8
 #endif
 template <int WPB, int STK, bool COUNT, int MK>
-__global__ __launch_bounds__(WPB * 64, COUNT ? 1 : ((MK & RT_FL_MASK) ? RT_DIFFUSE_WAVES_PER_EU_MASKED : RT_DIFFUSE_WAVES_PER_EU))
+__global__ __launch_bounds__(WPB * 64, COUNT ? 1 : ((MK & RT_FL_MASK) ? RT_DIFFUSE_WAVES_PER_EU_MASKED
+                                                                    : (MK & RT_FL_WT) ? RT_DIFFUSE_WAVES_PER_EU_WT
+                                                                                      : RT_DIFFUSE_WAVES_PER_EU))
 void k_diffuse(const DiffuseJob *__restrict__ job, uint32_t total,
                                                       uint32_t *counter, uint32_t chunk, int *err,
                                                       unsigned long long *stats) {
