@@ -73,12 +73,13 @@ def counting_variant(name):
 
 
 def kernel_flags(name):
-    """The traversal-flags template argument (the last one) of a ray kernel's name, as an int, else None."""
+    """The traversal-flags template argument of a ray kernel's name (its last integer argument; boolean
+    arguments such as the camera's specular switch may follow it), as an int, else None."""
     m = re.search(r"(k_\w+)<([^>]*)>", name)
     if not m:
         return None
-    last = m.group(2).split(",")[-1].strip()
-    return int(last) if last.isdigit() else None
+    nums = [a.strip() for a in m.group(2).split(",") if a.strip().isdigit()]
+    return int(nums[-1]) if nums else None
 
 
 def pmc_profile(kernel, profile="pmc_traffic_config3_diurnal.json", flags=0):

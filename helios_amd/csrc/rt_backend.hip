@@ -34,6 +34,22 @@ __global__ void k_select_bands(uint32_t P, uint32_t Bg, uint32_t Bl, const uint3
 
 inline uint32_t nblk(size_t n) { return (uint32_t)((n + 255) / 256); }
 
+// radiation_specular[s][i][b] += flux[s][b] * (top + bottom strength sum of source s on primitive i): the
+// radiation each source delivers to each primitive in a direct launch, which the camera's specular term
+// reads (R/shaders/direct_raygen.comp:717-726 adds flux * strength for every unoccluded ray).  Bands
+// that take no energy in the launch add nothing.
+__global__ void k_deposit_specular(uint32_t P, uint32_t B, uint32_t S, const float *__restrict__ dsum,
+                                   const float *__restrict__ flux, const uint8_t *__restrict__ active,
+                                   float *__restrict__ spec) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (size_t)S * P * B) return;
+    const uint32_t b = (uint32_t)(q % B);
+    const size_t si = q / B;  // s * P + i
+    const uint32_t s = (uint32_t)(si / P);
+    if (!active[b]) return;
+    spec[q] += flux[(size_t)s * B + b] * (dsum[si * 2] + dsum[si * 2 + 1]);
+}
+
 struct Carve {
     char *p;
     size_t off = 0;
@@ -134,9 +150,13 @@ void backend_release(BackendState &be) {
     if (be.rad) (void)hipFree(be.rad);
     if (be.cam) (void)hipFree(be.cam);
     if (be.lab) (void)hipFree(be.lab);
-    be.mat = be.flux = be.rad = be.cam = be.lab = nullptr;
-    be.mat_cap = be.flux_cap = be.rad_cap = be.cam_cap = be.lab_cap = 0;
+    if (be.spm) (void)hipFree(be.spm);
+    if (be.fcam) (void)hipFree(be.fcam);
+    if (be.spec) (void)hipFree(be.spec);
+    be.mat = be.flux = be.rad = be.cam = be.lab = be.spm = be.fcam = be.spec = nullptr;
+    be.mat_cap = be.flux_cap = be.rad_cap = be.cam_cap = be.lab_cap = be.spm_cap = be.fcam_cap = be.spec_cap = 0;
     be.cam_pix = be.lab_pix = 0;
+    be.has_specular = be.has_fcam = be.spec_live = false;
     be.ws.release();
 }
 
@@ -214,6 +234,9 @@ int rt_build_acceleration_structure(rt_ctx *ctx) {
         // per-primitive state of the previous geometry no longer applies
         be.has_materials = false;
         be.has_fluxes = false;
+        be.has_specular = false;
+        be.has_fcam = false;
+        be.spec_live = false;
         be.Bl = 0;
     }
     be.P = be.gP;
@@ -253,7 +276,7 @@ int rt_update_sources(rt_ctx *ctx, uint32_t nsources, const uint32_t *types, con
     if (source_shapes(nsources, types, widths, rotations, shapes, "rt_update_sources")) return -1;
     std::lock_guard<std::mutex> lk(ctx->mu);
     BackendState &be = ctx->be;
-    if (nsources != be.S) be.has_fluxes = false;
+    if (nsources != be.S) { be.has_fluxes = false; be.has_fcam = false; }
     be.S = nsources;
     be.src_dir.assign(positions, positions + (size_t)nsources * 3);
     be.shapes = std::move(shapes);
@@ -271,8 +294,9 @@ int rt_zero_radiation_buffers(rt_ctx *ctx, uint32_t launch_band_count) {
     }
     CKR(hipSetDevice(ctx->device));
     CKR(hipStreamSynchronize(ctx->sx.stream));
-    if (launch_band_count != be.Bl) be.has_fluxes = false;
+    if (launch_band_count != be.Bl) { be.has_fluxes = false; be.has_fcam = false; }
     be.Bl = launch_band_count;
+    be.spec_live = false;  // radiation_specular reads as zero until a direct launch with specular on
     size_t PB = (size_t)be.P * be.Bl;
     if (grow(be.rad, be.rad_cap, 5 * PB * sizeof(float) + 256)) return -1;
     // radiation_in and radiation_out; the scatter buffers are zeroed as well so that a new runBand
@@ -342,6 +366,69 @@ int rt_upload_source_fluxes(rt_ctx *ctx, const float *fluxes, size_t count) {
     if (grow(be.flux, be.flux_cap, count * sizeof(float) + 256)) return -1;
     if (count) CKR(hipMemcpy(be.flux, fluxes, count * sizeof(float), hipMemcpyHostToDevice));
     be.has_fluxes = true;
+    return 0;
+}
+
+int rt_update_specular_materials(rt_ctx *ctx, uint32_t nprims, const float *exponent, const float *scale) {
+    NN(ctx, "ctx");
+    NN(exponent, "exponent");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    BackendState &be = ctx->be;
+    if (need_scene(be, "rt_update_specular_materials")) return -1;
+    if (nprims != be.P) {
+        set_error("rt_update_specular_materials: " + std::to_string(nprims) + " primitives, the scene has " +
+                  std::to_string(be.P));
+        return -1;
+    }
+    CKR(hipSetDevice(ctx->device));
+    CKR(hipStreamSynchronize(ctx->sx.stream));
+    if (grow(be.spm, be.spm_cap, 2 * (size_t)be.P * sizeof(float) + 256)) return -1;
+    float *dexp = (float *)be.spm, *dscale = dexp + be.P;
+    CKR(hipMemcpy(dexp, exponent, (size_t)be.P * sizeof(float), hipMemcpyHostToDevice));
+    if (scale) CKR(hipMemcpy(dscale, scale, (size_t)be.P * sizeof(float), hipMemcpyHostToDevice));
+    else CKR(hipMemset(dscale, 0, (size_t)be.P * sizeof(float)));
+    be.has_specular = true;
+    return 0;
+}
+
+int rt_upload_source_fluxes_cam(rt_ctx *ctx, const float *fluxes_cam, size_t count) {
+    NN(ctx, "ctx");
+    NN(fluxes_cam, "fluxes_cam");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    BackendState &be = ctx->be;
+    if (need_launch_buffers(be, "rt_upload_source_fluxes_cam")) return -1;
+    if (count != (size_t)be.S * be.Bl) {
+        set_error("rt_upload_source_fluxes_cam: " + std::to_string(count) + " values, expected S*B_launch = " +
+                  std::to_string((size_t)be.S * be.Bl));
+        return -1;
+    }
+    CKR(hipSetDevice(ctx->device));
+    CKR(hipStreamSynchronize(ctx->sx.stream));
+    if (grow(be.fcam, be.fcam_cap, count * sizeof(float) + 256)) return -1;
+    if (count) CKR(hipMemcpy(be.fcam, fluxes_cam, count * sizeof(float), hipMemcpyHostToDevice));
+    be.has_fcam = true;
+    return 0;
+}
+
+int rt_get_radiation_specular(rt_ctx *ctx, float *out, size_t count) {
+    NN(ctx, "ctx");
+    NN(out, "out");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    BackendState &be = ctx->be;
+    if (need_launch_buffers(be, "rt_get_radiation_specular")) return -1;
+    const size_t n = (size_t)be.S * be.P * be.Bl;
+    if (count != n) {
+        set_error("rt_get_radiation_specular: " + std::to_string(count) + " values, expected S*P*B_launch = " +
+                  std::to_string(n));
+        return -1;
+    }
+    CKR(hipSetDevice(ctx->device));
+    CKR(hipStreamSynchronize(ctx->sx.stream));
+    if (be.spec_live && be.spec_S == be.S) {
+        if (n) CKR(hipMemcpy(out, be.spec, n * sizeof(float), hipMemcpyDeviceToHost));
+    } else {
+        std::fill(out, out + n, 0.0f);
+    }
     return 0;
 }
 
@@ -425,6 +512,18 @@ int rt_launch_direct_rays(rt_ctx *ctx, const rt_launch_params *p) {
     if (launch_deposit_direct(P, Bl, Bl, S, dsum, (const float *)be.flux, rl, tl, dact, be.rbuf(0), be.rbuf(1),
                               be.rbuf(2), st))
         return -1;
+    if (p->specular_reflection_enabled) {
+        const size_t n = (size_t)S * P * Bl;
+        if (!be.spec_live || be.spec_S != S) {
+            CKR(hipStreamSynchronize(st));
+            if (grow(be.spec, be.spec_cap, n * sizeof(float))) return -1;
+            CKR(hipMemsetAsync(be.spec, 0, n * sizeof(float), st));
+            be.spec_live = true;
+            be.spec_S = S;
+        }
+        k_deposit_specular<<<nblk(n), 256, 0, st>>>(P, Bl, S, dsum, (const float *)be.flux, dact, (float *)be.spec);
+        CKR(hipGetLastError());
+    }
     return check_err(ctx, err);
 }
 
@@ -611,8 +710,8 @@ namespace {
 int camera_setup(rt_ctx *ctx, const rt_camera_params *p, CameraLaunch &c, const char *who) {
     BackendState &be = ctx->be;
     if (need_scene(be, who)) return -1;
-    if (p->specular_reflection_enabled) {
-        set_error(std::string(who) + ": specular reflection is not supported by this backend");
+    if (p->specular_reflection_enabled > 2) {
+        set_error(std::string(who) + ": specular_reflection_enabled must be 0, 1 (scale 1) or 2 (specular_scale)");
         return -1;
     }
     const uint64_t rx = p->resolution[0], ry = p->resolution[1], fx = p->resolution_full[0], fy = p->resolution_full[1];
@@ -672,6 +771,24 @@ int rt_launch_camera_rays(rt_ctx *ctx, const rt_camera_params *p) {
         return -1;
     }
     if (S && !be.has_fluxes) { set_error(std::string(who) + ": no source fluxes (call rt_upload_source_fluxes)"); return -1; }
+    const bool specular = p->specular_reflection_enabled != 0 && S != 0;
+    if (specular) {
+        if (!be.has_specular) {
+            set_error(std::string(who) + ": specular reflection needs the primitives' specular exponents (call "
+                                         "rt_update_specular_materials)");
+            return -1;
+        }
+        if (!be.has_fcam) {
+            set_error(std::string(who) + ": specular reflection needs the camera's source weights (call "
+                                         "rt_upload_source_fluxes_cam)");
+            return -1;
+        }
+        if (be.spec_live && be.spec_S != S) {
+            set_error(std::string(who) + ": radiation_specular holds " + std::to_string(be.spec_S) +
+                      " sources, the backend has " + std::to_string(S) + " (call rt_zero_radiation_buffers)");
+            return -1;
+        }
+    }
     CKR(hipSetDevice(ctx->device));
     hipStream_t st = ctx->sx.stream;
     const uint64_t npix = (uint64_t)c.full_x * c.full_y;
@@ -733,6 +850,15 @@ int rt_launch_camera_rays(rt_ctx *ctx, const rt_camera_params *p) {
     for (int k = 0; k < 3; k++) c.sun[k] = sun ? sun[k] : (k == 2 ? 1.f : 0.f);
     c.solar_cos = be.has_sky ? be.solar_cos : 2.0f;
     c.pixels = (float *)be.cam;
+    // specular term: off without sums to read (radiation_specular all zero adds nothing)
+    if (specular && be.spec_live) {
+        c.spec = (const float *)be.spec;
+        c.fcam = (const float *)be.fcam;
+        c.spec_exp = (const float *)be.spm;
+        c.spec_scale = (const float *)be.spm + be.P;
+        c.spec_P = be.P;
+        c.spec_mode = p->specular_reflection_enabled;
+    }
     // source positions travel with the shapes table
     float *d_pos = cv.take<float>(3 * (size_t)(S ? S : 1));
     if (cv.off > be.ws.cap) { set_error(std::string(who) + ": workspace carve overflow"); return -1; }

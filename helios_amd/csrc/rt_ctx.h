@@ -55,6 +55,18 @@ struct BackendState {
     void *lab = nullptr;         // pixel labels [pixels] then depths [pixels]
     size_t lab_cap = 0;
     uint64_t lab_pix = 0;        // pixels the label/depth buffers hold
+    // specular reflection: per-primitive exponent and scale, per-camera source weights, and the incident
+    // radiation of each source that direct launches with specular on accumulate
+    void *spm = nullptr;         // specular_exponent [P] then specular_scale [P]
+    size_t spm_cap = 0;
+    bool has_specular = false;
+    void *fcam = nullptr;        // source_fluxes_cam [S][Bl]
+    size_t fcam_cap = 0;
+    bool has_fcam = false;
+    void *spec = nullptr;        // radiation_specular [S][P][Bl]
+    size_t spec_cap = 0;
+    bool spec_live = false;      // spec holds this launch set's sums (false: all zero)
+    uint32_t spec_S = 0;         // sources of those sums
     float *rho() const { return (float *)mat; }
     float *tau() const { return (float *)mat + (size_t)P * Bg; }
     float *rbuf(int k) const { return (float *)rad + (size_t)k * P * Bl; }  // 0 in 1 st 2 sb 3 ot 4 ob

@@ -2222,10 +2222,38 @@ __device__ __forceinline__ float cam_source_radiance(const CameraLaunch &c, v3f 
     return rad;
 }
 
+// Blinn-Phong specular term of a camera hit on primitive h for band b (camera_raygen.comp:362-409): for
+// each source, the radiation it delivered to h in the direct launch (radiation_specular) times the
+// camera's weight for that source and band and a Fresnel reflectance of 0.25; if positive, times the
+// primitive's scale (mode 2, else 1), (n . h)^e (e + 2) / (2 pi samples), with h the half vector between
+// the light direction (a collimated or sun-sphere source's direction, else towards the source's centre
+// from the hit point) and the reversed view direction, and n the primitive's stored normal.  0 when the
+// primitive's exponent is not positive.  FP32, the reference's operation order.
+__device__ __forceinline__ float cam_specular(const CameraLaunch &c, v3f o, v3f d, float t, uint32_t h, uint32_t b,
+                                              v3f n) {
+    float acc = 0.0f;
+    const float e = c.spec_exp[h];
+    if (!(e > 0.0f)) return 0.0f;
+    for (uint32_t s = 0; s < c.S; s++) {
+        float spec = c.spec[((size_t)s * c.spec_P + h) * c.B + b];
+        spec = spec * (c.fcam[(size_t)s * c.B + b] * 0.25f);
+        if (!(spec > 0.0f)) continue;
+        const uint32_t ty = c.src_type[s];
+        v3f L = mk3(c.src_pos[3 * s], c.src_pos[3 * s + 1], c.src_pos[3 * s + 2]);
+        if (!(ty == 0u || ty == 2u)) L = sub3(L, add3(o, mul3(d, t)));
+        L = nrm3(L);
+        const v3f hv = nrm3(sub3(L, d));
+        const float scale = c.spec_mode == 2u ? c.spec_scale[h] : 1.0f;
+        const float ch = fmaxf(0.0f, hv.x * n.x + hv.y * n.y + hv.z * n.z);
+        acc += rdiv(spec * scale * powf(ch, e) * (e + 2.0f), (float)c.spp * 2.0f * RT_PI_F);
+    }
+    return acc;
+}
+
 // Camera pixels: one thread per pixel of the tile traces its antialiasing samples in order and adds
 // their contributions to camera_radiation[b][pixel] (a fixed order, so a render is reproducible; the
-// reference adds the samples with float atomics).  No specular term (this backend rejects it).
-template <int STK, bool COUNT, int MK>
+// reference adds the samples with float atomics).
+template <int STK, bool COUNT, int MK, bool SPEC>
 __global__ __launch_bounds__(256) void k_camera(CameraLaunch c, int *err, unsigned long long *stats) {
     constexpr int LSTK = STK < RT_CAMERA_LDS_STACK ? STK : RT_CAMERA_LDS_STACK;
     __shared__ uint32_t stk_all[LSTK * 256];
@@ -2257,6 +2285,7 @@ __global__ __launch_bounds__(256) void k_camera(CameraLaunch c, int *err, unsign
                 const float srad = cam_source_radiance(c, o, d, b, t, false);
                 float surf = (top ? c.out_top : c.out_bot)[(size_t)h * c.B + b] * strength;
                 if (srad > 0.0f) surf += srad * strength;
+                if (SPEC) surf += cam_specular(c, o, d, t, h, b, mk3(n4.x, n4.y, n4.z));
                 c.pixels[(size_t)b * npix + pix] += rdiv(surf, RT_PI_F);
             }
         } else {
@@ -2279,7 +2308,7 @@ __global__ __launch_bounds__(256) void k_camera(CameraLaunch c, int *err, unsign
 #ifndef RT_CAMERA_REFILL
 #define RT_CAMERA_REFILL 0
 #endif
-template <int STK, bool COUNT, int MK>
+template <int STK, bool COUNT, int MK, bool SPEC>
 __global__ __launch_bounds__(256) void k_camera_r(CameraLaunch c, int *err, unsigned long long *stats) {
     constexpr int LSTK = STK < RT_CAMERA_LDS_STACK ? STK : RT_CAMERA_LDS_STACK;
     __shared__ uint32_t stk_all[LSTK * 256];
@@ -2341,6 +2370,7 @@ __global__ __launch_bounds__(256) void k_camera_r(CameraLaunch c, int *err, unsi
                         const float srad = cam_source_radiance(c, o, d, b, L.best_t, false);
                         float surf = (top ? c.out_top : c.out_bot)[(size_t)h * c.B + b] * strength;
                         if (srad > 0.0f) surf += srad * strength;
+                        if (SPEC) surf += cam_specular(c, o, d, L.best_t, h, b, mk3(n4.x, n4.y, n4.z));
                         c.pixels[(size_t)b * npix + pix] += rdiv(surf, RT_PI_F);
                     }
                 } else {
@@ -2978,14 +3008,24 @@ int launch_camera(const CameraLaunch &c, uint32_t depth, bool labels, int *err, 
     else if (stats)
         with_flags(counting_flags(flags), [&](auto fl) {
             constexpr int FL = decltype(fl)::value;
-            PICK_STK(depth, (K_CAM<32, true, FL><<<g, 256, 0, st>>>(c, err, stats)),
-                     (K_CAM<64, true, FL><<<g, 256, 0, st>>>(c, err, stats)));
+            if (c.spec)
+                PICK_STK(depth, (K_CAM<32, true, FL, true><<<g, 256, 0, st>>>(c, err, stats)),
+                         (K_CAM<64, true, FL, true><<<g, 256, 0, st>>>(c, err, stats)));
+            else
+                PICK_STK(depth, (K_CAM<32, true, FL, false><<<g, 256, 0, st>>>(c, err, stats)),
+                         (K_CAM<64, true, FL, false><<<g, 256, 0, st>>>(c, err, stats)));
         });
     else
+        // the specular term is a separate instantiation: inlined unconditionally it costs the plain
+        // kernel 3 VGPRs and one wave per SIMD (78 -> 81 registers)
         with_flags(flags, [&](auto fl) {
             constexpr int FL = decltype(fl)::value;
-            PICK_STK(depth, (K_CAM<32, false, FL><<<g, 256, 0, st>>>(c, err, nullptr)),
-                     (K_CAM<64, false, FL><<<g, 256, 0, st>>>(c, err, nullptr)));
+            if (c.spec)
+                PICK_STK(depth, (K_CAM<32, false, FL, true><<<g, 256, 0, st>>>(c, err, nullptr)),
+                         (K_CAM<64, false, FL, true><<<g, 256, 0, st>>>(c, err, nullptr)));
+            else
+                PICK_STK(depth, (K_CAM<32, false, FL, false><<<g, 256, 0, st>>>(c, err, nullptr)),
+                         (K_CAM<64, false, FL, false><<<g, 256, 0, st>>>(c, err, nullptr)));
         });
 #undef K_CAM
     LCHECK();

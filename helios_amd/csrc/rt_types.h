@@ -119,6 +119,12 @@ struct CameraLaunch {
     float *pixels;                     // [B][full_x * full_y]
     uint32_t *labels;                  // [full_x * full_y]
     float *depths;                     // [full_x * full_y]
+    // specular reflection (spec NULL: off)
+    const float *spec;                 // [S][spec_P][B] radiation_specular of the direct launch
+    const float *fcam;                 // [S][B] camera weights of the sources
+    const float *spec_exp;             // [P] specular exponents
+    const float *spec_scale;           // [P] specular scales (mode 2)
+    uint32_t spec_P, spec_mode;        // primitives; 1 = scale 1, 2 = spec_scale
 };
 
 // Translucent covers of one solve (device copy): primitives glass in every band pass rays through, and the

@@ -114,6 +114,14 @@ namespace helios {
         check(rt_update_materials(ctx_, static_cast<uint32_t>(materials.num_sources), static_cast<uint32_t>(materials.num_primitives), static_cast<uint32_t>(materials.num_bands), materials.reflectivity.data(),
                                   materials.transmissivity.data()),
               "updateMaterials");
+        // specular exponent and scale per primitive (VulkanComputeBackend.cpp:631-655)
+        if (!materials.specular_exponent.empty()) {
+            const size_t P = materials.num_primitives;
+            if (materials.specular_exponent.size() != P || (!materials.specular_scale.empty() && materials.specular_scale.size() != P)) {
+                helios_runtime_error("ERROR (HipRayTracingBackend::updateMaterials): specular_exponent / specular_scale need " + std::to_string(P) + " entries (Nprimitives).");
+            }
+            check(rt_update_specular_materials(ctx_, static_cast<uint32_t>(P), materials.specular_exponent.data(), materials.specular_scale.empty() ? nullptr : materials.specular_scale.data()), "updateMaterials");
+        }
     }
 
     void HipRayTracingBackend::updateSources(const std::vector<RayTracingSource> &sources) {
@@ -199,6 +207,7 @@ namespace helios {
         p.band_map = band_map_.data();
         p.band_active = band_active_.data();
         p.launch_face = params.launch_face;
+        p.specular_reflection_enabled = params.specular_reflection_enabled;
         p.diffuse_flux = params.diffuse_flux.empty() ? nullptr : params.diffuse_flux.data();
         if (!params.radiation_out_top.empty() && !params.radiation_out_bottom.empty()) {
             p.radiation_out_top = params.radiation_out_top.data();
@@ -210,9 +219,6 @@ namespace helios {
     void HipRayTracingBackend::launchDirectRays(const RayTracingLaunchParams &params) {
         if (primitive_count_ == 0) {
             return;
-        }
-        if (params.specular_reflection_enabled != 0) {
-            unsupported("launchDirectRays", "specular reflection");
         }
         bandTable(params, "launchDirectRays");
         rt_launch_params p = cParams(params);
@@ -272,9 +278,6 @@ namespace helios {
     void HipRayTracingBackend::launchCameraRays(const RayTracingLaunchParams &params) {
         if (primitive_count_ == 0) {
             return;
-        }
-        if (params.specular_reflection_enabled != 0) {
-            unsupported("launchCameraRays", "specular reflection");
         }
         rt_camera_params c = cameraParams(params);
         check(rt_launch_camera_rays(ctx_, &c), "launchCameraRays");
@@ -361,7 +364,7 @@ namespace helios {
     }
 
     void HipRayTracingBackend::zeroCameraScatterBuffers(size_t) {
-        // camera scatter buffers feed specular camera terms only, which this backend rejects
+        // the camera scatter buffers are the scatter buffers here (getRadiationResults), nothing to zero
     }
 
     void HipRayTracingBackend::uploadSourceFluxes(const std::vector<float> &fluxes) {
@@ -371,8 +374,11 @@ namespace helios {
         check(rt_upload_source_fluxes(ctx_, fluxes.data(), fluxes.size()), "uploadSourceFluxes");
     }
 
-    void HipRayTracingBackend::uploadSourceFluxesCam(const std::vector<float> &) {
-        // camera-weighted fluxes feed specular camera terms only, which this backend rejects
+    void HipRayTracingBackend::uploadSourceFluxesCam(const std::vector<float> &fluxes_cam) {
+        if (fluxes_cam.empty()) {
+            return; // no camera weights (VulkanComputeBackend.cpp:2221-2223)
+        }
+        check(rt_upload_source_fluxes_cam(ctx_, fluxes_cam.data(), fluxes_cam.size()), "uploadSourceFluxesCam");
     }
 
     void HipRayTracingBackend::queryGPUMemory() const {

@@ -77,7 +77,8 @@ class rt_launch_params(C.Structure):
                 ("random_seed", C.c_uint32), ("num_bands_global", C.c_uint32), ("num_bands_launch", C.c_uint32),
                 ("band_map", u32p), ("band_active", C.POINTER(C.c_uint8)), ("launch_face", C.c_uint32),
                 ("diffuse_flux", f32p), ("radiation_out_top", f32p), ("radiation_out_bottom", f32p),
-                ("diffuse_extinction", f32p), ("diffuse_peak_dir", f32p), ("diffuse_dist_norm", f32p)]
+                ("diffuse_extinction", f32p), ("diffuse_peak_dir", f32p), ("diffuse_dist_norm", f32p),
+                ("specular_reflection_enabled", C.c_uint32)]
 
 
 class rt_camera_params(C.Structure):
@@ -157,6 +158,9 @@ SIGNATURES = {
     "rt_copy_scatter_to_radiation": (C.c_int, [C.c_void_p]),
     "rt_upload_radiation_out": (C.c_int, [C.c_void_p, f32p, f32p, C.c_size_t]),
     "rt_upload_source_fluxes": (C.c_int, [C.c_void_p, f32p, C.c_size_t]),
+    "rt_update_specular_materials": (C.c_int, [C.c_void_p, C.c_uint32, f32p, f32p]),
+    "rt_upload_source_fluxes_cam": (C.c_int, [C.c_void_p, f32p, C.c_size_t]),
+    "rt_get_radiation_specular": (C.c_int, [C.c_void_p, f32p, C.c_size_t]),
     "rt_launch_direct_rays": (C.c_int, [C.c_void_p, C.POINTER(rt_launch_params)]),
     "rt_launch_diffuse_rays": (C.c_int, [C.c_void_p, C.POINTER(rt_launch_params)]),
     "rt_get_radiation_results": (C.c_int, [C.c_void_p, f32p, f32p, f32p, f32p, f32p]),
@@ -521,8 +525,26 @@ class Backend:
         f = _f32(fluxes).reshape(-1)
         _check(self.lib.rt_upload_source_fluxes(self.ctx._h, _pf(f), f.size))
 
+    def update_specular_materials(self, exponent, scale=None):
+        """specular_exponent [P] (not positive: no specular term) and specular_scale [P] (None = 0)."""
+        e = _f32(exponent).reshape(-1)
+        sc = None if scale is None else _f32(scale).reshape(-1)
+        _check(self.lib.rt_update_specular_materials(self.ctx._h, e.size, _pf(e), _pf(sc)))
+
+    def upload_source_fluxes_cam(self, weights):
+        """The camera's spectral weight of each source and launch band, [S][B_launch]."""
+        w = _f32(weights).reshape(-1)
+        _check(self.lib.rt_upload_source_fluxes_cam(self.ctx._h, _pf(w), w.size))
+
+    def get_radiation_specular(self) -> np.ndarray:
+        """radiation_specular [S][P][B_launch] accumulated by direct launches with specular on."""
+        S = self.info()["nsources"]
+        out = np.zeros((S, self.P, self.B_launch), np.float32)
+        _check(self.lib.rt_get_radiation_specular(self.ctx._h, _pf(out), out.size))
+        return out
+
     def _params(self, rays, seed, nbands_global, band_map=None, band_active=None, launch_face=1,
-                diffuse_flux=None, radiation_out=None, offset=0, count=None, sky=None):
+                diffuse_flux=None, radiation_out=None, offset=0, count=None, sky=None, specular=0):
         keep = []
         bm = None if band_map is None else _u32(band_map)
         ba = None if band_active is None else np.ascontiguousarray(band_active, dtype=np.uint8)
@@ -535,12 +557,14 @@ class Backend:
         p = rt_launch_params(int(offset), int(self.P - offset if count is None else count), int(rays), int(seed),
                              int(nbands_global), int(self.B_launch), _pu(bm),
                              None if ba is None else ba.ctypes.data_as(C.POINTER(C.c_uint8)), int(launch_face),
-                             _pf(df), _pf(ot), _pf(ob), _pf(sk[0]), _pf(sk[1]), _pf(sk[2]))
+                             _pf(df), _pf(ot), _pf(ob), _pf(sk[0]), _pf(sk[1]), _pf(sk[2]), int(specular))
         return p, keep
 
     def launch_direct_rays(self, rays_per_primitive: int, seed: int, nbands_global: int, band_map=None,
-                           offset: int = 0, count: Optional[int] = None):
-        p, keep = self._params(rays_per_primitive, seed, nbands_global, band_map, offset=offset, count=count)
+                           offset: int = 0, count: Optional[int] = None, specular: int = 0):
+        """specular: non-zero also adds each source's delivered radiation to radiation_specular."""
+        p, keep = self._params(rays_per_primitive, seed, nbands_global, band_map, offset=offset, count=count,
+                               specular=specular)
         _check(self.lib.rt_launch_direct_rays(self.ctx._h, C.byref(p)))
 
     def launch_diffuse_rays(self, rays_per_primitive: int, seed: int, nbands_global: int, launch_face: int,
@@ -585,16 +609,17 @@ class Backend:
                                             float(solar_disk_cos_angle), _pf(df), _pu(em)))
 
     @staticmethod
-    def camera_params(camera: "Camera", tile=None, seed: int = 0, specular: bool = False) -> rt_camera_params:
-        """Launch parameters of one tile (x0, y0, w, h) of a camera (the whole image when tile is None)."""
+    def camera_params(camera: "Camera", tile=None, seed: int = 0, specular: int = 0) -> rt_camera_params:
+        """Launch parameters of one tile (x0, y0, w, h) of a camera (the whole image when tile is None);
+        specular: 0 off, 1 Blinn-Phong term with scale 1, 2 with the primitives' specular_scale."""
         fx, fy = camera.resolution
         x0, y0, w, h = (0, 0, fx, fy) if tile is None else tile
         return rt_camera_params((C.c_float * 3)(*camera.position), (C.c_float * 2)(*camera.direction),
                                 camera.focal_length, camera.lens_diameter, camera.fov_aspect, camera.viewplane_length,
                                 (C.c_uint32 * 2)(w, h), (C.c_uint32 * 2)(fx, fy), (C.c_uint32 * 2)(x0, y0),
-                                camera.antialiasing_samples, int(seed), int(bool(specular)))
+                                camera.antialiasing_samples, int(seed), int(specular))
 
-    def launch_camera_rays(self, camera: "Camera", seed: int = 0, tile=None, specular: bool = False):
+    def launch_camera_rays(self, camera: "Camera", seed: int = 0, tile=None, specular: int = 0):
         p = self.camera_params(camera, tile, seed, specular)
         _check(self.lib.rt_launch_camera_rays(self.ctx._h, C.byref(p)))
 

@@ -273,6 +273,19 @@ int rt_upload_radiation_out(rt_ctx *ctx, const float *top, const float *bottom, 
 /* uploadSourceFluxes (RayTracingBackend.h:247): [S][B_launch] */
 int rt_upload_source_fluxes(rt_ctx *ctx, const float *fluxes, size_t count);
 
+/* updateMaterials' specular fields (RayTracingTypes.h:170-171; VulkanComputeBackend.cpp:631-655):
+ * specular_exponent [P] (not positive = no specular term) and specular_scale [P] (NULL = 0), read by
+ * camera launches with specular_reflection_enabled. */
+int rt_update_specular_materials(rt_ctx *ctx, uint32_t nprims, const float *exponent, const float *scale);
+/* uploadSourceFluxesCam (RayTracingBackend.h:250-258; VulkanComputeBackend.cpp:2220-2237): the camera's
+ * spectral weight of each source and launch band, [S][B_launch]; needed by camera launches with
+ * specular on. */
+int rt_upload_source_fluxes_cam(rt_ctx *ctx, const float *fluxes_cam, size_t count);
+/* RayTracingResults::radiation_specular (RayTracingTypes.h:265) of this backend's single camera slot:
+ * [S][P][B_launch] host memory, count = S*P*B_launch; zeros when no direct launch with specular on ran
+ * since rt_zero_radiation_buffers. */
+int rt_get_radiation_specular(rt_ctx *ctx, float *out, size_t count);
+
 /* RayTracingLaunchParams (RayTracingTypes.h:201-249), radiation fields.  The reference passes a
  * [B_global] band_launch_flag and derives the launch -> global band map from it; here the map is
  * explicit and a launch band that stops scattering keeps its slot with band_active = 0 (the
@@ -296,6 +309,10 @@ typedef struct {
     const float *diffuse_extinction;
     const float *diffuse_peak_dir;     /* [B_launch][3] */
     const float *diffuse_dist_norm;
+    /* direct launches (RayTracingLaunchParams::specular_reflection_enabled): non-zero adds each source's
+     * flux x strength over the unoccluded rays to radiation_specular [S][P][B_launch]
+     * (R/shaders/direct_raygen.comp:717-726), which camera launches with specular on read */
+    uint32_t specular_reflection_enabled;
 } rt_launch_params;
 /* launchDirectRays (RayTracingBackend.h:139; R/shaders/direct_raygen.comp): every source, then
  * radiation_in / scatter_top / scatter_bottom += the deposits. */
@@ -321,7 +338,8 @@ typedef struct {
     uint32_t pixel_offset[2];        /* tile origin in the whole image                            */
     uint32_t antialiasing_samples;   /* 0 is taken as 1                                           */
     uint32_t random_seed;
-    uint32_t specular_reflection_enabled; /* must be 0: specular camera terms are not supported   */
+    uint32_t specular_reflection_enabled; /* 0 off; 1 Blinn-Phong term with scale 1; 2 with the
+                                             primitives' specular_scale (camera_raygen.comp:362-409) */
 } rt_camera_params;
 /* updateSkyModel (RayTracingBackend.h:129): per launch band, sky_radiance_params [B][4] (Prague
  * circumsolar strength, circumsolar width in degrees, horizon brightness, normalisation; a

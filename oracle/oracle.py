@@ -37,7 +37,9 @@ class orc_camera_light(C.Structure):
     _fields_ = [("nbands", C.c_uint32), ("nsources", C.c_uint32), ("out_top", f32p), ("out_bottom", f32p),
                 ("source_flux", f32p), ("source_pos", f32p), ("source_type", u32p), ("source_width", f32p),
                 ("source_rotation", f32p), ("sky_params", f32p), ("sky_radiance", f32p), ("solar_radiance", f32p),
-                ("camera_diffuse_flux", f32p), ("emission", u32p), ("sun", C.c_float * 3), ("solar_cos", C.c_float)]
+                ("camera_diffuse_flux", f32p), ("emission", u32p), ("sun", C.c_float * 3), ("solar_cos", C.c_float),
+                ("spec", f32p), ("spec_weight", f32p), ("spec_exp", f32p), ("spec_scale", f32p),
+                ("spec_mode", C.c_uint32)]
 
 
 _lib = None
@@ -277,9 +279,10 @@ class OracleScene:
                           (C.c_uint32 * 2)(fx, fy), (C.c_uint32 * 2)(x0, y0), cam.antialiasing_samples, int(seed))
 
     def camera_pixels(self, cam, out_top, out_bottom, source_flux, source_pos, sky, seed=0, tile=None,
-                      source_types=None, source_widths=None, source_rotations=None, pixels=None):
+                      source_types=None, source_widths=None, source_rotations=None, pixels=None, specular=None):
         """Camera radiation [B][H*W] (FP64 sums of the FP32 samples), added to `pixels` when given.
-        sky: dict(params [B][4], radiance [B], sun [3], solar [B], solar_cos, diffuse_flux [B], emission [B])."""
+        sky: dict(params [B][4], radiance [B], sun [3], solar [B], solar_cos, diffuse_flux [B], emission [B]).
+        specular: dict(spec [S][P][B], weight [S][B], exponent [P], scale [P] or None, mode 1|2), or None."""
         ot, ob = _f(out_top).reshape(self.P, -1), _f(out_bottom).reshape(self.P, -1)
         B = ot.shape[1]
         X = _f(source_pos).reshape(-1, 3)
@@ -296,6 +299,15 @@ class OracleScene:
         L = orc_camera_light(B, S, pt(ot), pt(ob), pt(F), pt(X), None if ty is None else ty.ctypes.data_as(u32p),
                              pt(W), pt(R), pt(prm), pt(srad), pt(solar), pt(df), em.ctypes.data_as(u32p),
                              (C.c_float * 3)(*[float(v) for v in sky["sun"]]), float(sky["solar_cos"]))
+        keep = []
+        if specular is not None:
+            sp = _f(specular["spec"]).reshape(S, self.P, B)
+            sw = _f(specular["weight"]).reshape(S, B)
+            se = _f(specular["exponent"]).reshape(self.P)
+            ss = _f(specular["scale"] if specular.get("scale") is not None else np.zeros(self.P)).reshape(self.P)
+            keep = [sp, sw, se, ss]
+            L.spec, L.spec_weight, L.spec_exp, L.spec_scale = pt(sp), pt(sw), pt(se), pt(ss)
+            L.spec_mode = int(specular["mode"])
         fx, fy = cam.resolution
         out = np.zeros((B, fx * fy), np.float64) if pixels is None else pixels
         c = self._camera(cam, tile, seed)

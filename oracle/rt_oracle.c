@@ -1360,6 +1360,33 @@ static float cam_sky(v3 d, const float *sun, const float *prm) {
     return circ * horizon * prm[3] * pi;
 }
 
+/* Blinn-Phong specular term of a camera hit on primitive h for band b (camera_raygen.comp:362-409): per
+ * source, radiation_specular x the camera weight x 0.25 (Fresnel reflectance); when positive, x the scale
+ * (mode 2: specular_scale, else 1) x max(0, n.h)^e x (e + 2) / (samples x 2 pi), with the half vector h =
+ * normalize(l - d), l the direction of a collimated or sun-sphere source or towards the centre of any
+ * other source from the hit point o + d t, and n the primitive's normal.  FP32 throughout. */
+static float cam_specular(const orc_scene *s, const orc_camera_light *L, v3 o, v3 d, float t, uint32_t h,
+                          uint32_t b, uint32_t spp) {
+    const float pi = 3.14159265359f;
+    const float e = L->spec_exp[h];
+    float acc = 0.0f;
+    if (!(e > 0.0f)) return 0.0f;
+    for (uint32_t k = 0; k < L->nsources; k++) {
+        float spec = L->spec[((size_t)k * s->P + h) * L->nbands + b];
+        spec = spec * (L->spec_weight[(size_t)k * L->nbands + b] * 0.25f);
+        if (!(spec > 0.0f)) continue;
+        uint32_t ty = L->source_type ? L->source_type[k] : 0u;
+        v3 l = ld3(L->source_pos + 3 * k);
+        if (!(ty == 0u || ty == 2u)) l = sub(l, add(o, scl(d, t)));
+        l = nrm(l);
+        v3 hv = nrm(sub(l, d));
+        float scale = L->spec_mode == 2u ? L->spec_scale[h] : 1.0f;
+        float ch = fmaxf(0.0f, hv.x * s->normal[h].x + hv.y * s->normal[h].y + hv.z * s->normal[h].z);
+        acc += spec * scale * powf(ch, e) * (e + 2.0f) / ((float)spp * 2.0f * pi);
+    }
+    return acc;
+}
+
 void orc_camera_pixels(const orc_scene *s, const orc_camera *c, const orc_camera_light *L, double *pixels) {
     const float pi = 3.14159265359f;
     float st, ct, sp, cp;
@@ -1394,6 +1421,7 @@ void orc_camera_pixels(const orc_scene *s, const orc_camera *c, const orc_camera
                         float srad = cam_sources(L, mats, o, d, b, t, 0);
                         float surf = (top ? L->out_top : L->out_bottom)[(size_t)h * B + b] * strength;
                         if (srad > 0.0f) surf += srad * strength;
+                        if (L->spec) surf += cam_specular(s, L, o, d, t, h, b, spp);
                         pixels[(size_t)b * npix + pix] += (double)(surf / pi);
                     }
                 } else {

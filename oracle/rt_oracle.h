@@ -121,6 +121,12 @@ typedef struct {
     const uint32_t *emission;           /* [B]                                             */
     float sun[3];
     float solar_cos;
+    /* specular reflection (spec NULL: off) */
+    const float *spec;                  /* [S][P][B] radiation of each source delivered to each primitive */
+    const float *spec_weight;           /* [S][B] camera weights of the sources             */
+    const float *spec_exp;              /* [P] specular exponents                           */
+    const float *spec_scale;            /* [P] specular scales (mode 2)                     */
+    uint32_t spec_mode;                 /* 1 = scale 1, 2 = spec_scale                      */
 } orc_camera_light;
 /* Adds the tile's samples to pixels [B][full_x*full_y] (FP64 accumulation of FP32 samples). */
 void orc_camera_pixels(const orc_scene *s, const orc_camera *c, const orc_camera_light *L, double *pixels);

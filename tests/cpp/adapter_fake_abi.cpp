@@ -133,6 +133,17 @@ int rt_upload_source_fluxes(rt_ctx *, const float *fluxes, size_t count) {
     return status("rt_upload_source_fluxes");
 }
 
+int rt_update_specular_materials(rt_ctx *, uint32_t nprims, const float *exponent, const float *scale) {
+    g_fake.spec_exp = copy(exponent, nprims);
+    g_fake.spec_scale = copy(scale, nprims);
+    return status("rt_update_specular_materials");
+}
+
+int rt_upload_source_fluxes_cam(rt_ctx *, const float *fluxes_cam, size_t count) {
+    g_fake.fluxes_cam = copy(fluxes_cam, count);
+    return status("rt_upload_source_fluxes_cam");
+}
+
 int rt_launch_direct_rays(rt_ctx *, const rt_launch_params *params) {
     g_fake.direct.push_back(record(params));
     return status("rt_launch_direct_rays");

@@ -45,6 +45,8 @@ struct FakeAbi {
     std::vector<rt_camera_params> camera, label;
     // rt_upload_source_fluxes / rt_upload_radiation_out
     std::vector<float> fluxes, out_top, out_bottom;
+    // rt_update_specular_materials / rt_upload_source_fluxes_cam
+    std::vector<float> spec_exp, spec_scale, fluxes_cam;
     int fail_next = 0;  // a nonzero status makes the next call fail with this message
     std::string fail_msg = "injected failure";
 };

@@ -150,6 +150,14 @@ int main() {
     mat.is_glass.assign(12, 0);
     rb->updateMaterials(mat);
     CHECK(F.mat_S == 1 && F.mat_P == 4 && F.mat_B == 3 && F.rho == mat.reflectivity && F.tau == mat.transmissivity);
+    CHECK(F.spec_exp.empty());  // no specular fields, no specular call
+    RayTracingMaterial shiny = mat;
+    shiny.specular_exponent = {50.f, -1.f, 10.f, 0.f};
+    shiny.specular_scale = {0.5f, 0.f, 1.f, 2.f};
+    rb->updateMaterials(shiny);
+    CHECK(F.spec_exp == shiny.specular_exponent && F.spec_scale == shiny.specular_scale);
+    shiny.specular_scale.resize(3);
+    CHECK(throws_with([&] { rb->updateMaterials(shiny); }, "specular"));
     RayTracingMaterial glass = mat;
     glass.is_glass[5] = 1;
     CHECK(throws_with([&] { rb->updateMaterials(glass); }, "glass"));
@@ -218,11 +226,18 @@ int main() {
     rb->zeroRadiationBuffers(2);
     CHECK(throws_with([&] { rb->launchDirectRays(launch(3, 2, {true, false, false})); }, "selects 1 bands"));
     rb->zeroRadiationBuffers(2);
-    // specular reflection is rejected by name
+    // specular reflection: the mode travels with direct and camera launches, the camera weights and the
+    // per-primitive exponent / scale reach their entry points
     RayTracingLaunchParams sp = launch(3, 2, {true, false, true});
-    sp.specular_reflection_enabled = 1;
-    CHECK(throws_with([&] { rb->launchDirectRays(sp); }, "specular"));
-    CHECK(throws_with([&] { rb->launchCameraRays(sp); }, "specular"));
+    sp.specular_reflection_enabled = 2;
+    rb->launchDirectRays(sp);
+    CHECK(F.direct.back().p.specular_reflection_enabled == 2);
+    rb->launchCameraRays(sp);
+    CHECK(F.camera.back().specular_reflection_enabled == 2);
+    rb->uploadSourceFluxesCam({0.5f, 0.25f});
+    CHECK((F.fluxes_cam == std::vector<float>{0.5f, 0.25f}));
+    rb->uploadSourceFluxesCam({});
+    CHECK(F.calls.back() == "rt_upload_source_fluxes_cam" && F.fluxes_cam.size() == 2);
 
     // results: [P][B_launch] copies; the camera scatter buffers equal the scatter buffers
     rb->uploadRadiationOut(std::vector<float>(8, 3.f), std::vector<float>(8, 4.f));
