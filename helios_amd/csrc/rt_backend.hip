@@ -140,6 +140,76 @@ int launch_bands(const BackendState &be, const rt_launch_params *p, std::vector<
 // primitives (and the per-primitive rows of the output buffers) that a full launch would.
 inline uint32_t range_begin(uint32_t offset) { return 0u - offset; }
 
+// Translucent covers of a launch: primitives glass in every launch band (R/shaders/direct_raygen.comp:
+// 629-637), with their refractive index and absorption product per launch band.  Returns 1 when the launch
+// has covers, 0 when not, -1 on error (more launch bands than the glass kernels carry).
+int glass_launch(const BackendState &be, const std::vector<uint32_t> &map, std::vector<uint8_t> &cover,
+                 std::vector<float> &gn, std::vector<float> &gkl, const char *who) {
+    if (be.g_is.empty()) return 0;
+    const uint32_t P = be.P, Bl = (uint32_t)map.size(), Bg = be.Bg;
+    cover.assign(P, 0);
+    bool any = false;
+    for (uint32_t i = 0; i < P; i++) {
+        bool all = Bl > 0;
+        for (uint32_t b = 0; b < Bl && all; b++) all = be.g_is[(size_t)i * Bg + map[b]] != 0;
+        cover[i] = all ? 1 : 0;
+        any = any || all;
+    }
+    if (!any) return 0;
+    if (Bl > RT_GLASS_MAX_CH) {
+        set_error(std::string(who) + ": a launch with translucent covers takes at most " +
+                  std::to_string(RT_GLASS_MAX_CH) + " bands, this one has " + std::to_string(Bl));
+        return -1;
+    }
+    gn.assign((size_t)P * Bl, 1.0f);
+    gkl.assign((size_t)P * Bl, 0.0f);
+    for (uint32_t i = 0; i < P; i++)
+        for (uint32_t b = 0; b < Bl; b++) {
+            gn[(size_t)i * Bl + b] = be.g_n[(size_t)i * Bg + map[b]];
+            gkl[(size_t)i * Bl + b] = be.g_kl[(size_t)i * Bg + map[b]];
+        }
+    return 1;
+}
+
+// Device tables of a launch with covers, carved from the launch's workspace and filled on `st`.
+struct GlassTables {
+    uint8_t *cover = nullptr;
+    float *gn = nullptr, *gkl = nullptr;
+    double *push = nullptr;  // [P][B_launch][3] shares pushed into the covers
+    GlassCtx *ctx = nullptr;
+};
+
+size_t glass_bytes(uint32_t P, uint32_t Bl) {
+    return (size_t)P + 2 * (size_t)P * Bl * 4 + (size_t)P * Bl * 3 * 8 + sizeof(GlassCtx) + 6 * 256;
+}
+
+int glass_upload(GlassTables &g, const std::vector<uint8_t> &cover, const std::vector<float> &gn,
+                 const std::vector<float> &gkl, uint32_t P, uint32_t Bl, hipStream_t st) {
+    CKR(hipMemcpyAsync(g.cover, cover.data(), P, hipMemcpyHostToDevice, st));
+    CKR(hipMemcpyAsync(g.gn, gn.data(), (size_t)P * Bl * 4, hipMemcpyHostToDevice, st));
+    CKR(hipMemcpyAsync(g.gkl, gkl.data(), (size_t)P * Bl * 4, hipMemcpyHostToDevice, st));
+    CKR(hipMemsetAsync(g.push, 0, (size_t)P * Bl * 3 * 8, st));
+    GlassCtx h{};
+    h.cover = g.cover; h.gn = g.gn; h.gkl = g.gkl; h.push = g.push; h.B = Bl; h.C = Bl;
+    CKR(hipMemcpyAsync(g.ctx, &h, sizeof(GlassCtx), hipMemcpyHostToDevice, st));
+    // the host tables must outlive the copies
+    CKR(hipStreamSynchronize(st));
+    return 0;
+}
+
+// radiation_specular from the per-band sums of a launch with covers (gsum [S][P][B][2], the flux behind
+// the covers' transmittance, as the reference's specular sum takes it)
+__global__ void k_deposit_specular_bands(uint32_t P, uint32_t B, uint32_t S, const float *__restrict__ gsum,
+                                         const float *__restrict__ flux, const uint8_t *__restrict__ active,
+                                         float *__restrict__ spec) {
+    size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (size_t)S * P * B) return;
+    const uint32_t b = (uint32_t)(q % B);
+    const uint32_t s = (uint32_t)(q / B / P);
+    if (!active[b]) return;
+    spec[q] += flux[(size_t)s * B + b] * (gsum[q * 2] + gsum[q * 2 + 1]);
+}
+
 }  // namespace
 
 void backend_release(BackendState &be) {
@@ -237,6 +307,7 @@ int rt_build_acceleration_structure(rt_ctx *ctx) {
         be.has_specular = false;
         be.has_fcam = false;
         be.spec_live = false;
+        be.g_is.clear(); be.g_n.clear(); be.g_kl.clear();
         be.Bl = 0;
     }
     be.P = be.gP;
@@ -260,11 +331,44 @@ int rt_update_materials(rt_ctx *ctx, uint32_t nsources, uint32_t nprims, uint32_
     CKR(hipStreamSynchronize(ctx->sx.stream));
     size_t PB = (size_t)be.P * nbands_global;
     if (grow(be.mat, be.mat_cap, 2 * PB * sizeof(float) + 256)) return -1;
+    // new materials carry no covers until rt_update_glass_materials sets them
+    be.g_is.clear(); be.g_n.clear(); be.g_kl.clear();
     be.Bg = nbands_global;
     // source-0 slice, as the Vulkan backend's shaders index rho/tau (R/shaders/direct_raygen.comp:684-687)
     CKR(hipMemcpy(be.rho(), rho, PB * sizeof(float), hipMemcpyHostToDevice));
     CKR(hipMemcpy(be.tau(), tau, PB * sizeof(float), hipMemcpyHostToDevice));
     be.has_materials = true;
+    return 0;
+}
+
+int rt_update_glass_materials(rt_ctx *ctx, uint32_t nprims, uint32_t nbands_global, const uint8_t *is_glass,
+                              const float *glass_n, const float *glass_KL) {
+    NN(ctx, "ctx");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    BackendState &be = ctx->be;
+    if (need_scene(be, "rt_update_glass_materials")) return -1;
+    if (nprims != be.P || nbands_global != be.Bg || !be.has_materials) {
+        set_error("rt_update_glass_materials: " + std::to_string(nprims) + " primitives x " +
+                  std::to_string(nbands_global) + " bands, the materials have " + std::to_string(be.P) + " x " +
+                  std::to_string(be.Bg) + " (call rt_update_materials first)");
+        return -1;
+    }
+    if (!is_glass) {  // no covers
+        be.g_is.clear(); be.g_n.clear(); be.g_kl.clear();
+        return 0;
+    }
+    NN(glass_n, "glass_n");
+    const size_t n = (size_t)nprims * nbands_global;
+    for (size_t k = 0; k < n; k++)
+        if (is_glass[k] && (!(glass_n[k] >= 1.0f) || (glass_KL && !(glass_KL[k] >= 0.0f)))) {
+            set_error("rt_update_glass_materials: primitive " + std::to_string(k / nbands_global) + " band " +
+                      std::to_string(k % nbands_global) + ": glass needs a refractive index >= 1 and an absorption K*L >= 0");
+            return -1;
+        }
+    be.g_is.assign(is_glass, is_glass + n);
+    be.g_n.assign(glass_n, glass_n + n);
+    if (glass_KL) be.g_kl.assign(glass_KL, glass_KL + n);
+    else be.g_kl.assign(n, 0.0f);
     return 0;
 }
 
@@ -443,11 +547,17 @@ int rt_launch_direct_rays(rt_ctx *ctx, const rt_launch_params *p) {
     if (launch_bands(be, p, map, active, "rt_launch_direct_rays")) return -1;
     if (be.S == 0 || p->launch_count == 0 || p->rays_per_primitive == 0) return 0;
     if (!be.has_fluxes) { set_error("rt_launch_direct_rays: no source fluxes (call rt_upload_source_fluxes)"); return -1; }
+    std::vector<uint8_t> gcover;
+    std::vector<float> ggn, ggkl;
+    const int gl = glass_launch(be, map, gcover, ggn, ggkl, "rt_launch_direct_rays");
+    if (gl < 0) return -1;
+    const bool glass = gl == 1;
     CKR(hipSetDevice(ctx->device));
     hipStream_t st = ctx->sx.stream;
     const uint32_t P = be.P, Bl = be.Bl, S = be.S;
     const uint32_t d = (uint32_t)std::ceil(std::sqrt((double)p->rays_per_primitive));
     size_t bytes = (size_t)S * P * 2 * 4 + 2 * (size_t)P * Bl * 4 + Bl * 8 + S * sizeof(DirectJob) + 16 * 256;
+    if (glass) bytes += glass_bytes(P, Bl) + (size_t)S * P * Bl * 2 * 4 + S * sizeof(GlassDirectJob) + 2 * 256;
     if (be.ws.reserve(bytes)) return -1;
     Carve cv{(char *)be.ws.base};
     float *dsum = cv.take<float>((size_t)S * P * 2);
@@ -457,6 +567,19 @@ int rt_launch_direct_rays(rt_ctx *ctx, const rt_launch_params *p) {
     DirectJob *jobs = cv.take<DirectJob>(S);
     uint32_t *counter = cv.take<uint32_t>(16);
     int *err = reinterpret_cast<int *>(counter + 8);
+    GlassTables gt;
+    float *gsum = nullptr;            // [S][P][B_launch][2] strength x cover transmittance per band and face
+    GlassDirectJob *gjobs = nullptr;  // [S]
+    if (glass) {
+        gt.cover = cv.take<uint8_t>(P);
+        gt.gn = cv.take<float>((size_t)P * Bl);
+        gt.gkl = cv.take<float>((size_t)P * Bl);
+        gt.push = cv.take<double>((size_t)P * Bl * 3);
+        gt.ctx = cv.take<GlassCtx>(1);
+        gsum = cv.take<float>((size_t)S * P * Bl * 2);
+        gjobs = cv.take<GlassDirectJob>(S);
+    }
+    if (cv.off > be.ws.cap) { set_error("rt_launch_direct_rays: workspace carve overflow"); return -1; }
     // one job per source, collimated sources first (k_direct), then the other types (k_direct_src)
     std::vector<DirectJob> hj;
     std::vector<float> coldir;
@@ -476,8 +599,9 @@ int rt_launch_direct_rays(rt_ctx *ctx, const rt_launch_params *p) {
             hj.push_back(j);
         }
     const uint32_t ncol = (uint32_t)(coldir.size() / 3);
-    // occlusion grids per collimated source direction (non-periodic scenes), as in rt_run_band
-    if (ctx->sx.use_grid && ncol) {
+    // occlusion grids per collimated source direction (non-periodic scenes), as in rt_run_band; a launch with
+    // translucent covers traces its shadow rays through them instead
+    if (ctx->sx.use_grid && ncol && !glass) {
         ctx->sx.grids.density = (float)ctx->sx.cfg[0].grid_density;
         int g = build_sun_grids(ctx->sx.grids, be.scene, coldir.data(), ncol, st);
         if (g < 0) return -1;
@@ -494,7 +618,29 @@ int rt_launch_direct_rays(rt_ctx *ctx, const rt_launch_params *p) {
     CKR(hipGetLastError());
     // Sources are jobs over items [s*count, (s+1)*count).  find_job needs ascending item_begin, which
     // the modular offset of a sub-range breaks for S > 1: such launches go one source at a time.
-    if (p->launch_offset == 0) {
+    std::vector<GlassDirectJob> hg;
+    if (glass) {
+        if (glass_upload(gt, gcover, ggn, ggkl, P, Bl, st)) return -1;
+        CKR(hipMemsetAsync(gsum, 0, (size_t)S * P * Bl * 2 * 4, st));
+        for (const DirectJob &j : hj) {
+            GlassDirectJob g{};
+            g.dj = j;
+            const size_t s = (size_t)(j.out - dsum) / ((size_t)P * 2);  // the job's source (its output slot)
+            g.flux = (const float *)be.flux + s * Bl;
+            g.gsum = gsum + s * P * Bl * 2;
+            g.chan0 = 0;
+            hg.push_back(g);
+        }
+        for (uint32_t s = 0; s < (p->launch_offset == 0 ? 1u : S); s++) {
+            const uint32_t nj = p->launch_offset == 0 ? S : 1u;
+            if (p->launch_offset != 0) hg[s].dj.item_begin = range_begin(p->launch_offset);
+            CKR(hipMemcpyAsync(gjobs + s, hg.data() + s, nj * sizeof(GlassDirectJob), hipMemcpyHostToDevice, st));
+            CKR(hipStreamSynchronize(st));
+            if (launch_direct_glass(gjobs + s, nj, nj * p->launch_count, be.scene.h.depth, counter, gt.ctx, err,
+                                    ctx->sx.ncu, st, scene_flags(be.scene, ctx->sx)))
+                return -1;
+        }
+    } else if (p->launch_offset == 0) {
         if (launch_direct_jobs(hj.data(), jobs, S, S * p->launch_count, be.scene.h.depth, counter, err,
                                ctx->sx.stat_ptr(0), ctx->sx.cfg[0], ctx->sx.ncu, st, scene_flags(be.scene, ctx->sx)))
             return -1;
@@ -510,8 +656,9 @@ int rt_launch_direct_rays(rt_ctx *ctx, const rt_launch_params *p) {
         }
     }
     if (launch_deposit_direct(P, Bl, Bl, S, dsum, (const float *)be.flux, rl, tl, dact, be.rbuf(0), be.rbuf(1),
-                              be.rbuf(2), st))
+                              be.rbuf(2), st, gsum))
         return -1;
+    if (glass && launch_add_push((size_t)P * Bl, gt.push, be.rbuf(0), be.rbuf(1), be.rbuf(2), st)) return -1;
     if (p->specular_reflection_enabled) {
         const size_t n = (size_t)S * P * Bl;
         if (!be.spec_live || be.spec_S != S) {
@@ -521,7 +668,12 @@ int rt_launch_direct_rays(rt_ctx *ctx, const rt_launch_params *p) {
             be.spec_live = true;
             be.spec_S = S;
         }
-        k_deposit_specular<<<nblk(n), 256, 0, st>>>(P, Bl, S, dsum, (const float *)be.flux, dact, (float *)be.spec);
+        if (glass)
+            k_deposit_specular_bands<<<nblk(n), 256, 0, st>>>(P, Bl, S, gsum, (const float *)be.flux, dact,
+                                                              (float *)be.spec);
+        else
+            k_deposit_specular<<<nblk(n), 256, 0, st>>>(P, Bl, S, dsum, (const float *)be.flux, dact,
+                                                        (float *)be.spec);
         CKR(hipGetLastError());
     }
     return check_err(ctx, err);
@@ -541,6 +693,11 @@ int rt_launch_diffuse_rays(rt_ctx *ctx, const rt_launch_params *p) {
         set_error("rt_launch_diffuse_rays: radiation_out_top and radiation_out_bottom must both be given or both NULL");
         return -1;
     }
+    std::vector<uint8_t> gcover;
+    std::vector<float> ggn, ggkl;
+    const int gl = glass_launch(be, map, gcover, ggn, ggkl, "rt_launch_diffuse_rays");
+    if (gl < 0) return -1;
+    const bool glass = gl == 1;
     CKR(hipSetDevice(ctx->device));
     hipStream_t st = ctx->sx.stream;
     const uint32_t P = be.P, Bl = be.Bl;
@@ -586,11 +743,12 @@ int rt_launch_diffuse_rays(rt_ctx *ctx, const rt_launch_params *p) {
             }
         }
     }
-    size_t bytes = (size_t)P * 2 * nn * 4 + 2 * PB * 4 + Bl * 9 + sizeof(DiffuseJob) + sizeof(GatherJob) +
-                   hsky.size() * 4 + 17 * 256;
+    // a launch with covers traces and gathers in one kernel and keeps no hit table
+    size_t bytes = (glass ? 0 : (size_t)P * 2 * nn * 4) + 2 * PB * 4 + Bl * 9 + sizeof(DiffuseJob) +
+                   sizeof(GatherJob) + hsky.size() * 4 + 17 * 256 + (glass ? glass_bytes(P, Bl) : 0);
     if (be.ws.reserve(bytes)) return -1;
     Carve cv{(char *)be.ws.base};
-    uint32_t *hits = cv.take<uint32_t>((size_t)P * 2 * nn);
+    uint32_t *hits = glass ? nullptr : cv.take<uint32_t>((size_t)P * 2 * nn);
     float *rl = cv.take<float>(PB), *tl = cv.take<float>(PB);
     uint32_t *dmap = cv.take<uint32_t>(Bl);
     float *dfl = cv.take<float>(Bl);
@@ -600,6 +758,15 @@ int rt_launch_diffuse_rays(rt_ctx *ctx, const rt_launch_params *p) {
     float *dsky = hsky.empty() ? nullptr : cv.take<float>(hsky.size());
     uint32_t *counter = cv.take<uint32_t>(16);
     int *err = reinterpret_cast<int *>(counter + 8);
+    GlassTables gt;
+    if (glass) {
+        gt.cover = cv.take<uint8_t>(P);
+        gt.gn = cv.take<float>(PB);
+        gt.gkl = cv.take<float>(PB);
+        gt.push = cv.take<double>(PB * 3);
+        gt.ctx = cv.take<GlassCtx>(1);
+    }
+    if (cv.off > be.ws.cap) { set_error("rt_launch_diffuse_rays: workspace carve overflow"); return -1; }
     std::vector<float> hf(Bl, 0.f);
     if (p->diffuse_flux)
         for (uint32_t b = 0; b < Bl; b++) hf[b] = p->diffuse_flux[b];
@@ -640,6 +807,15 @@ int rt_launch_diffuse_rays(rt_ctx *ctx, const rt_launch_params *p) {
     CKR(hipStreamSynchronize(st));
     k_select_bands<<<nblk(PB), 256, 0, st>>>(P, be.Bg, Bl, dmap, be.rho(), be.tau(), rl, tl);
     CKR(hipGetLastError());
+    if (glass) {
+        // rays pass through covers, which take their shares of the sky flux (diffuse_raygen.comp:484-560)
+        if (glass_upload(gt, gcover, ggn, ggkl, P, Bl, st)) return -1;
+        if (launch_diffuse_glass(fjob, gjob, p->launch_count, be.scene.h.depth, counter, gt.ctx, err, ctx->sx.ncu, st,
+                                 scene_flags(be.scene, ctx->sx)))
+            return -1;
+        if (launch_add_push(PB, gt.push, be.rbuf(0), be.rbuf(1), be.rbuf(2), st)) return -1;
+        return check_err(ctx, err);
+    }
     if (launch_diffuse(fjob, p->launch_count, be.scene.h.depth, counter, err, ctx->sx.stat_ptr(1), ctx->sx.cfg[1],
                        ctx->sx.ncu, st, scene_flags(be.scene, ctx->sx)))
         return -1;

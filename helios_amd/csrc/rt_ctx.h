@@ -67,6 +67,10 @@ struct BackendState {
     size_t spec_cap = 0;
     bool spec_live = false;      // spec holds this launch set's sums (false: all zero)
     uint32_t spec_S = 0;         // sources of those sums
+    // translucent covers (updateMaterials' glass fields, source-0 slice): [P][Bg] each, host side; a launch
+    // selects its bands and uploads the covers' parameters with its other tables
+    std::vector<uint8_t> g_is;
+    std::vector<float> g_n, g_kl;
     float *rho() const { return (float *)mat; }
     float *tau() const { return (float *)mat + (size_t)P * Bg; }
     float *rbuf(int k) const { return (float *)rad + (size_t)k * P * Bl; }  // 0 in 1 st 2 sb 3 ot 4 ob

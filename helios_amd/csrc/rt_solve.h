@@ -113,10 +113,11 @@ size_t solve_workspace_bytes(uint32_t P, const rt_band_request &q, uint32_t K, b
 
 int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunSet &suns, float *R_out,
               int out_on_device, const DevMaterials *dm = nullptr);
-// radiation_in / scatter_top / scatter_bottom += direct strength sums x source flux x optical properties
+// radiation_in / scatter_top / scatter_bottom += direct strength sums x source flux x optical properties;
+// gsum (per-band sums [S][P][B][2] of a launch with translucent covers) replaces dsum when given
 int launch_deposit_direct(uint32_t P, uint32_t C, uint32_t B, uint32_t S, const float *dsum, const float *flux,
                           const float *rho, const float *tau, const uint8_t *mask, float *in, float *st, float *sb,
-                          hipStream_t s);
+                          hipStream_t s, const float *gsum = nullptr);
 int debug_diffuse_hits(SolveCtx &sx, const Scene &S, uint32_t n, uint32_t seed, uint32_t *out);
 int debug_direct_sums(SolveCtx &sx, const Scene &S, uint32_t d, uint32_t seed, uint32_t nsrc, const float *src_dir,
                       const uint32_t *types, const float *widths, const float *rotations, float *out);

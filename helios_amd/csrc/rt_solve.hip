@@ -250,10 +250,10 @@ static inline uint32_t nb(size_t n) { return (uint32_t)((n + 255) / 256); }
 
 int launch_deposit_direct(uint32_t P, uint32_t C, uint32_t B, uint32_t S, const float *dsum, const float *flux,
                           const float *rho, const float *tau, const uint8_t *mask, float *in, float *st, float *sb,
-                          hipStream_t s) {
+                          hipStream_t s, const float *gsum) {
     size_t PC = (size_t)P * C;
     if (PC == 0) return 0;
-    k_deposit_direct<<<nb(PC), 256, 0, s>>>(P, C, B, S, dsum, flux, rho, tau, mask, in, st, sb, nullptr);
+    k_deposit_direct<<<nb(PC), 256, 0, s>>>(P, C, B, S, dsum, flux, rho, tau, mask, in, st, sb, gsum);
     LK();
     return 0;
 }

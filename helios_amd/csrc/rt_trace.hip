@@ -2707,6 +2707,8 @@ __global__ __launch_bounds__(256) void k_diffuse_glass(const DiffuseJob *__restr
             const uint32_t r = base + lane;
             if (r >= nslot * nn) continue;
             const uint32_t slot = r >= nn ? 1u : 0u, rr = r - slot * nn;
+            // a launch of one face (faces bit 0: top, bit 1: bottom; 0: both) skips the other face's rays
+            if (F.faces && !(F.faces & (1u << slot))) continue;
             float spx, spy;
             v3f dl;
             if (!diffuse_sample_m(i, rr % n, rr / n, n, seed, patch, sc.rec, mdi, spx, spy, dl)) continue;

@@ -106,14 +106,26 @@ namespace helios {
         if (primitive_count_ == 0) {
             return; // no geometry yet (VulkanComputeBackend.cpp:592-598)
         }
-        for (char g: materials.is_glass) {
-            if (g != 0) {
-                unsupported("updateMaterials", "translucent cover (glass) material");
-            }
-        }
         check(rt_update_materials(ctx_, static_cast<uint32_t>(materials.num_sources), static_cast<uint32_t>(materials.num_primitives), static_cast<uint32_t>(materials.num_bands), materials.reflectivity.data(),
                                   materials.transmissivity.data()),
               "updateMaterials");
+        // translucent covers: the source-0 slice of is_glass / glass_n / glass_KL (VulkanComputeBackend.cpp:656-690)
+        const size_t PB = materials.num_primitives * materials.num_bands;
+        bool any_glass = false;
+        for (char g: materials.is_glass) {
+            any_glass = any_glass || g != 0;
+        }
+        if (any_glass) {
+            if (materials.is_glass.size() < PB || materials.glass_n.size() < PB || (!materials.glass_KL.empty() && materials.glass_KL.size() < PB)) {
+                helios_runtime_error("ERROR (HipRayTracingBackend::updateMaterials): is_glass / glass_n / glass_KL need Nsources x Nprims x Nbands entries.");
+            }
+            std::vector<uint8_t> is_glass(materials.is_glass.begin(), materials.is_glass.begin() + static_cast<std::ptrdiff_t>(PB));
+            check(rt_update_glass_materials(ctx_, static_cast<uint32_t>(materials.num_primitives), static_cast<uint32_t>(materials.num_bands), is_glass.data(), materials.glass_n.data(),
+                                            materials.glass_KL.empty() ? nullptr : materials.glass_KL.data()),
+                  "updateMaterials");
+        } else {
+            check(rt_update_glass_materials(ctx_, static_cast<uint32_t>(materials.num_primitives), static_cast<uint32_t>(materials.num_bands), nullptr, nullptr, nullptr), "updateMaterials");
+        }
         // specular exponent and scale per primitive (VulkanComputeBackend.cpp:631-655)
         if (!materials.specular_exponent.empty()) {
             const size_t P = materials.num_primitives;

@@ -4,8 +4,8 @@
 // it is not part of this repository's own build.  Each virtual maps to one or two C calls and every
 // failure is rethrown through helios_runtime_error, as the reference backends do.  Radiation launches
 // (direct, diffuse) for all five source types, texture-masked primitives and power-law / Prague skies,
-// camera and pixel-label launches (with the Blinn-Phong specular term) and their buffer management are
-// supported; glass covers raise an error naming the unsupported feature.
+// translucent (glass) covers, camera and pixel-label launches (with the Blinn-Phong specular term) and their
+// buffer management are supported.
 #ifndef HIP_RAYTRACING_BACKEND_H
 #define HIP_RAYTRACING_BACKEND_H
 

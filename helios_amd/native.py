@@ -159,6 +159,7 @@ SIGNATURES = {
     "rt_upload_radiation_out": (C.c_int, [C.c_void_p, f32p, f32p, C.c_size_t]),
     "rt_upload_source_fluxes": (C.c_int, [C.c_void_p, f32p, C.c_size_t]),
     "rt_update_specular_materials": (C.c_int, [C.c_void_p, C.c_uint32, f32p, f32p]),
+    "rt_update_glass_materials": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint8), f32p, f32p]),
     "rt_upload_source_fluxes_cam": (C.c_int, [C.c_void_p, f32p, C.c_size_t]),
     "rt_get_radiation_specular": (C.c_int, [C.c_void_p, f32p, C.c_size_t]),
     "rt_launch_direct_rays": (C.c_int, [C.c_void_p, C.POINTER(rt_launch_params)]),
@@ -524,6 +525,20 @@ class Backend:
     def upload_source_fluxes(self, fluxes):
         f = _f32(fluxes).reshape(-1)
         _check(self.lib.rt_upload_source_fluxes(self.ctx._h, _pf(f), f.size))
+
+    def update_glass_materials(self, is_glass, glass_n=None, glass_KL=None):
+        """Translucent covers: is_glass, glass_n (refractive index), glass_KL (absorption K*L, None = 0), each
+        [P][B_global]; is_glass None removes every cover."""
+        if is_glass is None:
+            _check(self.lib.rt_update_glass_materials(self.ctx._h, self.P, self.info()["nbands_global"], None, None,
+                                                      None))
+            return
+        g = np.ascontiguousarray(is_glass, np.uint8)
+        P, B = g.shape
+        gn = _f32(glass_n).reshape(P, B)
+        gkl = None if glass_KL is None else _f32(glass_KL).reshape(P, B)
+        _check(self.lib.rt_update_glass_materials(self.ctx._h, P, B, g.ctypes.data_as(C.POINTER(C.c_uint8)), _pf(gn),
+                                                  _pf(gkl)))
 
     def update_specular_materials(self, exponent, scale=None):
         """specular_exponent [P] (not positive: no specular term) and specular_scale [P] (None = 0)."""

@@ -256,6 +256,13 @@ int rt_build_acceleration_structure(rt_ctx *ctx);
  * (R/shaders/direct_raygen.comp:684-687, diffuse_raygen.comp:575-579). */
 int rt_update_materials(rt_ctx *ctx, uint32_t nsources, uint32_t nprims, uint32_t nbands_global, const float *rho,
                         const float *tau);
+/* updateMaterials' translucent-cover fields (RayTracingTypes.h:173-178; VulkanComputeBackend.cpp:656-690):
+ * is_glass, glass_n (refractive index >= 1), glass_KL (absorption K*L >= 0, NULL = 0), each
+ * [P][B_global] (the source-0 slice); call after rt_update_materials (which clears them), is_glass NULL =
+ * no covers.  A launch
+ * passes rays through the primitives that are glass in every launch band (at most 8 launch bands then). */
+int rt_update_glass_materials(rt_ctx *ctx, uint32_t nprims, uint32_t nbands_global, const uint8_t *is_glass,
+                              const float *glass_n, const float *glass_KL);
 /* updateSources (RayTracingBackend.h:99; RayTracingTypes.h:189-196): types[S] (RT_SOURCE_*),
  * positions[S][3] (direction towards a collimated source, else the source position), widths[S][2] and
  * rotations[S][3] (RayTracingSource::width / ::rotation; NULL when every source is collimated). */

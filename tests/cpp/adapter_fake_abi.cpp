@@ -133,6 +133,15 @@ int rt_upload_source_fluxes(rt_ctx *, const float *fluxes, size_t count) {
     return status("rt_upload_source_fluxes");
 }
 
+int rt_update_glass_materials(rt_ctx *, uint32_t nprims, uint32_t nbands_global, const uint8_t *is_glass,
+                              const float *glass_n, const float *glass_KL) {
+    const size_t n = (size_t)nprims * nbands_global;
+    g_fake.is_glass = copy(is_glass, n);
+    g_fake.glass_n = copy(glass_n, n);
+    g_fake.glass_KL = copy(glass_KL, n);
+    return status("rt_update_glass_materials");
+}
+
 int rt_update_specular_materials(rt_ctx *, uint32_t nprims, const float *exponent, const float *scale) {
     g_fake.spec_exp = copy(exponent, nprims);
     g_fake.spec_scale = copy(scale, nprims);

@@ -47,6 +47,9 @@ struct FakeAbi {
     std::vector<float> fluxes, out_top, out_bottom;
     // rt_update_specular_materials / rt_upload_source_fluxes_cam
     std::vector<float> spec_exp, spec_scale, fluxes_cam;
+    // rt_update_glass_materials
+    std::vector<uint8_t> is_glass;
+    std::vector<float> glass_n, glass_KL;
     int fail_next = 0;  // a nonzero status makes the next call fail with this message
     std::string fail_msg = "injected failure";
 };

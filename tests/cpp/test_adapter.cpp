@@ -158,9 +158,16 @@ int main() {
     CHECK(F.spec_exp == shiny.specular_exponent && F.spec_scale == shiny.specular_scale);
     shiny.specular_scale.resize(3);
     CHECK(throws_with([&] { rb->updateMaterials(shiny); }, "specular"));
+    // translucent covers: the source-0 slice reaches rt_update_glass_materials; no glass clears the covers
+    CHECK(F.is_glass.empty() && F.calls.back() == "rt_update_glass_materials");
     RayTracingMaterial glass = mat;
     glass.is_glass[5] = 1;
-    CHECK(throws_with([&] { rb->updateMaterials(glass); }, "glass"));
+    glass.glass_n.assign(12, 1.5f);
+    glass.glass_KL.assign(12, 0.1f);
+    rb->updateMaterials(glass);
+    CHECK(F.is_glass.size() == 12 && F.is_glass[5] == 1 && F.is_glass[4] == 0 && F.glass_n[5] == 1.5f && F.glass_KL[11] == 0.1f);
+    glass.glass_n.resize(4);
+    CHECK(throws_with([&] { rb->updateMaterials(glass); }, "glass_n"));
 
     // sources: flattened in order
     std::vector<RayTracingSource> src(2);

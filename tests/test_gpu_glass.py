@@ -10,6 +10,7 @@ from helios_amd import native, scenes
 from helios_amd import workloads as W
 from oracle.oracle import OracleScene, glass_tra
 from tests import glass_cases as gc
+from tests.backend_schedule import run_band_via_backend
 from tests.helpers import rel_err
 
 pytestmark = pytest.mark.gpu
@@ -82,6 +83,32 @@ def _greenhouse_request(tw, P0, P, schedule=0):
                               source_dirs=[native.sun_direction(35.0, 120.0)], source_flux=[[350.0, 500.0]],
                               direct_seed=21, diffuse_seeds=[31, 32, 33], schedule=schedule, is_glass=glass,
                               glass_n=gn, glass_KL=gkl)
+
+
+def test_greenhouse_call_by_call(ctx):
+    """The reference's pass loop driven call by call (tests/backend_schedule.py): covers pass the direct and
+    diffuse rays of the call-by-call launches as they do on the whole-solve path (same rays; per-face launches
+    and deposits change only the FP32 summation order: 1e-5), and the result matches the oracle (1e-4)."""
+    T, ty, tw, P0 = _greenhouse()
+    P = T.shape[0]
+    req = _greenhouse_request(tw, P0, P)
+    Rg = native.run_band(ctx.scene(T, ty, tw), req)
+    be = native.Backend(ctx)
+    be.update_geometry(T, ty, tw)
+    be.build_acceleration_structure()
+    be.update_materials(req.rho, req.tau)
+    be.update_glass_materials(req.is_glass, req.glass_n, req.glass_KL)
+    be.update_sources(req.sdir)
+    Rb = run_band_via_backend(be, req, tw)
+    e = rel_err(Rb, Rg, floor=1.0)
+    assert e.max() <= 1e-5, f"max rel err {e.max():.3e}"
+    Ro = OracleScene(T, ty, tw).run_band(req)
+    assert rel_err(Rb, Ro, floor=1.0).max() <= 1e-4
+    # the covers take a share (their own radiation_flux), and without them the same launches differ
+    assert np.all(Rb[P0:].sum(0) > 0)
+    be.update_glass_materials(None)
+    R0 = run_band_via_backend(be, req, tw)
+    assert np.abs(R0[:P0] - Rb[:P0]).max() > 1.0
 
 
 @pytest.mark.parametrize("schedule", [0, 1])
