@@ -423,6 +423,10 @@ int rt_set_energy_accounting(rt_ctx *ctx, int on) {
 }
 
 static int energy_ready(rt_ctx *ctx, uint32_t nchannels) {
+    if (!ctx->sx.energy_valid && ctx->sx.energy_glass) {
+        set_error("no energy bookkeeping: the last solve had translucent covers, which the bookkeeping does not follow");
+        return -1;
+    }
     if (!ctx->sx.energy_valid) {
         set_error("no energy bookkeeping: call rt_set_energy_accounting(ctx, 1) before the solve");
         return -1;

@@ -39,6 +39,9 @@ struct Model {
     std::vector<uint32_t> depth, emission, dseed, stype;
     std::vector<float> swidth, srot;
     std::vector<float> kext, kpeak, knorm, kprague;  // anisotropic sky of the model's bands
+    std::vector<uint8_t> gis;                        // translucent covers [P][B] (empty: none)
+    std::vector<float> gn, gkl;
+    bool covers = false;   // some primitive is glass in every band: at most RT_GLASS_MAX_CH channels per batch
     void *dmem = nullptr;  // resident rho, tau[, eps][, temp]
     DevMaterials dm{};
     size_t resident_bytes = 0;
@@ -139,6 +142,8 @@ void rt_server::worker(uint32_t w) {
         // the batch's direct pass must stay within one launch's work-item limit (a lone sun that exceeds it
         // fails in run_solve with that message)
         if (m.Sn) kmax = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(kmax, RT_MAX_WORK_ITEMS / ((uint64_t)m.Sn * m.P)));
+        // a solve with translucent covers carries at most RT_GLASS_MAX_CH channels (suns x bands)
+        if (m.covers) kmax = std::max(1u, std::min(kmax, (uint32_t)RT_GLASS_MAX_CH / m.B));
         std::vector<Request *> batch;
         for (Request *r : queue)
             if (r->model == head->model && batch.size() < kmax) batch.push_back(r);
@@ -360,6 +365,25 @@ int rt_server_add_model(rt_server *srv, const rt_scene *scene, const rt_band_req
         m->q.sky_params = keep(m->kprague, q->sky_params, (size_t)4 * Bq);
     }
     m->q.rho = m->q.tau = m->q.eps = m->q.temperature = nullptr;
+    // translucent covers: kept with the model, uploaded by each solve with its other per-solve tables
+    m->q.is_glass = nullptr;
+    m->q.glass_n = m->q.glass_KL = nullptr;
+    if (q->is_glass && P) {
+        NN(q->glass_n, "glass_n");
+        const size_t PBg = (size_t)P * B;
+        m->gis.assign(q->is_glass, q->is_glass + PBg);
+        m->gn.assign(q->glass_n, q->glass_n + PBg);
+        if (q->glass_KL) m->gkl.assign(q->glass_KL, q->glass_KL + PBg);
+        else m->gkl.assign(PBg, 0.0f);
+        m->q.is_glass = m->gis.data();
+        m->q.glass_n = m->gn.data();
+        m->q.glass_KL = m->gkl.data();
+        for (uint32_t i = 0; i < P && !m->covers; i++) {
+            bool all = true;
+            for (uint32_t b = 0; b < B && all; b++) all = m->gis[(size_t)i * B + b] != 0;
+            m->covers = all;
+        }
+    }
     // resident optical properties
     const size_t PB = (size_t)P * B;
     size_t bytes = PB * 2 * 4 + (q->eps ? PB * 4 : 0) + (q->temperature ? (size_t)P * 4 : 0);

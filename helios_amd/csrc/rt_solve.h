@@ -58,6 +58,7 @@ struct SolveCtx {
     // diffuse/scattering pass) of its channels
     bool energy_acct = false;
     bool energy_valid = false;
+    bool energy_glass = false;          // the last solve had translucent covers, which the bookkeeping skips
     uint32_t energy_C = 0, energy_npass = 0;
     std::vector<double> energy;         // [RT_ENERGY_ROWS][C]
     std::vector<double> energy_passes;  // [npass][RT_ENERGY_PASS_ROWS][C]

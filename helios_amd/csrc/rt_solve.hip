@@ -491,7 +491,6 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
     bool glass = false;
     if (q.is_glass && P) {
         if (!q.glass_n) { set_error("run_band: is_glass needs glass_n"); return -1; }
-        if (dm) { set_error("run_band: translucent covers are not supported by the request server"); return -1; }
         hcover.assign(P, 0);
         for (uint32_t i = 0; i < P; i++) {
             bool all = true;
@@ -773,6 +772,7 @@ int run_solve(SolveCtx &sx, const Scene &S, const rt_band_request &q, const SunS
     k_final<<<nb(PC), 256, 0, st>>>(P, C, B, in, stb, sbb, R);
     LK();
     sx.energy_valid = false;
+    sx.energy_glass = sx.energy_acct && glass;
     if (acct) {
         CK(hipMemsetAsync(energy, 0, (size_t)C * RT_ENERGY_ROWS * 8, st));
         const bool emits = any_primary && n > 0 && emission;

@@ -408,8 +408,9 @@ typedef struct {
 int rt_server_create(rt_ctx *ctx, const rt_server_cfg *cfg, rt_server **out);
 /* Finishes every queued request, then frees the server. */
 int rt_server_destroy(rt_server *srv);
-/* base: bands, ray counts, depths, emission, rho/tau/eps/temperature, diffuse seeds, schedule;
- * its source_dir / source_flux / diffuse_flux / direct_seed are ignored (they come per request). */
+/* base: bands, ray counts, depths, emission, rho/tau/eps/temperature, translucent covers, diffuse
+ * seeds, schedule; its source_dir / source_flux / diffuse_flux / direct_seed are ignored (they come per
+ * request).  A model with covers is batched at most 8 channels (suns x bands) at a time. */
 int rt_server_add_model(rt_server *srv, const rt_scene *scene, const rt_band_request *base, uint32_t *model_id);
 /* R_out: [P][B] host memory, written before the request completes; keep it alive until rt_server_wait. */
 int rt_server_submit(rt_server *srv, uint32_t model_id, const rt_solve_sun *sun, float *R_out, uint64_t *ticket);

@@ -148,6 +148,36 @@ def test_greenhouse_batch_equals_single_suns(ctx):
         assert rel_err(Rb[k], Rp, floor=1.0).max() <= 1e-5
 
 
+def test_greenhouse_served(ctx):
+    """The request server with a glass model: requests batched at most 4 suns (8 channels) at a time, each
+    result equal to its single-sun solve within 1e-5 (the covers' shares are FP64 atomic sums)."""
+    T, ty, tw, P0 = _greenhouse()
+    P = T.shape[0]
+    g = ctx.scene(T, ty, tw)
+    base = _greenhouse_request(tw, P0, P)
+    srv = native.Server(ctx, slots=2, max_batch_suns=16)
+    try:
+        mid = srv.add_model(g, base)
+        rng = np.random.default_rng(3)
+        jobs = []
+        for k in range(10):
+            d = np.asarray([native.sun_direction(float(rng.uniform(15, 70)), float(rng.uniform(60, 300)))],
+                           np.float32)
+            f = rng.uniform(200, 600, (1, 2)).astype(np.float32)
+            q = rng.uniform(40, 120, 2).astype(np.float32)
+            seed = int(rng.integers(1, 2**31))
+            jobs.append((d, f, q, seed, srv.submit(mid, d, f, q, seed)))
+        for d, f, q, seed, t in jobs:
+            R, _ = srv.wait(t)
+            single = W.single_sun_request(base, d, f, q, seed)
+            single.is_glass, single.glass_n, single.glass_KL = base.is_glass, base.glass_n, base.glass_KL
+            assert rel_err(R, native.run_band(g, single), floor=1.0).max() <= 1e-5
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert st["completed"] == 10 and st["failed"] == 0 and 1 <= st["max_batch"] <= 4, st
+
+
 def test_glass_limits(ctx):
     T, ty, tw, P0 = _greenhouse()
     P = T.shape[0]
@@ -158,6 +188,14 @@ def test_glass_limits(ctx):
     with pytest.raises(native.RTError, match="at most 8 channels"):
         native.run_band_batch(g, base, dirs, np.ones((K, 1, 2), np.float32), np.ones((K, 2), np.float32),
                               list(range(K)))
+    # the FP64 energy bookkeeping follows the hit-table passes, which a solve with covers does not run
+    native.set_energy_accounting(ctx, True)
+    try:
+        native.run_band(g, base)
+        with pytest.raises(native.RTError, match="translucent covers"):
+            native.energy_fp64(ctx, 2)
+    finally:
+        native.set_energy_accounting(ctx, False)
     bad = _greenhouse_request(tw, P0, P)
     bad.glass_n[P0, 0] = 0.5
     with pytest.raises(native.RTError, match="refractive index"):
